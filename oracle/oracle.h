@@ -1,0 +1,238 @@
+// Oracle (test infrastructure only — never linked into the product library).
+// A single-threaded C++ restatement of the reference JBullet hot path:
+// AABB update, DbvtBroadphase / AxisSweep3 / SimpleBroadphase pair sets,
+// collision dispatch (sphere-sphere, convex-plane, GJK + GJK-EPA), the
+// persistent manifold, the cache-friendly sequential-impulse solver, the
+// integrator, activation and islands.  Every function cites the Java file:line
+// it follows (paths relative to src/com/bulletphysics/).
+//
+// Parity pinning (what fixes each part, since the reference has no tests):
+//   jrandom          KATs of java.util.Random                 tests/test_oracle_basics.py
+//   jmath / libGDX   orthonormality, known rotations          tests/test_oracle_basics.py
+//   broadphase       O(N^2) brute force, closed-form quantised overlap, invariants
+//   sphere/plane     closed forms
+//   GJK / EPA        double-precision brute-force distance/depth on boxes & spheres
+//   manifold         hand-computed add/replace/evict/refresh sequences
+//   solver           resting-box impulse m*g*dt, head-on e=0, cone/complementarity
+//   integrator       exact free fall, constant-omega rotation
+// libGDX arithmetic (Quaternion.setFromMatrix / nor) is restated from libGDX's
+// documented behaviour; its bit parity against a libGDX build is unpinned.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "jmath.h"
+#include "jrandom.h"
+#include "../include/jbgpu_format.h"
+
+namespace jb {
+
+// ---- shapes (collision/shapes/*.java) --------------------------------------
+struct Shape {
+  int type;       // JBGPU_SHAPE_*
+  float margin;   // getMargin()
+  V3 dims;        // implicitShapeDimensions
+  int vfirst, vcount;  // hull points
+  V3 lmin, lmax;  // hull local AABB (PolyhedralConvexShape.recalcLocalAabb)
+  V3 pn;          // plane normal
+  float pc;       // plane constant
+};
+
+V3 supportNoMargin(const Shape& s, const V3* hullv, V3 dir);   // localGetSupportingVertexWithoutMargin
+V3 supportMargin(const Shape& s, const V3* hullv, V3 dir);     // localGetSupportingVertex
+void shapeAabb(const Shape& s, const Tr& t, V3& mn, V3& mx);   // getAabb
+V3 shapeLocalInertia(const Shape& s, const V3* hullv, float mass);  // calculateLocalInertia
+Shape makeSphere(float r);
+Shape makeBox(V3 halfExtents);
+Shape makeCapsule(float r, float height);
+Shape makeHull(const V3* pts, int first, int n);
+Shape makePlane(V3 n, float c);
+
+// ---- bodies (dynamics/RigidBody.java, collision/dispatch/CollisionObject.java)
+struct Body {
+  int shape;
+  Tr tr;
+  V3 lv, av;
+  float invMass;
+  V3 invInertiaLocal;
+  M3 invInertiaWorld;
+  V3 gravity;          // RigidBody.gravity (force, acceleration / invMass)
+  float friction, restitution;
+  int flags;           // JBGPU_BODY_STATIC
+  int activation;      // CollisionObject.activationState1
+  float deactivationTime;
+  int islandTag, companionId;
+  bool isStatic() const { return (flags & JBGPU_BODY_STATIC) != 0; }
+  bool isActive() const { return activation != JBGPU_ISLAND_SLEEPING && activation != JBGPU_DISABLE_SIMULATION; }
+};
+void updateInertiaTensor(Body& b);  // RigidBody.java:449-459
+
+// ---- broadphase -----------------------------------------------------------
+struct Aabb {
+  V3 mn, mx;
+};
+inline bool intersect(const Aabb& a, const Aabb& b) {  // DbvtAabbMm.java:209-212
+  return (a.mn.x <= b.mx.x) && (a.mx.x >= b.mn.x) && (a.mn.y <= b.mx.y) && (a.mx.y >= b.mn.y) &&
+         (a.mn.z <= b.mx.z) && (a.mx.z >= b.mn.z);
+}
+struct Proxy {
+  int stage;        // DbvtProxy.stage (0/1 dynamic set stages, 2 = fixed set)
+  Aabb leaf;        // leaf.volume in the dbvt
+  Aabb aabb;        // DbvtProxy.aabb (the stored, possibly fattened box)
+  int16_t group, mask;
+  int32_t q[6];     // AxisSweep3 quantised min xyz / max xyz
+};
+struct Broadphase {
+  int mode;  // JBGPU_BP_*
+  int stageCurrent = 0;
+  std::vector<Proxy> proxies;  // index = uid - 1
+  // AxisSweep3
+  V3 wmin, wmax, quant;
+  uint32_t handleMask = 0, sentinel = 0, qmask = 0;
+  void init(int mode, V3 wmin, V3 wmax);
+  void createProxy(Aabb a, int16_t group, int16_t mask);
+  void setAabb(int idx, Aabb a);
+  void collide();  // stage rotation of DbvtBroadphase.collide
+  // overlapping pair set, canonical (uid_lo, uid_hi), sorted ascending
+  void pairs(std::vector<uint64_t>& out) const;
+  void pairsBrute(std::vector<uint64_t>& out) const;
+  bool overlap(int i, int j) const;
+  void quantize(V3 p, int isMax, int32_t* out) const;
+};
+inline uint64_t pairKey(uint32_t uidLo, uint32_t uidHi) { return ((uint64_t)uidLo << 32) | uidHi; }
+
+// ---- manifold (collision/narrowphase/ManifoldPoint.java, PersistentManifold.java)
+struct MPoint {
+  V3 localA, localB, worldA, worldB, normal;
+  float distance, friction, restitution;
+  float applied, lat1, lat2;
+  bool latInit;
+  V3 latDir1, latDir2;
+  int lifetime;
+};
+struct Manifold {
+  int body0, body1;  // body indices (manifold.getBody0/1)
+  int n;
+  MPoint p[4];
+};
+int manifoldCacheEntry(const Manifold& m, const MPoint& np);               // PersistentManifold.java:214-233
+int manifoldAddPoint(Manifold& m, const MPoint& np);                       // :235-257
+void manifoldReplacePoint(Manifold& m, const MPoint& np, int idx);         // :280-305
+void manifoldRemovePoint(Manifold& m, int idx);                            // :259-278
+void manifoldRefresh(Manifold& m, const Tr& trA, const Tr& trB);           // :312-372
+int manifoldSortCachedPoints(const Manifold& m, const MPoint& pt);         // :83-156
+
+// ---- narrowphase -----------------------------------------------------------
+struct ContactCandidate {
+  bool valid;
+  V3 normalOnB, pointOnB;
+  float depth;
+};
+struct GjkStats {
+  int gjkIters = 0, epaCalls = 0, epaFails = 0;
+};
+ContactCandidate sphereSphere(const Shape& s0, const Tr& t0, const Shape& s1, const Tr& t1, bool& touching);
+ContactCandidate convexPlane(const Shape& convex, const V3* hv, const Tr& tc, const Shape& plane, const Tr& tp);
+ContactCandidate gjkPairDetector(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb, const V3* hv,
+                                 float maxDist2, GjkStats* st);
+bool gjkEpaPenDepth(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb, const V3* hv, V3& wA, V3& wB,
+                    int* epaIters);
+// ManifoldResult.addContactPoint (ManifoldResult.java:92-157)
+void addContactPoint(Manifold& m, const Body& r0, const Body& r1, int rbody0, const ContactCandidate& c);
+// ManifoldResult.refreshContactPoints (:177-191)
+void refreshContactPoints(Manifold& m, const Body* bodies);
+
+// ---- solver (dynamics/constraintsolver/SequentialImpulseConstraintSolver.java)
+struct SolverInfo {
+  float timeStep = 1.0f / 60.0f;
+  int numIterations = 10;
+  float erp = 0.2f;
+  float linearSlop = 0.0f;
+  float warmstartingFactor = 0.85f;
+  int solverMode = 1 | 8 | 4;  // RANDOMIZE | CACHE_FRIENDLY | WARMSTART (ContactSolverInfo.java:48)
+  // project additions: run to a residual tolerance (0 = fixed iterations)
+  float tolerance = 0.0f;
+  int maxIterations = 100;
+};
+struct SolverStats {
+  int iterations = 0;
+  float residual = 0.0f;
+  int rows = 0, frictionRows = 0;
+  std::vector<float> residualHistory;
+};
+struct SolverRow {
+  int bodyA, bodyB;  // solver body ids
+  V3 normal, rel1xn, rel2xn, angA, angB;
+  float jacInv, penetration, restitution, friction, applied, pushApplied;
+  int frictionIndex;
+  MPoint* cp;
+};
+struct SolverBody {
+  V3 lv, av;
+  float invMass;
+  int body;  // original body index (-1 for none)
+};
+struct SolverState;  // opaque per-call temporaries
+struct ImpulseInit { // learned warm start: override of the warm-start impulses per row
+  const float* values = nullptr;  // [row][3] normal, lat1, lat2 (already scaled)
+};
+
+// One solveGroup over an ordered list of manifolds (SequentialImpulseConstraintSolver.java:954-969).
+// shuffle: apply the Random shuffle at iterations with (iter & 7) == 0.
+void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,
+                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features);
+
+// ---- islands (collision/dispatch/SimulationIslandManager.java, UnionFind.java)
+struct UnionFind {
+  std::vector<int> id, sz;
+  void reset(int n);
+  int find(int x);
+  void unite(int p, int q);
+  void sortIslands();  // id := find(i), sz := i, quicksort by id
+};
+
+// ---- world -----------------------------------------------------------------
+struct WorldConfig {
+  int bpMode = JBGPU_BP_DBVT;
+  V3 worldMin, worldMax;  // AxisSweep3 bounds
+  SolverInfo solver;
+  bool deactivationDisabled = true;  // BulletGlobals.setDeactivationDisabled(true)
+  V3 gravity;
+  int solverOrder = JBGPU_ORDER_REFERENCE;
+};
+struct PairState {
+  uint64_t key;
+  Manifold m;
+};
+struct World {
+  WorldConfig cfg;
+  std::vector<Shape> shapes;
+  std::vector<V3> hullv;
+  std::vector<Body> bodies;
+  Broadphase bp;
+  std::vector<PairState> pairs;  // sorted by key
+  JavaRandom rng{0};
+  UnionFind uf;
+  SolverStats lastSolve;
+  GjkStats gjk;
+  std::vector<int> explicitOrder;  // manifold order for JBGPU_ORDER_EXPLICIT (indices into pairs)
+  const float* learnedInit = nullptr;
+  std::vector<float>* features = nullptr;
+  long stepCount = 0;
+
+  void addBody(const Body& b);
+  void step(float dt);
+  // phases
+  void predictUnconstraintMotion(float dt);
+  void updateAabbs();
+  void calculateOverlappingPairs();
+  void dispatchAllCollisionPairs();
+  void calculateSimulationIslands();
+  void solveConstraints();
+  void integrateTransforms(float dt);
+  void updateActivationState(float dt);
+  bool needsCollision(int a, int b) const;
+  bool needsResponse(int a, int b) const;
+};
+
+}  // namespace jb

@@ -1,0 +1,289 @@
+// Oracle (test infrastructure only): SequentialImpulseConstraintSolver, the
+// cache-friendly path (dynamics/constraintsolver/SequentialImpulseConstraintSolver.java:
+// setup :418-786, iterations :788-890, writeback :892-950) with SolverBody
+// (dynamics/constraintsolver/SolverBody.java) and TransformUtil.planeSpace1.
+//
+// Project additions, off unless requested: a residual r_k = ||l_k - l_{k-1}||_2 /
+// max(1, ||l_k||_2) over all rows (double accumulation), a tolerance-driven
+// iteration count, per-row warm-start overrides (learned warm start) and a
+// per-row feature dump.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "oracle.h"
+
+namespace jb {
+
+static void planeSpace1(V3 n, V3& p, V3& q) {  // TransformUtil.java:45-61
+  const float SIMDSQRT12 = 0.7071067811865475244008443621048490f;
+  if (jabs(n.z) > SIMDSQRT12) {
+    float a = n.y * n.y + n.z * n.z;
+    float k = 1.0f / jsqrt(a);
+    p = v3(0, -n.z * k, n.y * k);
+    q = v3(a * k, -n.x * p.z, n.x * p.y);
+  } else {
+    float a = n.x * n.x + n.y * n.y;
+    float k = 1.0f / jsqrt(a);
+    p = v3(-n.y * k, n.x * k, 0);
+    q = v3(-n.z * p.y, n.z * p.x, a * k);
+  }
+}
+
+static inline void applyImpulse(SolverBody& b, V3 lc, V3 ac, float mag) {  // SolverBody.java:61-72
+  if (b.invMass != 0.0f) {
+    b.lv.x += mag * 1.0f * lc.x;
+    b.lv.y += mag * 1.0f * lc.y;
+    b.lv.z += mag * 1.0f * lc.z;
+    b.av.x += mag * 1.0f * ac.x;
+    b.av.y += mag * 1.0f * ac.y;
+    b.av.z += mag * 1.0f * ac.z;
+  }
+}
+
+static SolverRow makeFrictionRow(V3 axis, int ida, int idb, int fi, const MPoint& cp, V3 rp1, V3 rp2, const Body& b0,
+                                 const Body& b1, float relaxation) {  // :349-416
+  SolverRow r{};
+  r.normal = axis;
+  r.bodyA = ida;
+  r.bodyB = idb;
+  r.frictionIndex = fi;
+  r.friction = cp.friction;
+  r.applied = 0;
+  r.pushApplied = 0;
+  r.penetration = 0;
+  V3 t = crs(rp1, r.normal);
+  r.rel1xn = t;
+  r.angA = mulMV(b0.invInertiaWorld, t);
+  t = crs(rp2, r.normal);
+  r.rel2xn = t;
+  r.angB = mulMV(b1.invInertiaWorld, t);
+  float d0 = b0.invMass + dot(axis, crs(r.angA, rp1));
+  float d1 = b1.invMass + dot(axis, crs(r.angB, rp2));
+  r.jacInv = relaxation / (d0 + d1);
+  return r;
+}
+
+void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,
+                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features) {
+  std::vector<SolverBody> sb;
+  std::vector<SolverRow> rows, frows;
+  std::vector<int> companion(bodies.size(), -1);
+  const bool warm = (info.solverMode & 4) != 0;
+  auto solverBodyFor = [&](int bi) -> int {
+    const Body& b = bodies[bi];
+    if (b.islandTag >= 0) {
+      if (companion[bi] >= 0) return companion[bi];
+      companion[bi] = (int)sb.size();
+    }
+    SolverBody s;  // initSolverBody (:151-177); static bodies get a fresh one per manifold
+    s.av = b.av;
+    s.lv = b.lv;
+    s.invMass = b.invMass;
+    s.body = bi;
+    sb.push_back(s);
+    return (int)sb.size() - 1;
+  };
+  // ---- setup
+  for (Manifold* mp : manifolds) {
+    Manifold& m = *mp;
+    const Body& c0 = bodies[m.body0];
+    const Body& c1 = bodies[m.body1];
+    int idA = -1, idB = -1;
+    if (m.n != 0) {
+      idA = solverBodyFor(m.body0);
+      idB = solverBodyFor(m.body1);
+    }
+    for (int j = 0; j < m.n; j++) {
+      MPoint& cp = m.p[j];
+      if (!(cp.distance <= 0.0f)) continue;
+      V3 rp1 = sub(cp.worldA, c0.tr.o);
+      V3 rp2 = sub(cp.worldB, c1.tr.o);
+      const float relaxation = 1.0f;
+      int frictionIndex = (int)rows.size();
+      SolverRow r{};
+      r.bodyA = idA;
+      r.bodyB = idB;
+      r.cp = &cp;
+      V3 ta0 = crs(rp1, cp.normal);
+      r.angA = mulMV(c0.invInertiaWorld, ta0);
+      V3 ta1 = crs(rp2, cp.normal);
+      r.angB = mulMV(c1.invInertiaWorld, ta1);
+      float d0 = c0.invMass + dot(cp.normal, crs(r.angA, rp1));
+      float d1 = c1.invMass + dot(cp.normal, crs(r.angB, rp2));
+      r.jacInv = relaxation / (d0 + d1);
+      r.normal = cp.normal;
+      r.rel1xn = crs(rp1, cp.normal);
+      r.rel2xn = crs(rp2, cp.normal);
+      V3 vel1 = add(crs(c0.av, rp1), c0.lv);  // RigidBody.getVelocityInLocalPoint
+      V3 vel2 = add(crs(c1.av, rp2), c1.lv);
+      V3 vel = sub(vel1, vel2);
+      float rel_vel = dot(cp.normal, vel);
+      r.penetration = jmin(cp.distance + info.linearSlop, 0.0f);
+      r.friction = cp.friction;
+      r.restitution = cp.restitution * -rel_vel;  // restitutionCurve
+      if (r.restitution <= 0.0f) r.restitution = 0.0f;
+      float penVel = -r.penetration / info.timeStep;
+      if (r.restitution > penVel) r.penetration = 0.0f;
+      if (features) {  // learned warm start input (16 floats per normal row)
+        const float f[16] = {cp.normal.x, cp.normal.y, cp.normal.z, vel.x, vel.y, vel.z,
+                             c0.av.x - c1.av.x, c0.av.y - c1.av.y, c0.av.z - c1.av.z, cp.distance,
+                             c0.invMass, c1.invMass, cp.applied, cp.lat1, cp.lat2, cp.friction};
+        features->insert(features->end(), f, f + 16);
+      }
+      const int rowIdx = (int)rows.size();
+      if (learnedInit) {
+        r.applied = learnedInit[rowIdx * 3 + 0];
+        applyImpulse(sb[r.bodyA], scl(r.normal, c0.invMass), r.angA, r.applied);
+        applyImpulse(sb[r.bodyB], scl(r.normal, c1.invMass), r.angB, -r.applied);
+      } else if (warm) {
+        r.applied = cp.applied * info.warmstartingFactor;
+        applyImpulse(sb[r.bodyA], scl(r.normal, c0.invMass), r.angA, r.applied);
+        applyImpulse(sb[r.bodyB], scl(r.normal, c1.invMass), r.angB, -r.applied);
+      } else {
+        r.applied = 0.0f;
+      }
+      r.pushApplied = 0.0f;
+      r.frictionIndex = (int)frows.size();
+      if (!cp.latInit) {  // :678-708
+        cp.latDir1 = sub(vel, scl(cp.normal, rel_vel));
+        float lat_rel_vel = len2(cp.latDir1);
+        if (lat_rel_vel > FLT_EPS) {
+          cp.latDir1 = scl(cp.latDir1, 1.0f / jsqrt(lat_rel_vel));
+          frows.push_back(makeFrictionRow(cp.latDir1, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+          cp.latDir2 = nor(crs(cp.latDir1, cp.normal));
+          frows.push_back(makeFrictionRow(cp.latDir2, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+        } else {
+          planeSpace1(cp.normal, cp.latDir1, cp.latDir2);
+          frows.push_back(makeFrictionRow(cp.latDir1, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+          frows.push_back(makeFrictionRow(cp.latDir2, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+        }
+        cp.latInit = true;
+      } else {
+        frows.push_back(makeFrictionRow(cp.latDir1, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+        frows.push_back(makeFrictionRow(cp.latDir2, idA, idB, frictionIndex, cp, rp1, rp2, c0, c1, relaxation));
+      }
+      for (int k = 0; k < 2; k++) {  // :710-747
+        SolverRow& f = frows[r.frictionIndex + k];
+        if (learnedInit) {
+          f.applied = learnedInit[rowIdx * 3 + 1 + k];
+        } else if (warm) {
+          f.applied = (k == 0 ? cp.lat1 : cp.lat2) * info.warmstartingFactor;
+        } else {
+          f.applied = 0.0f;
+          continue;
+        }
+        applyImpulse(sb[r.bodyA], scl(f.normal, c0.invMass), f.angA, f.applied);
+        applyImpulse(sb[r.bodyB], scl(f.normal, c1.invMass), f.angB, -f.applied);
+      }
+      rows.push_back(r);
+    }
+  }
+  std::vector<int> order(rows.size()), forder(frows.size());
+  for (size_t i = 0; i < rows.size(); i++) order[i] = (int)i;
+  for (size_t i = 0; i < frows.size(); i++) forder[i] = (int)i;
+  // ---- iterations (:788-890)
+  int iters = info.numIterations;
+  if (info.tolerance > 0.0f) iters = info.maxIterations;
+  std::vector<float> prev(rows.size() + frows.size());
+  int done = 0;
+  float residual = 0.0f;
+  if (st) st->residualHistory.clear();
+  for (int it = 0; it < iters; it++) {
+    if (shuffle && (info.solverMode & 1) && ((it & 7) == 0)) {
+      for (int j = 0; j < (int)order.size(); ++j) {
+        int tmp = order[j];
+        int swapi = rng.nextInt(j + 1);
+        order[j] = order[swapi];
+        order[swapi] = tmp;
+      }
+      for (int j = 0; j < (int)forder.size(); ++j) {
+        int tmp = forder[j];
+        int swapi = rng.nextInt(j + 1);
+        forder[j] = forder[swapi];
+        forder[swapi] = tmp;
+      }
+    }
+    for (size_t i = 0; i < rows.size(); i++) prev[i] = rows[i].applied;
+    for (size_t i = 0; i < frows.size(); i++) prev[rows.size() + i] = frows[i].applied;
+    for (int j = 0; j < (int)order.size(); j++) {  // resolveSingleCollisionCombinedCacheFriendly (:234-284)
+      SolverRow& c = rows[order[j]];
+      SolverBody& A = sb[c.bodyA];
+      SolverBody& B = sb[c.bodyB];
+      float v1 = dot(c.normal, A.lv) + dot(c.rel1xn, A.av);
+      float v2 = dot(c.normal, B.lv) + dot(c.rel2xn, B.av);
+      float rel_vel = v1 - v2;
+      float positionalError = -c.penetration * info.erp / info.timeStep;
+      float velocityError = c.restitution - rel_vel;
+      float pi = positionalError * c.jacInv;
+      float vi = velocityError * c.jacInv;
+      float ni = pi + vi;
+      float old = c.applied;
+      float sum = old + ni;
+      c.applied = 0.0f > sum ? 0.0f : sum;
+      ni = c.applied - old;
+      applyImpulse(A, scl(c.normal, A.invMass), c.angA, ni);
+      applyImpulse(B, scl(c.normal, B.invMass), c.angB, -ni);
+    }
+    for (int j = 0; j < (int)forder.size(); j++) {  // resolveSingleFrictionCacheFriendly (:286-346)
+      SolverRow& c = frows[forder[j]];
+      float total = rows[c.frictionIndex].applied + rows[c.frictionIndex].pushApplied;
+      float limit = total * c.friction;
+      if (total > 0.0f) {
+        SolverBody& A = sb[c.bodyA];
+        SolverBody& B = sb[c.bodyB];
+        float v1 = dot(c.normal, A.lv) + dot(c.rel1xn, A.av);
+        float v2 = dot(c.normal, B.lv) + dot(c.rel2xn, B.av);
+        float rel_vel = v1 - v2;
+        float j1 = -rel_vel * c.jacInv;
+        float old = c.applied;
+        c.applied = old + j1;
+        if (limit < c.applied) {
+          c.applied = limit;
+        } else if (c.applied < -limit) {
+          c.applied = -limit;
+        }
+        j1 = c.applied - old;
+        applyImpulse(A, scl(c.normal, A.invMass), c.angA, j1);
+        applyImpulse(B, scl(c.normal, B.invMass), c.angB, -j1);
+      }
+    }
+    done = it + 1;
+    double dd = 0, nn = 0;
+    for (size_t i = 0; i < rows.size(); i++) {
+      double d = (double)rows[i].applied - (double)prev[i];
+      dd += d * d;
+      nn += (double)rows[i].applied * (double)rows[i].applied;
+    }
+    for (size_t i = 0; i < frows.size(); i++) {
+      double d = (double)frows[i].applied - (double)prev[rows.size() + i];
+      dd += d * d;
+      nn += (double)frows[i].applied * (double)frows[i].applied;
+    }
+    double nrm = std::sqrt(nn);
+    residual = (float)(std::sqrt(dd) / (nrm > 1.0 ? nrm : 1.0));
+    if (st) st->residualHistory.push_back(residual);
+    if (info.tolerance > 0.0f && residual <= info.tolerance) break;
+  }
+  // ---- writeback (:892-924): appliedImpulseLateral1 is written twice, the second
+  // time with the second friction row; appliedImpulseLateral2 is never stored.
+  for (auto& r : rows) {
+    r.cp->applied = r.applied;
+    r.cp->lat1 = frows[r.frictionIndex].applied;
+    r.cp->lat1 = frows[r.frictionIndex + 1].applied;
+  }
+  for (auto& s : sb) {  // SolverBody.writebackVelocity (:87-93)
+    if (s.invMass != 0.0f) {
+      bodies[s.body].lv = s.lv;
+      bodies[s.body].av = s.av;
+    }
+  }
+  if (st) {
+    st->iterations += done;
+    st->residual = residual;
+    st->rows += (int)rows.size();
+    st->frictionRows += (int)frows.size();
+  }
+}
+
+}  // namespace jb
