@@ -1,0 +1,152 @@
+/*
+ * jbgpu.h — C ABI of libjbgpu, the MI355X (gfx950) rigid-body hot path.
+ *
+ * Plain C: opaque context handle, integer status codes (0 = OK, < 0 = error,
+ * message via jbgpu_last_error), caller-owned host buffers borrowed only for
+ * the duration of a call, records laid out as in jbgpu_format.h.  A context is
+ * used by one host thread at a time and owns one HIP stream.
+ *
+ * Each group of entry points stands in for one plugin seam of the reference
+ * (paths relative to src/com/bulletphysics/ in the reference):
+ *   BroadphaseInterface      collision/broadphase/BroadphaseInterface.java:33-52
+ *   Dispatcher (+ CollisionAlgorithm per pair)
+ *                            collision/broadphase/Dispatcher.java:38-68,
+ *                            collision/broadphase/CollisionAlgorithm.java:38-70
+ *   ConstraintSolver         dynamics/constraintsolver/ConstraintSolver.java:37-55
+ *   DynamicsWorld.stepSimulation (the fused device-resident step)
+ *                            dynamics/DynamicsWorld.java:53-74,
+ *                            dynamics/DiscreteDynamicsWorld.java:290-400
+ */
+#ifndef JBGPU_H
+#define JBGPU_H
+
+#include <stdint.h>
+
+#include "jbgpu_format.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JBGPU_OK 0
+#define JBGPU_ERR_ARG (-1)
+#define JBGPU_ERR_HIP (-2)
+#define JBGPU_ERR_STATE (-3)
+#define JBGPU_ERR_CAPACITY (-4)
+#define JBGPU_ERR_NO_DEVICE (-5)
+
+typedef struct jbgpu_ctx jbgpu_ctx;
+
+typedef struct jbgpu_config {
+  int32_t bp_mode;               /* JBGPU_BP_*: which BroadphaseInterface semantics */
+  float world_min[3];            /* AxisSweep3 world bounds (AxisSweep3Internal.java:80-103) */
+  float world_max[3];
+  int32_t num_iterations;        /* ContactSolverInfo.numIterations (ContactSolverInfo.java:38) */
+  int32_t solver_order;          /* JBGPU_ORDER_COLOR */
+  int32_t deactivation_disabled; /* BulletGlobals.setDeactivationDisabled (BulletGlobals.java:114-116) */
+  float tolerance;               /* > 0: iterate to residual <= tolerance (capped by max_iterations) */
+  int32_t max_iterations;
+  int32_t warmstart;             /* SolverMode.SOLVER_USE_WARMSTARTING */
+  float erp;                     /* ContactSolverInfo.erp (0.2) */
+  float warmstart_factor;        /* ContactSolverInfo.warmstartingFactor (0.85) */
+} jbgpu_config;
+
+typedef struct jbgpu_stats {
+  int64_t pairs;        /* overlapping pairs after the last broadphase pass */
+  int64_t contacts;     /* manifold points after the last narrowphase pass */
+  int64_t manifolds;    /* manifolds that produced solver rows */
+  int64_t colors;       /* colour batches of the last solve */
+  int64_t epa_calls;    /* pairs that ran GJK-EPA */
+  int64_t epa_fails;
+  int64_t iterations;   /* solver iterations run in the last solve */
+  float residual;       /* residual after the last iteration */
+  int64_t huge_proxies; /* proxies handled outside the BVH (e.g. planes) */
+} jbgpu_stats;
+
+/* ---- context ---------------------------------------------------------------- */
+int jbgpu_version(void);
+int jbgpu_device_count(int32_t* count);
+void jbgpu_config_default(jbgpu_config* cfg);
+/* New context on the current HIP device.  Replaces `new DbvtBroadphase()` / `new
+ * AxisSweep3(_32)(min, max)` + `new CollisionDispatcher(cfg)` + `new
+ * SequentialImpulseConstraintSolver()` (DiscreteDynamicsWorld.java:86-103). */
+int jbgpu_ctx_create(const jbgpu_config* cfg, jbgpu_ctx** out);
+int jbgpu_ctx_destroy(jbgpu_ctx* ctx);
+/* Copies the last error message of the context (or of the last failed create when ctx is NULL). */
+int jbgpu_last_error(const jbgpu_ctx* ctx, char* buf, int32_t len);
+
+/* ---- shape helpers (host only) ------------------------------------------------ */
+/* Shape record as built by the reference's constructors: SphereShape(r), BoxShape(he),
+ * CapsuleShape(r, height), ConvexHullShape(points) (recalcLocalAabb), StaticPlaneShape(n, c).
+ * params: sphere {r}; box {hx,hy,hz}; capsule {r,height}; hull {first,count} (points in verts);
+ * plane {nx,ny,nz,c}.  (collision/shapes/*.java constructors) */
+int jbgpu_make_shape(int32_t type, const float* params, const float* verts, float* rec);
+/* CollisionShape.calculateLocalInertia(mass, inertia) for a shape record. */
+int jbgpu_shape_local_inertia(const float* rec, const float* verts, float mass, float* inertia3);
+
+/* ---- BroadphaseInterface -------------------------------------------------------- */
+/* createProxy for n AABBs (min xyz, max xyz per proxy); uid_out[i] = creation index + 1
+ * (DbvtBroadphase.java:173-182).  world ids may be NULL (all 0); pairs only form inside a world. */
+int jbgpu_bp_create_proxies(jbgpu_ctx* ctx, int32_t n, const float* aabb6, const int16_t* group,
+                            const int16_t* mask, const int32_t* world, int32_t* uid_out);
+/* setAabb for every proxy (DbvtBroadphase.java:196-228 / AxisSweep3Internal.java:542-579).
+ * aabb6 is a host pointer, or a device pointer when on_device != 0. */
+int jbgpu_bp_set_aabbs(jbgpu_ctx* ctx, int32_t n, const float* aabb6, int32_t on_device);
+/* calculateOverlappingPairs (DbvtBroadphase.java:230, collide :89-150). */
+int jbgpu_bp_calculate_overlapping_pairs(jbgpu_ctx* ctx, int64_t* npairs);
+/* getOverlappingPairCache().getOverlappingPairArray() as sorted keys (uid_lo << 32 | uid_hi). */
+int jbgpu_bp_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, int64_t cap, int64_t* n);
+int jbgpu_bp_get_proxies(jbgpu_ctx* ctx, float* rec, int32_t* stage_current);
+int jbgpu_bp_set_proxies(jbgpu_ctx* ctx, const float* rec, int32_t stage_current);
+int jbgpu_bp_get_quantized(jbgpu_ctx* ctx, int32_t* q6);
+
+/* ---- DynamicsWorld ---------------------------------------------------------------- */
+/* addRigidBody for every body record (DiscreteDynamicsWorld.java:426-440): creates the proxies
+ * (static bodies: filter group 2, mask ~2) with the shapes' AABBs. */
+int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int32_t nverts, const float* verts,
+                       int32_t nbodies, const float* bodies, const int32_t* world_ids);
+/* stepSimulation(dt, 0) nsteps times: exactly one substep per call. */
+int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps);
+/* One step run up to and including phase p: 0 predictUnconstraintMotion, 1 AABB update +
+ * calculateOverlappingPairs, 2 dispatchAllCollisionPairs, 3 solveConstraints, 4 full step. */
+int jbgpu_world_step_until(jbgpu_ctx* ctx, float dt, int32_t phase);
+int jbgpu_world_get_bodies(jbgpu_ctx* ctx, float* bodies);
+int jbgpu_world_set_bodies(jbgpu_ctx* ctx, const float* bodies);
+int jbgpu_world_num_pairs(jbgpu_ctx* ctx, int64_t* n);
+/* pairs (sorted keys) with their persistent manifold records (JBGPU_MANIFOLD_WORDS each) */
+int jbgpu_world_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, float* manifolds);
+int jbgpu_world_set_pairs(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds);
+int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out);
+
+/* ---- ConstraintSolver -------------------------------------------------------------- */
+/* Manifold order of the last solve: pair indices, colour batch after colour batch; a
+ * SequentialImpulseConstraintSolver fed this manifold order without shuffling produces
+ * the same velocities and impulses.  offsets[k]..offsets[k+1] is colour k (may be NULL). */
+int jbgpu_solver_get_order(jbgpu_ctx* ctx, int32_t* order, int64_t cap, int64_t* n, int64_t* offsets,
+                           int32_t* ncolors);
+/* Learned warm start: per solver contact slot (active manifold k, point j -> slot 4k+j) the
+ * initial (normal, lateral1, lateral2) impulses, used instead of the 0.85 * cached impulses
+ * in the next solve; NULL turns it off. */
+int jbgpu_solver_set_warmstart(jbgpu_ctx* ctx, const float* init, int64_t nslots, int32_t on_device);
+/* Per-contact 16-float feature vectors (normal, relative velocity, relative angular velocity,
+ * depth, inverse masses, cached impulses, friction) of the last solve, by contact slot. */
+int jbgpu_solver_enable_features(jbgpu_ctx* ctx, int32_t on);
+int jbgpu_solver_get_features(jbgpu_ctx* ctx, float* out, int64_t nslots);
+int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations);
+/* Residual after each iteration of the last solve. */
+int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32_t* n);
+
+/* ---- measurement ---------------------------------------------------------------------- */
+/* Accumulated time (ms, HIP events recorded on the context stream around the launches) per
+ * region since enabling, JBGPU_TIMER_REGIONS entries: 0 predict, 1 setAabb, 2 BVH traversal,
+ * 3 GJK, 4 manifold update, 5 solver setup, 6 solver iterations, 7 integrate, 8 whole step,
+ * 9 EPA, 10 pair remap; launches[r] counts the kernel launches inside region r. */
+#define JBGPU_TIMER_REGIONS 12
+int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
+int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JBGPU_H */

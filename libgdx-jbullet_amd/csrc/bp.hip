@@ -1,0 +1,674 @@
+// bp.hip — broadphase kernel set for gfx950.
+//
+// Per proxy (one thread each): the world AABB with the contact threshold
+// (collision/dispatch/CollisionWorld.java:195-245), the broadphase's setAabb
+// semantics — DbvtBroadphase's stored-AABB fattening state machine
+// (bp/DbvtBroadphase.java:196-228, bp/Dbvt.java:157-169), AxisSweep3
+// quantisation (bp/AxisSweep3Internal.java:201-216) or raw boxes
+// (bp/SimpleBroadphase.java) — and the stage rotation of DbvtBroadphase.collide
+// (:96-113).  Pair discovery is a linear BVH over the stored boxes (Morton
+// codes, radix sort, Karras hierarchy, atomic bottom-up refit, per-leaf stack
+// traversal reporting only higher sorted leaves); boxes larger than 1e6 m (the
+// StaticPlaneShape's +-1e30 box) stay out of the tree and are tested against
+// every proxy of their world by a flat kernel.  Every reported pair passes the
+// reference predicate exactly (closed-interval float test DbvtAabbMm.java:209-212,
+// or the quantised edge test), the group/mask filter
+// (HashedOverlappingPairCache.java:179-188) and the world-id filter, so the
+// sorted (uid_lo, uid_hi) list equals the reference's overlapping-pair set.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+#include "shapes.h"
+
+namespace jbg {
+
+namespace {
+
+constexpr int STAGECOUNT = 2;
+constexpr float DBVT_BP_MARGIN = 0.05f;
+constexpr float PREDICTED_FRAMES = 2.0f;
+constexpr float HUGE_EXTENT = 1e6f;
+
+struct BpParams {
+  int mode;
+  int stageCurrent;  // before rotation
+  int newStage;
+  float wmin[3], wmax[3], quant[3];
+  uint32_t handleMask, qmask, sentinel;
+  double pad[3];
+};
+
+__device__ __forceinline__ void quantizeDev(const BpParams& P, V3 p, int isMax, int* out) {
+  V3 c = p;
+  c.x = jmax(c.x, P.wmin[0]); c.y = jmax(c.y, P.wmin[1]); c.z = jmax(c.z, P.wmin[2]);
+  c.x = jmin(c.x, P.wmax[0]); c.y = jmin(c.y, P.wmax[1]); c.z = jmin(c.z, P.wmax[2]);
+  V3 v = v3((c.x - P.wmin[0]) * P.quant[0], (c.y - P.wmin[1]) * P.quant[1], (c.z - P.wmin[2]) * P.quant[2]);
+  out[0] = (int)((((uint32_t)jint(v.x) & P.handleMask) | (uint32_t)isMax) & P.qmask);
+  out[1] = (int)((((uint32_t)jint(v.y) & P.handleMask) | (uint32_t)isMax) & P.qmask);
+  out[2] = (int)((((uint32_t)jint(v.z) & P.handleMask) | (uint32_t)isMax) & P.qmask);
+}
+
+// setAabb of the selected broadphase for proxy i, then the collide() stage rotation
+__device__ void setAabbDev(const BpParams& P, int i, V3 mn, V3 mx, bool update, int* stage, float* leaf, float* aabb,
+                           int* q) {
+  if (update) {
+    if (P.mode == JBGPU_BP_SAP16 || P.mode == JBGPU_BP_SAP32) {
+      quantizeDev(P, mn, 0, q + 6 * i);
+      quantizeDev(P, mx, 1, q + 6 * i + 3);
+    } else if (P.mode == JBGPU_BP_DBVT) {
+      float* L = leaf + 6 * i;
+      float* S = aabb + 6 * i;
+      int st = stage[i];
+      if (st == STAGECOUNT) {
+        st3(L, mn);
+        st3(L + 3, mx);
+      } else {
+        V3 lmn = ld3(L), lmx = ld3(L + 3);
+        bool inter = (lmn.x <= mx.x) && (lmx.x >= mn.x) && (lmn.y <= mx.y) && (lmx.y >= mn.y) && (lmn.z <= mx.z) &&
+                     (lmx.z >= mn.z);
+        if (inter) {
+          V3 delta = scl(add(mn, mx), 0.5f);
+          V3 c = scl(add(ld3(S), ld3(S + 3)), 0.5f);
+          delta = sub(delta, c);
+          delta = scl(delta, PREDICTED_FRAMES);
+          bool contained = (lmn.x <= mn.x) && (lmn.y <= mn.y) && (lmn.z <= mn.z) && (lmx.x >= mx.x) &&
+                           (lmx.y >= mx.y) && (lmx.z >= mx.z);
+          if (!contained) {
+            mn = sub(mn, v3(DBVT_BP_MARGIN, DBVT_BP_MARGIN, DBVT_BP_MARGIN));
+            mx = add(mx, v3(DBVT_BP_MARGIN, DBVT_BP_MARGIN, DBVT_BP_MARGIN));
+            if (delta.x > 0) mx.x += delta.x; else mn.x += delta.x;
+            if (delta.y > 0) mx.y += delta.y; else mn.y += delta.y;
+            if (delta.z > 0) mx.z += delta.z; else mn.z += delta.z;
+            st3(L, mn);
+            st3(L + 3, mx);
+          }
+        } else {
+          st3(L, mn);
+          st3(L + 3, mx);
+        }
+      }
+      stage[i] = P.stageCurrent;
+    }
+    st3(aabb + 6 * i, mn);
+    st3(aabb + 6 * i + 3, mx);
+  }
+  if (P.mode == JBGPU_BP_DBVT && stage[i] == P.newStage) {
+    for (int k = 0; k < 6; k++) leaf[6 * i + k] = aabb[6 * i + k];
+    stage[i] = STAGECOUNT;
+  }
+}
+
+__device__ void writeCandidate(const BpParams& P, int i, const float* aabb, const int* q, float* cand, int* isHuge) {
+  V3 mn = ld3(aabb + 6 * i), mx = ld3(aabb + 6 * i + 3);
+  if (P.mode == JBGPU_BP_SAP16 || P.mode == JBGPU_BP_SAP32) {
+    const int* Q = q + 6 * i;
+    float r[6];
+    for (int ax = 0; ax < 3; ax++) {
+      double s = (double)P.quant[ax], w0 = (double)P.wmin[ax];
+      r[ax] = (float)(w0 + (double)(uint32_t)Q[ax] / s - P.pad[ax]);
+      r[3 + ax] = (float)(w0 + ((double)(uint32_t)Q[3 + ax] + 1.0) / s + P.pad[ax]);
+    }
+    mn = v3(r[0], r[1], r[2]);
+    mx = v3(r[3], r[4], r[5]);
+  }
+  float* C = cand + 8 * i;
+  C[0] = mn.x; C[1] = mn.y; C[2] = mn.z; C[3] = 0.0f;
+  C[4] = mx.x; C[5] = mx.y; C[6] = mx.z; C[7] = 0.0f;
+  float e = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+  isHuge[i] = !(e < HUGE_EXTENT) ? 1 : 0;
+}
+
+__global__ void kSetAabbBodies(BpParams P, int n, const float* __restrict__ shapes, const float* __restrict__ tr,
+                               const float* __restrict__ props, int* act, int* stage, float* leaf, float* aabb, int* q,
+                               float* cand, int* isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = act[i];
+  bool active = (a != JBGPU_ISLAND_SLEEPING) && (a != JBGPU_DISABLE_SIMULATION);
+  bool update = false;
+  V3 mn = v3(0, 0, 0), mx = v3(0, 0, 0);
+  if (active) {
+    const float* pr = props + 8 * i;
+    Shape s = loadShape(shapes + JBGPU_SHAPE_WORDS * ldi(pr + 6));
+    Tr t;
+    for (int k = 0; k < 9; k++) t.b.m[k] = tr[12 * i + k];
+    t.o = ld3(tr + 12 * i + 9);
+    shapeAabb(s, t, mn, mx);
+    V3 ct = v3(kContactThreshold, kContactThreshold, kContactThreshold);
+    mn = sub(mn, ct);
+    mx = add(mx, ct);
+    bool isStatic = (ldi(pr + 7) & JBGPU_BODY_STATIC) != 0;
+    if (isStatic || len2(sub(mx, mn)) < 1e12f) {
+      update = true;
+    } else if (a != JBGPU_DISABLE_DEACTIVATION && a != JBGPU_DISABLE_SIMULATION) {
+      act[i] = JBGPU_DISABLE_SIMULATION;
+    }
+  }
+  setAabbDev(P, i, mn, mx, update, stage, leaf, aabb, q);
+  writeCandidate(P, i, aabb, q, cand, isHuge);
+}
+
+__global__ void kSetAabbArray(BpParams P, int n, const float* __restrict__ in6, int* stage, float* leaf, float* aabb,
+                              int* q, float* cand, int* isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  setAabbDev(P, i, ld3(in6 + 6 * i), ld3(in6 + 6 * i + 3), true, stage, leaf, aabb, q);
+  writeCandidate(P, i, aabb, q, cand, isHuge);
+}
+
+__device__ __forceinline__ unsigned int fOrd(float f) {  // order-preserving float -> uint
+  unsigned int u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ordF(unsigned int u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void kBoundsInit(unsigned int* b) {
+  if (threadIdx.x < 3) b[threadIdx.x] = 0xffffffffu;
+  else if (threadIdx.x < 6) b[threadIdx.x] = 0u;
+}
+
+__global__ void kBounds(int n, const float* __restrict__ cand, const int* __restrict__ isHuge, unsigned int* b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  if (i < n && !isHuge[i]) {
+    const float* C = cand + 8 * i;
+    for (int k = 0; k < 3; k++) {
+      float c = (C[k] + C[4 + k]) * 0.5f;
+      if (c == c) {
+        lo[k] = fOrd(c);
+        hi[k] = fOrd(c);
+      }
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[k] = min(lo[k], (unsigned int)__shfl_xor((int)lo[k], off));
+      hi[k] = max(hi[k], (unsigned int)__shfl_xor((int)hi[k], off));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int k = 0; k < 3; k++) {
+      atomicMin(b + k, lo[k]);
+      atomicMax(b + 3 + k, hi[k]);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void kMorton(int n, const float* __restrict__ cand, const int* __restrict__ isHuge,
+                        const unsigned int* __restrict__ b, uint32_t* code, int* idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  idx[i] = i;
+  if (isHuge[i]) {
+    code[i] = 0xffffffffu;
+    return;
+  }
+  const float* C = cand + 8 * i;
+  uint32_t q3[3];
+  for (int k = 0; k < 3; k++) {
+    float lo = ordF(b[k]), hi = ordF(b[3 + k]);
+    float c = (C[k] + C[4 + k]) * 0.5f;
+    float ext = hi - lo;
+    float u = ext > 0.0f ? (c - lo) / ext : 0.0f;
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    if (!(u == u)) u = 0.0f;
+    q3[k] = min((uint32_t)(u * 1024.0f), 1023u);
+  }
+  code[i] = (expand10(q3[0]) << 2) | (expand10(q3[1]) << 1) | expand10(q3[2]);
+}
+
+__global__ void kGatherLeaves(int n, const int* __restrict__ order, const float* __restrict__ cand, float* leafBox) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4* src = (const float4*)(cand + 8 * order[s]);
+  float4* dst = (float4*)(leafBox + 8 * s);
+  dst[0] = src[0];
+  dst[1] = src[1];
+}
+
+__device__ __forceinline__ int deltaK(const uint32_t* code, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = code[i], b = code[j];
+  if (a == b) return 32 + __clz((unsigned)(i ^ j));
+  return __clz(a ^ b);
+}
+
+// Karras (2012) hierarchy: internal node i in [0, n-2]; child < 0 encodes leaf -(child+1)
+__global__ void kBuild(int n, const uint32_t* __restrict__ code, int* child, int* last, int* parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (deltaK(code, n, i, i + 1) - deltaK(code, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = deltaK(code, n, i, i - d);
+  int lmax = 2;
+  while (deltaK(code, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (deltaK(code, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = deltaK(code, n, i, j);
+  int s = 0;
+  int div = 2;
+  int t = (l + div - 1) / div;
+  while (true) {
+    if (deltaK(code, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t == 1) break;
+    div <<= 1;
+    t = (l + div - 1) / div;
+  }
+  int gamma = i + s * d + min(d, 0);
+  int lo = min(i, j), hi = max(i, j);
+  int left = (lo == gamma) ? -(gamma + 1) : gamma;
+  int right = (hi == gamma + 1) ? -(gamma + 2) : gamma + 1;
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  last[i] = hi;
+  // parent array: internal nodes at [0, n-1), leaves at [n-1, 2n-1)
+  parent[left >= 0 ? left : (n - 1) + (-left - 1)] = i;
+  parent[right >= 0 ? right : (n - 1) + (-right - 1)] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+__global__ void kRefit(int n, const int* __restrict__ child, const int* __restrict__ parent, const float* leafBox,
+                       float* nodeBox, int* flags) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int node = parent[(n - 1) + s];
+  while (node >= 0) {
+    __threadfence();
+    if (atomicAdd(flags + node, 1) == 0) return;
+    __threadfence();
+    float r[8];
+    for (int c = 0; c < 2; c++) {
+      int ch = child[2 * node + c];
+      const float* B = ch < 0 ? leafBox + 8 * (-ch - 1) : nodeBox + 8 * ch;
+      // boxes written by other waves in this launch: read them coherently (agent scope, past L1)
+      float4 b0, b1;
+      b0.x = __hip_atomic_load(B + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b0.y = __hip_atomic_load(B + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b0.z = __hip_atomic_load(B + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b1.x = __hip_atomic_load(B + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b1.y = __hip_atomic_load(B + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b1.z = __hip_atomic_load(B + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c == 0) {
+        r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[4] = b1.x; r[5] = b1.y; r[6] = b1.z;
+      } else {
+        r[0] = fminf(r[0], b0.x); r[1] = fminf(r[1], b0.y); r[2] = fminf(r[2], b0.z);
+        r[4] = fmaxf(r[4], b1.x); r[5] = fmaxf(r[5], b1.y); r[6] = fmaxf(r[6], b1.z);
+      }
+    }
+    float4* D = (float4*)(nodeBox + 8 * node);
+    D[0] = make_float4(r[0], r[1], r[2], 0.0f);
+    D[1] = make_float4(r[4], r[5], r[6], 0.0f);
+    node = parent[node];
+  }
+}
+
+struct PairTest {
+  int mode;
+  const float* aabb;
+  const int* q;
+  const int* filt;
+  const int* world;
+};
+
+__device__ __forceinline__ bool exactOverlap(const PairTest& T, int a, int b) {
+  if ((T.world[a] != T.world[b])) return false;
+  int fa = T.filt[a], fb = T.filt[b];
+  short ga = (short)(fa & 0xffff), ma = (short)(fa >> 16), gb = (short)(fb & 0xffff), mb = (short)(fb >> 16);
+  if (!(((ga & mb) != 0) && ((gb & ma) != 0))) return false;
+  if (T.mode == JBGPU_BP_SAP16 || T.mode == JBGPU_BP_SAP32) {
+    const int* A = T.q + 6 * a;
+    const int* B = T.q + 6 * b;
+    for (int ax = 0; ax < 3; ax++)
+      if (A[3 + ax] < B[ax] || B[3 + ax] < A[ax]) return false;
+    return true;
+  }
+  const float* A = T.aabb + 6 * a;
+  const float* B = T.aabb + 6 * b;
+  return (A[0] <= B[3]) && (A[3] >= B[0]) && (A[1] <= B[4]) && (A[4] >= B[1]) && (A[2] <= B[5]) && (A[5] >= B[2]);
+}
+
+__device__ __forceinline__ void emit(int a, int b, int bits, unsigned long long* counter, uint64_t* out, int64_t cap) {
+  uint64_t lo = (uint64_t)min(a, b) + 1, hi = (uint64_t)max(a, b) + 1;
+  unsigned long long k = atomicAdd(counter, 1ull);
+  if ((int64_t)k < cap) out[k] = (lo << bits) | hi;
+}
+
+__device__ __forceinline__ bool boxOverlap(const float4& amn, const float4& amx, const float4& bmn, const float4& bmx) {
+  return (amn.x <= bmx.x) && (amx.x >= bmn.x) && (amn.y <= bmx.y) && (amx.y >= bmn.y) && (amn.z <= bmx.z) &&
+         (amx.z >= bmn.z);
+}
+
+__global__ __launch_bounds__(256) void kTraverse(int n, const int* __restrict__ order, const float* __restrict__ leafBox,
+                                                 const float* __restrict__ nodeBox, const int* __restrict__ child,
+                                                 const int* __restrict__ last, PairTest T, int bits,
+                                                 unsigned long long* counter, uint64_t* out, int64_t cap, int* err) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || n < 2) return;
+  const float4 qmn = ((const float4*)leafBox)[2 * s];
+  const float4 qmx = ((const float4*)leafBox)[2 * s + 1];
+  const int pa = order[s];
+  int stack[64];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    int node = stack[--sp];
+    for (int c = 0; c < 2; c++) {
+      int ch = child[2 * node + c];
+      if (ch < 0) {
+        int t = -ch - 1;
+        if (t > s) {
+          const float4 bmn = ((const float4*)leafBox)[2 * t];
+          const float4 bmx = ((const float4*)leafBox)[2 * t + 1];
+          if (boxOverlap(qmn, qmx, bmn, bmx)) {
+            int pb = order[t];
+            if (exactOverlap(T, pa, pb)) emit(pa, pb, bits, counter, out, cap);
+          }
+        }
+      } else if (last[ch] > s) {
+        const float4 bmn = ((const float4*)nodeBox)[2 * ch];
+        const float4 bmx = ((const float4*)nodeBox)[2 * ch + 1];
+        if (boxOverlap(qmn, qmx, bmn, bmx)) {
+          if (sp < 64)
+            stack[sp++] = ch;
+          else
+            atomicOr(err, 1);
+        }
+      }
+    }
+  }
+}
+
+// every proxy against the huge proxies of its world (huge list sorted by world, then index)
+__global__ void kHuge(int n, int nh, const int* __restrict__ hugeList, const int* __restrict__ hugeWorld,
+                      const int* __restrict__ isHuge, const float* __restrict__ cand, PairTest T, int bits,
+                      unsigned long long* counter, uint64_t* out, int64_t cap) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int w = T.world[j];
+  int lo = 0, hi = nh;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (hugeWorld[mid] < w) lo = mid + 1; else hi = mid;
+  }
+  const float4 bmn = ((const float4*)cand)[2 * j];
+  const float4 bmx = ((const float4*)cand)[2 * j + 1];
+  bool jh = isHuge[j] != 0;
+  for (int k = lo; k < nh && hugeWorld[k] == w; k++) {
+    int h = hugeList[k];
+    if (h == j) continue;
+    if (jh && h > j) continue;
+    const float4 amn = ((const float4*)cand)[2 * h];
+    const float4 amx = ((const float4*)cand)[2 * h + 1];
+    if (!boxOverlap(amn, amx, bmn, bmx)) continue;
+    if (exactOverlap(T, h, j)) emit(h, j, bits, counter, out, cap);
+  }
+}
+
+__global__ void kFlagHugeList(int n, const int* __restrict__ isHuge, const int* __restrict__ world, int* list, int* wk,
+                              unsigned long long* counter) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !isHuge[i]) return;
+  unsigned long long k = atomicAdd(counter, 1ull);
+  list[k] = i;
+  wk[k] = world[i];
+}
+
+// remap: unpack sorted keys, carry manifolds of surviving pairs, initialise new ones
+__global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits, uint64_t* keysNew, float* manNew,
+                       int64_t nOld, const uint64_t* __restrict__ keysOld, const float* __restrict__ manOld,
+                       const float* __restrict__ shapes, const float* __restrict__ props, int* ptype) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  uint64_t p = packed[i];
+  uint64_t mask = (1ull << bits) - 1ull;
+  uint64_t key = ((p >> bits) << 32) | (p & mask);
+  keysNew[i] = key;
+  int a = (int)(key >> 32) - 1, b = (int)(key & 0xffffffffu) - 1;
+  int ta = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
+  int tb = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * b + 6));
+  int type;
+  if (ta == JBGPU_SHAPE_SPHERE && tb == JBGPU_SHAPE_SPHERE) type = JBGPU_PAIR_SPHERE_SPHERE;
+  else if (ta == JBGPU_SHAPE_PLANE && tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_NONE;
+  else if (ta == JBGPU_SHAPE_PLANE || tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_CONVEX_PLANE;
+  else type = JBGPU_PAIR_CONVEX_CONVEX;
+  ptype[i] = type;
+  int64_t lo = 0, hi = nOld;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keysOld[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  float4* dst = (float4*)(manNew + (size_t)JBGPU_MANIFOLD_WORDS * i);
+  if (lo < nOld && keysOld[lo] == key) {
+    const float4* src = (const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * lo);
+    for (int k = 0; k < JBGPU_MANIFOLD_WORDS / 4; k++) dst[k] = src[k];
+  } else {
+    int b0 = a, b1 = b;
+    if (type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE) { b0 = b; b1 = a; }
+    float4 z = make_float4(0, 0, 0, 0);
+    for (int k = 0; k < JBGPU_MANIFOLD_WORDS / 4; k++) dst[k] = z;
+    float* hdr = manNew + (size_t)JBGPU_MANIFOLD_WORDS * i;
+    sti(hdr + 0, 0);
+    sti(hdr + 1, b0);
+    sti(hdr + 2, b1);
+    sti(hdr + 3, type);
+  }
+}
+
+// broadphase-only contexts (no bodies, no shapes): unpack the sorted keys, no manifolds
+__global__ void kUnpackKeys(int64_t np, const uint64_t* __restrict__ packed, int bits, uint64_t* keysNew) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  uint64_t p = packed[i];
+  uint64_t mask = (1ull << bits) - 1ull;
+  keysNew[i] = ((p >> bits) << 32) | (p & mask);
+}
+
+BpParams makeParams(const Ctx& c) {
+  BpParams P{};
+  P.mode = c.bpMode;
+  P.stageCurrent = c.stageCurrent;
+  P.newStage = (c.stageCurrent + 1) % STAGECOUNT;
+  for (int k = 0; k < 3; k++) {
+    P.wmin[k] = c.wmin[k];
+    P.wmax[k] = c.wmax[k];
+    P.quant[k] = c.quant[k];
+    P.pad[k] = 2.5 * ((double)c.wmax[k] - (double)c.wmin[k]) / (double)c.sentinel;
+  }
+  P.handleMask = c.handleMask;
+  P.qmask = c.qmask;
+  P.sentinel = c.sentinel;
+  return P;
+}
+
+void ensureProxyBuffers(Ctx& c) {
+  size_t n = (size_t)c.np;
+  c.cand.ensure(8 * n);
+  c.isHuge.ensure(n);
+  c.morton.ensure(n);
+  c.mortonAlt.ensure(n);
+  c.order.ensure(n);
+  c.orderAlt.ensure(n);
+  c.leafBox.ensure(8 * n);
+  c.nodeBox.ensure(8 * n);
+  c.nodeChild.ensure(2 * n);
+  c.nodeLast.ensure(n);
+  c.parent.ensure(2 * n);
+  c.flags.ensure(n);
+  c.hugeList.ensure(n);
+  c.hugeListAlt.ensure(n);
+  c.hugeWorld.ensure(n);
+  c.hugeWorldAlt.ensure(n);
+  c.bounds.ensure(8);
+  c.counter.ensure(8);
+}
+
+}  // namespace
+
+void bpInitQuant(Ctx& c) {
+  if (c.bpMode == JBGPU_BP_SAP16) {
+    c.handleMask = 0xfffe;
+    c.sentinel = 0xffff;
+    c.qmask = 0xffff;
+  } else {
+    c.handleMask = 0xfffffffe;
+    c.sentinel = 0x7fffffff;
+    c.qmask = 0xffffffff;
+  }
+  float maxInt = (float)(int32_t)c.sentinel;
+  for (int k = 0; k < 3; k++) c.quant[k] = maxInt / (c.wmax[k] - c.wmin[k]);
+}
+
+void bpUpdateFromBodies(Ctx& c) {
+  ensureProxyBuffers(c);
+  BpParams P = makeParams(c);
+  tBegin(c, T_SETAABB);
+  hipLaunchKernelGGL(kSetAabbBodies, dim3(gridFor(c.np)), dim3(256), 0, c.stream, P, c.np, c.shapes.p, c.tr.p,
+                     c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
+  tEnd(c, T_SETAABB);
+  c.stageCurrent = P.newStage;
+}
+
+void bpUpdateFromAabbs(Ctx& c, const float* dAabb6) {
+  ensureProxyBuffers(c);
+  BpParams P = makeParams(c);
+  tBegin(c, T_SETAABB);
+  hipLaunchKernelGGL(kSetAabbArray, dim3(gridFor(c.np)), dim3(256), 0, c.stream, P, c.np, dAabb6, c.stage.p, c.leaf.p,
+                     c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
+  tEnd(c, T_SETAABB);
+  c.stageCurrent = P.newStage;
+}
+
+static int bitsFor(int64_t n) {
+  int b = 1;
+  while ((1ll << b) <= n) b++;
+  return b;
+}
+
+void bpFindPairs(Ctx& c) {
+  const int n = c.np;
+  const int bits = bitsFor((int64_t)n + 1);
+  hipStream_t s = c.stream;
+  // huge proxies: list sorted by (world, index)
+  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(kFlagHugeList, dim3(gridFor(n)), dim3(256), 0, s, n, c.isHuge.p, c.world.p, c.hugeList.p,
+                     c.hugeWorld.p, c.counter.p + 1);
+  // centroid bounds and Morton codes
+  hipLaunchKernelGGL(kBoundsInit, dim3(1), dim3(64), 0, s, (unsigned int*)c.bounds.p);
+  hipLaunchKernelGGL(kBounds, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, (unsigned int*)c.bounds.p);
+  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p,
+                     (const unsigned int*)c.bounds.p, c.morton.p, c.order.p);
+  unsigned long long hcount = 0;
+  JB_CHECK(hipMemcpyAsync(&hcount, c.counter.p + 1, sizeof(hcount), hipMemcpyDeviceToHost, s));
+  // sort leaves by Morton code (huge proxies carry 0xffffffff and sort last)
+  size_t tmpBytes = 0;
+  hipcub::DoubleBuffer<uint32_t> dk(c.morton.p, c.mortonAlt.p);
+  hipcub::DoubleBuffer<int> dv(c.order.p, c.orderAlt.p);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, 32, s));
+  c.cubTmp.ensure(tmpBytes);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, 32, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  c.nHuge = (int)hcount;
+  c.st.huge = c.nHuge;
+  const uint32_t* code = dk.Current();
+  const int* order = dv.Current();
+  const int nn = n - c.nHuge;  // leaves in the tree
+  if (c.nHuge > 0) {  // sort the huge list by (world, index) for the per-world lookup
+    size_t tb = 0;
+    hipcub::DoubleBuffer<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
+    hipcub::DoubleBuffer<int> hv(c.hugeList.p, c.hugeListAlt.p);
+    // stable order by index first, then by world
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hv, hk, c.nHuge, 0, bits, s));
+    c.cubTmp.ensure(tb);
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hv, hk, c.nHuge, 0, bits, s));
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, c.nHuge, 0, 32, s));
+    c.cubTmp.ensure(tb);
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, c.nHuge, 0, 32, s));
+    if (hk.Current() != c.hugeWorld.p)
+      JB_CHECK(hipMemcpyAsync(c.hugeWorld.p, hk.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (hv.Current() != c.hugeList.p)
+      JB_CHECK(hipMemcpyAsync(c.hugeList.p, hv.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+  }
+  if (nn >= 2) {
+    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.leafBox.p);
+    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeLast.p,
+                       c.parent.p);
+    JB_CHECK(hipMemsetAsync(c.flags.p, 0, (size_t)nn * sizeof(int), s));
+    hipLaunchKernelGGL(kRefit, dim3(gridFor(nn)), dim3(256), 0, s, nn, c.nodeChild.p, c.parent.p, c.leafBox.p,
+                       c.nodeBox.p, c.flags.p);
+  }
+  PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
+  for (int attempt = 0; attempt < 3; attempt++) {
+    if (c.pairCap < 1024) c.pairCap = std::max<int64_t>(1024, 8 * (int64_t)n);
+    c.rawKeys.ensure(c.pairCap);
+    c.rawKeysAlt.ensure(c.pairCap);
+    JB_CHECK(hipMemsetAsync(c.counter.p, 0, 2 * sizeof(unsigned long long), s));
+    if (attempt == 0) tBegin(c, T_TRAVERSE);
+    if (nn >= 2)
+      hipLaunchKernelGGL(kTraverse, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.leafBox.p, c.nodeBox.p,
+                         c.nodeChild.p, c.nodeLast.p, T, bits, c.counter.p, c.rawKeys.p, c.pairCap,
+                         (int*)(c.counter.p + 2));
+    if (attempt == 0) tEnd(c, T_TRAVERSE);
+    if (c.nHuge > 0)
+      hipLaunchKernelGGL(kHuge, dim3(gridFor(n)), dim3(256), 0, s, n, c.nHuge, c.hugeList.p, c.hugeWorld.p,
+                         c.isHuge.p, c.cand.p, T, bits, c.counter.p, c.rawKeys.p, c.pairCap);
+    unsigned long long cnt[3] = {0, 0, 0};
+    JB_CHECK(hipMemcpyAsync(cnt, c.counter.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
+    if (cnt[2]) throw std::runtime_error("broadphase traversal stack overflow");
+    if ((int64_t)cnt[0] <= c.pairCap) {
+      c.npairsOld = c.npairs;
+      c.npairs = (int64_t)cnt[0];
+      break;
+    }
+    c.pairCap = (int64_t)cnt[0] + (int64_t)cnt[0] / 2;
+    if (attempt == 2) throw std::runtime_error("broadphase pair capacity");
+  }
+  // sort the packed keys (2*bits significant bits)
+  if (c.npairs > 1) {
+    size_t tb = 0;
+    hipcub::DoubleBuffer<uint64_t> kk(c.rawKeys.p, c.rawKeysAlt.p);
+    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kk, (int)c.npairs, 0, 2 * bits, s));
+    c.cubTmp.ensure(tb);
+    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(c.cubTmp.p, tb, kk, (int)c.npairs, 0, 2 * bits, s));
+    if (kk.Current() != c.rawKeys.p)
+      JB_CHECK(hipMemcpyAsync(c.rawKeys.p, kk.Current(), c.npairs * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  }
+  c.st.pairs = c.npairs;
+}
+
+void bpRemapManifolds(Ctx& c) {
+  const int bits = bitsFor((int64_t)c.np + 1);
+  int nxt = 1 - c.cur;
+  c.keys[nxt].ensure(std::max<int64_t>(c.npairs, 1));
+  if (!c.haveWorld) {  // no bodies: the pair set alone
+    tBegin(c, T_REMAP);
+    if (c.npairs > 0)
+      hipLaunchKernelGGL(kUnpackKeys, dim3(gridFor(c.npairs)), dim3(256), 0, c.stream, c.npairs, c.rawKeys.p, bits,
+                         c.keys[nxt].p);
+    tEnd(c, T_REMAP);
+    c.cur = nxt;
+    return;
+  }
+  c.man[nxt].ensure((size_t)JBGPU_MANIFOLD_WORDS * std::max<int64_t>(c.npairs, 1));
+  c.ptype.ensure(std::max<int64_t>(c.npairs, 1));
+  tBegin(c, T_REMAP);
+  if (c.npairs > 0)
+    hipLaunchKernelGGL(kRemap, dim3(gridFor(c.npairs)), dim3(256), 0, c.stream, c.npairs, c.rawKeys.p, bits,
+                       c.keys[nxt].p, c.man[nxt].p, c.npairsOld, c.keys[c.cur].p, c.man[c.cur].p, c.shapes.p,
+                       c.props.p, c.ptype.p);
+  tEnd(c, T_REMAP);
+  c.cur = nxt;
+}
+
+}  // namespace jbg

@@ -1,0 +1,197 @@
+// internal.h — libjbgpu context, device buffers and the launchers shared by
+// the kernel files.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/jbgpu_format.h"
+
+namespace jbg {
+
+#define JB_CHECK(x)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t k) {
+    if (k <= n) return;
+    if (p) {  // no kernel may still be reading the old buffer
+      JB_CHECK(hipDeviceSynchronize());
+      (void)hipFree(p);
+    }
+    size_t cap = k + k / 4 + 64;
+    p = nullptr;
+    JB_CHECK(hipMalloc((void**)&p, cap * sizeof(T)));
+    n = cap;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// EPA job workspace sizes (per queued pair)
+constexpr int kEpaFaces = 1024;
+constexpr int kEpaVerts = 300;
+constexpr int kEpaHe = 160;
+constexpr int kEpaFaceWords = 16;
+constexpr int kEpaSlotWords = kEpaFaces * kEpaFaceWords + kEpaVerts * 6 + kEpaHe * 4 + 8;
+constexpr int kEpaBatch = 8192;
+constexpr int kTimerRegions = 12;
+
+// solver rows per manifold: 4 contacts x (normal row + 2 friction rows)
+constexpr int kRowWords = 20;   // normal(3) rel1xn(3) rel2xn(3) angA(3) angB(3) jacInv pen rest fric applied
+constexpr int kContactRowWords = 3 * kRowWords;
+constexpr int kManifoldRowWords = 4 * kContactRowWords + 4;  // + [valid mask, bodyA, bodyB, _]
+
+struct Stats {
+  int64_t pairs = 0, manifoldsActive = 0, contacts = 0, rows = 0, colors = 0, epaCalls = 0, epaFails = 0;
+  int64_t iterations = 0;
+  float residual = 0;
+  int64_t huge = 0;
+};
+
+struct Ctx {
+  hipStream_t stream = nullptr;
+  std::string err;
+  // configuration
+  int bpMode = JBGPU_BP_DBVT;
+  float wmin[3] = {-1000, -1000, -1000}, wmax[3] = {1000, 1000, 1000};
+  float quant[3] = {0, 0, 0};
+  uint32_t handleMask = 0xfffffffe, sentinel = 0x7fffffff, qmask = 0xffffffff;
+  int numIterations = 10;
+  int solverOrder = JBGPU_ORDER_COLOR;
+  int deactivationDisabled = 1;
+  float tolerance = 0.0f;
+  int maxIterations = 100;
+  int warmstart = 1;
+  float erp = 0.2f, warmFactor = 0.85f, linearSlop = 0.0f;
+  float gravity[3] = {0.0f, -9.81f, 0.0f};
+  // world
+  bool haveWorld = false;
+  int nb = 0, ns = 0, nv = 0;
+  DBuf<float> shapes, verts;
+  DBuf<float> tr;     // nb*12
+  DBuf<float> vel;    // nb*8  (lv xyz _, av xyz _)
+  DBuf<float> svel;   // solver working velocities
+  DBuf<float> props;  // nb*8  (invMass, invI xyz, friction, restitution, shape(int), flags(int))
+  DBuf<float> grav;   // nb*4
+  DBuf<int> act;      // activation
+  DBuf<float> deact;  // deactivation time
+  DBuf<int> world;    // world id per body/proxy
+  // broadphase
+  int np = 0;  // proxies
+  int stageCurrent = 0;
+  DBuf<int> stage;
+  DBuf<float> leaf, aabb;  // np*6
+  DBuf<int> filt;          // group | mask << 16
+  DBuf<int> q;             // quantised edges np*6
+  DBuf<float> cand;        // candidate boxes np*6 (float4 pairs: min xyz pad, max xyz pad -> np*8)
+  DBuf<uint32_t> morton, mortonAlt;
+  DBuf<int> order, orderAlt;  // sorted leaf -> proxy
+  DBuf<int> isHuge;
+  DBuf<int> hugeList, hugeListAlt, hugeWorld, hugeWorldAlt;
+  int nHuge = 0;
+  DBuf<float> leafBox;   // sorted leaves np*8
+  DBuf<float> nodeBox;   // internal nodes np*8
+  DBuf<int> nodeChild;   // np*2
+  DBuf<int> nodeLast;    // np
+  DBuf<int> parent;      // 2*np
+  DBuf<int> flags;       // np
+  DBuf<float> bounds;    // 8
+  DBuf<uint64_t> rawKeys, rawKeysAlt;
+  DBuf<unsigned long long> counter;  // misc device counters
+  DBuf<uint8_t> cubTmp;
+  int64_t pairCap = 0;
+  // pairs and manifolds (double buffered for the remap)
+  int cur = 0;
+  int64_t npairs = 0, npairsOld = 0;
+  DBuf<uint64_t> keys[2];
+  DBuf<float> man[2];
+  DBuf<int> ptype;
+  DBuf<int> typeKey, typeKeyAlt, perm, permAlt;
+  int64_t binOff[5] = {0, 0, 0, 0, 0};
+  DBuf<float> candC;  // contact candidates npairs*8
+  DBuf<int> epaQueue;
+  DBuf<float> epaWork;
+  DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
+  // solver
+  DBuf<float> rows;
+  DBuf<int> mActive;  // active manifold list
+  DBuf<unsigned long long> bodyPrio;
+  DBuf<unsigned long long> bodyMask;
+  DBuf<int> color, colorAlt;
+  DBuf<uint64_t> colorKey, colorKeyAlt;
+  DBuf<int> colorOrder, colorOrderAlt;
+  std::vector<int64_t> colorOff;
+  DBuf<double> resid;  // per iteration: dd, nn
+  DBuf<int> done;
+  DBuf<float> learned;  // per-contact learned warm-start impulses (npairs*4*3)
+  bool useLearned = false;
+  int64_t learnedSlots = 0;
+  DBuf<float> features;  // per-contact features npairs*4*16
+  bool dumpFeatures = false;
+  int64_t nActive = 0;
+  int ncolors = 0;
+  Stats st;
+  // timing of kernel regions (HIP events on the context stream)
+  bool timing = false;
+  hipEvent_t evA[kTimerRegions] = {}, evB[kTimerRegions] = {};
+  bool pending[kTimerRegions] = {};
+  double ms[kTimerRegions] = {};
+  int64_t launches[kTimerRegions] = {};
+  int64_t timedSteps = 0;
+  std::vector<float> residHist;
+};
+
+// timer regions
+enum {
+  T_PREDICT = 0, T_SETAABB, T_TRAVERSE, T_GJK, T_MANIFOLD, T_SOLVE_SETUP, T_SOLVE_ITER, T_INTEGRATE, T_STEP,
+  T_EPA, T_REMAP
+};
+inline void tBegin(Ctx& c, int r) {
+  if (c.timing) (void)hipEventRecord(c.evA[r], c.stream);
+}
+inline void tEnd(Ctx& c, int r, int64_t nlaunch = 1) {
+  if (!c.timing) return;
+  (void)hipEventRecord(c.evB[r], c.stream);
+  c.pending[r] = true;
+  c.launches[r] += nlaunch;
+}
+inline void tCollect(Ctx& c) {  // after a stream synchronisation
+  for (int r = 0; r < kTimerRegions; r++) {
+    if (!c.pending[r]) continue;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, c.evA[r], c.evB[r]) == hipSuccess) c.ms[r] += t;
+    c.pending[r] = false;
+  }
+}
+
+// broadphase
+void bpInitQuant(Ctx& c);
+void bpUpdateFromBodies(Ctx& c);                        // A1 + A2 (+ collide stage rotation)
+void bpUpdateFromAabbs(Ctx& c, const float* dAabb6);    // setAabb for all proxies from a device array
+void bpFindPairs(Ctx& c);                                // A3-A7: sorted canonical keys in keys[cur]
+void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
+// narrowphase
+void npDispatchAll(Ctx& c);                              // A8-A16
+// solver
+void solverSolve(Ctx& c, float dt);                      // A18-A20
+// integration
+void worldPredict(Ctx& c, float dt);
+void worldIntegrate(Ctx& c, float dt);
+void worldActivation(Ctx& c, float dt);
+
+inline int gridFor(int64_t n, int block = 256) { return (int)((n + block - 1) / block); }
+
+}  // namespace jbg

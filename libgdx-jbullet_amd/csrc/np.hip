@@ -1,0 +1,1216 @@
+// np.hip — narrowphase kernel set for gfx950: the collision dispatcher's work
+// over the overlapping-pair list, binned by shape-pair type.
+//
+//   sphere-sphere  SphereSphereCollisionAlgorithm.processCollision (collision/dispatch/
+//                  SphereSphereCollisionAlgorithm.java:73-134)
+//   convex-plane   ConvexPlaneCollisionAlgorithm.processCollision (:75-136)
+//   convex-convex  GjkPairDetector.getClosestPoints (collision/narrowphase/
+//                  GjkPairDetector.java:73-316) with the Voronoi simplex solver
+//                  (VoronoiSimplexSolver.java), falling back to GjkEpaSolver
+//                  (GjkEpaSolver.java:347-911) for penetrations
+//   manifold       ManifoldResult.addContactPoint / refreshContactPoints
+//                  (ManifoldResult.java:92-191) on PersistentManifold
+//                  (PersistentManifold.java:83-372)
+// One thread per pair; the expanding-polytope stage runs as a second kernel over
+// a queue of the pairs that need it, each with a face/vertex pool in a global
+// workspace slot.  Candidates (valid, normal on B, point on B, depth) are
+// written per pair and folded into the manifold by a final per-pair kernel.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.h"
+#include "shapes.h"
+
+namespace jbg {
+
+namespace {
+
+struct Cand {  // 8 floats per pair
+  float valid, nx, ny, nz, px, py, pz, depth;
+};
+
+__device__ __forceinline__ Tr loadTr(const float* tr, int i) {
+  Tr t;
+  const float* p = tr + 12 * i;
+  for (int k = 0; k < 9; k++) t.b.m[k] = p[k];
+  t.o = v3(p[9], p[10], p[11]);
+  return t;
+}
+__device__ __forceinline__ Shape bodyShape(const float* shapes, const float* props, int i) {
+  return loadShape(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * i + 6));
+}
+__device__ __forceinline__ bool isActiveState(int a) {
+  return a != JBGPU_ISLAND_SLEEPING && a != JBGPU_DISABLE_SIMULATION;
+}
+__device__ __forceinline__ void putCand(float* out, int i, bool valid, V3 n, V3 p, float d) {
+  float* c = out + 8 * i;
+  c[0] = valid ? 1.0f : 0.0f;
+  c[1] = n.x; c[2] = n.y; c[3] = n.z;
+  c[4] = p.x; c[5] = p.y; c[6] = p.z;
+  c[7] = d;
+}
+
+// ------------------------------------------------------------------ binning
+__global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
+                         const int* __restrict__ act, int* tkey, int* idx, unsigned long long* counts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  int t = ptype[i];
+  if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
+  tkey[i] = t;
+  idx[i] = (int)i;
+  atomicAdd(counts + t, 1ull);
+}
+
+// ------------------------------------------------------------------ sphere-sphere
+__global__ void kSphereSphere(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+                              const float* __restrict__ shapes, const float* __restrict__ props,
+                              const float* __restrict__ tr, float* cand) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  int i = list[k];
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  Shape s0 = bodyShape(shapes, props, a), s1 = bodyShape(shapes, props, b);
+  V3 o0 = ld3(tr + 12 * a + 9), o1 = ld3(tr + 12 * b + 9);
+  V3 diff = sub(o0, o1);
+  float l = len(diff);
+  float r0 = s0.dims.x, r1 = s1.dims.x;
+  if (l > (r0 + r1)) {
+    putCand(cand, i, false, v3(0, 0, 0), v3(0, 0, 0), 0.0f);
+    return;
+  }
+  float dist = l - (r0 + r1);
+  V3 nrm = v3(1, 0, 0);
+  if (l > kFltEps) nrm = scl(diff, 1.0f / l);
+  V3 pos1 = add(o1, scl(nrm, r1));
+  putCand(cand, i, true, nrm, pos1, dist);
+}
+
+// ------------------------------------------------------------------ convex-plane
+__global__ void kConvexPlane(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+                             const float* __restrict__ shapes, const float* __restrict__ verts,
+                             const float* __restrict__ props, const float* __restrict__ tr, float* cand) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  int i = list[k];
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
+  int cv = (sb.type == JBGPU_SHAPE_PLANE) ? a : b;
+  int pl = (cv == a) ? b : a;
+  Shape convex = (cv == a) ? sa : sb;
+  Shape plane = (cv == a) ? sb : sa;
+  Tr tc = loadTr(tr, cv), tp = loadTr(tr, pl);
+  V3 pn = plane.pn;
+  Tr planeInConvex = trMul(trInverse(tc), tp);
+  Tr convexInPlane = trMul(trInverse(tp), tc);
+  V3 dir = mulMV(planeInConvex.b, scl(pn, -1.0f));
+  V3 vtx = supportMargin(convex, verts, dir);
+  V3 vip = xform(convexInPlane, vtx);
+  float distance = (dot(pn, vip) - plane.pc);
+  V3 proj = sub(vip, scl(pn, distance));
+  V3 world = xform(tp, proj);
+  bool has = distance < kContactThreshold;
+  putCand(cand, i, has, mulMV(tp.b, pn), world, distance);
+}
+
+// ------------------------------------------------------------------ Voronoi simplex solver
+struct Sub {
+  V3 closest;
+  uint32_t used;  // bit0 A .. bit3 D
+  float bc[4];
+  bool degenerate;
+  __device__ void reset() {
+    degenerate = false;
+    bc[0] = bc[1] = bc[2] = bc[3] = 0.0f;
+    used = 0;
+  }
+  __device__ bool isValid() const { return (bc[0] >= 0.0f) && (bc[1] >= 0.0f) && (bc[2] >= 0.0f) && (bc[3] >= 0.0f); }
+  __device__ void setBC(float a, float b, float c, float d) { bc[0] = a; bc[1] = b; bc[2] = c; bc[3] = d; }
+};
+
+__device__ bool triClosest(V3 p, V3 a, V3 b, V3 c, Sub& r) {
+  r.used = 0;
+  V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) {
+    r.closest = a; r.used = 1; r.setBC(1, 0, 0, 0);
+    return true;
+  }
+  V3 bp = sub(p, b);
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) {
+    r.closest = b; r.used = 2; r.setBC(0, 1, 0, 0);
+    return true;
+  }
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float v = d1 / (d1 - d3);
+    r.closest = v3(v * ab.x + a.x, v * ab.y + a.y, v * ab.z + a.z);
+    r.used = 3;
+    r.setBC(1.0f - v, v, 0, 0);
+    return true;
+  }
+  V3 cp = sub(p, c);
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) {
+    r.closest = c; r.used = 4; r.setBC(0, 0, 1, 0);
+    return true;
+  }
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    r.closest = v3(w * ac.x + a.x, w * ac.y + a.y, w * ac.z + a.z);
+    r.used = 5;
+    r.setBC(1.0f - w, 0, w, 0);
+    return true;
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    V3 t = sub(c, b);
+    r.closest = v3(w * t.x + b.x, w * t.y + b.y, w * t.z + b.z);
+    r.used = 6;
+    r.setBC(0, 1.0f - w, w, 0);
+    return true;
+  }
+  float denom = 1.0f / (va + vb + vc);
+  float v = vb * denom, w = vc * denom;
+  V3 t1 = scl(ab, v), t2 = scl(ac, w);
+  r.closest = v3(a.x + t1.x + t2.x, a.y + t1.y + t2.y, a.z + t1.z + t2.z);
+  r.used = 7;
+  r.setBC(1.0f - v - w, v, w, 0);
+  return true;
+}
+
+__device__ __forceinline__ int outsidePlane(V3 p, V3 a, V3 b, V3 c, V3 d) {
+  V3 normal = crs(sub(b, a), sub(c, a));
+  float signp = dot(sub(p, a), normal);
+  float signd = dot(sub(d, a), normal);
+  if (signd * signd < ((1e-4f) * (1e-4f))) return -1;
+  return (signp * signd < 0.0f) ? 1 : 0;
+}
+
+__device__ __forceinline__ uint32_t bit(uint32_t m, int k) { return (m >> k) & 1u; }
+
+__device__ bool tetClosest(V3 p, V3 a, V3 b, V3 c, V3 d, Sub& fr) {
+  Sub tr;
+  tr.reset();
+  fr.closest = p;
+  fr.used = 15;
+  int oABC = outsidePlane(p, a, b, c, d);
+  int oACD = outsidePlane(p, a, c, d, b);
+  int oADB = outsidePlane(p, a, d, b, c);
+  int oBDC = outsidePlane(p, b, d, c, a);
+  if (oABC < 0 || oACD < 0 || oADB < 0 || oBDC < 0) {
+    fr.degenerate = true;
+    return false;
+  }
+  if (oABC == 0 && oACD == 0 && oADB == 0 && oBDC == 0) return false;
+  float best = 3.40282347e+38f;
+  if (oABC != 0) {
+    triClosest(p, a, b, c, tr);
+    V3 t = sub(tr.closest, p);
+    float sq = dot(t, t);
+    if (sq < best) {
+      best = sq;
+      fr.closest = tr.closest;
+      fr.used = bit(tr.used, 0) | (bit(tr.used, 1) << 1) | (bit(tr.used, 2) << 2);
+      fr.setBC(tr.bc[0], tr.bc[1], tr.bc[2], 0);
+    }
+  }
+  if (oACD != 0) {
+    triClosest(p, a, c, d, tr);
+    V3 t = sub(tr.closest, p);
+    float sq = dot(t, t);
+    if (sq < best) {
+      best = sq;
+      fr.closest = tr.closest;
+      fr.used = bit(tr.used, 0) | (bit(tr.used, 1) << 2) | (bit(tr.used, 2) << 3);
+      fr.setBC(tr.bc[0], 0, tr.bc[1], tr.bc[2]);
+    }
+  }
+  if (oADB != 0) {
+    triClosest(p, a, d, b, tr);
+    V3 t = sub(tr.closest, p);
+    float sq = dot(t, t);
+    if (sq < best) {
+      best = sq;
+      fr.closest = tr.closest;
+      fr.used = bit(tr.used, 0) | (bit(tr.used, 2) << 1) | (bit(tr.used, 1) << 3);
+      fr.setBC(tr.bc[0], tr.bc[2], 0, tr.bc[1]);
+    }
+  }
+  if (oBDC != 0) {
+    triClosest(p, b, d, c, tr);
+    V3 t = sub(tr.closest, p);
+    float sq = dot(t, t);
+    if (sq < best) {
+      best = sq;
+      fr.closest = tr.closest;
+      fr.used = (bit(tr.used, 0) << 1) | (bit(tr.used, 2) << 2) | (bit(tr.used, 1) << 3);
+      fr.setBC(0, tr.bc[0], tr.bc[2], tr.bc[1]);
+    }
+  }
+  return true;
+}
+
+struct Simplex {
+  int nv;
+  V3 W[5], P[5], Q[5];
+  V3 cP1, cP2, cV, lastW;
+  bool validClosest, needsUpdate;
+  Sub bc;
+
+  __device__ void reset() {
+    validClosest = false;
+    nv = 0;
+    needsUpdate = true;
+    lastW = v3(1e30f, 1e30f, 1e30f);
+    bc.reset();
+  }
+  __device__ void removeVertex(int idx) {
+    nv--;
+    W[idx] = W[nv];
+    P[idx] = P[nv];
+    Q[idx] = Q[nv];
+  }
+  __device__ void reduce(uint32_t used) {
+    if ((nv >= 4) && !bit(used, 3)) removeVertex(3);
+    if ((nv >= 3) && !bit(used, 2)) removeVertex(2);
+    if ((nv >= 2) && !bit(used, 1)) removeVertex(1);
+    if ((nv >= 1) && !bit(used, 0)) removeVertex(0);
+  }
+  __device__ void addVertex(V3 w, V3 p, V3 q) {
+    lastW = w;
+    needsUpdate = true;
+    W[nv] = w;
+    P[nv] = p;
+    Q[nv] = q;
+    nv++;
+  }
+  __device__ bool inSimplex(V3 w) const {
+    bool found = false;
+    for (int i = 0; i < nv; i++)
+      if (veq(W[i], w)) found = true;
+    if (veq(w, lastW)) return true;
+    return found;
+  }
+  __device__ bool update() {
+    if (needsUpdate) {
+      bc.reset();
+      needsUpdate = false;
+      switch (nv) {
+        case 0:
+          validClosest = false;
+          break;
+        case 1:
+          cP1 = P[0];
+          cP2 = Q[0];
+          cV = sub(cP1, cP2);
+          bc.reset();
+          bc.setBC(1, 0, 0, 0);
+          validClosest = bc.isValid();
+          break;
+        case 2: {
+          V3 from = W[0], to = W[1];
+          V3 diff = sub(v3(0, 0, 0), from);
+          V3 v = sub(to, from);
+          float t = dot(v, diff);
+          if (t > 0) {
+            float dotVV = dot(v, v);
+            if (t < dotVV) {
+              t /= dotVV;
+              diff = sub(diff, scl(v, t));
+              bc.used |= 3;
+            } else {
+              t = 1;
+              diff = sub(diff, v);
+              bc.used |= 2;
+            }
+          } else {
+            t = 0;
+            bc.used |= 1;
+          }
+          bc.setBC(1.0f - t, t, 0, 0);
+          cP1 = add(P[0], scl(sub(P[1], P[0]), t));
+          cP2 = add(Q[0], scl(sub(Q[1], Q[0]), t));
+          cV = sub(cP1, cP2);
+          reduce(bc.used);
+          validClosest = bc.isValid();
+          break;
+        }
+        case 3: {
+          triClosest(v3(0, 0, 0), W[0], W[1], W[2], bc);
+          V3 t1 = scl(P[0], bc.bc[0]), t2 = scl(P[1], bc.bc[1]), t3 = scl(P[2], bc.bc[2]);
+          cP1 = v3(t1.x + t2.x + t3.x, t1.y + t2.y + t3.y, t1.z + t2.z + t3.z);
+          t1 = scl(Q[0], bc.bc[0]); t2 = scl(Q[1], bc.bc[1]); t3 = scl(Q[2], bc.bc[2]);
+          cP2 = v3(t1.x + t2.x + t3.x, t1.y + t2.y + t3.y, t1.z + t2.z + t3.z);
+          cV = sub(cP1, cP2);
+          reduce(bc.used);
+          validClosest = bc.isValid();
+          break;
+        }
+        case 4: {
+          bool sep = tetClosest(v3(0, 0, 0), W[0], W[1], W[2], W[3], bc);
+          if (sep) {
+            V3 t1 = scl(P[0], bc.bc[0]), t2 = scl(P[1], bc.bc[1]), t3 = scl(P[2], bc.bc[2]), t4 = scl(P[3], bc.bc[3]);
+            cP1 = v3(t1.x + t2.x + t3.x + t4.x, t1.y + t2.y + t3.y + t4.y, t1.z + t2.z + t3.z + t4.z);
+            t1 = scl(Q[0], bc.bc[0]); t2 = scl(Q[1], bc.bc[1]); t3 = scl(Q[2], bc.bc[2]); t4 = scl(Q[3], bc.bc[3]);
+            cP2 = v3(t1.x + t2.x + t3.x + t4.x, t1.y + t2.y + t3.y + t4.y, t1.z + t2.z + t3.z + t4.z);
+            cV = sub(cP1, cP2);
+            reduce(bc.used);
+          } else {
+            if (bc.degenerate) {
+              validClosest = false;
+            } else {
+              validClosest = true;
+              cV = v3(0, 0, 0);
+            }
+            break;
+          }
+          validClosest = bc.isValid();
+          break;
+        }
+        default:
+          validClosest = false;
+      }
+    }
+    return validClosest;
+  }
+};
+
+// ------------------------------------------------------------------ GJK pair detector
+// epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
+__global__ __launch_bounds__(128) void kGjk(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+                                            const float* __restrict__ shapes, const float* __restrict__ verts,
+                                            const float* __restrict__ props, const float* __restrict__ tr,
+                                            float* cand, float* epaState, int* epaQueue,
+                                            unsigned long long* epaCount) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  int i = list[k];
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
+  Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
+  float md = sa.margin + sb.margin + kContactThreshold;  // ConvexConvexAlgorithm.java:122-123
+  md *= md;
+  const float REL_ERROR2 = 1.0e-6f;
+  float distance = 0.0f;
+  V3 normalInB = v3(0, 0, 0), pointOnA = v3(0, 0, 0), pointOnB = v3(0, 0, 0);
+  Tr la = ta, lb = tb;
+  V3 off = scl(add(la.o, lb.o), 0.5f);
+  la.o = sub(la.o, off);
+  lb.o = sub(lb.o, off);
+  float marginA = sa.margin, marginB = sb.margin;
+  int curIter = 0;
+  V3 sep = v3(0, 1, 0);
+  bool isValid = false, checkSimplex = false, checkPenetration = true;
+  int degen = 0;
+  float sqd = kInf;
+  float margin = marginA + marginB;
+  Simplex vs;
+  vs.reset();
+  for (;;) {
+    V3 sepA = mulMtV(ta.b, scl(sep, -1.0f));
+    V3 sepB = mulMtV(tb.b, sep);
+    V3 pInA = supportNoMargin(sa, verts, sepA);
+    V3 qInB = supportNoMargin(sb, verts, sepB);
+    V3 pW = xform(la, pInA);
+    V3 qW = xform(lb, qInB);
+    V3 w = sub(pW, qW);
+    float delta = dot(sep, w);
+    if ((delta > 0.0f) && (delta * delta > sqd * md)) {
+      checkPenetration = false;
+      break;
+    }
+    if (vs.inSimplex(w)) {
+      degen = 1;
+      checkSimplex = true;
+      break;
+    }
+    float f0 = sqd - delta;
+    float f1 = sqd * REL_ERROR2;
+    if (f0 <= f1) {
+      if (f0 <= 0.0f) degen = 2;
+      checkSimplex = true;
+      break;
+    }
+    vs.addVertex(w, pW, qW);
+    bool ok = vs.update();
+    sep = vs.cV;
+    if (!ok) {
+      degen = 3;
+      checkSimplex = true;
+      break;
+    }
+    if (len2(sep) < REL_ERROR2) {
+      degen = 6;
+      checkSimplex = true;
+      break;
+    }
+    float prev = sqd;
+    sqd = len2(sep);
+    if (prev - sqd <= kFltEps * prev) {
+      sep = vs.cV;
+      checkSimplex = true;
+      break;
+    }
+    if (curIter++ > 1000) break;
+    if (vs.nv == 4) {
+      sep = vs.cV;
+      break;
+    }
+  }
+  if (checkSimplex) {
+    vs.update();
+    pointOnA = vs.cP1;
+    pointOnB = vs.cP2;
+    normalInB = sub(pointOnA, pointOnB);
+    float lenSqr = len2(sep);
+    if (lenSqr < 0.0001f) degen = 5;
+    if (lenSqr > kFltEps * kFltEps) {
+      float rlen = 1.0f / jsqrt(lenSqr);
+      normalInB = scl(normalInB, rlen);
+      float s = jsqrt(sqd);
+      pointOnA = sub(pointOnA, scl(sep, marginA / s));
+      pointOnB = add(pointOnB, scl(sep, marginB / s));
+      distance = ((1.0f / rlen) - margin);
+      isValid = true;
+    }
+  }
+  bool catchDegen = (degen != 0 && ((distance + margin) < 0.01f));
+  float* es = epaState + 12 * i;
+  if (checkPenetration && (!isValid || catchDegen)) {
+    es[0] = 1.0f;
+    es[1] = isValid ? 1.0f : 0.0f;
+    es[2] = distance;
+    st3(es + 3, pointOnA);
+    st3(es + 6, pointOnB);
+    st3(es + 9, normalInB);
+    unsigned long long q = atomicAdd(epaCount, 1ull);
+    epaQueue[q] = i;
+    putCand(cand, i, false, v3(0, 0, 0), v3(0, 0, 0), 0.0f);
+    return;
+  }
+  es[0] = 0.0f;
+  putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
+}
+
+// ------------------------------------------------------------------ GJK-EPA (GjkEpaSolver)
+constexpr int GJK_maxiterations = 128;
+constexpr int GJK_hashmask = 63;
+constexpr float GJK_insimplex_eps = 0.0001f;
+constexpr float GJK_sqinsimplex_eps = GJK_insimplex_eps * GJK_insimplex_eps;
+constexpr int EPA_maxiterations = 256;
+constexpr float EPA_inface_eps = 0.01f;
+constexpr float EPA_accuracy = 0.001f;
+
+struct Mkv {
+  V3 w, r;
+};
+
+// workspace slot layout (floats): faces [kEpaFaces*16] | vertices [kEpaVerts*6] | he [kEpaHe*4]
+struct EpaWork {
+  float* face;
+  float* vtx;
+  float* he;
+  int nface, nvtx, nhe;
+  bool overflow;
+  // face fields: 0-2 v, 3-5 f, 6-8 e, 9-11 n, 12 d, 13 mark, 14 prev, 15 next
+  __device__ int& fi(int f, int k) { return *reinterpret_cast<int*>(face + kEpaFaceWords * f + k); }
+  __device__ float& ff(int f, int k) { return face[kEpaFaceWords * f + k]; }
+  __device__ V3 fn(int f) { return ld3(face + kEpaFaceWords * f + 9); }
+  __device__ V3 vw(int v) { return ld3(vtx + 6 * v); }
+  __device__ V3 vr(int v) { return ld3(vtx + 6 * v + 3); }
+};
+
+struct EpaSolver {
+  Shape sh[2];
+  M3 wrot[2];
+  V3 pos[2];
+  const float* verts;
+  float margin;
+  int table[64];
+  Mkv simplex[5];
+  V3 ray;
+  int order, iterations;
+  bool failed;
+  EpaWork W;
+  int root, nfaces, epaIter;
+  V3 nearest[2];
+
+  __device__ static int Hash(V3 v) {
+    int h = jint(v.x * 15461) ^ jint(v.y * 83003) ^ jint(v.z * 15473);
+    return (int)((uint32_t)h * 169639u) & GJK_hashmask;
+  }
+  __device__ V3 LocalSupport(V3 d, int i) {
+    V3 t = mulMtV(wrot[i], d);
+    V3 o = supportMargin(sh[i], verts, t);
+    o = mulMV(wrot[i], o);
+    return add(o, pos[i]);
+  }
+  __device__ void Support(V3 d, Mkv& v) {
+    v.r = d;
+    V3 t1 = LocalSupport(d, 0);
+    V3 t2 = LocalSupport(scl(d, -1.0f), 1);
+    v.w = sub(t1, t2);
+    v.w.x += margin * d.x;
+    v.w.y += margin * d.y;
+    v.w.z += margin * d.z;
+  }
+  __device__ bool FetchSupport() {
+    int h = Hash(ray);
+    int e = table[h];
+    int guardHops = 0;
+    while (e >= 0) {
+      if (veq(ld3(W.he + 4 * e), ray)) {
+        --order;
+        return false;
+      }
+      e = __builtin_bit_cast(int, W.he[4 * e + 3]);
+      if (++guardHops > kEpaHe) {  // chains are acyclic; bound the walk regardless
+        W.overflow = true;
+        return false;
+      }
+    }
+    if (W.nhe >= kEpaHe) {
+      W.overflow = true;
+      return false;
+    }
+    int ne = W.nhe++;
+    st3(W.he + 4 * ne, ray);
+    W.he[4 * ne + 3] = __builtin_bit_cast(float, table[h]);
+    table[h] = ne;
+    Support(ray, simplex[++order]);
+    return (dot(ray, simplex[order].w) > 0);
+  }
+  __device__ bool SolveSimplex2(V3 ao, V3 ab) {
+    if (dot(ab, ao) >= 0) {
+      V3 cabo = crs(ab, ao);
+      if (len2(cabo) > GJK_sqinsimplex_eps)
+        ray = crs(cabo, ab);
+      else
+        return true;
+    } else {
+      order = 0;
+      simplex[0] = simplex[1];
+      ray = ao;
+    }
+    return false;
+  }
+  __device__ bool SolveSimplex3a(V3 ao, V3 ab, V3 ac, V3 cabc) {
+    V3 t = crs(cabc, ab);
+    V3 t2 = crs(cabc, ac);
+    if (dot(t, ao) < -GJK_insimplex_eps) {
+      order = 1;
+      simplex[0] = simplex[1];
+      simplex[1] = simplex[2];
+      return SolveSimplex2(ao, ab);
+    } else if (dot(t2, ao) > +GJK_insimplex_eps) {
+      order = 1;
+      simplex[1] = simplex[2];
+      return SolveSimplex2(ao, ac);
+    } else {
+      float d = dot(cabc, ao);
+      if (jabs(d) > GJK_insimplex_eps) {
+        if (d > 0) {
+          ray = cabc;
+        } else {
+          ray = scl(cabc, -1.0f);
+          Mkv sw = simplex[0];
+          simplex[0] = simplex[1];
+          simplex[1] = sw;
+        }
+        return false;
+      }
+      return true;
+    }
+  }
+  __device__ bool SolveSimplex4(V3 ao, V3 ab, V3 ac, V3 ad) {
+    V3 t = crs(ab, ac), t2 = crs(ac, ad), t3 = crs(ad, ab);
+    if (dot(t, ao) > GJK_insimplex_eps) {
+      order = 2;
+      simplex[0] = simplex[1];
+      simplex[1] = simplex[2];
+      simplex[2] = simplex[3];
+      return SolveSimplex3a(ao, ab, ac, t);
+    } else if (dot(t2, ao) > GJK_insimplex_eps) {
+      order = 2;
+      simplex[2] = simplex[3];
+      return SolveSimplex3a(ao, ac, ad, t2);
+    } else if (dot(t3, ao) > GJK_insimplex_eps) {
+      order = 2;
+      simplex[1] = simplex[0];
+      simplex[0] = simplex[2];
+      simplex[2] = simplex[3];
+      return SolveSimplex3a(ao, ad, ab, t3);
+    }
+    return true;
+  }
+  __device__ bool SearchOrigin() {
+    iterations = 0;
+    order = -1;
+    failed = false;
+    ray = nor(v3(1, 0, 0));
+    for (int i = 0; i < 64; i++) table[i] = -1;
+    W.nhe = 0;
+    FetchSupport();
+    ray = scl(simplex[0].w, -1.0f);
+    for (; iterations < GJK_maxiterations; ++iterations) {
+      if (W.overflow) return false;
+      float rl = len(ray);
+      ray = scl(ray, 1.0f / (rl > 0.0f ? rl : 1.0f));
+      if (FetchSupport()) {
+        bool found = false;
+        switch (order) {
+          case 1:
+            found = SolveSimplex2(scl(simplex[1].w, -1.0f), sub(simplex[0].w, simplex[1].w));
+            break;
+          case 2: {
+            V3 ab = sub(simplex[1].w, simplex[2].w), ac = sub(simplex[0].w, simplex[2].w);
+            found = SolveSimplex3a(scl(simplex[2].w, -1.0f), ab, ac, crs(ab, ac));
+            break;
+          }
+          case 3:
+            found = SolveSimplex4(scl(simplex[3].w, -1.0f), sub(simplex[2].w, simplex[3].w),
+                                  sub(simplex[1].w, simplex[3].w), sub(simplex[0].w, simplex[3].w));
+            break;
+        }
+        if (found) return true;
+      } else {
+        return false;
+      }
+    }
+    failed = true;
+    return false;
+  }
+  __device__ bool EncloseOrigin() {
+    switch (order) {
+      case 0:
+        break;
+      case 1: {
+        V3 ab = sub(simplex[1].w, simplex[0].w);
+        V3 b0 = crs(ab, v3(1, 0, 0)), b1 = crs(ab, v3(0, 1, 0)), b2 = crs(ab, v3(0, 0, 1));
+        float m0 = len2(b0), m1 = len2(b1), m2 = len2(b2);
+        V3 ax = nor(ab);
+        float ang = k2Pi / 3.0f;
+        float dd = len(ax);
+        float s = (float)sin((double)(ang * 0.5f)) / dd;
+        Q4 q = Q4{ax.x * s, ax.y * s, ax.z * s, (float)cos((double)(ang * 0.5f))};
+        M3 r = mFromQ(q);
+        int sel = m0 > m1 ? (m0 > m2 ? 0 : 2) : (m1 > m2 ? 1 : 2);
+        V3 w = sel == 0 ? b0 : (sel == 1 ? b1 : b2);
+        Support(nor(w), simplex[4]);
+        w = mulMV(r, w);
+        Support(nor(w), simplex[2]);
+        w = mulMV(r, w);
+        Support(nor(w), simplex[3]);
+        order = 4;
+        return true;
+      }
+      case 2: {
+        V3 n = nor(crs(sub(simplex[1].w, simplex[0].w), sub(simplex[2].w, simplex[0].w)));
+        Support(n, simplex[3]);
+        Support(scl(n, -1.0f), simplex[4]);
+        order = 4;
+        return true;
+      }
+      case 3:
+      case 4:
+        return true;
+    }
+    return false;
+  }
+  __device__ int addVertex(const Mkv& m) {
+    if (W.nvtx >= kEpaVerts) {
+      W.overflow = true;
+      return -1;
+    }
+    int v = W.nvtx++;
+    st3(W.vtx + 6 * v, m.w);
+    st3(W.vtx + 6 * v + 3, m.r);
+    return v;
+  }
+  __device__ bool Set(int f, int a, int b, int c) {
+    V3 aw = W.vw(a), bw = W.vw(b), cw = W.vw(c);
+    V3 nrm = crs(sub(bw, aw), sub(cw, aw));
+    float l = len(nrm);
+    V3 t1 = crs(aw, bw), t2 = crs(bw, cw), t3 = crs(cw, aw);
+    bool valid = (dot(t1, nrm) >= -EPA_inface_eps) && (dot(t2, nrm) >= -EPA_inface_eps) &&
+                 (dot(t3, nrm) >= -EPA_inface_eps);
+    W.fi(f, 0) = a;
+    W.fi(f, 1) = b;
+    W.fi(f, 2) = c;
+    W.fi(f, 13) = 0;
+    V3 n = scl(nrm, 1.0f / (l > 0.0f ? l : kInf));
+    st3(W.face + kEpaFaceWords * f + 9, n);
+    W.ff(f, 12) = jmax(0.0f, -dot(n, aw));
+    return valid;
+  }
+  __device__ int NewFace(int a, int b, int c) {
+    if (W.nface >= kEpaFaces) {
+      W.overflow = true;
+      return -1;
+    }
+    int pf = W.nface++;
+    W.fi(pf, 3) = W.fi(pf, 4) = W.fi(pf, 5) = -1;
+    if (Set(pf, a, b, c)) {
+      if (root >= 0) W.fi(root, 14) = pf;
+      W.fi(pf, 14) = -1;
+      W.fi(pf, 15) = root;
+      root = pf;
+      ++nfaces;
+    } else {
+      W.fi(pf, 14) = W.fi(pf, 15) = -1;
+    }
+    return pf;
+  }
+  __device__ void Detach(int f) {
+    int pv = W.fi(f, 14), nx = W.fi(f, 15);
+    if (pv >= 0 || nx >= 0) {
+      --nfaces;
+      if (f == root) {
+        root = nx;
+        W.fi(root, 14) = -1;
+      } else {
+        if (nx < 0) {
+          W.fi(pv, 15) = -1;
+        } else {
+          W.fi(pv, 15) = nx;
+          W.fi(nx, 14) = pv;
+        }
+      }
+      W.fi(f, 14) = W.fi(f, 15) = -1;
+    }
+  }
+  __device__ void Link(int f0, int e0, int f1, int e1) {
+    W.fi(f0, 3 + e0) = f1;
+    W.fi(f1, 6 + e1) = e0;
+    W.fi(f1, 3 + e1) = f0;
+    W.fi(f0, 6 + e0) = e1;
+  }
+  // BuildHorizon (GjkEpaSolver.java:686-706) as an explicit-stack traversal in the same visiting order
+  __device__ int BuildHorizon(int markid, int w, int f0, int e0, int* cf, int* ff) {
+    const int mod3[5] = {0, 1, 2, 0, 1};
+    int stkF[64], stkE[64];
+    int sp = 0;
+    int ne = 0;
+    stkF[sp] = f0;
+    stkE[sp] = e0;
+    sp++;
+    while (sp > 0) {
+      if (W.overflow) return ne;
+      sp--;
+      int f = stkF[sp], e = stkE[sp];
+      if (f < 0 || f >= W.nface || e < 0 || e > 2) {  // unlinked edge: treat as a pool failure
+        W.overflow = true;
+        return ne;
+      }
+      if (W.fi(f, 13) == markid) continue;
+      int e1 = mod3[e + 1];
+      if ((dot(W.fn(f), W.vw(w)) + W.ff(f, 12)) > 0) {
+        int nf = NewFace(W.fi(f, e1), W.fi(f, e), w);
+        if (nf < 0) return ne;
+        Link(nf, 0, f, e);
+        if (*cf >= 0)
+          Link(*cf, 1, nf, 2);
+        else
+          *ff = nf;
+        *cf = nf;
+        ne++;
+      } else {
+        int e2 = mod3[e + 2];
+        Detach(f);
+        W.fi(f, 13) = markid;
+        if (sp + 2 > 64) {
+          W.overflow = true;
+          return ne;
+        }
+        // recursion visits f.f[e1] first, then f.f[e2]: push in reverse
+        stkF[sp] = W.fi(f, 3 + e2);
+        stkE[sp] = W.fi(f, 6 + e2);
+        sp++;
+        stkF[sp] = W.fi(f, 3 + e1);
+        stkE[sp] = W.fi(f, 6 + e1);
+        sp++;
+      }
+    }
+    return ne;
+  }
+  __device__ int FindBest() {
+    int bf = -1;
+    if (root >= 0) {
+      int cf = root;
+      float bd = kInf;
+      int hops = 0;
+      do {
+        if (W.ff(cf, 12) < bd) {
+          bd = W.ff(cf, 12);
+          bf = cf;
+        }
+        if (++hops > W.nface) {  // the face list is acyclic; bound the walk regardless
+          W.overflow = true;
+          return -1;
+        }
+      } while (-1 != (cf = W.fi(cf, 15)));
+    }
+    return bf;
+  }
+  __device__ float EvaluatePD() {
+    const int tet_f[4][3] = {{2, 1, 0}, {3, 0, 1}, {3, 1, 2}, {3, 2, 0}};
+    const int tet_e[6][4] = {{0, 0, 2, 1}, {0, 1, 1, 1}, {0, 2, 3, 1}, {1, 0, 3, 2}, {2, 0, 1, 2}, {3, 0, 2, 2}};
+    const int hex_f[6][3] = {{2, 0, 4}, {4, 1, 2}, {1, 4, 0}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+    const int hex_e[9][4] = {{0, 0, 4, 0}, {0, 1, 2, 1}, {0, 2, 1, 2}, {1, 1, 5, 2}, {1, 0, 2, 0},
+                             {2, 2, 3, 2}, {3, 1, 5, 0}, {3, 0, 4, 2}, {5, 1, 4, 1}};
+    int bestface = -1;
+    int markid = 1;
+    float depth = -kInf;
+    root = -1;
+    nfaces = 0;
+    epaIter = 0;
+    W.nface = 0;
+    W.nvtx = 0;
+    if (EncloseOrigin()) {
+      int nf = 0, ne = 0;
+      if (order == 3) { nf = 4; ne = 6; }
+      else if (order == 4) { nf = 6; ne = 9; }
+      for (int i = 0; i <= order; ++i) addVertex(simplex[i]);
+      int base[6];
+      for (int i = 0; i < nf; ++i) {
+        const int* fx = (order == 3) ? tet_f[i] : hex_f[i];
+        base[i] = NewFace(fx[0], fx[1], fx[2]);
+      }
+      for (int i = 0; i < ne; ++i) {
+        const int* ex = (order == 3) ? tet_e[i] : hex_e[i];
+        Link(base[ex[0]], ex[1], base[ex[2]], ex[3]);
+      }
+    }
+    if (0 == nfaces) return depth;
+    for (; epaIter < EPA_maxiterations; ++epaIter) {
+      if (W.overflow) return depth;
+      int bf = FindBest();
+      if (bf >= 0) {
+        Mkv wv;
+        Support(scl(W.fn(bf), -1.0f), wv);
+        int w = addVertex(wv);
+        if (w < 0) return depth;
+        float d = dot(W.fn(bf), W.vw(w)) + W.ff(bf, 12);
+        bestface = bf;
+        if (d < -EPA_accuracy) {
+          int cf = -1, ff = -1;
+          int nf = 0;
+          Detach(bf);
+          W.fi(bf, 13) = ++markid;
+          for (int i = 0; i < 3; ++i) nf += BuildHorizon(markid, w, W.fi(bf, 3 + i), W.fi(bf, 6 + i), &cf, &ff);
+          if (W.overflow) return depth;
+          if (nf <= 2) break;
+          Link(cf, 1, ff, 2);
+        } else {
+          break;
+        }
+      } else {
+        break;
+      }
+    }
+    if (bestface >= 0) {
+      V3 n = W.fn(bestface);
+      float fd = W.ff(bestface, 12);
+      V3 o = scl(n, -fd);
+      int v0 = W.fi(bestface, 0), v1 = W.fi(bestface, 1), v2 = W.fi(bestface, 2);
+      float a0 = len(crs(sub(W.vw(v0), o), sub(W.vw(v1), o)));
+      float a1 = len(crs(sub(W.vw(v1), o), sub(W.vw(v2), o)));
+      float a2 = len(crs(sub(W.vw(v2), o), sub(W.vw(v0), o)));
+      float sm = a0 + a1 + a2;
+      V3 bcw = scl(v3(a1, a2, a0), 1.0f / (sm > 0.0f ? sm : 1.0f));
+      depth = jmax(0.0f, fd);
+      int vv[3] = {v0, v1, v2};
+      for (int i = 0; i < 2; ++i) {
+        float s = i != 0 ? -1.0f : 1.0f;
+        V3 f0 = LocalSupport(scl(W.vr(vv[0]), s), i);
+        V3 f1 = LocalSupport(scl(W.vr(vv[1]), s), i);
+        V3 f2 = LocalSupport(scl(W.vr(vv[2]), s), i);
+        V3 t1 = scl(f0, bcw.x), t2 = scl(f1, bcw.y), t3 = scl(f2, bcw.z);
+        nearest[i] = v3(t1.x + t2.x + t3.x, t1.y + t2.y + t3.y, t1.z + t2.z + t3.z);
+      }
+    }
+    return depth;
+  }
+};
+
+// one thread per queued pair; the GJK partial result is merged as in GjkPairDetector.java:263-303
+__global__ __launch_bounds__(64) void kEpa(int cnt, int base, const int* __restrict__ queue,
+                                           const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
+                                           const float* __restrict__ verts, const float* __restrict__ props,
+                                           const float* __restrict__ tr, const float* __restrict__ epaState,
+                                           float* cand, float* work, unsigned long long* fails) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  int i = queue[base + k];
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  EpaSolver S;
+  S.sh[0] = bodyShape(shapes, props, a);
+  S.sh[1] = bodyShape(shapes, props, b);
+  Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
+  V3 off = scl(add(ta.o, tb.o), 0.5f);
+  S.wrot[0] = ta.b;
+  S.pos[0] = sub(ta.o, off);
+  S.wrot[1] = tb.b;
+  S.pos[1] = sub(tb.o, off);
+  S.verts = verts;
+  S.margin = 0.0f + EPA_accuracy;
+  float* slot = work + (size_t)k * kEpaSlotWords;
+  S.W.face = slot;
+  S.W.vtx = slot + kEpaFaces * kEpaFaceWords;
+  S.W.he = S.W.vtx + kEpaVerts * 6;
+  S.W.nface = S.W.nvtx = S.W.nhe = 0;
+  S.W.overflow = false;
+  const float* es = epaState + 12 * i;
+  bool isValid = es[1] != 0.0f;
+  float distance = es[2];
+  V3 pointOnA = ld3(es + 3), pointOnB = ld3(es + 6), normalInB = ld3(es + 9);
+  bool ok = false;
+  V3 wA, wB;
+  if (S.SearchOrigin()) {
+    float pd = S.EvaluatePD();
+    if (pd > 0 && !S.W.overflow) {
+      ok = true;
+      wA = S.nearest[0];
+      wB = S.nearest[1];
+    }
+  }
+  if (S.W.overflow) atomicAdd(fails + 1, 1ull);
+  if (ok) {
+    V3 tn = sub(wB, wA);
+    float lenSqr = len2(tn);
+    if (lenSqr > (kFltEps * kFltEps)) {
+      tn = scl(tn, 1.0f / jsqrt(lenSqr));
+      float distance2 = -len(sub(wA, wB));
+      if (!isValid || (distance2 < distance)) {
+        distance = distance2;
+        pointOnA = wA;
+        pointOnB = wB;
+        normalInB = tn;
+        isValid = true;
+      }
+    }
+  } else {
+    atomicAdd(fails, 1ull);
+  }
+  putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
+}
+
+// ------------------------------------------------------------------ manifold update
+struct Pt {
+  float* p;
+  __device__ V3 g3(int o) const { return ld3(p + o); }
+  __device__ void s3(int o, V3 v) { st3(p + o, v); }
+};
+
+__device__ __forceinline__ void copyPoint(float* dst, const float* src) {
+  for (int k = 0; k < JBGPU_POINT_WORDS; k++) dst[k] = src[k];
+}
+
+__device__ void removePoint(float* m, int idx) {  // PersistentManifold.java:259-278
+  int n = ldi(m);
+  int last = n - 1;
+  float* P = m + JBGPU_MANIFOLD_HDR;
+  if (idx != last) {
+    copyPoint(P + JBGPU_POINT_WORDS * idx, P + JBGPU_POINT_WORDS * last);
+    float* L = P + JBGPU_POINT_WORDS * last;
+    L[18] = 0.0f;
+    sti(L + 27, 0);
+    L[19] = 0.0f;
+    L[20] = 0.0f;
+    sti(L + 28, 0);
+  }
+  sti(m, n - 1);
+}
+
+__global__ void kManifold(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ ptype,
+                          const int* __restrict__ act, const float* __restrict__ props, const float* __restrict__ tr,
+                          const float* __restrict__ cand, float* man, unsigned long long* contactCount) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  if (ptype[i] == JBGPU_PAIR_NONE) return;
+  if (!(isActiveState(act[a]) || isActiveState(act[b]))) return;  // needsCollision
+  float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * i;
+  float* P = m + JBGPU_MANIFOLD_HDR;
+  int body0 = ldi(m + 1), body1 = ldi(m + 2);
+  const float* c = cand + 8 * i;
+  Tr t0 = loadTr(tr, body0), t1 = loadTr(tr, body1);
+  if (c[0] != 0.0f && !(c[7] > kContactThreshold)) {
+    // ManifoldResult.addContactPoint (ManifoldResult.java:92-157); result body0 = a
+    V3 nB = v3(c[1], c[2], c[3]), pB = v3(c[4], c[5], c[6]);
+    float depth = c[7];
+    V3 pointA = add(scl(nB, depth), pB);
+    // localA on the manifold's body0, localB on body1 (swapped when body0 != a)
+    V3 localA = invXform(t0, pointA);
+    V3 localB = invXform(t1, pB);
+    float np_[JBGPU_POINT_WORDS];
+    for (int k = 0; k < JBGPU_POINT_WORDS; k++) np_[k] = 0.0f;
+    st3(np_ + 0, localA);
+    st3(np_ + 3, localB);
+    st3(np_ + 6, pointA);
+    st3(np_ + 9, pB);
+    st3(np_ + 12, nB);
+    np_[15] = depth;
+    // getCacheEntry (PersistentManifold.java:214-233)
+    int n = ldi(m);
+    float shortest = kContactThreshold * kContactThreshold;
+    int idx = -1;
+    for (int k = 0; k < n; k++) {
+      V3 d = sub(ld3(P + JBGPU_POINT_WORDS * k), localA);
+      float dd = dot(d, d);
+      if (dd < shortest) {
+        shortest = dd;
+        idx = k;
+      }
+    }
+    float fr = props[8 * a + 4] * props[8 * b + 4];
+    if (fr < -10.0f) fr = -10.0f;
+    if (fr > 10.0f) fr = 10.0f;
+    np_[16] = fr;
+    np_[17] = props[8 * a + 5] * props[8 * b + 5];
+    if (idx >= 0) {  // replaceContactPoint (:280-305)
+      float* O = P + JBGPU_POINT_WORDS * idx;
+      float ai = O[18], l1 = O[19], l2 = O[20];
+      int life = ldi(O + 28);
+      copyPoint(O, np_);
+      O[18] = ai;
+      O[19] = l1;
+      O[20] = l2;
+      sti(O + 28, life);
+    } else {  // addManifoldPoint (:235-257)
+      int ins = n;
+      if (ins == 4) {  // sortCachedPoints (:83-156)
+        int maxPen = -1;
+        float maxPenV = depth;
+        for (int k = 0; k < 4; k++) {
+          float dk = P[JBGPU_POINT_WORDS * k + 15];
+          if (dk < maxPenV) {
+            maxPen = k;
+            maxPenV = dk;
+          }
+        }
+        V3 l0 = ld3(P), l1v = ld3(P + JBGPU_POINT_WORDS), l2v = ld3(P + 2 * JBGPU_POINT_WORDS),
+           l3v = ld3(P + 3 * JBGPU_POINT_WORDS);
+        float r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        if (maxPen != 0) r0 = len2(crs(sub(localA, l1v), sub(l3v, l2v)));
+        if (maxPen != 1) r1 = len2(crs(sub(localA, l0), sub(l3v, l2v)));
+        if (maxPen != 2) r2 = len2(crs(sub(localA, l0), sub(l3v, l1v)));
+        if (maxPen != 3) r3 = len2(crs(sub(localA, l0), sub(l2v, l1v)));
+        float v[4] = {jabs(r0), jabs(r1), jabs(r2), jabs(r3)};
+        int mi = -1;
+        float mv = -1e30f;
+        for (int k = 0; k < 4; k++)
+          if (v[k] > mv) {
+            mi = k;
+            mv = v[k];
+          }
+        ins = mi;
+      } else {
+        sti(m, n + 1);
+      }
+      copyPoint(P + JBGPU_POINT_WORDS * ins, np_);
+    }
+  }
+  // refreshContactPoints (PersistentManifold.java:312-372) if the manifold has points
+  int n = ldi(m);
+  if (n == 0) return;
+  for (int k = n - 1; k >= 0; k--) {
+    float* Q = P + JBGPU_POINT_WORDS * k;
+    V3 wa = xform(t0, ld3(Q + 0));
+    V3 wb = xform(t1, ld3(Q + 3));
+    st3(Q + 6, wa);
+    st3(Q + 9, wb);
+    Q[15] = dot(sub(wa, wb), ld3(Q + 12));
+    sti(Q + 28, ldi(Q + 28) + 1);
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    float* Q = P + JBGPU_POINT_WORDS * k;
+    float d = Q[15];
+    if (!(d <= kContactThreshold)) {
+      removePoint(m, k);
+    } else {
+      V3 proj = sub(ld3(Q + 6), scl(ld3(Q + 12), d));
+      V3 pd = sub(ld3(Q + 9), proj);
+      float d2 = dot(pd, pd);
+      if (d2 > kContactThreshold * kContactThreshold) removePoint(m, k);
+    }
+  }
+  if (contactCount) atomicAdd(contactCount, (unsigned long long)ldi(m));
+}
+
+}  // namespace
+
+void npDispatchAll(Ctx& c) {
+  hipStream_t s = c.stream;
+  const int64_t n = c.npairs;
+  c.st.contacts = 0;
+  if (n == 0) return;
+  c.typeKey.ensure(n);
+  c.typeKeyAlt.ensure(n);
+  c.perm.ensure(n);
+  c.permAlt.ensure(n);
+  c.candC.ensure(8 * n);
+  c.epaState.ensure(12 * n);
+  c.epaQueue.ensure(n);
+  c.counter.ensure(8);
+  const uint64_t* keys = c.keys[c.cur].p;
+  float* man = c.man[c.cur].p;
+  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p, c.typeKey.p, c.perm.p,
+                     c.counter.p);
+  size_t tb = 0;
+  hipcub::DoubleBuffer<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, 2, s));
+  c.cubTmp.ensure(tb);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, 2, s));
+  unsigned long long cnt[4];
+  JB_CHECK(hipMemcpyAsync(cnt, c.counter.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  int64_t off[5] = {0, 0, 0, 0, 0};
+  for (int t = 0; t < 4; t++) off[t + 1] = off[t] + (int64_t)cnt[t];
+  for (int t = 0; t < 5; t++) c.binOff[t] = off[t];
+  const int* list = vv.Current();
+  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+  if (cnt[JBGPU_PAIR_SPHERE_SPHERE])
+    hipLaunchKernelGGL(kSphereSphere, dim3(gridFor(cnt[0])), dim3(256), 0, s, (int)cnt[0], list + off[0], keys,
+                       c.shapes.p, c.props.p, c.tr.p, c.candC.p);
+  if (cnt[JBGPU_PAIR_CONVEX_PLANE])
+    hipLaunchKernelGGL(kConvexPlane, dim3(gridFor(cnt[1])), dim3(256), 0, s, (int)cnt[1], list + off[1], keys,
+                       c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p);
+  tBegin(c, T_GJK);
+  if (cnt[JBGPU_PAIR_CONVEX_CONVEX])
+    hipLaunchKernelGGL(kGjk, dim3(gridFor(cnt[2], 128)), dim3(128), 0, s, (int)cnt[2], list + off[2], keys,
+                       c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p,
+                       c.counter.p);
+  tEnd(c, T_GJK);
+  unsigned long long nepa = 0;
+  JB_CHECK(hipMemcpyAsync(&nepa, c.counter.p, sizeof(nepa), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  c.st.epaCalls = (int64_t)nepa;
+  if (nepa) {
+    c.epaWork.ensure((size_t)kEpaSlotWords * std::min<int64_t>((int64_t)nepa, kEpaBatch));
+    tBegin(c, T_EPA);
+    for (int64_t b = 0; b < (int64_t)nepa; b += kEpaBatch) {
+      int m = (int)std::min<int64_t>(kEpaBatch, (int64_t)nepa - b);
+      hipLaunchKernelGGL(kEpa, dim3(gridFor(m, 64)), dim3(64), 0, s, m, (int)b, c.epaQueue.p, keys, c.shapes.p,
+                         c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p, c.epaWork.p, c.counter.p + 2);
+    }
+    tEnd(c, T_EPA);
+  }
+  tBegin(c, T_MANIFOLD);
+  hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
+                     c.candC.p, man, c.counter.p + 4);
+  tEnd(c, T_MANIFOLD);
+  unsigned long long tail[3];
+  JB_CHECK(hipMemcpyAsync(tail, c.counter.p + 2, sizeof(tail), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  c.st.epaFails = (int64_t)tail[0];
+  c.st.contacts = (int64_t)tail[2];
+  if (tail[1]) c.err = "EPA workspace overflow";
+}
+
+}  // namespace jbg

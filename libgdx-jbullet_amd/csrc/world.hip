@@ -1,0 +1,850 @@
+// world.hip — libjbgpu host side: the C ABI of include/jbgpu.h, the
+// device-resident step (DiscreteDynamicsWorld.stepSimulation with one substep,
+// dynamics/DiscreteDynamicsWorld.java:290-400) and the per-body kernels:
+// integrateVelocities/applyDamping (dynamics/RigidBody.java:246-342),
+// integrateTransform (linearmath/TransformUtil.java:64-103) and the
+// activation update (DiscreteDynamicsWorld.java:475-512, RigidBody.java:543-576).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "../../include/jbgpu.h"
+#include "internal.h"
+#include "shapes.h"
+
+using namespace jbg;
+
+struct jbgpu_ctx {
+  Ctx c;
+};
+
+namespace {
+
+std::string g_createError;
+
+constexpr float ANGULAR_MOTION_THRESHOLD = 0.5f * kHalfPi;
+constexpr float MAX_ANGVEL = kHalfPi;
+
+__global__ void kPredict(int n, float dt, const float* __restrict__ tr, const float* __restrict__ props,
+                         const float* __restrict__ grav, const int* __restrict__ act, float* vel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* pr = props + 8 * i;
+  if ((ldi(pr + 7) & JBGPU_BODY_STATIC) != 0) return;
+  int a = act[i];
+  if (a == JBGPU_ISLAND_SLEEPING || a == JBGPU_DISABLE_SIMULATION) return;
+  float* v = vel + 8 * i;
+  V3 lv = ld3(v), av = ld3(v + 4);
+  float im = pr[0];
+  // applyGravity: totalForce = 0 + gravity * linearFactor
+  V3 g = ld3(grav + 4 * i);
+  V3 tf = v3(0.0f + g.x * 1.0f, 0.0f + g.y * 1.0f, 0.0f + g.z * 1.0f);
+  lv.x += im * dt * tf.x;
+  lv.y += im * dt * tf.y;
+  lv.z += im * dt * tf.z;
+  // angular: torque 0 through the world inverse inertia (RigidBody.java:331-334)
+  M3 b;
+  for (int k = 0; k < 9; k++) b.m[k] = tr[12 * i + k];
+  M3 m1;
+  for (int r = 0; r < 3; r++) {
+    m1.m[r * 3 + 0] = b.m[r * 3 + 0] * pr[1];
+    m1.m[r * 3 + 1] = b.m[r * 3 + 1] * pr[2];
+    m1.m[r * 3 + 2] = b.m[r * 3 + 2] * pr[3];
+  }
+  M3 iw = mulMM(m1, transpose(b));
+  V3 t = scl(mulMV(iw, v3(0, 0, 0)), dt);
+  av = add(av, t);
+  float angvel = len(av);
+  if (angvel * dt > MAX_ANGVEL) av = scl(av, (MAX_ANGVEL / dt) / angvel);
+  float ld = (float)pow(1.0 - 0.0, (double)dt);  // applyDamping with zero damping
+  lv = scl(lv, ld);
+  av = scl(av, ld);
+  st3(v, lv);
+  st3(v + 4, av);
+}
+
+__global__ void kIntegrate(int n, float dt, float* tr, const float* __restrict__ props, const int* __restrict__ act,
+                           const float* __restrict__ vel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* pr = props + 8 * i;
+  if ((ldi(pr + 7) & JBGPU_BODY_STATIC) != 0) return;
+  int a = act[i];
+  if (a == JBGPU_ISLAND_SLEEPING || a == JBGPU_DISABLE_SIMULATION) return;
+  float* T = tr + 12 * i;
+  V3 lv = ld3(vel + 8 * i), av = ld3(vel + 8 * i + 4);
+  V3 o = v3(T[9] + dt * lv.x, T[10] + dt * lv.y, T[11] + dt * lv.z);
+  float fAngle = len(av);
+  if (fAngle * dt > ANGULAR_MOTION_THRESHOLD) fAngle = ANGULAR_MOTION_THRESHOLD / dt;
+  V3 axis;
+  if (fAngle < 0.001f)
+    axis = scl(av, 0.5f * dt - (dt * dt * dt) * (0.020833333333f) * fAngle * fAngle);
+  else
+    axis = scl(av, (float)sin((double)(0.5f * fAngle * dt)) / fAngle);
+  Q4 dorn = Q4{axis.x, axis.y, axis.z, (float)cos((double)(fAngle * dt * 0.5f))};
+  M3 b;
+  for (int k = 0; k < 9; k++) b.m[k] = T[k];
+  Q4 orn0 = qFromMatrix(b);
+  Q4 pq = qnor(qmul(dorn, orn0));
+  M3 nb = mFromQ(pq);
+  for (int k = 0; k < 9; k++) T[k] = nb.m[k];
+  T[9] = o.x;
+  T[10] = o.y;
+  T[11] = o.z;
+}
+
+__global__ void kActivation(int n, float dt, int deactDisabled, const float* __restrict__ props, int* act,
+                            float* deact, float* vel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = act[i];
+  float t = deact[i];
+  float* v = vel + 8 * i;
+  auto setAct = [&](int s) {
+    if (a != JBGPU_DISABLE_DEACTIVATION && a != JBGPU_DISABLE_SIMULATION) a = s;
+  };
+  if (!(a == JBGPU_ISLAND_SLEEPING || a == JBGPU_DISABLE_DEACTIVATION)) {
+    if ((len2(ld3(v)) < 0.8f * 0.8f) && (len2(ld3(v + 4)) < 1.0f * 1.0f)) {
+      t += dt;
+    } else {
+      t = 0.0f;
+      setAct(0);
+    }
+  }
+  bool wants;
+  if (a == JBGPU_DISABLE_DEACTIVATION) wants = false;
+  else if (deactDisabled) wants = false;
+  else if (a == JBGPU_ISLAND_SLEEPING || a == JBGPU_WANTS_DEACTIVATION) wants = true;
+  else wants = t > 2.0f;
+  bool isStatic = (ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) != 0;
+  if (wants) {
+    if (isStatic) {
+      setAct(JBGPU_ISLAND_SLEEPING);
+    } else {
+      if (a == JBGPU_ACTIVE_TAG) setAct(JBGPU_WANTS_DEACTIVATION);
+      if (a == JBGPU_ISLAND_SLEEPING) {
+        st3(v, v3(0, 0, 0));
+        st3(v + 4, v3(0, 0, 0));
+      }
+    }
+  } else {
+    if (a != JBGPU_DISABLE_DEACTIVATION) setAct(JBGPU_ACTIVE_TAG);
+  }
+  act[i] = a;
+  deact[i] = t;
+}
+
+template <class F>
+int guard(jbgpu_ctx* ctx, F f) {
+  if (!ctx) return JBGPU_ERR_ARG;
+  try {
+    f();
+    return JBGPU_OK;
+  } catch (const std::invalid_argument& e) {
+    ctx->c.err = e.what();
+    return JBGPU_ERR_ARG;
+  } catch (const std::length_error& e) {
+    ctx->c.err = e.what();
+    return JBGPU_ERR_CAPACITY;
+  } catch (const std::logic_error& e) {
+    ctx->c.err = e.what();
+    return JBGPU_ERR_STATE;
+  } catch (const std::exception& e) {
+    ctx->c.err = e.what();
+    return JBGPU_ERR_HIP;
+  }
+}
+
+void requireWorld(const Ctx& c) {
+  if (!c.haveWorld) throw std::logic_error("no world: call jbgpu_world_create first");
+}
+
+template <class T>
+void upload(DBuf<T>& b, const T* src, size_t n, hipStream_t s) {
+  b.ensure(std::max<size_t>(n, 1));
+  if (n) JB_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+void allocProxyState(Ctx& c, int n) {
+  c.np = n;
+  c.stage.ensure(n);
+  c.leaf.ensure(6 * (size_t)n);
+  c.aabb.ensure(6 * (size_t)n);
+  c.q.ensure(6 * (size_t)n);
+  c.filt.ensure(n);
+  c.world.ensure(n);
+}
+
+void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, const int16_t* mask, const int32_t* world) {
+  allocProxyState(c, n);
+  std::vector<int> stage(n, c.stageCurrent), filt(n), wid(n);
+  std::vector<int> q(6 * (size_t)n, 0);
+  std::vector<float> box(aabb6, aabb6 + 6 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    int16_t g = group ? group[i] : (int16_t)1, m = mask ? mask[i] : (int16_t)-1;
+    filt[i] = (int)(uint16_t)g | ((int)(uint16_t)m << 16);
+    wid[i] = world ? world[i] : 0;
+  }
+  if (c.bpMode == JBGPU_BP_SAP16 || c.bpMode == JBGPU_BP_SAP32) {
+    // AxisSweep3Internal.quantize (host copy of the device routine)
+    for (int i = 0; i < n; i++)
+      for (int mx = 0; mx < 2; mx++)
+        for (int k = 0; k < 3; k++) {
+          float p = box[6 * i + 3 * mx + k];
+          p = jmax(p, c.wmin[k]);
+          p = jmin(p, c.wmax[k]);
+          float v = (p - c.wmin[k]) * c.quant[k];
+          q[6 * i + 3 * mx + k] = (int)((((uint32_t)jint(v) & c.handleMask) | (uint32_t)mx) & c.qmask);
+        }
+  }
+  hipStream_t s = c.stream;
+  JB_CHECK(hipMemcpyAsync(c.stage.p, stage.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.leaf.p, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.aabb.p, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.q.p, q.data(), 6 * (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.filt.p, filt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.world.p, wid.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  c.npairs = 0;
+  c.npairsOld = 0;
+}
+
+void bodiesToDevice(Ctx& c, const float* bodies) {
+  int n = c.nb;
+  std::vector<float> tr(12 * (size_t)n), vel(8 * (size_t)n, 0.0f), props(8 * (size_t)n), grav(4 * (size_t)n, 0.0f),
+      deact(n);
+  std::vector<int> act(n);
+  for (int i = 0; i < n; i++) {
+    const float* r = bodies + (size_t)JBGPU_BODY_WORDS * i;
+    for (int k = 0; k < 12; k++) tr[12 * i + k] = r[k];
+    for (int k = 0; k < 3; k++) {
+      vel[8 * i + k] = r[12 + k];
+      vel[8 * i + 4 + k] = r[15 + k];
+      grav[4 * i + k] = r[22 + k];
+    }
+    props[8 * i + 0] = r[18];
+    props[8 * i + 1] = r[19];
+    props[8 * i + 2] = r[20];
+    props[8 * i + 3] = r[21];
+    props[8 * i + 4] = r[25];
+    props[8 * i + 5] = r[26];
+    props[8 * i + 6] = r[27];
+    props[8 * i + 7] = r[28];
+    act[i] = ldi(r + 29);
+    deact[i] = r[30];
+    int si = ldi(r + 27);
+    if (si < 0 || si >= c.ns) throw std::invalid_argument("body shape index out of range");
+  }
+  hipStream_t s = c.stream;
+  upload(c.tr, tr.data(), tr.size(), s);
+  upload(c.vel, vel.data(), vel.size(), s);
+  upload(c.props, props.data(), props.size(), s);
+  upload(c.grav, grav.data(), grav.size(), s);
+  upload(c.act, act.data(), act.size(), s);
+  upload(c.deact, deact.data(), deact.size(), s);
+  JB_CHECK(hipStreamSynchronize(s));
+}
+
+void bodiesFromDevice(Ctx& c, float* bodies) {
+  int n = c.nb;
+  std::vector<float> tr(12 * (size_t)n), vel(8 * (size_t)n), props(8 * (size_t)n), grav(4 * (size_t)n), deact(n);
+  std::vector<int> act(n);
+  hipStream_t s = c.stream;
+  JB_CHECK(hipMemcpyAsync(tr.data(), c.tr.p, tr.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(vel.data(), c.vel.p, vel.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(props.data(), c.props.p, props.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(grav.data(), c.grav.p, grav.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(act.data(), c.act.p, act.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(deact.data(), c.deact.p, deact.size() * 4, hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) {
+    float* r = bodies + (size_t)JBGPU_BODY_WORDS * i;
+    std::memset(r, 0, JBGPU_BODY_WORDS * sizeof(float));
+    for (int k = 0; k < 12; k++) r[k] = tr[12 * i + k];
+    for (int k = 0; k < 3; k++) {
+      r[12 + k] = vel[8 * i + k];
+      r[15 + k] = vel[8 * i + 4 + k];
+      r[22 + k] = grav[4 * i + k];
+    }
+    r[18] = props[8 * i + 0];
+    r[19] = props[8 * i + 1];
+    r[20] = props[8 * i + 2];
+    r[21] = props[8 * i + 3];
+    r[25] = props[8 * i + 4];
+    r[26] = props[8 * i + 5];
+    r[27] = props[8 * i + 6];
+    r[28] = props[8 * i + 7];
+    sti(r + 29, act[i]);
+    r[30] = deact[i];
+  }
+}
+
+void stepPhases(Ctx& c, float dt, int last) {
+  hipStream_t s = c.stream;
+  tBegin(c, T_STEP);
+  tBegin(c, T_PREDICT);
+  hipLaunchKernelGGL(kPredict, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.grav.p, c.act.p,
+                     c.vel.p);
+  tEnd(c, T_PREDICT);
+  if (last >= 1) {
+    bpUpdateFromBodies(c);
+    bpFindPairs(c);
+    bpRemapManifolds(c);
+  }
+  if (last >= 2) npDispatchAll(c);
+  if (last >= 3) solverSolve(c, dt);
+  if (last >= 4) {
+    tBegin(c, T_INTEGRATE);
+    hipLaunchKernelGGL(kIntegrate, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.act.p,
+                       c.vel.p);
+    tEnd(c, T_INTEGRATE);
+    hipLaunchKernelGGL(kActivation, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.deactivationDisabled,
+                       c.props.p, c.act.p, c.deact.p, c.vel.p);
+  }
+  tEnd(c, T_STEP);
+  JB_CHECK(hipStreamSynchronize(s));
+  if (c.timing) {
+    tCollect(c);
+    c.timedSteps++;
+  }
+  if (!c.err.empty() && c.err == "EPA workspace overflow") throw std::length_error(c.err);
+}
+
+}  // namespace
+
+extern "C" {
+
+int jbgpu_version(void) { return 1; }
+
+int jbgpu_device_count(int32_t* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (count) *count = n;
+  return JBGPU_OK;
+}
+
+void jbgpu_config_default(jbgpu_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->bp_mode = JBGPU_BP_DBVT;
+  for (int k = 0; k < 3; k++) {
+    cfg->world_min[k] = -1000.0f;
+    cfg->world_max[k] = 1000.0f;
+  }
+  cfg->num_iterations = 10;
+  cfg->solver_order = JBGPU_ORDER_COLOR;
+  cfg->deactivation_disabled = 1;
+  cfg->tolerance = 0.0f;
+  cfg->max_iterations = 100;
+  cfg->warmstart = 1;
+  cfg->erp = 0.2f;
+  cfg->warmstart_factor = 0.85f;
+}
+
+int jbgpu_ctx_create(const jbgpu_config* cfg, jbgpu_ctx** out) {
+  if (!out) return JBGPU_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    g_createError = "no HIP device";
+    return JBGPU_ERR_NO_DEVICE;
+  }
+  jbgpu_config d;
+  jbgpu_config_default(&d);
+  if (!cfg) cfg = &d;
+  if (cfg->bp_mode < 0 || cfg->bp_mode > 3 || cfg->num_iterations < 0 || cfg->max_iterations < 1) {
+    g_createError = "invalid configuration";
+    return JBGPU_ERR_ARG;
+  }
+  jbgpu_ctx* ctx = new jbgpu_ctx();
+  Ctx& c = ctx->c;
+  c.bpMode = cfg->bp_mode;
+  for (int k = 0; k < 3; k++) {
+    c.wmin[k] = cfg->world_min[k];
+    c.wmax[k] = cfg->world_max[k];
+  }
+  c.numIterations = cfg->num_iterations;
+  c.solverOrder = cfg->solver_order;
+  c.deactivationDisabled = cfg->deactivation_disabled;
+  c.tolerance = cfg->tolerance;
+  c.maxIterations = cfg->max_iterations;
+  c.warmstart = cfg->warmstart;
+  c.erp = cfg->erp;
+  c.warmFactor = cfg->warmstart_factor;
+  if (hipStreamCreate(&c.stream) != hipSuccess) {
+    g_createError = "hipStreamCreate failed";
+    delete ctx;
+    return JBGPU_ERR_HIP;
+  }
+  bpInitQuant(c);
+  *out = ctx;
+  return JBGPU_OK;
+}
+
+int jbgpu_ctx_destroy(jbgpu_ctx* ctx) {
+  if (!ctx) return JBGPU_ERR_ARG;
+  Ctx& c = ctx->c;
+  (void)hipStreamSynchronize(c.stream);
+  DBuf<float>* fb[] = {&c.shapes, &c.verts, &c.tr, &c.vel, &c.svel, &c.props, &c.grav, &c.deact, &c.leaf, &c.aabb,
+                       &c.cand, &c.leafBox, &c.nodeBox, &c.bounds, &c.man[0], &c.man[1], &c.candC, &c.epaWork,
+                       &c.epaState, &c.rows, &c.learned, &c.features};
+  for (auto* b : fb) b->release();
+  DBuf<int>* ib[] = {&c.act, &c.world, &c.stage, &c.filt, &c.q, &c.order, &c.orderAlt, &c.isHuge, &c.hugeList,
+                     &c.hugeListAlt, &c.hugeWorld, &c.hugeWorldAlt, &c.nodeChild, &c.nodeLast, &c.parent, &c.flags,
+                     &c.ptype, &c.typeKey, &c.typeKeyAlt, &c.perm, &c.permAlt, &c.epaQueue, &c.mActive, &c.color,
+                     &c.colorAlt, &c.colorOrder, &c.colorOrderAlt, &c.done};
+  for (auto* b : ib) b->release();
+  c.morton.release();
+  c.mortonAlt.release();
+  c.rawKeys.release();
+  c.rawKeysAlt.release();
+  c.keys[0].release();
+  c.keys[1].release();
+  c.counter.release();
+  c.cubTmp.release();
+  c.bodyPrio.release();
+  c.bodyMask.release();
+  c.colorKey.release();
+  c.colorKeyAlt.release();
+  c.resid.release();
+  for (int r = 0; r < kTimerRegions; r++) {
+    if (c.evA[r]) (void)hipEventDestroy(c.evA[r]);
+    if (c.evB[r]) (void)hipEventDestroy(c.evB[r]);
+  }
+  (void)hipStreamDestroy(c.stream);
+  delete ctx;
+  return JBGPU_OK;
+}
+
+int jbgpu_last_error(const jbgpu_ctx* ctx, char* buf, int32_t len) {
+  if (!buf || len <= 0) return JBGPU_ERR_ARG;
+  const std::string& e = ctx ? ctx->c.err : g_createError;
+  std::strncpy(buf, e.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return JBGPU_OK;
+}
+
+// ---- shape helpers -------------------------------------------------------------
+int jbgpu_make_shape(int32_t type, const float* params, const float* verts, float* rec) {
+  if (!params || !rec) return JBGPU_ERR_ARG;
+  Shape s{};
+  s.margin = 0.04f;  // ConvexInternalShape.collisionMargin = CONVEX_DISTANCE_MARGIN
+  s.type = type;
+  switch (type) {
+    case JBGPU_SHAPE_SPHERE:  // SphereShape.java:38-41
+      s.dims = v3(params[0], 0, 0);
+      s.margin = params[0] * 1.0f;
+      break;
+    case JBGPU_SHAPE_BOX:  // BoxShape.java:46-50
+      s.dims = sub(mulc(v3(params[0], params[1], params[2]), v3(1, 1, 1)), v3(s.margin, s.margin, s.margin));
+      break;
+    case JBGPU_SHAPE_CAPSULE:  // CapsuleShape.java:54-57
+      s.dims = v3(params[0], 0.5f * params[1], params[0]);
+      break;
+    case JBGPU_SHAPE_HULL: {  // ConvexHullShape.java:45-53 + PolyhedralConvexShape.recalcLocalAabb
+      if (!verts) return JBGPU_ERR_ARG;
+      s.vfirst = (int)params[0];
+      s.vcount = (int)params[1];
+      const V3 dirs[6] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+      V3 sup[6];
+      float w[6];
+      for (int j = 0; j < 6; j++) {
+        w[j] = -1e30f;
+        sup[j] = v3(0, 0, 0);
+      }
+      for (int i = 0; i < s.vcount; i++) {
+        const float* p = verts + 3 * (s.vfirst + i);
+        V3 vtx = v3(p[0] * 1.0f, p[1] * 1.0f, p[2] * 1.0f);
+        for (int j = 0; j < 6; j++) {
+          float d = dot(dirs[j], vtx);
+          if (d > w[j]) {
+            sup[j] = vtx;
+            w[j] = d;
+          }
+        }
+      }
+      s.lmax = v3(sup[0].x + s.margin, sup[1].y + s.margin, sup[2].z + s.margin);
+      s.lmin = v3(sup[3].x - s.margin, sup[4].y - s.margin, sup[5].z - s.margin);
+      break;
+    }
+    case JBGPU_SHAPE_PLANE:  // StaticPlaneShape.java:44-47
+      s.pn = nor(v3(params[0], params[1], params[2]));
+      s.pc = params[3];
+      break;
+    default:
+      return JBGPU_ERR_ARG;
+  }
+  storeShape(s, rec);
+  return JBGPU_OK;
+}
+
+int jbgpu_shape_local_inertia(const float* rec, const float* verts, float mass, float* out) {
+  if (!rec || !out) return JBGPU_ERR_ARG;
+  (void)verts;
+  Shape s = loadShape(rec);
+  V3 r = v3(0, 0, 0);
+  switch (s.type) {
+    case JBGPU_SHAPE_BOX: {  // BoxShape.java:154-167
+      V3 he = v3(s.dims.x + s.margin, s.dims.y + s.margin, s.dims.z + s.margin);
+      float lx = 2.0f * he.x, ly = 2.0f * he.y, lz = 2.0f * he.z;
+      r = v3(mass / 12.0f * (ly * ly + lz * lz), mass / 12.0f * (lx * lx + lz * lz), mass / 12.0f * (lx * lx + ly * ly));
+      break;
+    }
+    case JBGPU_SHAPE_SPHERE: {  // SphereShape.java:73-76
+      float e = 0.4f * mass * s.margin * s.margin;
+      r = v3(e, e, e);
+      break;
+    }
+    case JBGPU_SHAPE_CAPSULE: {  // CapsuleShape.java:126-153
+      float rad = s.dims.x;
+      V3 he = v3(rad, rad + s.dims.y, rad);
+      float m = 0.04f;
+      float lx = 2.0f * (he.x + m), ly = 2.0f * (he.y + m), lz = 2.0f * (he.z + m);
+      float x2 = lx * lx, y2 = ly * ly, z2 = lz * lz;
+      float sm = mass * 0.08333333f;
+      r = v3(sm * (y2 + z2), sm * (x2 + z2), sm * (x2 + y2));
+      break;
+    }
+    case JBGPU_SHAPE_HULL: {  // PolyhedralConvexShape.java:134-159 (AABB in the identity frame)
+      Tr id;
+      id.b = mident();
+      id.o = v3(0, 0, 0);
+      V3 mn, mx;
+      shapeAabb(s, id, mn, mx);
+      V3 he = scl(sub(mx, mn), 0.5f);
+      float m = s.margin;
+      float lx = 2.0f * (he.x + m), ly = 2.0f * (he.y + m), lz = 2.0f * (he.z + m);
+      float x2 = lx * lx, y2 = ly * ly, z2 = lz * lz;
+      float sm = mass * 0.08333333f;
+      r = v3((y2 + z2) * sm, (x2 + z2) * sm, (x2 + y2) * sm);
+      break;
+    }
+    default:
+      break;
+  }
+  st3(out, r);
+  return JBGPU_OK;
+}
+
+// ---- broadphase ------------------------------------------------------------------
+int jbgpu_bp_create_proxies(jbgpu_ctx* ctx, int32_t n, const float* aabb6, const int16_t* group, const int16_t* mask,
+                            const int32_t* world, int32_t* uid_out) {
+  return guard(ctx, [&] {
+    if (n <= 0 || !aabb6) throw std::invalid_argument("bad proxy arguments");
+    if (ctx->c.haveWorld) throw std::logic_error("proxies of a world are created by jbgpu_world_create");
+    uploadProxies(ctx->c, n, aabb6, group, mask, world);
+    if (uid_out)
+      for (int i = 0; i < n; i++) uid_out[i] = i + 1;
+  });
+}
+
+int jbgpu_bp_set_aabbs(jbgpu_ctx* ctx, int32_t n, const float* aabb6, int32_t on_device) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (n != c.np || !aabb6) throw std::invalid_argument("setAabb needs one box per proxy");
+    const float* d = aabb6;
+    if (!on_device) {
+      c.candC.ensure(6 * (size_t)n);
+      JB_CHECK(hipMemcpyAsync(c.candC.p, aabb6, 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+      d = c.candC.p;
+    }
+    bpUpdateFromAabbs(c, d);
+  });
+}
+
+int jbgpu_bp_calculate_overlapping_pairs(jbgpu_ctx* ctx, int64_t* npairs) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (c.np == 0) throw std::logic_error("no proxies");
+    tBegin(c, T_STEP);
+    bpFindPairs(c);
+    bpRemapManifolds(c);
+    tEnd(c, T_STEP);
+    JB_CHECK(hipStreamSynchronize(c.stream));
+    if (c.timing) {
+      tCollect(c);
+      c.timedSteps++;
+    }
+    if (npairs) *npairs = c.npairs;
+  });
+}
+
+int jbgpu_bp_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, int64_t cap, int64_t* n) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (n) *n = c.npairs;
+    if (!keys) return;
+    if (cap < c.npairs) throw std::length_error("pair buffer too small");
+    if (c.npairs)
+      JB_CHECK(hipMemcpy(keys, c.keys[c.cur].p, c.npairs * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int jbgpu_bp_get_proxies(jbgpu_ctx* ctx, float* rec, int32_t* stage_current) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    int n = c.np;
+    std::vector<int> stage(n), filt(n);
+    std::vector<float> leaf(6 * (size_t)n), aabb(6 * (size_t)n);
+    JB_CHECK(hipMemcpy(stage.data(), c.stage.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    JB_CHECK(hipMemcpy(filt.data(), c.filt.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    JB_CHECK(hipMemcpy(leaf.data(), c.leaf.p, leaf.size() * 4, hipMemcpyDeviceToHost));
+    JB_CHECK(hipMemcpy(aabb.data(), c.aabb.p, aabb.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+      float* r = rec + (size_t)JBGPU_PROXY_WORDS * i;
+      std::memset(r, 0, JBGPU_PROXY_WORDS * 4);
+      sti(r, stage[i]);
+      for (int k = 0; k < 6; k++) {
+        r[1 + k] = leaf[6 * i + k];
+        r[7 + k] = aabb[6 * i + k];
+      }
+      sti(r + 13, (int)(int16_t)(filt[i] & 0xffff));
+      sti(r + 14, (int)(int16_t)(filt[i] >> 16));
+    }
+    if (stage_current) *stage_current = c.stageCurrent;
+  });
+}
+
+int jbgpu_bp_set_proxies(jbgpu_ctx* ctx, const float* rec, int32_t stage_current) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    int n = c.np;
+    std::vector<int> stage(n);
+    std::vector<float> leaf(6 * (size_t)n), aabb(6 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+      const float* r = rec + (size_t)JBGPU_PROXY_WORDS * i;
+      stage[i] = ldi(r);
+      for (int k = 0; k < 6; k++) {
+        leaf[6 * i + k] = r[1 + k];
+        aabb[6 * i + k] = r[7 + k];
+      }
+    }
+    JB_CHECK(hipMemcpy(c.stage.p, stage.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    JB_CHECK(hipMemcpy(c.leaf.p, leaf.data(), leaf.size() * 4, hipMemcpyHostToDevice));
+    JB_CHECK(hipMemcpy(c.aabb.p, aabb.data(), aabb.size() * 4, hipMemcpyHostToDevice));
+    c.stageCurrent = stage_current;
+  });
+}
+
+int jbgpu_bp_get_quantized(jbgpu_ctx* ctx, int32_t* q6) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    JB_CHECK(hipMemcpy(q6, c.q.p, 6 * (size_t)c.np * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+// ---- world ------------------------------------------------------------------------
+int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int32_t nverts, const float* verts,
+                       int32_t nbodies, const float* bodies, const int32_t* world_ids) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (nshapes <= 0 || !shapes || nbodies <= 0 || !bodies) throw std::invalid_argument("empty world");
+    c.ns = nshapes;
+    c.nv = nverts;
+    c.nb = nbodies;
+    upload(c.shapes, shapes, (size_t)JBGPU_SHAPE_WORDS * nshapes, c.stream);
+    std::vector<float> vz(3, 0.0f);
+    upload(c.verts, nverts > 0 ? verts : vz.data(), nverts > 0 ? 3 * (size_t)nverts : 3, c.stream);
+    bodiesToDevice(c, bodies);
+    // addRigidBody -> addCollisionObject: proxy with the shape's AABB (no contact threshold)
+    std::vector<float> box(6 * (size_t)nbodies);
+    std::vector<int16_t> group(nbodies), mask(nbodies);
+    for (int i = 0; i < nbodies; i++) {
+      const float* r = bodies + (size_t)JBGPU_BODY_WORDS * i;
+      Shape s = loadShape(shapes + JBGPU_SHAPE_WORDS * ldi(r + 27));
+      if (s.type == JBGPU_SHAPE_HULL && (s.vfirst < 0 || s.vfirst + s.vcount > nverts))
+        throw std::invalid_argument("hull points out of range");
+      Tr t;
+      for (int k = 0; k < 9; k++) t.b.m[k] = r[k];
+      t.o = v3(r[9], r[10], r[11]);
+      V3 mn, mx;
+      Shape sh = s;
+      shapeAabb(sh, t, mn, mx);
+      st3(&box[6 * i], mn);
+      st3(&box[6 * i + 3], mx);
+      bool dyn = (ldi(r + 28) & JBGPU_BODY_STATIC) == 0;
+      group[i] = dyn ? 1 : 2;
+      mask[i] = dyn ? (int16_t)-1 : (int16_t)(-1 ^ 2);
+    }
+    // hull AABBs read the hull points through the shape record only (host copy above uses records)
+    c.stageCurrent = 0;
+    uploadProxies(c, nbodies, box.data(), group.data(), mask.data(), world_ids);
+    c.haveWorld = true;
+  });
+}
+
+int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (!(dt > 0.0f) || nsteps < 0) throw std::invalid_argument("bad step arguments");
+    for (int i = 0; i < nsteps; i++) stepPhases(c, dt, 4);
+  });
+}
+
+int jbgpu_world_step_until(jbgpu_ctx* ctx, float dt, int32_t phase) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (phase < 0 || phase > 4) throw std::invalid_argument("phase must be 0..4");
+    stepPhases(c, dt, phase);
+  });
+}
+
+int jbgpu_world_get_bodies(jbgpu_ctx* ctx, float* bodies) {
+  return guard(ctx, [&] {
+    requireWorld(ctx->c);
+    bodiesFromDevice(ctx->c, bodies);
+  });
+}
+
+int jbgpu_world_set_bodies(jbgpu_ctx* ctx, const float* bodies) {
+  return guard(ctx, [&] {
+    requireWorld(ctx->c);
+    bodiesToDevice(ctx->c, bodies);
+  });
+}
+
+int jbgpu_world_num_pairs(jbgpu_ctx* ctx, int64_t* n) {
+  return guard(ctx, [&] {
+    if (n) *n = ctx->c.npairs;
+  });
+}
+
+int jbgpu_world_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, float* manifolds) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (c.npairs == 0) return;
+    if (keys) JB_CHECK(hipMemcpy(keys, c.keys[c.cur].p, c.npairs * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (manifolds)
+      JB_CHECK(hipMemcpy(manifolds, c.man[c.cur].p, (size_t)c.npairs * JBGPU_MANIFOLD_WORDS * sizeof(float),
+                         hipMemcpyDeviceToHost));
+  });
+}
+
+int jbgpu_world_set_pairs(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (n < 0) throw std::invalid_argument("negative pair count");
+    c.keys[c.cur].ensure(std::max<int64_t>(n, 1));
+    c.man[c.cur].ensure((size_t)JBGPU_MANIFOLD_WORDS * std::max<int64_t>(n, 1));
+    if (n) {
+      JB_CHECK(hipMemcpy(c.keys[c.cur].p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+      JB_CHECK(hipMemcpy(c.man[c.cur].p, manifolds, (size_t)n * JBGPU_MANIFOLD_WORDS * sizeof(float),
+                         hipMemcpyHostToDevice));
+    }
+    c.npairs = n;
+  });
+}
+
+int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
+  return guard(ctx, [&] {
+    const Stats& s = ctx->c.st;
+    out->pairs = s.pairs;
+    out->contacts = s.contacts;
+    out->manifolds = ctx->c.nActive;
+    out->colors = s.colors;
+    out->epa_calls = s.epaCalls;
+    out->epa_fails = s.epaFails;
+    out->iterations = s.iterations;
+    out->residual = s.residual;
+    out->huge_proxies = s.huge;
+  });
+}
+
+// ---- solver ------------------------------------------------------------------------
+int jbgpu_solver_get_order(jbgpu_ctx* ctx, int32_t* order, int64_t cap, int64_t* n, int64_t* offsets,
+                           int32_t* ncolors) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    int64_t na = c.nActive;
+    if (n) *n = na;
+    if (ncolors) *ncolors = c.ncolors;
+    if (offsets)
+      for (int k = 0; k <= c.ncolors; k++) offsets[k] = c.colorOff[k];
+    if (!order || na == 0) return;
+    if (cap < na) throw std::length_error("order buffer too small");
+    std::vector<int> co(na), act(na);
+    JB_CHECK(hipMemcpy(co.data(), c.colorOrder.p, na * sizeof(int), hipMemcpyDeviceToHost));
+    JB_CHECK(hipMemcpy(act.data(), c.mActive.p, na * sizeof(int), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < na; k++) order[k] = act[co[k]];
+  });
+}
+
+int jbgpu_solver_set_warmstart(jbgpu_ctx* ctx, const float* init, int64_t nslots, int32_t on_device) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (!init) {
+      c.useLearned = false;
+      return;
+    }
+    c.learned.ensure(3 * (size_t)std::max<int64_t>(nslots, 1));
+    JB_CHECK(hipMemcpy(c.learned.p, init, 3 * (size_t)nslots * sizeof(float),
+                       on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    c.useLearned = true;
+    c.learnedSlots = nslots;
+  });
+}
+
+int jbgpu_solver_enable_features(jbgpu_ctx* ctx, int32_t on) {
+  return guard(ctx, [&] { ctx->c.dumpFeatures = on != 0; });
+}
+
+int jbgpu_solver_get_features(jbgpu_ctx* ctx, float* out, int64_t nslots) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (!c.dumpFeatures) throw std::logic_error("features not enabled");
+    if (nslots > 4 * c.nActive) throw std::length_error("more slots than the last solve had");
+    if (nslots) JB_CHECK(hipMemcpy(out, c.features.p, 16 * (size_t)nslots * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations) {
+  return guard(ctx, [&] {
+    if (max_iterations < 1 || num_iterations < 0) throw std::invalid_argument("bad iteration counts");
+    ctx->c.tolerance = tolerance;
+    ctx->c.maxIterations = max_iterations;
+    ctx->c.numIterations = num_iterations;
+  });
+}
+
+int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32_t* n) {
+  return guard(ctx, [&] {
+    const auto& h = ctx->c.residHist;
+    if (n) *n = (int32_t)h.size();
+    for (int32_t k = 0; k < cap && k < (int32_t)h.size(); k++) out[k] = h[k];
+  });
+}
+
+// ---- timing -------------------------------------------------------------------------
+int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (on && !c.evA[0])
+      for (int r = 0; r < kTimerRegions; r++) {
+        JB_CHECK(hipEventCreate(&c.evA[r]));
+        JB_CHECK(hipEventCreate(&c.evB[r]));
+      }
+    c.timing = on != 0;
+    for (int r = 0; r < kTimerRegions; r++) {
+      c.ms[r] = 0;
+      c.launches[r] = 0;
+      c.pending[r] = false;
+    }
+    c.timedSteps = 0;
+  });
+}
+
+int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    for (int r = 0; r < kTimerRegions; r++) {
+      if (ms) ms[r] = c.ms[r];
+      if (launches) launches[r] = c.launches[r];
+    }
+    if (steps) *steps = c.timedSteps;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+"""jbgpu — Python host binding of libjbgpu (the gfx950 rigid-body hot path).
+
+The product is the C ABI in ``include/jbgpu.h``; this module is a thin ctypes
+mirror of it with numpy buffers, arranged after the reference's plugin seams:
+:class:`Context` exposes the BroadphaseInterface entry points (``bp_*``), the
+fused DynamicsWorld step (``world_*``) and the ConstraintSolver hooks
+(``solver_*``).  It raises :class:`JbgpuError` on any non-zero status; there is
+no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "libjbgpu.so")
+
+SHAPE_WORDS, BODY_WORDS, PROXY_WORDS, POINT_WORDS, MANIFOLD_WORDS = 20, 32, 16, 32, 132
+BP_DBVT, BP_SAP16, BP_SAP32, BP_SIMPLE = 0, 1, 2, 3
+ORDER_REFERENCE, ORDER_COLOR, ORDER_EXPLICIT = 0, 1, 2
+TIMER_REGIONS = 12
+TIMER_NAMES = ["predict", "setaabb", "traverse", "gjk", "manifold", "solver_setup", "solver_iter", "integrate",
+               "step", "epa", "remap", "unused"]
+
+# every exported symbol of include/jbgpu.h: (restype, argtypes)
+_f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class Config(C.Structure):
+    _fields_ = [("bp_mode", C.c_int32), ("world_min", C.c_float * 3), ("world_max", C.c_float * 3),
+                ("num_iterations", C.c_int32), ("solver_order", C.c_int32), ("deactivation_disabled", C.c_int32),
+                ("tolerance", C.c_float), ("max_iterations", C.c_int32), ("warmstart", C.c_int32),
+                ("erp", C.c_float), ("warmstart_factor", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64), ("manifolds", C.c_int64), ("colors", C.c_int64),
+                ("epa_calls", C.c_int64), ("epa_fails", C.c_int64), ("iterations", C.c_int64),
+                ("residual", C.c_float), ("huge_proxies", C.c_int64)]
+
+
+SYMBOLS = {
+    "jbgpu_version": (C.c_int, []),
+    "jbgpu_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "jbgpu_config_default": (None, [C.POINTER(Config)]),
+    "jbgpu_ctx_create": (C.c_int, [C.POINTER(Config), C.POINTER(_vp)]),
+    "jbgpu_ctx_destroy": (C.c_int, [_vp]),
+    "jbgpu_last_error": (C.c_int, [_vp, C.c_char_p, C.c_int32]),
+    "jbgpu_make_shape": (C.c_int, [C.c_int32, _f, _vp, _f]),
+    "jbgpu_shape_local_inertia": (C.c_int, [_f, _vp, C.c_float, _f]),
+    "jbgpu_bp_create_proxies": (C.c_int, [_vp, C.c_int32, _f, _vp, _vp, _vp, _vp]),
+    "jbgpu_bp_set_aabbs": (C.c_int, [_vp, C.c_int32, _vp, C.c_int32]),
+    "jbgpu_bp_calculate_overlapping_pairs": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "jbgpu_bp_get_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64)]),
+    "jbgpu_bp_get_proxies": (C.c_int, [_vp, _f, C.POINTER(C.c_int32)]),
+    "jbgpu_bp_set_proxies": (C.c_int, [_vp, _f, C.c_int32]),
+    "jbgpu_bp_get_quantized": (C.c_int, [_vp, _i32]),
+    "jbgpu_world_create": (C.c_int, [_vp, C.c_int32, _f, C.c_int32, _f, C.c_int32, _f, _vp]),
+    "jbgpu_world_step": (C.c_int, [_vp, C.c_float, C.c_int32]),
+    "jbgpu_world_step_until": (C.c_int, [_vp, C.c_float, C.c_int32]),
+    "jbgpu_world_get_bodies": (C.c_int, [_vp, _f]),
+    "jbgpu_world_set_bodies": (C.c_int, [_vp, _f]),
+    "jbgpu_world_num_pairs": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "jbgpu_world_get_pairs": (C.c_int, [_vp, _vp, _vp]),
+    "jbgpu_world_set_pairs": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
+    "jbgpu_world_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+    "jbgpu_solver_get_order": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64), _vp, C.POINTER(C.c_int32)]),
+    "jbgpu_solver_set_warmstart": (C.c_int, [_vp, _vp, C.c_int64, C.c_int32]),
+    "jbgpu_solver_enable_features": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_solver_get_features": (C.c_int, [_vp, _f, C.c_int64]),
+    "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
+    "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
+    "jbgpu_timing_enable": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_timing_get": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int64)]),
+}
+
+
+class JbgpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libjbgpu.so (built in-tree by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise JbgpuError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def device_count():
+    n = C.c_int32(0)
+    load().jbgpu_device_count(C.byref(n))
+    return n.value
+
+
+def default_config(**kw):
+    cfg = Config()
+    load().jbgpu_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        if k in ("world_min", "world_max"):
+            getattr(cfg, k)[:] = [float(x) for x in v]
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+# ---- shape helpers (host; the record builder for jbgpu.scenes.build_records) ----
+def make_shape(stype, params, verts=None):
+    rec = np.zeros(SHAPE_WORDS, np.float32)
+    vv = None if verts is None else f32(verts).reshape(-1)
+    rc = load().jbgpu_make_shape(int(stype), f32(np.asarray(params, np.float32).reshape(-1)), _p(vv), rec)
+    if rc:
+        raise JbgpuError(f"jbgpu_make_shape failed ({rc})")
+    return rec
+
+
+def shape_inertia(rec, mass, verts=None):
+    out = np.zeros(3, np.float32)
+    vv = None if verts is None else f32(verts).reshape(-1)
+    rc = load().jbgpu_shape_local_inertia(f32(rec), _p(vv), float(mass), out)
+    if rc:
+        raise JbgpuError(f"jbgpu_shape_local_inertia failed ({rc})")
+    return out
+
+
+class Context:
+    """One jbgpu context (one HIP stream) with a broadphase, dispatcher and solver."""
+
+    def __init__(self, **cfg):
+        L = load()
+        self._cfg = default_config(**cfg)
+        h = C.c_void_p()
+        rc = L.jbgpu_ctx_create(C.byref(self._cfg), C.byref(h))
+        if rc:
+            buf = C.create_string_buffer(256)
+            L.jbgpu_last_error(None, buf, 256)
+            raise JbgpuError(f"jbgpu_ctx_create failed ({rc}): {buf.value.decode()}")
+        self.h = h
+        self.nb = 0
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().jbgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc:
+            buf = C.create_string_buffer(512)
+            load().jbgpu_last_error(self.h, buf, 512)
+            raise JbgpuError(f"{what} failed ({rc}): {buf.value.decode()}")
+
+    # ---- BroadphaseInterface
+    def bp_create_proxies(self, aabb6, group=None, mask=None, world=None):
+        a = f32(aabb6).reshape(-1)
+        n = a.size // 6
+        g = None if group is None else np.ascontiguousarray(group, np.int16)
+        m = None if mask is None else np.ascontiguousarray(mask, np.int16)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        uid = np.zeros(n, np.int32)
+        self._ck(load().jbgpu_bp_create_proxies(self.h, n, a, _p(g), _p(m), _p(w), _p(uid)), "bp_create_proxies")
+        self.np = n
+        return uid
+
+    def bp_set_aabbs(self, aabb6):
+        a = f32(aabb6).reshape(-1)
+        self._ck(load().jbgpu_bp_set_aabbs(self.h, a.size // 6, _p(a), 0), "bp_set_aabbs")
+
+    def bp_set_aabbs_device(self, ptr, n):
+        self._ck(load().jbgpu_bp_set_aabbs(self.h, int(n), C.c_void_p(int(ptr)), 1), "bp_set_aabbs")
+
+    def bp_calculate_overlapping_pairs(self):
+        n = C.c_int64(0)
+        self._ck(load().jbgpu_bp_calculate_overlapping_pairs(self.h, C.byref(n)), "calculate_overlapping_pairs")
+        return n.value
+
+    def bp_pairs(self):
+        n = C.c_int64(0)
+        self._ck(load().jbgpu_bp_get_pairs(self.h, None, 0, C.byref(n)), "bp_get_pairs")
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        self._ck(load().jbgpu_bp_get_pairs(self.h, _p(keys), keys.size, C.byref(n)), "bp_get_pairs")
+        return keys[:n.value]
+
+    def bp_proxies(self):
+        out = np.zeros(self.np * PROXY_WORDS, np.float32)
+        sc = C.c_int32(0)
+        self._ck(load().jbgpu_bp_get_proxies(self.h, out, C.byref(sc)), "bp_get_proxies")
+        return out.reshape(self.np, PROXY_WORDS), sc.value
+
+    def bp_set_proxies(self, rec, stage_current):
+        self._ck(load().jbgpu_bp_set_proxies(self.h, f32(rec).reshape(-1), int(stage_current)), "bp_set_proxies")
+
+    def bp_quantized(self):
+        out = np.zeros(self.np * 6, np.int32)
+        self._ck(load().jbgpu_bp_get_quantized(self.h, out), "bp_get_quantized")
+        return out.reshape(self.np, 6)
+
+    # ---- DynamicsWorld
+    def world_create(self, shapes, verts, bodies, world_ids=None):
+        s = f32(shapes).reshape(-1)
+        v = f32(verts if verts is not None and len(verts) else np.zeros((1, 3))).reshape(-1)
+        b = f32(bodies).reshape(-1)
+        w = None if world_ids is None else np.ascontiguousarray(world_ids, np.int32)
+        self.nb = b.size // BODY_WORDS
+        self._ck(load().jbgpu_world_create(self.h, s.size // SHAPE_WORDS, s, v.size // 3, v, self.nb, b, _p(w)),
+                 "world_create")
+        self.np = self.nb
+
+    def step(self, dt=1.0 / 60.0, n=1):
+        self._ck(load().jbgpu_world_step(self.h, dt, int(n)), "world_step")
+
+    def step_until(self, phase, dt=1.0 / 60.0):
+        self._ck(load().jbgpu_world_step_until(self.h, dt, int(phase)), "world_step_until")
+
+    def bodies(self):
+        out = np.zeros(self.nb * BODY_WORDS, np.float32)
+        self._ck(load().jbgpu_world_get_bodies(self.h, out), "world_get_bodies")
+        return out.reshape(self.nb, BODY_WORDS)
+
+    def set_bodies(self, b):
+        self._ck(load().jbgpu_world_set_bodies(self.h, f32(b).reshape(-1)), "world_set_bodies")
+
+    def pairs(self):
+        n = C.c_int64(0)
+        self._ck(load().jbgpu_world_num_pairs(self.h, C.byref(n)), "world_num_pairs")
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        man = np.zeros(max(n.value, 1) * MANIFOLD_WORDS, np.float32)
+        self._ck(load().jbgpu_world_get_pairs(self.h, _p(keys), _p(man)), "world_get_pairs")
+        return keys[:n.value], man.reshape(-1, MANIFOLD_WORDS)[:n.value]
+
+    def set_pairs(self, keys, man):
+        k = np.ascontiguousarray(keys, np.uint64)
+        m = f32(man).reshape(-1)
+        self._ck(load().jbgpu_world_set_pairs(self.h, len(k), _p(k), _p(m)), "world_set_pairs")
+
+    def stats(self):
+        s = Stats()
+        self._ck(load().jbgpu_world_stats(self.h, C.byref(s)), "world_stats")
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    # ---- ConstraintSolver
+    def solver_order(self):
+        n = C.c_int64(0)
+        nc = C.c_int32(0)
+        self._ck(load().jbgpu_solver_get_order(self.h, None, 0, C.byref(n), None, C.byref(nc)), "solver_get_order")
+        order = np.zeros(max(n.value, 1), np.int32)
+        offs = np.zeros(nc.value + 1, np.int64)
+        self._ck(load().jbgpu_solver_get_order(self.h, _p(order), order.size, C.byref(n), _p(offs), C.byref(nc)),
+                 "solver_get_order")
+        return order[:n.value], offs
+
+    def solver_set_warmstart(self, init):
+        if init is None:
+            self._ck(load().jbgpu_solver_set_warmstart(self.h, None, 0, 0), "solver_set_warmstart")
+            return
+        a = f32(init).reshape(-1)
+        self._keep = a
+        self._ck(load().jbgpu_solver_set_warmstart(self.h, _p(a), a.size // 3, 0), "solver_set_warmstart")
+
+    def solver_enable_features(self, on=True):
+        self._ck(load().jbgpu_solver_enable_features(self.h, int(on)), "solver_enable_features")
+
+    def solver_features(self, nslots):
+        out = np.zeros(max(nslots, 1) * 16, np.float32)
+        self._ck(load().jbgpu_solver_get_features(self.h, out, int(nslots)), "solver_get_features")
+        return out.reshape(-1, 16)[:nslots]
+
+    def solver_set_params(self, tolerance=0.0, max_iterations=100, num_iterations=10):
+        self._ck(load().jbgpu_solver_set_params(self.h, float(tolerance), int(max_iterations), int(num_iterations)),
+                 "solver_set_params")
+
+    def residual_history(self):
+        out = np.zeros(1024, np.float32)
+        n = C.c_int32(0)
+        self._ck(load().jbgpu_solver_residual_history(self.h, out, 1024, C.byref(n)), "residual_history")
+        return out[:min(n.value, 1024)].copy()
+
+    # ---- measurement
+    def timing(self, on=True):
+        self._ck(load().jbgpu_timing_enable(self.h, int(on)), "timing_enable")
+
+    def timing_get(self):
+        ms = np.zeros(TIMER_REGIONS, np.float64)
+        ln = np.zeros(TIMER_REGIONS, np.int64)
+        st = C.c_int64(0)
+        self._ck(load().jbgpu_timing_get(self.h, _p(ms), _p(ln), C.byref(st)), "timing_get")
+        return {TIMER_NAMES[i]: (float(ms[i]), int(ln[i])) for i in range(TIMER_REGIONS)}, st.value
