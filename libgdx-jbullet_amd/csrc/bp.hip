@@ -448,20 +448,19 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
     int64_t mid = (lo + hi) >> 1;
     if (keysOld[mid] < key) lo = mid + 1; else hi = mid;
   }
+  // only the header and the live points travel (slots past the point count are never read)
   float4* dst = (float4*)(manNew + (size_t)JBGPU_MANIFOLD_WORDS * i);
   if (lo < nOld && keysOld[lo] == key) {
     const float4* src = (const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * lo);
-    for (int k = 0; k < JBGPU_MANIFOLD_WORDS / 4; k++) dst[k] = src[k];
+    float4 h = src[0];
+    dst[0] = h;
+    int words = JBGPU_POINT_WORDS * min(max(__builtin_bit_cast(int, h.x), 0), 4);
+    for (int k = 0; k < words / 4; k++) dst[1 + k] = src[1 + k];
   } else {
     int b0 = a, b1 = b;
     if (type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE) { b0 = b; b1 = a; }
-    float4 z = make_float4(0, 0, 0, 0);
-    for (int k = 0; k < JBGPU_MANIFOLD_WORDS / 4; k++) dst[k] = z;
-    float* hdr = manNew + (size_t)JBGPU_MANIFOLD_WORDS * i;
-    sti(hdr + 0, 0);
-    sti(hdr + 1, b0);
-    sti(hdr + 2, b1);
-    sti(hdr + 3, type);
+    dst[0] = make_float4(__builtin_bit_cast(float, 0), __builtin_bit_cast(float, b0), __builtin_bit_cast(float, b1),
+                         __builtin_bit_cast(float, type));
   }
 }
 

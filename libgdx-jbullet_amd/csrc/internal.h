@@ -127,6 +127,16 @@ struct Ctx {
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
   // solver
   DBuf<float> rows;
+  DBuf<int4> hdr;            // per colour slot: point mask, body A, body B, pair index
+  DBuf<float> nrow, frow;    // per colour slot: 4 normal rows, 8 friction rows
+  DBuf<int2> activeBodies;   // per active manifold: body A, body B
+  DBuf<int> colorStart;
+  // learned warm start (16-64-64-64-3 MLP, bf16 weights, fp32 accumulate)
+  bool mlpReady = false;
+  DBuf<float> mlpW;          // fp32 master parameters
+  DBuf<uint16_t> mlpWb;      // bf16 copies used by the MFMA kernels
+  DBuf<float> mlpNorm;       // input / output normalisation
+  DBuf<float> mlpOut;        // per slot predictions (3 floats)
   DBuf<int> mActive;  // active manifold list
   DBuf<unsigned long long> bodyPrio;
   DBuf<unsigned long long> bodyMask;

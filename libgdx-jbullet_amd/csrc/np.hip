@@ -53,13 +53,18 @@ __device__ __forceinline__ void putCand(float* out, int i, bool valid, V3 n, V3 
 __global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
                          const int* __restrict__ act, int* tkey, int* idx, unsigned long long* counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
-  int t = ptype[i];
-  if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
-  tkey[i] = t;
-  idx[i] = (int)i;
-  atomicAdd(counts + t, 1ull);
+  int t = -1;
+  if (i < n) {
+    int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+    t = ptype[i];
+    if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
+    tkey[i] = t;
+    idx[i] = (int)i;
+  }
+  for (int k = 0; k < 4; k++) {  // one atomic per wave and bin
+    unsigned long long m = __ballot(t == k);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(counts + k, (unsigned long long)__popcll(m));
+  }
 }
 
 // ------------------------------------------------------------------ sphere-sphere

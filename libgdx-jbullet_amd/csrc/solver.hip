@@ -8,25 +8,33 @@
 // :61-93).  Instead of the reference's Random(0)-shuffled row order the
 // manifolds that carry rows are coloured (Jones-Plassmann with hashed
 // priorities, smallest free colour per round) so that manifolds of one colour
-// share no dynamic body; one thread handles one manifold's points in order,
-// and a colour is one launch.  Processing colour after colour is then exactly
-// the sequential solver fed the manifold order "colour 0 manifolds, colour 1
-// manifolds, ..." (any order inside a colour gives the same bits), which the
-// library reports so the CPU oracle can be fed the same order.
+// share no dynamic body; one thread handles one manifold's points in order and
+// a colour is one launch.  Processing colour after colour is exactly the
+// sequential solver fed the manifold order "colour 0 manifolds, colour 1
+// manifolds, ..." (any order inside a colour gives the same bits); the library
+// reports that order so the CPU oracle can be fed the same order.
+//
+// Row storage is in colour-slot order (slot p = position in the colour order):
+// a 16-byte header {point mask, body A, body B, pair index} and the rows of the
+// slot's four contacts, so a sweep thread issues its header and row loads in one
+// round trip and the two body velocities in a second.  Setup is split into a
+// fully parallel row build (rows depend only on the pre-solve state) and a
+// per-colour warm-start application (impulses are applied in manifold order).
 #include <cmath>
 
 #include <hipcub/hipcub.hpp>
 
 #include "internal.h"
+#include "mlp.h"
 #include "shapes.h"
 
 namespace jbg {
 
 namespace {
 
-constexpr int RW = kRowWords;          // words per row
-constexpr int CW = kContactRowWords;   // words per contact (normal + 2 friction rows)
-constexpr int MW = kManifoldRowWords;  // words per manifold
+constexpr int RW = kRowWords;  // words per row
+constexpr int NW = 4 * RW;     // normal rows per slot
+constexpr int FW = 8 * RW;     // friction rows per slot
 
 struct SolverParams {
   float dt, erp, warmFactor, linearSlop;
@@ -68,54 +76,58 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
   x ^= x >> 16;
   return x;
 }
-
-__device__ __forceinline__ void bodiesOf(const float* man, const int* list, int k, int& a, int& b) {
-  const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[k];
-  a = ldi(m + 1);
-  b = ldi(m + 2);
-}
 __device__ __forceinline__ bool dynamicBody(const float* props, int i) {
   return (ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) == 0;
 }
-
-__global__ void kColorPropose(int n, int round, const int* __restrict__ list, const float* __restrict__ man,
-                              const float* __restrict__ props, const int* __restrict__ color,
-                              unsigned long long* bodyPrio) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || color[k] >= 0) return;
-  int a, b;
-  bodiesOf(man, list, k, a, b);
-  unsigned long long p = ((unsigned long long)round << 52) | ((unsigned long long)(hash32(k) >> 12) << 32) |
-                         (unsigned long long)k;
-  if (dynamicBody(props, a)) atomicMax(bodyPrio + a, p);
-  if (dynamicBody(props, b)) atomicMax(bodyPrio + b, p);
+__device__ __forceinline__ unsigned long long prio(int round, int k) {
+  return ((unsigned long long)round << 52) | ((unsigned long long)(hash32(k) >> 12) << 32) | (unsigned long long)k;
 }
 
-__global__ void kColorResolve(int n, int round, const int* __restrict__ list, const float* __restrict__ man,
-                              const float* __restrict__ props, int* color, const unsigned long long* bodyPrio,
-                              unsigned long long* bodyMask, unsigned long long* remaining) {
+// body pair of active manifold k
+__global__ void kActiveBodies(int n, const int* __restrict__ list, const float* __restrict__ man, int2* ab) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[k];
+  ab[k] = make_int2(ldi(m + 1), ldi(m + 2));
+}
+
+__global__ void kColorPropose(int n, int round, const int2* __restrict__ ab, const float* __restrict__ props,
+                              const int* __restrict__ color, unsigned long long* bodyPrio) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n || color[k] >= 0) return;
-  int a, b;
-  bodiesOf(man, list, k, a, b);
-  unsigned long long p = ((unsigned long long)round << 52) | ((unsigned long long)(hash32(k) >> 12) << 32) |
-                         (unsigned long long)k;
-  bool da = dynamicBody(props, a), db = dynamicBody(props, b);
-  bool win = (!da || bodyPrio[a] == p) && (!db || bodyPrio[b] == p);
-  if (!win) {
-    atomicAdd(remaining, 1ull);
-    return;
+  int2 b = ab[k];
+  unsigned long long p = prio(round, k);
+  if (dynamicBody(props, b.x)) atomicMax(bodyPrio + b.x, p);
+  if (dynamicBody(props, b.y)) atomicMax(bodyPrio + b.y, p);
+}
+
+__global__ void kColorResolve(int n, int round, const int2* __restrict__ ab, const float* __restrict__ props, int* color,
+                              const unsigned long long* bodyPrio, unsigned long long* bodyMask,
+                              unsigned long long* remaining) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pending = false;
+  if (k < n && color[k] < 0) {
+    int2 b = ab[k];
+    unsigned long long p = prio(round, k);
+    bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
+    bool win = (!da || bodyPrio[b.x] == p) && (!db || bodyPrio[b.y] == p);
+    if (win) {
+      unsigned long long used = (da ? bodyMask[b.x] : 0ull) | (db ? bodyMask[b.y] : 0ull);
+      int c;
+      if (used == ~0ull) {
+        c = 64 + round;
+      } else {
+        c = __ffsll((long long)~used) - 1;
+        if (da) bodyMask[b.x] |= (1ull << c);
+        if (db) bodyMask[b.y] |= (1ull << c);
+      }
+      color[k] = c;
+    } else {
+      pending = true;
+    }
   }
-  unsigned long long used = (da ? bodyMask[a] : 0ull) | (db ? bodyMask[b] : 0ull);
-  int c;
-  if (used == ~0ull) {
-    c = 64 + round;
-  } else {
-    c = __ffsll((long long)~used) - 1;
-    if (da) bodyMask[a] |= (1ull << c);
-    if (db) bodyMask[b] |= (1ull << c);
-  }
-  color[k] = c;
+  unsigned long long m = __ballot(pending);  // one atomic per wave
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(remaining, (unsigned long long)__popcll(m));
 }
 
 __global__ void kIota(int n, int* out) {
@@ -123,10 +135,11 @@ __global__ void kIota(int n, int* out) {
   if (k < n) out[k] = k;
 }
 
-__global__ void kColorCount(int n, const int* __restrict__ color, unsigned long long* counts) {
+// first slot of every colour in the colour-sorted key array
+__global__ void kColorStarts(int n, const int* __restrict__ sorted, int* starts) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  atomicAdd(counts + color[k], 1ull);
+  if (k == 0 || sorted[k] != sorted[k - 1]) starts[sorted[k]] = k;
 }
 
 __device__ __forceinline__ M3 invInertiaWorld(const float* tr, const float* props, int i) {  // RigidBody.java:449-459
@@ -146,17 +159,6 @@ struct SBody {
   V3 lv, av;
   float invMass;
 };
-__device__ __forceinline__ SBody loadS(const float* v, const float* props, int i) {
-  SBody s;
-  s.lv = ld3(v + 8 * i);
-  s.av = ld3(v + 8 * i + 4);
-  s.invMass = props[8 * i];
-  return s;
-}
-__device__ __forceinline__ void storeS(float* v, int i, const SBody& s) {
-  st3(v + 8 * i, s.lv);
-  st3(v + 8 * i + 4, s.av);
-}
 __device__ __forceinline__ void applyImpulse(SBody& b, V3 lc, V3 ac, float mag) {  // SolverBody.java:61-72
   if (b.invMass != 0.0f) {
     b.lv.x += mag * 1.0f * lc.x;
@@ -166,6 +168,20 @@ __device__ __forceinline__ void applyImpulse(SBody& b, V3 lc, V3 ac, float mag) 
     b.av.y += mag * 1.0f * ac.y;
     b.av.z += mag * 1.0f * ac.z;
   }
+}
+__device__ __forceinline__ SBody loadBody(const float* __restrict__ v, const float* __restrict__ props, int i) {
+  const float4* p = (const float4*)(v + 8 * i);
+  float4 a = p[0], b = p[1];
+  SBody s;
+  s.lv = v3(a.x, a.y, a.z);
+  s.av = v3(b.x, b.y, b.z);
+  s.invMass = props[8 * i];
+  return s;
+}
+__device__ __forceinline__ void storeBody(float* v, int i, const SBody& s) {
+  float4* p = (float4*)(v + 8 * i);
+  p[0] = make_float4(s.lv.x, s.lv.y, s.lv.z, 0.0f);
+  p[1] = make_float4(s.av.x, s.av.y, s.av.z, 0.0f);
 }
 
 __device__ void frictionRow(float* R, V3 axis, V3 rp1, V3 rp2, const M3& I0, const M3& I1, float im0, float im1,
@@ -203,29 +219,29 @@ __device__ __forceinline__ void planeSpace1(V3 n, V3& p, V3& q) {  // TransformU
   }
 }
 
-// one thread per manifold of colour c (solveGroupCacheFriendlySetup :494-747)
-__global__ void kSetup(int cnt, const int* __restrict__ corder, const int* __restrict__ list, SolverParams SP,
-                       float* man, const float* __restrict__ tr, const float* __restrict__ vel,
-                       const float* __restrict__ props, float* svel, float* rows, const float* __restrict__ learned,
-                       float* features) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  int ai = corder[k];
-  int pi = list[ai];
+// rows of every slot (solveGroupCacheFriendlySetup :494-747 without the impulse application);
+// the warm-start impulse of each row is left in its applied word
+__global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __restrict__ list, SolverParams SP,
+                           float* man, const float* __restrict__ tr, const float* __restrict__ vel,
+                           const float* __restrict__ props, int4* hdr, float* nrow, float* frow,
+                           const float* __restrict__ learned, float* features) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= na) return;
+  int pi = list[corder[p]];
   float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * pi;
   int n = ldi(m), b0 = ldi(m + 1), b1 = ldi(m + 2);
-  float* R = rows + (size_t)MW * ai;
+  float* N0 = nrow + (size_t)NW * p;
+  float* F0 = frow + (size_t)FW * p;
   M3 I0 = invInertiaWorld(tr, props, b0), I1 = invInertiaWorld(tr, props, b1);
   V3 o0 = ld3(tr + 12 * b0 + 9), o1 = ld3(tr + 12 * b1 + 9);
   V3 lv0 = ld3(vel + 8 * b0), av0 = ld3(vel + 8 * b0 + 4), lv1 = ld3(vel + 8 * b1), av1 = ld3(vel + 8 * b1 + 4);
   float im0 = props[8 * b0], im1 = props[8 * b1];
-  SBody A = loadS(svel, props, b0), B = loadS(svel, props, b1);
   int mask = 0;
   for (int j = 0; j < n; j++) {
     float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
     if (!(P[15] <= 0.0f)) continue;
     mask |= 1 << j;
-    float* N = R + CW * j;
+    float* N = N0 + RW * j;
     V3 normal = ld3(P + 12);
     V3 rp1 = sub(ld3(P + 6), o0);
     V3 rp2 = sub(ld3(P + 9), o1);
@@ -252,7 +268,7 @@ __global__ void kSetup(int cnt, const int* __restrict__ corder, const int* __res
     N[16] = pen;
     N[17] = rest;
     N[18] = friction;
-    const int slot = ai * 4 + j;
+    const int slot = p * 4 + j;
     if (SP.dumpFeatures) {
       float* F = features + (size_t)16 * slot;
       F[0] = normal.x; F[1] = normal.y; F[2] = normal.z;
@@ -261,18 +277,8 @@ __global__ void kSetup(int cnt, const int* __restrict__ corder, const int* __res
       F[9] = P[15]; F[10] = im0; F[11] = im1;
       F[12] = P[18]; F[13] = P[19]; F[14] = P[20]; F[15] = friction;
     }
-    float applied;
-    if (SP.useLearned) applied = learned[3 * slot];
-    else if (SP.warm) applied = P[18] * SP.warmFactor;
-    else applied = 0.0f;
-    N[19] = applied;
-    if (SP.useLearned || SP.warm) {
-      applyImpulse(A, scl(normal, im0), angA, applied);
-      applyImpulse(B, scl(normal, im1), angB, -applied);
-    }
+    N[19] = SP.useLearned ? learned[3 * slot] : (SP.warm ? P[18] * SP.warmFactor : 0.0f);
     // lateral friction directions (:678-708)
-    float* F1 = N + RW;
-    float* F2 = N + 2 * RW;
     if (ldi(P + 27) == 0) {
       V3 d1v = sub(vv, scl(normal, rel_vel));
       float lat = len2(d1v);
@@ -287,24 +293,44 @@ __global__ void kSetup(int cnt, const int* __restrict__ corder, const int* __res
       st3(P + 24, d2v);
       sti(P + 27, 1);
     }
+    float* F1 = F0 + RW * (2 * j);
+    float* F2 = F0 + RW * (2 * j + 1);
     frictionRow(F1, ld3(P + 21), rp1, rp2, I0, I1, im0, im1, friction);
     frictionRow(F2, ld3(P + 24), rp1, rp2, I0, I1, im0, im1, friction);
-    for (int f = 0; f < 2; f++) {  // friction warm start (:710-747)
-      float* FR = f == 0 ? F1 : F2;
-      float fa;
-      if (SP.useLearned) fa = learned[3 * slot + 1 + f];
-      else if (SP.warm) fa = P[19 + f] * SP.warmFactor;
-      else continue;
-      FR[19] = fa;
-      applyImpulse(A, scl(ld3(FR), im0), ld3(FR + 9), fa);
-      applyImpulse(B, scl(ld3(FR), im1), ld3(FR + 12), -fa);
+    if (SP.useLearned) {
+      F1[19] = learned[3 * slot + 1];
+      F2[19] = learned[3 * slot + 2];
+    } else if (SP.warm) {
+      F1[19] = P[19] * SP.warmFactor;
+      F2[19] = P[20] * SP.warmFactor;
     }
   }
-  R[4 * CW + 0] = __builtin_bit_cast(float, mask);
-  R[4 * CW + 1] = __builtin_bit_cast(float, b0);
-  R[4 * CW + 2] = __builtin_bit_cast(float, b1);
-  if (A.invMass != 0.0f) storeS(svel, b0, A);
-  if (B.invMass != 0.0f) storeS(svel, b1, B);
+  hdr[p] = make_int4(mask, b0, b1, pi);
+}
+
+// warm-start impulses of colour c in row order: per point normal, lateral 1, lateral 2 (:659-747)
+__global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, const int4* __restrict__ hdr,
+                                                 const float* __restrict__ nrow, const float* __restrict__ frow,
+                                                 const float* __restrict__ props, float* svel) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt || !apply) return;
+  int p = off + k;
+  int4 h = hdr[p];
+  SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+  const float* N0 = nrow + (size_t)NW * p;
+  const float* F0 = frow + (size_t)FW * p;
+  for (int j = 0; j < 4; j++) {
+    if (!((h.x >> j) & 1)) continue;
+    for (int r = 0; r < 3; r++) {
+      const float* R = r == 0 ? N0 + RW * j : F0 + RW * (2 * j + r - 1);
+      V3 nrm = ld3(R);
+      float ap = R[19];
+      applyImpulse(A, scl(nrm, A.invMass), ld3(R + 9), ap);
+      applyImpulse(B, scl(nrm, B.invMass), ld3(R + 12), -ap);
+    }
+  }
+  if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+  if (B.invMass != 0.0f) storeBody(svel, h.z, B);
 }
 
 __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
@@ -318,89 +344,110 @@ __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
   }
 }
 
-// normal rows of colour c: resolveSingleCollisionCombinedCacheFriendly (:234-284)
-__global__ void kNormal(int cnt, const int* __restrict__ corder, SolverParams SP, float* rows,
-                        const float* __restrict__ props, float* svel, double* res, const int* done) {
-  if (*done) return;
+// normal rows of one colour: resolveSingleCollisionCombinedCacheFriendly (:234-284)
+__global__ __launch_bounds__(64) void kNormal(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
+                                              float* nrow, const float* __restrict__ props, float* svel, double* res,
+                                              const int* __restrict__ done) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
-  if (k < cnt) {
-    float* R = rows + (size_t)MW * corder[k];
-    int mask = __builtin_bit_cast(int, R[4 * CW]);
-    int b0 = __builtin_bit_cast(int, R[4 * CW + 1]), b1 = __builtin_bit_cast(int, R[4 * CW + 2]);
-    SBody A = loadS(svel, props, b0), B = loadS(svel, props, b1);
+  const int stop = *done;
+  if (k < cnt && !stop) {
+    const int p = off + k;
+    const int4 h = hdr[p];
+    float r[NW];
+    const float4* src = (const float4*)(nrow + (size_t)NW * p);
+#pragma unroll
+    for (int q = 0; q < NW / 4; q++) {
+      float4 v = src[q];
+      r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+    }
+    SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+#pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (!(mask & (1 << j))) continue;
-      float* N = R + CW * j;
-      V3 nrm = ld3(N);
-      float v1 = dot(nrm, A.lv) + dot(ld3(N + 3), A.av);
-      float v2 = dot(nrm, B.lv) + dot(ld3(N + 6), B.av);
+      if (!((h.x >> j) & 1)) continue;
+      const float* N = r + RW * j;
+      V3 nrm = v3(N[0], N[1], N[2]);
+      float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+      float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
       float rel_vel = v1 - v2;
       float positionalError = -N[16] * SP.erp / SP.dt;
       float velocityError = N[17] - rel_vel;
-      float pi = positionalError * N[15];
-      float vi = velocityError * N[15];
-      float ni = pi + vi;
+      float pimp = positionalError * N[15];
+      float vimp = velocityError * N[15];
+      float ni = pimp + vimp;
       float old = N[19];
       float sum = old + ni;
       float nw = 0.0f > sum ? 0.0f : sum;
-      N[19] = nw;
       ni = nw - old;
-      applyImpulse(A, scl(nrm, A.invMass), ld3(N + 9), ni);
-      applyImpulse(B, scl(nrm, B.invMass), ld3(N + 12), -ni);
+      applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), ni);
+      applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -ni);
+      nrow[(size_t)NW * p + RW * j + 19] = nw;
       double d = (double)nw - (double)old;
       dd += d * d;
       nn += (double)nw * (double)nw;
     }
-    if (A.invMass != 0.0f) storeS(svel, b0, A);
-    if (B.invMass != 0.0f) storeS(svel, b1, B);
+    if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+    if (B.invMass != 0.0f) storeBody(svel, h.z, B);
   }
   accumulate(dd, nn, res);
 }
 
-// friction rows of colour c: resolveSingleFrictionCacheFriendly (:286-346)
-__global__ void kFriction(int cnt, const int* __restrict__ corder, SolverParams SP, float* rows,
-                          const float* __restrict__ props, float* svel, double* res, const int* done) {
-  if (*done) return;
+// friction rows of one colour: resolveSingleFrictionCacheFriendly (:286-346)
+__global__ __launch_bounds__(64) void kFriction(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
+                                                const float* __restrict__ nrow, float* frow,
+                                                const float* __restrict__ props, float* svel, double* res,
+                                                const int* __restrict__ done) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
-  if (k < cnt) {
-    float* R = rows + (size_t)MW * corder[k];
-    int mask = __builtin_bit_cast(int, R[4 * CW]);
-    int b0 = __builtin_bit_cast(int, R[4 * CW + 1]), b1 = __builtin_bit_cast(int, R[4 * CW + 2]);
-    SBody A = loadS(svel, props, b0), B = loadS(svel, props, b1);
+  const int stop = *done;
+  if (k < cnt && !stop) {
+    const int p = off + k;
+    const int4 h = hdr[p];
+    float r[FW];
+    const float4* src = (const float4*)(frow + (size_t)FW * p);
+#pragma unroll
+    for (int q = 0; q < FW / 4; q++) {
+      float4 v = src[q];
+      r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+    }
+    float napp[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) napp[j] = nrow[(size_t)NW * p + RW * j + 19];
+    SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+#pragma unroll
     for (int j = 0; j < 4; j++) {
-      if (!(mask & (1 << j))) continue;
-      float total = R[CW * j + 19];  // appliedImpulse of this contact's normal row (+ push impulse 0)
-      total = total + 0.0f;
-      for (int f = 1; f <= 2; f++) {
-        float* N = R + CW * j + RW * f;
+      if (!((h.x >> j) & 1)) continue;
+      float total = napp[j] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
+#pragma unroll
+      for (int f = 0; f < 2; f++) {
+        const float* N = r + RW * (2 * j + f);
         float old = N[19];
+        float ap = old;
         float limit = total * N[18];
         if (total > 0.0f) {
-          V3 nrm = ld3(N);
-          float v1 = dot(nrm, A.lv) + dot(ld3(N + 3), A.av);
-          float v2 = dot(nrm, B.lv) + dot(ld3(N + 6), B.av);
+          V3 nrm = v3(N[0], N[1], N[2]);
+          float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+          float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
           float rel_vel = v1 - v2;
           float j1 = -rel_vel * N[15];
-          float ap = old + j1;
+          ap = old + j1;
           if (limit < ap) {
             ap = limit;
           } else if (ap < -limit) {
             ap = -limit;
           }
-          N[19] = ap;
           j1 = ap - old;
-          applyImpulse(A, scl(nrm, A.invMass), ld3(N + 9), j1);
-          applyImpulse(B, scl(nrm, B.invMass), ld3(N + 12), -j1);
+          applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), j1);
+          applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -j1);
+          frow[(size_t)FW * p + RW * (2 * j + f) + 19] = ap;
         }
-        double d = (double)N[19] - (double)old;
+        double d = (double)ap - (double)old;
         dd += d * d;
-        nn += (double)N[19] * (double)N[19];
+        nn += (double)ap * (double)ap;
       }
     }
-    if (A.invMass != 0.0f) storeS(svel, b0, A);
-    if (B.invMass != 0.0f) storeS(svel, b1, B);
+    if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+    if (B.invMass != 0.0f) storeBody(svel, h.z, B);
   }
   accumulate(dd, nn, res);
 }
@@ -416,18 +463,18 @@ __global__ void kResidual(int it, float tol, double* res, int* done, float* out)
   if (tol > 0.0f && r <= tol) *done = 1;
 }
 
-__global__ void kWriteback(int n, const int* __restrict__ list, const float* __restrict__ rows, float* man) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const float* R = rows + (size_t)MW * k;
-  int mask = __builtin_bit_cast(int, R[4 * CW]);
-  float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[k];
+__global__ void kWriteback(int n, const int4* __restrict__ hdr, const float* __restrict__ nrow,
+                           const float* __restrict__ frow, float* man) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int4 h = hdr[p];
+  float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * h.w;
   for (int j = 0; j < 4; j++) {
-    if (!(mask & (1 << j))) continue;
+    if (!((h.x >> j) & 1)) continue;
     float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
-    P[18] = R[CW * j + 19];
-    P[19] = R[CW * j + RW + 19];
-    P[19] = R[CW * j + 2 * RW + 19];  // appliedImpulseLateral1 written twice; Lateral2 never stored
+    P[18] = nrow[(size_t)NW * p + RW * j + 19];
+    P[19] = frow[(size_t)FW * p + RW * (2 * j) + 19];
+    P[19] = frow[(size_t)FW * p + RW * (2 * j + 1) + 19];  // Lateral1 written twice; Lateral2 never stored
   }
 }
 
@@ -449,12 +496,14 @@ void solverSolve(Ctx& c, float dt) {
   const int64_t np = c.npairs;
   c.nActive = 0;
   c.ncolors = 0;
+  c.colorOff.assign(1, 0);
   c.st.rows = 0;
   c.st.colors = 0;
   c.st.iterations = 0;
   c.st.residual = 0.0f;
+  c.residHist.clear();
   if (np == 0) return;
-  const float* man = c.man[c.cur].p;
+  float* man = c.man[c.cur].p;
   // active manifolds (stable compaction keeps pair-key order)
   c.flags.ensure(np);
   c.mActive.ensure(np);
@@ -472,26 +521,27 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipStreamSynchronize(s));
   c.nActive = na;
   if (na == 0) return;
-  if (c.useLearned && c.learnedSlots < 4 * (int64_t)na)
+  if (c.useLearned && !c.mlpReady && c.learnedSlots < 4 * (int64_t)na)
     throw std::length_error("learned warm start has fewer slots than 4 x active manifolds");
   // colouring
   c.color.ensure(na);
   c.colorAlt.ensure(na);
   c.colorOrder.ensure(na);
   c.colorOrderAlt.ensure(na);
+  c.activeBodies.ensure(na);
   c.bodyPrio.ensure(c.nb);
   c.bodyMask.ensure(c.nb);
+  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   int round = 0;
   for (;;) {
-    JB_CHECK(hipMemsetAsync(c.counter.p + 1, 0, sizeof(unsigned long long), s));
-    for (int r = 0; r < 4; r++, round++) {
-      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.mActive.p, man, c.props.p,
+    for (int r = 0; r < 6; r++, round++) {
+      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.activeBodies.p, c.props.p,
                          c.color.p, c.bodyPrio.p);
-      if (r == 3) JB_CHECK(hipMemsetAsync(c.counter.p + 1, 0, sizeof(unsigned long long), s));
-      hipLaunchKernelGGL(kColorResolve, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.mActive.p, man, c.props.p,
+      if (r == 5) JB_CHECK(hipMemsetAsync(c.counter.p + 1, 0, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(kColorResolve, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.activeBodies.p, c.props.p,
                          c.color.p, c.bodyPrio.p, c.bodyMask.p, c.counter.p + 1);
     }
     unsigned long long rem = 0;
@@ -500,51 +550,55 @@ void solverSolve(Ctx& c, float dt) {
     if (rem == 0) break;
     if (round > 4000) throw std::runtime_error("colouring did not converge");
   }
-  // order by colour, stable in active index
-  int maxColor = 64 + round + 1;
-  std::vector<unsigned long long> counts(maxColor, 0);
-  DBuf<unsigned long long>& cc = c.bodyPrio;  // reuse as colour histogram
-  cc.ensure(std::max<int64_t>(c.nb, maxColor));
-  JB_CHECK(hipMemsetAsync(cc.p, 0, maxColor * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(kColorCount, dim3(gridFor(na)), dim3(256), 0, s, na, c.color.p, cc.p);
-  {
-    hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p);
-    JB_CHECK(hipMemcpyAsync(c.colorAlt.p, c.color.p, (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
-    int bitsC = 1;
-    while ((1 << bitsC) <= maxColor) bitsC++;
-    c.typeKeyAlt.ensure(na);
-    c.permAlt.ensure(na);
-    hipcub::DoubleBuffer<int> dk(c.colorAlt.p, c.typeKeyAlt.p);
-    hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
-    size_t tb2 = 0;
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, 0, bitsC, s));
-    c.cubTmp.ensure(tb2);
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, 0, bitsC, s));
-    if (dv.Current() != c.colorOrder.p)
-      JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
-  }
-  JB_CHECK(hipMemcpyAsync(counts.data(), cc.p, maxColor * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  // order by colour (stable in active index) and colour offsets
+  const int maxColor = 64 + round + 1;
+  int bitsC = 1;
+  while ((1 << bitsC) <= maxColor) bitsC++;
+  c.typeKeyAlt.ensure(na);
+  c.permAlt.ensure(na);
+  c.colorStart.ensure(maxColor);
+  hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p);
+  JB_CHECK(hipMemcpyAsync(c.colorAlt.p, c.color.p, (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
+  hipcub::DoubleBuffer<int> dk(c.colorAlt.p, c.typeKeyAlt.p);
+  hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
+  size_t tb2 = 0;
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, 0, bitsC, s));
+  c.cubTmp.ensure(tb2);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, 0, bitsC, s));
+  if (dv.Current() != c.colorOrder.p)
+    JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
+  JB_CHECK(hipMemsetAsync(c.colorStart.p, 0xff, maxColor * sizeof(int), s));
+  hipLaunchKernelGGL(kColorStarts, dim3(gridFor(na)), dim3(256), 0, s, na, dk.Current(), c.colorStart.p);
+  std::vector<int> starts(maxColor);
+  JB_CHECK(hipMemcpyAsync(starts.data(), c.colorStart.p, maxColor * sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
-  c.colorOff.assign(1, 0);
   for (int k = 0; k < maxColor; k++)
-    if (counts[k]) c.colorOff.push_back(c.colorOff.back() + (int64_t)counts[k]);
+    if (starts[k] > 0) c.colorOff.push_back(starts[k]);
+  c.colorOff.push_back(na);
   c.ncolors = (int)c.colorOff.size() - 1;
   c.st.colors = c.ncolors;
-  // setup
+  // setup: rows for every slot, then the warm-start impulses colour by colour
   tBegin(c, T_SOLVE_SETUP);
-  c.rows.ensure((size_t)MW * na);
+  c.hdr.ensure(na);
+  c.nrow.ensure((size_t)NW * na);
+  c.frow.ensure((size_t)FW * na);
   c.svel.ensure((size_t)8 * c.nb);
   JB_CHECK(hipMemcpyAsync(c.svel.p, c.vel.p, (size_t)8 * c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
-  SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, c.useLearned ? 1 : 0, c.dumpFeatures ? 1 : 0,
-                  c.tolerance};
-  if (c.dumpFeatures) c.features.ensure((size_t)16 * 4 * na);
+  const bool learnedHere = c.useLearned && c.mlpReady;
+  SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.mlpReady) ? 1 : 0,
+                  (c.dumpFeatures || learnedHere) ? 1 : 0, c.tolerance};
+  if (SP.dumpFeatures) c.features.ensure((size_t)16 * 4 * na);
+  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
+                     c.tr.p, c.vel.p, c.props.p, c.hdr.p, c.nrow.p, c.frow.p, SP.useLearned ? c.learned.p : nullptr,
+                     SP.dumpFeatures ? c.features.p : nullptr);
+  if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
+  const int apply = (c.warmstart || c.useLearned) ? 1 : 0;
   for (int k = 0; k < c.ncolors; k++) {
     int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-    hipLaunchKernelGGL(kSetup, dim3(gridFor(cnt, 128)), dim3(128), 0, s, cnt, c.colorOrder.p + c.colorOff[k],
-                       c.mActive.p, SP, c.man[c.cur].p, c.tr.p, c.vel.p, c.props.p, c.svel.p, c.rows.p,
-                       c.useLearned ? c.learned.p : nullptr, c.dumpFeatures ? c.features.p : nullptr);
+    hipLaunchKernelGGL(kWarmApply, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, apply, c.hdr.p,
+                       c.nrow.p, c.frow.p, c.props.p, c.svel.p);
   }
-  tEnd(c, T_SOLVE_SETUP, c.ncolors);
+  tEnd(c, T_SOLVE_SETUP, c.ncolors + 1);
   // iterations
   int iters = c.tolerance > 0.0f ? c.maxIterations : c.numIterations;
   c.resid.ensure(2 * (size_t)iters + 4);
@@ -552,23 +606,24 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemsetAsync(c.resid.p, 0, (2 * (size_t)iters + 4) * sizeof(double), s));
   JB_CHECK(hipMemsetAsync(c.done.p, 0, 4 * sizeof(int), s));
   float* resOut = (float*)(c.counter.p + 6);
+  JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
   tBegin(c, T_SOLVE_ITER);
   for (int itn = 0; itn < iters; itn++) {
     double* res = c.resid.p + 2 * itn;
     for (int k = 0; k < c.ncolors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      hipLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 128)), dim3(128), 0, s, cnt, c.colorOrder.p + c.colorOff[k], SP,
-                         c.rows.p, c.props.p, c.svel.p, res, c.done.p);
+      hipLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
+                         c.nrow.p, c.props.p, c.svel.p, res, c.done.p);
     }
     for (int k = 0; k < c.ncolors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      hipLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 128)), dim3(128), 0, s, cnt, c.colorOrder.p + c.colorOff[k],
-                         SP, c.rows.p, c.props.p, c.svel.p, res, c.done.p);
+      hipLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
+                         c.nrow.p, c.frow.p, c.props.p, c.svel.p, res, c.done.p);
     }
     hipLaunchKernelGGL(kResidual, dim3(1), dim3(1), 0, s, itn, c.tolerance, c.resid.p, c.done.p, resOut);
   }
   tEnd(c, T_SOLVE_ITER, (int64_t)iters * 2 * c.ncolors);
-  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, c.rows.p, c.man[c.cur].p);
+  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
   float ro[2] = {0, 0};
   std::vector<double> rh(2 * (size_t)iters);
@@ -577,12 +632,10 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipStreamSynchronize(s));
   c.st.residual = ro[0];
   c.st.iterations = (int64_t)ro[1];
-  c.residHist.clear();
   for (int k = 0; k < (int)c.st.iterations; k++) {
     double nrm = std::sqrt(rh[2 * k + 1]);
     c.residHist.push_back((float)(std::sqrt(rh[2 * k]) / (nrm > 1.0 ? nrm : 1.0)));
   }
-  c.st.rows = 0;
 }
 
 }  // namespace jbg
