@@ -1,0 +1,83 @@
+"""CPU checks of the C ABI (no GPU): the library loads, exports exactly what
+include/jbgpu.h declares, and its host-side record builders agree bit for bit
+with the oracle's.  No compute entry point is called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import jbgpu
+import pyoracle as po
+from jbgpu import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "jbgpu.h")).read()
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(jbgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_set():
+    assert declared_symbols() == sorted(jbgpu.SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(jbgpu.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    out = os.popen(f"nm -D --defined-only {jbgpu.LIB_PATH}").read()
+    exported = set(re.findall(r"\bT (jbgpu_\w+)", out))
+    assert exported == set(declared_symbols())
+
+
+def test_version_and_default_config():
+    L = jbgpu.load()
+    assert L.jbgpu_version() >= 1
+    cfg = jbgpu.default_config()
+    assert cfg.bp_mode == jbgpu.BP_DBVT and cfg.num_iterations == 10
+    assert cfg.solver_order == jbgpu.ORDER_COLOR and cfg.deactivation_disabled == 1
+    assert np.isclose(cfg.erp, 0.2) and np.isclose(cfg.warmstart_factor, 0.85)
+
+
+def test_argument_errors_are_status_codes():
+    rec = np.zeros(jbgpu.SHAPE_WORDS, np.float32)
+    L = jbgpu.load()
+    assert L.jbgpu_make_shape(99, np.zeros(4, np.float32), None, rec) == -1          # JBGPU_ERR_ARG
+    assert L.jbgpu_make_shape(3, np.float32([0, 0, 0, 0]), None, rec) == -1         # hull without points
+    buf = C.create_string_buffer(64)
+    assert L.jbgpu_last_error(None, buf, 64) >= 0
+
+
+def _shape_cases():
+    rng = np.random.default_rng(1)
+    pts = rng.normal(size=(20, 3)).astype(np.float32)
+    return [(0, [0.37], None), (1, [0.5, 0.25, 1.5], None), (2, [0.3, 1.2], None),
+            (3, [0, 20], pts), (4, [0.0, 1.0, 0.0, -2.0], None), (1, [0.1, 0.1, 0.1], None)]
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_make_shape_matches_oracle(case):
+    t, params, verts = _shape_cases()[case]
+    a = jbgpu.make_shape(t, params, verts)
+    b = po.make_shape(t, params, verts)
+    assert a.view(np.uint32).tolist() == b.view(np.uint32).tolist()
+    if t != 4:
+        ia = jbgpu.shape_inertia(a, 2.5, verts)
+        ib = po.shape_inertia(b, 2.5, verts)
+        assert ia.view(np.uint32).tolist() == ib.view(np.uint32).tolist()
+
+
+@pytest.mark.parametrize("gen", ["boxes_grid", "mixed_pile", "batched_towers", "pyramids"])
+def test_scene_records_match_oracle(gen):
+    kw = dict(boxes_grid=dict(nx=4, ny=3, nz=2), mixed_pile=dict(n=500, bin_half=8.0),
+              batched_towers=dict(n_worlds=8, lo=16, hi=40), pyramids=dict(n_boxes=300, n_pyramids=2, n_spheres=20,
+                                                                           area=30.0))[gen]
+    sc = getattr(scenes, gen)(seed=3, **kw)
+    s1, v1, b1 = scenes.build_records(sc, jbgpu)
+    s2, v2, b2 = scenes.build_records(sc, po)
+    for x, y in ((s1, s2), (v1, v2), (b1, b2)):
+        assert x.shape == y.shape and np.array_equal(x.view(np.uint32), y.view(np.uint32))

@@ -1,0 +1,550 @@
+"""CPU pins of the oracle (no GPU).
+
+The reference ships no tests or golden vectors, so each part of the oracle is
+pinned by something independent of its own code: published known answers
+(java.util.Random), closed forms (sphere/plane contacts, inertia of a cube,
+rotations of known axes, free fall), brute force (O(N^2) pair sets, polytope
+distances in float64) and invariants (pair-set symmetry and filtering,
+friction-cone complementarity, manifold capacity and deepest-point retention).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from jbgpu import scenes
+
+SPHERE, BOX, CAPSULE, HULL, PLANE = 0, 1, 2, 3, 4
+
+
+def tr(basis=None, origin=(0, 0, 0)):
+    b = np.eye(3, dtype=np.float32) if basis is None else np.asarray(basis, np.float32).reshape(3, 3)
+    return np.concatenate([b.reshape(-1), np.asarray(origin, np.float32)]).astype(np.float32)
+
+
+def rot(axis, ang):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def random_rot(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+# ------------------------------------------------------------------ java.util.Random
+def test_java_random_known_answers():
+    # values published for the Java SE LCG: new Random(0).nextInt(), new Random(42).nextInt()
+    assert po.random_ints(0, 1)[0] == -1155484576
+    assert po.random_ints(42, 1)[0] == -1170105035
+    # the solver's shuffle draws nextInt(j + 1), j = 0..9, from one Random(0) stream
+    assert list(po.random_bounded(0, list(range(1, 11)))) == [0, 1, 1, 2, 0, 5, 2, 0, 6, 4]
+
+
+def test_java_random_bounds():
+    b = np.arange(1, 2000)
+    v = po.random_bounded(7, b)
+    assert (v >= 0).all() and (v < b).all()
+    assert (po.random_bounded(7, [1] * 50) == 0).all()
+
+
+# ------------------------------------------------------------------ libGDX rotation semantics
+def test_quaternion_matrix_round_trip():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        q = rng.normal(size=4).astype(np.float32)
+        q /= np.linalg.norm(q)
+        m = np.zeros(9, np.float32)
+        po.lib().orc_matrix_from_quat(q, m)
+        q2 = np.zeros(4, np.float32)
+        po.lib().orc_quat_from_matrix(m, q2)
+        assert min(np.abs(q2 - q).max(), np.abs(q2 + q).max()) < 2e-6
+        M = m.reshape(3, 3).astype(np.float64)
+        assert np.abs(M @ M.T - np.eye(3)).max() < 1e-6
+
+
+def test_matrix_of_known_rotation():
+    # quarter turn about z: (x, y, z, w) = (0, 0, sin 45, cos 45) maps x to y
+    s = np.float32(np.sqrt(0.5))
+    m = np.zeros(9, np.float32)
+    po.lib().orc_matrix_from_quat(np.array([0, 0, s, s], np.float32), m)
+    assert np.abs(m.reshape(3, 3) @ np.array([1, 0, 0]) - np.array([0, 1, 0])).max() < 1e-6
+
+
+# ------------------------------------------------------------------ shapes
+def test_box_record_support_and_inertia():
+    rec = po.make_shape(BOX, [0.5, 0.5, 0.5])
+    assert rec[1] == np.float32(0.04) and np.allclose(rec[2:5], 0.46)
+    # support with margin is the full box; -0.0 picks the positive side (fsel: a >= 0)
+    assert np.array_equal(po.support(rec, [1, -1, -0.0], True), np.float32([0.5, -0.5, 0.5]))
+    assert np.array_equal(po.support(rec, [-1, 1, 2], False), np.float32([-0.46, 0.46, 0.46]))
+    # a cube of side 1 and mass 6 has I = m s^2 / 6 = 1 on every axis
+    assert np.allclose(po.shape_inertia(rec, 6.0), 1.0, atol=1e-6)
+
+
+def test_sphere_record_aabb_inertia():
+    rec = po.make_shape(SPHERE, [0.7])
+    assert rec[1] == np.float32(0.7)
+    a = po.aabb(rec, tr(origin=(1, 2, 3)))
+    assert np.allclose(a, [0.3, 1.3, 2.3, 1.7, 2.7, 3.7])
+    assert np.allclose(po.shape_inertia(rec, 2.0), 0.4 * 2 * 0.49, rtol=1e-6)  # 2/5 m r^2
+
+
+def test_rotated_box_aabb_closed_form():
+    rec = po.make_shape(BOX, [1.0, 0.25, 0.5])
+    ang = 0.6
+    a = po.aabb(rec, tr(rot([0, 1, 0], ang), (0, 0, 0)))
+    ex = 1.0 * np.cos(ang) + 0.5 * np.sin(ang)
+    ez = 1.0 * np.sin(ang) + 0.5 * np.cos(ang)
+    assert np.allclose(a[3:], [ex, 0.25, ez], atol=1e-6) and np.allclose(a[:3], -a[3:])
+
+
+def test_hull_support_is_first_maximum():
+    rng = np.random.default_rng(3)
+    pts = rng.normal(size=(24, 3)).astype(np.float32)
+    rec = po.make_shape(HULL, [0, 24], pts)
+    for _ in range(100):
+        d = rng.normal(size=3).astype(np.float32)
+        dn = d / np.float32(np.sqrt(np.float32(d @ d)))
+        dots = pts.astype(np.float64) @ dn.astype(np.float64)
+        want = pts[int(np.argmax(dots))]
+        got = po.support(rec, d, False, pts)
+        assert np.array_equal(got, want)
+    # local AABB = support along the axes +- margin
+    assert np.allclose(rec[10:13], pts.max(0) + 0.04) and np.allclose(rec[7:10], pts.min(0) - 0.04)
+
+
+def test_capsule_support_closed_form():
+    rec = po.make_shape(CAPSULE, [0.3, 1.0])   # radius 0.3, cylinder height 1 (half 0.5)
+    for d in ([0, 1, 0], [0, -1, 0], [1, 0.2, 0], [0.3, -0.8, 0.5]):
+        d = np.asarray(d, np.float64)
+        dn = d / np.linalg.norm(d)
+        cap = np.array([0, 0.5 if d[1] >= 0 else -0.5, 0])
+        want = cap + dn * 0.3  # full radius once the margin is added back
+        got = po.support(rec, d, True)
+        assert np.abs(got - want).max() < 1e-6
+
+
+# ------------------------------------------------------------------ narrowphase closed forms
+def test_sphere_sphere_contact():
+    a = po.make_shape(SPHERE, [0.5])
+    b = po.make_shape(SPHERE, [0.4])
+    out = po.sphere_sphere(a, tr(origin=(0.8, 0, 0)), b, tr(origin=(0, 0, 0)))
+    assert out[0] == 1 and np.allclose(out[1:4], [1, 0, 0])
+    assert np.allclose(out[4:7], [0.4, 0, 0]) and np.isclose(out[7], 0.8 - 0.9)
+    far = po.sphere_sphere(a, tr(origin=(2, 0, 0)), b, tr())
+    assert far[0] == 0
+
+
+def test_box_on_plane_contact():
+    box = po.make_shape(BOX, [0.5, 0.5, 0.5])
+    plane = po.make_shape(PLANE, [0, 1, 0, 0])
+    out = po.convex_plane(box, tr(origin=(1, 0.49, 2)), plane, tr())
+    assert out[0] == 1 and np.allclose(out[1:4], [0, 1, 0])
+    assert np.isclose(out[7], -0.01, atol=1e-6)
+    assert np.isclose(out[5], 0.0, atol=1e-6)  # projected onto the plane
+    assert convex_plane_far(box, plane)
+
+
+def convex_plane_far(box, plane):
+    return po.convex_plane(box, tr(origin=(0, 0.6, 0)), plane, tr())[0] == 0
+
+
+def _box_vertices(h):
+    return np.array(list(itertools.product(*[(-x, x) for x in h])), np.float64)
+
+
+def _seg_seg(p1, q1, p2, q2):
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    c, b = d1 @ r, d1 @ d2
+    den = a * e - b * b
+    s = np.clip((b * f - c * e) / den, 0, 1) if den > 1e-15 else 0.0
+    t = (b * s + f) / e
+    if t < 0:
+        t, s = 0.0, np.clip(-c / a, 0, 1)
+    elif t > 1:
+        t, s = 1.0, np.clip((b - c) / a, 0, 1)
+    return np.linalg.norm((p1 + d1 * s) - (p2 + d2 * t))
+
+
+def box_box_distance(ha, Ra, ta, hb, Rb, tb):
+    """Exact distance between two separated boxes (vertex-box and edge-edge features), float64."""
+    va = _box_vertices(ha) @ Ra.T + ta
+    vb = _box_vertices(hb) @ Rb.T + tb
+    best = np.inf
+    for (verts, h, R, t) in ((va, hb, Rb, tb), (vb, ha, Ra, ta)):
+        loc = (verts - t) @ R
+        clamped = np.clip(loc, -h, h)
+        best = min(best, np.linalg.norm(loc - clamped, axis=1).min())
+    ea = [(i, j) for i in range(8) for j in range(i + 1, 8) if bin(i ^ j).count("1") == 1]
+    for i, j in ea:
+        for k, l in ea:
+            best = min(best, _seg_seg(va[i], va[j], vb[k], vb[l]))
+    return best
+
+
+def test_gjk_distance_against_brute_force():
+    rng = np.random.default_rng(11)
+    checked = 0
+    for _ in range(60):
+        ha = rng.uniform(0.2, 1.0, 3)
+        hb = rng.uniform(0.2, 1.0, 3)
+        Ra, Rb = random_rot(rng), random_rot(rng)
+        ta = np.zeros(3)
+        d = rng.normal(size=3)
+        tb = d / np.linalg.norm(d) * (np.linalg.norm(ha) + np.linalg.norm(hb) + rng.uniform(0.05, 1.0))
+        recA = po.make_shape(BOX, ha)
+        recB = po.make_shape(BOX, hb)
+        core_a, core_b = recA[2:5].astype(np.float64), recB[2:5].astype(np.float64)
+        TA = tr(Ra, ta)
+        TB = tr(Rb, tb)
+        out = po.gjk(recA, TA, recB, TB, 1e30)
+        want = box_box_distance(core_a, TA[:9].reshape(3, 3).astype(np.float64), TA[9:].astype(np.float64),
+                                core_b, TB[:9].reshape(3, 3).astype(np.float64), TB[9:].astype(np.float64)) - 0.08
+        assert out[0] == 1
+        assert abs(out[7] - want) < 2e-4 * max(1.0, want), (out[7], want)
+        assert abs(np.linalg.norm(out[1:4]) - 1) < 1e-5
+        checked += 1
+    assert checked == 60
+
+
+def test_gjk_sphere_box_distance():
+    rng = np.random.default_rng(5)
+    box = po.make_shape(BOX, [0.7, 0.4, 0.5])
+    sph = po.make_shape(SPHERE, [0.3])
+    for _ in range(40):
+        R = random_rot(rng)
+        c = rng.normal(size=3)
+        c = c / np.linalg.norm(c) * rng.uniform(1.5, 3.0)
+        TB = tr(R, (0, 0, 0))
+        Rf = TB[:9].reshape(3, 3).astype(np.float64)
+        loc = Rf.T @ c
+        core = box[2:5].astype(np.float64)
+        want = np.linalg.norm(loc - np.clip(loc, -core, core)) - 0.04 - 0.3
+        out = po.gjk(sph, tr(origin=c), box, TB, 1e30)
+        assert out[0] == 1 and abs(out[7] - want) < 2e-4, (out[7], want)
+
+
+def test_epa_depth_axis_aligned_boxes():
+    for (ha, hb, off, axis) in [((1, 1, 1), (0.5, 0.5, 0.5), (1.3, 0.1, 0.05), 0),
+                                ((0.6, 0.3, 0.8), (0.6, 0.3, 0.8), (0.2, 0.5, 0.1), 1),
+                                ((2, 1, 0.4), (0.5, 0.5, 0.5), (0.3, 0.2, 0.75), 2)]:
+        recA = po.make_shape(BOX, ha)
+        recB = po.make_shape(BOX, hb)
+        out = po.epa(recA, tr(), recB, tr(origin=off))
+        overlap = np.array(ha) + np.array(hb) - np.abs(np.array(off))
+        assert out[0] == 1
+        depth = np.linalg.norm(out[1:4] - out[4:7])
+        assert abs(depth - overlap.min()) < 2e-3, (depth, overlap)
+        n = (out[4:7] - out[1:4]) / depth
+        assert abs(abs(n[axis]) - 1) < 1e-3
+
+
+def test_epa_sphere_depth():
+    a = po.make_shape(SPHERE, [0.5])
+    b = po.make_shape(SPHERE, [0.4])
+    out = po.epa(a, tr(), b, tr(origin=(0.6, 0.2, -0.1)))
+    d = np.linalg.norm([0.6, 0.2, -0.1])
+    assert out[0] == 1 and abs(np.linalg.norm(out[1:4] - out[4:7]) - (0.9 - d)) < 2e-3
+
+
+# ------------------------------------------------------------------ persistent manifold
+def _cand(n, p, depth):
+    return np.array([1, *n, *p, depth], np.float32)
+
+
+def _manifold():
+    m = np.zeros(po.MANIFOLD_WORDS, np.float32)
+    m[1:3] = np.array([0, 1], np.int32).view(np.float32)
+    return m
+
+
+def test_manifold_replace_keeps_impulse_and_lifetime():
+    I = tr()
+    fr = [0.5, 0.5, 0.0, 0.0]
+    m = po.manifold_add(_manifold(), I, I, fr, _cand([0, 1, 0], [0, 0, 0], -0.01), refresh=False)
+    assert m[0].view(np.int32) == 1
+    m[4 + 18] = 3.5  # cached impulse
+    m = po.manifold_add(m, I, I, fr, _cand([0, 1, 0], [0.005, 0, 0], -0.012), refresh=True)
+    assert m[0].view(np.int32) == 1 and m[4 + 18] == np.float32(3.5)
+    assert np.isclose(m[4 + 15], -0.012) and m[4 + 28].view(np.int32) == 1
+    assert np.isclose(m[4 + 16], 0.25)  # combined friction 0.5 * 0.5
+
+
+def test_manifold_capacity_and_deepest_point():
+    I = tr()
+    fr = [0.5, 0.5, 0.0, 0.0]
+    m = _manifold()
+    corners = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)]
+    depths = [-0.01, -0.015, -0.005, -0.018]
+    for p, d in zip(corners, depths):
+        m = po.manifold_add(m, I, I, fr, _cand([0, 1, 0], p, d), refresh=False)
+    assert m[0].view(np.int32) == 4
+    m = po.manifold_add(m, I, I, fr, _cand([0, 1, 0], (0.5, 0, 0.5), -0.001), refresh=False)
+    assert m[0].view(np.int32) == 4
+    pts = m[4:].reshape(4, 32)
+    assert np.isclose(pts[:, 15].min(), -0.018)  # the deepest point survives the eviction
+
+
+def test_manifold_refresh_drops_separated_and_drifted():
+    I = tr()
+    fr = [0.5, 0.5, 0.0, 0.0]
+    m = po.manifold_add(_manifold(), I, I, fr, _cand([0, 1, 0], (0, 0, 0), -0.01), refresh=False)
+    m = po.manifold_add(m, I, I, fr, _cand([0, 1, 0], (1, 0, 0), -0.01), refresh=False)
+    assert m[0].view(np.int32) == 2
+    # body A moves up 5 cm: both points separate beyond the 2 cm breaking threshold
+    out = po.manifold_add(m, tr(origin=(0, 0.05, 0)), I, fr, np.zeros(8, np.float32), refresh=True)
+    assert out[0].view(np.int32) == 0
+    # body A slides 3 cm sideways: tangential drift beyond 2 cm
+    out = po.manifold_add(m, tr(origin=(0.03, 0, 0)), I, fr, np.zeros(8, np.float32), refresh=True)
+    assert out[0].view(np.int32) == 0
+    out = po.manifold_add(m, tr(origin=(0.005, 0.004, 0)), I, fr, np.zeros(8, np.float32), refresh=True)
+    assert out[0].view(np.int32) == 2
+
+
+# ------------------------------------------------------------------ broadphase
+def _boxes(rng, n, extent=20.0, size=1.0):
+    c = rng.uniform(0, extent, (n, 3))
+    h = rng.uniform(0.05, size, (n, 3))
+    return np.concatenate([c - h, c + h], 1).astype(np.float32)
+
+
+def _brute_pairs(boxes):
+    n = len(boxes)
+    mn, mx = boxes[:, :3], boxes[:, 3:]
+    out = []
+    for i in range(n):
+        ok = (mn[i] <= mx[i + 1:]).all(1) & (mx[i] >= mn[i + 1:]).all(1)
+        out += [((i + 1) << 32) | (j + i + 2) for j in np.nonzero(ok)[0]]
+    return np.array(sorted(out), np.uint64)
+
+
+@pytest.mark.parametrize("mode", [0, 3])
+def test_pair_set_equals_brute_force(mode):
+    rng = np.random.default_rng(4)
+    boxes = _boxes(rng, 1500)
+    boxes = np.concatenate([boxes, np.array([[-1e30] * 3 + [1e30] * 3], np.float32)])  # a plane's AABB
+    bp = po.Broadphase(mode)
+    bp.create(boxes)
+    bp.set_all(boxes)
+    got = bp.step()
+    if mode == 0:  # the stored (possibly fattened) boxes are what the set is computed on
+        stored = bp.proxies()[:, 7:13]
+        want = _brute_pairs(stored)
+    else:
+        want = _brute_pairs(boxes)
+    assert np.array_equal(got, want)
+    assert np.array_equal(got, bp.step(brute=True))
+
+
+def test_pair_filter_groups():
+    rng = np.random.default_rng(9)
+    boxes = _boxes(rng, 400, extent=5.0)
+    group = np.where(np.arange(400) % 3 == 0, 2, 1).astype(np.int16)    # every third proxy static
+    mask = np.where(group == 2, np.int16(-1 ^ 2), np.int16(-1)).astype(np.int16)
+    bp = po.Broadphase(3)
+    bp.create(boxes, group, mask)
+    bp.set_all(boxes)
+    got = bp.step()
+    lo = (got >> np.uint64(32)).astype(np.int64) - 1
+    hi = (got & np.uint64(0xffffffff)).astype(np.int64) - 1
+    assert (lo < hi).all()
+    assert not ((group[lo] == 2) & (group[hi] == 2)).any()  # static-static never pairs
+    allp = _brute_pairs(boxes)
+    a = (allp >> np.uint64(32)).astype(np.int64) - 1
+    b = (allp & np.uint64(0xffffffff)).astype(np.int64) - 1
+    keep = ~((group[a] == 2) & (group[b] == 2))
+    assert np.array_equal(got, allp[keep])
+
+
+def test_dbvt_fattening_state_machine():
+    box = np.array([[0, 0, 0, 1, 1, 1]], np.float32)
+    bp = po.Broadphase(0)
+    bp.create(box)
+    moved = box + np.float32([0.1, 0, -0.02, 0.1, 0, -0.02])
+    bp.set_all(moved)
+    p = bp.proxies()[0]
+    # not contained in the old leaf: expand by 0.05, then by 2*delta on the side of motion
+    want = moved[0].copy()
+    want[:3] -= np.float32(0.05)
+    want[3:] += np.float32(0.05)
+    delta = np.float32(2.0) * ((moved[0, :3] + moved[0, 3:]) * np.float32(0.5) - (box[0, :3] + box[0, 3:]) *
+                               np.float32(0.5))
+    for k in range(3):
+        if delta[k] > 0:
+            want[3 + k] += delta[k]
+        else:
+            want[k] += delta[k]
+    assert np.array_equal(p[7:13], want) and np.array_equal(p[1:7], want)
+    # a small move inside the fattened leaf keeps the leaf and stores the raw box
+    small = moved + np.float32([0.01, 0, 0, 0.01, 0, 0])
+    bp.set_all(small)
+    p2 = bp.proxies()[0]
+    assert np.array_equal(p2[1:7], want) and np.array_equal(p2[7:13], small[0])
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_axis_sweep_quantised_overlap(mode):
+    rng = np.random.default_rng(6)
+    wmin, wmax = np.float32([-10, -10, -10]), np.float32([30, 30, 30])
+    boxes = _boxes(rng, 800)
+    bp = po.Broadphase(mode, wmin, wmax)
+    bp.create(boxes)
+    bp.set_all(boxes)
+    got = bp.step()
+    q = bp.quantized().astype(np.int64)
+    mn, mx = q[:, :3], q[:, 3:]
+    assert (mn % 2 == 0).all() and (mx % 2 == 1).all()  # min edges even, max edges odd
+    want = []
+    for i in range(len(q)):
+        ok = (mn[i] < mx[i + 1:]).all(1) & (mx[i] > mn[i + 1:]).all(1)
+        want += [((i + 1) << 32) | (j + i + 2) for j in np.nonzero(ok)[0]]
+    assert np.array_equal(got, np.array(sorted(want), np.uint64))
+    # the quantised set contains the float set
+    fl = _brute_pairs(boxes)
+    assert np.isin(fl, got).all()
+    # the world corners quantise to the edge values
+    lim = 0xfffe if mode == 1 else 0x7ffffffe
+    assert q[:, :3].min() >= 0 and q[:, 3:].max() <= lim + 1
+
+
+# ------------------------------------------------------------------ solver and integrator
+def _single_world(bodies_spec, iterations=10, order=0, **kw):
+    b = scenes._Builder("t")
+    pl = b.shape(PLANE, [0, 1, 0, 0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    for (stype, params, pos, mass) in bodies_spec:
+        s = b.shape(stype, params)
+        b.body(s, scenes.IDENT, np.asarray(pos, np.float32), mass, **kw)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    return po.World(shapes, verts, bodies, iterations=iterations, order=order)
+
+
+def test_free_fall_is_semi_implicit_euler():
+    w = _single_world([(SPHERE, [0.5], (0, 100, 0), 2.0)])
+    dt = np.float32(1 / 60)
+    v = np.float32(0)
+    y = np.float32(100)
+    g = np.float32(-9.81)
+    for k in range(60):
+        w.step()
+        v = v + np.float32(dt * g)  # (invMass * dt) * (g / invMass) == dt * g for mass 2
+        y = y + dt * v
+        b = w.bodies()[1]
+        assert b[13] == v and b[10] == y, k
+
+
+def test_constant_spin_matches_analytic_rotation():
+    w = _single_world([(BOX, [0.5, 0.2, 0.3], (0, 50, 0), 1.0)])
+    b = w.bodies()
+    b[1, 15:18] = [0.0, 1.0, 0.0]
+    b[1, 22:25] = 0.0  # no gravity
+    w.set_bodies(b)
+    for k in range(1, 121):
+        w.step()
+    R = w.bodies()[1, :9].reshape(3, 3).astype(np.float64)
+    assert np.abs(R - rot([0, 1, 0], 120 / 60.0)).max() < 2e-5
+
+
+def test_resting_box_normal_impulse_balances_gravity():
+    w = _single_world([(BOX, [0.5, 0.5, 0.5], (0, 0.5, 0), 3.0)])
+    w.step(n=240)
+    b = w.bodies()[1]
+    assert np.abs(b[12:18]).max() < 2e-2
+    keys, man = w.pairs()
+    m = man[0]
+    n = m[0].view(np.int32)
+    total = m[4:].reshape(4, 32)[:n, 18].sum()
+    assert abs(total - 3.0 * 9.81 / 60) < 0.05 * 3.0 * 9.81 / 60
+    pts = m[4:].reshape(4, 32)[:n]
+    assert (np.abs(pts[:, 19]) <= pts[:, 16] * pts[:, 18] + 1e-6).all()  # friction cone
+
+
+def test_head_on_collision_without_restitution():
+    b = scenes._Builder("t")
+    s = b.shape(SPHERE, [0.5])
+    b.body(s, scenes.IDENT, np.array([-1.0, 5, 0], np.float32), 1.0)
+    b.body(s, scenes.IDENT, np.array([1.0, 5, 0], np.float32), 1.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    bodies[:, 22:25] = 0.0
+    bodies[0, 12] = 3.0
+    bodies[1, 12] = -3.0
+    w = po.World(shapes, verts, bodies, iterations=10)
+    w.step(n=40)
+    bb = w.bodies()
+    assert abs(bb[0, 12] - bb[1, 12]) < 1e-3  # relative normal velocity removed, no bounce
+    assert abs(bb[0, 12] + bb[1, 12]) < 1e-5  # momentum conserved
+
+
+def test_reference_order_residual_decreases():
+    sc = scenes.boxes_grid(seed=2, nx=4, ny=4, nz=4, base=0.02, spacing=1.0001)
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=60, order=0)
+    w.step(n=30)
+    h = w.residual_history()
+    assert len(h) > 0 and h[-1] < h[0]
+
+
+def test_oracle_world_runs_clean():
+    sc = scenes.mixed_pile(seed=2, n=400, bin_half=6.0)
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=10)
+    w.step(n=60)
+    b = w.bodies()
+    assert np.isfinite(b).all()
+
+
+def _colour_order(keys, man, bodies, reverse_within=False):
+    """Greedy colouring of the manifolds on their dynamic bodies; manifolds listed colour by colour."""
+    static = (bodies[:, 28].view(np.int32) & 1) != 0
+    colours = []
+    used = []
+    for i in range(len(keys)):
+        if man[i, 0].view(np.int32) == 0:
+            continue
+        bs = [b for b in man[i, 1:3].view(np.int32) if not static[b]]
+        c = 0
+        while c < len(used) and any(b in used[c] for b in bs):
+            c += 1
+        if c == len(used):
+            used.append(set())
+            colours.append([])
+        used[c].update(bs)
+        colours[c].append(i)
+    if reverse_within:
+        colours = [c[::-1] for c in colours]
+    return [i for c in colours for i in c], len(colours)
+
+
+def test_permutation_within_a_colour_is_exact():
+    """Rows of one colour share no dynamic body, so their order cannot change a single bit."""
+    sc = scenes.boxes_grid(seed=5, nx=3, ny=4, nz=3, base=0.5, spacing=1.0)
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    worlds = [po.World(shapes, verts, bodies, iterations=10, order=2) for _ in range(2)]
+    probe = po.World(shapes, verts, bodies, iterations=10, order=2)
+    ncol_max = 0
+    for step in range(40):
+        b, (k, m) = worlds[0].bodies(), worlds[0].pairs()
+        prox, stage = worlds[0].proxies()
+        probe.set_bodies(b)
+        probe.set_pairs(k, m)
+        probe.set_proxies(prox, stage)
+        probe.step_until(2)
+        keys, man = probe.pairs()
+        for w, rev in zip(worlds, (False, True)):
+            order, ncol = _colour_order(keys, man, b, rev)
+            w.set_order(order)
+            w.step()
+        ncol_max = max(ncol_max, ncol)
+        assert np.array_equal(worlds[0].bodies().view(np.uint32), worlds[1].bodies().view(np.uint32)), step
+    assert ncol_max >= 2
