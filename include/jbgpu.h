@@ -132,15 +132,45 @@ int jbgpu_solver_set_warmstart(jbgpu_ctx* ctx, const float* init, int64_t nslots
  * depth, inverse masses, cached impulses, friction) of the last solve, by contact slot. */
 int jbgpu_solver_enable_features(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_solver_get_features(jbgpu_ctx* ctx, float* out, int64_t nslots);
+/* Final (normal, lateral1, lateral2) impulses of the last solve by contact slot, 3 floats per
+ * slot, zeros for unused slots.  The manifold keeps only part of them: the reference stores the
+ * second lateral impulse into appliedImpulseLateral1 and never sets appliedImpulseLateral2
+ * (dynamics/constraintsolver/SequentialImpulseConstraintSolver.java:908-909). */
+int jbgpu_solver_get_impulses(jbgpu_ctx* ctx, float* out, int64_t nslots);
+/* tolerance > 0: iterate until the residual ||dlambda|| / max(1, ||lambda||) <= tolerance or
+ * max_iterations; tolerance 0: exactly num_iterations (ContactSolverInfo.numIterations). */
 int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations);
 /* Residual after each iteration of the last solve. */
 int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32_t* n);
+
+/* ---- learned warm start ------------------------------------------------------------------ */
+/* A 16-64-64-64-3 MLP (ReLU) maps the 16 contact features above to the initial (normal,
+ * lateral1, lateral2) impulses of that contact; bf16 MFMA operands, fp32 accumulation.  No
+ * counterpart exists in the reference: it replaces the constant warm-start factor applied in
+ * SequentialImpulseConstraintSolver.solveGroupCacheFriendlySetup (dynamics/constraintsolver/
+ * SequentialImpulseConstraintSolver.java:659-747).
+ * params: JBGPU_MLP_PARAMS floats, W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+ * W4[3][64] b4[3]; norm: mean_in[16] inv_std_in[16] std_out[3] mean_out[3]. */
+#define JBGPU_MLP_PARAMS 9603
+#define JBGPU_MLP_NORM_WORDS 38
+int jbgpu_mlp_set_params(jbgpu_ctx* ctx, const float* params, const float* norm);
+int jbgpu_mlp_get_params(jbgpu_ctx* ctx, float* params, float* norm);
+/* y[n][3] = model(x[n][16]) (host arrays). */
+int jbgpu_mlp_infer(jbgpu_ctx* ctx, const float* x, int64_t n, float* y);
+/* Gradient (JBGPU_MLP_PARAMS floats) of mean((y' - t')^2) over the n x 3 normalised outputs. */
+int jbgpu_mlp_grad(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* Adam training from He-uniform weights (seeded), normalisation taken from the data;
+ * loss_per_epoch may be NULL. */
+int jbgpu_mlp_train(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
+                    float lr, uint64_t seed, float* loss_per_epoch);
+/* on != 0: every solve takes its warm-start impulses from the model (needs a model). */
+int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 
 /* ---- measurement ---------------------------------------------------------------------- */
 /* Accumulated time (ms, HIP events recorded on the context stream around the launches) per
  * region since enabling, JBGPU_TIMER_REGIONS entries: 0 predict, 1 setAabb, 2 BVH traversal,
  * 3 GJK, 4 manifold update, 5 solver setup, 6 solver iterations, 7 integrate, 8 whole step,
- * 9 EPA, 10 pair remap; launches[r] counts the kernel launches inside region r. */
+ * 9 EPA, 10 pair remap, 11 MLP inference; launches[r] counts the kernel launches inside region r. */
 #define JBGPU_TIMER_REGIONS 12
 int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);

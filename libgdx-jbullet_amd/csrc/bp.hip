@@ -244,7 +244,7 @@ __device__ __forceinline__ int deltaK(const uint32_t* code, int n, int i, int j)
 }
 
 // Karras (2012) hierarchy: internal node i in [0, n-2]; child < 0 encodes leaf -(child+1)
-__global__ void kBuild(int n, const uint32_t* __restrict__ code, int* child, int* last, int* parent) {
+__global__ void kBuild(int n, const uint32_t* __restrict__ code, int* child, int* first, int* last) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (deltaK(code, n, i, i + 1) - deltaK(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -271,46 +271,63 @@ __global__ void kBuild(int n, const uint32_t* __restrict__ code, int* child, int
   int right = (hi == gamma + 1) ? -(gamma + 2) : gamma + 1;
   child[2 * i] = left;
   child[2 * i + 1] = right;
+  first[i] = lo;
   last[i] = hi;
-  // parent array: internal nodes at [0, n-1), leaves at [n-1, 2n-1)
-  parent[left >= 0 ? left : (n - 1) + (-left - 1)] = i;
-  parent[right >= 0 ? right : (n - 1) + (-right - 1)] = i;
-  if (i == 0) parent[0] = -1;
 }
 
-__global__ void kRefit(int n, const int* __restrict__ child, const int* __restrict__ parent, const float* leafBox,
-                       float* nodeBox, int* flags) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int node = parent[(n - 1) + s];
-  while (node >= 0) {
-    __threadfence();
-    if (atomicAdd(flags + node, 1) == 0) return;
-    __threadfence();
-    float r[8];
-    for (int c = 0; c < 2; c++) {
-      int ch = child[2 * node + c];
-      const float* B = ch < 0 ? leafBox + 8 * (-ch - 1) : nodeBox + 8 * ch;
-      // boxes written by other waves in this launch: read them coherently (agent scope, past L1)
-      float4 b0, b1;
-      b0.x = __hip_atomic_load(B + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b0.y = __hip_atomic_load(B + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b0.z = __hip_atomic_load(B + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b1.x = __hip_atomic_load(B + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b1.y = __hip_atomic_load(B + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b1.z = __hip_atomic_load(B + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (c == 0) {
-        r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[4] = b1.x; r[5] = b1.y; r[6] = b1.z;
-      } else {
-        r[0] = fminf(r[0], b0.x); r[1] = fminf(r[1], b0.y); r[2] = fminf(r[2], b0.z);
-        r[4] = fmaxf(r[4], b1.x); r[5] = fmaxf(r[5], b1.y); r[6] = fmaxf(r[6], b1.z);
-      }
-    }
-    float4* D = (float4*)(nodeBox + 8 * node);
-    D[0] = make_float4(r[0], r[1], r[2], 0.0f);
-    D[1] = make_float4(r[4], r[5], r[6], 0.0f);
-    node = parent[node];
+// union of 32 consecutive boxes (8 floats each) per output box; the last group may be partial
+__global__ __launch_bounds__(256) void kChunk(int n, const float* __restrict__ src, float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float inf = __builtin_huge_valf();
+  float4 mn = make_float4(inf, inf, inf, 0.0f), mx = make_float4(-inf, -inf, -inf, 0.0f);
+  if (i < n) {
+    mn = ((const float4*)src)[2 * i];
+    mx = ((const float4*)src)[2 * i + 1];
   }
+  for (int o = 16; o > 0; o >>= 1) {  // stays inside each 32-lane half of the wave
+    mn.x = fminf(mn.x, __shfl_xor(mn.x, o));
+    mn.y = fminf(mn.y, __shfl_xor(mn.y, o));
+    mn.z = fminf(mn.z, __shfl_xor(mn.z, o));
+    mx.x = fmaxf(mx.x, __shfl_xor(mx.x, o));
+    mx.y = fmaxf(mx.y, __shfl_xor(mx.y, o));
+    mx.z = fmaxf(mx.z, __shfl_xor(mx.z, o));
+  }
+  if ((i & 31) == 0 && i < n) {
+    ((float4*)dst)[2 * (i >> 5)] = make_float4(mn.x, mn.y, mn.z, 0.0f);
+    ((float4*)dst)[2 * (i >> 5) + 1] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+  }
+}
+
+struct BoxAcc {
+  float4 mn, mx;
+  __device__ void add(const float* B) {
+    float4 a = ((const float4*)B)[0], b = ((const float4*)B)[1];
+    mn.x = fminf(mn.x, a.x); mn.y = fminf(mn.y, a.y); mn.z = fminf(mn.z, a.z);
+    mx.x = fmaxf(mx.x, b.x); mx.y = fmaxf(mx.y, b.y); mx.z = fmaxf(mx.z, b.z);
+  }
+};
+
+// internal node boxes: the union of the node's sorted-leaf range [first, last], assembled from
+// leaves at the ragged ends and from full 32-, 1024- and 32768-leaf chunk boxes in between
+__global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__ first, const int* __restrict__ last,
+                                                  const float* __restrict__ leafBox, const float* __restrict__ c1,
+                                                  const float* __restrict__ c2, const float* __restrict__ c3,
+                                                  float* __restrict__ nodeBox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const float inf = __builtin_huge_valf();
+  BoxAcc B{make_float4(inf, inf, inf, 0.0f), make_float4(-inf, -inf, -inf, 0.0f)};
+  int a = first[i], e = last[i] + 1;
+  const float* lv[4] = {leafBox, c1, c2, c3};
+  for (int L = 0; L < 3; L++) {
+    while (a < e && (a & 31)) B.add(lv[L] + 8 * (a++));
+    while (e > a && (e & 31)) B.add(lv[L] + 8 * (--e));
+    a >>= 5;
+    e >>= 5;
+  }
+  while (a < e) B.add(c3 + 8 * (a++));
+  ((float4*)nodeBox)[2 * i] = make_float4(B.mn.x, B.mn.y, B.mn.z, 0.0f);
+  ((float4*)nodeBox)[2 * i + 1] = make_float4(B.mx.x, B.mx.y, B.mx.z, 0.0f);
 }
 
 struct PairTest {
@@ -502,7 +519,8 @@ void ensureProxyBuffers(Ctx& c) {
   c.nodeBox.ensure(8 * n);
   c.nodeChild.ensure(2 * n);
   c.nodeLast.ensure(n);
-  c.parent.ensure(2 * n);
+  c.nodeFirst.ensure(n);
+  c.chunkBox.ensure(8 * ((size_t)n / 31 + 64));
   c.flags.ensure(n);
   c.hugeList.ensure(n);
   c.hugeListAlt.ensure(n);
@@ -600,11 +618,18 @@ void bpFindPairs(Ctx& c) {
   }
   if (nn >= 2) {
     hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.leafBox.p);
-    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeLast.p,
-                       c.parent.p);
-    JB_CHECK(hipMemsetAsync(c.flags.p, 0, (size_t)nn * sizeof(int), s));
-    hipLaunchKernelGGL(kRefit, dim3(gridFor(nn)), dim3(256), 0, s, nn, c.nodeChild.p, c.parent.p, c.leafBox.p,
-                       c.nodeBox.p, c.flags.p);
+    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeFirst.p,
+                       c.nodeLast.p);
+    const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32, n3 = (n2 + 31) / 32;
+    float* c1 = c.chunkBox.p;
+    float* c2 = c1 + 8 * (size_t)n1;
+    float* c3 = c2 + 8 * (size_t)n2;
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, c.leafBox.p, c1);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, c1, c2);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, c2, c3);
+    (void)n3;
+    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, c.nodeFirst.p, c.nodeLast.p,
+                       c.leafBox.p, c1, c2, c3, c.nodeBox.p);
   }
   PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
   for (int attempt = 0; attempt < 3; attempt++) {

@@ -19,9 +19,13 @@ namespace jbg {
   } while (0)
 
 template <class T>
-struct DBuf {
+struct DBuf {  // device buffer owned by a context; freed with it
   T* p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
   void ensure(size_t k) {
     if (k <= n) return;
     if (p) {  // no kernel may still be reading the old buffer
@@ -106,7 +110,8 @@ struct Ctx {
   DBuf<float> nodeBox;   // internal nodes np*8
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
-  DBuf<int> parent;      // 2*np
+  DBuf<int> nodeFirst;   // np: internal node i covers sorted leaves [nodeFirst[i], nodeLast[i]]
+  DBuf<float> chunkBox;  // unions of 32, 1024 and 32768 consecutive leaves (8 floats each)
   DBuf<int> flags;       // np
   DBuf<float> bounds;    // 8
   DBuf<uint64_t> rawKeys, rawKeysAlt;
@@ -133,10 +138,14 @@ struct Ctx {
   DBuf<int> colorStart;
   // learned warm start (16-64-64-64-3 MLP, bf16 weights, fp32 accumulate)
   bool mlpReady = false;
+  bool useMlp = false;       // the model predicts the warm-start impulses of every solve
   DBuf<float> mlpW;          // fp32 master parameters
-  DBuf<uint16_t> mlpWb;      // bf16 copies used by the MFMA kernels
+  DBuf<uint16_t> mlpWb;      // bf16 MFMA operand fragments (forward W, backward W^T)
   DBuf<float> mlpNorm;       // input / output normalisation
-  DBuf<float> mlpOut;        // per slot predictions (3 floats)
+  DBuf<float> mlpX, mlpY;    // inference / training samples
+  DBuf<int> mlpIdx;          // training permutation
+  DBuf<float> mlpG, mlpM, mlpV, mlpLoss;  // gradient, Adam moments, loss accumulator
+  DBuf<uint16_t> mlpFrag;    // per training tile: activations and gradients, fragment-ready
   DBuf<int> mActive;  // active manifold list
   DBuf<unsigned long long> bodyPrio;
   DBuf<unsigned long long> bodyMask;
@@ -167,7 +176,7 @@ struct Ctx {
 // timer regions
 enum {
   T_PREDICT = 0, T_SETAABB, T_TRAVERSE, T_GJK, T_MANIFOLD, T_SOLVE_SETUP, T_SOLVE_ITER, T_INTEGRATE, T_STEP,
-  T_EPA, T_REMAP
+  T_EPA, T_REMAP, T_MLP
 };
 inline void tBegin(Ctx& c, int r) {
   if (c.timing) (void)hipEventRecord(c.evA[r], c.stream);
@@ -197,6 +206,7 @@ void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
 void npDispatchAll(Ctx& c);                              // A8-A16
 // solver
 void solverSolve(Ctx& c, float dt);                      // A18-A20
+void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)
 // integration
 void worldPredict(Ctx& c, float dt);
 void worldIntegrate(Ctx& c, float dt);

@@ -478,6 +478,19 @@ __global__ void kWriteback(int n, const int4* __restrict__ hdr, const float* __r
   }
 }
 
+// final (normal, lateral 1, lateral 2) impulses of every colour slot; zeros for unused slots
+__global__ void kGatherImpulses(int64_t nslots, const int4* __restrict__ hdr, const float* __restrict__ nrow,
+                                const float* __restrict__ frow, float* out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslots) return;
+  const int64_t p = q >> 2;
+  const int j = (int)(q & 3);
+  const bool ok = (hdr[p].x >> j) & 1;
+  out[3 * q] = ok ? nrow[(size_t)NW * p + RW * j + 19] : 0.0f;
+  out[3 * q + 1] = ok ? frow[(size_t)FW * p + RW * (2 * j) + 19] : 0.0f;
+  out[3 * q + 2] = ok ? frow[(size_t)FW * p + RW * (2 * j + 1) + 19] : 0.0f;
+}
+
 __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const float* __restrict__ props) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -490,6 +503,18 @@ __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const
 }
 
 }  // namespace
+
+void solverGetImpulses(Ctx& c, float* out, int64_t nslots) {
+  if (nslots > 4 * c.nActive) throw std::length_error("more slots than the last solve had");
+  if (nslots <= 0) return;
+  hipStream_t s = c.stream;
+  c.mlpY.ensure((size_t)3 * nslots);
+  hipLaunchKernelGGL(kGatherImpulses, dim3(gridFor(nslots)), dim3(256), 0, s, nslots, c.hdr.p, c.nrow.p, c.frow.p,
+                     c.mlpY.p);
+  JB_CHECK(hipGetLastError());
+  JB_CHECK(hipMemcpyAsync(out, c.mlpY.p, 3 * (size_t)nslots * sizeof(float), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+}
 
 void solverSolve(Ctx& c, float dt) {
   hipStream_t s = c.stream;
@@ -521,7 +546,8 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipStreamSynchronize(s));
   c.nActive = na;
   if (na == 0) return;
-  if (c.useLearned && !c.mlpReady && c.learnedSlots < 4 * (int64_t)na)
+  if (c.useMlp && !c.mlpReady) throw std::logic_error("learned warm start enabled without a model");
+  if (!c.useMlp && c.useLearned && c.learnedSlots < 4 * (int64_t)na)
     throw std::length_error("learned warm start has fewer slots than 4 x active manifolds");
   // colouring
   c.color.ensure(na);
@@ -584,15 +610,15 @@ void solverSolve(Ctx& c, float dt) {
   c.frow.ensure((size_t)FW * na);
   c.svel.ensure((size_t)8 * c.nb);
   JB_CHECK(hipMemcpyAsync(c.svel.p, c.vel.p, (size_t)8 * c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
-  const bool learnedHere = c.useLearned && c.mlpReady;
-  SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.mlpReady) ? 1 : 0,
+  const bool learnedHere = c.useMlp;
+  SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.useMlp) ? 1 : 0,
                   (c.dumpFeatures || learnedHere) ? 1 : 0, c.tolerance};
   if (SP.dumpFeatures) c.features.ensure((size_t)16 * 4 * na);
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, c.hdr.p, c.nrow.p, c.frow.p, SP.useLearned ? c.learned.p : nullptr,
                      SP.dumpFeatures ? c.features.p : nullptr);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
-  const int apply = (c.warmstart || c.useLearned) ? 1 : 0;
+  const int apply = (c.warmstart || c.useLearned || c.useMlp) ? 1 : 0;
   for (int k = 0; k < c.ncolors; k++) {
     int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
     hipLaunchKernelGGL(kWarmApply, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, apply, c.hdr.p,

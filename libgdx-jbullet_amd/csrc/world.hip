@@ -11,6 +11,7 @@
 
 #include "../../include/jbgpu.h"
 #include "internal.h"
+#include "mlp.h"
 #include "shapes.h"
 
 using namespace jbg;
@@ -386,28 +387,7 @@ int jbgpu_ctx_destroy(jbgpu_ctx* ctx) {
   if (!ctx) return JBGPU_ERR_ARG;
   Ctx& c = ctx->c;
   (void)hipStreamSynchronize(c.stream);
-  DBuf<float>* fb[] = {&c.shapes, &c.verts, &c.tr, &c.vel, &c.svel, &c.props, &c.grav, &c.deact, &c.leaf, &c.aabb,
-                       &c.cand, &c.leafBox, &c.nodeBox, &c.bounds, &c.man[0], &c.man[1], &c.candC, &c.epaWork,
-                       &c.epaState, &c.rows, &c.learned, &c.features};
-  for (auto* b : fb) b->release();
-  DBuf<int>* ib[] = {&c.act, &c.world, &c.stage, &c.filt, &c.q, &c.order, &c.orderAlt, &c.isHuge, &c.hugeList,
-                     &c.hugeListAlt, &c.hugeWorld, &c.hugeWorldAlt, &c.nodeChild, &c.nodeLast, &c.parent, &c.flags,
-                     &c.ptype, &c.typeKey, &c.typeKeyAlt, &c.perm, &c.permAlt, &c.epaQueue, &c.mActive, &c.color,
-                     &c.colorAlt, &c.colorOrder, &c.colorOrderAlt, &c.done};
-  for (auto* b : ib) b->release();
-  c.morton.release();
-  c.mortonAlt.release();
-  c.rawKeys.release();
-  c.rawKeysAlt.release();
-  c.keys[0].release();
-  c.keys[1].release();
-  c.counter.release();
-  c.cubTmp.release();
-  c.bodyPrio.release();
-  c.bodyMask.release();
-  c.colorKey.release();
-  c.colorKeyAlt.release();
-  c.resid.release();
+  // device buffers are freed by their destructors (after the stream has drained)
   for (int r = 0; r < kTimerRegions; r++) {
     if (c.evA[r]) (void)hipEventDestroy(c.evA[r]);
     if (c.evB[r]) (void)hipEventDestroy(c.evB[r]);
@@ -800,6 +780,13 @@ int jbgpu_solver_get_features(jbgpu_ctx* ctx, float* out, int64_t nslots) {
   });
 }
 
+int jbgpu_solver_get_impulses(jbgpu_ctx* ctx, float* out, int64_t nslots) {
+  return guard(ctx, [&] {
+    if (nslots > 0 && !out) throw std::invalid_argument("null output");
+    solverGetImpulses(ctx->c, out, nslots);
+  });
+}
+
 int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations) {
   return guard(ctx, [&] {
     if (max_iterations < 1 || num_iterations < 0) throw std::invalid_argument("bad iteration counts");
@@ -814,6 +801,37 @@ int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32
     const auto& h = ctx->c.residHist;
     if (n) *n = (int32_t)h.size();
     for (int32_t k = 0; k < cap && k < (int32_t)h.size(); k++) out[k] = h[k];
+  });
+}
+
+// ---- learned warm start ----------------------------------------------------------------
+static_assert(jbg::kMlpParams == JBGPU_MLP_PARAMS && jbg::kMlpNormWords == JBGPU_MLP_NORM_WORDS, "MLP layout");
+
+int jbgpu_mlp_set_params(jbgpu_ctx* ctx, const float* params, const float* norm) {
+  return guard(ctx, [&] { mlpSetParams(ctx->c, params, norm); });
+}
+
+int jbgpu_mlp_get_params(jbgpu_ctx* ctx, float* params, float* norm) {
+  return guard(ctx, [&] { mlpGetParams(ctx->c, params, norm); });
+}
+
+int jbgpu_mlp_infer(jbgpu_ctx* ctx, const float* x, int64_t n, float* y) {
+  return guard(ctx, [&] { mlpInfer(ctx->c, x, n, y); });
+}
+
+int jbgpu_mlp_grad(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, float* grad, float* loss) {
+  return guard(ctx, [&] { mlpGrad(ctx->c, x, t, n, grad, loss); });
+}
+
+int jbgpu_mlp_train(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
+                    float lr, uint64_t seed, float* loss_per_epoch) {
+  return guard(ctx, [&] { mlpTrain(ctx->c, x, t, n, epochs, batch, lr, seed, loss_per_epoch); });
+}
+
+int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on) {
+  return guard(ctx, [&] {
+    if (on && !ctx->c.mlpReady) throw std::logic_error("no model loaded or trained");
+    ctx->c.useMlp = on != 0;
   });
 }
 

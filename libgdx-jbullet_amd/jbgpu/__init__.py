@@ -20,7 +20,8 @@ BP_DBVT, BP_SAP16, BP_SAP32, BP_SIMPLE = 0, 1, 2, 3
 ORDER_REFERENCE, ORDER_COLOR, ORDER_EXPLICIT = 0, 1, 2
 TIMER_REGIONS = 12
 TIMER_NAMES = ["predict", "setaabb", "traverse", "gjk", "manifold", "solver_setup", "solver_iter", "integrate",
-               "step", "epa", "remap", "unused"]
+               "step", "epa", "remap", "mlp"]
+MLP_PARAMS, MLP_NORM_WORDS = 9603, 38
 
 # every exported symbol of include/jbgpu.h: (restype, argtypes)
 _f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -71,8 +72,15 @@ SYMBOLS = {
     "jbgpu_solver_set_warmstart": (C.c_int, [_vp, _vp, C.c_int64, C.c_int32]),
     "jbgpu_solver_enable_features": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_solver_get_features": (C.c_int, [_vp, _f, C.c_int64]),
+    "jbgpu_solver_get_impulses": (C.c_int, [_vp, _f, C.c_int64]),
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
+    "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
+    "jbgpu_mlp_get_params": (C.c_int, [_vp, _f, _f]),
+    "jbgpu_mlp_infer": (C.c_int, [_vp, _f, C.c_int64, _f]),
+    "jbgpu_mlp_grad": (C.c_int, [_vp, _f, _f, C.c_int64, _f, C.POINTER(C.c_float)]),
+    "jbgpu_mlp_train": (C.c_int, [_vp, _f, _f, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _vp]),
+    "jbgpu_solver_use_mlp": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_enable": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_get": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int64)]),
 }
@@ -288,6 +296,49 @@ class Context:
         out = np.zeros(max(nslots, 1) * 16, np.float32)
         self._ck(load().jbgpu_solver_get_features(self.h, out, int(nslots)), "solver_get_features")
         return out.reshape(-1, 16)[:nslots]
+
+    def solver_impulses(self, nslots):
+        """Final (normal, lateral1, lateral2) impulses of the last solve, by contact slot."""
+        out = np.zeros(max(nslots, 1) * 3, np.float32)
+        self._ck(load().jbgpu_solver_get_impulses(self.h, out, int(nslots)), "solver_get_impulses")
+        return out.reshape(-1, 3)[:nslots]
+
+    def solver_use_mlp(self, on=True):
+        self._ck(load().jbgpu_solver_use_mlp(self.h, int(on)), "solver_use_mlp")
+
+    # ---- learned warm start model
+    def mlp_set_params(self, params, norm):
+        self._ck(load().jbgpu_mlp_set_params(self.h, f32(params).reshape(-1), f32(norm).reshape(-1)),
+                 "mlp_set_params")
+
+    def mlp_get_params(self):
+        w = np.zeros(MLP_PARAMS, np.float32)
+        nm = np.zeros(MLP_NORM_WORDS, np.float32)
+        self._ck(load().jbgpu_mlp_get_params(self.h, w, nm), "mlp_get_params")
+        return w, nm
+
+    def mlp_infer(self, x):
+        x = f32(x).reshape(-1, 16)
+        y = np.zeros((max(len(x), 1), 3), np.float32)
+        self._ck(load().jbgpu_mlp_infer(self.h, x.reshape(-1) if len(x) else np.zeros(16, np.float32), len(x),
+                                        y.reshape(-1)), "mlp_infer")
+        return y[:len(x)]
+
+    def mlp_grad(self, x, t):
+        x = f32(x).reshape(-1, 16)
+        t = f32(t).reshape(-1, 3)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float(0)
+        self._ck(load().jbgpu_mlp_grad(self.h, x.reshape(-1), t.reshape(-1), len(x), g, C.byref(loss)), "mlp_grad")
+        return g, loss.value
+
+    def mlp_train(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=0):
+        x = f32(x).reshape(-1, 16)
+        t = f32(t).reshape(-1, 3)
+        losses = np.zeros(epochs, np.float32)
+        self._ck(load().jbgpu_mlp_train(self.h, x.reshape(-1), t.reshape(-1), len(x), int(epochs), int(batch),
+                                        float(lr), int(seed), _p(losses)), "mlp_train")
+        return losses
 
     def solver_set_params(self, tolerance=0.0, max_iterations=100, num_iterations=10):
         self._ck(load().jbgpu_solver_set_params(self.h, float(tolerance), int(max_iterations), int(num_iterations)),

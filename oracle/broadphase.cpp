@@ -129,7 +129,8 @@ void Broadphase::collide() {
 }
 
 static inline bool filterPass(const Proxy& a, const Proxy& b) {  // HashedOverlappingPairCache.java:184-185
-  return ((a.group & b.mask) != 0) && ((b.group & a.mask) != 0);
+  // separate DynamicsWorld instances (one broadphase each) never see each other's proxies
+  return a.world == b.world && ((a.group & b.mask) != 0) && ((b.group & a.mask) != 0);
 }
 
 bool Broadphase::overlap(int i, int j) const {

@@ -275,6 +275,10 @@ void orc_bp_create(void* h, int n, const float* aabb6, const int16_t* group, con
   for (int i = 0; i < n; i++)
     bp->createProxy(Aabb{ld3(aabb6 + 6 * i), ld3(aabb6 + 6 * i + 3)}, group ? group[i] : 1, mask ? mask[i] : -1);
 }
+void orc_bp_set_worlds(void* h, const int32_t* world) {
+  Broadphase* bp = (Broadphase*)h;
+  for (size_t i = 0; i < bp->proxies.size(); i++) bp->proxies[i].world = world[i];
+}
 void orc_bp_set_all(void* h, const float* aabb6) {
   Broadphase* bp = (Broadphase*)h;
   for (size_t i = 0; i < bp->proxies.size(); i++) bp->setAabb((int)i, Aabb{ld3(aabb6 + 6 * i), ld3(aabb6 + 6 * i + 3)});
@@ -353,6 +357,11 @@ void orc_world_set_solver(void* h, int order, float tol, int maxIt, int numIt) {
   w->cfg.solver.tolerance = tol;
   w->cfg.solver.maxIterations = maxIt;
   w->cfg.solver.numIterations = numIt;
+}
+// world id per body (batched independent worlds; bodies of different worlds never pair)
+void orc_world_set_worlds(void* h, const int32_t* world) {
+  World* w = (World*)h;
+  for (size_t i = 0; i < w->bp.proxies.size(); i++) w->bp.proxies[i].world = world[i];
 }
 int orc_world_step(void* h, float dt, int nsteps) {
   World* w = (World*)h;

@@ -81,6 +81,7 @@ struct Proxy {
   Aabb aabb;        // DbvtProxy.aabb (the stored, possibly fattened box)
   int16_t group, mask;
   int32_t q[6];     // AxisSweep3 quantised min xyz / max xyz
+  int32_t world = 0;  // batched independent worlds: proxies of different worlds never pair
 };
 struct Broadphase {
   int mode;  // JBGPU_BP_*

@@ -59,6 +59,7 @@ def lib():
             "orc_bp_free": (None, [vp]),
             "orc_bp_create": (None, [vp, C.c_int, _f, vp, vp]),
             "orc_bp_set_all": (None, [vp, _f]),
+            "orc_bp_set_worlds": (None, [vp, _i32]),
             "orc_bp_step": (C.c_int64, [vp, _u64, C.c_int64, C.c_int]),
             "orc_bp_get_proxies": (None, [vp, _f]),
             "orc_bp_get_quantized": (None, [vp, _i32]),
@@ -66,6 +67,7 @@ def lib():
             "orc_world_new": (vp, [C.POINTER(WorldCfg), C.c_int, _f, C.c_int, _f, C.c_int, _f]),
             "orc_world_free": (None, [vp]),
             "orc_world_set_order": (None, [vp, C.c_int, _i32]),
+            "orc_world_set_worlds": (None, [vp, _i32]),
             "orc_world_set_solver": (None, [vp, C.c_int, C.c_float, C.c_int, C.c_int]),
             "orc_world_step": (C.c_int, [vp, C.c_float, C.c_int]),
             "orc_world_step_until": (C.c_int, [vp, C.c_float, C.c_int]),
@@ -196,6 +198,9 @@ class Broadphase:
         lib().orc_bp_create(self.h, n, a, _ptr(g), _ptr(m))
         self.n += n
 
+    def set_worlds(self, world):
+        lib().orc_bp_set_worlds(self.h, np.ascontiguousarray(world, np.int32))
+
     def set_all(self, aabb6):
         lib().orc_bp_set_all(self.h, f32(aabb6).reshape(-1))
 
@@ -252,6 +257,10 @@ class World:
 
     def step_until(self, phase, dt=1.0 / 60.0):
         lib().orc_world_step_until(self.h, dt, phase)
+
+    def set_worlds(self, world):
+        """World id per body: bodies of different worlds never pair (batched independent worlds)."""
+        lib().orc_world_set_worlds(self.h, np.ascontiguousarray(world, np.int32))
 
     def set_order(self, order):
         o = np.ascontiguousarray(order, np.int32)

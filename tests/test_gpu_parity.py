@@ -53,8 +53,10 @@ def test_broadphase_pairs_match_oracle(mode):
 def _world_pair(sc, iterations=10, **kw):
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
     ctx = jbgpu.Context(num_iterations=iterations, **kw)
-    ctx.world_create(shapes, verts, bodies)
+    ctx.world_create(shapes, verts, bodies, world_ids=sc.world)
     orc = pyoracle.World(shapes, verts, bodies, iterations=iterations, order=jbgpu.ORDER_EXPLICIT)
+    if sc.world is not None:
+        orc.set_worlds(sc.world)
     return ctx, orc
 
 
@@ -119,4 +121,22 @@ def test_world_mixed_shapes_step_parity():
             print(f"[mixed] step {step}: pairs {st['pairs']} contacts {st['contacts']} manifolds {st['manifolds']} "
                   f"colors {st['colors']} epa {st['epa_calls']}", flush=True)
         _compare_world(ctx, orc, f"mixed step {step}")
+    ctx.close()
+
+
+def test_batched_worlds_step_parity():
+    """Independent worlds batched in one context (own floor plane each, pairs never cross worlds)."""
+    sc = scenes.batched_towers(seed=3, n_worlds=24, lo=16, hi=120)
+    ctx, orc = _world_pair(sc, iterations=10)
+    for step in range(80):
+        _sync_and_step(ctx, orc)
+        st = ctx.stats()
+        if step % 20 == 0:
+            print(f"[batched] step {step}: pairs {st['pairs']} contacts {st['contacts']} manifolds {st['manifolds']} "
+                  f"colors {st['colors']}", flush=True)
+        _compare_world(ctx, orc, f"batched step {step}")
+    kg, _ = ctx.pairs()
+    lo = (kg >> np.uint64(32)).astype(np.int64) - 1
+    hi = (kg & np.uint64(0xffffffff)).astype(np.int64) - 1
+    assert np.array_equal(sc.world[lo], sc.world[hi])
     ctx.close()

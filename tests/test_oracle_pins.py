@@ -548,3 +548,86 @@ def test_permutation_within_a_colour_is_exact():
         ncol_max = max(ncol_max, ncol)
         assert np.array_equal(worlds[0].bodies().view(np.uint32), worlds[1].bodies().view(np.uint32)), step
     assert ncol_max >= 2
+
+
+# ------------------------------------------------------------------ learned warm-start model reference
+def test_mlp_reference_backprop_against_finite_differences():
+    import mlp_reference as mr
+    rng = np.random.default_rng(21)
+    p = mr.init_params(rng, last_scale=1.0).astype(np.float64)
+    xn = rng.normal(size=(40, 16))
+    tn = rng.normal(size=(40, 3))
+    g = mr.exact_grad(p, xn, tn)
+    assert mr.NPARAMS == 9603 and g.shape == (9603,)
+    idx = np.concatenate([rng.choice(9603, 60, replace=False), [0, 1087, 1088, 5247, 9407, 9600, 9602]])
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (mr.exact_loss(p + e, xn, tn) - mr.exact_loss(p - e, xn, tn)) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def test_mlp_reference_rounding_points():
+    import mlp_reference as mr
+    # bf16 round to nearest even: 1 + 2^-8 is a tie and rounds to 1; 1 + 3 * 2^-8 rounds up
+    assert mr.bf16(np.float32(1 + 2 ** -8)) == 1.0
+    assert mr.bf16(np.float32(1 + 3 * 2 ** -8)) == 1 + 2 ** -6
+    rng = np.random.default_rng(2)
+    p = mr.init_params(rng)
+    norm = mr.random_norm(rng)
+    x = rng.normal(size=(64, 16)).astype(np.float32)
+    t = rng.normal(size=(64, 3)).astype(np.float32)
+    g, loss = mr.grad(p, norm, x, t)
+    n = norm.astype(np.float64)
+    xn = (x.astype(np.float64) - n[:16]) * n[16:32]
+    tn = (t - n[35:38]) / n[32:35]
+    ge = mr.exact_grad(p.astype(np.float64), xn, tn)
+    cos = g @ ge / np.linalg.norm(g) / np.linalg.norm(ge)
+    assert cos > 0.999 and abs(loss - mr.exact_loss(p.astype(np.float64), xn, tn)) < 1e-2 * loss
+
+
+def test_batched_worlds_never_pair_across_worlds():
+    rng = np.random.default_rng(12)
+    one = _boxes(rng, 300, extent=6.0)
+    boxes = np.concatenate([one, one, one])            # three identical worlds, same place
+    world = np.repeat(np.arange(3), 300).astype(np.int32)
+    bp = po.Broadphase(3)
+    bp.create(boxes)
+    bp.set_worlds(world)
+    bp.set_all(boxes)
+    got = bp.step()
+    single = _brute_pairs(one)
+    want = np.sort(np.concatenate([single + np.uint64(((300 * w) << 32) + 300 * w) for w in range(3)]))
+    assert np.array_equal(got, want)
+
+
+def test_batched_world_steps_like_independent_worlds():
+    """A batched world fed a manifold order equals each world alone fed the same order restricted to it."""
+    sc = scenes.batched_towers(seed=2, n_worlds=3, lo=16, hi=40)
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    wid = sc.world.astype(np.int32)
+    comb = po.World(shapes, verts, bodies, iterations=10, order=2)
+    comb.set_worlds(wid)
+    probe = po.World(shapes, verts, bodies, iterations=10, order=2)
+    probe.set_worlds(wid)
+    sel = [np.nonzero(wid == w)[0] for w in range(3)]
+    subs = [po.World(shapes, verts, bodies[s_], iterations=10, order=2) for s_ in sel]
+    for step in range(60):
+        b = comb.bodies()
+        k, m = comb.pairs()
+        prox, stage = comb.proxies()
+        probe.set_bodies(b)
+        probe.set_pairs(k, m)
+        probe.set_proxies(prox, stage)
+        probe.step_until(2)
+        keys, man = probe.pairs()
+        order, _ = _colour_order(keys, man, b)
+        comb.set_order(order)
+        comb.step()
+        pw = wid[man[:, 1].view(np.int32)]
+        for w, sub in enumerate(subs):
+            mine = np.nonzero(pw == w)[0]
+            rank = {int(i): r for r, i in enumerate(mine)}
+            sub.set_order([rank[i] for i in order if i in rank])
+            sub.step()
+            assert np.array_equal(sub.bodies().view(np.uint32), comb.bodies()[sel[w]].view(np.uint32)), (step, w)
