@@ -61,6 +61,7 @@ typedef struct jbgpu_stats {
   int64_t iterations;   /* solver iterations run in the last solve */
   float residual;       /* residual after the last iteration */
   int64_t huge_proxies; /* proxies handled outside the BVH (e.g. planes) */
+  int64_t rows;         /* normal rows (contacts with distance <= 0) of the last solve */
 } jbgpu_stats;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -140,7 +141,11 @@ int jbgpu_solver_get_impulses(jbgpu_ctx* ctx, float* out, int64_t nslots);
 /* tolerance > 0: iterate until the residual ||dlambda|| / max(1, ||lambda||) <= tolerance or
  * max_iterations; tolerance 0: exactly num_iterations (ContactSolverInfo.numIterations). */
 int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations);
-/* Residual after each iteration of the last solve. */
+/* Launch shaping of the colour sweeps (no reference counterpart; results are identical for every
+ * value): trailing colours of at most `threshold` manifolds are swept by one single-workgroup launch,
+ * colour after colour; 0 gives every colour its own launch.  Default 1024. */
+int jbgpu_solver_set_tail(jbgpu_ctx* ctx, int32_t threshold);
+/* Residual after each iteration of the last solve (in fixed-iteration mode only the last one). */
 int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32_t* n);
 
 /* ---- learned warm start ------------------------------------------------------------------ */
@@ -174,6 +179,12 @@ int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 #define JBGPU_TIMER_REGIONS 12
 int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
+/* Kernel-level timing of the solver sweeps while timing is enabled: on every `every`-th timed step
+ * (0 = off) each normal-row and friction-row colour launch carries its own start/stop events
+ * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps.
+ * [0] normal rows, [1] friction rows: summed ms, timed launches, rows processed by them. */
+int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every);
+int jbgpu_timing_kernels_get(jbgpu_ctx* ctx, double* ms2, int64_t* launches2, int64_t* rows2);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,7 @@
 // internal.h — libjbgpu context, device buffers and the launchers shared by
 // the kernel files.  Not part of the C ABI.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -153,6 +154,9 @@ struct Ctx {
   DBuf<uint64_t> colorKey, colorKeyAlt;
   DBuf<int> colorOrder, colorOrderAlt;
   std::vector<int64_t> colorOff;
+  DBuf<int> colorOffDev;     // colour offsets on the device (tail sweeps)
+  int tailThreshold = 1024;  // trailing colours of at most this many manifolds share one workgroup launch
+  int headColors = 0;
   DBuf<double> resid;  // per iteration: dd, nn
   DBuf<int> done;
   DBuf<float> learned;  // per-contact learned warm-start impulses (npairs*4*3)
@@ -170,6 +174,13 @@ struct Ctx {
   double ms[kTimerRegions] = {};
   int64_t launches[kTimerRegions] = {};
   int64_t timedSteps = 0;
+  // sampled kernel timing of the solver sweeps (start/stop events per launch)
+  int kernelEvery = 0;
+  std::vector<hipEvent_t> kEv;  // event pool, two per timed launch
+  std::vector<int> kEvKind;     // 0 normal rows, 1 friction rows
+  double kMs[2] = {0, 0};
+  int64_t kLaunches[2] = {0, 0}, kRows[2] = {0, 0};
+  DBuf<unsigned long long> rowCount;  // normal rows of the current solve
   std::vector<float> residHist;
 };
 

@@ -25,6 +25,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include "internal.h"
+#include <algorithm>
+
 #include "mlp.h"
 #include "shapes.h"
 
@@ -308,6 +310,19 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
   hdr[p] = make_int4(mask, b0, b1, pi);
 }
 
+// normal rows of the solve (valid points of every slot): out[0] all slots, out[1] slots below headEnd
+__global__ void kCountRows(int na, int headEnd, const int4* __restrict__ hdr, unsigned long long* out) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long k = p < na ? (unsigned long long)__popc(hdr[p].x & 15) : 0ull;
+  unsigned long long kh = p < headEnd ? k : 0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    k += __shfl_xor(k, o);
+    kh += __shfl_xor(kh, o);
+  }
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, k);
+  if ((threadIdx.x & 63) == 0 && kh) atomicAdd(out + 1, kh);
+}
+
 // warm-start impulses of colour c in row order: per point normal, lateral 1, lateral 2 (:659-747)
 __global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, const int4* __restrict__ hdr,
                                                  const float* __restrict__ nrow, const float* __restrict__ frow,
@@ -333,129 +348,181 @@ __global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, co
   if (B.invMass != 0.0f) storeBody(svel, h.z, B);
 }
 
+// residual terms of one wave into one of kResidStripes (dd, nn) accumulators (res == nullptr: not tracked)
+constexpr int kResidStripes = 64;
 __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
+  if (!res) return;
   for (int off = 32; off > 0; off >>= 1) {
     dd += __shfl_xor(dd, off);
     nn += __shfl_xor(nn, off);
   }
   if ((threadIdx.x & 63) == 0 && (dd != 0.0 || nn != 0.0)) {
-    atomicAdd(res, dd);
-    atomicAdd(res + 1, nn);
+    double* r = res + 2 * (blockIdx.x & (kResidStripes - 1));
+    atomicAdd(r, dd);
+    atomicAdd(r + 1, nn);
   }
 }
 
-// normal rows of one colour: resolveSingleCollisionCombinedCacheFriendly (:234-284)
+// normal rows of one manifold slot p: resolveSingleCollisionCombinedCacheFriendly (:234-284)
+__device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const int4* __restrict__ hdr, float* nrow,
+                                           const float* __restrict__ props, float* svel, double& dd, double& nn) {
+  const int4 h = hdr[p];
+  float r[NW];
+  const float4* src = (const float4*)(nrow + (size_t)NW * p);
+#pragma unroll
+  for (int q = 0; q < NW / 4; q++) {
+    float4 v = src[q];
+    r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+  }
+  SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (!((h.x >> j) & 1)) continue;
+    const float* N = r + RW * j;
+    V3 nrm = v3(N[0], N[1], N[2]);
+    float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+    float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
+    float rel_vel = v1 - v2;
+    float positionalError = -N[16] * SP.erp / SP.dt;
+    float velocityError = N[17] - rel_vel;
+    float pimp = positionalError * N[15];
+    float vimp = velocityError * N[15];
+    float ni = pimp + vimp;
+    float old = N[19];
+    float sum = old + ni;
+    float nw = 0.0f > sum ? 0.0f : sum;
+    ni = nw - old;
+    applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), ni);
+    applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -ni);
+    nrow[(size_t)NW * p + RW * j + 19] = nw;
+    double d = (double)nw - (double)old;
+    dd += d * d;
+    nn += (double)nw * (double)nw;
+  }
+  if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+  if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+}
+
+// friction rows of one manifold slot p: resolveSingleFrictionCacheFriendly (:286-346); the limit uses the
+// normal impulse of the row's own contact from this iteration
+__device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, const int4* __restrict__ hdr,
+                                             const float* __restrict__ nrow, float* frow,
+                                             const float* __restrict__ props, float* svel, double& dd, double& nn) {
+  const int4 h = hdr[p];
+  float r[FW];
+  const float4* src = (const float4*)(frow + (size_t)FW * p);
+#pragma unroll
+  for (int q = 0; q < FW / 4; q++) {
+    float4 v = src[q];
+    r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+  }
+  float napp[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) napp[j] = nrow[(size_t)NW * p + RW * j + 19];
+  SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (!((h.x >> j) & 1)) continue;
+    float total = napp[j] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+      const float* N = r + RW * (2 * j + f);
+      float old = N[19];
+      float ap = old;
+      float limit = total * N[18];
+      if (total > 0.0f) {
+        V3 nrm = v3(N[0], N[1], N[2]);
+        float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+        float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
+        float rel_vel = v1 - v2;
+        float j1 = -rel_vel * N[15];
+        ap = old + j1;
+        if (limit < ap) {
+          ap = limit;
+        } else if (ap < -limit) {
+          ap = -limit;
+        }
+        j1 = ap - old;
+        applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), j1);
+        applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -j1);
+        frow[(size_t)FW * p + RW * (2 * j + f) + 19] = ap;
+      }
+      double d = (double)ap - (double)old;
+      dd += d * d;
+      nn += (double)ap * (double)ap;
+    }
+  }
+  if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+  if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+}
+
+// one colour, one thread per manifold slot
 __global__ __launch_bounds__(64) void kNormal(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
                                               float* nrow, const float* __restrict__ props, float* svel, double* res,
                                               const int* __restrict__ done) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
-  const int stop = *done;
-  if (k < cnt && !stop) {
-    const int p = off + k;
-    const int4 h = hdr[p];
-    float r[NW];
-    const float4* src = (const float4*)(nrow + (size_t)NW * p);
-#pragma unroll
-    for (int q = 0; q < NW / 4; q++) {
-      float4 v = src[q];
-      r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
-    }
-    SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (!((h.x >> j) & 1)) continue;
-      const float* N = r + RW * j;
-      V3 nrm = v3(N[0], N[1], N[2]);
-      float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
-      float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
-      float rel_vel = v1 - v2;
-      float positionalError = -N[16] * SP.erp / SP.dt;
-      float velocityError = N[17] - rel_vel;
-      float pimp = positionalError * N[15];
-      float vimp = velocityError * N[15];
-      float ni = pimp + vimp;
-      float old = N[19];
-      float sum = old + ni;
-      float nw = 0.0f > sum ? 0.0f : sum;
-      ni = nw - old;
-      applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), ni);
-      applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -ni);
-      nrow[(size_t)NW * p + RW * j + 19] = nw;
-      double d = (double)nw - (double)old;
-      dd += d * d;
-      nn += (double)nw * (double)nw;
-    }
-    if (A.invMass != 0.0f) storeBody(svel, h.y, A);
-    if (B.invMass != 0.0f) storeBody(svel, h.z, B);
-  }
+  const int stop = done ? *done : 0;  // tolerance mode only
+  if (k < cnt && !stop) normalSlot(off + k, SP, hdr, nrow, props, svel, dd, nn);
   accumulate(dd, nn, res);
 }
-
-// friction rows of one colour: resolveSingleFrictionCacheFriendly (:286-346)
 __global__ __launch_bounds__(64) void kFriction(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
                                                 const float* __restrict__ nrow, float* frow,
                                                 const float* __restrict__ props, float* svel, double* res,
                                                 const int* __restrict__ done) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
-  const int stop = *done;
-  if (k < cnt && !stop) {
-    const int p = off + k;
-    const int4 h = hdr[p];
-    float r[FW];
-    const float4* src = (const float4*)(frow + (size_t)FW * p);
-#pragma unroll
-    for (int q = 0; q < FW / 4; q++) {
-      float4 v = src[q];
-      r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
-    }
-    float napp[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) napp[j] = nrow[(size_t)NW * p + RW * j + 19];
-    SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if (!((h.x >> j) & 1)) continue;
-      float total = napp[j] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
-#pragma unroll
-      for (int f = 0; f < 2; f++) {
-        const float* N = r + RW * (2 * j + f);
-        float old = N[19];
-        float ap = old;
-        float limit = total * N[18];
-        if (total > 0.0f) {
-          V3 nrm = v3(N[0], N[1], N[2]);
-          float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
-          float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
-          float rel_vel = v1 - v2;
-          float j1 = -rel_vel * N[15];
-          ap = old + j1;
-          if (limit < ap) {
-            ap = limit;
-          } else if (ap < -limit) {
-            ap = -limit;
-          }
-          j1 = ap - old;
-          applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), j1);
-          applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -j1);
-          frow[(size_t)FW * p + RW * (2 * j + f) + 19] = ap;
-        }
-        double d = (double)ap - (double)old;
-        dd += d * d;
-        nn += (double)ap * (double)ap;
-      }
-    }
-    if (A.invMass != 0.0f) storeBody(svel, h.y, A);
-    if (B.invMass != 0.0f) storeBody(svel, h.z, B);
-  }
+  const int stop = done ? *done : 0;
+  if (k < cnt && !stop) frictionSlot(off + k, SP, hdr, nrow, frow, props, svel, dd, nn);
   accumulate(dd, nn, res);
 }
 
-// residual after iteration `it`: r = ||dl|| / max(1, ||l||); stop when r <= tol
-__global__ void kResidual(int it, float tol, double* res, int* done, float* out) {
-  if (*done) return;
-  double dd = res[2 * it], nn = res[2 * it + 1];
+// the small colours [c0, c1) of a sweep in one workgroup: colour after colour, separated by workgroup
+// barriers (all waves of a workgroup share one CU and its L1, so a barrier orders their global accesses)
+constexpr int kTailThreads = 512;
+__global__ __launch_bounds__(kTailThreads) void kNormalTail(int c0, int c1, const int* __restrict__ colorOff,
+                                                            SolverParams SP, const int4* __restrict__ hdr,
+                                                            float* nrow, const float* __restrict__ props,
+                                                            float* svel, double* res, const int* __restrict__ done) {
+  double dd = 0.0, nn = 0.0;
+  const int stop = done ? *done : 0;
+  if (!stop)
+    for (int c = c0; c < c1; c++) {
+      const int a = colorOff[c], b = colorOff[c + 1];
+      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads) normalSlot(p, SP, hdr, nrow, props, svel, dd, nn);
+      __syncthreads();
+    }
+  accumulate(dd, nn, res);
+}
+__global__ __launch_bounds__(kTailThreads) void kFrictionTail(int c0, int c1, const int* __restrict__ colorOff,
+                                                              SolverParams SP, const int4* __restrict__ hdr,
+                                                              const float* __restrict__ nrow, float* frow,
+                                                              const float* __restrict__ props, float* svel,
+                                                              double* res, const int* __restrict__ done) {
+  double dd = 0.0, nn = 0.0;
+  const int stop = done ? *done : 0;
+  if (!stop)
+    for (int c = c0; c < c1; c++) {
+      const int a = colorOff[c], b = colorOff[c + 1];
+      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads)
+        frictionSlot(p, SP, hdr, nrow, frow, props, svel, dd, nn);
+      __syncthreads();
+    }
+  accumulate(dd, nn, res);
+}
+
+// one wave: sum the stripes of iteration `it` into hist[2*it..], then r = ||dl|| / max(1, ||l||)
+__global__ __launch_bounds__(64) void kResidual(int it, float tol, const double* __restrict__ stripes, double* hist,
+                                                int* done, float* out) {
+  const int t = threadIdx.x;
+  double dd = stripes[2 * t], nn = stripes[2 * t + 1];
+  for (int off = 32; off > 0; off >>= 1) {
+    dd += __shfl_xor(dd, off);
+    nn += __shfl_xor(nn, off);
+  }
+  if (t != 0 || *done) return;
+  hist[2 * it] = dd;
+  hist[2 * it + 1] = nn;
   double nrm = sqrt(nn);
   float r = (float)(sqrt(dd) / (nrm > 1.0 ? nrm : 1.0));
   out[0] = r;
@@ -617,6 +684,22 @@ void solverSolve(Ctx& c, float dt) {
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, c.hdr.p, c.nrow.p, c.frow.p, SP.useLearned ? c.learned.p : nullptr,
                      SP.dumpFeatures ? c.features.p : nullptr);
+  // colours [0, headColors) get a launch each; the trailing small ones share one single-workgroup launch
+  int headColors = c.ncolors;
+  if (c.tailThreshold > 0) {
+    while (headColors > 0 && c.colorOff[headColors] - c.colorOff[headColors - 1] <= c.tailThreshold) headColors--;
+    if (c.ncolors - headColors < 2) headColors = c.ncolors;
+  }
+  c.headColors = headColors;
+  if (headColors < c.ncolors) {
+    std::vector<int> offs(c.colorOff.begin(), c.colorOff.end());
+    c.colorOffDev.ensure(offs.size());
+    JB_CHECK(hipMemcpyAsync(c.colorOffDev.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  c.rowCount.ensure(2);
+  JB_CHECK(hipMemsetAsync(c.rowCount.p, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(kCountRows, dim3(gridFor(na)), dim3(256), 0, s, na, (int)c.colorOff[headColors], c.hdr.p,
+                     c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
   const int apply = (c.warmstart || c.useLearned || c.useMlp) ? 1 : 0;
   for (int k = 0; k < c.ncolors; k++) {
@@ -626,39 +709,96 @@ void solverSolve(Ctx& c, float dt) {
   }
   tEnd(c, T_SOLVE_SETUP, c.ncolors + 1);
   // iterations
-  int iters = c.tolerance > 0.0f ? c.maxIterations : c.numIterations;
-  c.resid.ensure(2 * (size_t)iters + 4);
+  // The residual is accumulated only where it is used: every iteration in tolerance mode, else the last one.
+  const bool tolMode = c.tolerance > 0.0f;
+  int iters = tolMode ? c.maxIterations : c.numIterations;
+  const size_t histWords = 2 * (size_t)iters + 4, stripeWords = 2 * (size_t)kResidStripes * iters;
+  c.resid.ensure(histWords + stripeWords);
   c.done.ensure(4);
-  JB_CHECK(hipMemsetAsync(c.resid.p, 0, (2 * (size_t)iters + 4) * sizeof(double), s));
+  JB_CHECK(hipMemsetAsync(c.resid.p, 0, (histWords + stripeWords) * sizeof(double), s));
   JB_CHECK(hipMemsetAsync(c.done.p, 0, 4 * sizeof(int), s));
   float* resOut = (float*)(c.counter.p + 6);
   JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
+  int* done = tolMode ? c.done.p : nullptr;
+  // kernel-level timing of this step's sweeps (sampled)
+  const bool sampleK = c.timing && c.kernelEvery > 0 && c.timedSteps % c.kernelEvery == 0;
+  size_t kUsed = 0;
+  auto kPair = [&](int kind) -> hipEvent_t* {
+    if (kUsed + 2 > c.kEv.size()) {
+      const size_t old = c.kEv.size();
+      c.kEv.resize(old + 256);
+      for (size_t e = old; e < c.kEv.size(); e++) JB_CHECK(hipEventCreate(&c.kEv[e]));
+    }
+    c.kEvKind.resize(c.kEv.size() / 2);
+    c.kEvKind[kUsed / 2] = kind;
+    kUsed += 2;
+    return &c.kEv[kUsed - 2];
+  };
   tBegin(c, T_SOLVE_ITER);
   for (int itn = 0; itn < iters; itn++) {
-    double* res = c.resid.p + 2 * itn;
-    for (int k = 0; k < c.ncolors; k++) {
+    const bool track = tolMode || itn == iters - 1;
+    double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
+    for (int k = 0; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      hipLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
-                         c.nrow.p, c.props.p, c.svel.p, res, c.done.p);
+      if (sampleK) {
+        hipEvent_t* ev = kPair(0);
+        hipExtLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, ev[0], ev[1], 0, (int)c.colorOff[k],
+                              cnt, SP, (const int4*)c.hdr.p, c.nrow.p, (const float*)c.props.p, c.svel.p, res,
+                              (const int*)done);
+      } else {
+        hipLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
+                           c.nrow.p, c.props.p, c.svel.p, res, done);
+      }
     }
-    for (int k = 0; k < c.ncolors; k++) {
+    if (headColors < c.ncolors)
+      hipLaunchKernelGGL(kNormalTail, dim3(1), dim3(kTailThreads), 0, s, headColors, c.ncolors,
+                         (const int*)c.colorOffDev.p, SP, (const int4*)c.hdr.p, c.nrow.p, (const float*)c.props.p,
+                         c.svel.p, res, (const int*)done);
+    for (int k = 0; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      hipLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
-                         c.nrow.p, c.frow.p, c.props.p, c.svel.p, res, c.done.p);
+      if (sampleK) {
+        hipEvent_t* ev = kPair(1);
+        hipExtLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, ev[0], ev[1], 0,
+                              (int)c.colorOff[k], cnt, SP, (const int4*)c.hdr.p, (const float*)c.nrow.p, c.frow.p,
+                              (const float*)c.props.p, c.svel.p, res, (const int*)done);
+      } else {
+        hipLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP,
+                           c.hdr.p, c.nrow.p, c.frow.p, c.props.p, c.svel.p, res, done);
+      }
     }
-    hipLaunchKernelGGL(kResidual, dim3(1), dim3(1), 0, s, itn, c.tolerance, c.resid.p, c.done.p, resOut);
+    if (headColors < c.ncolors)
+      hipLaunchKernelGGL(kFrictionTail, dim3(1), dim3(kTailThreads), 0, s, headColors, c.ncolors,
+                         (const int*)c.colorOffDev.p, SP, (const int4*)c.hdr.p, (const float*)c.nrow.p, c.frow.p,
+                         (const float*)c.props.p, c.svel.p, res, (const int*)done);
+    if (track)
+      hipLaunchKernelGGL(kResidual, dim3(1), dim3(64), 0, s, itn, c.tolerance, res, c.resid.p, c.done.p, resOut);
   }
-  tEnd(c, T_SOLVE_ITER, (int64_t)iters * 2 * c.ncolors);
+  tEnd(c, T_SOLVE_ITER, (int64_t)iters * 2 * (headColors + (headColors < c.ncolors ? 1 : 0)));
   hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
   float ro[2] = {0, 0};
   std::vector<double> rh(2 * (size_t)iters);
   JB_CHECK(hipMemcpyAsync(ro, resOut, sizeof(ro), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipMemcpyAsync(rh.data(), c.resid.p, rh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  unsigned long long rowsHT[2] = {0, 0};
+  JB_CHECK(hipMemcpyAsync(rowsHT, c.rowCount.p, sizeof(rowsHT), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
+  c.st.rows = (int64_t)rowsHT[0];
+  const int64_t rows = (int64_t)rowsHT[1];  // rows swept by the per-colour (timed) launches
+  if (sampleK) {
+    for (size_t e = 0; e < kUsed; e += 2) {
+      float ms = 0.0f;
+      JB_CHECK(hipEventElapsedTime(&ms, c.kEv[e], c.kEv[e + 1]));
+      const int kind = c.kEvKind[e / 2];
+      c.kMs[kind] += ms;
+      c.kLaunches[kind] += 1;
+    }
+    c.kRows[0] += (int64_t)rows * iters;
+    c.kRows[1] += 2 * (int64_t)rows * iters;
+  }
   c.st.residual = ro[0];
   c.st.iterations = (int64_t)ro[1];
-  for (int k = 0; k < (int)c.st.iterations; k++) {
+  for (int k = tolMode ? 0 : std::max(iters - 1, 0); k < (int)c.st.iterations; k++) {
     double nrm = std::sqrt(rh[2 * k + 1]);
     c.residHist.push_back((float)(std::sqrt(rh[2 * k]) / (nrm > 1.0 ? nrm : 1.0)));
   }

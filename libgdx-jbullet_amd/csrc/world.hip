@@ -392,6 +392,7 @@ int jbgpu_ctx_destroy(jbgpu_ctx* ctx) {
     if (c.evA[r]) (void)hipEventDestroy(c.evA[r]);
     if (c.evB[r]) (void)hipEventDestroy(c.evB[r]);
   }
+  for (hipEvent_t e : c.kEv) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c.stream);
   delete ctx;
   return JBGPU_OK;
@@ -730,6 +731,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
     out->iterations = s.iterations;
     out->residual = s.residual;
     out->huge_proxies = s.huge;
+    out->rows = s.rows;
   });
 }
 
@@ -796,6 +798,13 @@ int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterati
   });
 }
 
+int jbgpu_solver_set_tail(jbgpu_ctx* ctx, int32_t threshold) {
+  return guard(ctx, [&] {
+    if (threshold < 0) throw std::invalid_argument("negative tail threshold");
+    ctx->c.tailThreshold = threshold;
+  });
+}
+
 int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32_t* n) {
   return guard(ctx, [&] {
     const auto& h = ctx->c.residHist;
@@ -851,6 +860,30 @@ int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on) {
       c.pending[r] = false;
     }
     c.timedSteps = 0;
+  });
+}
+
+int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every) {
+  return guard(ctx, [&] {
+    if (every < 0) throw std::invalid_argument("negative sampling period");
+    Ctx& c = ctx->c;
+    c.kernelEvery = every;
+    for (int k = 0; k < 2; k++) {
+      c.kMs[k] = 0;
+      c.kLaunches[k] = 0;
+      c.kRows[k] = 0;
+    }
+  });
+}
+
+int jbgpu_timing_kernels_get(jbgpu_ctx* ctx, double* ms2, int64_t* launches2, int64_t* rows2) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    for (int k = 0; k < 2; k++) {
+      if (ms2) ms2[k] = c.kMs[k];
+      if (launches2) launches2[k] = c.kLaunches[k];
+      if (rows2) rows2[k] = c.kRows[k];
+    }
   });
 }
 

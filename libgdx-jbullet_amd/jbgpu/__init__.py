@@ -40,7 +40,7 @@ class Config(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64), ("manifolds", C.c_int64), ("colors", C.c_int64),
                 ("epa_calls", C.c_int64), ("epa_fails", C.c_int64), ("iterations", C.c_int64),
-                ("residual", C.c_float), ("huge_proxies", C.c_int64)]
+                ("residual", C.c_float), ("huge_proxies", C.c_int64), ("rows", C.c_int64)]
 
 
 SYMBOLS = {
@@ -74,6 +74,7 @@ SYMBOLS = {
     "jbgpu_solver_get_features": (C.c_int, [_vp, _f, C.c_int64]),
     "jbgpu_solver_get_impulses": (C.c_int, [_vp, _f, C.c_int64]),
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
+    "jbgpu_solver_set_tail": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
     "jbgpu_mlp_get_params": (C.c_int, [_vp, _f, _f]),
@@ -83,6 +84,8 @@ SYMBOLS = {
     "jbgpu_solver_use_mlp": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_enable": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_get": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int64)]),
+    "jbgpu_timing_kernels": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_timing_kernels_get": (C.c_int, [_vp, _vp, _vp, _vp]),
 }
 
 
@@ -344,6 +347,9 @@ class Context:
         self._ck(load().jbgpu_solver_set_params(self.h, float(tolerance), int(max_iterations), int(num_iterations)),
                  "solver_set_params")
 
+    def solver_set_tail(self, threshold):
+        self._ck(load().jbgpu_solver_set_tail(self.h, int(threshold)), "solver_set_tail")
+
     def residual_history(self):
         out = np.zeros(1024, np.float32)
         n = C.c_int32(0)
@@ -353,6 +359,18 @@ class Context:
     # ---- measurement
     def timing(self, on=True):
         self._ck(load().jbgpu_timing_enable(self.h, int(on)), "timing_enable")
+
+    def timing_kernels(self, every):
+        """Time each solver sweep launch by its own events on every `every`-th timed step (0 = off)."""
+        self._ck(load().jbgpu_timing_kernels(self.h, int(every)), "timing_kernels")
+
+    def timing_kernels_get(self):
+        ms = np.zeros(2, np.float64)
+        ln = np.zeros(2, np.int64)
+        rows = np.zeros(2, np.int64)
+        self._ck(load().jbgpu_timing_kernels_get(self.h, _p(ms), _p(ln), _p(rows)), "timing_kernels_get")
+        return {name: dict(ms=float(ms[i]), launches=int(ln[i]), rows=int(rows[i]))
+                for i, name in enumerate(("normal", "friction"))}
 
     def timing_get(self):
         ms = np.zeros(TIMER_REGIONS, np.float64)

@@ -98,9 +98,12 @@ def _compare_world(ctx, orc, tag):
         raise AssertionError(f"{tag}: {len(bad)} manifolds differ, first pair index {bad[0]}")
 
 
-def test_world_boxes_grid_step_parity():
+@pytest.mark.parametrize("tail", [1024, 0])
+def test_world_boxes_grid_step_parity(tail):
+    """tail: trailing colours of at most this many manifolds are swept in one workgroup (0: one launch each)."""
     sc = scenes.boxes_grid(seed=1)
     ctx, orc = _world_pair(sc)
+    ctx.solver_set_tail(tail)
     for step in range(90):
         _sync_and_step(ctx, orc)
         st = ctx.stats()
