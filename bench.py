@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-broadphase", action="store_true")
+    ap.add_argument("--bp-frames", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -123,6 +125,8 @@ def main():
         out["batched"] = batched(args)
     if not args.no_learned:
         out["learned"] = learned_warm_start(args)
+    if not args.no_broadphase:
+        out["broadphase_1m"] = broadphase_1m(args)
     print(json.dumps(out))
 
 
@@ -188,6 +192,41 @@ def batched(args):
             "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
             round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
             "last_step_colors": st["colors"]}
+
+
+def broadphase_1m(args):
+    """Broadphase stress: 1M AABBs (centres U[0,1000)^3, lognormal half extents, constant velocities); each
+    frame's AABBs are already resident in HBM; a step = setAabb of every proxy + calculateOverlappingPairs
+    (sorted pair keys on the device)."""
+    import torch
+    import jbgpu
+    from jbgpu import scenes
+    n = 1_000_000
+    c, h, v = scenes.aabb_cloud(seed=args.seed, n=n)
+    first = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    frames = np.stack([scenes.aabb_cloud_step(c, h, v) for _ in range(args.bp_frames)])
+    dev = torch.from_numpy(frames).to("cuda")
+    torch.cuda.synchronize()
+    out = {"aabbs": n, "frames": args.bp_frames}
+    for name, mode in (("dbvt", jbgpu.BP_DBVT), ("sap32", jbgpu.BP_SAP32)):
+        ctx = jbgpu.Context(bp_mode=mode, world_min=(-100.0,) * 3, world_max=(1100.0,) * 3)
+        ctx.bp_create_proxies(first)
+        for k in range(3):  # untimed: first frames build the stored state
+            ctx.bp_set_aabbs_device(dev[k].data_ptr(), n)
+            ctx.bp_calculate_overlapping_pairs()
+        pairs = 0
+        t0 = time.perf_counter()
+        for k in range(3, args.bp_frames):
+            ctx.bp_set_aabbs_device(dev[k].data_ptr(), n)
+            pairs += ctx.bp_calculate_overlapping_pairs()
+        secs = time.perf_counter() - t0
+        steps = args.bp_frames - 3
+        ctx.close()
+        out[name] = {"ms_per_step": round(1e3 * secs / steps, 3), "pairs_per_step": pairs // steps,
+                     "proxies_per_s": round(n * steps / secs, 1), "pairs_per_s": round(pairs / secs, 1)}
+        log(f"[bench] broadphase 1M {name}: {out[name]}")
+    del dev
+    return out
 
 
 def learned_warm_start(args):

@@ -175,8 +175,10 @@ int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 /* Accumulated time (ms, HIP events recorded on the context stream around the launches) per
  * region since enabling, JBGPU_TIMER_REGIONS entries: 0 predict, 1 setAabb, 2 BVH traversal,
  * 3 GJK, 4 manifold update, 5 solver setup, 6 solver iterations, 7 integrate, 8 whole step,
- * 9 EPA, 10 pair remap, 11 MLP inference; launches[r] counts the kernel launches inside region r. */
-#define JBGPU_TIMER_REGIONS 12
+ * 9 EPA, 10 pair remap, 11 MLP inference, 12 BVH build (bounds, Morton sort, build, node boxes),
+ * 13 pair-key sort, 14 narrowphase binning, 15 colouring (host waits included);
+ * launches[r] counts the kernel launches inside region r. */
+#define JBGPU_TIMER_REGIONS 16
 int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
 /* Kernel-level timing of the solver sweeps while timing is enabled: on every `every`-th timed step

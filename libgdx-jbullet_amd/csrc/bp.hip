@@ -366,43 +366,71 @@ __device__ __forceinline__ bool boxOverlap(const float4& amn, const float4& amx,
          (amx.z >= bmn.z);
 }
 
-__global__ __launch_bounds__(256) void kTraverse(int n, const int* __restrict__ order, const float* __restrict__ leafBox,
-                                                 const float* __restrict__ nodeBox, const int* __restrict__ child,
-                                                 const int* __restrict__ last, PairTest T, int bits,
-                                                 unsigned long long* counter, uint64_t* out, int64_t cap, int* err) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || n < 2) return;
-  const float4 qmn = ((const float4*)leafBox)[2 * s];
-  const float4 qmx = ((const float4*)leafBox)[2 * s + 1];
-  const int pa = order[s];
-  int stack[64];
-  int sp = 0;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    int node = stack[--sp];
-    for (int c = 0; c < 2; c++) {
-      int ch = child[2 * node + c];
-      if (ch < 0) {
-        int t = -ch - 1;
-        if (t > s) {
-          const float4 bmn = ((const float4*)leafBox)[2 * t];
-          const float4 bmx = ((const float4*)leafBox)[2 * t + 1];
-          if (boxOverlap(qmn, qmx, bmn, bmx)) {
-            int pb = order[t];
-            if (exactOverlap(T, pa, pb)) emit(pa, pb, bits, counter, out, cap);
+// Pairs found by a block are first collected in its staging area (slots from an LDS counter), then copied
+// to the output behind one global reservation per block; only staging overflow takes the global counter.
+constexpr int kTraverseBlock = 256;
+constexpr int kStagePerBlock = 16 * kTraverseBlock;
+__global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ order,
+                                                            const float* __restrict__ leafBox,
+                                                            const float* __restrict__ nodeBox,
+                                                            const int* __restrict__ child, const int* __restrict__ last,
+                                                            PairTest T, int bits, unsigned long long* counter,
+                                                            uint64_t* out, int64_t cap, int* err,
+                                                            uint64_t* __restrict__ stage) {
+  __shared__ unsigned int sCount;
+  __shared__ unsigned long long sBase;
+  if (threadIdx.x == 0) sCount = 0;
+  __syncthreads();
+  uint64_t* myStage = stage + (size_t)blockIdx.x * kStagePerBlock;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && n >= 2) {
+    const float4 qmn = ((const float4*)leafBox)[2 * s];
+    const float4 qmx = ((const float4*)leafBox)[2 * s + 1];
+    const int pa = order[s];
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      int node = stack[--sp];
+      for (int c = 0; c < 2; c++) {
+        int ch = child[2 * node + c];
+        if (ch < 0) {
+          int t = -ch - 1;
+          if (t > s) {
+            const float4 bmn = ((const float4*)leafBox)[2 * t];
+            const float4 bmx = ((const float4*)leafBox)[2 * t + 1];
+            if (boxOverlap(qmn, qmx, bmn, bmx)) {
+              int pb = order[t];
+              if (exactOverlap(T, pa, pb)) {
+                const uint64_t lo = (uint64_t)min(pa, pb) + 1, hi = (uint64_t)max(pa, pb) + 1;
+                const unsigned slot = atomicAdd(&sCount, 1u);
+                if (slot < (unsigned)kStagePerBlock)
+                  myStage[slot] = (lo << bits) | hi;
+                else
+                  emit(pa, pb, bits, counter, out, cap);
+              }
+            }
           }
-        }
-      } else if (last[ch] > s) {
-        const float4 bmn = ((const float4*)nodeBox)[2 * ch];
-        const float4 bmx = ((const float4*)nodeBox)[2 * ch + 1];
-        if (boxOverlap(qmn, qmx, bmn, bmx)) {
-          if (sp < 64)
-            stack[sp++] = ch;
-          else
-            atomicOr(err, 1);
+        } else if (last[ch] > s) {
+          const float4 bmn = ((const float4*)nodeBox)[2 * ch];
+          const float4 bmx = ((const float4*)nodeBox)[2 * ch + 1];
+          if (boxOverlap(qmn, qmx, bmn, bmx)) {
+            if (sp < 64)
+              stack[sp++] = ch;
+            else
+              atomicOr(err, 1);
+          }
         }
       }
     }
+  }
+  __syncthreads();
+  const unsigned staged = min(sCount, (unsigned)kStagePerBlock);
+  if (threadIdx.x == 0) sBase = staged ? atomicAdd(counter, (unsigned long long)staged) : 0ull;
+  __syncthreads();
+  for (unsigned i = threadIdx.x; i < staged; i += kTraverseBlock) {
+    const unsigned long long k = sBase + i;
+    if ((int64_t)k < cap) out[k] = myStage[i];
   }
 }
 
@@ -520,6 +548,7 @@ void ensureProxyBuffers(Ctx& c) {
   c.nodeChild.ensure(2 * n);
   c.nodeLast.ensure(n);
   c.nodeFirst.ensure(n);
+  c.pairStage.ensure((size_t)gridFor(n, kTraverseBlock) * kStagePerBlock);
   c.chunkBox.ensure(8 * ((size_t)n / 31 + 64));
   c.flags.ensure(n);
   c.hugeList.ensure(n);
@@ -581,6 +610,7 @@ void bpFindPairs(Ctx& c) {
   hipLaunchKernelGGL(kFlagHugeList, dim3(gridFor(n)), dim3(256), 0, s, n, c.isHuge.p, c.world.p, c.hugeList.p,
                      c.hugeWorld.p, c.counter.p + 1);
   // centroid bounds and Morton codes
+  tBegin(c, T_BVH);
   hipLaunchKernelGGL(kBoundsInit, dim3(1), dim3(64), 0, s, (unsigned int*)c.bounds.p);
   hipLaunchKernelGGL(kBounds, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, (unsigned int*)c.bounds.p);
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p,
@@ -631,6 +661,7 @@ void bpFindPairs(Ctx& c) {
     hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, c.nodeFirst.p, c.nodeLast.p,
                        c.leafBox.p, c1, c2, c3, c.nodeBox.p);
   }
+  tEnd(c, T_BVH);
   PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
   for (int attempt = 0; attempt < 3; attempt++) {
     if (c.pairCap < 1024) c.pairCap = std::max<int64_t>(1024, 8 * (int64_t)n);
@@ -639,9 +670,9 @@ void bpFindPairs(Ctx& c) {
     JB_CHECK(hipMemsetAsync(c.counter.p, 0, 2 * sizeof(unsigned long long), s));
     if (attempt == 0) tBegin(c, T_TRAVERSE);
     if (nn >= 2)
-      hipLaunchKernelGGL(kTraverse, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.leafBox.p, c.nodeBox.p,
-                         c.nodeChild.p, c.nodeLast.p, T, bits, c.counter.p, c.rawKeys.p, c.pairCap,
-                         (int*)(c.counter.p + 2));
+      hipLaunchKernelGGL(kTraverse, dim3(gridFor(nn, kTraverseBlock)), dim3(kTraverseBlock), 0, s, nn, order,
+                         c.leafBox.p, c.nodeBox.p, c.nodeChild.p, c.nodeLast.p, T, bits, c.counter.p, c.rawKeys.p,
+                         c.pairCap, (int*)(c.counter.p + 2), c.pairStage.p);
     if (attempt == 0) tEnd(c, T_TRAVERSE);
     if (c.nHuge > 0)
       hipLaunchKernelGGL(kHuge, dim3(gridFor(n)), dim3(256), 0, s, n, c.nHuge, c.hugeList.p, c.hugeWorld.p,
@@ -659,6 +690,7 @@ void bpFindPairs(Ctx& c) {
     if (attempt == 2) throw std::runtime_error("broadphase pair capacity");
   }
   // sort the packed keys (2*bits significant bits)
+  tBegin(c, T_KEYSORT);
   if (c.npairs > 1) {
     size_t tb = 0;
     hipcub::DoubleBuffer<uint64_t> kk(c.rawKeys.p, c.rawKeysAlt.p);
@@ -668,6 +700,7 @@ void bpFindPairs(Ctx& c) {
     if (kk.Current() != c.rawKeys.p)
       JB_CHECK(hipMemcpyAsync(c.rawKeys.p, kk.Current(), c.npairs * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
   }
+  tEnd(c, T_KEYSORT);
   c.st.pairs = c.npairs;
 }
 

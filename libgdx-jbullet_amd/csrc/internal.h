@@ -52,7 +52,7 @@ constexpr int kEpaHe = 160;
 constexpr int kEpaFaceWords = 16;
 constexpr int kEpaSlotWords = kEpaFaces * kEpaFaceWords + kEpaVerts * 6 + kEpaHe * 4 + 8;
 constexpr int kEpaBatch = 8192;
-constexpr int kTimerRegions = 12;
+constexpr int kTimerRegions = 16;
 
 // solver rows per manifold: 4 contacts x (normal row + 2 friction rows)
 constexpr int kRowWords = 20;   // normal(3) rel1xn(3) rel2xn(3) angA(3) angB(3) jacInv pen rest fric applied
@@ -116,6 +116,7 @@ struct Ctx {
   DBuf<int> flags;       // np
   DBuf<float> bounds;    // 8
   DBuf<uint64_t> rawKeys, rawKeysAlt;
+  DBuf<uint64_t> pairStage;  // per traversal block: staged pair keys
   DBuf<unsigned long long> counter;  // misc device counters
   DBuf<uint8_t> cubTmp;
   int64_t pairCap = 0;
@@ -129,7 +130,6 @@ struct Ctx {
   int64_t binOff[5] = {0, 0, 0, 0, 0};
   DBuf<float> candC;  // contact candidates npairs*8
   DBuf<int> epaQueue;
-  DBuf<float> epaWork;
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
   // solver
   DBuf<float> rows;
@@ -187,7 +187,7 @@ struct Ctx {
 // timer regions
 enum {
   T_PREDICT = 0, T_SETAABB, T_TRAVERSE, T_GJK, T_MANIFOLD, T_SOLVE_SETUP, T_SOLVE_ITER, T_INTEGRATE, T_STEP,
-  T_EPA, T_REMAP, T_MLP
+  T_EPA, T_REMAP, T_MLP, T_BVH, T_KEYSORT, T_BIN, T_COLOR
 };
 inline void tBegin(Ctx& c, int r) {
   if (c.timing) (void)hipEventRecord(c.evA[r], c.stream);

@@ -943,13 +943,16 @@ struct EpaSolver {
 };
 
 // one thread per queued pair; the GJK partial result is merged as in GjkPairDetector.java:263-303
+// one EPA job per workgroup: its polytope workspace (faces, vertices, hash entries) lives in the
+// workgroup's LDS; the job itself is sequential and runs on the first lane
 __global__ __launch_bounds__(64) void kEpa(int cnt, int base, const int* __restrict__ queue,
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                                            const float* __restrict__ verts, const float* __restrict__ props,
                                            const float* __restrict__ tr, const float* __restrict__ epaState,
-                                           float* cand, float* work, unsigned long long* fails) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
+                                           float* cand, unsigned long long* fails) {
+  __shared__ float slot[kEpaSlotWords];
+  const int k = blockIdx.x;
+  if (k >= cnt || threadIdx.x != 0) return;
   int i = queue[base + k];
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
@@ -963,7 +966,6 @@ __global__ __launch_bounds__(64) void kEpa(int cnt, int base, const int* __restr
   S.pos[1] = sub(tb.o, off);
   S.verts = verts;
   S.margin = 0.0f + EPA_accuracy;
-  float* slot = work + (size_t)k * kEpaSlotWords;
   S.W.face = slot;
   S.W.vtx = slot + kEpaFaces * kEpaFaceWords;
   S.W.he = S.W.vtx + kEpaVerts * 6;
@@ -1165,6 +1167,7 @@ void npDispatchAll(Ctx& c) {
   const uint64_t* keys = c.keys[c.cur].p;
   float* man = c.man[c.cur].p;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+  tBegin(c, T_BIN);
   hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p, c.typeKey.p, c.perm.p,
                      c.counter.p);
   size_t tb = 0;
@@ -1172,6 +1175,7 @@ void npDispatchAll(Ctx& c) {
   JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, 2, s));
   c.cubTmp.ensure(tb);
   JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, 2, s));
+  tEnd(c, T_BIN);
   unsigned long long cnt[4];
   JB_CHECK(hipMemcpyAsync(cnt, c.counter.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
@@ -1197,12 +1201,11 @@ void npDispatchAll(Ctx& c) {
   JB_CHECK(hipStreamSynchronize(s));
   c.st.epaCalls = (int64_t)nepa;
   if (nepa) {
-    c.epaWork.ensure((size_t)kEpaSlotWords * std::min<int64_t>((int64_t)nepa, kEpaBatch));
     tBegin(c, T_EPA);
     for (int64_t b = 0; b < (int64_t)nepa; b += kEpaBatch) {
       int m = (int)std::min<int64_t>(kEpaBatch, (int64_t)nepa - b);
-      hipLaunchKernelGGL(kEpa, dim3(gridFor(m, 64)), dim3(64), 0, s, m, (int)b, c.epaQueue.p, keys, c.shapes.p,
-                         c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p, c.epaWork.p, c.counter.p + 2);
+      hipLaunchKernelGGL(kEpa, dim3(m), dim3(64), 0, s, m, (int)b, c.epaQueue.p, keys, c.shapes.p, c.verts.p,
+                         c.props.p, c.tr.p, c.epaState.p, c.candC.p, c.counter.p + 2);
     }
     tEnd(c, T_EPA);
   }

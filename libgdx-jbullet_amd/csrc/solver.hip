@@ -624,6 +624,7 @@ void solverSolve(Ctx& c, float dt) {
   c.activeBodies.ensure(na);
   c.bodyPrio.ensure(c.nb);
   c.bodyMask.ensure(c.nb);
+  tBegin(c, T_COLOR);
   hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
@@ -670,6 +671,7 @@ void solverSolve(Ctx& c, float dt) {
   c.colorOff.push_back(na);
   c.ncolors = (int)c.colorOff.size() - 1;
   c.st.colors = c.ncolors;
+  tEnd(c, T_COLOR);
   // setup: rows for every slot, then the warm-start impulses colour by colour
   tBegin(c, T_SOLVE_SETUP);
   c.hdr.ensure(na);

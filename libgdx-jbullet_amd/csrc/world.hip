@@ -814,6 +814,7 @@ int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32
 }
 
 // ---- learned warm start ----------------------------------------------------------------
+static_assert(jbg::kTimerRegions == JBGPU_TIMER_REGIONS, "timer regions");
 static_assert(jbg::kMlpParams == JBGPU_MLP_PARAMS && jbg::kMlpNormWords == JBGPU_MLP_NORM_WORDS, "MLP layout");
 
 int jbgpu_mlp_set_params(jbgpu_ctx* ctx, const float* params, const float* norm) {

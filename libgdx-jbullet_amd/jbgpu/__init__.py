@@ -18,9 +18,9 @@ LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "libjbgpu.so")
 SHAPE_WORDS, BODY_WORDS, PROXY_WORDS, POINT_WORDS, MANIFOLD_WORDS = 20, 32, 16, 32, 132
 BP_DBVT, BP_SAP16, BP_SAP32, BP_SIMPLE = 0, 1, 2, 3
 ORDER_REFERENCE, ORDER_COLOR, ORDER_EXPLICIT = 0, 1, 2
-TIMER_REGIONS = 12
+TIMER_REGIONS = 16
 TIMER_NAMES = ["predict", "setaabb", "traverse", "gjk", "manifold", "solver_setup", "solver_iter", "integrate",
-               "step", "epa", "remap", "mlp"]
+               "step", "epa", "remap", "mlp", "bvh", "keysort", "bin", "color"]
 MLP_PARAMS, MLP_NORM_WORDS = 9603, 38
 
 # every exported symbol of include/jbgpu.h: (restype, argtypes)

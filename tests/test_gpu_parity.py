@@ -143,3 +143,25 @@ def test_batched_worlds_step_parity():
     hi = (kg & np.uint64(0xffffffff)).astype(np.int64) - 1
     assert np.array_equal(sc.world[lo], sc.world[hi])
     ctx.close()
+
+
+@pytest.mark.parametrize("mode", ["dbvt", "sap32"])
+def test_broadphase_1m_aabbs_match_oracle(mode):
+    """Full-size broadphase stress: 1M moving AABBs, pair keys bit-exact every step (count + SHA-256)."""
+    import hashlib
+    c, h, v = scenes.aabb_cloud(seed=3, n=1_000_000)
+    boxes = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    ctx = jbgpu.Context(bp_mode=MODES[mode], world_min=WMIN, world_max=WMAX)
+    ctx.bp_create_proxies(boxes)
+    orc = pyoracle.Broadphase(MODES[mode], np.array(WMIN, np.float32), np.array(WMAX, np.float32))
+    orc.create(boxes)
+    for step in range(4):
+        boxes = scenes.aabb_cloud_step(c, h, v)
+        ctx.bp_set_aabbs(boxes)
+        orc.set_all(boxes)
+        ctx.bp_calculate_overlapping_pairs()
+        got, want = ctx.bp_pairs(), orc.step()
+        dg, do = hashlib.sha256(got.tobytes()).hexdigest()[:16], hashlib.sha256(want.tobytes()).hexdigest()[:16]
+        print(f"[bp1m {mode}] step {step}: {len(got)} pairs sha {dg} (oracle {len(want)} {do})", flush=True)
+        assert len(got) == len(want) and dg == do
+    ctx.close()
