@@ -148,6 +148,8 @@ struct Ctx {
   DBuf<float> mlpG, mlpM, mlpV, mlpLoss;  // gradient, Adam moments, loss accumulator
   DBuf<uint16_t> mlpFrag;    // per training tile: activations and gradients, fragment-ready
   DBuf<int> mActive;  // active manifold list
+  DBuf<int> island;      // island parent forest (root = smallest body index of the island)
+  DBuf<int> islandFlag;  // per root: awake, inactive; then the convergence flag
   DBuf<unsigned long long> bodyPrio;
   DBuf<unsigned long long> bodyMask;
   DBuf<int> color, colorAlt;

@@ -51,17 +51,76 @@ __device__ __forceinline__ bool isActiveState(int a) {
 
 // manifolds that produce rows: needsCollision (one body awake) and needsResponse,
 // and at least one point with distance <= 0
+__device__ __forceinline__ bool isStaticBody(const float* __restrict__ props, int i) {
+  return (ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) != 0;
+}
+
+// ---- simulation islands (SimulationIslandManager.java:57-110,120-240): connected components of the
+// non-static bodies over the broadphase pairs, as a parent forest with parent[i] <= i
+__global__ void kIslandInit(int n, int* parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) parent[i] = i;
+}
+__device__ __forceinline__ int islandRoot(const int* parent, int i) {
+  int r = parent[i];
+  while (r != parent[r]) r = parent[r];
+  return r;
+}
+// findUnions (:57-70): every pair of island-merging (non-static) objects joins their islands; the larger
+// root is hooked under the smaller, so parents only decrease and never form cycles
+__global__ void kIslandHook(int64_t np, const uint64_t* __restrict__ keys, const float* __restrict__ props,
+                            int* parent, int* changed) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int a = (int)(keys[k] >> 32) - 1, b = (int)(keys[k] & 0xffffffffu) - 1;
+  if (isStaticBody(props, a) || isStaticBody(props, b)) return;
+  int ra = islandRoot(parent, a), rb = islandRoot(parent, b);
+  if (ra == rb) return;
+  atomicMin(parent + max(ra, rb), min(ra, rb));
+  *changed = 1;
+}
+__global__ void kIslandCompress(int n, int* parent) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) parent[i] = islandRoot(parent, i);
+}
+// buildIslands (:120-240): an island whose bodies are all neither ACTIVE_TAG nor DISABLE_DEACTIVATION
+// goes to sleep; otherwise its sleeping bodies want deactivation (i.e. wake up)
+__global__ void kIslandAwake(int n, const int* __restrict__ parent, const float* __restrict__ props,
+                             const int* __restrict__ act, int* awake) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || isStaticBody(props, i)) return;
+  if (act[i] == JBGPU_ACTIVE_TAG || act[i] == JBGPU_DISABLE_DEACTIVATION) awake[parent[i]] = 1;
+}
+__global__ void kIslandApply(int n, const int* __restrict__ parent, const float* __restrict__ props,
+                             const int* __restrict__ awake, int* act, int* inactive) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || isStaticBody(props, i)) return;
+  int a = act[i];
+  const bool settable = a != JBGPU_DISABLE_DEACTIVATION && a != JBGPU_DISABLE_SIMULATION;  // setActivationState
+  if (!awake[parent[i]]) {
+    if (settable) a = JBGPU_ISLAND_SLEEPING;
+  } else if (a == JBGPU_ISLAND_SLEEPING) {
+    a = JBGPU_WANTS_DEACTIVATION;
+  }
+  act[i] = a;
+  if (!isActiveState(a)) inactive[parent[i]] = 1;  // buildAndProcessIslands skips such an island (:283-289)
+}
+
+// manifolds solved this step: response needed, not both bodies asleep, island without inactive bodies
+// (island == nullptr: deactivation disabled, every island is awake)
 __global__ void kMarkActive(int64_t npairs, const float* __restrict__ man, const int* __restrict__ ptype,
-                            const int* __restrict__ act, const float* __restrict__ props, int* flag) {
+                            const int* __restrict__ act, const float* __restrict__ props,
+                            const int* __restrict__ island, const int* __restrict__ inactive, int* flag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * i;
   int n = ldi(m), b0 = ldi(m + 1), b1 = ldi(m + 2);
   int f = 0;
   if (ptype[i] != JBGPU_PAIR_NONE && n > 0) {
-    bool s0 = (ldi(props + 8 * b0 + 7) & JBGPU_BODY_STATIC) != 0, s1 = (ldi(props + 8 * b1 + 7) & JBGPU_BODY_STATIC) != 0;
+    bool s0 = isStaticBody(props, b0), s1 = isStaticBody(props, b1);
     bool awake = (act[b0] != JBGPU_ISLAND_SLEEPING) || (act[b1] != JBGPU_ISLAND_SLEEPING);
     bool solvable = isActiveState(act[b0]) || isActiveState(act[b1]);
+    if (island && (!s0 || !s1)) solvable = !inactive[island[s0 ? b1 : b0]];
     if ((!s0 || !s1) && awake && solvable) {
       for (int k = 0; k < n; k++)
         if (m[JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * k + 15] <= 0.0f) f = 1;
@@ -600,8 +659,35 @@ void solverSolve(Ctx& c, float dt) {
   c.flags.ensure(np);
   c.mActive.ensure(np);
   c.counter.ensure(8);
-  hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p,
-                     c.flags.p);
+  const int* island = nullptr;
+  if (!c.deactivationDisabled) {  // islands decide sleeping; with deactivation disabled every body stays awake
+    const int nb = c.nb;
+    c.island.ensure(nb);
+    c.islandFlag.ensure(2 * (size_t)nb + 1);
+    int* awake = c.islandFlag.p;
+    int* inactive = awake + nb;
+    int* changed = inactive + nb;
+    hipLaunchKernelGGL(kIslandInit, dim3(gridFor(nb)), dim3(256), 0, s, nb, c.island.p);
+    for (int round = 0;; round++) {
+      JB_CHECK(hipMemsetAsync(changed, 0, sizeof(int), s));
+      hipLaunchKernelGGL(kIslandHook, dim3(gridFor(np)), dim3(256), 0, s, np, (const uint64_t*)c.keys[c.cur].p,
+                         (const float*)c.props.p, c.island.p, changed);
+      hipLaunchKernelGGL(kIslandCompress, dim3(gridFor(nb)), dim3(256), 0, s, nb, c.island.p);
+      int h = 0;
+      JB_CHECK(hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, s));
+      JB_CHECK(hipStreamSynchronize(s));
+      if (!h) break;
+      if (round > 64) throw std::runtime_error("island labelling did not converge");
+    }
+    JB_CHECK(hipMemsetAsync(awake, 0, 2 * (size_t)nb * sizeof(int), s));
+    hipLaunchKernelGGL(kIslandAwake, dim3(gridFor(nb)), dim3(256), 0, s, nb, (const int*)c.island.p,
+                       (const float*)c.props.p, (const int*)c.act.p, awake);
+    hipLaunchKernelGGL(kIslandApply, dim3(gridFor(nb)), dim3(256), 0, s, nb, (const int*)c.island.p,
+                       (const float*)c.props.p, (const int*)awake, c.act.p, inactive);
+    island = c.island.p;
+  }
+  hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p, island,
+                     island ? (const int*)(c.islandFlag.p + c.nb) : nullptr, c.flags.p);
   size_t tb = 0;
   hipcub::CountingInputIterator<int> it(0);
   JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));

@@ -281,6 +281,21 @@ def pyramids(seed=1, n_boxes=64_000, n_pyramids=32, n_spheres=2000, area=200.0):
     return b.done(dt=1 / 60, iterations=100, steps=200)
 
 
+def sleep_and_wake(n=3, layers=2, drop_height=30.0):
+    """Resting boxes that fall asleep (deactivation enabled) and a sphere that lands on them later."""
+    b = _Builder("sleep_and_wake")
+    plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
+    b.body(plane, IDENT, np.zeros(3, np.float32), 0.0)
+    box = b.shape(BOX, [0.5, 0.5, 0.5])
+    for i in range(n):
+        for j in range(n):
+            for k in range(layers):
+                b.body(box, IDENT, np.array([i * 1.5, 0.5 + k * 1.0, j * 1.5], np.float32), 1.0)
+    sphere = b.shape(SPHERE, [0.4])
+    b.body(sphere, IDENT, np.array([(n - 1) * 0.75, drop_height, (n - 1) * 0.75], np.float32), 1.0)
+    return b.done(dt=1 / 60, iterations=10, steps=400)
+
+
 def build_records(scene, builder, gravity=GRAVITY):
     """Shape and body records for ``scene``.
 

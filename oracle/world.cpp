@@ -240,12 +240,6 @@ static inline void setActivation(Body& b, int s) {  // CollisionObject.java:149-
 void World::solveConstraints() {  // DiscreteDynamicsWorld.java:620-647 -> buildAndProcessIslands
   SolverInfo info = cfg.solver;
   lastSolve = SolverStats();
-  if (cfg.solverOrder == JBGPU_ORDER_EXPLICIT) {
-    std::vector<Manifold*> ms;
-    for (int idx : explicitOrder) ms.push_back(&pairs[idx].m);
-    solveGroup(bodies, ms, info, false, rng, &lastSolve, learnedInit, features);
-    return;
-  }
   // buildIslands (SimulationIslandManager.java:120-240)
   uf.sortIslands();
   int numElem = (int)uf.id.size();
@@ -269,6 +263,12 @@ void World::solveConstraints() {  // DiscreteDynamicsWorld.java:620-647 -> build
       else if (b.activation == JBGPU_ISLAND_SLEEPING)
         setActivation(b, JBGPU_WANTS_DEACTIVATION);
     }
+  }
+  if (cfg.solverOrder == JBGPU_ORDER_EXPLICIT) {  // island sleep states above, then the caller's manifold order
+    std::vector<Manifold*> ms;
+    for (int idx : explicitOrder) ms.push_back(&pairs[idx].m);
+    solveGroup(bodies, ms, info, false, rng, &lastSolve, learnedInit, features);
+    return;
   }
   std::vector<Manifold*> im;
   for (auto& ps : pairs) {

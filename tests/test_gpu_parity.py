@@ -165,3 +165,21 @@ def test_broadphase_1m_aabbs_match_oracle(mode):
         print(f"[bp1m {mode}] step {step}: {len(got)} pairs sha {dg} (oracle {len(want)} {do})", flush=True)
         assert len(got) == len(want) and dg == do
     ctx.close()
+
+
+def test_sleeping_islands_step_parity():
+    """Deactivation enabled: island sleep/wake decisions on the device match the oracle every step."""
+    sc = scenes.sleep_and_wake()
+    shapes, verts, bodies = scenes.build_records(sc, jbgpu)
+    ctx = jbgpu.Context(num_iterations=10, deactivation_disabled=0)
+    ctx.world_create(shapes, verts, bodies)
+    orc = pyoracle.World(shapes, verts, bodies, iterations=10, order=jbgpu.ORDER_EXPLICIT, deactivation_disabled=False)
+    seen = set()
+    for step in range(400):
+        _sync_and_step(ctx, orc)
+        _compare_world(ctx, orc, f"sleep step {step}")
+        seen |= set(ctx.bodies()[:, 29].view(np.int32).tolist())
+        if step % 50 == 0:
+            print(f"[sleep] step {step}: states {sorted(seen)}", flush=True)
+    assert {1, 2, 3} <= seen
+    ctx.close()

@@ -631,3 +631,32 @@ def test_batched_world_steps_like_independent_worlds():
             sub.set_order([rank[i] for i in order if i in rank])
             sub.step()
             assert np.array_equal(sub.bodies().view(np.uint32), comb.bodies()[sel[w]].view(np.uint32)), (step, w)
+
+
+def test_islands_sleep_and_wake():
+    """Deactivation enabled: bodies at rest for more than 2 s sleep with zero velocity and stay put;
+    an active body touching a sleeping island wakes it (DiscreteDynamicsWorld.updateActivationState,
+    SimulationIslandManager.buildIslands)."""
+    sc = scenes.sleep_and_wake()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=10, deactivation_disabled=False)
+    states, pos = [], []
+    for _ in range(400):
+        w.step()
+        b = w.bodies()
+        states.append(b[:, 29].view(np.int32).copy())
+        pos.append(b[:, 9:12].copy())
+        asleep = states[-1] == 2
+        assert np.all(b[asleep, 12:18] == 0.0)
+    states = np.array(states)
+    boxes = slice(1, 19)
+    assert (states[:100, boxes] == 1).all()                       # moving at first (< 2 s of rest)
+    first_sleep = np.argmax((states[:, boxes] == 2).all(axis=1))
+    assert 100 < first_sleep < 141                                  # whole pile asleep after > 2 s
+    # the sphere's island reaches the pile: sleeping boxes are woken (ISLAND_SLEEPING -> WANTS_DEACTIVATION)
+    woken = (states[:-1, boxes] == 2) & (states[1:, boxes] == 3)
+    assert woken[first_sleep:].any()
+    for k in range(1, 400):                                         # a body asleep on both steps does not move
+        still = (states[k - 1] == 2) & (states[k] == 2)
+        assert np.array_equal(pos[k][still], pos[k - 1][still])
+    assert (states[-1, boxes] == 2).all()                           # and the pile sleeps again
