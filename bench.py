@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-broadphase", action="store_true")
     ap.add_argument("--bp-frames", type=int, default=20)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -65,6 +67,8 @@ def main():
     import jbgpu
     from jbgpu import scenes
 
+    if args.pmc_child:
+        return pmc_child(args)
     t = time.time()
     sc = scenes.mixed_pile(seed=args.seed, n=args.bodies)
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
@@ -95,6 +99,8 @@ def main():
 
     tim = {k: v for k, v in regions.items() if v[1] > 0}
     roof = roofline(ctx, args)
+    if roof is not None:
+        roof["copy_ceiling_gbs"] = round(ctx.measure_copy(1 << 30, 10), 1)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -121,13 +127,89 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, shapes, verts, bodies, args)
     ctx.close()
-    if not args.no_batched:
-        out["batched"] = batched(args)
-    if not args.no_learned:
-        out["learned"] = learned_warm_start(args)
-    if not args.no_broadphase:
-        out["broadphase_1m"] = broadphase_1m(args)
+    if roof is not None and not args.no_traffic:
+        try:
+            roof["traffic"], roof["traffic_detail"] = traffic(args)
+        except Exception as e:
+            roof["traffic"], roof["traffic_detail"] = None, f"counter passes failed: {e!r}"
+    for key, flag, fn in (("batched", args.no_batched, batched), ("learned", args.no_learned, learned_warm_start),
+                          ("broadphase_1m", args.no_broadphase, broadphase_1m)):
+        if flag:
+            continue
+        try:
+            out[key] = fn(args)
+        except Exception as e:  # an optional section must not lose the headline line
+            log(f"[bench] {key} failed: {e!r}")
+            out[key] = {"error": repr(e)}
     print(json.dumps(out))
+
+
+PMC_WARMUP, PMC_STEPS, PMC_COPY_BYTES = 150, 2, 64 << 20
+
+
+def pmc_child(args):
+    """Workload of one counter pass: the same pile for PMC_WARMUP + PMC_STEPS steps (every friction sweep launch
+    is counted, and its rows are tallied for the algorithmic bytes), then one copy of PMC_COPY_BYTES as the
+    counter's scale reference."""
+    import jbgpu
+    from jbgpu import scenes
+    sc = scenes.mixed_pile(seed=args.seed, n=args.bodies)
+    shapes, verts, bodies = scenes.build_records(sc, jbgpu)
+    ctx = jbgpu.Context(num_iterations=args.iterations)
+    ctx.world_create(shapes, verts, bodies)
+    ctx.timing(True)
+    ctx.timing_kernels(1)
+    ctx.step(n=PMC_WARMUP + PMC_STEPS)
+    ks = ctx.timing_kernels_get()["friction"]
+    ctx.measure_copy(PMC_COPY_BYTES, 1)
+    ctx.close()
+    print(json.dumps({"friction_rows": ks["rows"], "friction_launches": ks["launches"]}))
+
+
+def traffic(args):
+    """HBM bytes per kFriction launch from two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by the ratio of known bytes to what the same counter reads for the plain copy probe
+    (kCopyProbe, PMC_COPY_BYTES read and written) in the same pass."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="jbgpu_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--kernel-include-regex", r"kFriction\(|kCopyProbe",
+               "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--bodies",
+               str(args.bodies), "--iterations", str(args.iterations), "--seed", str(args.seed)]
+        t = time.time()
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        log(f"[bench] counter pass {counter}: rc {p.returncode} in {time.time() - t:.1f}s")
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("results.db")]
+        lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+        if p.returncode != 0 or not dbs or not lines:
+            return None, f"counter pass {counter} failed (rc {p.returncode})"
+        child = json.loads(lines[-1])
+        cur = sqlite3.connect(dbs[0]).cursor()
+        q = "select avg(value), count(*) from counters_collection where counter_name = ? and kernel_name like ?"
+        kf, nkf = cur.execute(q, (counter, "%kFriction(%")).fetchone()
+        cp, ncp = cur.execute(q, (counter, "%kCopyProbe%")).fetchone()
+        if not nkf or not ncp or not cp:
+            return None, f"counter pass {counter}: no dispatches recorded"
+        scale = PMC_COPY_BYTES / (cp * 1024.0)
+        res[counter] = dict(kib_per_dispatch=kf, dispatches=nkf, copy_kib=cp, scale=scale, child=child)
+        shutil.rmtree(d, ignore_errors=True)
+    fetch = res["FETCH_SIZE"]["kib_per_dispatch"] * 1024.0 * res["FETCH_SIZE"]["scale"]
+    write = res["WRITE_SIZE"]["kib_per_dispatch"] * 1024.0 * res["WRITE_SIZE"]["scale"]
+    child = res["FETCH_SIZE"]["child"]
+    algo = ROW_BYTES * child["friction_rows"] / max(child["friction_launches"], 1)
+    detail = {"workload": f"same pile, every kFriction launch of steps 1-{PMC_WARMUP + PMC_STEPS}",
+              "fetch_bytes": int(fetch),
+              "write_bytes": int(write), "algorithmic_bytes": int(algo), "traffic_over_algorithmic":
+              round((fetch + write) / max(algo, 1.0), 3), "scale_fetch": round(res["FETCH_SIZE"]["scale"], 4),
+              "scale_write": round(res["WRITE_SIZE"]["scale"], 4), "dispatches": res["FETCH_SIZE"]["dispatches"]}
+    return int(fetch + write), detail
 
 
 def autotune(ctx, dt, steps=8):
@@ -198,34 +280,34 @@ def broadphase_1m(args):
     """Broadphase stress: 1M AABBs (centres U[0,1000)^3, lognormal half extents, constant velocities); each
     frame's AABBs are already resident in HBM; a step = setAabb of every proxy + calculateOverlappingPairs
     (sorted pair keys on the device)."""
-    import torch
     import jbgpu
     from jbgpu import scenes
     n = 1_000_000
     c, h, v = scenes.aabb_cloud(seed=args.seed, n=n)
     first = np.concatenate([c - h, c + h], 1).astype(np.float32)
     frames = np.stack([scenes.aabb_cloud_step(c, h, v) for _ in range(args.bp_frames)])
-    dev = torch.from_numpy(frames).to("cuda")
-    torch.cuda.synchronize()
     out = {"aabbs": n, "frames": args.bp_frames}
     for name, mode in (("dbvt", jbgpu.BP_DBVT), ("sap32", jbgpu.BP_SAP32)):
         ctx = jbgpu.Context(bp_mode=mode, world_min=(-100.0,) * 3, world_max=(1100.0,) * 3)
+        dev = ctx.buffer_alloc(frames.nbytes)
+        ctx.buffer_upload(dev, frames)
+        frame_bytes = frames[0].nbytes
         ctx.bp_create_proxies(first)
         for k in range(3):  # untimed: first frames build the stored state
-            ctx.bp_set_aabbs_device(dev[k].data_ptr(), n)
+            ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
             ctx.bp_calculate_overlapping_pairs()
         pairs = 0
         t0 = time.perf_counter()
         for k in range(3, args.bp_frames):
-            ctx.bp_set_aabbs_device(dev[k].data_ptr(), n)
+            ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
             pairs += ctx.bp_calculate_overlapping_pairs()
         secs = time.perf_counter() - t0
         steps = args.bp_frames - 3
+        ctx.buffer_free(dev)
         ctx.close()
         out[name] = {"ms_per_step": round(1e3 * secs / steps, 3), "pairs_per_step": pairs // steps,
                      "proxies_per_s": round(n * steps / secs, 1), "pairs_per_s": round(pairs / secs, 1)}
         log(f"[bench] broadphase 1M {name}: {out[name]}")
-    del dev
     return out
 
 

@@ -186,6 +186,14 @@ int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* ste
  * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps.
  * [0] normal rows, [1] friction rows: summed ms, timed launches, rows processed by them. */
 int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every);
+/* Device memory for callers that keep their own inputs resident in HBM (e.g. AABBs for
+ * jbgpu_bp_set_aabbs with on_device = 1) without linking another GPU runtime; freed by the caller. */
+int jbgpu_buffer_alloc(jbgpu_ctx* ctx, int64_t bytes, void** dev);
+int jbgpu_buffer_upload(jbgpu_ctx* ctx, void* dev, const void* host, int64_t bytes);
+int jbgpu_buffer_free(jbgpu_ctx* ctx, void* dev);
+/* Device copy of `bytes` (read + write counted) repeated `reps` times on the context stream: the
+ * measured HBM copy ceiling in GB/s (no reference counterpart; a measurement aid). */
+int jbgpu_measure_copy(jbgpu_ctx* ctx, int64_t bytes, int32_t reps, double* gbs);
 int jbgpu_timing_kernels_get(jbgpu_ctx* ctx, double* ms2, int64_t* launches2, int64_t* rows2);
 
 #ifdef __cplusplus

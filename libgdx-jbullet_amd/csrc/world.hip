@@ -95,6 +95,12 @@ __global__ void kIntegrate(int n, float dt, float* tr, const float* __restrict__
   T[11] = o.z;
 }
 
+// plain device copy of a known size (bandwidth ceiling and counter scale reference)
+__global__ __launch_bounds__(256) void kCopyProbe(int64_t n4, const float4* __restrict__ src, float4* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void kActivation(int n, float dt, int deactDisabled, const float* __restrict__ props, int* act,
                             float* deact, float* vel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -861,6 +867,56 @@ int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on) {
       c.pending[r] = false;
     }
     c.timedSteps = 0;
+  });
+}
+
+int jbgpu_buffer_alloc(jbgpu_ctx* ctx, int64_t bytes, void** dev) {
+  return guard(ctx, [&] {
+    if (bytes <= 0 || !dev) throw std::invalid_argument("bad buffer size");
+    *dev = nullptr;
+    JB_CHECK(hipMalloc(dev, (size_t)bytes));
+  });
+}
+
+int jbgpu_buffer_upload(jbgpu_ctx* ctx, void* dev, const void* host, int64_t bytes) {
+  return guard(ctx, [&] {
+    if (!dev || !host || bytes < 0) throw std::invalid_argument("bad upload");
+    JB_CHECK(hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    JB_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int jbgpu_buffer_free(jbgpu_ctx* ctx, void* dev) {
+  return guard(ctx, [&] {
+    JB_CHECK(hipStreamSynchronize(ctx->c.stream));
+    if (dev) JB_CHECK(hipFree(dev));
+  });
+}
+
+int jbgpu_measure_copy(jbgpu_ctx* ctx, int64_t bytes, int32_t reps, double* gbs) {
+  return guard(ctx, [&] {
+    if (bytes < 16 || reps < 1 || !gbs) throw std::invalid_argument("bad copy probe arguments");
+    Ctx& c = ctx->c;
+    const int64_t n4 = bytes / 16;
+    DBuf<float4> a, b;
+    a.ensure(n4);
+    b.ensure(n4);
+    JB_CHECK(hipMemsetAsync(a.p, 0, n4 * 16, c.stream));
+    hipEvent_t e0, e1;
+    JB_CHECK(hipEventCreate(&e0));
+    JB_CHECK(hipEventCreate(&e1));
+    const int grid = 8 * 256 * 4;  // four workgroups per SIMD-pair across the 256 CUs, grid-stride
+    hipLaunchKernelGGL(kCopyProbe, dim3(grid), dim3(256), 0, c.stream, n4, (const float4*)a.p, b.p);  // warm
+    JB_CHECK(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(kCopyProbe, dim3(grid), dim3(256), 0, c.stream, n4, (const float4*)a.p, b.p);
+    JB_CHECK(hipEventRecord(e1, c.stream));
+    JB_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    JB_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * 16.0 * (double)n4 * reps / (ms * 1e-3) / 1e9;  // bytes read + written
   });
 }
 

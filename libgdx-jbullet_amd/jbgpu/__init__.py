@@ -85,6 +85,10 @@ SYMBOLS = {
     "jbgpu_timing_enable": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_get": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int64)]),
     "jbgpu_timing_kernels": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_measure_copy": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "jbgpu_buffer_alloc": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp)]),
+    "jbgpu_buffer_upload": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "jbgpu_buffer_free": (C.c_int, [_vp, _vp]),
     "jbgpu_timing_kernels_get": (C.c_int, [_vp, _vp, _vp, _vp]),
 }
 
@@ -359,6 +363,26 @@ class Context:
     # ---- measurement
     def timing(self, on=True):
         self._ck(load().jbgpu_timing_enable(self.h, int(on)), "timing_enable")
+
+    def buffer_alloc(self, nbytes):
+        """Device buffer owned by the caller (free with buffer_free); returns the device address."""
+        p = C.c_void_p()
+        self._ck(load().jbgpu_buffer_alloc(self.h, int(nbytes), C.byref(p)), "buffer_alloc")
+        return p.value
+
+    def buffer_upload(self, dev, host):
+        host = np.ascontiguousarray(host)
+        self._ck(load().jbgpu_buffer_upload(self.h, C.c_void_p(dev), host.ctypes.data_as(C.c_void_p), host.nbytes),
+                 "buffer_upload")
+
+    def buffer_free(self, dev):
+        self._ck(load().jbgpu_buffer_free(self.h, C.c_void_p(dev)), "buffer_free")
+
+    def measure_copy(self, nbytes=1 << 30, reps=10):
+        """Device copy ceiling (GB/s, read + write) of `nbytes` copied `reps` times."""
+        g = C.c_double(0)
+        self._ck(load().jbgpu_measure_copy(self.h, int(nbytes), int(reps), C.byref(g)), "measure_copy")
+        return g.value
 
     def timing_kernels(self, every):
         """Time each solver sweep launch by its own events on every `every`-th timed step (0 = off)."""

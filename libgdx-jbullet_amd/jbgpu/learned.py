@@ -77,3 +77,52 @@ def evaluate(params, norm, seed, scene_kw, settle=90, steps=30, tol=1e-4, cap=10
     out["iteration_ratio"] = out["learned"]["iterations"] / max(out["plain"]["iterations"], 1e-9)
     out["time_ratio"] = out["learned"]["solve_ms"] / max(out["plain"]["solve_ms"], 1e-9)
     return out
+
+
+def main():
+    """Full protocol: pyramids of 64k boxes + 2000 spheres; training contacts from seeds 1000..; evaluation on
+    seeds 1-3 to a 1e-4 residual (cap 100 iterations).  Prints one JSON line."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--boxes", type=int, default=64_000)
+    ap.add_argument("--pyramids", type=int, default=32)
+    ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--area", type=float, default=200.0)
+    ap.add_argument("--train-seeds", type=int, default=8)
+    ap.add_argument("--train-contacts", type=int, default=200_000)
+    ap.add_argument("--settle", type=int, default=90)
+    ap.add_argument("--eval-steps", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=40)
+    args = ap.parse_args()
+    kw = dict(n_boxes=args.boxes, n_pyramids=args.pyramids, n_spheres=args.spheres, area=args.area)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    t = time.time()
+    x, y = collect(range(1000, 1000 + args.train_seeds), kw, settle=args.settle, samples=3, every=10, log=log)
+    if len(x) > args.train_contacts:
+        pick = np.random.default_rng(0).choice(len(x), args.train_contacts, replace=False)
+        x, y = x[pick], y[pick]
+    t_collect = time.time() - t
+    ctx = jbgpu.Context()
+    t = time.time()
+    losses = ctx.mlp_train(x, y, epochs=args.epochs, batch=4096, lr=2e-3, seed=0)
+    t_train = time.time() - t
+    params, norm = ctx.mlp_get_params()
+    ctx.close()
+    log(f"trained on {len(x)} contacts: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {t_train:.1f}s")
+    evals = {s: evaluate(params, norm, s, kw, settle=args.settle, steps=args.eval_steps) for s in (1, 2, 3)}
+    out = {"scene": kw, "training_contacts": int(len(x)), "collect_s": round(t_collect, 1),
+           "train_s": round(t_train, 2), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "eval": evals,
+           "iteration_ratio_mean": float(np.mean([e["iteration_ratio"] for e in evals.values()])),
+           "time_ratio_mean": float(np.mean([e["time_ratio"] for e in evals.values()]))}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
