@@ -264,16 +264,19 @@ def batched(args):
     ctx = jbgpu.Context(num_iterations=10)
     ctx.world_create(shapes, verts, bodies, world_ids=sc.world)
     ctx.step(n=max(args.batched_steps // 2, 1))
+    ctx.timing(True)
     t0 = time.perf_counter()
     ctx.step(n=args.batched_steps)
     secs = time.perf_counter() - t0
     st = ctx.stats()
+    regions, nsteps = ctx.timing_get()
     ctx.close()
     return {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": args.batched_steps,
             "warmup": max(args.batched_steps // 2, 1), "ms_per_step": round(1e3 * secs / args.batched_steps, 3),
             "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
             round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
-            "last_step_colors": st["colors"]}
+            "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
+            "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0}}
 
 
 def broadphase_1m(args):

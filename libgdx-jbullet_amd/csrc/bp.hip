@@ -204,13 +204,15 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   return v;
 }
 
+// sort key of proxy i: world id above a 30-bit Morton code of its candidate-box centre, so that every
+// world's leaves are contiguous and its subtrees separate; huge proxies carry all ones and sort last
 __global__ void kMorton(int n, const float* __restrict__ cand, const int* __restrict__ isHuge,
-                        const unsigned int* __restrict__ b, uint32_t* code, int* idx) {
+                        const int* __restrict__ world, const unsigned int* __restrict__ b, uint64_t* code, int* idx) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   idx[i] = i;
   if (isHuge[i]) {
-    code[i] = 0xffffffffu;
+    code[i] = ~0ull;
     return;
   }
   const float* C = cand + 8 * i;
@@ -224,27 +226,34 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
     if (!(u == u)) u = 0.0f;
     q3[k] = min((uint32_t)(u * 1024.0f), 1023u);
   }
-  code[i] = (expand10(q3[0]) << 2) | (expand10(q3[1]) << 1) | expand10(q3[2]);
+  const uint32_t m = (expand10(q3[0]) << 2) | (expand10(q3[1]) << 1) | expand10(q3[2]);
+  code[i] = ((uint64_t)(uint32_t)world[i] << 32) | m;
 }
 
-__global__ void kGatherLeaves(int n, const int* __restrict__ order, const float* __restrict__ cand, float* leafBox) {
+// sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
+__global__ void kGatherLeaves(int n, const int* __restrict__ order, const float* __restrict__ cand,
+                              const int* __restrict__ world, float* leafBox) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const float4* src = (const float4*)(cand + 8 * order[s]);
+  const int o = order[s];
+  const float4* src = (const float4*)(cand + 8 * o);
   float4* dst = (float4*)(leafBox + 8 * s);
-  dst[0] = src[0];
-  dst[1] = src[1];
+  float4 a = src[0], b = src[1];
+  a.w = __int_as_float(world[o]);
+  b.w = a.w;
+  dst[0] = a;
+  dst[1] = b;
 }
 
-__device__ __forceinline__ int deltaK(const uint32_t* code, int n, int i, int j) {
+__device__ __forceinline__ int deltaK(const uint64_t* code, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
-  uint32_t a = code[i], b = code[j];
-  if (a == b) return 32 + __clz((unsigned)(i ^ j));
-  return __clz(a ^ b);
+  uint64_t a = code[i], b = code[j];
+  if (a == b) return 64 + __clz((unsigned)(i ^ j));
+  return __clzll((long long)(a ^ b));
 }
 
 // Karras (2012) hierarchy: internal node i in [0, n-2]; child < 0 encodes leaf -(child+1)
-__global__ void kBuild(int n, const uint32_t* __restrict__ code, int* child, int* first, int* last) {
+__global__ void kBuild(int n, const uint64_t* __restrict__ code, int* child, int* first, int* last) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (deltaK(code, n, i, i + 1) - deltaK(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -326,8 +335,10 @@ __global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__
     e >>= 5;
   }
   while (a < e) B.add(c3 + 8 * (a++));
-  ((float4*)nodeBox)[2 * i] = make_float4(B.mn.x, B.mn.y, B.mn.z, 0.0f);
-  ((float4*)nodeBox)[2 * i + 1] = make_float4(B.mx.x, B.mx.y, B.mx.z, 0.0f);
+  // leaves are sorted by world first: the node's worlds are those of its first and last leaf
+  const float wlo = leafBox[8 * first[i] + 3], whi = leafBox[8 * last[i] + 3];
+  ((float4*)nodeBox)[2 * i] = make_float4(B.mn.x, B.mn.y, B.mn.z, wlo);
+  ((float4*)nodeBox)[2 * i + 1] = make_float4(B.mx.x, B.mx.y, B.mx.z, whi);
 }
 
 struct PairTest {
@@ -386,6 +397,7 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
   if (s < n && n >= 2) {
     const float4 qmn = ((const float4*)leafBox)[2 * s];
     const float4 qmx = ((const float4*)leafBox)[2 * s + 1];
+    const int qw = __float_as_int(qmn.w);
     const int pa = order[s];
     int stack[64];
     int sp = 0;
@@ -414,7 +426,8 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
         } else if (last[ch] > s) {
           const float4 bmn = ((const float4*)nodeBox)[2 * ch];
           const float4 bmx = ((const float4*)nodeBox)[2 * ch + 1];
-          if (boxOverlap(qmn, qmx, bmn, bmx)) {
+          const bool worldIn = qw >= __float_as_int(bmn.w) && qw <= __float_as_int(bmx.w);
+          if (worldIn && boxOverlap(qmn, qmx, bmn, bmx)) {
             if (sp < 64)
               stack[sp++] = ch;
             else
@@ -613,21 +626,22 @@ void bpFindPairs(Ctx& c) {
   tBegin(c, T_BVH);
   hipLaunchKernelGGL(kBoundsInit, dim3(1), dim3(64), 0, s, (unsigned int*)c.bounds.p);
   hipLaunchKernelGGL(kBounds, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, (unsigned int*)c.bounds.p);
-  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p,
+  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
                      (const unsigned int*)c.bounds.p, c.morton.p, c.order.p);
   unsigned long long hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, c.counter.p + 1, sizeof(hcount), hipMemcpyDeviceToHost, s));
   // sort leaves by Morton code (huge proxies carry 0xffffffff and sort last)
   size_t tmpBytes = 0;
-  hipcub::DoubleBuffer<uint32_t> dk(c.morton.p, c.mortonAlt.p);
+  hipcub::DoubleBuffer<uint64_t> dk(c.morton.p, c.mortonAlt.p);
   hipcub::DoubleBuffer<int> dv(c.order.p, c.orderAlt.p);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, 32, s));
+  const int keyBits = 32 + c.worldBits;  // one world: the 32-bit Morton sort
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, keyBits, s));
   c.cubTmp.ensure(tmpBytes);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, 32, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, keyBits, s));
   JB_CHECK(hipStreamSynchronize(s));
   c.nHuge = (int)hcount;
   c.st.huge = c.nHuge;
-  const uint32_t* code = dk.Current();
+  const uint64_t* code = dk.Current();
   const int* order = dv.Current();
   const int nn = n - c.nHuge;  // leaves in the tree
   if (c.nHuge > 0) {  // sort the huge list by (world, index) for the per-world lookup
@@ -647,7 +661,8 @@ void bpFindPairs(Ctx& c) {
       JB_CHECK(hipMemcpyAsync(c.hugeList.p, hv.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
   }
   if (nn >= 2) {
-    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.leafBox.p);
+    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.world.p,
+                       c.leafBox.p);
     hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeFirst.p,
                        c.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32, n3 = (n2 + 31) / 32;

@@ -102,7 +102,8 @@ struct Ctx {
   DBuf<int> filt;          // group | mask << 16
   DBuf<int> q;             // quantised edges np*6
   DBuf<float> cand;        // candidate boxes np*6 (float4 pairs: min xyz pad, max xyz pad -> np*8)
-  DBuf<uint32_t> morton, mortonAlt;
+  DBuf<uint64_t> morton, mortonAlt;  // sort keys: world id << 32 | Morton code
+  int worldBits = 0;                  // bits of the largest world id (0: a single world)
   DBuf<int> order, orderAlt;  // sorted leaf -> proxy
   DBuf<int> isHuge;
   DBuf<int> hugeList, hugeListAlt, hugeWorld, hugeWorldAlt;
@@ -133,8 +134,9 @@ struct Ctx {
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
   // solver
   DBuf<float> rows;
-  DBuf<int4> hdr;            // per colour slot: point mask, body A, body B, pair index
-  DBuf<float> nrow, frow;    // per colour slot: 4 normal rows, 8 friction rows
+  DBuf<int4> hdr;            // per colour slot: point mask | first row << 4, body A, body B, pair index
+  DBuf<int> slotCount, slotStart;  // rows per slot and their exclusive prefix sum
+  DBuf<float> nrow, frow;    // dense rows: normal row r, friction rows 2r and 2r+1
   DBuf<int2> activeBodies;   // per active manifold: body A, body B
   DBuf<int> colorStart;
   // learned warm start (16-64-64-64-3 MLP, bf16 weights, fp32 accumulate)
@@ -153,6 +155,7 @@ struct Ctx {
   DBuf<unsigned long long> bodyPrio;
   DBuf<unsigned long long> bodyMask;
   DBuf<int> color, colorAlt;
+  DBuf<int> pend, pendAlt, pendFlag;  // uncoloured manifolds between colouring batches
   DBuf<uint64_t> colorKey, colorKeyAlt;
   DBuf<int> colorOrder, colorOrderAlt;
   std::vector<int64_t> colorOff;

@@ -22,7 +22,6 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NW = 4 * kRowWords, FW = 8 * kRowWords;  // words of the normal / friction rows per slot
 constexpr int kTileFrags = 26;  // per 16-sample training tile: H0 H1x4 H2x4 H3x4 dZ1x4 dZ2x4 dZ3x4 dZ4
 
 // fp32 -> bf16, round to nearest even (finite inputs; NaN kept quiet)
@@ -174,12 +173,13 @@ __global__ __launch_bounds__(256) void kMlpInfer(int64_t n, const float* __restr
     f32x4 y = forward<false>(P, W, h0, lane, nullptr);
     if (lane < 16 && ok) {
       float y0 = y.x * norm[32] + norm[35], y1 = y.y * norm[33] + norm[36], y2 = y.z * norm[34] + norm[37];
-      if (hdr) {
-        const int64_t p = c >> 2;
+      if (hdr) {  // dense rows: the slot's first row, then one row per valid point in point order
+        const int4 h = hdr[c >> 2];
         const int j = (int)(c & 3);
-        nrow[NW * p + kRowWords * j + 19] = y0;
-        frow[FW * p + kRowWords * (2 * j) + 19] = y1;
-        frow[FW * p + kRowWords * (2 * j + 1) + 19] = y2;
+        const size_t row = (size_t)((unsigned)h.x >> 4) + __popc(h.x & 15 & ((1 << j) - 1));
+        nrow[kRowWords * row + 19] = y0;
+        frow[kRowWords * (2 * row) + 19] = y1;
+        frow[kRowWords * (2 * row + 1) + 19] = y2;
       } else {
         Y[3 * c] = y0;
         Y[3 * c + 1] = y1;

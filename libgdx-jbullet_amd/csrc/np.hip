@@ -51,20 +51,22 @@ __device__ __forceinline__ void putCand(float* out, int i, bool valid, V3 n, V3 
 
 // ------------------------------------------------------------------ binning
 __global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
-                         const int* __restrict__ act, int* tkey, int* idx, unsigned long long* counts) {
+                         const int* __restrict__ act, int* tkey, int* idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int t = -1;
-  if (i < n) {
-    int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
-    t = ptype[i];
-    if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
-    tkey[i] = t;
-    idx[i] = (int)i;
-  }
-  for (int k = 0; k < 4; k++) {  // one atomic per wave and bin
-    unsigned long long m = __ballot(t == k);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(counts + k, (unsigned long long)__popcll(m));
-  }
+  if (i >= n) return;
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  int t = ptype[i];
+  if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
+  tkey[i] = t;
+  idx[i] = (int)i;
+}
+
+// first position of each pair type in the sorted type keys (types absent from the step keep -1)
+__global__ void kBinStarts(int64_t n, const int* __restrict__ sorted, int* starts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = sorted[i];
+  if (i == 0 || sorted[i - 1] != t) starts[t] = (int)i;
 }
 
 // ------------------------------------------------------------------ sphere-sphere
@@ -1168,17 +1170,30 @@ void npDispatchAll(Ctx& c) {
   float* man = c.man[c.cur].p;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   tBegin(c, T_BIN);
-  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p, c.typeKey.p, c.perm.p,
-                     c.counter.p);
+  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p, c.typeKey.p,
+                     c.perm.p);
   size_t tb = 0;
   hipcub::DoubleBuffer<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
   JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, 2, s));
   c.cubTmp.ensure(tb);
   JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, 2, s));
+  int* starts = (int*)(c.counter.p + 4);  // 4 ints
+  JB_CHECK(hipMemsetAsync(starts, 0xff, 4 * sizeof(int), s));
+  hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)kk.Current(), starts);
   tEnd(c, T_BIN);
-  unsigned long long cnt[4];
-  JB_CHECK(hipMemcpyAsync(cnt, c.counter.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  int st4[4];
+  JB_CHECK(hipMemcpyAsync(st4, starts, sizeof(st4), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
+  unsigned long long cnt[4];
+  for (int t = 0; t < 4; t++) {
+    int64_t end = n;
+    for (int u = t + 1; u < 4; u++)
+      if (st4[u] >= 0) {
+        end = st4[u];
+        break;
+      }
+    cnt[t] = st4[t] >= 0 ? (unsigned long long)(end - st4[t]) : 0ull;
+  }
   int64_t off[5] = {0, 0, 0, 0, 0};
   for (int t = 0; t < 4; t++) off[t + 1] = off[t] + (int64_t)cnt[t];
   for (int t = 0; t < 5; t++) c.binOff[t] = off[t];

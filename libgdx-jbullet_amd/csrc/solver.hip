@@ -35,8 +35,8 @@ namespace jbg {
 namespace {
 
 constexpr int RW = kRowWords;  // words per row
-constexpr int NW = 4 * RW;     // normal rows per slot
-constexpr int FW = 8 * RW;     // friction rows per slot
+constexpr int NW = 4 * RW;     // normal rows of one slot (4 points), register staging and capacity
+constexpr int FW = 8 * RW;     // friction rows of one slot (4 points), register staging and capacity
 
 struct SolverParams {
   float dt, erp, warmFactor, linearSlop;
@@ -152,22 +152,28 @@ __global__ void kActiveBodies(int n, const int* __restrict__ list, const float* 
   ab[k] = make_int2(ldi(m + 1), ldi(m + 2));
 }
 
-__global__ void kColorPropose(int n, int round, const int2* __restrict__ ab, const float* __restrict__ props,
-                              const int* __restrict__ color, unsigned long long* bodyPrio) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || color[k] >= 0) return;
+// Jones-Plassmann rounds over the still uncoloured manifolds pend[0, m): a manifold wins a round when its
+// priority is the largest among the uncoloured manifolds of both its dynamic bodies, and then takes the
+// smallest colour free at both
+__global__ void kColorPropose(int m, const int* __restrict__ pend, int round, const int2* __restrict__ ab,
+                              const float* __restrict__ props, const int* __restrict__ color,
+                              unsigned long long* bodyPrio) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int k = pend[t];
+  if (color[k] >= 0) return;
   int2 b = ab[k];
   unsigned long long p = prio(round, k);
   if (dynamicBody(props, b.x)) atomicMax(bodyPrio + b.x, p);
   if (dynamicBody(props, b.y)) atomicMax(bodyPrio + b.y, p);
 }
 
-__global__ void kColorResolve(int n, int round, const int2* __restrict__ ab, const float* __restrict__ props, int* color,
-                              const unsigned long long* bodyPrio, unsigned long long* bodyMask,
-                              unsigned long long* remaining) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool pending = false;
-  if (k < n && color[k] < 0) {
+__global__ void kColorResolve(int m, const int* __restrict__ pend, int round, const int2* __restrict__ ab,
+                              const float* __restrict__ props, int* color, const unsigned long long* bodyPrio,
+                              unsigned long long* bodyMask) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t < m ? pend[t] : 0;
+  if (t < m && color[k] < 0) {
     int2 b = ab[k];
     unsigned long long p = prio(round, k);
     bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
@@ -183,12 +189,13 @@ __global__ void kColorResolve(int n, int round, const int2* __restrict__ ab, con
         if (db) bodyMask[b.y] |= (1ull << c);
       }
       color[k] = c;
-    } else {
-      pending = true;
     }
   }
-  unsigned long long m = __ballot(pending);  // one atomic per wave
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(remaining, (unsigned long long)__popcll(m));
+}
+
+__global__ void kUncoloured(int m, const int* __restrict__ pend, const int* __restrict__ color, int* flag) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) flag[t] = color[pend[t]] < 0;
 }
 
 __global__ void kIota(int n, int* out) {
@@ -215,6 +222,10 @@ __device__ __forceinline__ M3 invInertiaWorld(const float* tr, const float* prop
   }
   return mulMM(m1, transpose(b));
 }
+
+// slot header: x = valid-point mask (bits 0-3) | first dense row << 4, y / z = bodies, w = pair index
+__device__ __forceinline__ int slotMask(const int4& h) { return h.x & 15; }
+__device__ __forceinline__ int slotRow(const int4& h) { return (int)((unsigned)h.x >> 4); }
 
 struct SBody {
   V3 lv, av;
@@ -282,17 +293,34 @@ __device__ __forceinline__ void planeSpace1(V3 n, V3& p, V3& q) {  // TransformU
 
 // rows of every slot (solveGroupCacheFriendlySetup :494-747 without the impulse application);
 // the warm-start impulse of each row is left in its applied word
+// contact rows of each slot: points with distance <= 0 (the rows solveGroupCacheFriendlySetup creates)
+__global__ void kSlotRows(int na, const int* __restrict__ corder, const int* __restrict__ list,
+                          const float* __restrict__ man, int* count) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= na) return;
+  const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[corder[p]];
+  const int n = ldi(m);
+  int k = 0;
+  for (int j = 0; j < n; j++)
+    if (m[JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j + 15] <= 0.0f) k++;
+  count[p] = k;
+}
+
+// rows of every slot, stored densely from the slot's first row (rowStart, exclusive prefix sum of the
+// slot row counts): normal row r at nrow[r], its two friction rows at frow[2r], frow[2r+1]
+// (solveGroupCacheFriendlySetup :494-747 without the impulse application); the warm-start impulse of each
+// row is left in its applied word
 __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __restrict__ list, SolverParams SP,
                            float* man, const float* __restrict__ tr, const float* __restrict__ vel,
-                           const float* __restrict__ props, int4* hdr, float* nrow, float* frow,
-                           const float* __restrict__ learned, float* features) {
+                           const float* __restrict__ props, const int* __restrict__ rowStart, int4* hdr, float* nrow,
+                           float* frow, const float* __restrict__ learned, float* features) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= na) return;
   int pi = list[corder[p]];
   float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * pi;
   int n = ldi(m), b0 = ldi(m + 1), b1 = ldi(m + 2);
-  float* N0 = nrow + (size_t)NW * p;
-  float* F0 = frow + (size_t)FW * p;
+  const int row0 = rowStart[p];
+  int row = row0;
   M3 I0 = invInertiaWorld(tr, props, b0), I1 = invInertiaWorld(tr, props, b1);
   V3 o0 = ld3(tr + 12 * b0 + 9), o1 = ld3(tr + 12 * b1 + 9);
   V3 lv0 = ld3(vel + 8 * b0), av0 = ld3(vel + 8 * b0 + 4), lv1 = ld3(vel + 8 * b1), av1 = ld3(vel + 8 * b1 + 4);
@@ -302,7 +330,7 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
     float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
     if (!(P[15] <= 0.0f)) continue;
     mask |= 1 << j;
-    float* N = N0 + RW * j;
+    float* N = nrow + (size_t)RW * row;
     V3 normal = ld3(P + 12);
     V3 rp1 = sub(ld3(P + 6), o0);
     V3 rp2 = sub(ld3(P + 9), o1);
@@ -354,8 +382,9 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       st3(P + 24, d2v);
       sti(P + 27, 1);
     }
-    float* F1 = F0 + RW * (2 * j);
-    float* F2 = F0 + RW * (2 * j + 1);
+    float* F1 = frow + (size_t)RW * (2 * row);
+    float* F2 = frow + (size_t)RW * (2 * row + 1);
+    row++;
     frictionRow(F1, ld3(P + 21), rp1, rp2, I0, I1, im0, im1, friction);
     frictionRow(F2, ld3(P + 24), rp1, rp2, I0, I1, im0, im1, friction);
     if (SP.useLearned) {
@@ -366,13 +395,13 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       F2[19] = P[20] * SP.warmFactor;
     }
   }
-  hdr[p] = make_int4(mask, b0, b1, pi);
+  hdr[p] = make_int4(mask | (row0 << 4), b0, b1, pi);
 }
 
 // normal rows of the solve (valid points of every slot): out[0] all slots, out[1] slots below headEnd
 __global__ void kCountRows(int na, int headEnd, const int4* __restrict__ hdr, unsigned long long* out) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long k = p < na ? (unsigned long long)__popc(hdr[p].x & 15) : 0ull;
+  unsigned long long k = p < na ? (unsigned long long)__popc(slotMask(hdr[p])) : 0ull;
   unsigned long long kh = p < headEnd ? k : 0ull;
   for (int o = 32; o > 0; o >>= 1) {
     k += __shfl_xor(k, o);
@@ -391,12 +420,10 @@ __global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, co
   int p = off + k;
   int4 h = hdr[p];
   SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
-  const float* N0 = nrow + (size_t)NW * p;
-  const float* F0 = frow + (size_t)FW * p;
-  for (int j = 0; j < 4; j++) {
-    if (!((h.x >> j) & 1)) continue;
+  const int rows = __popc(slotMask(h)), row0 = slotRow(h);
+  for (int q = 0; q < rows; q++) {
     for (int r = 0; r < 3; r++) {
-      const float* R = r == 0 ? N0 + RW * j : F0 + RW * (2 * j + r - 1);
+      const float* R = r == 0 ? nrow + (size_t)RW * (row0 + q) : frow + (size_t)RW * (2 * (row0 + q) + r - 1);
       V3 nrm = ld3(R);
       float ap = R[19];
       applyImpulse(A, scl(nrm, A.invMass), ld3(R + 9), ap);
@@ -426,17 +453,22 @@ __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
 __device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const int4* __restrict__ hdr, float* nrow,
                                            const float* __restrict__ props, float* svel, double& dd, double& nn) {
   const int4 h = hdr[p];
-  float r[NW];
-  const float4* src = (const float4*)(nrow + (size_t)NW * p);
+  const int mask = slotMask(h), row0 = slotRow(h);
+  float r[NW];  // the slot's rows in point order; only the valid points' rows are loaded
 #pragma unroll
-  for (int q = 0; q < NW / 4; q++) {
-    float4 v = src[q];
-    r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+  for (int j = 0; j < 4; j++) {
+    if (!((mask >> j) & 1)) continue;
+    const float4* src = (const float4*)(nrow + (size_t)RW * (row0 + __popc(mask & ((1 << j) - 1))));
+#pragma unroll
+    for (int q = 0; q < RW / 4; q++) {
+      float4 v = src[q];
+      r[RW * j + 4 * q] = v.x; r[RW * j + 4 * q + 1] = v.y; r[RW * j + 4 * q + 2] = v.z; r[RW * j + 4 * q + 3] = v.w;
+    }
   }
   SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    if (!((h.x >> j) & 1)) continue;
+    if (!((mask >> j) & 1)) continue;
     const float* N = r + RW * j;
     V3 nrm = v3(N[0], N[1], N[2]);
     float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
@@ -453,7 +485,7 @@ __device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const 
     ni = nw - old;
     applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), ni);
     applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -ni);
-    nrow[(size_t)NW * p + RW * j + 19] = nw;
+    nrow[(size_t)RW * (row0 + __popc(mask & ((1 << j) - 1))) + 19] = nw;
     double d = (double)nw - (double)old;
     dd += d * d;
     nn += (double)nw * (double)nw;
@@ -468,20 +500,26 @@ __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, cons
                                              const float* __restrict__ nrow, float* frow,
                                              const float* __restrict__ props, float* svel, double& dd, double& nn) {
   const int4 h = hdr[p];
-  float r[FW];
-  const float4* src = (const float4*)(frow + (size_t)FW * p);
-#pragma unroll
-  for (int q = 0; q < FW / 4; q++) {
-    float4 v = src[q];
-    r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
-  }
+  const int mask = slotMask(h), row0 = slotRow(h);
+  float r[FW];  // the slot's friction rows in point order; only the valid points' rows are loaded
   float napp[4];
 #pragma unroll
-  for (int j = 0; j < 4; j++) napp[j] = nrow[(size_t)NW * p + RW * j + 19];
+  for (int j = 0; j < 4; j++) {
+    if (!((mask >> j) & 1)) continue;
+    const int row = row0 + __popc(mask & ((1 << j) - 1));
+    const float4* src = (const float4*)(frow + (size_t)RW * (2 * row));
+#pragma unroll
+    for (int q = 0; q < 2 * RW / 4; q++) {
+      float4 v = src[q];
+      r[2 * RW * j + 4 * q] = v.x; r[2 * RW * j + 4 * q + 1] = v.y; r[2 * RW * j + 4 * q + 2] = v.z;
+      r[2 * RW * j + 4 * q + 3] = v.w;
+    }
+    napp[j] = nrow[(size_t)RW * row + 19];
+  }
   SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    if (!((h.x >> j) & 1)) continue;
+    if (!((mask >> j) & 1)) continue;
     float total = napp[j] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
 #pragma unroll
     for (int f = 0; f < 2; f++) {
@@ -504,7 +542,7 @@ __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, cons
         j1 = ap - old;
         applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), j1);
         applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -j1);
-        frow[(size_t)FW * p + RW * (2 * j + f) + 19] = ap;
+        frow[(size_t)RW * (2 * (row0 + __popc(mask & ((1 << j) - 1))) + f) + 19] = ap;
       }
       double d = (double)ap - (double)old;
       dd += d * d;
@@ -595,12 +633,15 @@ __global__ void kWriteback(int n, const int4* __restrict__ hdr, const float* __r
   if (p >= n) return;
   int4 h = hdr[p];
   float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * h.w;
+  const int mask = slotMask(h);
+  int row = slotRow(h);
   for (int j = 0; j < 4; j++) {
-    if (!((h.x >> j) & 1)) continue;
+    if (!((mask >> j) & 1)) continue;
     float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
-    P[18] = nrow[(size_t)NW * p + RW * j + 19];
-    P[19] = frow[(size_t)FW * p + RW * (2 * j) + 19];
-    P[19] = frow[(size_t)FW * p + RW * (2 * j + 1) + 19];  // Lateral1 written twice; Lateral2 never stored
+    P[18] = nrow[(size_t)RW * row + 19];
+    P[19] = frow[(size_t)RW * (2 * row) + 19];
+    P[19] = frow[(size_t)RW * (2 * row + 1) + 19];  // Lateral1 written twice; Lateral2 never stored (:908-909)
+    row++;
   }
 }
 
@@ -611,10 +652,13 @@ __global__ void kGatherImpulses(int64_t nslots, const int4* __restrict__ hdr, co
   if (q >= nslots) return;
   const int64_t p = q >> 2;
   const int j = (int)(q & 3);
-  const bool ok = (hdr[p].x >> j) & 1;
-  out[3 * q] = ok ? nrow[(size_t)NW * p + RW * j + 19] : 0.0f;
-  out[3 * q + 1] = ok ? frow[(size_t)FW * p + RW * (2 * j) + 19] : 0.0f;
-  out[3 * q + 2] = ok ? frow[(size_t)FW * p + RW * (2 * j + 1) + 19] : 0.0f;
+  const int4 h = hdr[p];
+  const int mask = slotMask(h);
+  const bool ok = (mask >> j) & 1;
+  const size_t row = (size_t)slotRow(h) + __popc(mask & ((1 << j) - 1));
+  out[3 * q] = ok ? nrow[RW * row + 19] : 0.0f;
+  out[3 * q + 1] = ok ? frow[RW * (2 * row) + 19] : 0.0f;
+  out[3 * q + 2] = ok ? frow[RW * (2 * row + 1) + 19] : 0.0f;
 }
 
 __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const float* __restrict__ props) {
@@ -715,19 +759,36 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
-  int round = 0;
-  for (;;) {
-    for (int r = 0; r < 6; r++, round++) {
-      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.activeBodies.p, c.props.p,
-                         c.color.p, c.bodyPrio.p);
-      if (r == 5) JB_CHECK(hipMemsetAsync(c.counter.p + 1, 0, sizeof(unsigned long long), s));
-      hipLaunchKernelGGL(kColorResolve, dim3(gridFor(na)), dim3(256), 0, s, na, round, c.activeBodies.p, c.props.p,
-                         c.color.p, c.bodyPrio.p, c.bodyMask.p, c.counter.p + 1);
+  // batches of rounds over the pending list, which is compacted between batches (the host waits only there)
+  c.pend.ensure(na);
+  c.pendAlt.ensure(na);
+  c.pendFlag.ensure(na);
+  hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
+  int round = 0, m = na;
+  for (int batch = 0;; batch++) {
+    const int rounds = batch == 0 ? 12 : 6;
+    for (int r = 0; r < rounds; r++, round++) {
+      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                         (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p,
+                         c.bodyPrio.p);
+      hipLaunchKernelGGL(kColorResolve, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                         (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p,
+                         (const unsigned long long*)c.bodyPrio.p, c.bodyMask.p);
     }
-    unsigned long long rem = 0;
-    JB_CHECK(hipMemcpyAsync(&rem, c.counter.p + 1, sizeof(rem), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p,
+                       (const int*)c.color.p, c.pendFlag.p);
+    size_t tbs = 0;
+    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m, s));
+    c.cubTmp.ensure(tbs);
+    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m,
+                                           s));
+    int rem = 0;
+    JB_CHECK(hipMemcpyAsync(&rem, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
     JB_CHECK(hipStreamSynchronize(s));
-    if (rem == 0) break;
+    std::swap(c.pend.p, c.pendAlt.p);
+    std::swap(c.pend.n, c.pendAlt.n);
+    m = rem;
+    if (m == 0) break;
     if (round > 4000) throw std::runtime_error("colouring did not converge");
   }
   // order by colour (stable in active index) and colour offsets
@@ -769,9 +830,18 @@ void solverSolve(Ctx& c, float dt) {
   SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.useMlp) ? 1 : 0,
                   (c.dumpFeatures || learnedHere) ? 1 : 0, c.tolerance};
   if (SP.dumpFeatures) c.features.ensure((size_t)16 * 4 * na);
+  if ((int64_t)na * 4 >= (1ll << 27)) throw std::length_error("too many active manifolds for the slot header");
+  c.slotCount.ensure(na);
+  c.slotStart.ensure(na);
+  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p, c.mActive.p, man,
+                     c.slotCount.p);
+  size_t tbScan = 0;
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbScan, c.slotCount.p, c.slotStart.p, na, s));
+  c.cubTmp.ensure(tbScan);
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tbScan, c.slotCount.p, c.slotStart.p, na, s));
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
-                     c.tr.p, c.vel.p, c.props.p, c.hdr.p, c.nrow.p, c.frow.p, SP.useLearned ? c.learned.p : nullptr,
-                     SP.dumpFeatures ? c.features.p : nullptr);
+                     c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
+                     SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
   // colours [0, headColors) get a launch each; the trailing small ones share one single-workgroup launch
   int headColors = c.ncolors;
   if (c.tailThreshold > 0) {

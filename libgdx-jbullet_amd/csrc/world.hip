@@ -192,7 +192,12 @@ void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, cons
     int16_t g = group ? group[i] : (int16_t)1, m = mask ? mask[i] : (int16_t)-1;
     filt[i] = (int)(uint16_t)g | ((int)(uint16_t)m << 16);
     wid[i] = world ? world[i] : 0;
+    if (wid[i] < 0) throw std::invalid_argument("negative world id");
   }
+  int maxWorld = 0;
+  for (int i = 0; i < n; i++) maxWorld = std::max(maxWorld, wid[i]);
+  c.worldBits = 0;
+  while (c.worldBits < 31 && (1ll << c.worldBits) <= maxWorld) c.worldBits++;
   if (c.bpMode == JBGPU_BP_SAP16 || c.bpMode == JBGPU_BP_SAP32) {
     // AxisSweep3Internal.quantize (host copy of the device routine)
     for (int i = 0; i < n; i++)

@@ -248,7 +248,9 @@ def batched_towers(seed=1, n_worlds=4096, lo=16, hi=512, spacing=3.0, world_gap=
 
 
 def pyramids(seed=1, n_boxes=64_000, n_pyramids=32, n_spheres=2000, area=200.0):
-    """Box pyramids and walls plus falling spheres (learned warm-start workload)."""
+    """Box structures plus falling spheres (learned warm-start workload): even structures are square-base
+    pyramids (level L of a base-k pyramid holds (k-L)^2 boxes), odd ones are walls eight times wider than tall;
+    each structure uses one box size, half extents U(0.2, 1) per axis."""
     rng = np.random.default_rng(seed)
     b = _Builder("pyramids")
     plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
@@ -258,20 +260,37 @@ def pyramids(seed=1, n_boxes=64_000, n_pyramids=32, n_spheres=2000, area=200.0):
         cx, cz = rng.uniform(-area / 2, area / 2, 2)
         h = rng.uniform(0.2, 1.0, 3).astype(np.float32)
         box = b.shape(BOX, list(h))
-        wall = p % 2 == 1
+        mass = np.float32(1000.0 * 8 * h[0] * h[1] * h[2])
+        gx, gz = 2 * h[0] + 0.01, 2 * h[2] + 0.01
         placed, level = 0, 0
-        base = int(np.ceil(np.sqrt(2 * per))) if not wall else int(np.ceil(np.sqrt(per)))
-        while placed < per:
-            cols = max(base - level, 1) if not wall else base
-            for c in range(cols):
-                if placed >= per:
-                    break
-                x = cx + (c - (cols - 1) / 2.0) * (2 * h[0] + 0.01)
-                y = h[1] + level * (2 * h[1] + 0.001)
-                mass = np.float32(1000.0 * 8 * h[0] * h[1] * h[2])
-                b.body(box, IDENT, np.array([x, y, cz], np.float32), mass)
-                placed += 1
-            level += 1
+        if p % 2 == 0:
+            k = 1
+            while k * (k + 1) * (2 * k + 1) // 6 < per:
+                k += 1
+            while placed < per and level < k:
+                side = k - level
+                for a in range(side):
+                    for c in range(side):
+                        if placed >= per:
+                            break
+                        x = cx + (a - (side - 1) / 2.0) * gx
+                        z = cz + (c - (side - 1) / 2.0) * gz
+                        y = h[1] + level * (2 * h[1] + 0.001)
+                        b.body(box, IDENT, np.array([x, y, z], np.float32), mass)
+                        placed += 1
+                level += 1
+        else:
+            height = max(int(np.sqrt(per / 8.0)), 1)
+            width = int(np.ceil(per / height))
+            while placed < per:
+                for a in range(width):
+                    if placed >= per:
+                        break
+                    x = cx + (a - (width - 1) / 2.0) * gx
+                    y = h[1] + level * (2 * h[1] + 0.001)
+                    b.body(box, IDENT, np.array([x, y, cz], np.float32), mass)
+                    placed += 1
+                level += 1
     for _ in range(n_spheres):
         r = float(rng.uniform(0.2, 0.6))
         s = b.shape(SPHERE, [r])
