@@ -141,6 +141,8 @@ int jbgpu_solver_get_impulses(jbgpu_ctx* ctx, float* out, int64_t nslots);
 /* tolerance > 0: iterate until the residual ||dlambda|| / max(1, ||lambda||) <= tolerance or
  * max_iterations; tolerance 0: exactly num_iterations (ContactSolverInfo.numIterations). */
 int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations);
+/* ContactSolverInfo.warmstartingFactor for the following solves (0.85 by default). */
+int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor);
 /* Launch shaping of the colour sweeps (no reference counterpart; results are identical for every
  * value): trailing colours of at most `threshold` manifolds are swept by one single-workgroup launch,
  * colour after colour; 0 gives every colour its own launch.  Default 1024. */

@@ -809,6 +809,13 @@ int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterati
   });
 }
 
+int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor) {
+  return guard(ctx, [&] {
+    if (!std::isfinite(factor)) throw std::invalid_argument("warm-start factor must be finite");
+    ctx->c.warmFactor = factor;
+  });
+}
+
 int jbgpu_solver_set_tail(jbgpu_ctx* ctx, int32_t threshold) {
   return guard(ctx, [&] {
     if (threshold < 0) throw std::invalid_argument("negative tail threshold");

@@ -75,6 +75,7 @@ SYMBOLS = {
     "jbgpu_solver_get_impulses": (C.c_int, [_vp, _f, C.c_int64]),
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_tail": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_solver_set_warmstart_factor": (C.c_int, [_vp, C.c_float]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
     "jbgpu_mlp_get_params": (C.c_int, [_vp, _f, _f]),
@@ -350,6 +351,9 @@ class Context:
     def solver_set_params(self, tolerance=0.0, max_iterations=100, num_iterations=10):
         self._ck(load().jbgpu_solver_set_params(self.h, float(tolerance), int(max_iterations), int(num_iterations)),
                  "solver_set_params")
+
+    def solver_set_warmstart_factor(self, factor):
+        self._ck(load().jbgpu_solver_set_warmstart_factor(self.h, float(factor)), "solver_set_warmstart_factor")
 
     def solver_set_tail(self, threshold):
         self._ck(load().jbgpu_solver_set_tail(self.h, int(threshold)), "solver_set_tail")

@@ -104,7 +104,7 @@ def test_world_boxes_grid_step_parity(tail):
     sc = scenes.boxes_grid(seed=1)
     ctx, orc = _world_pair(sc)
     ctx.solver_set_tail(tail)
-    for step in range(90):
+    for step in range(300 if tail else 90):  # the 1000-box configuration runs 300 steps
         _sync_and_step(ctx, orc)
         st = ctx.stats()
         if step % 15 == 0:
@@ -182,4 +182,18 @@ def test_sleeping_islands_step_parity():
         if step % 50 == 0:
             print(f"[sleep] step {step}: states {sorted(seen)}", flush=True)
     assert {1, 2, 3} <= seen
+    ctx.close()
+
+
+def test_world_mixed_pile_100k_parity():
+    """Full-size pile (100,000 bodies): 300 device-only steps, then 4 steps checked bit for bit."""
+    sc = scenes.mixed_pile(seed=1, n=100_000)
+    ctx, orc = _world_pair(sc, iterations=20)
+    ctx.step(n=300)
+    st = ctx.stats()
+    print(f"[100k] after 300 steps: pairs {st['pairs']} manifolds {st['manifolds']} colors {st['colors']}", flush=True)
+    for step in range(4):
+        _sync_and_step(ctx, orc)
+        _compare_world(ctx, orc, f"100k step {301 + step}")
+        print(f"[100k] step {301 + step} bit-exact", flush=True)
     ctx.close()
