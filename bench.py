@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--bp-frames", type=int, default=20)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-kind", default="friction", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-blocks", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -129,7 +131,7 @@ def main():
     ctx.close()
     if roof is not None and not args.no_traffic:
         try:
-            roof["traffic"], roof["traffic_detail"] = traffic(args)
+            roof["traffic"], roof["traffic_detail"] = traffic(args, roof["kind"], tune["block_size"])
         except Exception as e:
             roof["traffic"], roof["traffic_detail"] = None, f"counter passes failed: {e!r}"
     for key, flag, fn in (("batched", args.no_batched, batched), ("learned", args.no_learned, learned_warm_start),
@@ -157,19 +159,27 @@ def pmc_child(args):
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
     ctx = jbgpu.Context(num_iterations=args.iterations)
     ctx.world_create(shapes, verts, bodies)
+    ctx.solver_set_blocks(args.pmc_blocks)
+    ctx.solver_set_tail(0)
     ctx.timing(True)
     ctx.timing_kernels(1)
     ctx.step(n=PMC_WARMUP + PMC_STEPS)
-    ks = ctx.timing_kernels_get()["friction"]
+    ks = ctx.timing_kernels_get()[args.pmc_kind]
     ctx.measure_copy(PMC_COPY_BYTES, 1)
     ctx.close()
-    print(json.dumps({"friction_rows": ks["rows"], "friction_launches": ks["launches"]}))
+    print(json.dumps({"rows": ks["rows"], "launches": ks["launches"]}))
 
 
-def traffic(args):
-    """HBM bytes per kFriction launch from two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
-    dispatch), each scaled by the ratio of known bytes to what the same counter reads for the plain copy probe
-    (kCopyProbe, PMC_COPY_BYTES read and written) in the same pass."""
+KERNEL_MATCH = {"friction": (r"kFriction\(", "%kFriction(%"), "normal": (r"kNormal\(", "%kNormal(%"),
+                "friction_blocks": (r"kSweepBlocks<true>", "%kSweepBlocks<true>%"),
+                "normal_blocks": (r"kSweepBlocks<false>", "%kSweepBlocks<false>%")}
+
+
+def traffic(args, kind, blocks):
+    """HBM bytes per launch of the dominant sweep kernel from two rocprofv3 counter passes (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by the ratio of known bytes to what the same counter reads for
+    the plain copy probe (kCopyProbe, PMC_COPY_BYTES read and written) in the same pass."""
+    regex, like = KERNEL_MATCH[kind]
     import shutil
     import sqlite3
     import subprocess
@@ -180,9 +190,10 @@ def traffic(args):
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="jbgpu_pmc_", dir="/tmp")
-        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--kernel-include-regex", r"kFriction\(|kCopyProbe",
+        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--kernel-include-regex", regex + "|kCopyProbe",
                "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--bodies",
-               str(args.bodies), "--iterations", str(args.iterations), "--seed", str(args.seed)]
+               str(args.bodies), "--iterations", str(args.iterations), "--seed", str(args.seed), "--pmc-kind", kind,
+               "--pmc-blocks", str(blocks)]
         t = time.time()
         p = subprocess.run(cmd, capture_output=True, text=True)
         log(f"[bench] counter pass {counter}: rc {p.returncode} in {time.time() - t:.1f}s")
@@ -193,7 +204,7 @@ def traffic(args):
         child = json.loads(lines[-1])
         cur = sqlite3.connect(dbs[0]).cursor()
         q = "select avg(value), count(*) from counters_collection where counter_name = ? and kernel_name like ?"
-        kf, nkf = cur.execute(q, (counter, "%kFriction(%")).fetchone()
+        kf, nkf = cur.execute(q, (counter, like)).fetchone()
         cp, ncp = cur.execute(q, (counter, "%kCopyProbe%")).fetchone()
         if not nkf or not ncp or not cp:
             return None, f"counter pass {counter}: no dispatches recorded"
@@ -203,8 +214,8 @@ def traffic(args):
     fetch = res["FETCH_SIZE"]["kib_per_dispatch"] * 1024.0 * res["FETCH_SIZE"]["scale"]
     write = res["WRITE_SIZE"]["kib_per_dispatch"] * 1024.0 * res["WRITE_SIZE"]["scale"]
     child = res["FETCH_SIZE"]["child"]
-    algo = ROW_BYTES * child["friction_rows"] / max(child["friction_launches"], 1)
-    detail = {"workload": f"same pile, every kFriction launch of steps 1-{PMC_WARMUP + PMC_STEPS}",
+    algo = ROW_BYTES * child["rows"] / max(child["launches"], 1)
+    detail = {"workload": f"same pile, block size {blocks}, every {regex} launch of steps 1-{PMC_WARMUP + PMC_STEPS}",
               "fetch_bytes": int(fetch),
               "write_bytes": int(write), "algorithmic_bytes": int(algo), "traffic_over_algorithmic":
               round((fetch + write) / max(algo, 1.0), 3), "scale_fetch": round(res["FETCH_SIZE"]["scale"], 4),
@@ -213,20 +224,24 @@ def traffic(args):
 
 
 def autotune(ctx, dt, steps=8):
-    """Launch shaping of the colour sweeps (results are bit-identical for every choice): the largest colour
-    size swept by the single-workgroup tail launch.  Each candidate runs a few untimed warmup steps."""
-    cands = (0, 256, 1024, 4096)
+    """Launch shaping of the sweeps (results are bit-identical for every choice): the block size of the
+    block-partitioned sweeps (0: one launch per colour) and, without blocks, the largest colour swept by the
+    single-workgroup tail launch.  Each candidate runs a few untimed warmup steps."""
+    cands = ((2048, 0), (1024, 0), (512, 0), (0, 0), (0, 256))
     ms = {}
-    for thr in cands:
-        ctx.solver_set_tail(thr)
+    for blocks, tail in cands:
+        ctx.solver_set_blocks(blocks)
+        ctx.solver_set_tail(tail)
         ctx.step(dt)
         t = time.perf_counter()
         ctx.step(dt, n=steps)
-        ms[thr] = 1e3 * (time.perf_counter() - t) / steps
+        ms[(blocks, tail)] = 1e3 * (time.perf_counter() - t) / steps
     best = min(ms, key=ms.get)
-    ctx.solver_set_tail(best)
-    log(f"[bench] autotune tail threshold: {ms} -> {best}")
-    return {"tail_threshold": best, "ms_per_step": {str(k): round(v, 3) for k, v in ms.items()}}
+    ctx.solver_set_blocks(best[0])
+    ctx.solver_set_tail(best[1])
+    log(f"[bench] autotune (block size, tail threshold): {ms} -> {best}")
+    return {"block_size": best[0], "tail_threshold": best[1],
+            "ms_per_step": {f"blocks={b},tail={t}": round(v, 3) for (b, t), v in ms.items()}}
 
 
 def roofline(ctx, args):
@@ -241,15 +256,18 @@ def roofline(ctx, args):
     avg_s = k["ms"] / k["launches"] / 1000.0
     bytes_per_launch = ROW_BYTES * k["rows"] / k["launches"]
     achieved = bytes_per_launch / avg_s / 1e9
-    kernel = {"normal": "kNormal (normal-row colour sweep)", "friction": "kFriction (friction-row colour sweep)"}[name]
-    other = "friction" if name == "normal" else "normal"
-    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    kernel = {"normal": "kNormal (normal-row colour sweep)", "friction": "kFriction (friction-row colour sweep)",
+              "normal_blocks": "kSweepBlocks<false> (normal rows, block-partitioned sweep)",
+              "friction_blocks": "kSweepBlocks<true> (friction rows, block-partitioned sweep)"}[name]
+    others = {n: {"avg_launch_us": round(v["ms"] * 1e3 / v["launches"], 3), "launches_timed": v["launches"],
+                  "achieved_gbs": round(ROW_BYTES * v["rows"] / max(v["ms"], 1e-9) / 1e6, 2)}
+              for n, v in ks.items() if n != name and v["launches"] > 0}
+    return {"bound": "hbm", "kernel": kernel, "kind": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
             "bytes_per_launch": int(bytes_per_launch), "launches_timed": k["launches"],
             "unit_bytes": f"{ROW_BYTES} B per row per sweep; {k['rows']} rows over the timed launches",
-            "other_sweep": {"kernel": other, "avg_launch_us": round(ks[other]["ms"] * 1e3 / max(ks[other]["launches"], 1),
-                                                                   3),
-                            "achieved": round(ROW_BYTES * ks[other]["rows"] / max(ks[other]["ms"], 1e-9) / 1e6, 2)}}
+            "other_sweeps": others}
 
 
 def batched(args):

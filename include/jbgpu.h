@@ -141,6 +141,11 @@ int jbgpu_solver_get_impulses(jbgpu_ctx* ctx, float* out, int64_t nslots);
 /* tolerance > 0: iterate until the residual ||dlambda|| / max(1, ||lambda||) <= tolerance or
  * max_iterations; tolerance 0: exactly num_iterations (ContactSolverInfo.numIterations). */
 int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterations, int32_t num_iterations);
+/* Launch shaping of the sweeps (no reference counterpart; results are identical for every value):
+ * dynamic bodies are grouped into blocks of `block_size` consecutive Morton positions (at most 2048);
+ * manifolds inside a block are swept by one workgroup per block with the block's velocities in LDS,
+ * colour after colour, the others by one launch per colour.  0 disables blocks.  Default 2048. */
+int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size);
 /* ContactSolverInfo.warmstartingFactor for the following solves (0.85 by default). */
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor);
 /* Launch shaping of the colour sweeps (no reference counterpart; results are identical for every
@@ -185,9 +190,11 @@ int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
 /* Kernel-level timing of the solver sweeps while timing is enabled: on every `every`-th timed step
  * (0 = off) each normal-row and friction-row colour launch carries its own start/stop events
- * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps.
- * [0] normal rows, [1] friction rows: summed ms, timed launches, rows processed by them. */
+ * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps: summed ms,
+ * timed launches and rows processed by them. */
 int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every);
+/* four entries each: [0] normal rows, per-colour launches, [1] friction rows, per-colour launches,
+ * [2] normal rows, block launches, [3] friction rows, block launches */
 /* Device memory for callers that keep their own inputs resident in HBM (e.g. AABBs for
  * jbgpu_bp_set_aabbs with on_device = 1) without linking another GPU runtime; freed by the caller. */
 int jbgpu_buffer_alloc(jbgpu_ctx* ctx, int64_t bytes, void** dev);

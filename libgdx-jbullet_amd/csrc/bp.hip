@@ -643,6 +643,7 @@ void bpFindPairs(Ctx& c) {
   c.st.huge = c.nHuge;
   const uint64_t* code = dk.Current();
   const int* order = dv.Current();
+  c.sortedOrder = order;
   const int nn = n - c.nHuge;  // leaves in the tree
   if (c.nHuge > 0) {  // sort the huge list by (world, index) for the per-world lookup
     size_t tb = 0;

@@ -160,6 +160,19 @@ struct Ctx {
   DBuf<int> colorOrder, colorOrderAlt;
   std::vector<int64_t> colorOff;
   DBuf<int> colorOffDev;     // colour offsets on the device (tail sweeps)
+  // block-partitioned sweeps: dynamic bodies in blocks of blockSize consecutive Morton positions; manifolds
+  // inside one block ("interior") are swept by one workgroup per block with the block's velocities in LDS,
+  // the others ("boundary") by one launch per colour
+  int blockSize = 2048;               // 0: every colour swept by its own launch
+  const int* sortedOrder = nullptr;   // Morton-sorted proxy order of this step's broadphase
+  int nBlocks = 0, interiorColors = 0;
+  int64_t interiorEnd = 0;            // slots [0, interiorEnd) are interior
+  DBuf<int> blockLocal;               // per body: position inside its block, -1 for static bodies
+  DBuf<int> mPhase;                   // per active manifold: block index (interior) or -1 (boundary)
+  DBuf<unsigned long long> slotKey, slotKeyAlt;
+  DBuf<int> runStart, runEnd;         // per (interior colour, block): slot range
+  DBuf<int2> slotLocal;               // per interior slot: LDS positions of its bodies (-1: static)
+  DBuf<int> batchFlag;
   int tailThreshold = 1024;  // trailing colours of at most this many manifolds share one workgroup launch
   int headColors = 0;
   DBuf<double> resid;  // per iteration: dd, nn
@@ -183,8 +196,8 @@ struct Ctx {
   int kernelEvery = 0;
   std::vector<hipEvent_t> kEv;  // event pool, two per timed launch
   std::vector<int> kEvKind;     // 0 normal rows, 1 friction rows
-  double kMs[2] = {0, 0};
-  int64_t kLaunches[2] = {0, 0}, kRows[2] = {0, 0};
+  double kMs[4] = {0, 0, 0, 0};  // 0/1: per-colour normal/friction launches, 2/3: block launches
+  int64_t kLaunches[4] = {0, 0, 0, 0}, kRows[4] = {0, 0, 0, 0};
   DBuf<unsigned long long> rowCount;  // normal rows of the current solve
   std::vector<float> residHist;
 };

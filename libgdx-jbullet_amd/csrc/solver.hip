@@ -449,9 +449,54 @@ __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
   }
 }
 
+// where a sweep reads and writes the solver velocities of a slot's two bodies
+struct GlobalBodies {  // the solver body array in global memory
+  float* svel;
+  const float* __restrict__ props;
+  __device__ void load(int p, const int4& h, SBody& A, SBody& B) const {
+    (void)p;
+    A = loadBody(svel, props, h.y);
+    B = loadBody(svel, props, h.z);
+  }
+  __device__ void store(int p, const int4& h, const SBody& A, const SBody& B) const {
+    (void)p;
+    if (A.invMass != 0.0f) storeBody(svel, h.y, A);
+    if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+  }
+};
+struct LdsBodies {  // a block's dynamic bodies in LDS (8 floats each: lv, invMass, av, -), static ones in global
+  float* lds;
+  const int2* __restrict__ local;  // per slot: LDS positions of body A and B, -1 for a static body
+  const float* __restrict__ svel;
+  const float* __restrict__ props;
+  __device__ static SBody get(const float* l) {
+    SBody s;
+    s.lv = v3(l[0], l[1], l[2]);
+    s.invMass = l[3];
+    s.av = v3(l[4], l[5], l[6]);
+    return s;
+  }
+  __device__ static void put(float* l, const SBody& s) {
+    l[0] = s.lv.x; l[1] = s.lv.y; l[2] = s.lv.z;
+    l[4] = s.av.x; l[5] = s.av.y; l[6] = s.av.z;
+  }
+  __device__ void load(int p, const int4& h, SBody& A, SBody& B) const {
+    const int2 q = local[p];
+    A = q.x >= 0 ? get(lds + 8 * q.x) : loadBody(svel, props, h.y);
+    B = q.y >= 0 ? get(lds + 8 * q.y) : loadBody(svel, props, h.z);
+  }
+  __device__ void store(int p, const int4& h, const SBody& A, const SBody& B) const {
+    (void)h;
+    const int2 q = local[p];
+    if (q.x >= 0) put(lds + 8 * q.x, A);
+    if (q.y >= 0) put(lds + 8 * q.y, B);
+  }
+};
+
 // normal rows of one manifold slot p: resolveSingleCollisionCombinedCacheFriendly (:234-284)
+template <class Bodies>
 __device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const int4* __restrict__ hdr, float* nrow,
-                                           const float* __restrict__ props, float* svel, double& dd, double& nn) {
+                                           const Bodies& io, double& dd, double& nn) {
   const int4 h = hdr[p];
   const int mask = slotMask(h), row0 = slotRow(h);
   float r[NW];  // the slot's rows in point order; only the valid points' rows are loaded
@@ -465,7 +510,8 @@ __device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const 
       r[RW * j + 4 * q] = v.x; r[RW * j + 4 * q + 1] = v.y; r[RW * j + 4 * q + 2] = v.z; r[RW * j + 4 * q + 3] = v.w;
     }
   }
-  SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+  SBody A, B;
+  io.load(p, h, A, B);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (!((mask >> j) & 1)) continue;
@@ -490,15 +536,15 @@ __device__ __forceinline__ void normalSlot(int p, const SolverParams& SP, const 
     dd += d * d;
     nn += (double)nw * (double)nw;
   }
-  if (A.invMass != 0.0f) storeBody(svel, h.y, A);
-  if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+  io.store(p, h, A, B);
 }
 
 // friction rows of one manifold slot p: resolveSingleFrictionCacheFriendly (:286-346); the limit uses the
 // normal impulse of the row's own contact from this iteration
+template <class Bodies>
 __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, const int4* __restrict__ hdr,
-                                             const float* __restrict__ nrow, float* frow,
-                                             const float* __restrict__ props, float* svel, double& dd, double& nn) {
+                                             const float* __restrict__ nrow, float* frow, const Bodies& io,
+                                             double& dd, double& nn) {
   const int4 h = hdr[p];
   const int mask = slotMask(h), row0 = slotRow(h);
   float r[FW];  // the slot's friction rows in point order; only the valid points' rows are loaded
@@ -516,7 +562,8 @@ __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, cons
     }
     napp[j] = nrow[(size_t)RW * row + 19];
   }
-  SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
+  SBody A, B;
+  io.load(p, h, A, B);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (!((mask >> j) & 1)) continue;
@@ -549,8 +596,7 @@ __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, cons
       nn += (double)ap * (double)ap;
     }
   }
-  if (A.invMass != 0.0f) storeBody(svel, h.y, A);
-  if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+  io.store(p, h, A, B);
 }
 
 // one colour, one thread per manifold slot
@@ -560,7 +606,7 @@ __global__ __launch_bounds__(64) void kNormal(int off, int cnt, SolverParams SP,
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
   const int stop = done ? *done : 0;  // tolerance mode only
-  if (k < cnt && !stop) normalSlot(off + k, SP, hdr, nrow, props, svel, dd, nn);
+  if (k < cnt && !stop) normalSlot(off + k, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
   accumulate(dd, nn, res);
 }
 __global__ __launch_bounds__(64) void kFriction(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
@@ -570,7 +616,7 @@ __global__ __launch_bounds__(64) void kFriction(int off, int cnt, SolverParams S
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   double dd = 0.0, nn = 0.0;
   const int stop = done ? *done : 0;
-  if (k < cnt && !stop) frictionSlot(off + k, SP, hdr, nrow, frow, props, svel, dd, nn);
+  if (k < cnt && !stop) frictionSlot(off + k, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
   accumulate(dd, nn, res);
 }
 
@@ -586,7 +632,8 @@ __global__ __launch_bounds__(kTailThreads) void kNormalTail(int c0, int c1, cons
   if (!stop)
     for (int c = c0; c < c1; c++) {
       const int a = colorOff[c], b = colorOff[c + 1];
-      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads) normalSlot(p, SP, hdr, nrow, props, svel, dd, nn);
+      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads)
+        normalSlot(p, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
       __syncthreads();
     }
   accumulate(dd, nn, res);
@@ -602,9 +649,132 @@ __global__ __launch_bounds__(kTailThreads) void kFrictionTail(int c0, int c1, co
     for (int c = c0; c < c1; c++) {
       const int a = colorOff[c], b = colorOff[c + 1];
       for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads)
-        frictionSlot(p, SP, hdr, nrow, frow, props, svel, dd, nn);
+        frictionSlot(p, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
       __syncthreads();
     }
+  accumulate(dd, nn, res);
+}
+
+// ---- block-partitioned sweeps ---------------------------------------------------------------------
+// position of every body in this step's Morton order (its block is position / B), -1 for static bodies
+__global__ void kBlockPos(int n, const int* __restrict__ sorted, const float* __restrict__ props, int* pos) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int body = sorted[q];
+  pos[body] = isStaticBody(props, body) ? -1 : q;
+}
+// per active manifold: its block when its dynamic bodies share one (interior), else -1 (boundary)
+__global__ void kClassify(int na, int B, const int2* __restrict__ ab, const int* __restrict__ pos, int* phase) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= na) return;
+  const int2 b = ab[k];
+  const int pa = pos[b.x], pb = pos[b.y];
+  int blk;
+  if (pa < 0) blk = pb / B;
+  else if (pb < 0) blk = pa / B;
+  else blk = (pa / B == pb / B) ? pa / B : -1;
+  phase[k] = blk;
+}
+__global__ void kPassFlag(int na, const int* __restrict__ phase, int interior, int* flag) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < na) flag[k] = interior ? (phase[k] >= 0) : (phase[k] < 0);
+}
+// slot order key: boundary bit, colour, block (interior slots: colour-major, then block)
+__global__ void kSlotKeys(int na, const int* __restrict__ phase, const int* __restrict__ color,
+                          unsigned long long* key, int* idx) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= na) return;
+  const int ph = phase ? phase[k] : 0;
+  key[k] = (ph < 0 ? (1ull << 52) : 0ull) | ((unsigned long long)color[k] << 20) | (unsigned long long)(ph < 0 ? 0 : ph);
+  idx[k] = k;
+}
+// first slot of every (boundary bit, colour) batch
+__global__ void kBatchFlags(int na, const unsigned long long* __restrict__ key, int* flag) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < na) flag[p] = (p == 0) || ((key[p] >> 20) != (key[p - 1] >> 20));
+}
+// 1 for the batches made of boundary slots
+__global__ void kBatchBoundary(int nbatch, const int* __restrict__ batchStart, const unsigned long long* __restrict__ key,
+                               int* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbatch) out[i] = (int)((key[batchStart[i]] >> 52) & 1ull);
+}
+// slot range of every (interior batch, block) run; batchStart holds the first slot of each batch
+__global__ void kRunBounds(int interiorEnd, int nBatchInterior, const int* __restrict__ batchStart, int nBlocks,
+                           const unsigned long long* __restrict__ key, int* runStart, int* runEnd) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= interiorEnd) return;
+  int lo = 0, hi = nBatchInterior;  // batch = last start <= p
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (batchStart[mid] <= p) lo = mid; else hi = mid;
+  }
+  const int blk = (int)(key[p] & 0xfffffull);
+  const int r = lo * nBlocks + blk;
+  if (p == 0 || key[p] != key[p - 1]) runStart[r] = p;
+  if (p == interiorEnd - 1 || key[p] != key[p + 1]) runEnd[r] = p + 1;
+}
+// LDS positions of an interior slot's bodies (-1: static)
+__global__ void kSlotLocal(int n, int B, const int* __restrict__ corder, const int2* __restrict__ ab,
+                           const int* __restrict__ pos, int2* local) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int2 b = ab[corder[p]];
+  const int pa = pos[b.x], pb = pos[b.y];
+  local[p] = make_int2(pa < 0 ? -1 : pa % B, pb < 0 ? -1 : pb % B);
+}
+
+// One workgroup per block: the block's dynamic bodies are held in LDS while the workgroup sweeps the
+// block's interior slots colour after colour (workgroup barriers between colours); blocks share no
+// dynamic body, so the result equals sweeping the interior slots in slot order.
+constexpr int kBlockThreads = 256;
+template <bool Friction>
+__global__ __launch_bounds__(kBlockThreads) void kSweepBlocks(int nBatches, int nBlocks, int B, int nSorted,
+                                                              const int* __restrict__ sorted,
+                                                              const int* __restrict__ runStart,
+                                                              const int* __restrict__ runEnd, SolverParams SP,
+                                                              const int4* __restrict__ hdr,
+                                                              const int2* __restrict__ local, float* nrow, float* frow,
+                                                              const float* __restrict__ props, float* svel,
+                                                              double* res, const int* __restrict__ done) {
+  extern __shared__ float lds[];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int q0 = b * B, q1 = min(q0 + B, nSorted);
+  const int stop = done ? *done : 0;
+  for (int q = q0 + t; q < q1; q += kBlockThreads) {
+    const int body = sorted[q];
+    const float4* v = (const float4*)(svel + 8 * body);
+    float4 lv = v[0], av = v[1];
+    lv.w = props[8 * body];
+    ((float4*)lds)[2 * (q - q0)] = lv;
+    ((float4*)lds)[2 * (q - q0) + 1] = av;
+  }
+  __syncthreads();
+  double dd = 0.0, nn = 0.0;
+  const LdsBodies io{lds, local, svel, props};
+  if (!stop)
+    for (int ci = 0; ci < nBatches; ci++) {
+      const int s0 = runStart[ci * nBlocks + b];
+      if (s0 >= 0) {
+        const int s1 = runEnd[ci * nBlocks + b];
+        for (int p = s0 + t; p < s1; p += kBlockThreads) {
+          if (Friction)
+            frictionSlot(p, SP, hdr, nrow, frow, io, dd, nn);
+          else
+            normalSlot(p, SP, hdr, nrow, io, dd, nn);
+        }
+      }
+      __syncthreads();
+    }
+  for (int q = q0 + t; q < q1; q += kBlockThreads) {
+    const int body = sorted[q];
+    const float4 lv = ((const float4*)lds)[2 * (q - q0)];
+    if (lv.w == 0.0f) continue;  // static bodies keep their (unchanged) velocities
+    const float4 av = ((const float4*)lds)[2 * (q - q0) + 1];
+    float4* v = (float4*)(svel + 8 * body);
+    v[0] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+    v[1] = make_float4(av.x, av.y, av.z, 0.0f);
+  }
   accumulate(dd, nn, res);
 }
 
@@ -759,65 +929,136 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
-  // batches of rounds over the pending list, which is compacted between batches (the host waits only there)
+  // block partition of the dynamic bodies along this step's Morton order: interior / boundary manifolds
+  const bool blocked = c.blockSize > 0 && c.sortedOrder != nullptr && c.np == c.nb;
+  const int B = c.blockSize;
+  const int* phase = nullptr;
+  c.nBlocks = 0;
+  if (blocked) {
+    c.blockLocal.ensure(c.nb);
+    c.mPhase.ensure(na);
+    hipLaunchKernelGGL(kBlockPos, dim3(gridFor(c.np)), dim3(256), 0, s, c.np, c.sortedOrder, (const float*)c.props.p,
+                       c.blockLocal.p);
+    hipLaunchKernelGGL(kClassify, dim3(gridFor(na)), dim3(256), 0, s, na, B, (const int2*)c.activeBodies.p,
+                       (const int*)c.blockLocal.p, c.mPhase.p);
+    phase = c.mPhase.p;
+    c.nBlocks = (c.np + B - 1) / B;
+    if (c.nBlocks >= (1 << 20)) throw std::length_error("too many sweep blocks for the slot key");
+  }
+  // colouring: batches of rounds over the pending list, which is compacted between batches (the host waits
+  // only there); with blocks, interior and boundary manifolds are coloured separately
   c.pend.ensure(na);
   c.pendAlt.ensure(na);
   c.pendFlag.ensure(na);
-  hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
-  int round = 0, m = na;
-  for (int batch = 0;; batch++) {
-    const int rounds = batch == 0 ? 12 : 6;
-    for (int r = 0; r < rounds; r++, round++) {
-      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                         (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p,
-                         c.bodyPrio.p);
-      hipLaunchKernelGGL(kColorResolve, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                         (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p,
-                         (const unsigned long long*)c.bodyPrio.p, c.bodyMask.p);
+  int round = 0;
+  auto colourPass = [&](int m) {
+    for (int batch = 0; m > 0; batch++) {
+      const int rounds = batch == 0 ? 12 : 6;
+      for (int r = 0; r < rounds; r++, round++) {
+        hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                           (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p,
+                           c.bodyPrio.p);
+        hipLaunchKernelGGL(kColorResolve, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                           (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p,
+                           (const unsigned long long*)c.bodyPrio.p, c.bodyMask.p);
+      }
+      hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p,
+                         (const int*)c.color.p, c.pendFlag.p);
+      size_t tbs = 0;
+      JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m,
+                                             s));
+      c.cubTmp.ensure(tbs);
+      JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p,
+                                             (int*)c.counter.p, m, s));
+      int rem = 0;
+      JB_CHECK(hipMemcpyAsync(&rem, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      JB_CHECK(hipStreamSynchronize(s));
+      std::swap(c.pend.p, c.pendAlt.p);
+      std::swap(c.pend.n, c.pendAlt.n);
+      m = rem;
+      if (round > 4000) throw std::runtime_error("colouring did not converge");
     }
-    hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p,
-                       (const int*)c.color.p, c.pendFlag.p);
-    size_t tbs = 0;
-    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m, s));
-    c.cubTmp.ensure(tbs);
-    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m,
-                                           s));
-    int rem = 0;
-    JB_CHECK(hipMemcpyAsync(&rem, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    JB_CHECK(hipStreamSynchronize(s));
-    std::swap(c.pend.p, c.pendAlt.p);
-    std::swap(c.pend.n, c.pendAlt.n);
-    m = rem;
-    if (m == 0) break;
-    if (round > 4000) throw std::runtime_error("colouring did not converge");
+  };
+  if (!blocked) {
+    hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
+    colourPass(na);
+  } else {
+    for (int interior = 1; interior >= 0; interior--) {
+      hipLaunchKernelGGL(kPassFlag, dim3(gridFor(na)), dim3(256), 0, s, na, phase, interior, c.pendFlag.p);
+      hipcub::CountingInputIterator<int> it0(0);
+      size_t tbs = 0;
+      JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, it0, c.pendFlag.p, c.pend.p, (int*)c.counter.p, na, s));
+      c.cubTmp.ensure(tbs);
+      JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.pendFlag.p, c.pend.p, (int*)c.counter.p, na, s));
+      int m = 0;
+      JB_CHECK(hipMemcpyAsync(&m, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
+      JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
+      JB_CHECK(hipStreamSynchronize(s));
+      colourPass(m);
+    }
   }
-  // order by colour (stable in active index) and colour offsets
-  const int maxColor = 64 + round + 1;
-  int bitsC = 1;
-  while ((1 << bitsC) <= maxColor) bitsC++;
-  c.typeKeyAlt.ensure(na);
+  if (round >= (1 << 10) - 64) throw std::runtime_error("colour index exceeds the slot key");
+  // slot order: interior slots by (colour, block), then boundary slots by colour
+  c.slotKey.ensure(na);
+  c.slotKeyAlt.ensure(na);
   c.permAlt.ensure(na);
-  c.colorStart.ensure(maxColor);
-  hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p);
-  JB_CHECK(hipMemcpyAsync(c.colorAlt.p, c.color.p, (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
-  hipcub::DoubleBuffer<int> dk(c.colorAlt.p, c.typeKeyAlt.p);
+  hipLaunchKernelGGL(kSlotKeys, dim3(gridFor(na)), dim3(256), 0, s, na, phase, (const int*)c.color.p, c.slotKey.p,
+                     c.colorOrder.p);
+  hipcub::DoubleBuffer<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
   hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
   size_t tb2 = 0;
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, 0, bitsC, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, 0, 53, s));
   c.cubTmp.ensure(tb2);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, 0, bitsC, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, 0, 53, s));
   if (dv.Current() != c.colorOrder.p)
     JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
-  JB_CHECK(hipMemsetAsync(c.colorStart.p, 0xff, maxColor * sizeof(int), s));
-  hipLaunchKernelGGL(kColorStarts, dim3(gridFor(na)), dim3(256), 0, s, na, dk.Current(), c.colorStart.p);
-  std::vector<int> starts(maxColor);
-  JB_CHECK(hipMemcpyAsync(starts.data(), c.colorStart.p, maxColor * sizeof(int), hipMemcpyDeviceToHost, s));
+  const unsigned long long* skey = dk.Current();
+  // batches: first slot of every (boundary bit, colour)
+  c.batchFlag.ensure(na);
+  c.colorStart.ensure(na + 1);
+  hipLaunchKernelGGL(kBatchFlags, dim3(gridFor(na)), dim3(256), 0, s, na, skey, c.batchFlag.p);
+  {
+    hipcub::CountingInputIterator<int> it0(0);
+    size_t tbs = 0;
+    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na, s));
+    c.cubTmp.ensure(tbs);
+    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na,
+                                           s));
+  }
+  int nbatch = 0;
+  JB_CHECK(hipMemcpyAsync(&nbatch, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
-  for (int k = 0; k < maxColor; k++)
-    if (starts[k] > 0) c.colorOff.push_back(starts[k]);
+  std::vector<int> starts(nbatch), boundary(nbatch);
+  c.batchFlag.ensure(na + (size_t)nbatch);
+  int* bnd = c.batchFlag.p + na;
+  hipLaunchKernelGGL(kBatchBoundary, dim3(gridFor(nbatch)), dim3(256), 0, s, nbatch, (const int*)c.colorStart.p, skey,
+                     bnd);
+  JB_CHECK(hipMemcpyAsync(starts.data(), c.colorStart.p, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(boundary.data(), bnd, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  int nInterior = 0;
+  for (int k = 0; k < nbatch; k++) {
+    if (k > 0) c.colorOff.push_back(starts[k]);
+    if (blocked && !boundary[k]) nInterior++;
+  }
   c.colorOff.push_back(na);
   c.ncolors = (int)c.colorOff.size() - 1;
   c.st.colors = c.ncolors;
+  c.interiorColors = nInterior;
+  c.interiorEnd = c.colorOff[nInterior];
+  if (nInterior > 0) {  // per (interior batch, block): slot range; per interior slot: LDS body positions
+    const size_t nr = (size_t)nInterior * c.nBlocks;
+    c.runStart.ensure(nr);
+    c.runEnd.ensure(nr);
+    JB_CHECK(hipMemsetAsync(c.runStart.p, 0xff, nr * sizeof(int), s));
+    hipLaunchKernelGGL(kRunBounds, dim3(gridFor(c.interiorEnd)), dim3(256), 0, s, (int)c.interiorEnd, nInterior,
+                       (const int*)c.colorStart.p, c.nBlocks, skey, c.runStart.p, c.runEnd.p);
+    c.slotLocal.ensure(c.interiorEnd);
+    hipLaunchKernelGGL(kSlotLocal, dim3(gridFor(c.interiorEnd)), dim3(256), 0, s, (int)c.interiorEnd, B,
+                       (const int*)c.colorOrder.p, (const int2*)c.activeBodies.p, (const int*)c.blockLocal.p,
+                       c.slotLocal.p);
+  }
   tEnd(c, T_COLOR);
   // setup: rows for every slot, then the warm-start impulses colour by colour
   tBegin(c, T_SOLVE_SETUP);
@@ -842,9 +1083,12 @@ void solverSolve(Ctx& c, float dt) {
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
                      SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
-  // colours [0, headColors) get a launch each; the trailing small ones share one single-workgroup launch
+  // per-colour launches: with blocks, the boundary colours [firstColor, ncolors); without, colours
+  // [0, headColors) get a launch each and the trailing small ones share one single-workgroup launch
+  const int firstColor = c.interiorColors;
+  const bool blockSweeps = firstColor > 0;
   int headColors = c.ncolors;
-  if (c.tailThreshold > 0) {
+  if (!blocked && c.tailThreshold > 0) {
     while (headColors > 0 && c.colorOff[headColors] - c.colorOff[headColors - 1] <= c.tailThreshold) headColors--;
     if (c.ncolors - headColors < 2) headColors = c.ncolors;
   }
@@ -856,8 +1100,10 @@ void solverSolve(Ctx& c, float dt) {
   }
   c.rowCount.ensure(2);
   JB_CHECK(hipMemsetAsync(c.rowCount.p, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(kCountRows, dim3(gridFor(na)), dim3(256), 0, s, na, (int)c.colorOff[headColors], c.hdr.p,
-                     c.rowCount.p);
+  // rows: all slots, and the slots swept by the timed per-colour launches' counterpart (block sweeps: the
+  // interior slots; otherwise the head colours)
+  hipLaunchKernelGGL(kCountRows, dim3(gridFor(na)), dim3(256), 0, s, na,
+                     blockSweeps ? (int)c.interiorEnd : (int)c.colorOff[headColors], c.hdr.p, c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
   const int apply = (c.warmstart || c.useLearned || c.useMlp) ? 1 : 0;
   for (int k = 0; k < c.ncolors; k++) {
@@ -892,11 +1138,39 @@ void solverSolve(Ctx& c, float dt) {
     kUsed += 2;
     return &c.kEv[kUsed - 2];
   };
+  const size_t blockLds = (size_t)B * 8 * sizeof(float);
+  auto sweepBlocks = [&](bool friction, double* res) {
+    if (!blockSweeps) return;
+    const int nb = c.nBlocks, nbt = c.interiorColors;
+    const dim3 grid(nb), block(kBlockThreads);
+    const int4* hd = c.hdr.p;
+    const int2* lo = c.slotLocal.p;
+    const float* pr = c.props.p;
+    const int* rs = c.runStart.p;
+    const int* re = c.runEnd.p;
+    const int* dn = done;
+    if (sampleK) {
+      hipEvent_t* ev = kPair(friction ? 3 : 2);
+      if (friction)
+        hipExtLaunchKernelGGL(kSweepBlocks<true>, grid, block, blockLds, s, ev[0], ev[1], 0, nbt, nb, B, c.np,
+                              c.sortedOrder, rs, re, SP, hd, lo, c.nrow.p, c.frow.p, pr, c.svel.p, res, dn);
+      else
+        hipExtLaunchKernelGGL(kSweepBlocks<false>, grid, block, blockLds, s, ev[0], ev[1], 0, nbt, nb, B, c.np,
+                              c.sortedOrder, rs, re, SP, hd, lo, c.nrow.p, c.frow.p, pr, c.svel.p, res, dn);
+    } else if (friction) {
+      hipLaunchKernelGGL(kSweepBlocks<true>, grid, block, blockLds, s, nbt, nb, B, c.np, c.sortedOrder, rs, re, SP, hd,
+                         lo, c.nrow.p, c.frow.p, pr, c.svel.p, res, dn);
+    } else {
+      hipLaunchKernelGGL(kSweepBlocks<false>, grid, block, blockLds, s, nbt, nb, B, c.np, c.sortedOrder, rs, re, SP,
+                         hd, lo, c.nrow.p, c.frow.p, pr, c.svel.p, res, dn);
+    }
+  };
   tBegin(c, T_SOLVE_ITER);
   for (int itn = 0; itn < iters; itn++) {
     const bool track = tolMode || itn == iters - 1;
     double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
-    for (int k = 0; k < headColors; k++) {
+    sweepBlocks(false, res);
+    for (int k = firstColor; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
       if (sampleK) {
         hipEvent_t* ev = kPair(0);
@@ -912,7 +1186,8 @@ void solverSolve(Ctx& c, float dt) {
       hipLaunchKernelGGL(kNormalTail, dim3(1), dim3(kTailThreads), 0, s, headColors, c.ncolors,
                          (const int*)c.colorOffDev.p, SP, (const int4*)c.hdr.p, c.nrow.p, (const float*)c.props.p,
                          c.svel.p, res, (const int*)done);
-    for (int k = 0; k < headColors; k++) {
+    sweepBlocks(true, res);
+    for (int k = firstColor; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
       if (sampleK) {
         hipEvent_t* ev = kPair(1);
@@ -931,7 +1206,9 @@ void solverSolve(Ctx& c, float dt) {
     if (track)
       hipLaunchKernelGGL(kResidual, dim3(1), dim3(64), 0, s, itn, c.tolerance, res, c.resid.p, c.done.p, resOut);
   }
-  tEnd(c, T_SOLVE_ITER, (int64_t)iters * 2 * (headColors + (headColors < c.ncolors ? 1 : 0)));
+  JB_CHECK(hipGetLastError());
+  tEnd(c, T_SOLVE_ITER,
+       (int64_t)iters * 2 * (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)));
   hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
   float ro[2] = {0, 0};
@@ -942,7 +1219,9 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemcpyAsync(rowsHT, c.rowCount.p, sizeof(rowsHT), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
   c.st.rows = (int64_t)rowsHT[0];
-  const int64_t rows = (int64_t)rowsHT[1];  // rows swept by the per-colour (timed) launches
+  // rows swept per sweep by the per-colour launches (kinds 0, 1) and by the block launches (kinds 2, 3)
+  const int64_t colourRows = blockSweeps ? (int64_t)(rowsHT[0] - rowsHT[1]) : (int64_t)rowsHT[1];
+  const int64_t blockRows = blockSweeps ? (int64_t)rowsHT[1] : 0;
   if (sampleK) {
     for (size_t e = 0; e < kUsed; e += 2) {
       float ms = 0.0f;
@@ -951,8 +1230,10 @@ void solverSolve(Ctx& c, float dt) {
       c.kMs[kind] += ms;
       c.kLaunches[kind] += 1;
     }
-    c.kRows[0] += (int64_t)rows * iters;
-    c.kRows[1] += 2 * (int64_t)rows * iters;
+    c.kRows[0] += colourRows * iters;
+    c.kRows[1] += 2 * colourRows * iters;
+    c.kRows[2] += blockRows * iters;
+    c.kRows[3] += 2 * blockRows * iters;
   }
   c.st.residual = ro[0];
   c.st.iterations = (int64_t)ro[1];

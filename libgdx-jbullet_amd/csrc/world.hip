@@ -809,6 +809,13 @@ int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterati
   });
 }
 
+int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size) {
+  return guard(ctx, [&] {
+    if (block_size < 0 || block_size > 2048) throw std::invalid_argument("block size must be in [0, 2048]");
+    ctx->c.blockSize = block_size;
+  });
+}
+
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor) {
   return guard(ctx, [&] {
     if (!std::isfinite(factor)) throw std::invalid_argument("warm-start factor must be finite");
@@ -937,7 +944,7 @@ int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every) {
     if (every < 0) throw std::invalid_argument("negative sampling period");
     Ctx& c = ctx->c;
     c.kernelEvery = every;
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 4; k++) {
       c.kMs[k] = 0;
       c.kLaunches[k] = 0;
       c.kRows[k] = 0;
@@ -948,7 +955,7 @@ int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every) {
 int jbgpu_timing_kernels_get(jbgpu_ctx* ctx, double* ms2, int64_t* launches2, int64_t* rows2) {
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < 4; k++) {
       if (ms2) ms2[k] = c.kMs[k];
       if (launches2) launches2[k] = c.kLaunches[k];
       if (rows2) rows2[k] = c.kRows[k];

@@ -75,6 +75,7 @@ SYMBOLS = {
     "jbgpu_solver_get_impulses": (C.c_int, [_vp, _f, C.c_int64]),
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_tail": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_solver_set_blocks": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_solver_set_warmstart_factor": (C.c_int, [_vp, C.c_float]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
@@ -355,6 +356,9 @@ class Context:
     def solver_set_warmstart_factor(self, factor):
         self._ck(load().jbgpu_solver_set_warmstart_factor(self.h, float(factor)), "solver_set_warmstart_factor")
 
+    def solver_set_blocks(self, block_size):
+        self._ck(load().jbgpu_solver_set_blocks(self.h, int(block_size)), "solver_set_blocks")
+
     def solver_set_tail(self, threshold):
         self._ck(load().jbgpu_solver_set_tail(self.h, int(threshold)), "solver_set_tail")
 
@@ -393,12 +397,12 @@ class Context:
         self._ck(load().jbgpu_timing_kernels(self.h, int(every)), "timing_kernels")
 
     def timing_kernels_get(self):
-        ms = np.zeros(2, np.float64)
-        ln = np.zeros(2, np.int64)
-        rows = np.zeros(2, np.int64)
+        ms = np.zeros(4, np.float64)
+        ln = np.zeros(4, np.int64)
+        rows = np.zeros(4, np.int64)
         self._ck(load().jbgpu_timing_kernels_get(self.h, _p(ms), _p(ln), _p(rows)), "timing_kernels_get")
         return {name: dict(ms=float(ms[i]), launches=int(ln[i]), rows=int(rows[i]))
-                for i, name in enumerate(("normal", "friction"))}
+                for i, name in enumerate(("normal", "friction", "normal_blocks", "friction_blocks"))}
 
     def timing_get(self):
         ms = np.zeros(TIMER_REGIONS, np.float64)

@@ -98,13 +98,15 @@ def _compare_world(ctx, orc, tag):
         raise AssertionError(f"{tag}: {len(bad)} manifolds differ, first pair index {bad[0]}")
 
 
-@pytest.mark.parametrize("tail", [1024, 0])
-def test_world_boxes_grid_step_parity(tail):
-    """tail: trailing colours of at most this many manifolds are swept in one workgroup (0: one launch each)."""
+@pytest.mark.parametrize("blocks,tail", [(2048, 0), (256, 0), (0, 1024), (0, 0)])
+def test_world_boxes_grid_step_parity(blocks, tail):
+    """Launch shapes of the sweeps: blocks = bodies per block of the block-partitioned sweep (0: none);
+    tail: trailing colours of at most this many manifolds swept in one workgroup (0: one launch each)."""
     sc = scenes.boxes_grid(seed=1)
     ctx, orc = _world_pair(sc)
+    ctx.solver_set_blocks(blocks)
     ctx.solver_set_tail(tail)
-    for step in range(300 if tail else 90):  # the 1000-box configuration runs 300 steps
+    for step in range(300 if blocks == 2048 else 90):  # the 1000-box configuration runs 300 steps
         _sync_and_step(ctx, orc)
         st = ctx.stats()
         if step % 15 == 0:
