@@ -144,7 +144,7 @@ int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterati
 /* Launch shaping of the sweeps (no reference counterpart; results are identical for every value):
  * dynamic bodies are grouped into blocks of `block_size` consecutive Morton positions (at most 2048);
  * manifolds inside a block are swept by one workgroup per block with the block's velocities in LDS,
- * colour after colour, the others by one launch per colour.  0 disables blocks.  Default 2048. */
+ * colour after colour, the others by one launch per colour.  0 (the default) disables blocks. */
 int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size);
 /* ContactSolverInfo.warmstartingFactor for the following solves (0.85 by default). */
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor);

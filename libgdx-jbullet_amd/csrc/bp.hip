@@ -28,6 +28,8 @@ constexpr int STAGECOUNT = 2;
 constexpr float DBVT_BP_MARGIN = 0.05f;
 constexpr float PREDICTED_FRAMES = 2.0f;
 constexpr float HUGE_EXTENT = 1e6f;
+constexpr float kLargeFactor = 32.0f;  // relative size that also sends a box to the flat huge-proxy test
+constexpr int kLargeCap = 64;          // ... when no more than this many boxes qualify
 
 struct BpParams {
   int mode;
@@ -473,6 +475,49 @@ __global__ void kHuge(int n, int nh, const int* __restrict__ hugeList, const int
   }
 }
 
+// mean largest extent of this step's non-huge candidate boxes (sum and count, one atomic pair per wave)
+__global__ void kExtentStats(int n, const float* __restrict__ cand, const int* __restrict__ isHuge, float* stats) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float e = 0.0f, k = 0.0f;
+  if (i < n && !isHuge[i]) {
+    const float* C = cand + 8 * i;
+    e = fmaxf(fmaxf(C[4] - C[0], C[5] - C[1]), C[6] - C[2]);
+    if (!(e == e)) e = 0.0f;
+    k = 1.0f;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    e += __shfl_xor(e, o);
+    k += __shfl_xor(k, o);
+  }
+  if ((threadIdx.x & 63) == 0 && k > 0.0f) {
+    atomicAdd(stats, e);
+    atomicAdd(stats + 1, k);
+  }
+}
+// boxes far larger than the typical one (static walls, terrain slabs) leave the tree for the flat per-world
+// test of kHuge: one such box overlaps thousands of others, which a single traversal thread would otherwise
+// have to report one after another.  Two passes: count the candidates, then mark them only when there are at
+// most kLargeCap (kHuge tests every proxy against each of them)
+__device__ __forceinline__ bool isLarge(int i, float factor, const float* cand, const float* stats) {
+  const float thr = factor * (stats[0] / stats[1]);
+  const float* C = cand + 8 * i;
+  const float e = fmaxf(fmaxf(C[4] - C[0], C[5] - C[1]), C[6] - C[2]);
+  return e > thr;
+}
+__global__ void kCountLarge(int n, float factor, const float* __restrict__ cand, float* stats,
+                            const int* __restrict__ isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool large = i < n && !isHuge[i] && stats[1] > 0.0f && isLarge(i, factor, cand, stats);
+  const unsigned long long b = __ballot(large);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(stats + 2, (float)__popcll(b));
+}
+__global__ void kMarkLarge(int n, float factor, int cap, const float* __restrict__ cand,
+                           const float* __restrict__ stats, int* isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || isHuge[i] || !(stats[1] > 0.0f) || stats[2] > (float)cap) return;
+  if (isLarge(i, factor, cand, stats)) isHuge[i] = 2;
+}
+
 __global__ void kFlagHugeList(int n, const int* __restrict__ isHuge, const int* __restrict__ world, int* list, int* wk,
                               unsigned long long* counter) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -568,7 +613,7 @@ void ensureProxyBuffers(Ctx& c) {
   c.hugeListAlt.ensure(n);
   c.hugeWorld.ensure(n);
   c.hugeWorldAlt.ensure(n);
-  c.bounds.ensure(8);
+  c.bounds.ensure(16);  // [0, 6) centroid bounds, [8, 11) extent statistics
   c.counter.ensure(8);
 }
 
@@ -618,7 +663,15 @@ void bpFindPairs(Ctx& c) {
   const int n = c.np;
   const int bits = bitsFor((int64_t)n + 1);
   hipStream_t s = c.stream;
-  // huge proxies: list sorted by (world, index)
+  // huge proxies (absolutely or relatively large): list sorted by (world, index)
+  float* stats = c.bounds.p + 8;
+  JB_CHECK(hipMemsetAsync(stats, 0, 3 * sizeof(float), s));
+  hipLaunchKernelGGL(kExtentStats, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)c.cand.p,
+                     (const int*)c.isHuge.p, stats);
+  hipLaunchKernelGGL(kCountLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, (const float*)c.cand.p,
+                     stats, (const int*)c.isHuge.p);
+  hipLaunchKernelGGL(kMarkLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, kLargeCap,
+                     (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   hipLaunchKernelGGL(kFlagHugeList, dim3(gridFor(n)), dim3(256), 0, s, n, c.isHuge.p, c.world.p, c.hugeList.p,
                      c.hugeWorld.p, c.counter.p + 1);

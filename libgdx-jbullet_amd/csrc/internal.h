@@ -163,7 +163,7 @@ struct Ctx {
   // block-partitioned sweeps: dynamic bodies in blocks of blockSize consecutive Morton positions; manifolds
   // inside one block ("interior") are swept by one workgroup per block with the block's velocities in LDS,
   // the others ("boundary") by one launch per colour
-  int blockSize = 2048;               // 0: every colour swept by its own launch
+  int blockSize = 0;                  // 0: every colour swept by its own launch
   const int* sortedOrder = nullptr;   // Morton-sorted proxy order of this step's broadphase
   int nBlocks = 0, interiorColors = 0;
   int64_t interiorEnd = 0;            // slots [0, interiorEnd) are interior
