@@ -48,7 +48,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-every", type=int, default=10,
+    ap.add_argument("--kernel-every", type=int, default=25,
                     help="time each solver sweep launch with its own events on every k-th timed step")
     ap.add_argument("--batched-worlds", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=100)
@@ -56,7 +56,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-broadphase", action="store_true")
     ap.add_argument("--bp-frames", type=int, default=20)
+    ap.add_argument("--phase-steps", type=int, default=30, help="instrumented steps after the timed region")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--sweep-shape", default=None, metavar="BLOCKS,TAIL",
+                    help="fix the sweep launch shape instead of autotuning it (e.g. 0,256)")
+    ap.add_argument("--section", choices=("batched", "learned", "broadphase_1m"),
+                    help="run only this secondary section and print its JSON (for profiling it alone)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-kind", default="friction", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-blocks", type=int, default=0, help=argparse.SUPPRESS)
@@ -71,6 +76,10 @@ def main():
 
     if args.pmc_child:
         return pmc_child(args)
+    if args.section:
+        fn = {"batched": batched, "learned": learned_warm_start, "broadphase_1m": broadphase_1m}[args.section]
+        print(json.dumps({args.section: fn(args)}))
+        return
     t = time.time()
     sc = scenes.mixed_pile(seed=args.seed, n=args.bodies)
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
@@ -85,8 +94,14 @@ def main():
             st = ctx.stats()
             log(f"[bench] warmup {k + 1}/{args.warmup}: pairs {st['pairs']} manifolds {st['manifolds']} "
                 f"colors {st['colors']} epa {st['epa_calls']} ({time.time() - t:.1f}s)")
-    tune = autotune(ctx, dt)
-    ctx.timing(True)
+    if args.sweep_shape:
+        blocks, tail = (int(v) for v in args.sweep_shape.split(","))
+        ctx.solver_set_blocks(blocks)
+        ctx.solver_set_tail(tail)
+        tune = {"block_size": blocks, "tail_threshold": tail, "ms_per_step": None}
+    else:
+        tune = autotune(ctx, dt)
+    # timed region: only the sampled per-launch sweep events (the roofline's kernel durations) are recorded
     ctx.timing_kernels(args.kernel_every)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -96,11 +111,19 @@ def main():
     t1 = time.perf_counter()
     secs = t1 - t0
     st = ctx.stats()
-    regions, nsteps = ctx.timing_get()
     value = args.bodies * args.steps / secs
+    # phase breakdown from an instrumented pass after the timed region (events between phases)
+    kstats = ctx.timing_kernels_get()
+    ctx.timing_kernels(0)
+    regions, nsteps = {}, 0
+    if args.phase_steps > 0:
+        ctx.timing(True)
+        ctx.step(dt, n=args.phase_steps)
+        regions, nsteps = ctx.timing_get()
+        ctx.timing(False)
 
     tim = {k: v for k, v in regions.items() if v[1] > 0}
-    roof = roofline(ctx, args)
+    roof = roofline(kstats, args)
     if roof is not None:
         roof["copy_ceiling_gbs"] = round(ctx.measure_copy(1 << 30, 10), 1)
     out = {
@@ -122,6 +145,7 @@ def main():
                    "bodies": args.bodies, "solver_iterations": args.iterations, "broadphase": "dbvt",
                    "solver_order": "colour batches"},
         "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in tim.items()},
+        "phase_note": f"event-instrumented pass of {nsteps} steps after the timed region",
         "last_step": {k: st[k] for k in ("pairs", "contacts", "manifolds", "colors", "epa_calls", "epa_fails")},
         "roofline": roof,
         "autotune": tune,
@@ -244,11 +268,10 @@ def autotune(ctx, dt, steps=8):
             "ms_per_step": {f"blocks={b},tail={t}": round(v, 3) for (b, t), v in ms.items()}}
 
 
-def roofline(ctx, args):
+def roofline(ks, args):
     """Dominant kernel: the solver's colour sweeps.  Each sampled launch is timed by its own start/stop events
     on the library stream (kernel execution only); achieved = algorithmic bytes per launch / average duration,
     with ROW_BYTES per row per sweep (80 B row + 4 B impulse + 2 x 48 B body velocity read and write)."""
-    ks = ctx.timing_kernels_get()
     name = max(ks, key=lambda k: ks[k]["ms"])
     k = ks[name]
     if k["launches"] == 0:

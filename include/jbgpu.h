@@ -188,8 +188,8 @@ int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 #define JBGPU_TIMER_REGIONS 16
 int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
-/* Kernel-level timing of the solver sweeps while timing is enabled: on every `every`-th timed step
- * (0 = off) each normal-row and friction-row colour launch carries its own start/stop events
+/* Kernel-level timing of the solver sweeps (independent of jbgpu_timing_enable): on every `every`-th
+ * solve after this call (0 = off) each sweep launch carries its own start/stop events
  * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps: summed ms,
  * timed launches and rows processed by them. */
 int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every);

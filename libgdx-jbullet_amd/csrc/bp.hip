@@ -166,38 +166,118 @@ __device__ __forceinline__ float ordF(unsigned int u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void kBoundsInit(unsigned int* b) {
-  if (threadIdx.x < 3) b[threadIdx.x] = 0xffffffffu;
-  else if (threadIdx.x < 6) b[threadIdx.x] = 0u;
+// Per-step proxy statistics in two stages (per-block partials, then one block), with no atomics: a few
+// addresses updated by every wave serialise.  Over the proxies that are not huge: the bounds of the candidate-box
+// centres (order-preserving uint form) and the sum and count of the largest box extents.
+constexpr int kStatBlocks = 256;
+constexpr int kStatWords = 8;  // lo[3], hi[3] (uint), extent sum, count (float)
+
+struct ProxyStat {
+  unsigned int lo[3], hi[3];
+  float esum, ecnt;
+  __device__ void init() {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = 0xffffffffu;
+      hi[k] = 0u;
+    }
+    esum = 0.0f;
+    ecnt = 0.0f;
+  }
+  __device__ void merge(const ProxyStat& o) {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = min(lo[k], o.lo[k]);
+      hi[k] = max(hi[k], o.hi[k]);
+    }
+    esum += o.esum;
+    ecnt += o.ecnt;
+  }
+  __device__ void shfl(int off) {
+    ProxyStat o;
+    for (int k = 0; k < 3; k++) {
+      o.lo[k] = (unsigned int)__shfl_xor((int)lo[k], off);
+      o.hi[k] = (unsigned int)__shfl_xor((int)hi[k], off);
+    }
+    o.esum = __shfl_xor(esum, off);
+    o.ecnt = __shfl_xor(ecnt, off);
+    merge(o);
+  }
+  __device__ void store(unsigned int* w) const {
+    for (int k = 0; k < 3; k++) {
+      w[k] = lo[k];
+      w[3 + k] = hi[k];
+    }
+    w[6] = __float_as_uint(esum);
+    w[7] = __float_as_uint(ecnt);
+  }
+  __device__ void load(const unsigned int* w) {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = w[k];
+      hi[k] = w[3 + k];
+    }
+    esum = __uint_as_float(w[6]);
+    ecnt = __uint_as_float(w[7]);
+  }
+};
+
+// block-wide merge in a fixed order (waves by shuffles, then the block's waves in index order): deterministic
+__device__ void blockMerge(ProxyStat& st, unsigned int* lds) {
+  for (int off = 32; off > 0; off >>= 1) st.shfl(off);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) st.store(lds + kStatWords * wave);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st.load(lds);
+    for (int w = 1; w < nw; w++) {
+      ProxyStat o;
+      o.load(lds + kStatWords * w);
+      st.merge(o);
+    }
+  }
 }
 
-__global__ void kBounds(int n, const float* __restrict__ cand, const int* __restrict__ isHuge, unsigned int* b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned int lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
-  if (i < n && !isHuge[i]) {
-    const float* C = cand + 8 * i;
-    for (int k = 0; k < 3; k++) {
-      float c = (C[k] + C[4 + k]) * 0.5f;
-      if (c == c) {
-        lo[k] = fOrd(c);
-        hi[k] = fOrd(c);
+__global__ __launch_bounds__(256) void kProxyStats(int n, const float* __restrict__ cand,
+                                                   const int* __restrict__ isHuge, unsigned int* part) {
+  __shared__ unsigned int lds[kStatWords * 4];
+  ProxyStat st;
+  st.init();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (isHuge[i]) continue;
+    const float4 mn = ((const float4*)cand)[2 * i], mx = ((const float4*)cand)[2 * i + 1];
+    const float c3[3] = {(mn.x + mx.x) * 0.5f, (mn.y + mx.y) * 0.5f, (mn.z + mx.z) * 0.5f};
+    for (int k = 0; k < 3; k++)
+      if (c3[k] == c3[k]) {
+        st.lo[k] = min(st.lo[k], fOrd(c3[k]));
+        st.hi[k] = max(st.hi[k], fOrd(c3[k]));
       }
-    }
+    float e = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+    if (!(e == e)) e = 0.0f;
+    st.esum += e;
+    st.ecnt += 1.0f;
   }
-  for (int k = 0; k < 3; k++) {
-    for (int off = 32; off > 0; off >>= 1) {
-      lo[k] = min(lo[k], (unsigned int)__shfl_xor((int)lo[k], off));
-      hi[k] = max(hi[k], (unsigned int)__shfl_xor((int)hi[k], off));
-    }
-  }
-  if ((threadIdx.x & 63) == 0) {
+  blockMerge(st, lds);
+  if (threadIdx.x == 0) st.store(part + kStatWords * blockIdx.x);
+}
+
+// bounds[0, 6): centre bounds; stats[0, 3): extent sum, count, and the large-box count (cleared)
+__global__ __launch_bounds__(256) void kProxyStatsFinal(int nparts, const unsigned int* __restrict__ part,
+                                                        unsigned int* bounds, float* stats) {
+  __shared__ unsigned int lds[kStatWords * 4];
+  ProxyStat st;
+  st.init();
+  if ((int)threadIdx.x < nparts) st.load(part + kStatWords * threadIdx.x);
+  blockMerge(st, lds);
+  if (threadIdx.x == 0) {
     for (int k = 0; k < 3; k++) {
-      atomicMin(b + k, lo[k]);
-      atomicMax(b + 3 + k, hi[k]);
+      bounds[k] = st.lo[k];
+      bounds[3 + k] = st.hi[k];
     }
+    stats[0] = st.esum;
+    stats[1] = st.ecnt;
+    stats[2] = 0.0f;
   }
 }
 
+// spreads the low 10 bits of v to every third bit
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   v = (v * 0x00010001u) & 0xFF0000FFu;
   v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -475,25 +555,6 @@ __global__ void kHuge(int n, int nh, const int* __restrict__ hugeList, const int
   }
 }
 
-// mean largest extent of this step's non-huge candidate boxes (sum and count, one atomic pair per wave)
-__global__ void kExtentStats(int n, const float* __restrict__ cand, const int* __restrict__ isHuge, float* stats) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float e = 0.0f, k = 0.0f;
-  if (i < n && !isHuge[i]) {
-    const float* C = cand + 8 * i;
-    e = fmaxf(fmaxf(C[4] - C[0], C[5] - C[1]), C[6] - C[2]);
-    if (!(e == e)) e = 0.0f;
-    k = 1.0f;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    e += __shfl_xor(e, o);
-    k += __shfl_xor(k, o);
-  }
-  if ((threadIdx.x & 63) == 0 && k > 0.0f) {
-    atomicAdd(stats, e);
-    atomicAdd(stats + 1, k);
-  }
-}
 // boxes far larger than the typical one (static walls, terrain slabs) leave the tree for the flat per-world
 // test of kHuge: one such box overlaps thousands of others, which a single traversal thread would otherwise
 // have to report one after another.  Two passes: count the candidates, then mark them only when there are at
@@ -518,13 +579,9 @@ __global__ void kMarkLarge(int n, float factor, int cap, const float* __restrict
   if (isLarge(i, factor, cand, stats)) isHuge[i] = 2;
 }
 
-__global__ void kFlagHugeList(int n, const int* __restrict__ isHuge, const int* __restrict__ world, int* list, int* wk,
-                              unsigned long long* counter) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !isHuge[i]) return;
-  unsigned long long k = atomicAdd(counter, 1ull);
-  list[k] = i;
-  wk[k] = world[i];
+__global__ void kHugeWorlds(int nh, const int* __restrict__ list, const int* __restrict__ world, int* wk) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nh) wk[k] = world[list[k]];
 }
 
 // remap: unpack sorted keys, carry manifolds of surviving pairs, initialise new ones
@@ -613,7 +670,7 @@ void ensureProxyBuffers(Ctx& c) {
   c.hugeListAlt.ensure(n);
   c.hugeWorld.ensure(n);
   c.hugeWorldAlt.ensure(n);
-  c.bounds.ensure(16);  // [0, 6) centroid bounds, [8, 11) extent statistics
+  c.bounds.ensure(16 + kStatWords * kStatBlocks);  // [0, 6) centre bounds, [8, 11) extent statistics, partials
   c.counter.ensure(8);
 }
 
@@ -663,27 +720,33 @@ void bpFindPairs(Ctx& c) {
   const int n = c.np;
   const int bits = bitsFor((int64_t)n + 1);
   hipStream_t s = c.stream;
-  // huge proxies (absolutely or relatively large): list sorted by (world, index)
+  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+  // centre bounds and extent statistics (two-stage reduction)
+  tBegin(c, T_BVH);
+  unsigned int* bounds = (unsigned int*)c.bounds.p;
   float* stats = c.bounds.p + 8;
-  JB_CHECK(hipMemsetAsync(stats, 0, 3 * sizeof(float), s));
-  hipLaunchKernelGGL(kExtentStats, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)c.cand.p,
-                     (const int*)c.isHuge.p, stats);
+  unsigned int* part = (unsigned int*)c.bounds.p + 16;
+  const int nparts = std::min(gridFor(n), kStatBlocks);
+  hipLaunchKernelGGL(kProxyStats, dim3(nparts), dim3(256), 0, s, n, (const float*)c.cand.p, (const int*)c.isHuge.p,
+                     part);
+  hipLaunchKernelGGL(kProxyStatsFinal, dim3(1), dim3(256), 0, s, nparts, (const unsigned int*)part, bounds, stats);
+  // huge proxies (absolutely or relatively large), listed in index order
   hipLaunchKernelGGL(kCountLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, (const float*)c.cand.p,
                      stats, (const int*)c.isHuge.p);
   hipLaunchKernelGGL(kMarkLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, kLargeCap,
                      (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
-  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(kFlagHugeList, dim3(gridFor(n)), dim3(256), 0, s, n, c.isHuge.p, c.world.p, c.hugeList.p,
-                     c.hugeWorld.p, c.counter.p + 1);
-  // centroid bounds and Morton codes
-  tBegin(c, T_BVH);
-  hipLaunchKernelGGL(kBoundsInit, dim3(1), dim3(64), 0, s, (unsigned int*)c.bounds.p);
-  hipLaunchKernelGGL(kBounds, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, (unsigned int*)c.bounds.p);
+  int* hugeCount = (int*)(c.counter.p + 1);
+  size_t tbSel = 0;
+  hipcub::CountingInputIterator<int> ids(0);
+  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbSel, ids, (const int*)c.isHuge.p, c.hugeList.p, hugeCount, n, s));
+  c.cubTmp.ensure(tbSel);
+  JB_CHECK(
+      hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbSel, ids, (const int*)c.isHuge.p, c.hugeList.p, hugeCount, n, s));
+  // Morton codes (huge proxies carry all ones and sort last)
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
-                     (const unsigned int*)c.bounds.p, c.morton.p, c.order.p);
-  unsigned long long hcount = 0;
-  JB_CHECK(hipMemcpyAsync(&hcount, c.counter.p + 1, sizeof(hcount), hipMemcpyDeviceToHost, s));
-  // sort leaves by Morton code (huge proxies carry 0xffffffff and sort last)
+                     (const unsigned int*)bounds, c.morton.p, c.order.p);
+  int hcount = 0;
+  JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
   size_t tmpBytes = 0;
   hipcub::DoubleBuffer<uint64_t> dk(c.morton.p, c.mortonAlt.p);
   hipcub::DoubleBuffer<int> dv(c.order.p, c.orderAlt.p);
@@ -692,27 +755,27 @@ void bpFindPairs(Ctx& c) {
   c.cubTmp.ensure(tmpBytes);
   JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, keyBits, s));
   JB_CHECK(hipStreamSynchronize(s));
-  c.nHuge = (int)hcount;
+  c.nHuge = hcount;
   c.st.huge = c.nHuge;
   const uint64_t* code = dk.Current();
   const int* order = dv.Current();
   c.sortedOrder = order;
   const int nn = n - c.nHuge;  // leaves in the tree
-  if (c.nHuge > 0) {  // sort the huge list by (world, index) for the per-world lookup
-    size_t tb = 0;
-    hipcub::DoubleBuffer<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
-    hipcub::DoubleBuffer<int> hv(c.hugeList.p, c.hugeListAlt.p);
-    // stable order by index first, then by world
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hv, hk, c.nHuge, 0, bits, s));
-    c.cubTmp.ensure(tb);
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hv, hk, c.nHuge, 0, bits, s));
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, c.nHuge, 0, 32, s));
-    c.cubTmp.ensure(tb);
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, c.nHuge, 0, 32, s));
-    if (hk.Current() != c.hugeWorld.p)
-      JB_CHECK(hipMemcpyAsync(c.hugeWorld.p, hk.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
-    if (hv.Current() != c.hugeList.p)
-      JB_CHECK(hipMemcpyAsync(c.hugeList.p, hv.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+  if (c.nHuge > 0) {  // per-world lookup: the index-ordered list sorted stably by world, i.e. by (world, index)
+    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)c.hugeList.p,
+                       (const int*)c.world.p, c.hugeWorld.p);
+    if (c.worldBits > 0) {
+      size_t tb = 0;
+      hipcub::DoubleBuffer<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
+      hipcub::DoubleBuffer<int> hv(c.hugeList.p, c.hugeListAlt.p);
+      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, c.nHuge, 0, c.worldBits, s));
+      c.cubTmp.ensure(tb);
+      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, c.nHuge, 0, c.worldBits, s));
+      if (hk.Current() != c.hugeWorld.p)
+        JB_CHECK(hipMemcpyAsync(c.hugeWorld.p, hk.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+      if (hv.Current() != c.hugeList.p)
+        JB_CHECK(hipMemcpyAsync(c.hugeList.p, hv.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
   }
   if (nn >= 2) {
     hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.world.p,

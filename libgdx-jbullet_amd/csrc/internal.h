@@ -194,6 +194,7 @@ struct Ctx {
   int64_t timedSteps = 0;
   // sampled kernel timing of the solver sweeps (start/stop events per launch)
   int kernelEvery = 0;
+  int64_t kernelSteps = 0;  // solves since the sampling period was set
   std::vector<hipEvent_t> kEv;  // event pool, two per timed launch
   std::vector<int> kEvKind;     // 0 normal rows, 1 friction rows
   double kMs[4] = {0, 0, 0, 0};  // 0/1: per-colour normal/friction launches, 2/3: block launches
@@ -233,6 +234,7 @@ void bpFindPairs(Ctx& c);                                // A3-A7: sorted canoni
 void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
 // narrowphase
 void npDispatchAll(Ctx& c);                              // A8-A16
+int64_t npCountContacts(Ctx& c);                         // contact points of the current manifolds
 // solver
 void solverSolve(Ctx& c, float dt);                      // A18-A20
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)

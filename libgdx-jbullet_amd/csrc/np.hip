@@ -1037,7 +1037,7 @@ __device__ void removePoint(float* m, int idx) {  // PersistentManifold.java:259
 
 __global__ void kManifold(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ ptype,
                           const int* __restrict__ act, const float* __restrict__ props, const float* __restrict__ tr,
-                          const float* __restrict__ cand, float* man, unsigned long long* contactCount) {
+                          const float* __restrict__ cand, float* man) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
@@ -1148,15 +1148,42 @@ __global__ void kManifold(int64_t npairs, const uint64_t* __restrict__ keys, con
       if (d2 > kContactThreshold * kContactThreshold) removePoint(m, k);
     }
   }
-  if (contactCount) atomicAdd(contactCount, (unsigned long long)ldi(m));
+}
+
+// contact points over all manifolds (per-block sums, one atomic per block); run only when statistics are read
+__global__ __launch_bounds__(256) void kCountPoints(int64_t n, const float* __restrict__ man,
+                                                    unsigned long long* out) {
+  __shared__ unsigned long long part[4];
+  unsigned long long k = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    k += (unsigned long long)ldi(man + (size_t)JBGPU_MANIFOLD_WORDS * i);
+  for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(out, t);
+  }
 }
 
 }  // namespace
 
+int64_t npCountContacts(Ctx& c) {
+  if (c.npairs == 0 || !c.man[c.cur].p) return 0;
+  c.counter.ensure(8);
+  unsigned long long* out = c.counter.p + 7;  // words 0-6 hold the step's own counters
+  JB_CHECK(hipMemsetAsync(out, 0, sizeof(*out), c.stream));
+  hipLaunchKernelGGL(kCountPoints, dim3(std::min(gridFor(c.npairs), 1024)), dim3(256), 0, c.stream, c.npairs,
+                     (const float*)c.man[c.cur].p, out);
+  unsigned long long k = 0;
+  JB_CHECK(hipMemcpyAsync(&k, out, sizeof(k), hipMemcpyDeviceToHost, c.stream));
+  JB_CHECK(hipStreamSynchronize(c.stream));
+  return (int64_t)k;
+}
+
 void npDispatchAll(Ctx& c) {
   hipStream_t s = c.stream;
   const int64_t n = c.npairs;
-  c.st.contacts = 0;
   if (n == 0) return;
   c.typeKey.ensure(n);
   c.typeKeyAlt.ensure(n);
@@ -1226,13 +1253,12 @@ void npDispatchAll(Ctx& c) {
   }
   tBegin(c, T_MANIFOLD);
   hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
-                     c.candC.p, man, c.counter.p + 4);
+                     c.candC.p, man);
   tEnd(c, T_MANIFOLD);
-  unsigned long long tail[3];
+  unsigned long long tail[2];
   JB_CHECK(hipMemcpyAsync(tail, c.counter.p + 2, sizeof(tail), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
   c.st.epaFails = (int64_t)tail[0];
-  c.st.contacts = (int64_t)tail[2];
   if (tail[1]) c.err = "EPA workspace overflow";
 }
 

@@ -398,17 +398,13 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
   hdr[p] = make_int4(mask | (row0 << 4), b0, b1, pi);
 }
 
-// normal rows of the solve (valid points of every slot): out[0] all slots, out[1] slots below headEnd
-__global__ void kCountRows(int na, int headEnd, const int4* __restrict__ hdr, unsigned long long* out) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long k = p < na ? (unsigned long long)__popc(slotMask(hdr[p])) : 0ull;
-  unsigned long long kh = p < headEnd ? k : 0ull;
-  for (int o = 32; o > 0; o >>= 1) {
-    k += __shfl_xor(k, o);
-    kh += __shfl_xor(kh, o);
-  }
-  if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, k);
-  if ((threadIdx.x & 63) == 0 && kh) atomicAdd(out + 1, kh);
+// normal rows of the solve, read off the rows' prefix sum: out[0] all slots, out[1] slots below headEnd
+__global__ void kRowTotals(int na, int headEnd, const int* __restrict__ count, const int* __restrict__ start,
+                           unsigned long long* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long all = na > 0 ? (unsigned long long)(start[na - 1] + count[na - 1]) : 0ull;
+  out[0] = all;
+  out[1] = headEnd >= na ? all : (unsigned long long)start[max(headEnd, 0)];
 }
 
 // warm-start impulses of colour c in row order: per point normal, lateral 1, lateral 2 (:659-747)
@@ -1099,11 +1095,11 @@ void solverSolve(Ctx& c, float dt) {
     JB_CHECK(hipMemcpyAsync(c.colorOffDev.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, s));
   }
   c.rowCount.ensure(2);
-  JB_CHECK(hipMemsetAsync(c.rowCount.p, 0, 2 * sizeof(unsigned long long), s));
   // rows: all slots, and the slots swept by the timed per-colour launches' counterpart (block sweeps: the
   // interior slots; otherwise the head colours)
-  hipLaunchKernelGGL(kCountRows, dim3(gridFor(na)), dim3(256), 0, s, na,
-                     blockSweeps ? (int)c.interiorEnd : (int)c.colorOff[headColors], c.hdr.p, c.rowCount.p);
+  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na,
+                     blockSweeps ? (int)c.interiorEnd : (int)c.colorOff[headColors], (const int*)c.slotCount.p,
+                     (const int*)c.slotStart.p, c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
   const int apply = (c.warmstart || c.useLearned || c.useMlp) ? 1 : 0;
   for (int k = 0; k < c.ncolors; k++) {
@@ -1125,7 +1121,7 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
   int* done = tolMode ? c.done.p : nullptr;
   // kernel-level timing of this step's sweeps (sampled)
-  const bool sampleK = c.timing && c.kernelEvery > 0 && c.timedSteps % c.kernelEvery == 0;
+  const bool sampleK = c.kernelEvery > 0 && c.kernelSteps++ % c.kernelEvery == 0;
   size_t kUsed = 0;
   auto kPair = [&](int kind) -> hipEvent_t* {
     if (kUsed + 2 > c.kEv.size()) {

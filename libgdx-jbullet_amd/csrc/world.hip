@@ -734,7 +734,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
   return guard(ctx, [&] {
     const Stats& s = ctx->c.st;
     out->pairs = s.pairs;
-    out->contacts = s.contacts;
+    out->contacts = ctx->c.nb > 0 ? npCountContacts(ctx->c) : 0;
     out->manifolds = ctx->c.nActive;
     out->colors = s.colors;
     out->epa_calls = s.epaCalls;
@@ -944,6 +944,7 @@ int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every) {
     if (every < 0) throw std::invalid_argument("negative sampling period");
     Ctx& c = ctx->c;
     c.kernelEvery = every;
+    c.kernelSteps = 0;
     for (int k = 0; k < 4; k++) {
       c.kMs[k] = 0;
       c.kLaunches[k] = 0;
