@@ -313,11 +313,17 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
 }
 
 // sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
+// leafExact[s]: the exact-test data of sorted leaf s in one 32-byte record: the stored AABB (Dbvt,
+// Simple) or the quantised box (AxisSweep3) as 6 words, the group/mask word and the proxy index
 __global__ void kGatherLeaves(int n, const int* __restrict__ order, const float* __restrict__ cand,
-                              const int* __restrict__ world, float* leafBox) {
+                              const int* __restrict__ world, float* leafBox, int mode, const float* __restrict__ aabb,
+                              const int* __restrict__ q, const int* __restrict__ filt, int4* __restrict__ leafExact) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int o = order[s];
+  const int* X = (mode == JBGPU_BP_SAP16 || mode == JBGPU_BP_SAP32) ? q + 6 * o : (const int*)aabb + 6 * o;
+  leafExact[2 * s] = make_int4(X[0], X[1], X[2], X[3]);
+  leafExact[2 * s + 1] = make_int4(X[4], X[5], filt[o], o);
   const float4* src = (const float4*)(cand + 8 * o);
   float4* dst = (float4*)(leafBox + 8 * s);
   float4 a = src[0], b = src[1];
@@ -423,6 +429,29 @@ __global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__
   ((float4*)nodeBox)[2 * i + 1] = make_float4(B.mx.x, B.mx.y, B.mx.z, whi);
 }
 
+// traversal record of internal node i: both children's boxes (.w lanes: world range; a leaf's world in
+// both), then (child0, child1, last0, last1) where child < 0 is leaf -child-1 and last is the highest
+// sorted leaf below the child; one visit reads the five float4 at once
+constexpr int kRecF4 = 5;
+__global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict__ child, const int* __restrict__ last,
+                                                    const float* __restrict__ leafBox,
+                                                    const float* __restrict__ nodeBox, float4* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int ch[2], lst[2];
+  for (int c = 0; c < 2; c++) {
+    ch[c] = child[2 * i + c];
+    const bool leaf = ch[c] < 0;
+    const int k = leaf ? -ch[c] - 1 : ch[c];
+    const float4* src = (const float4*)(leaf ? leafBox : nodeBox) + 2 * k;
+    rec[kRecF4 * i + 2 * c] = src[0];
+    rec[kRecF4 * i + 2 * c + 1] = src[1];
+    lst[c] = leaf ? k : last[k];
+  }
+  rec[kRecF4 * i + 4] = make_float4(__int_as_float(ch[0]), __int_as_float(ch[1]), __int_as_float(lst[0]),
+                                    __int_as_float(lst[1]));
+}
+
 struct PairTest {
   int mode;
   const float* aabb;
@@ -448,6 +477,26 @@ __device__ __forceinline__ bool exactOverlap(const PairTest& T, int a, int b) {
   return (A[0] <= B[3]) && (A[3] >= B[0]) && (A[1] <= B[4]) && (A[4] >= B[1]) && (A[2] <= B[5]) && (A[5] >= B[2]);
 }
 
+// exactOverlap for two leaves of the same world from their leaf records (world equality is implied by the
+// world lanes of the traversal record)
+__device__ __forceinline__ bool leafExactOverlap(int mode, const int4& a0, const int4& a1, const int4& b0,
+                                                 const int4& b1) {
+  const int fa = a1.z, fb = b1.z;
+  short ga = (short)(fa & 0xffff), ma = (short)(fa >> 16), gb = (short)(fb & 0xffff), mb = (short)(fb >> 16);
+  if (!(((ga & mb) != 0) && ((gb & ma) != 0))) return false;
+  if (mode == JBGPU_BP_SAP16 || mode == JBGPU_BP_SAP32) {
+    const int A[6] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, B[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
+    for (int ax = 0; ax < 3; ax++)
+      if (A[3 + ax] < B[ax] || B[3 + ax] < A[ax]) return false;
+    return true;
+  }
+  const float A[6] = {__int_as_float(a0.x), __int_as_float(a0.y), __int_as_float(a0.z),
+                      __int_as_float(a0.w), __int_as_float(a1.x), __int_as_float(a1.y)};
+  const float B[6] = {__int_as_float(b0.x), __int_as_float(b0.y), __int_as_float(b0.z),
+                      __int_as_float(b0.w), __int_as_float(b1.x), __int_as_float(b1.y)};
+  return (A[0] <= B[3]) && (A[3] >= B[0]) && (A[1] <= B[4]) && (A[4] >= B[1]) && (A[2] <= B[5]) && (A[5] >= B[2]);
+}
+
 __device__ __forceinline__ void emit(int a, int b, int bits, unsigned long long* counter, uint64_t* out, int64_t cap) {
   uint64_t lo = (uint64_t)min(a, b) + 1, hi = (uint64_t)max(a, b) + 1;
   unsigned long long k = atomicAdd(counter, 1ull);
@@ -465,11 +514,10 @@ constexpr int kTraverseBlock = 256;
 constexpr int kStagePerBlock = 16 * kTraverseBlock;
 __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ order,
                                                             const float* __restrict__ leafBox,
-                                                            const float* __restrict__ nodeBox,
-                                                            const int* __restrict__ child, const int* __restrict__ last,
-                                                            PairTest T, int bits, unsigned long long* counter,
-                                                            uint64_t* out, int64_t cap, int* err,
-                                                            uint64_t* __restrict__ stage) {
+                                                            const float4* __restrict__ rec,
+                                                            const int4* __restrict__ leafExact, int mode, int bits,
+                                                            unsigned long long* counter, uint64_t* out, int64_t cap,
+                                                            int* err, uint64_t* __restrict__ stage) {
   __shared__ unsigned int sCount;
   __shared__ unsigned long long sBase;
   if (threadIdx.x == 0) sCount = 0;
@@ -480,42 +528,48 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
     const float4 qmn = ((const float4*)leafBox)[2 * s];
     const float4 qmx = ((const float4*)leafBox)[2 * s + 1];
     const int qw = __float_as_int(qmn.w);
-    const int pa = order[s];
+    const int4 qe0 = leafExact[2 * s], qe1 = leafExact[2 * s + 1];
+    const int pa = qe1.w;
     int stack[64];
     int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-      int node = stack[--sp];
+    int node = 0;
+    // descend into a hit child directly; push only the second of two hits
+    while (true) {
+      const float4* R = rec + kRecF4 * node;
+      const float4 m = R[4];
+      int next = -1;
       for (int c = 0; c < 2; c++) {
-        int ch = child[2 * node + c];
+        const int ch = __float_as_int(c == 0 ? m.x : m.y);
+        const int lst = __float_as_int(c == 0 ? m.z : m.w);
+        if (lst <= s) continue;  // only leaves after s pair with s
+        const float4 bmn = R[2 * c], bmx = R[2 * c + 1];
+        const bool worldIn = qw >= __float_as_int(bmn.w) && qw <= __float_as_int(bmx.w);
+        if (!worldIn || !boxOverlap(qmn, qmx, bmn, bmx)) continue;
         if (ch < 0) {
-          int t = -ch - 1;
-          if (t > s) {
-            const float4 bmn = ((const float4*)leafBox)[2 * t];
-            const float4 bmx = ((const float4*)leafBox)[2 * t + 1];
-            if (boxOverlap(qmn, qmx, bmn, bmx)) {
-              int pb = order[t];
-              if (exactOverlap(T, pa, pb)) {
-                const uint64_t lo = (uint64_t)min(pa, pb) + 1, hi = (uint64_t)max(pa, pb) + 1;
-                const unsigned slot = atomicAdd(&sCount, 1u);
-                if (slot < (unsigned)kStagePerBlock)
-                  myStage[slot] = (lo << bits) | hi;
-                else
-                  emit(pa, pb, bits, counter, out, cap);
-              }
-            }
-          }
-        } else if (last[ch] > s) {
-          const float4 bmn = ((const float4*)nodeBox)[2 * ch];
-          const float4 bmx = ((const float4*)nodeBox)[2 * ch + 1];
-          const bool worldIn = qw >= __float_as_int(bmn.w) && qw <= __float_as_int(bmx.w);
-          if (worldIn && boxOverlap(qmn, qmx, bmn, bmx)) {
-            if (sp < 64)
-              stack[sp++] = ch;
+          const int4 e0 = leafExact[2 * (-ch - 1)], e1 = leafExact[2 * (-ch - 1) + 1];
+          const int pb = e1.w;
+          if (leafExactOverlap(mode, qe0, qe1, e0, e1)) {
+            const uint64_t lo = (uint64_t)min(pa, pb) + 1, hi = (uint64_t)max(pa, pb) + 1;
+            const unsigned slot = atomicAdd(&sCount, 1u);
+            if (slot < (unsigned)kStagePerBlock)
+              myStage[slot] = (lo << bits) | hi;
             else
-              atomicOr(err, 1);
+              emit(pa, pb, bits, counter, out, cap);
           }
+        } else if (next < 0) {
+          next = ch;
+        } else if (sp < 64) {
+          stack[sp++] = ch;
+        } else {
+          atomicOr(err, 1);
         }
+      }
+      if (next >= 0) {
+        node = next;
+      } else if (sp > 0) {
+        node = stack[--sp];
+      } else {
+        break;
       }
     }
   }
@@ -660,6 +714,8 @@ void ensureProxyBuffers(Ctx& c) {
   c.orderAlt.ensure(n);
   c.leafBox.ensure(8 * n);
   c.nodeBox.ensure(8 * n);
+  c.nodeRec.ensure(4 * kRecF4 * (size_t)n);
+  c.leafExact.ensure(8 * (size_t)n);
   c.nodeChild.ensure(2 * n);
   c.nodeLast.ensure(n);
   c.nodeFirst.ensure(n);
@@ -779,7 +835,8 @@ void bpFindPairs(Ctx& c) {
   }
   if (nn >= 2) {
     hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.world.p,
-                       c.leafBox.p);
+                       c.leafBox.p, c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p,
+                       (int4*)c.leafExact.p);
     hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeFirst.p,
                        c.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32, n3 = (n2 + 31) / 32;
@@ -792,6 +849,9 @@ void bpFindPairs(Ctx& c) {
     (void)n3;
     hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, c.nodeFirst.p, c.nodeLast.p,
                        c.leafBox.p, c1, c2, c3, c.nodeBox.p);
+    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)c.nodeChild.p,
+                       (const int*)c.nodeLast.p, (const float*)c.leafBox.p, (const float*)c.nodeBox.p,
+                       (float4*)c.nodeRec.p);
   }
   tEnd(c, T_BVH);
   PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
@@ -803,7 +863,8 @@ void bpFindPairs(Ctx& c) {
     if (attempt == 0) tBegin(c, T_TRAVERSE);
     if (nn >= 2)
       hipLaunchKernelGGL(kTraverse, dim3(gridFor(nn, kTraverseBlock)), dim3(kTraverseBlock), 0, s, nn, order,
-                         c.leafBox.p, c.nodeBox.p, c.nodeChild.p, c.nodeLast.p, T, bits, c.counter.p, c.rawKeys.p,
+                         c.leafBox.p, (const float4*)c.nodeRec.p, (const int4*)c.leafExact.p, c.bpMode, bits,
+                         c.counter.p, c.rawKeys.p,
                          c.pairCap, (int*)(c.counter.p + 2), c.pairStage.p);
     if (attempt == 0) tEnd(c, T_TRAVERSE);
     if (c.nHuge > 0)

@@ -110,6 +110,8 @@ struct Ctx {
   int nHuge = 0;
   DBuf<float> leafBox;   // sorted leaves np*8
   DBuf<float> nodeBox;   // internal nodes np*8
+  DBuf<float> nodeRec;   // traversal records np*20: both children's boxes, indices and last leaves
+  DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
   DBuf<int> nodeFirst;   // np: internal node i covers sorted leaves [nodeFirst[i], nodeLast[i]]
