@@ -512,6 +512,8 @@ __device__ __forceinline__ bool boxOverlap(const float4& amn, const float4& amx,
 // to the output behind one global reservation per block; only staging overflow takes the global counter.
 constexpr int kTraverseBlock = 256;
 constexpr int kStagePerBlock = 16 * kTraverseBlock;
+constexpr int kStackDepth = 64;  // traversal stack entries per thread
+constexpr int kLdsStack = 16;    // of which in LDS
 __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ order,
                                                             const float* __restrict__ leafBox,
                                                             const float4* __restrict__ rec,
@@ -520,6 +522,9 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
                                                             int* err, uint64_t* __restrict__ stage) {
   __shared__ unsigned int sCount;
   __shared__ unsigned long long sBase;
+  // the top kLdsStack entries of every thread's stack live in LDS ([depth][thread]: no bank conflicts),
+  // deeper ones in private memory
+  __shared__ int sStack[kLdsStack][kTraverseBlock];
   if (threadIdx.x == 0) sCount = 0;
   __syncthreads();
   uint64_t* myStage = stage + (size_t)blockIdx.x * kStagePerBlock;
@@ -530,7 +535,7 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
     const int qw = __float_as_int(qmn.w);
     const int4 qe0 = leafExact[2 * s], qe1 = leafExact[2 * s + 1];
     const int pa = qe1.w;
-    int stack[64];
+    int deep[kStackDepth - kLdsStack];
     int sp = 0;
     int node = 0;
     // descend into a hit child directly; push only the second of two hits
@@ -558,8 +563,10 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
           }
         } else if (next < 0) {
           next = ch;
-        } else if (sp < 64) {
-          stack[sp++] = ch;
+        } else if (sp < kLdsStack) {
+          sStack[sp++][threadIdx.x] = ch;
+        } else if (sp < kStackDepth) {
+          deep[sp++ - kLdsStack] = ch;
         } else {
           atomicOr(err, 1);
         }
@@ -567,7 +574,8 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
       if (next >= 0) {
         node = next;
       } else if (sp > 0) {
-        node = stack[--sp];
+        --sp;
+        node = sp < kLdsStack ? sStack[sp][threadIdx.x] : deep[sp - kLdsStack];
       } else {
         break;
       }
