@@ -261,12 +261,31 @@ __device__ bool tetClosest(V3 p, V3 a, V3 b, V3 c, V3 d, Sub& fr) {
   return true;
 }
 
+// Vertex storage of the simplex (w, p, q of up to 5 vertices) in LDS, [row][thread] with row = 15 * array
+// + 3 * vertex + component: indexed by the vertex count, it would otherwise sit in scratch memory
+constexpr int kGjkBlock = 128;
+constexpr int kSimplexRows = 45;
+
 struct Simplex {
   int nv;
-  V3 W[5], P[5], Q[5];
+  float* lw;  // this thread's column of the LDS block
   V3 cP1, cP2, cV, lastW;
   bool validClosest, needsUpdate;
   Sub bc;
+
+  __device__ V3 get(int arr, int i) const {
+    const float* p = lw + (15 * arr + 3 * i) * kGjkBlock;
+    return v3(p[0], p[kGjkBlock], p[2 * kGjkBlock]);
+  }
+  __device__ void set(int arr, int i, V3 v) {
+    float* p = lw + (15 * arr + 3 * i) * kGjkBlock;
+    p[0] = v.x;
+    p[kGjkBlock] = v.y;
+    p[2 * kGjkBlock] = v.z;
+  }
+  __device__ V3 W(int i) const { return get(0, i); }
+  __device__ V3 P(int i) const { return get(1, i); }
+  __device__ V3 Q(int i) const { return get(2, i); }
 
   __device__ void reset() {
     validClosest = false;
@@ -277,9 +296,9 @@ struct Simplex {
   }
   __device__ void removeVertex(int idx) {
     nv--;
-    W[idx] = W[nv];
-    P[idx] = P[nv];
-    Q[idx] = Q[nv];
+    set(0, idx, W(nv));
+    set(1, idx, P(nv));
+    set(2, idx, Q(nv));
   }
   __device__ void reduce(uint32_t used) {
     if ((nv >= 4) && !bit(used, 3)) removeVertex(3);
@@ -290,15 +309,15 @@ struct Simplex {
   __device__ void addVertex(V3 w, V3 p, V3 q) {
     lastW = w;
     needsUpdate = true;
-    W[nv] = w;
-    P[nv] = p;
-    Q[nv] = q;
+    set(0, nv, w);
+    set(1, nv, p);
+    set(2, nv, q);
     nv++;
   }
   __device__ bool inSimplex(V3 w) const {
     bool found = false;
     for (int i = 0; i < nv; i++)
-      if (veq(W[i], w)) found = true;
+      if (veq(W(i), w)) found = true;
     if (veq(w, lastW)) return true;
     return found;
   }
@@ -311,15 +330,15 @@ struct Simplex {
           validClosest = false;
           break;
         case 1:
-          cP1 = P[0];
-          cP2 = Q[0];
+          cP1 = P(0);
+          cP2 = Q(0);
           cV = sub(cP1, cP2);
           bc.reset();
           bc.setBC(1, 0, 0, 0);
           validClosest = bc.isValid();
           break;
         case 2: {
-          V3 from = W[0], to = W[1];
+          V3 from = W(0), to = W(1);
           V3 diff = sub(v3(0, 0, 0), from);
           V3 v = sub(to, from);
           float t = dot(v, diff);
@@ -339,18 +358,18 @@ struct Simplex {
             bc.used |= 1;
           }
           bc.setBC(1.0f - t, t, 0, 0);
-          cP1 = add(P[0], scl(sub(P[1], P[0]), t));
-          cP2 = add(Q[0], scl(sub(Q[1], Q[0]), t));
+          cP1 = add(P(0), scl(sub(P(1), P(0)), t));
+          cP2 = add(Q(0), scl(sub(Q(1), Q(0)), t));
           cV = sub(cP1, cP2);
           reduce(bc.used);
           validClosest = bc.isValid();
           break;
         }
         case 3: {
-          triClosest(v3(0, 0, 0), W[0], W[1], W[2], bc);
-          V3 t1 = scl(P[0], bc.bc[0]), t2 = scl(P[1], bc.bc[1]), t3 = scl(P[2], bc.bc[2]);
+          triClosest(v3(0, 0, 0), W(0), W(1), W(2), bc);
+          V3 t1 = scl(P(0), bc.bc[0]), t2 = scl(P(1), bc.bc[1]), t3 = scl(P(2), bc.bc[2]);
           cP1 = v3(t1.x + t2.x + t3.x, t1.y + t2.y + t3.y, t1.z + t2.z + t3.z);
-          t1 = scl(Q[0], bc.bc[0]); t2 = scl(Q[1], bc.bc[1]); t3 = scl(Q[2], bc.bc[2]);
+          t1 = scl(Q(0), bc.bc[0]); t2 = scl(Q(1), bc.bc[1]); t3 = scl(Q(2), bc.bc[2]);
           cP2 = v3(t1.x + t2.x + t3.x, t1.y + t2.y + t3.y, t1.z + t2.z + t3.z);
           cV = sub(cP1, cP2);
           reduce(bc.used);
@@ -358,11 +377,11 @@ struct Simplex {
           break;
         }
         case 4: {
-          bool sep = tetClosest(v3(0, 0, 0), W[0], W[1], W[2], W[3], bc);
+          bool sep = tetClosest(v3(0, 0, 0), W(0), W(1), W(2), W(3), bc);
           if (sep) {
-            V3 t1 = scl(P[0], bc.bc[0]), t2 = scl(P[1], bc.bc[1]), t3 = scl(P[2], bc.bc[2]), t4 = scl(P[3], bc.bc[3]);
+            V3 t1 = scl(P(0), bc.bc[0]), t2 = scl(P(1), bc.bc[1]), t3 = scl(P(2), bc.bc[2]), t4 = scl(P(3), bc.bc[3]);
             cP1 = v3(t1.x + t2.x + t3.x + t4.x, t1.y + t2.y + t3.y + t4.y, t1.z + t2.z + t3.z + t4.z);
-            t1 = scl(Q[0], bc.bc[0]); t2 = scl(Q[1], bc.bc[1]); t3 = scl(Q[2], bc.bc[2]); t4 = scl(Q[3], bc.bc[3]);
+            t1 = scl(Q(0), bc.bc[0]); t2 = scl(Q(1), bc.bc[1]); t3 = scl(Q(2), bc.bc[2]); t4 = scl(Q(3), bc.bc[3]);
             cP2 = v3(t1.x + t2.x + t3.x + t4.x, t1.y + t2.y + t3.y + t4.y, t1.z + t2.z + t3.z + t4.z);
             cV = sub(cP1, cP2);
             reduce(bc.used);
@@ -388,11 +407,12 @@ struct Simplex {
 
 // ------------------------------------------------------------------ GJK pair detector
 // epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
-__global__ __launch_bounds__(128) void kGjk(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(kGjkBlock) void kGjk(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
                                             const float* __restrict__ shapes, const float* __restrict__ verts,
                                             const float* __restrict__ props, const float* __restrict__ tr,
                                             float* cand, float* epaState, int* epaQueue,
                                             unsigned long long* epaCount) {
+  __shared__ float simplexLds[kSimplexRows * kGjkBlock];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
   int i = list[k];
@@ -416,6 +436,7 @@ __global__ __launch_bounds__(128) void kGjk(int cnt, const int* __restrict__ lis
   float sqd = kInf;
   float margin = marginA + marginB;
   Simplex vs;
+  vs.lw = simplexLds + threadIdx.x;
   vs.reset();
   for (;;) {
     V3 sepA = mulMtV(ta.b, scl(sep, -1.0f));
@@ -1234,7 +1255,7 @@ void npDispatchAll(Ctx& c) {
                        c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p);
   tBegin(c, T_GJK);
   if (cnt[JBGPU_PAIR_CONVEX_CONVEX])
-    hipLaunchKernelGGL(kGjk, dim3(gridFor(cnt[2], 128)), dim3(128), 0, s, (int)cnt[2], list + off[2], keys,
+    hipLaunchKernelGGL(kGjk, dim3(gridFor(cnt[2], kGjkBlock)), dim3(kGjkBlock), 0, s, (int)cnt[2], list + off[2], keys,
                        c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p,
                        c.counter.p);
   tEnd(c, T_GJK);
