@@ -595,30 +595,171 @@ __device__ __forceinline__ void frictionSlot(int p, const SolverParams& SP, cons
   io.store(p, h, A, B);
 }
 
-// one colour, one thread per manifold slot
-__global__ __launch_bounds__(64) void kNormal(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
-                                              float* nrow, const float* __restrict__ props, float* svel, double* res,
-                                              const int* __restrict__ done) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+// ---- sweeps with four lanes per manifold slot --------------------------------------------------------
+// Lane q of a slot's quad holds the rows of the slot's q-th valid point (rows are dense, so they are rows
+// row0 + q): a quad's loads are contiguous and a lane stages one point's rows, not four.  The points are
+// applied in order: in round t lane t updates the quad's copy of the two bodies, then every lane takes
+// lane t's copy (a DPP quad broadcast), so each body sees exactly the sequence of a one-thread sweep.
+template <int L>
+__device__ __forceinline__ float quadBcast(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), L * 0x55, 0xf, 0xf, false));
+}
+template <int L>
+__device__ __forceinline__ void quadBcast(SBody& s) {
+  s.lv = v3(quadBcast<L>(s.lv.x), quadBcast<L>(s.lv.y), quadBcast<L>(s.lv.z));
+  s.av = v3(quadBcast<L>(s.av.x), quadBcast<L>(s.av.y), quadBcast<L>(s.av.z));
+}
+
+// one normal row: resolveSingleCollisionCombinedCacheFriendly (:234-284)
+__device__ __forceinline__ void normalRow(const float* N, const SolverParams& SP, SBody& A, SBody& B, float* out,
+                                          double& dd, double& nn) {
+  V3 nrm = v3(N[0], N[1], N[2]);
+  float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+  float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
+  float rel_vel = v1 - v2;
+  float positionalError = -N[16] * SP.erp / SP.dt;
+  float velocityError = N[17] - rel_vel;
+  float pimp = positionalError * N[15];
+  float vimp = velocityError * N[15];
+  float ni = pimp + vimp;
+  float old = N[19];
+  float sum = old + ni;
+  float nw = 0.0f > sum ? 0.0f : sum;
+  ni = nw - old;
+  applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), ni);
+  applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -ni);
+  *out = nw;
+  double d = (double)nw - (double)old;
+  dd += d * d;
+  nn += (double)nw * (double)nw;
+}
+
+// one friction row: resolveSingleFrictionCacheFriendly (:286-346), limit from the contact's normal impulse
+__device__ __forceinline__ void frictionRow(const float* N, float total, SBody& A, SBody& B, float* out, double& dd,
+                                            double& nn) {
+  float old = N[19];
+  float ap = old;
+  float limit = total * N[18];
+  if (total > 0.0f) {
+    V3 nrm = v3(N[0], N[1], N[2]);
+    float v1 = dot(nrm, A.lv) + dot(v3(N[3], N[4], N[5]), A.av);
+    float v2 = dot(nrm, B.lv) + dot(v3(N[6], N[7], N[8]), B.av);
+    float rel_vel = v1 - v2;
+    float j1 = -rel_vel * N[15];
+    ap = old + j1;
+    if (limit < ap) {
+      ap = limit;
+    } else if (ap < -limit) {
+      ap = -limit;
+    }
+    j1 = ap - old;
+    applyImpulse(A, scl(nrm, A.invMass), v3(N[9], N[10], N[11]), j1);
+    applyImpulse(B, scl(nrm, B.invMass), v3(N[12], N[13], N[14]), -j1);
+    *out = ap;
+  }
+  double d = (double)ap - (double)old;
+  dd += d * d;
+  nn += (double)ap * (double)ap;
+}
+
+__device__ __forceinline__ void loadRow(float* r, const float* src) {
+#pragma unroll
+  for (int k = 0; k < RW / 4; k++) {
+    const float4 v = ((const float4*)src)[k];
+    r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void normalRound(int q, bool mine, const float* N, const SolverParams& SP, SBody& A,
+                                            SBody& B, float* out, double& dd, double& nn) {
+  if (q == L && mine) normalRow(N, SP, A, B, out, dd, nn);
+  quadBcast<L>(A);
+  quadBcast<L>(B);
+}
+// normal rows of slot p, lane q of its quad
+__device__ __forceinline__ void normalQuad(int p, int q, const SolverParams& SP, const int4* __restrict__ hdr,
+                                           float* nrow, const GlobalBodies& io, double& dd, double& nn) {
+  const int4 h = hdr[p];
+  const int nr = __popc(slotMask(h));
+  const bool mine = q < nr;
+  float* row = nrow + (size_t)RW * (slotRow(h) + q);
+  float N[RW];
+  if (mine) loadRow(N, row);
+  SBody A, B;
+  io.load(p, h, A, B);
+  normalRound<0>(q, mine, N, SP, A, B, row + 19, dd, nn);
+  normalRound<1>(q, mine, N, SP, A, B, row + 19, dd, nn);
+  normalRound<2>(q, mine, N, SP, A, B, row + 19, dd, nn);
+  normalRound<3>(q, mine, N, SP, A, B, row + 19, dd, nn);
+  if (q == 0) io.store(p, h, A, B);
+}
+
+template <int L>
+__device__ __forceinline__ void frictionRound(int q, bool mine, const float* F, float total, SBody& A, SBody& B,
+                                              float* out, double& dd, double& nn) {
+  if (q == L && mine) {
+    frictionRow(F, total, A, B, out + 19, dd, nn);
+    frictionRow(F + RW, total, A, B, out + RW + 19, dd, nn);
+  }
+  quadBcast<L>(A);
+  quadBcast<L>(B);
+}
+// friction rows of slot p, lane q of its quad (the point's two lateral rows in order)
+__device__ __forceinline__ void frictionQuad(int p, int q, const SolverParams& SP, const int4* __restrict__ hdr,
+                                             const float* __restrict__ nrow, float* frow, const GlobalBodies& io,
+                                             double& dd, double& nn) {
+  (void)SP;
+  const int4 h = hdr[p];
+  const int nr = __popc(slotMask(h));
+  const bool mine = q < nr;
+  const int r = slotRow(h) + q;
+  float* rows = frow + (size_t)RW * (2 * r);
+  float F[2 * RW];
+  float total = 0.0f;
+  if (mine) {
+    loadRow(F, rows);
+    loadRow(F + RW, rows + RW);
+    total = nrow[(size_t)RW * r + 19] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
+  }
+  SBody A, B;
+  io.load(p, h, A, B);
+  frictionRound<0>(q, mine, F, total, A, B, rows, dd, nn);
+  frictionRound<1>(q, mine, F, total, A, B, rows, dd, nn);
+  frictionRound<2>(q, mine, F, total, A, B, rows, dd, nn);
+  frictionRound<3>(q, mine, F, total, A, B, rows, dd, nn);
+  if (q == 0) io.store(p, h, A, B);
+}
+
+// one colour: slots [off, off + cnt), four lanes each
+constexpr int kSweepThreads = 256;
+__global__ __launch_bounds__(kSweepThreads) void kNormal(int off, int cnt, SolverParams SP,
+                                                         const int4* __restrict__ hdr, float* nrow,
+                                                         const float* __restrict__ props, float* svel, double* res,
+                                                         const int* __restrict__ done) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 2;
   double dd = 0.0, nn = 0.0;
   const int stop = done ? *done : 0;  // tolerance mode only
-  if (k < cnt && !stop) normalSlot(off + k, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
+  if (k < cnt && !stop) normalQuad(off + k, t & 3, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
   accumulate(dd, nn, res);
 }
-__global__ __launch_bounds__(64) void kFriction(int off, int cnt, SolverParams SP, const int4* __restrict__ hdr,
-                                                const float* __restrict__ nrow, float* frow,
-                                                const float* __restrict__ props, float* svel, double* res,
-                                                const int* __restrict__ done) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kSweepThreads) void kFriction(int off, int cnt, SolverParams SP,
+                                                           const int4* __restrict__ hdr,
+                                                           const float* __restrict__ nrow, float* frow,
+                                                           const float* __restrict__ props, float* svel, double* res,
+                                                           const int* __restrict__ done) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 2;
   double dd = 0.0, nn = 0.0;
   const int stop = done ? *done : 0;
-  if (k < cnt && !stop) frictionSlot(off + k, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
+  if (k < cnt && !stop) frictionQuad(off + k, t & 3, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
   accumulate(dd, nn, res);
 }
 
 // the small colours [c0, c1) of a sweep in one workgroup: colour after colour, separated by workgroup
 // barriers (all waves of a workgroup share one CU and its L1, so a barrier orders their global accesses)
-constexpr int kTailThreads = 512;
+constexpr int kTailThreads = 1024;
 __global__ __launch_bounds__(kTailThreads) void kNormalTail(int c0, int c1, const int* __restrict__ colorOff,
                                                             SolverParams SP, const int4* __restrict__ hdr,
                                                             float* nrow, const float* __restrict__ props,
@@ -628,8 +769,8 @@ __global__ __launch_bounds__(kTailThreads) void kNormalTail(int c0, int c1, cons
   if (!stop)
     for (int c = c0; c < c1; c++) {
       const int a = colorOff[c], b = colorOff[c + 1];
-      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads)
-        normalSlot(p, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
+      for (int p = a + (int)(threadIdx.x >> 2); p < b; p += kTailThreads / 4)
+        normalQuad(p, threadIdx.x & 3, SP, hdr, nrow, GlobalBodies{svel, props}, dd, nn);
       __syncthreads();
     }
   accumulate(dd, nn, res);
@@ -644,8 +785,8 @@ __global__ __launch_bounds__(kTailThreads) void kFrictionTail(int c0, int c1, co
   if (!stop)
     for (int c = c0; c < c1; c++) {
       const int a = colorOff[c], b = colorOff[c + 1];
-      for (int p = a + (int)threadIdx.x; p < b; p += kTailThreads)
-        frictionSlot(p, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
+      for (int p = a + (int)(threadIdx.x >> 2); p < b; p += kTailThreads / 4)
+        frictionQuad(p, threadIdx.x & 3, SP, hdr, nrow, frow, GlobalBodies{svel, props}, dd, nn);
       __syncthreads();
     }
   accumulate(dd, nn, res);
@@ -1170,11 +1311,11 @@ void solverSolve(Ctx& c, float dt) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
       if (sampleK) {
         hipEvent_t* ev = kPair(0);
-        hipExtLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, ev[0], ev[1], 0, (int)c.colorOff[k],
+        hipExtLaunchKernelGGL(kNormal, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, ev[0], ev[1], 0, (int)c.colorOff[k],
                               cnt, SP, (const int4*)c.hdr.p, c.nrow.p, (const float*)c.props.p, c.svel.p, res,
                               (const int*)done);
       } else {
-        hipLaunchKernelGGL(kNormal, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
+        hipLaunchKernelGGL(kNormal, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, (int)c.colorOff[k], cnt, SP, c.hdr.p,
                            c.nrow.p, c.props.p, c.svel.p, res, done);
       }
     }
@@ -1187,11 +1328,11 @@ void solverSolve(Ctx& c, float dt) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
       if (sampleK) {
         hipEvent_t* ev = kPair(1);
-        hipExtLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, ev[0], ev[1], 0,
+        hipExtLaunchKernelGGL(kFriction, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, ev[0], ev[1], 0,
                               (int)c.colorOff[k], cnt, SP, (const int4*)c.hdr.p, (const float*)c.nrow.p, c.frow.p,
                               (const float*)c.props.p, c.svel.p, res, (const int*)done);
       } else {
-        hipLaunchKernelGGL(kFriction, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, SP,
+        hipLaunchKernelGGL(kFriction, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, (int)c.colorOff[k], cnt, SP,
                            c.hdr.p, c.nrow.p, c.frow.p, c.props.p, c.svel.p, res, done);
       }
     }
