@@ -51,7 +51,7 @@ constexpr int kEpaVerts = 300;
 constexpr int kEpaHe = 160;
 constexpr int kEpaFaceWords = 16;
 constexpr int kEpaSlotWords = kEpaFaces * kEpaFaceWords + kEpaVerts * 6 + kEpaHe * 4 + 8;
-constexpr int kEpaBatch = 8192;
+constexpr int kEpaGrid = 512;  // EPA blocks (LDS workspace limits them to two per CU)
 constexpr int kTimerRegions = 16;
 
 // solver rows per manifold: 4 contacts x (normal row + 2 friction rows)
@@ -130,7 +130,10 @@ struct Ctx {
   DBuf<float> man[2];
   DBuf<int> ptype;
   DBuf<int> typeKey, typeKeyAlt, perm, permAlt;
-  int64_t binOff[5] = {0, 0, 0, 0, 0};
+  DBuf<unsigned long long> npStat;           // EPA queue length, failures, workspace overflows
+  DBuf<int> binRange;                        // per pair type: first sorted position and count
+  unsigned long long npStatHost[3] = {0, 0, 0};  // their copy, valid after the step's final synchronisation
+  bool npStatPending = false;
   DBuf<float> candC;  // contact candidates npairs*8
   DBuf<int> epaQueue;
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
@@ -197,6 +200,7 @@ struct Ctx {
   // sampled kernel timing of the solver sweeps (start/stop events per launch)
   int kernelEvery = 0;
   int64_t kernelSteps = 0;  // solves since the sampling period was set
+  int colourRounds = 0;     // colouring rounds the last solve needed
   std::vector<hipEvent_t> kEv;  // event pool, two per timed launch
   std::vector<int> kEvKind;     // 0 normal rows, 1 friction rows
   double kMs[4] = {0, 0, 0, 0};  // 0/1: per-colour normal/friction launches, 2/3: block launches
@@ -237,6 +241,7 @@ void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
 // narrowphase
 void npDispatchAll(Ctx& c);                              // A8-A16
 int64_t npCountContacts(Ctx& c);                         // contact points of the current manifolds
+void npCollectStats(Ctx& c);                             // EPA statistics of the last narrowphase
 // solver
 void solverSolve(Ctx& c, float dt);                      // A18-A20
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)

@@ -70,12 +70,26 @@ __global__ void kBinStarts(int64_t n, const int* __restrict__ sorted, int* start
 }
 
 // ------------------------------------------------------------------ sphere-sphere
-__global__ void kSphereSphere(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+// per pair type t: range[2t] first sorted position, range[2t + 1] count (types absent from the sort: 0, 0)
+__global__ void kBinRanges(int64_t n, const int* __restrict__ starts, int* range) {
+  const int t = threadIdx.x;
+  if (t >= 4) return;
+  int end = (int)n;
+  for (int u = t + 1; u < 4; u++)
+    if (starts[u] >= 0) {
+      end = starts[u];
+      break;
+    }
+  range[2 * t] = starts[t] >= 0 ? starts[t] : 0;
+  range[2 * t + 1] = starts[t] >= 0 ? end - starts[t] : 0;
+}
+
+__global__ void kSphereSphere(const int* __restrict__ range, const int* __restrict__ list, const uint64_t* __restrict__ keys,
                               const float* __restrict__ shapes, const float* __restrict__ props,
                               const float* __restrict__ tr, float* cand) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  int i = list[k];
+  if (k >= range[1]) return;
+  int i = list[range[0] + k];
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   Shape s0 = bodyShape(shapes, props, a), s1 = bodyShape(shapes, props, b);
   V3 o0 = ld3(tr + 12 * a + 9), o1 = ld3(tr + 12 * b + 9);
@@ -94,12 +108,12 @@ __global__ void kSphereSphere(int cnt, const int* __restrict__ list, const uint6
 }
 
 // ------------------------------------------------------------------ convex-plane
-__global__ void kConvexPlane(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+__global__ void kConvexPlane(const int* __restrict__ range, const int* __restrict__ list, const uint64_t* __restrict__ keys,
                              const float* __restrict__ shapes, const float* __restrict__ verts,
                              const float* __restrict__ props, const float* __restrict__ tr, float* cand) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  int i = list[k];
+  if (k >= range[1]) return;
+  int i = list[range[0] + k];
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
   int cv = (sb.type == JBGPU_SHAPE_PLANE) ? a : b;
@@ -407,15 +421,15 @@ struct Simplex {
 
 // ------------------------------------------------------------------ GJK pair detector
 // epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
-__global__ __launch_bounds__(kGjkBlock) void kGjk(int cnt, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(kGjkBlock) void kGjk(const int* __restrict__ range, const int* __restrict__ list, const uint64_t* __restrict__ keys,
                                             const float* __restrict__ shapes, const float* __restrict__ verts,
                                             const float* __restrict__ props, const float* __restrict__ tr,
                                             float* cand, float* epaState, int* epaQueue,
                                             unsigned long long* epaCount) {
   __shared__ float simplexLds[kSimplexRows * kGjkBlock];
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  int i = list[k];
+  if (k >= range[1]) return;
+  int i = list[range[0] + k];
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
   Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
@@ -968,15 +982,10 @@ struct EpaSolver {
 // one thread per queued pair; the GJK partial result is merged as in GjkPairDetector.java:263-303
 // one EPA job per workgroup: its polytope workspace (faces, vertices, hash entries) lives in the
 // workgroup's LDS; the job itself is sequential and runs on the first lane
-__global__ __launch_bounds__(64) void kEpa(int cnt, int base, const int* __restrict__ queue,
-                                           const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
-                                           const float* __restrict__ verts, const float* __restrict__ props,
-                                           const float* __restrict__ tr, const float* __restrict__ epaState,
-                                           float* cand, unsigned long long* fails) {
-  __shared__ float slot[kEpaSlotWords];
-  const int k = blockIdx.x;
-  if (k >= cnt || threadIdx.x != 0) return;
-  int i = queue[base + k];
+// one EPA job (pair i) with the workspace `slot` in LDS
+__device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
+                       const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
+                       const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot) {
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
   S.sh[0] = bodyShape(shapes, props, a);
@@ -1027,6 +1036,20 @@ __global__ __launch_bounds__(64) void kEpa(int cnt, int base, const int* __restr
     atomicAdd(fails, 1ull);
   }
   putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
+}
+
+// EPA jobs in a fixed grid: every block steps through the queue (its length read on the device) until it
+// is exhausted; lane 0 of each block runs the job
+__global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict__ count, const int* __restrict__ queue,
+                                           const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
+                                           const float* __restrict__ verts, const float* __restrict__ props,
+                                           const float* __restrict__ tr, const float* __restrict__ epaState,
+                                           float* cand, unsigned long long* fails) {
+  __shared__ float slot[kEpaSlotWords];
+  if (threadIdx.x != 0) return;
+  const int jobs = (int)*count;
+  for (int k = blockIdx.x; k < jobs; k += gridDim.x)
+    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot);
 }
 
 // ------------------------------------------------------------------ manifold update
@@ -1205,6 +1228,7 @@ int64_t npCountContacts(Ctx& c) {
 void npDispatchAll(Ctx& c) {
   hipStream_t s = c.stream;
   const int64_t n = c.npairs;
+  c.st.epaCalls = c.st.epaFails = 0;
   if (n == 0) return;
   c.typeKey.ensure(n);
   c.typeKeyAlt.ensure(n);
@@ -1228,59 +1252,43 @@ void npDispatchAll(Ctx& c) {
   int* starts = (int*)(c.counter.p + 4);  // 4 ints
   JB_CHECK(hipMemsetAsync(starts, 0xff, 4 * sizeof(int), s));
   hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)kk.Current(), starts);
+  // per-type ranges stay on the device: each type's kernel covers every pair position and reads its range
+  c.binRange.ensure(8);
+  int* range = c.binRange.p;
+  hipLaunchKernelGGL(kBinRanges, dim3(1), dim3(64), 0, s, n, (const int*)starts, range);
   tEnd(c, T_BIN);
-  int st4[4];
-  JB_CHECK(hipMemcpyAsync(st4, starts, sizeof(st4), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
-  unsigned long long cnt[4];
-  for (int t = 0; t < 4; t++) {
-    int64_t end = n;
-    for (int u = t + 1; u < 4; u++)
-      if (st4[u] >= 0) {
-        end = st4[u];
-        break;
-      }
-    cnt[t] = st4[t] >= 0 ? (unsigned long long)(end - st4[t]) : 0ull;
-  }
-  int64_t off[5] = {0, 0, 0, 0, 0};
-  for (int t = 0; t < 4; t++) off[t + 1] = off[t] + (int64_t)cnt[t];
-  for (int t = 0; t < 5; t++) c.binOff[t] = off[t];
   const int* list = vv.Current();
-  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
-  if (cnt[JBGPU_PAIR_SPHERE_SPHERE])
-    hipLaunchKernelGGL(kSphereSphere, dim3(gridFor(cnt[0])), dim3(256), 0, s, (int)cnt[0], list + off[0], keys,
-                       c.shapes.p, c.props.p, c.tr.p, c.candC.p);
-  if (cnt[JBGPU_PAIR_CONVEX_PLANE])
-    hipLaunchKernelGGL(kConvexPlane, dim3(gridFor(cnt[1])), dim3(256), 0, s, (int)cnt[1], list + off[1], keys,
-                       c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p);
+  // EPA queue length, failures, workspace overflows: read back with the step's final synchronisation
+  c.npStat.ensure(4);
+  JB_CHECK(hipMemsetAsync(c.npStat.p, 0, 4 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(kSphereSphere, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_SPHERE_SPHERE,
+                     list, keys, c.shapes.p, c.props.p, c.tr.p, c.candC.p);
+  hipLaunchKernelGGL(kConvexPlane, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_CONVEX_PLANE,
+                     list, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p);
   tBegin(c, T_GJK);
-  if (cnt[JBGPU_PAIR_CONVEX_CONVEX])
-    hipLaunchKernelGGL(kGjk, dim3(gridFor(cnt[2], kGjkBlock)), dim3(kGjkBlock), 0, s, (int)cnt[2], list + off[2], keys,
-                       c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p,
-                       c.counter.p);
+  hipLaunchKernelGGL(kGjk, dim3(gridFor(n, kGjkBlock)), dim3(kGjkBlock), 0, s,
+                     (const int*)range + 2 * JBGPU_PAIR_CONVEX_CONVEX, list, keys, c.shapes.p, c.verts.p, c.props.p,
+                     c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p, c.npStat.p);
   tEnd(c, T_GJK);
-  unsigned long long nepa = 0;
-  JB_CHECK(hipMemcpyAsync(&nepa, c.counter.p, sizeof(nepa), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
-  c.st.epaCalls = (int64_t)nepa;
-  if (nepa) {
-    tBegin(c, T_EPA);
-    for (int64_t b = 0; b < (int64_t)nepa; b += kEpaBatch) {
-      int m = (int)std::min<int64_t>(kEpaBatch, (int64_t)nepa - b);
-      hipLaunchKernelGGL(kEpa, dim3(m), dim3(64), 0, s, m, (int)b, c.epaQueue.p, keys, c.shapes.p, c.verts.p,
-                         c.props.p, c.tr.p, c.epaState.p, c.candC.p, c.counter.p + 2);
-    }
-    tEnd(c, T_EPA);
-  }
+  tBegin(c, T_EPA);
+  hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
+                     (const int*)c.epaQueue.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p,
+                     c.npStat.p + 1);
+  tEnd(c, T_EPA);
   tBegin(c, T_MANIFOLD);
   hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
                      c.candC.p, man);
   tEnd(c, T_MANIFOLD);
-  unsigned long long tail[2];
-  JB_CHECK(hipMemcpyAsync(tail, c.counter.p + 2, sizeof(tail), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
-  c.st.epaFails = (int64_t)tail[0];
-  if (tail[1]) c.err = "EPA workspace overflow";
+  JB_CHECK(hipMemcpyAsync(c.npStatHost, c.npStat.p, sizeof(c.npStatHost), hipMemcpyDeviceToHost, s));
+  c.npStatPending = true;
+}
+
+void npCollectStats(Ctx& c) {  // after the step's final synchronisation
+  if (!c.npStatPending) return;
+  c.npStatPending = false;
+  c.st.epaCalls = (int64_t)c.npStatHost[0];
+  c.st.epaFails = (int64_t)c.npStatHost[1];
+  if (c.npStatHost[2]) c.err = "EPA workspace overflow";
 }
 
 }  // namespace jbg

@@ -1088,9 +1088,13 @@ void solverSolve(Ctx& c, float dt) {
   c.pendAlt.ensure(na);
   c.pendFlag.ensure(na);
   int round = 0;
+  // the first batch's length adapts to the scene (one round shorter after a step whose first batch
+  // coloured everything, the rounds then needed after one that did not): usually one host wait per solve
+  const int firstRounds = std::max(12, c.colourRounds);
+  int batches = 0;
   auto colourPass = [&](int m) {
-    for (int batch = 0; m > 0; batch++) {
-      const int rounds = batch == 0 ? 12 : 6;
+    for (int batch = 0; m > 0; batch++, batches++) {
+      const int rounds = batch == 0 ? firstRounds : 6;
       for (int r = 0; r < rounds; r++, round++) {
         hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
                            (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p,
@@ -1119,6 +1123,7 @@ void solverSolve(Ctx& c, float dt) {
   if (!blocked) {
     hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
     colourPass(na);
+    c.colourRounds = batches <= 1 ? firstRounds - 1 : round;
   } else {
     for (int interior = 1; interior >= 0; interior--) {
       hipLaunchKernelGGL(kPassFlag, dim3(gridFor(na)), dim3(256), 0, s, na, phase, interior, c.pendFlag.p);
@@ -1144,10 +1149,12 @@ void solverSolve(Ctx& c, float dt) {
                      c.colorOrder.p);
   hipcub::DoubleBuffer<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
   hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
+  // key bits in use: without blocks only the colour (bits 20-29; stable sort, so equal keys keep slot order)
+  const int keyLo = blocked ? 0 : 20, keyHi = blocked ? 53 : 30;
   size_t tb2 = 0;
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, 0, 53, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, keyLo, keyHi, s));
   c.cubTmp.ensure(tb2);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, 0, 53, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, keyLo, keyHi, s));
   if (dv.Current() != c.colorOrder.p)
     JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
   const unsigned long long* skey = dk.Current();
