@@ -547,6 +547,57 @@ constexpr int EPA_maxiterations = 256;
 constexpr float EPA_inface_eps = 0.01f;
 constexpr float EPA_accuracy = 0.001f;
 
+// hull support map spread over a wave's lanes (all lanes call it with the same arguments): per lane the
+// first maximum of its stride, then the largest dot product with the lowest index among equals, i.e. the
+// vertex hullSupport's strict '>' scan keeps
+__device__ V3 hullSupportWave(const float* verts, int first, int n, V3 vec) {
+  float l2 = len2(vec);
+  if (l2 < 0.0001f)
+    vec = v3(1, 0, 0);
+  else
+    vec = scl(vec, 1.0f / jsqrt(l2));
+  const float* p = verts + 3 * first;
+  float best = -1e30f;
+  int bi = -1;
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    V3 vtx = v3(p[3 * i] * 1.0f, p[3 * i + 1] * 1.0f, p[3 * i + 2] * 1.0f);
+    float d = dot(vec, vtx);
+    if (d > best) {
+      best = d;
+      bi = i;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float od = __shfl_xor(best, o);
+    const int oi = __shfl_xor(bi, o);
+    if (oi >= 0 && (bi < 0 || od > best || (od == best && oi < bi))) {
+      best = od;
+      bi = oi;
+    }
+  }
+  return bi >= 0 ? v3(p[3 * bi] * 1.0f, p[3 * bi + 1] * 1.0f, p[3 * bi + 2] * 1.0f) : v3(0, 0, 0);
+}
+__device__ V3 supportMarginWave(const Shape& s, const float* verts, V3 vec) {
+  if (s.type != JBGPU_SHAPE_HULL) return supportMargin(s, verts, vec);
+  V3 sup = hullSupportWave(verts, s.vfirst, s.vcount, vec);
+  if (s.margin != 0.0f) {
+    V3 vn = vec;
+    if (len2(vn) < (kFltEps * kFltEps)) vn = v3(-1, -1, -1);
+    vn = nor(vn);
+    vn = scl(vn, s.margin);
+    sup = add(sup, vn);
+  }
+  return sup;
+}
+
+// initial polytopes of EvaluatePD: faces and links of the tetrahedron and of the hexahedron
+// (GjkEpaSolver.java EPA.EvaluatePD tables)
+__constant__ int kTetF[4][3] = {{2, 1, 0}, {3, 0, 1}, {3, 1, 2}, {3, 2, 0}};
+__constant__ int kTetE[6][4] = {{0, 0, 2, 1}, {0, 1, 1, 1}, {0, 2, 3, 1}, {1, 0, 3, 2}, {2, 0, 1, 2}, {3, 0, 2, 2}};
+__constant__ int kHexF[6][3] = {{2, 0, 4}, {4, 1, 2}, {1, 4, 0}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+__constant__ int kHexE[9][4] = {{0, 0, 4, 0}, {0, 1, 2, 1}, {0, 2, 1, 2}, {1, 1, 5, 2}, {1, 0, 2, 0},
+                                {2, 2, 3, 2}, {3, 1, 5, 0}, {3, 0, 4, 2}, {5, 1, 4, 1}};
+
 struct Mkv {
   V3 w, r;
 };
@@ -572,8 +623,10 @@ struct EpaSolver {
   V3 pos[2];
   const float* verts;
   float margin;
-  int table[64];
-  Mkv simplex[5];
+  // shared by the wave's lanes (all of them run the job in lockstep on identical values): LDS
+  int* table;     // 64 hash heads
+  Mkv* simplex;   // 5 vertices
+  int* horizon;   // BuildHorizon stack: 64 faces, then 64 edges
   V3 ray;
   int order, iterations;
   bool failed;
@@ -581,17 +634,17 @@ struct EpaSolver {
   int root, nfaces, epaIter;
   V3 nearest[2];
 
-  __device__ static int Hash(V3 v) {
+  __device__ __forceinline__ static int Hash(V3 v) {
     int h = jint(v.x * 15461) ^ jint(v.y * 83003) ^ jint(v.z * 15473);
     return (int)((uint32_t)h * 169639u) & GJK_hashmask;
   }
-  __device__ V3 LocalSupport(V3 d, int i) {
+  __device__ __forceinline__ V3 LocalSupport(V3 d, int i) {
     V3 t = mulMtV(wrot[i], d);
-    V3 o = supportMargin(sh[i], verts, t);
+    V3 o = supportMarginWave(sh[i], verts, t);
     o = mulMV(wrot[i], o);
     return add(o, pos[i]);
   }
-  __device__ void Support(V3 d, Mkv& v) {
+  __device__ __forceinline__ void Support(V3 d, Mkv& v) {
     v.r = d;
     V3 t1 = LocalSupport(d, 0);
     V3 t2 = LocalSupport(scl(d, -1.0f), 1);
@@ -600,7 +653,7 @@ struct EpaSolver {
     v.w.y += margin * d.y;
     v.w.z += margin * d.z;
   }
-  __device__ bool FetchSupport() {
+  __device__ __forceinline__ bool FetchSupport() {
     int h = Hash(ray);
     int e = table[h];
     int guardHops = 0;
@@ -626,7 +679,7 @@ struct EpaSolver {
     Support(ray, simplex[++order]);
     return (dot(ray, simplex[order].w) > 0);
   }
-  __device__ bool SolveSimplex2(V3 ao, V3 ab) {
+  __device__ __forceinline__ bool SolveSimplex2(V3 ao, V3 ab) {
     if (dot(ab, ao) >= 0) {
       V3 cabo = crs(ab, ao);
       if (len2(cabo) > GJK_sqinsimplex_eps)
@@ -640,7 +693,7 @@ struct EpaSolver {
     }
     return false;
   }
-  __device__ bool SolveSimplex3a(V3 ao, V3 ab, V3 ac, V3 cabc) {
+  __device__ __forceinline__ bool SolveSimplex3a(V3 ao, V3 ab, V3 ac, V3 cabc) {
     V3 t = crs(cabc, ab);
     V3 t2 = crs(cabc, ac);
     if (dot(t, ao) < -GJK_insimplex_eps) {
@@ -668,7 +721,7 @@ struct EpaSolver {
       return true;
     }
   }
-  __device__ bool SolveSimplex4(V3 ao, V3 ab, V3 ac, V3 ad) {
+  __device__ __forceinline__ bool SolveSimplex4(V3 ao, V3 ab, V3 ac, V3 ad) {
     V3 t = crs(ab, ac), t2 = crs(ac, ad), t3 = crs(ad, ab);
     if (dot(t, ao) > GJK_insimplex_eps) {
       order = 2;
@@ -689,12 +742,12 @@ struct EpaSolver {
     }
     return true;
   }
-  __device__ bool SearchOrigin() {
+  __device__ __forceinline__ bool SearchOrigin() {
     iterations = 0;
     order = -1;
     failed = false;
     ray = nor(v3(1, 0, 0));
-    for (int i = 0; i < 64; i++) table[i] = -1;
+    table[threadIdx.x & 63] = -1;  // one entry per lane
     W.nhe = 0;
     FetchSupport();
     ray = scl(simplex[0].w, -1.0f);
@@ -726,7 +779,7 @@ struct EpaSolver {
     failed = true;
     return false;
   }
-  __device__ bool EncloseOrigin() {
+  __device__ __forceinline__ bool EncloseOrigin() {
     switch (order) {
       case 0:
         break;
@@ -763,7 +816,7 @@ struct EpaSolver {
     }
     return false;
   }
-  __device__ int addVertex(const Mkv& m) {
+  __device__ __forceinline__ int addVertex(const Mkv& m) {
     if (W.nvtx >= kEpaVerts) {
       W.overflow = true;
       return -1;
@@ -773,7 +826,7 @@ struct EpaSolver {
     st3(W.vtx + 6 * v + 3, m.r);
     return v;
   }
-  __device__ bool Set(int f, int a, int b, int c) {
+  __device__ __forceinline__ bool Set(int f, int a, int b, int c) {
     V3 aw = W.vw(a), bw = W.vw(b), cw = W.vw(c);
     V3 nrm = crs(sub(bw, aw), sub(cw, aw));
     float l = len(nrm);
@@ -789,7 +842,7 @@ struct EpaSolver {
     W.ff(f, 12) = jmax(0.0f, -dot(n, aw));
     return valid;
   }
-  __device__ int NewFace(int a, int b, int c) {
+  __device__ __forceinline__ int NewFace(int a, int b, int c) {
     if (W.nface >= kEpaFaces) {
       W.overflow = true;
       return -1;
@@ -807,13 +860,13 @@ struct EpaSolver {
     }
     return pf;
   }
-  __device__ void Detach(int f) {
+  __device__ __forceinline__ void Detach(int f) {
     int pv = W.fi(f, 14), nx = W.fi(f, 15);
     if (pv >= 0 || nx >= 0) {
       --nfaces;
       if (f == root) {
         root = nx;
-        W.fi(root, 14) = -1;
+        if (root >= 0) W.fi(root, 14) = -1;
       } else {
         if (nx < 0) {
           W.fi(pv, 15) = -1;
@@ -825,16 +878,17 @@ struct EpaSolver {
       W.fi(f, 14) = W.fi(f, 15) = -1;
     }
   }
-  __device__ void Link(int f0, int e0, int f1, int e1) {
+  __device__ __forceinline__ void Link(int f0, int e0, int f1, int e1) {
     W.fi(f0, 3 + e0) = f1;
     W.fi(f1, 6 + e1) = e0;
     W.fi(f1, 3 + e1) = f0;
     W.fi(f0, 6 + e0) = e1;
   }
   // BuildHorizon (GjkEpaSolver.java:686-706) as an explicit-stack traversal in the same visiting order
-  __device__ int BuildHorizon(int markid, int w, int f0, int e0, int* cf, int* ff) {
+  __device__ __forceinline__ int BuildHorizon(int markid, int w, int f0, int e0, int* cf, int* ff) {
     const int mod3[5] = {0, 1, 2, 0, 1};
-    int stkF[64], stkE[64];
+    int* stkF = horizon;
+    int* stkE = horizon + 64;
     int sp = 0;
     int ne = 0;
     stkF[sp] = f0;
@@ -879,31 +933,37 @@ struct EpaSolver {
     }
     return ne;
   }
-  __device__ int FindBest() {
+  // the list holds the live faces newest first (NewFace inserts at the root), so the serial walk's first
+  // smallest distance is the newest face among the smallest: every lane scans a stride of the pool, then a
+  // wave reduction (smaller distance, equal distance: higher index) picks the same face
+  __device__ __forceinline__ int FindBest() {
+    if (root < 0) return -1;
+    const int lane = threadIdx.x & 63;
+    float bd = kInf;
     int bf = -1;
-    if (root >= 0) {
-      int cf = root;
-      float bd = kInf;
-      int hops = 0;
-      do {
-        if (W.ff(cf, 12) < bd) {
-          bd = W.ff(cf, 12);
-          bf = cf;
-        }
-        if (++hops > W.nface) {  // the face list is acyclic; bound the walk regardless
-          W.overflow = true;
-          return -1;
-        }
-      } while (-1 != (cf = W.fi(cf, 15)));
+    for (int f = lane; f < W.nface; f += 64) {
+      if (f != root && W.fi(f, 14) < 0) continue;  // not in the list: only the root has no predecessor
+      const float d = W.ff(f, 12);
+      if (d < bd || (d == bd && bf >= 0)) {
+        bd = d;
+        bf = f;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const float od = __shfl_xor(bd, o);
+      const int of = __shfl_xor(bf, o);
+      if (of >= 0 && (bf < 0 || od < bd || (od == bd && of > bf))) {
+        bd = od;
+        bf = of;
+      }
     }
     return bf;
   }
-  __device__ float EvaluatePD() {
-    const int tet_f[4][3] = {{2, 1, 0}, {3, 0, 1}, {3, 1, 2}, {3, 2, 0}};
-    const int tet_e[6][4] = {{0, 0, 2, 1}, {0, 1, 1, 1}, {0, 2, 3, 1}, {1, 0, 3, 2}, {2, 0, 1, 2}, {3, 0, 2, 2}};
-    const int hex_f[6][3] = {{2, 0, 4}, {4, 1, 2}, {1, 4, 0}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
-    const int hex_e[9][4] = {{0, 0, 4, 0}, {0, 1, 2, 1}, {0, 2, 1, 2}, {1, 1, 5, 2}, {1, 0, 2, 0},
-                             {2, 2, 3, 2}, {3, 1, 5, 0}, {3, 0, 4, 2}, {5, 1, 4, 1}};
+  __device__ __forceinline__ float EvaluatePD() {
+    const int(*tet_f)[3] = kTetF;
+    const int(*tet_e)[4] = kTetE;
+    const int(*hex_f)[3] = kHexF;
+    const int(*hex_e)[4] = kHexE;
     int bestface = -1;
     int markid = 1;
     float depth = -kInf;
@@ -966,6 +1026,7 @@ struct EpaSolver {
       V3 bcw = scl(v3(a1, a2, a0), 1.0f / (sm > 0.0f ? sm : 1.0f));
       depth = jmax(0.0f, fd);
       int vv[3] = {v0, v1, v2};
+#pragma unroll
       for (int i = 0; i < 2; ++i) {
         float s = i != 0 ? -1.0f : 1.0f;
         V3 f0 = LocalSupport(scl(W.vr(vv[0]), s), i);
@@ -979,15 +1040,18 @@ struct EpaSolver {
   }
 };
 
-// one thread per queued pair; the GJK partial result is merged as in GjkPairDetector.java:263-303
-// one EPA job per workgroup: its polytope workspace (faces, vertices, hash entries) lives in the
-// workgroup's LDS; the job itself is sequential and runs on the first lane
-// one EPA job (pair i) with the workspace `slot` in LDS
+// one EPA job (pair i), run by all lanes of a wave on identical values: the polytope workspace (faces,
+// vertices, hash entries) and the solver's arrays are in LDS, the face search and hull support maps are
+// spread over the lanes; the GJK partial result is merged as in GjkPairDetector.java:263-303
 __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                        const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
-                       const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot) {
+                       const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot,
+                       int* aux) {
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
+  S.table = aux;
+  S.horizon = aux + 64;
+  S.simplex = reinterpret_cast<Mkv*>(aux + 192);
   S.sh[0] = bodyShape(shapes, props, a);
   S.sh[1] = bodyShape(shapes, props, b);
   Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
@@ -1017,7 +1081,8 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
       wB = S.nearest[1];
     }
   }
-  if (S.W.overflow) atomicAdd(fails + 1, 1ull);
+  const bool lane0 = (threadIdx.x & 63) == 0;
+  if (S.W.overflow && lane0) atomicAdd(fails + 1, 1ull);
   if (ok) {
     V3 tn = sub(wB, wA);
     float lenSqr = len2(tn);
@@ -1032,24 +1097,24 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
         isValid = true;
       }
     }
-  } else {
+  } else if (lane0) {
     atomicAdd(fails, 1ull);
   }
-  putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
+  if (lane0) putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
 }
 
-// EPA jobs in a fixed grid: every block steps through the queue (its length read on the device) until it
-// is exhausted; lane 0 of each block runs the job
+// EPA jobs in a fixed grid: every block (one wave) steps through the queue (its length read on the device)
+// until it is exhausted; all lanes run each job together, lane 0 writes its result
 __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict__ count, const int* __restrict__ queue,
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                                            const float* __restrict__ verts, const float* __restrict__ props,
                                            const float* __restrict__ tr, const float* __restrict__ epaState,
                                            float* cand, unsigned long long* fails) {
   __shared__ float slot[kEpaSlotWords];
-  if (threadIdx.x != 0) return;
+  __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
-    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot);
+    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux);
 }
 
 // ------------------------------------------------------------------ manifold update
