@@ -170,7 +170,7 @@ def main():
     print(json.dumps(out))
 
 
-PMC_WARMUP, PMC_STEPS, PMC_COPY_BYTES = 150, 2, 64 << 20
+PMC_WARMUP, PMC_STEPS, PMC_COPY_BYTES = 300, 2, 64 << 20
 
 
 def pmc_child(args):
