@@ -42,7 +42,7 @@ typedef struct jbgpu_config {
   float world_min[3];            /* AxisSweep3 world bounds (AxisSweep3Internal.java:80-103) */
   float world_max[3];
   int32_t num_iterations;        /* ContactSolverInfo.numIterations (ContactSolverInfo.java:38) */
-  int32_t solver_order;          /* JBGPU_ORDER_COLOR */
+  int32_t solver_order;          /* JBGPU_ORDER_COLOR (the only order on the device; others: JBGPU_ERR_ARG) */
   int32_t deactivation_disabled; /* BulletGlobals.setDeactivationDisabled (BulletGlobals.java:114-116) */
   float tolerance;               /* > 0: iterate to residual <= tolerance (capped by max_iterations) */
   int32_t max_iterations;
@@ -80,14 +80,16 @@ int jbgpu_last_error(const jbgpu_ctx* ctx, char* buf, int32_t len);
 /* Shape record as built by the reference's constructors: SphereShape(r), BoxShape(he),
  * CapsuleShape(r, height), ConvexHullShape(points) (recalcLocalAabb), StaticPlaneShape(n, c).
  * params: sphere {r}; box {hx,hy,hz}; capsule {r,height}; hull {first,count} (points in verts);
- * plane {nx,ny,nz,c}.  (collision/shapes/*.java constructors) */
+ * plane {nx,ny,nz,c}.  (the shape constructors in collision/shapes/) */
 int jbgpu_make_shape(int32_t type, const float* params, const float* verts, float* rec);
 /* CollisionShape.calculateLocalInertia(mass, inertia) for a shape record. */
 int jbgpu_shape_local_inertia(const float* rec, const float* verts, float mass, float* inertia3);
 
 /* ---- BroadphaseInterface -------------------------------------------------------- */
-/* createProxy for n AABBs (min xyz, max xyz per proxy); uid_out[i] = creation index + 1
- * (DbvtBroadphase.java:173-182).  world ids may be NULL (all 0); pairs only form inside a world. */
+/* createProxy for n AABBs (min xyz, max xyz per proxy), appended after the proxies created so far;
+ * uid_out[i] = creation index + 1 in every mode, never reused (DbvtBroadphase.java:173-182 gid;
+ * AxisSweep3 and SimpleBroadphase number their handles differently, the pair set is the same).
+ * world ids may be NULL (all 0); pairs only form inside a world. */
 int jbgpu_bp_create_proxies(jbgpu_ctx* ctx, int32_t n, const float* aabb6, const int16_t* group,
                             const int16_t* mask, const int32_t* world, int32_t* uid_out);
 /* setAabb for every proxy (DbvtBroadphase.java:196-228 / AxisSweep3Internal.java:542-579).
@@ -97,6 +99,18 @@ int jbgpu_bp_set_aabbs(jbgpu_ctx* ctx, int32_t n, const float* aabb6, int32_t on
 int jbgpu_bp_calculate_overlapping_pairs(jbgpu_ctx* ctx, int64_t* npairs);
 /* getOverlappingPairCache().getOverlappingPairArray() as sorted keys (uid_lo << 32 | uid_hi). */
 int jbgpu_bp_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, int64_t cap, int64_t* n);
+/* destroyProxy for n uids (DbvtBroadphase.java:184-194, AxisSweep3Internal.java:498-540,
+ * SimpleBroadphase.destroyProxy): the proxies never pair again and, as
+ * removeOverlappingPairsContainingProxy (HashedOverlappingPairCache.java), their pairs leave the current
+ * pair set at once.  Later setAabb calls still pass one box per created proxy (ignored for destroyed
+ * ones).  Broadphase-only contexts; a uid destroyed twice is an error. */
+int jbgpu_bp_destroy_proxies(jbgpu_ctx* ctx, int32_t n, const int32_t* uids);
+/* The pairs the last pair computation (calculate_overlapping_pairs or a world step) added and removed,
+ * as sorted keys: what addOverlappingPair / removeOverlappingPair calls a pair cache would have seen
+ * (HashedOverlappingPairCache.java:65-110), so a caller keeps its per-pair objects across steps.
+ * Either array may be NULL (counts only); error if a count exceeds its capacity. */
+int jbgpu_bp_get_pair_deltas(jbgpu_ctx* ctx, uint64_t* added, int64_t cap_added, int64_t* n_added,
+                             uint64_t* removed, int64_t cap_removed, int64_t* n_removed);
 int jbgpu_bp_get_proxies(jbgpu_ctx* ctx, float* rec, int32_t* stage_current);
 int jbgpu_bp_set_proxies(jbgpu_ctx* ctx, const float* rec, int32_t stage_current);
 int jbgpu_bp_get_quantized(jbgpu_ctx* ctx, int32_t* q6);

@@ -905,6 +905,121 @@ void bpFindPairs(Ctx& c) {
   c.st.pairs = c.npairs;
 }
 
+// ---- proxy removal and pair deltas ------------------------------------------------------------------
+__global__ void kKillProxies(int n, const int* __restrict__ uids, int* filt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) filt[uids[i] - 1] = 0;
+}
+// 1 for the keys whose proxies both still pair (a destroyed proxy has group and mask 0)
+__global__ void kPairLive(int64_t n, const uint64_t* __restrict__ keys, const int* __restrict__ filt, int* flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  flag[i] = filt[(int)(k >> 32) - 1] != 0 && filt[(int)(k & 0xffffffffu) - 1] != 0;
+}
+// 1 for the keys of a that are not in the sorted keys b
+__global__ void kKeyAbsent(int64_t n, const uint64_t* __restrict__ a, int64_t m, const uint64_t* __restrict__ b,
+                           int* flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = a[i];
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (b[mid] < k) lo = mid + 1; else hi = mid;
+  }
+  flag[i] = !(lo < m && b[lo] == k);
+}
+
+// keys[0, n) := the flagged keys, in order; returns their count
+static int64_t compactKeys(Ctx& c, uint64_t* keys, int64_t n, const int* flag) {
+  if (n == 0) return 0;
+  hipStream_t s = c.stream;
+  c.deltaKeys.ensure(n);
+  int* cnt = (int*)(c.counter.p + 7);
+  size_t tb = 0;
+  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, keys, flag, c.deltaKeys.p, cnt, (int)n, s));
+  c.cubTmp.ensure(tb);
+  JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, keys, flag, c.deltaKeys.p, cnt, (int)n, s));
+  int k = 0;
+  JB_CHECK(hipMemcpyAsync(&k, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  if (k) JB_CHECK(hipMemcpyAsync(keys, c.deltaKeys.p, (size_t)k * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  return k;
+}
+
+void bpDestroyProxies(Ctx& c, int n, const int32_t* uids) {
+  std::vector<int> u(uids, uids + n);
+  for (int i = 0; i < n; i++) {
+    if (u[i] < 1 || u[i] > c.np) throw std::invalid_argument("proxy uid out of range");
+    if (c.proxyDead[u[i] - 1]) throw std::invalid_argument("proxy already destroyed");
+    c.proxyDead[u[i] - 1] = 1;
+  }
+  if (n == 0) return;
+  hipStream_t s = c.stream;
+  c.counter.ensure(8);
+  c.deltaFlag.ensure(n);
+  JB_CHECK(hipMemcpyAsync(c.deltaFlag.p, u.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(kKillProxies, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)c.deltaFlag.p, c.filt.p);
+  // removeOverlappingPairsContainingProxy: the pairs leave the current set and the previous one (so
+  // the next delta lists them in neither direction)
+  const int64_t sizes[2] = {c.npairs, c.npairsOld};
+  int64_t kept[2] = {0, 0};
+  for (int w = 0; w < 2; w++) {
+    const int64_t m = sizes[w];
+    uint64_t* keys = c.keys[w == 0 ? c.cur : 1 - c.cur].p;
+    if (m == 0 || !keys) continue;
+    c.deltaFlag.ensure(m);
+    hipLaunchKernelGGL(kPairLive, dim3(gridFor(m)), dim3(256), 0, s, m, (const uint64_t*)keys, (const int*)c.filt.p,
+                       c.deltaFlag.p);
+    kept[w] = compactKeys(c, keys, m, c.deltaFlag.p);
+  }
+  JB_CHECK(hipStreamSynchronize(s));
+  c.npairs = kept[0];
+  c.npairsOld = kept[1];
+  c.st.pairs = c.npairs;
+}
+
+// pairs the last pair computation added (in the current set, not the previous) and removed
+void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, uint64_t* removed, int64_t capRemoved,
+                  int64_t* nRemoved) {
+  hipStream_t s = c.stream;
+  c.counter.ensure(8);
+  for (int w = 0; w < 2; w++) {
+    const int64_t n = w == 0 ? c.npairs : c.npairsOld, m = w == 0 ? c.npairsOld : c.npairs;
+    const uint64_t* a = c.keys[w == 0 ? c.cur : 1 - c.cur].p;
+    const uint64_t* b = c.keys[w == 0 ? 1 - c.cur : c.cur].p;
+    uint64_t* out = w == 0 ? added : removed;
+    const int64_t cap = w == 0 ? capAdded : capRemoved;
+    int64_t* cnt = w == 0 ? nAdded : nRemoved;
+    int64_t k = 0;
+    if (n > 0) {
+      c.deltaFlag.ensure(n);
+      c.deltaKeys.ensure(n);
+      if (m > 0) {
+        hipLaunchKernelGGL(kKeyAbsent, dim3(gridFor(n)), dim3(256), 0, s, n, a, m, b, c.deltaFlag.p);
+        int* dc = (int*)(c.counter.p + 7);
+        size_t tb = 0;
+        JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, a, (const int*)c.deltaFlag.p, c.deltaKeys.p, dc, (int)n, s));
+        c.cubTmp.ensure(tb);
+        JB_CHECK(
+            hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, a, (const int*)c.deltaFlag.p, c.deltaKeys.p, dc, (int)n, s));
+        int kk = 0;
+        JB_CHECK(hipMemcpyAsync(&kk, dc, sizeof(int), hipMemcpyDeviceToHost, s));
+        JB_CHECK(hipStreamSynchronize(s));
+        k = kk;
+        if (out && k > cap) throw std::length_error("pair delta buffer too small");
+        if (out && k) JB_CHECK(hipMemcpy(out, c.deltaKeys.p, (size_t)k * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      } else {  // nothing before: every key is new
+        k = n;
+        if (out && k > cap) throw std::length_error("pair delta buffer too small");
+        if (out) JB_CHECK(hipMemcpy(out, a, (size_t)k * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      }
+    }
+    if (cnt) *cnt = k;
+  }
+}
+
 void bpRemapManifolds(Ctx& c) {
   const int bits = bitsFor((int64_t)c.np + 1);
   int nxt = 1 - c.cur;

@@ -38,6 +38,19 @@ struct DBuf {  // device buffer owned by a context; freed with it
     JB_CHECK(hipMalloc((void**)&p, cap * sizeof(T)));
     n = cap;
   }
+  void ensureKeep(size_t k) {  // as ensure, keeping the first n elements
+    if (k <= n) return;
+    size_t cap = k + k / 4 + 64;
+    T* q = nullptr;
+    JB_CHECK(hipMalloc((void**)&q, cap * sizeof(T)));
+    if (p) {
+      JB_CHECK(hipDeviceSynchronize());
+      JB_CHECK(hipMemcpy(q, p, n * sizeof(T), hipMemcpyDeviceToDevice));
+      (void)hipFree(p);
+    }
+    p = q;
+    n = cap;
+  }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -104,6 +117,10 @@ struct Ctx {
   DBuf<float> cand;        // candidate boxes np*6 (float4 pairs: min xyz pad, max xyz pad -> np*8)
   DBuf<uint64_t> morton, mortonAlt;  // sort keys: world id << 32 | Morton code
   int worldBits = 0;                  // bits of the largest world id (0: a single world)
+  int maxWorld = 0;                   // largest world id of the proxies
+  std::vector<char> proxyDead;        // destroyed proxies (slots are not reused)
+  DBuf<uint64_t> deltaKeys;           // pair deltas: compacted keys
+  DBuf<int> deltaFlag;
   DBuf<int> order, orderAlt;  // sorted leaf -> proxy
   DBuf<int> isHuge;
   DBuf<int> hugeList, hugeListAlt, hugeWorld, hugeWorldAlt;
@@ -239,6 +256,9 @@ void bpUpdateFromAabbs(Ctx& c, const float* dAabb6);    // setAabb for all proxi
 void bpFindPairs(Ctx& c);                                // A3-A7: sorted canonical keys in keys[cur]
 void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
 // narrowphase
+void bpDestroyProxies(Ctx& c, int n, const int32_t* uids);  // BroadphaseInterface.destroyProxy
+void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, uint64_t* removed, int64_t capRemoved,
+                  int64_t* nRemoved);
 void npDispatchAll(Ctx& c);                              // A8-A16
 int64_t npCountContacts(Ctx& c);                         // contact points of the current manifolds
 void npCollectStats(Ctx& c);                             // EPA statistics of the last narrowphase

@@ -173,18 +173,30 @@ void upload(DBuf<T>& b, const T* src, size_t n, hipStream_t s) {
   if (n) JB_CHECK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// proxy state for n proxies, keeping the first c.np (appended proxies follow the existing ones)
 void allocProxyState(Ctx& c, int n) {
+  c.stage.ensureKeep(n);
+  c.leaf.ensureKeep(6 * (size_t)n);
+  c.aabb.ensureKeep(6 * (size_t)n);
+  c.q.ensureKeep(6 * (size_t)n);
+  c.filt.ensureKeep(n);
+  c.world.ensureKeep(n);
+  c.proxyDead.resize(n, 0);
   c.np = n;
-  c.stage.ensure(n);
-  c.leaf.ensure(6 * (size_t)n);
-  c.aabb.ensure(6 * (size_t)n);
-  c.q.ensure(6 * (size_t)n);
-  c.filt.ensure(n);
-  c.world.ensure(n);
 }
 
-void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, const int16_t* mask, const int32_t* world) {
-  allocProxyState(c, n);
+// createProxy for n new proxies after the existing ones (uids np + 1 ..); `replace` starts from none
+void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, const int16_t* mask, const int32_t* world,
+                   bool replace) {
+  if (replace) {
+    c.np = 0;
+    c.maxWorld = 0;
+    c.proxyDead.clear();
+    c.npairs = 0;
+    c.npairsOld = 0;
+  }
+  const int n0 = c.np;
+  if ((int64_t)n0 + n >= (1ll << 30)) throw std::length_error("too many proxies");
   std::vector<int> stage(n, c.stageCurrent), filt(n), wid(n);
   std::vector<int> q(6 * (size_t)n, 0);
   std::vector<float> box(aabb6, aabb6 + 6 * (size_t)n);
@@ -194,10 +206,10 @@ void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, cons
     wid[i] = world ? world[i] : 0;
     if (wid[i] < 0) throw std::invalid_argument("negative world id");
   }
-  int maxWorld = 0;
-  for (int i = 0; i < n; i++) maxWorld = std::max(maxWorld, wid[i]);
+  allocProxyState(c, n0 + n);
+  for (int i = 0; i < n; i++) c.maxWorld = std::max(c.maxWorld, wid[i]);
   c.worldBits = 0;
-  while (c.worldBits < 31 && (1ll << c.worldBits) <= maxWorld) c.worldBits++;
+  while (c.worldBits < 31 && (1ll << c.worldBits) <= c.maxWorld) c.worldBits++;
   if (c.bpMode == JBGPU_BP_SAP16 || c.bpMode == JBGPU_BP_SAP32) {
     // AxisSweep3Internal.quantize (host copy of the device routine)
     for (int i = 0; i < n; i++)
@@ -211,15 +223,13 @@ void uploadProxies(Ctx& c, int n, const float* aabb6, const int16_t* group, cons
         }
   }
   hipStream_t s = c.stream;
-  JB_CHECK(hipMemcpyAsync(c.stage.p, stage.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
-  JB_CHECK(hipMemcpyAsync(c.leaf.p, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
-  JB_CHECK(hipMemcpyAsync(c.aabb.p, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
-  JB_CHECK(hipMemcpyAsync(c.q.p, q.data(), 6 * (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
-  JB_CHECK(hipMemcpyAsync(c.filt.p, filt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
-  JB_CHECK(hipMemcpyAsync(c.world.p, wid.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.stage.p + n0, stage.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.leaf.p + 6 * (size_t)n0, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.aabb.p + 6 * (size_t)n0, box.data(), 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.q.p + 6 * (size_t)n0, q.data(), 6 * (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.filt.p + n0, filt.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.world.p + n0, wid.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
   JB_CHECK(hipStreamSynchronize(s));
-  c.npairs = 0;
-  c.npairsOld = 0;
 }
 
 void bodiesToDevice(Ctx& c, const float* bodies) {
@@ -358,17 +368,22 @@ void jbgpu_config_default(jbgpu_config* cfg) {
 int jbgpu_ctx_create(const jbgpu_config* cfg, jbgpu_ctx** out) {
   if (!out) return JBGPU_ERR_ARG;
   *out = nullptr;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-    g_createError = "no HIP device";
-    return JBGPU_ERR_NO_DEVICE;
-  }
   jbgpu_config d;
   jbgpu_config_default(&d);
   if (!cfg) cfg = &d;
   if (cfg->bp_mode < 0 || cfg->bp_mode > 3 || cfg->num_iterations < 0 || cfg->max_iterations < 1) {
     g_createError = "invalid configuration";
     return JBGPU_ERR_ARG;
+  }
+  if (cfg->solver_order != JBGPU_ORDER_COLOR) {
+    g_createError = "the device solves in graph-colour batches (JBGPU_ORDER_COLOR); the island order with the "
+                    "Random(0) shuffle is not available on the device";
+    return JBGPU_ERR_ARG;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    g_createError = "no HIP device";
+    return JBGPU_ERR_NO_DEVICE;
   }
   jbgpu_ctx* ctx = new jbgpu_ctx();
   Ctx& c = ctx->c;
@@ -526,9 +541,10 @@ int jbgpu_bp_create_proxies(jbgpu_ctx* ctx, int32_t n, const float* aabb6, const
   return guard(ctx, [&] {
     if (n <= 0 || !aabb6) throw std::invalid_argument("bad proxy arguments");
     if (ctx->c.haveWorld) throw std::logic_error("proxies of a world are created by jbgpu_world_create");
-    uploadProxies(ctx->c, n, aabb6, group, mask, world);
+    const int n0 = ctx->c.np;
+    uploadProxies(ctx->c, n, aabb6, group, mask, world, false);
     if (uid_out)
-      for (int i = 0; i < n; i++) uid_out[i] = i + 1;
+      for (int i = 0; i < n; i++) uid_out[i] = n0 + i + 1;
   });
 }
 
@@ -571,6 +587,22 @@ int jbgpu_bp_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, int64_t cap, int64_t* n) 
     if (cap < c.npairs) throw std::length_error("pair buffer too small");
     if (c.npairs)
       JB_CHECK(hipMemcpy(keys, c.keys[c.cur].p, c.npairs * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  });
+}
+
+int jbgpu_bp_destroy_proxies(jbgpu_ctx* ctx, int32_t n, const int32_t* uids) {
+  return guard(ctx, [&] {
+    if (n < 0 || (n > 0 && !uids)) throw std::invalid_argument("bad proxy arguments");
+    if (ctx->c.haveWorld) throw std::logic_error("proxies of a world live as long as the world");
+    bpDestroyProxies(ctx->c, n, uids);
+  });
+}
+
+int jbgpu_bp_get_pair_deltas(jbgpu_ctx* ctx, uint64_t* added, int64_t cap_added, int64_t* n_added, uint64_t* removed,
+                             int64_t cap_removed, int64_t* n_removed) {
+  return guard(ctx, [&] {
+    if (cap_added < 0 || cap_removed < 0) throw std::invalid_argument("negative capacity");
+    bpPairDeltas(ctx->c, added, cap_added, n_added, removed, cap_removed, n_removed);
   });
 }
 
@@ -662,7 +694,7 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     }
     // hull AABBs read the hull points through the shape record only (host copy above uses records)
     c.stageCurrent = 0;
-    uploadProxies(c, nbodies, box.data(), group.data(), mask.data(), world_ids);
+    uploadProxies(c, nbodies, box.data(), group.data(), mask.data(), world_ids, true);
     c.haveWorld = true;
   });
 }

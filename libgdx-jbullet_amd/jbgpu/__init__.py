@@ -56,6 +56,9 @@ SYMBOLS = {
     "jbgpu_bp_set_aabbs": (C.c_int, [_vp, C.c_int32, _vp, C.c_int32]),
     "jbgpu_bp_calculate_overlapping_pairs": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "jbgpu_bp_get_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64)]),
+    "jbgpu_bp_destroy_proxies": (C.c_int, [_vp, C.c_int32, _vp]),
+    "jbgpu_bp_get_pair_deltas": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64), _vp, C.c_int64,
+                                           C.POINTER(C.c_int64)]),
     "jbgpu_bp_get_proxies": (C.c_int, [_vp, _f, C.POINTER(C.c_int32)]),
     "jbgpu_bp_set_proxies": (C.c_int, [_vp, _f, C.c_int32]),
     "jbgpu_bp_get_quantized": (C.c_int, [_vp, _i32]),
@@ -200,8 +203,23 @@ class Context:
         w = None if world is None else np.ascontiguousarray(world, np.int32)
         uid = np.zeros(n, np.int32)
         self._ck(load().jbgpu_bp_create_proxies(self.h, n, a, _p(g), _p(m), _p(w), _p(uid)), "bp_create_proxies")
-        self.np = n
+        self.np += n
         return uid
+
+    def bp_destroy_proxies(self, uids):
+        u = np.ascontiguousarray(uids, np.int32)
+        self._ck(load().jbgpu_bp_destroy_proxies(self.h, int(u.size), _p(u)), "bp_destroy_proxies")
+
+    def bp_pair_deltas(self):
+        """(added, removed) sorted keys of the last pair computation."""
+        na, nr = C.c_int64(0), C.c_int64(0)
+        self._ck(load().jbgpu_bp_get_pair_deltas(self.h, None, 0, C.byref(na), None, 0, C.byref(nr)),
+                 "bp_get_pair_deltas")
+        a = np.zeros(max(na.value, 1), np.uint64)
+        r = np.zeros(max(nr.value, 1), np.uint64)
+        self._ck(load().jbgpu_bp_get_pair_deltas(self.h, _p(a), a.size, C.byref(na), _p(r), r.size, C.byref(nr)),
+                 "bp_get_pair_deltas")
+        return a[:na.value], r[:nr.value]
 
     def bp_set_aabbs(self, aabb6):
         a = f32(aabb6).reshape(-1)

@@ -72,6 +72,11 @@ void Broadphase::createProxy(Aabb a, int16_t group, int16_t mask) {
   proxies.push_back(p);
 }
 
+// destroyProxy (DbvtBroadphase.java:184-194; AxisSweep3Internal.removeHandle :498-540;
+// SimpleBroadphase.destroyProxy): the proxy leaves every structure and
+// removeOverlappingPairsContainingProxy drops its pairs; ids of the remaining proxies are unchanged
+void Broadphase::destroyProxy(int idx) { proxies[idx].dead = true; }
+
 void Broadphase::setAabb(int idx, Aabb A) {
   Proxy& p = proxies[idx];
   if (mode == JBGPU_BP_SAP16 || mode == JBGPU_BP_SAP32) {  // updateHandle :542-579
@@ -129,8 +134,9 @@ void Broadphase::collide() {
 }
 
 static inline bool filterPass(const Proxy& a, const Proxy& b) {  // HashedOverlappingPairCache.java:184-185
-  // separate DynamicsWorld instances (one broadphase each) never see each other's proxies
-  return a.world == b.world && ((a.group & b.mask) != 0) && ((b.group & a.mask) != 0);
+  // separate DynamicsWorld instances (one broadphase each) never see each other's proxies;
+  // destroyed proxies are in no structure any more
+  return !a.dead && !b.dead && a.world == b.world && ((a.group & b.mask) != 0) && ((b.group & a.mask) != 0);
 }
 
 bool Broadphase::overlap(int i, int j) const {

@@ -275,6 +275,10 @@ void orc_bp_create(void* h, int n, const float* aabb6, const int16_t* group, con
   for (int i = 0; i < n; i++)
     bp->createProxy(Aabb{ld3(aabb6 + 6 * i), ld3(aabb6 + 6 * i + 3)}, group ? group[i] : 1, mask ? mask[i] : -1);
 }
+void orc_bp_destroy(void* h, int n, const int32_t* uids) {
+  Broadphase* bp = (Broadphase*)h;
+  for (int i = 0; i < n; i++) bp->destroyProxy(uids[i] - 1);
+}
 void orc_bp_set_worlds(void* h, const int32_t* world) {
   Broadphase* bp = (Broadphase*)h;
   for (size_t i = 0; i < bp->proxies.size(); i++) bp->proxies[i].world = world[i];

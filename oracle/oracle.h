@@ -82,6 +82,7 @@ struct Proxy {
   int16_t group, mask;
   int32_t q[6];     // AxisSweep3 quantised min xyz / max xyz
   int32_t world = 0;  // batched independent worlds: proxies of different worlds never pair
+  bool dead = false;  // destroyProxy: out of the tree and the stage lists, never pairs again
 };
 struct Broadphase {
   int mode;  // JBGPU_BP_*
@@ -92,6 +93,7 @@ struct Broadphase {
   uint32_t handleMask = 0, sentinel = 0, qmask = 0;
   void init(int mode, V3 wmin, V3 wmax);
   void createProxy(Aabb a, int16_t group, int16_t mask);
+  void destroyProxy(int idx);  // DbvtBroadphase.java:184-194, AxisSweep3Internal.removeHandle, SimpleBroadphase
   void setAabb(int idx, Aabb a);
   void collide();  // stage rotation of DbvtBroadphase.collide
   // overlapping pair set, canonical (uid_lo, uid_hi), sorted ascending

@@ -60,6 +60,7 @@ def lib():
             "orc_bp_create": (None, [vp, C.c_int, _f, vp, vp]),
             "orc_bp_set_all": (None, [vp, _f]),
             "orc_bp_set_worlds": (None, [vp, _i32]),
+            "orc_bp_destroy": (None, [vp, C.c_int, _i32]),
             "orc_bp_step": (C.c_int64, [vp, _u64, C.c_int64, C.c_int]),
             "orc_bp_get_proxies": (None, [vp, _f]),
             "orc_bp_get_quantized": (None, [vp, _i32]),
@@ -200,6 +201,10 @@ class Broadphase:
 
     def set_worlds(self, world):
         lib().orc_bp_set_worlds(self.h, np.ascontiguousarray(world, np.int32))
+
+    def destroy(self, uids):
+        u = np.ascontiguousarray(uids, np.int32)
+        lib().orc_bp_destroy(self.h, int(u.size), u)
 
     def set_all(self, aabb6):
         lib().orc_bp_set_all(self.h, f32(aabb6).reshape(-1))

@@ -52,6 +52,13 @@ def test_argument_errors_are_status_codes():
     assert L.jbgpu_last_error(None, buf, 64) >= 0
 
 
+def test_reference_solver_order_is_rejected():
+    """The device solves in colour batches only: asking for the island order with the Random(0) shuffle is an
+    argument error (checked before any device is touched)."""
+    with pytest.raises(jbgpu.JbgpuError, match=r"\(-1\).*graph-colour"):
+        jbgpu.Context(solver_order=jbgpu.ORDER_REFERENCE)
+
+
 def _shape_cases():
     rng = np.random.default_rng(1)
     pts = rng.normal(size=(20, 3)).astype(np.float32)

@@ -50,6 +50,56 @@ def test_broadphase_pairs_match_oracle(mode):
     ctx.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", list(MODES))
+def test_broadphase_destroy_append_and_deltas(mode):
+    """destroyProxy / later createProxy calls and the pair deltas: pair sets bit-exact against the oracle
+    doing the same operations; deltas equal the set differences of consecutive pair sets."""
+    rng = np.random.default_rng(5)
+    n = 8000
+    c, h, v = _cloud(n, seed=21)
+    boxes = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    ctx = jbgpu.Context(bp_mode=MODES[mode], world_min=WMIN, world_max=WMAX)
+    uids = ctx.bp_create_proxies(boxes)
+    assert np.array_equal(uids, np.arange(1, n + 1))
+    orc = pyoracle.Broadphase(MODES[mode], np.array(WMIN, np.float32), np.array(WMAX, np.float32))
+    orc.create(boxes)
+    prev = np.zeros(0, np.uint64)
+    dead = np.zeros(0, np.int64)
+    for step in range(6):
+        if step == 2:  # destroy a tenth of the proxies: their pairs leave the current set at once
+            dead = rng.choice(n, n // 10, replace=False) + 1
+            ctx.bp_destroy_proxies(dead)
+            orc.destroy(dead)
+            cur = ctx.bp_pairs()
+            lo, hi = (cur >> np.uint64(32)).astype(np.int64), (cur & np.uint64(0xffffffff)).astype(np.int64)
+            assert not (np.isin(lo, dead) | np.isin(hi, dead)).any()
+            keep = ~(np.isin((prev >> np.uint64(32)).astype(np.int64), dead) |
+                     np.isin((prev & np.uint64(0xffffffff)).astype(np.int64), dead))
+            assert np.array_equal(cur, prev[keep])
+            prev = cur
+        if step == 4:  # createProxy after the first batch: uids continue
+            c2, h2, v2 = _cloud(1000, seed=22)
+            more = np.concatenate([c2 - h2, c2 + h2], 1).astype(np.float32)
+            u2 = ctx.bp_create_proxies(more)
+            assert np.array_equal(u2, np.arange(n + 1, n + 1001))
+            orc.create(more)
+            c, h, v = np.concatenate([c, c2]), np.concatenate([h, h2]), np.concatenate([v, v2])
+        boxes = scenes.aabb_cloud_step(c, h, v)
+        ctx.bp_set_aabbs(boxes)
+        orc.set_all(boxes)
+        ctx.bp_calculate_overlapping_pairs()
+        got, want = ctx.bp_pairs(), orc.step()
+        added, removed = ctx.bp_pair_deltas()
+        print(f"[bp-destroy {mode}] step {step}: {len(got)} pairs (oracle {len(want)}), +{len(added)} -{len(removed)}",
+              flush=True)
+        assert np.array_equal(got, want), f"{mode} step {step}"
+        assert np.array_equal(added, np.setdiff1d(got, prev))
+        assert np.array_equal(removed, np.setdiff1d(prev, got))
+        prev = got
+    ctx.close()
+
+
 def _world_pair(sc, iterations=10, **kw):
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
     ctx = jbgpu.Context(num_iterations=iterations, **kw)

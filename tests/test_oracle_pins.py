@@ -345,6 +345,38 @@ def test_pair_set_equals_brute_force(mode):
     assert np.array_equal(got, bp.step(brute=True))
 
 
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_destroy_and_append_proxies(mode):
+    """destroyProxy removes a proxy from every pair; later createProxy calls append (uids of the remaining
+    proxies unchanged): the set equals brute force over the live proxies."""
+    rng = np.random.default_rng(11)
+    boxes = _boxes(rng, 900)
+    bp = po.Broadphase(mode) if mode != 2 else po.Broadphase(2, (-50.0,) * 3, (150.0,) * 3)
+    bp.create(boxes)
+    bp.set_all(boxes)
+    bp.step()
+    dead = rng.choice(900, 120, replace=False) + 1
+    bp.destroy(dead)
+    more = _boxes(rng, 200)
+    bp.create(more)
+    allb = np.concatenate([boxes, more])
+    bp.set_all(allb)
+    got = bp.step()
+    lo = (got >> np.uint64(32)).astype(np.int64)
+    hi = (got & np.uint64(0xffffffff)).astype(np.int64)
+    assert not np.isin(lo, dead).any() and not np.isin(hi, dead).any()
+    if mode == 0:
+        allb = bp.proxies()[:, 7:13]
+    if mode != 2:
+        want = _brute_pairs(allb)
+        wl = (want >> np.uint64(32)).astype(np.int64)
+        wh = (want & np.uint64(0xffffffff)).astype(np.int64)
+        want = want[~(np.isin(wl, dead) | np.isin(wh, dead))]
+        assert np.array_equal(got, want)
+    assert np.array_equal(got, bp.step(brute=True))
+    assert hi.max() > 900  # appended proxies take uids after the first batch
+
+
 def test_pair_filter_groups():
     rng = np.random.default_rng(9)
     boxes = _boxes(rng, 400, extent=5.0)
