@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-kind", default="friction", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-blocks", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-tail", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-warmup", type=int, default=300, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -98,9 +100,10 @@ def main():
         blocks, tail = (int(v) for v in args.sweep_shape.split(","))
         ctx.solver_set_blocks(blocks)
         ctx.solver_set_tail(tail)
-        tune = {"block_size": blocks, "tail_threshold": tail, "ms_per_step": None}
+        tune = {"block_size": blocks, "tail_threshold": tail, "ms_per_step": None, "steps": 0}
     else:
         tune = autotune(ctx, dt)
+    first_timed = args.warmup + tune["steps"] + 1  # the pile's step number of the first timed step
     # timed region: only the sampled per-launch sweep events (the roofline's kernel durations) are recorded
     ctx.timing_kernels(args.kernel_every)
     t0 = time.perf_counter()
@@ -141,7 +144,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"mixed_pile: {args.bodies} convex bodies (spheres/boxes/capsules/hulls) in a walled "
                                f"100x100 m bin, {args.iterations} solver iterations, dt 1/60, steps "
-                               f"{args.warmup + 1}-{args.warmup + args.steps} of the drop",
+                               f"{first_timed}-{first_timed + args.steps - 1} of the drop",
                    "bodies": args.bodies, "solver_iterations": args.iterations, "broadphase": "dbvt",
                    "solver_order": "colour batches"},
         "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in tim.items()},
@@ -151,11 +154,13 @@ def main():
         "autotune": tune,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, shapes, verts, bodies, args)
+        out["cpu_baseline"] = cpu_baseline(ctx, shapes, verts, bodies, args,
+                                           first_timed - 1 + args.steps + (args.phase_steps if args.phase_steps > 0 else 0))
     ctx.close()
     if roof is not None and not args.no_traffic:
         try:
-            roof["traffic"], roof["traffic_detail"] = traffic(args, roof["kind"], tune["block_size"])
+            roof["traffic"], roof["traffic_detail"] = traffic(args, roof["kind"], tune["block_size"],
+                                                              tune["tail_threshold"], first_timed - 1)
         except Exception as e:
             roof["traffic"], roof["traffic_detail"] = None, f"counter passes failed: {e!r}"
     for key, flag, fn in (("batched", args.no_batched, batched), ("learned", args.no_learned, learned_warm_start),
@@ -170,13 +175,13 @@ def main():
     print(json.dumps(out))
 
 
-PMC_WARMUP, PMC_STEPS, PMC_COPY_BYTES = 300, 2, 64 << 20
+PMC_STEPS, PMC_COPY_BYTES = 10, 64 << 20
 
 
 def pmc_child(args):
-    """Workload of one counter pass: the same pile for PMC_WARMUP + PMC_STEPS steps (every friction sweep launch
-    is counted, and its rows are tallied for the algorithmic bytes), then one copy of PMC_COPY_BYTES as the
-    counter's scale reference."""
+    """Workload of one counter pass: the same pile up to the bench's first timed step (--pmc-warmup steps), then
+    PMC_STEPS steps whose sweep launches are tallied (launch count and rows, for the algorithmic bytes), then one
+    copy of PMC_COPY_BYTES as the counter's scale reference."""
     import jbgpu
     from jbgpu import scenes
     sc = scenes.mixed_pile(seed=args.seed, n=args.bodies)
@@ -184,10 +189,12 @@ def pmc_child(args):
     ctx = jbgpu.Context(num_iterations=args.iterations)
     ctx.world_create(shapes, verts, bodies)
     ctx.solver_set_blocks(args.pmc_blocks)
-    ctx.solver_set_tail(0)
-    ctx.timing(True)
+    ctx.solver_set_tail(args.pmc_tail)
+    # every sweep launch is counted; the launches (and rows) of the last PMC_STEPS steps are tallied here and
+    # the parent keeps that many of the last counter records
+    ctx.step(n=args.pmc_warmup)
     ctx.timing_kernels(1)
-    ctx.step(n=PMC_WARMUP + PMC_STEPS)
+    ctx.step(n=PMC_STEPS)
     ks = ctx.timing_kernels_get()[args.pmc_kind]
     ctx.measure_copy(PMC_COPY_BYTES, 1)
     ctx.close()
@@ -199,7 +206,7 @@ KERNEL_MATCH = {"friction": (r"kFriction\(", "%kFriction(%"), "normal": (r"kNorm
                 "normal_blocks": (r"kSweepBlocks<false>", "%kSweepBlocks<false>%")}
 
 
-def traffic(args, kind, blocks):
+def traffic(args, kind, blocks, tail, warmup):
     """HBM bytes per launch of the dominant sweep kernel from two rocprofv3 counter passes (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch), each scaled by the ratio of known bytes to what the same counter reads for
     the plain copy probe (kCopyProbe, PMC_COPY_BYTES read and written) in the same pass."""
@@ -217,7 +224,7 @@ def traffic(args, kind, blocks):
         cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "--kernel-include-regex", regex + "|kCopyProbe",
                "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--bodies",
                str(args.bodies), "--iterations", str(args.iterations), "--seed", str(args.seed), "--pmc-kind", kind,
-               "--pmc-blocks", str(blocks)]
+               "--pmc-blocks", str(blocks), "--pmc-tail", str(tail), "--pmc-warmup", str(warmup)]
         t = time.time()
         p = subprocess.run(cmd, capture_output=True, text=True)
         log(f"[bench] counter pass {counter}: rc {p.returncode} in {time.time() - t:.1f}s")
@@ -227,10 +234,15 @@ def traffic(args, kind, blocks):
             return None, f"counter pass {counter} failed (rc {p.returncode})"
         child = json.loads(lines[-1])
         cur = sqlite3.connect(dbs[0]).cursor()
-        q = "select avg(value), count(*) from counters_collection where counter_name = ? and kernel_name like ?"
-        kf, nkf = cur.execute(q, (counter, like)).fetchone()
-        cp, ncp = cur.execute(q, (counter, "%kCopyProbe%")).fetchone()
-        if not nkf or not ncp or not cp:
+        cols = [r[1] for r in cur.execute("pragma table_info(counters_collection)")]
+        order = "dispatch_id" if "dispatch_id" in cols else "rowid"
+        # the sweep launches of the counted steps are the last `launches` dispatches of the kernel
+        q = (f"select avg(value), count(*) from (select value from counters_collection where counter_name = ? and "
+             f"kernel_name like ? order by {order} desc limit ?)")
+        kf, nkf = cur.execute(q, (counter, like, int(child["launches"]))).fetchone()
+        cp, ncp = cur.execute(q, (counter, "%kCopyProbe%", 1)).fetchone()
+        log(f"[bench] counter pass {counter}: ordered by {order}, {nkf} sweep dispatches of the counted steps")
+        if not nkf or not ncp or not cp or nkf != child["launches"]:
             return None, f"counter pass {counter}: no dispatches recorded"
         scale = PMC_COPY_BYTES / (cp * 1024.0)
         res[counter] = dict(kib_per_dispatch=kf, dispatches=nkf, copy_kib=cp, scale=scale, child=child)
@@ -239,7 +251,9 @@ def traffic(args, kind, blocks):
     write = res["WRITE_SIZE"]["kib_per_dispatch"] * 1024.0 * res["WRITE_SIZE"]["scale"]
     child = res["FETCH_SIZE"]["child"]
     algo = ROW_BYTES * child["rows"] / max(child["launches"], 1)
-    detail = {"workload": f"same pile, block size {blocks}, every {regex} launch of steps 1-{PMC_WARMUP + PMC_STEPS}",
+    detail = {"workload": f"same pile, block size {blocks}, tail threshold {tail}, every {regex} launch of steps "
+                          f"{warmup + 1}-{warmup + PMC_STEPS} (the first timed steps; the last dispatches of the "
+                          f"counter pass)",
               "fetch_bytes": int(fetch),
               "write_bytes": int(write), "algorithmic_bytes": int(algo), "traffic_over_algorithmic":
               round((fetch + write) / max(algo, 1.0), 3), "scale_fetch": round(res["FETCH_SIZE"]["scale"], 4),
@@ -265,7 +279,8 @@ def autotune(ctx, dt, steps=8):
     ctx.solver_set_tail(best[1])
     log(f"[bench] autotune (block size, tail threshold): {ms} -> {best}")
     return {"block_size": best[0], "tail_threshold": best[1],
-            "ms_per_step": {f"blocks={b},tail={t}": round(v, 3) for (b, t), v in ms.items()}}
+            "ms_per_step": {f"blocks={b},tail={t}": round(v, 3) for (b, t), v in ms.items()},
+            "steps": len(cands) * (steps + 1)}
 
 
 def roofline(ks, args):
@@ -381,7 +396,7 @@ def learned_warm_start(args):
     return ev
 
 
-def cpu_baseline(ctx, shapes, verts, bodies, args):
+def cpu_baseline(ctx, shapes, verts, bodies, args, after_step):
     """The oracle (C++ restatement of the Java path, 1 core) stepping the same pile from the GPU's current state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -401,7 +416,7 @@ def cpu_baseline(ctx, shapes, verts, bodies, args):
     v = args.bodies * n / secs
     return {"value": round(v, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
             "sample": f"{n} steps of the same {args.bodies}-body pile continued from the GPU state after step "
-                      f"{args.warmup + args.steps}, single-threaded C++ restatement of the Java path "
+                      f"{after_step}, single-threaded C++ restatement of the Java path "
                       f"(reference island order, Random(0) shuffle), host {os.cpu_count()} cores, 1 used"}
 
 

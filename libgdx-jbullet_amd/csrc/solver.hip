@@ -241,19 +241,31 @@ __device__ __forceinline__ void applyImpulse(SBody& b, V3 lc, V3 ac, float mag) 
     b.av.z += mag * 1.0f * ac.z;
   }
 }
+// solver body record svel[i][8]: linear velocity, inverse mass, angular velocity, 0 (one 32-byte read per body)
 __device__ __forceinline__ SBody loadBody(const float* __restrict__ v, const float* __restrict__ props, int i) {
+  (void)props;
   const float4* p = (const float4*)(v + 8 * i);
   float4 a = p[0], b = p[1];
   SBody s;
   s.lv = v3(a.x, a.y, a.z);
   s.av = v3(b.x, b.y, b.z);
-  s.invMass = props[8 * i];
+  s.invMass = a.w;
   return s;
 }
 __device__ __forceinline__ void storeBody(float* v, int i, const SBody& s) {
   float4* p = (float4*)(v + 8 * i);
-  p[0] = make_float4(s.lv.x, s.lv.y, s.lv.z, 0.0f);
+  p[0] = make_float4(s.lv.x, s.lv.y, s.lv.z, s.invMass);
   p[1] = make_float4(s.av.x, s.av.y, s.av.z, 0.0f);
+}
+// the solver body records of a solve: velocities of the step with each body's inverse mass beside them
+__global__ void kSolverBodies(int n, const float* __restrict__ vel, const float* __restrict__ props, float* svel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* s = (const float4*)(vel + 8 * i);
+  float4 a = s[0], b = s[1];
+  a.w = props[8 * i];
+  ((float4*)(svel + 8 * i))[0] = a;
+  ((float4*)(svel + 8 * i))[1] = b;
 }
 
 __device__ void frictionRow(float* R, V3 axis, V3 rp1, V3 rp2, const M3& I0, const M3& I1, float im0, float im1,
@@ -881,8 +893,7 @@ __global__ __launch_bounds__(kBlockThreads) void kSweepBlocks(int nBatches, int 
   for (int q = q0 + t; q < q1; q += kBlockThreads) {
     const int body = sorted[q];
     const float4* v = (const float4*)(svel + 8 * body);
-    float4 lv = v[0], av = v[1];
-    lv.w = props[8 * body];
+    float4 lv = v[0], av = v[1];  // lv.w: inverse mass
     ((float4*)lds)[2 * (q - q0)] = lv;
     ((float4*)lds)[2 * (q - q0) + 1] = av;
   }
@@ -909,7 +920,7 @@ __global__ __launch_bounds__(kBlockThreads) void kSweepBlocks(int nBatches, int 
     if (lv.w == 0.0f) continue;  // static bodies keep their (unchanged) velocities
     const float4 av = ((const float4*)lds)[2 * (q - q0) + 1];
     float4* v = (float4*)(svel + 8 * body);
-    v[0] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+    v[0] = lv;
     v[1] = make_float4(av.x, av.y, av.z, 0.0f);
   }
   accumulate(dd, nn, res);
@@ -974,7 +985,9 @@ __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const
   if (props[8 * i] != 0.0f) {
     const float4* s = (const float4*)(src + 8 * i);
     float4* d = (float4*)(dst + 8 * i);
-    d[0] = s[0];
+    float4 a = s[0];
+    a.w = 0.0f;  // the solver record's inverse mass stays out of the velocity array
+    d[0] = a;
     d[1] = s[1];
   }
 }
@@ -1210,7 +1223,8 @@ void solverSolve(Ctx& c, float dt) {
   c.nrow.ensure((size_t)NW * na);
   c.frow.ensure((size_t)FW * na);
   c.svel.ensure((size_t)8 * c.nb);
-  JB_CHECK(hipMemcpyAsync(c.svel.p, c.vel.p, (size_t)8 * c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(kSolverBodies, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, (const float*)c.vel.p,
+                     (const float*)c.props.p, c.svel.p);
   const bool learnedHere = c.useMlp;
   SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.useMlp) ? 1 : 0,
                   (c.dumpFeatures || learnedHere) ? 1 : 0, c.tolerance};
