@@ -265,7 +265,7 @@ def autotune(ctx, dt, steps=8):
     """Launch shaping of the sweeps (results are bit-identical for every choice): the block size of the
     block-partitioned sweeps (0: one launch per colour) and, without blocks, the largest colour swept by the
     single-workgroup tail launch.  Each candidate runs a few untimed warmup steps."""
-    cands = ((2048, 0), (1024, 0), (512, 0), (0, 0), (0, 256))
+    cands = ((2048, 0), (512, 0), (0, 0), (0, 256), (0, 512), (0, 1024))
     ms = {}
     for blocks, tail in cands:
         ctx.solver_set_blocks(blocks)

@@ -591,30 +591,60 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
   }
 }
 
-// every proxy against the huge proxies of its world (huge list sorted by world, then index)
-__global__ void kHuge(int n, int nh, const int* __restrict__ hugeList, const int* __restrict__ hugeWorld,
-                      const int* __restrict__ isHuge, const float* __restrict__ cand, PairTest T, int bits,
-                      unsigned long long* counter, uint64_t* out, int64_t cap) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int w = T.world[j];
-  int lo = 0, hi = nh;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (hugeWorld[mid] < w) lo = mid + 1; else hi = mid;
+// per world w: range[2w], range[2w + 1] = its part of the huge list (sorted by world, then index); the
+// table is cleared to empty ranges beforehand
+__global__ void kHugeRanges(int nh, const int* __restrict__ hugeWorld, int* range) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nh) return;
+  const int w = hugeWorld[k];
+  if (k == 0 || hugeWorld[k - 1] != w) range[2 * w] = k;
+  if (k == nh - 1 || hugeWorld[k + 1] != w) range[2 * w + 1] = k + 1;
+}
+
+// every proxy against the huge proxies of its world; each thread counts its pairs, the block reserves room for
+// all of them with one global atomic (one per pair on a single counter would serialise), then writes them
+constexpr int kHugeBlock = 256;
+__global__ __launch_bounds__(kHugeBlock) void kHuge(int n, const int* __restrict__ hugeRange,
+                                                    const int* __restrict__ hugeList, const int* __restrict__ isHuge,
+                                                    const float* __restrict__ cand, PairTest T, int bits,
+                                                    unsigned long long* counter, uint64_t* out, int64_t cap) {
+  __shared__ unsigned int sCount;
+  __shared__ unsigned long long sBase;
+  if (threadIdx.x == 0) sCount = 0;
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int lo = 0, hi = 0;
+  float4 bmn, bmx;
+  bool jh = false;
+  if (j < n) {
+    const int w = T.world[j];
+    lo = hugeRange[2 * w];
+    hi = hugeRange[2 * w + 1];
+    bmn = ((const float4*)cand)[2 * j];
+    bmx = ((const float4*)cand)[2 * j + 1];
+    jh = isHuge[j] != 0;
   }
-  const float4 bmn = ((const float4*)cand)[2 * j];
-  const float4 bmx = ((const float4*)cand)[2 * j + 1];
-  bool jh = isHuge[j] != 0;
-  for (int k = lo; k < nh && hugeWorld[k] == w; k++) {
-    int h = hugeList[k];
-    if (h == j) continue;
-    if (jh && h > j) continue;
+  auto hit = [&](int h) {
+    if (h == j || (jh && h > j)) return false;
     const float4 amn = ((const float4*)cand)[2 * h];
     const float4 amx = ((const float4*)cand)[2 * h + 1];
-    if (!boxOverlap(amn, amx, bmn, bmx)) continue;
-    if (exactOverlap(T, h, j)) emit(h, j, bits, counter, out, cap);
-  }
+    return boxOverlap(amn, amx, bmn, bmx) && exactOverlap(T, h, j);
+  };
+  unsigned int mine = 0;
+  for (int k = lo; k < hi; k++) mine += hit(hugeList[k]) ? 1u : 0u;
+  const unsigned int off = mine ? atomicAdd(&sCount, mine) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) sBase = sCount ? atomicAdd(counter, (unsigned long long)sCount) : 0ull;
+  __syncthreads();
+  unsigned long long at = sBase + off;
+  if (mine)
+    for (int k = lo; k < hi; k++) {
+      const int h = hugeList[k];
+      if (!hit(h)) continue;
+      const uint64_t a = (uint64_t)min(h, j) + 1, b = (uint64_t)max(h, j) + 1;
+      if ((int64_t)at < cap) out[at] = (a << bits) | b;
+      at++;
+    }
 }
 
 // boxes far larger than the typical one (static walls, terrain slabs) leave the tree for the flat per-world
@@ -653,8 +683,7 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   uint64_t p = packed[i];
-  uint64_t mask = (1ull << bits) - 1ull;
-  uint64_t key = ((p >> bits) << 32) | (p & mask);
+  uint64_t key = ((p >> bits) << 32) | (p & ((1ull << bits) - 1ull));
   keysNew[i] = key;
   int a = (int)(key >> 32) - 1, b = (int)(key & 0xffffffffu) - 1;
   int ta = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
@@ -670,9 +699,10 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
     int64_t mid = (lo + hi) >> 1;
     if (keysOld[mid] < key) lo = mid + 1; else hi = mid;
   }
+  const bool found = lo < nOld && keysOld[lo] == key;
   // only the header and the live points travel (slots past the point count are never read)
   float4* dst = (float4*)(manNew + (size_t)JBGPU_MANIFOLD_WORDS * i);
-  if (lo < nOld && keysOld[lo] == key) {
+  if (found) {
     const float4* src = (const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * lo);
     float4 h = src[0];
     dst[0] = h;
@@ -841,6 +871,12 @@ void bpFindPairs(Ctx& c) {
         JB_CHECK(hipMemcpyAsync(c.hugeList.p, hv.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
     }
   }
+  if (c.nHuge > 0) {
+    c.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
+    JB_CHECK(hipMemsetAsync(c.hugeRange.p, 0, 2 * ((size_t)c.maxWorld + 1) * sizeof(int), s));
+    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)c.hugeWorld.p,
+                       c.hugeRange.p);
+  }
   if (nn >= 2) {
     hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.world.p,
                        c.leafBox.p, c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p,
@@ -876,8 +912,9 @@ void bpFindPairs(Ctx& c) {
                          c.pairCap, (int*)(c.counter.p + 2), c.pairStage.p);
     if (attempt == 0) tEnd(c, T_TRAVERSE);
     if (c.nHuge > 0)
-      hipLaunchKernelGGL(kHuge, dim3(gridFor(n)), dim3(256), 0, s, n, c.nHuge, c.hugeList.p, c.hugeWorld.p,
-                         c.isHuge.p, c.cand.p, T, bits, c.counter.p, c.rawKeys.p, c.pairCap);
+      hipLaunchKernelGGL(kHuge, dim3(gridFor(n, kHugeBlock)), dim3(kHugeBlock), 0, s, n, (const int*)c.hugeRange.p,
+                         (const int*)c.hugeList.p, c.isHuge.p, c.cand.p, T, bits, c.counter.p, c.rawKeys.p,
+                         c.pairCap);
     unsigned long long cnt[3] = {0, 0, 0};
     JB_CHECK(hipMemcpyAsync(cnt, c.counter.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     JB_CHECK(hipStreamSynchronize(s));

@@ -128,6 +128,7 @@ struct Ctx {
   DBuf<float> leafBox;   // sorted leaves np*8
   DBuf<float> nodeBox;   // internal nodes np*8
   DBuf<float> nodeRec;   // traversal records np*20: both children's boxes, indices and last leaves
+  DBuf<int> hugeRange;   // per world: its range of the huge list
   DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
