@@ -148,6 +148,7 @@ def main():
                    "bodies": args.bodies, "solver_iterations": args.iterations, "broadphase": "dbvt",
                    "solver_order": "colour batches"},
         "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in tim.items()},
+        "phase_roofline": phase_roofline(tim, nsteps, ctx.stats(), args),
         "phase_note": f"event-instrumented pass of {nsteps} steps after the timed region",
         "last_step": {k: st[k] for k in ("pairs", "contacts", "manifolds", "colors", "epa_calls", "epa_fails")},
         "roofline": roof,
@@ -259,6 +260,35 @@ def traffic(args, kind, blocks, tail, warmup):
               round((fetch + write) / max(algo, 1.0), 3), "scale_fetch": round(res["FETCH_SIZE"]["scale"], 4),
               "scale_write": round(res["WRITE_SIZE"]["scale"], 4), "dispatches": res["FETCH_SIZE"]["dispatches"]}
     return int(fetch + write), detail
+
+
+def phase_roofline(tim, nsteps, st, args):
+    """Fraction of the 8 TB/s HBM roofline per phase: compulsory bytes per unit of work (predict 208 B/body, AABB
+    88 B/body, broadphase 128 B/proxy + 8 B/pair, narrowphase 1056 B/pair, solver setup 552 B/contact, solver
+    180 B per row per sweep, integrate 156 B/body) x this step's counts, over the phase's event-timed duration."""
+    if not tim or nsteps <= 0:
+        return None
+    ms = lambda *names: sum(tim[n][0] for n in names if n in tim) / nsteps
+    nb, pairs, contacts, rows = args.bodies + 5, st["pairs"], st["contacts"], st["rows"]
+    sweeps = args.iterations
+    phases = {
+        "predict": (208 * args.bodies, ms("predict")),
+        "aabb": (88 * nb, ms("setaabb")),
+        "broadphase": (128 * nb + 8 * pairs, ms("bvh", "traverse", "keysort", "remap")),
+        "narrowphase": (1056 * pairs, ms("bin", "gjk", "epa", "manifold")),
+        "solver_setup": (552 * contacts, ms("color", "solver_setup")),
+        "solver_iterations": (180 * 3 * rows * sweeps, ms("solver_iter")),
+        "integrate": (156 * args.bodies, ms("integrate")),
+    }
+    out = {}
+    for k, (b, t) in phases.items():
+        if t <= 0:
+            continue
+        gbs = b / (t * 1e-3) / 1e9
+        out[k] = {"bytes": int(b), "ms": round(t, 4), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["units"] = {"bodies": args.bodies, "pairs": pairs, "contacts": contacts, "normal_rows": rows,
+                    "rows_per_sweep_pair": 3 * rows, "iterations": sweeps}
+    return out
 
 
 def autotune(ctx, dt, steps=8):
@@ -388,12 +418,46 @@ def learned_warm_start(args):
     log(f"[bench] learned: {len(x)} training contacts ({t_collect:.1f}s), loss {losses[0]:.4f} -> {losses[-1]:.4f} "
         f"({t_train:.1f}s)")
     ev = learned.evaluate(params, norm, seed=1, scene_kw=kw, settle=60, steps=30)
+    ev["mfma"] = mlp_throughput(params, norm, ev, len(x) * len(losses), t_train)
     ev.update({"training_contacts": int(len(x)), "epochs": len(losses), "loss_first": round(float(losses[0]), 5),
                "loss_last": round(float(losses[-1]), 5), "train_s": round(t_train, 2),
                "scene": "pyramids: 8000 boxes in 4 square pyramids + 4 walls, 250 spheres; eval seed 1 after 60 "
                         "steps, 30 steps of 100 iterations each; training seeds 1000-1005",
                "residual": "r_k = ||l_k - l_(k-1)|| / max(1, ||l_k||) over normal and friction rows"})
     return ev
+
+
+MLP_FWD_FLOP = 2 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # 18,816 per contact
+BF16_DENSE_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+
+
+def mlp_throughput(params, norm, ev, train_samples, train_s):
+    """MFMA use of the 16-64-64-64-3 bf16 MLP: inference inside the solve (4 slots per active manifold, kernel
+    time from the solve's timing region), a 2M-contact inference batch (kernel time only), and training
+    (forward + backward = 3 x forward flops per sample, wall time of jbgpu_mlp_train)."""
+    import jbgpu
+    ctx = jbgpu.Context()
+    ctx.mlp_set_params(params, norm)
+    x = np.random.default_rng(0).normal(size=(1 << 21, 16)).astype(np.float32)
+    ctx.mlp_infer(x[:4096])
+    ctx.timing(True)
+    reps = 3
+    for _ in range(reps):
+        ctx.mlp_infer(x)
+    reg, _ = ctx.timing_get()
+    ctx.close()
+    big_ms = reg["mlp"][0] / reps
+    big = MLP_FWD_FLOP * len(x) / (big_ms * 1e-3) / 1e12
+    le = ev["learned"]
+    slots = 4 * le["manifolds"]
+    solve = MLP_FWD_FLOP * slots / (le["mlp_ms"] * 1e-3) / 1e12 if le["mlp_ms"] > 0 else None
+    train = 3 * MLP_FWD_FLOP * train_samples / train_s / 1e12
+    return {"peak_tflops": BF16_DENSE_PEAK_TFLOPS, "flop_per_contact_forward": MLP_FWD_FLOP,
+            "infer_2m_contacts_ms": round(big_ms, 4), "infer_2m_tflops": round(big, 2),
+            "infer_2m_frac": round(big / BF16_DENSE_PEAK_TFLOPS, 5),
+            "in_solve_slots": slots, "in_solve_ms": le["mlp_ms"],
+            "in_solve_tflops": None if solve is None else round(solve, 3),
+            "train_tflops_wall": round(train, 3), "train_frac": round(train / BF16_DENSE_PEAK_TFLOPS, 6)}
 
 
 def cpu_baseline(ctx, shapes, verts, bodies, args, after_step):

@@ -88,9 +88,11 @@ def evaluate(params, norm, seed, scene_kw, settle=90, steps=30, cap=100, modes=(
         reg, nsteps = ctx.timing_get()
         it_ms = reg["solver_iter"][0] / max(nsteps, 1) / cap
         setup_ms = (reg["solver_setup"][0] + reg["mlp"][0]) / max(nsteps, 1)
+        st = ctx.stats()
         out[mode] = dict(iterations_to={f"{t:g}": float(np.mean(v)) for t, v in to.items()},
                          residual_after_20=float(np.mean(r20)), ms_per_iteration=round(it_ms, 4),
-                         setup_ms=round(setup_ms, 4), contacts=int(ctx.stats()["contacts"]))
+                         setup_ms=round(setup_ms, 4), mlp_ms=round(reg["mlp"][0] / max(nsteps, 1), 4),
+                         contacts=int(st["contacts"]), manifolds=int(st["manifolds"]))
         ctx.close()
     pl, le = out["plain"]["iterations_to"], out["learned"]["iterations_to"]
     out["iteration_ratio"] = {t: le[t] / max(pl[t], 1e-9) for t in pl}
