@@ -363,6 +363,18 @@ void orc_world_set_solver(void* h, int order, float tol, int maxIt, int numIt) {
   w->cfg.solver.numIterations = numIt;
 }
 // world id per body (batched independent worlds; bodies of different worlds never pair)
+// rayTest with a ClosestRayResultCallback per ray: object index (-1: none), fraction, normal, point
+void orc_world_ray_test(void* h, int n, const float* from3, const float* to3, int16_t group, int16_t mask,
+                        const int32_t* world, int32_t* obj, float* frac, float* normal3, float* point3) {
+  World* w = (World*)h;
+  for (int i = 0; i < n; i++) {
+    RayHit r = rayTestClosest(*w, ld3(from3 + 3 * i), ld3(to3 + 3 * i), group, mask, world ? world[i] : 0);
+    obj[i] = r.object;
+    frac[i] = r.fraction;
+    st3(normal3 + 3 * i, r.normal);
+    st3(point3 + 3 * i, r.point);
+  }
+}
 void orc_world_set_worlds(void* h, const int32_t* world) {
   World* w = (World*)h;
   for (size_t i = 0; i < w->bp.proxies.size(); i++) w->bp.proxies[i].world = world[i];

@@ -1029,4 +1029,225 @@ void refreshContactPoints(Manifold& m, const Body* bodies) {  // ManifoldResult.
   manifoldRefresh(m, bodies[m.body0].tr, bodies[m.body1].tr);
 }
 
+
+// ============================ rayTest ======================================
+// CollisionWorld.rayTest (CollisionWorld.java:552-591) with rayTestSingle (:260-336) for convex shapes
+// (SubsimplexConvexCast) and StaticPlaneShape (concave: processAllTriangles + TriangleRaycastCallback), reported
+// to a ClosestRayResultCallback (:697-729).
+static int outcode(V3 p, V3 h) {  // AabbUtil2.java:40-43
+  return (p.x < -h.x ? 0x01 : 0x0) | (p.x > h.x ? 0x08 : 0x0) | (p.y < -h.y ? 0x02 : 0x0) | (p.y > h.y ? 0x10 : 0x0) |
+         (p.z < -h.z ? 0x4 : 0x0) | (p.z > h.z ? 0x20 : 0x0);
+}
+
+bool rayAabb(V3 from, V3 to, V3 mn, V3 mx, float& param, V3& normal) {  // AabbUtil2.java:45-104
+  V3 he = scl(sub(mx, mn), 0.5f);
+  V3 c = scl(add(mx, mn), 0.5f);
+  V3 source = sub(from, c), target = sub(to, c);
+  int so = outcode(source, he), tc = outcode(target, he);
+  if ((so & tc) == 0x0) {
+    float enter = 0.0f, exit = param;
+    V3 r = sub(target, source);
+    float normSign = 1.0f;
+    V3 hn = v3(0, 0, 0);
+    int bit = 1;
+    for (int j = 0; j < 2; j++) {
+      for (int i = 0; i != 3; ++i) {
+        if ((so & bit) != 0) {
+          float l = (-coord(source, i) - coord(he, i) * normSign) / coord(r, i);
+          if (enter <= l) {
+            enter = l;
+            hn = v3(0, 0, 0);
+            setcoord(hn, i, normSign);
+          }
+        } else if ((tc & bit) != 0) {
+          float l = (-coord(source, i) - coord(he, i) * normSign) / coord(r, i);
+          exit = jmin(exit, l);
+        }
+        bit <<= 1;
+      }
+      normSign = -1.0f;
+    }
+    if (enter <= exit) {
+      param = enter;
+      normal = hn;
+      return true;
+    }
+  }
+  return false;
+}
+
+// SubsimplexConvexCast.calcTimeOfImpact (SubsimplexConvexCast.java:61-196), shape A the ray's point
+// (SphereShape(0), margin 0: its support is the origin of its transform)
+bool subsimplexRayCast(const Shape& sb, const V3* hv, const Tr& fromA, const Tr& toA, const Tr& fromB, const Tr& toB,
+                       float& fraction, V3& normal) {
+  Voronoi ss;
+  ss.reset();
+  V3 linVelA = sub(toA.o, fromA.o), linVelB = sub(toB.o, fromB.o);
+  float lambda = 0.0f;
+  Tr iA = fromA, iB = fromB;
+  V3 r = sub(linVelA, linVelB);
+  V3 supA = xform(fromA, v3(0, 0, 0));
+  V3 supB = xform(fromB, supportMargin(sb, hv, mulMtV(fromB.b, r)));
+  V3 v = sub(supA, supB);
+  int maxIter = 32;
+  V3 n = v3(0, 0, 0);
+  float dist2 = len2(v);
+  const float epsilon = 0.0001f;
+  while ((dist2 > epsilon) && (maxIter--) != 0) {
+    supA = xform(iA, v3(0, 0, 0));
+    supB = xform(iB, supportMargin(sb, hv, mulMtV(iB.b, v)));
+    V3 w = sub(supA, supB);
+    float VdotW = dot(v, w);
+    if (lambda > 1.0f) return false;
+    if (VdotW > 0.0f) {
+      float VdotR = dot(v, r);
+      if (VdotR >= -(FLT_EPS * FLT_EPS)) return false;
+      lambda = lambda - VdotW / VdotR;
+      const float s = 1.0f - lambda;  // VectorUtil.setInterpolate3
+      iA.o = v3(s * fromA.o.x + lambda * toA.o.x, s * fromA.o.y + lambda * toA.o.y, s * fromA.o.z + lambda * toA.o.z);
+      iB.o = v3(s * fromB.o.x + lambda * toB.o.x, s * fromB.o.y + lambda * toB.o.y, s * fromB.o.z + lambda * toB.o.z);
+      w = sub(supA, supB);
+      n = v;
+    }
+    ss.addVertex(w, supA, supB);
+    if (ss.closest(v))
+      dist2 = len2(v);
+    else
+      dist2 = 0.0f;
+  }
+  fraction = lambda;
+  if (len2(n) >= FLT_EPS * FLT_EPS)
+    normal = nor(n);
+  else
+    normal = v3(0, 0, 0);
+  if (dot(normal, r) >= -0.0f) return false;  // CastResult.allowedPenetration = 0
+  return true;
+}
+
+static void planeSpace1(V3 n, V3& p, V3& q) {  // TransformUtil.java:45-61
+  if (jabs(n.z) > 0.7071067811865475244008443621048490f) {
+    float a = n.y * n.y + n.z * n.z;
+    float k = 1.0f / jsqrt(a);
+    p = v3(0, -n.z * k, n.y * k);
+    q = v3(a * k, -n.x * p.z, n.x * p.y);
+  } else {
+    float a = n.x * n.x + n.y * n.y;
+    float k = 1.0f / jsqrt(a);
+    p = v3(-n.y * k, n.x * k, 0);
+    q = v3(-n.z * p.y, n.z * p.x, a * k);
+  }
+}
+
+static V3 interp3(V3 a, V3 b, float t) {  // VectorUtil.setInterpolate3
+  float s = 1.0f - t;
+  return v3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
+}
+
+// TriangleRaycastCallback.processTriangle (TriangleRaycastCallback.java:51-124); returns true and the hit
+// (distance, unnormalised local normal) when it would call reportHit
+bool rayTriangle(const V3 tri[3], V3 from, V3 to, float hitFraction, float& distance, V3& normal) {
+  V3 v10 = sub(tri[1], tri[0]), v20 = sub(tri[2], tri[0]);
+  V3 tn = crs(v10, v20);
+  float dist = dot(tri[0], tn);
+  float da = dot(tn, from);
+  da -= dist;
+  float db = dot(tn, to);
+  db -= dist;
+  if (da * db >= 0.0f) return false;
+  float proj = da - db;
+  distance = da / proj;
+  if (!(distance < hitFraction)) return false;
+  float tol = len2(tn);
+  tol *= -0.0001f;
+  V3 point = interp3(from, to, distance);
+  V3 v0p = sub(tri[0], point), v1p = sub(tri[1], point);
+  if (!(dot(crs(v0p, v1p), tn) >= tol)) return false;
+  V3 v2p = sub(tri[2], point);
+  if (!(dot(crs(v1p, v2p), tn) >= tol)) return false;
+  if (!(dot(crs(v2p, v0p), tn) >= tol)) return false;
+  normal = da > 0.0f ? tn : scl(tn, -1.0f);
+  return true;
+}
+
+// StaticPlaneShape.processAllTriangles (StaticPlaneShape.java:60-121) for the ray's local box; the two
+// triangles in order.  The reference sets halfExtents from aabbMax and then from aabbMin, so the radius is
+// half the length of aabbMin (kept).
+void planeTriangles(const Shape& s, V3 aabbMin, V3 aabbMax, V3 tri[2][3]) {
+  V3 he = scl(aabbMin, 0.5f);
+  float radius = len(he);
+  V3 center = scl(add(aabbMax, aabbMin), 0.5f);
+  V3 t0, t1;
+  planeSpace1(s.pn, t0, t1);
+  V3 pc = sub(center, scl(s.pn, dot(s.pn, center) - s.pc));
+  V3 a = scl(t0, radius), b = scl(t1, radius);
+  tri[0][0] = v3(pc.x + a.x + b.x, pc.y + a.y + b.y, pc.z + a.z + b.z);
+  tri[0][1] = add(pc, sub(a, b));
+  tri[0][2] = sub(pc, sub(a, b));
+  tri[1][0] = sub(pc, sub(a, b));
+  tri[1][1] = sub(pc, add(a, b));
+  tri[1][2] = v3(pc.x + a.x + b.x, pc.y + a.y + b.y, pc.z + a.z + b.z);
+}
+
+RayHit rayTestClosest(const World& w, V3 from, V3 to, int16_t group, int16_t mask, int world) {
+  RayHit out;
+  out.object = -1;
+  out.fraction = 1.0f;  // RayResultCallback.closestHitFraction
+  out.normal = v3(0, 0, 0);
+  out.point = v3(0, 0, 0);
+  const V3* hv = w.hullv.empty() ? nullptr : w.hullv.data();
+  Tr rayFrom, rayTo;
+  rayFrom.b = m3ident();
+  rayFrom.o = from;
+  rayTo.b = m3ident();
+  rayTo.o = to;
+  for (size_t i = 0; i < w.bodies.size(); i++) {
+    if (out.fraction == 0.0f) break;
+    const Proxy& px = w.bp.proxies[i];
+    if (px.dead || px.world != world) continue;  // separate worlds: separate collision object lists
+    bool collides = (((int)px.group & (int)mask) & 0xFFFF) != 0;  // RayResultCallback.needsCollision
+    collides = collides && (((int)group & (int)px.mask) & 0xFFFF) != 0;
+    if (!collides) continue;
+    const Body& b = w.bodies[i];
+    const Shape& sh = w.shapes[b.shape];
+    V3 mn, mx;
+    shapeAabb(sh, b.tr, mn, mx);
+    float lambda = out.fraction;
+    V3 hn;
+    if (!rayAabb(from, to, mn, mx, lambda, hn)) continue;
+    if (sh.type != JBGPU_SHAPE_PLANE) {
+      float f;
+      V3 n;
+      if (subsimplexRayCast(sh, hv, rayFrom, rayTo, b.tr, b.tr, f, n)) {
+        if (len2(n) > 0.0001f && f < out.fraction) {
+          n = nor(mulMV(rayFrom.b, n));
+          out.fraction = f;
+          out.object = (int)i;
+          out.normal = n;
+          out.point = interp3(from, to, f);
+        }
+      }
+    } else {
+      Tr w2o = inverse(b.tr);
+      V3 fl = xform(w2o, from), tl = xform(w2o, to);
+      V3 rmin = v3(jmin(fl.x, tl.x), jmin(fl.y, tl.y), jmin(fl.z, tl.z));
+      V3 rmax = v3(jmax(fl.x, tl.x), jmax(fl.y, tl.y), jmax(fl.z, tl.z));
+      V3 tri[2][3];
+      planeTriangles(sh, rmin, rmax, tri);
+      float hitFraction = out.fraction;
+      for (int t = 0; t < 2; t++) {
+        float d;
+        V3 nl;
+        if (rayTriangle(tri[t], fl, tl, hitFraction, d, nl)) {
+          out.fraction = d;  // BridgeTriangleRaycastCallback.reportHit -> addSingleResult(normal in local space)
+          out.object = (int)i;
+          out.normal = mulMV(b.tr.b, nl);
+          out.point = interp3(from, to, d);
+          hitFraction = d;
+        }
+      }
+    }
+  }
+  return out;
+}
+
 }  // namespace jb

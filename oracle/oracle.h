@@ -207,6 +207,11 @@ struct PairState {
   uint64_t key;
   Manifold m;
 };
+struct RayHit {  // ClosestRayResultCallback: collisionObject, closestHitFraction, hitNormalWorld, hitPointWorld
+  int object;
+  float fraction;
+  V3 normal, point;
+};
 struct World {
   WorldConfig cfg;
   std::vector<Shape> shapes;
@@ -237,5 +242,11 @@ struct World {
   bool needsCollision(int a, int b) const;
   bool needsResponse(int a, int b) const;
 };
+
+// CollisionWorld.rayTest with a ClosestRayResultCallback over the bodies of `world`
+RayHit rayTestClosest(const World& w, V3 from, V3 to, int16_t group, int16_t mask, int world);
+bool rayAabb(V3 from, V3 to, V3 mn, V3 mx, float& param, V3& normal);
+bool subsimplexRayCast(const Shape& sb, const V3* hv, const Tr& fromA, const Tr& toA, const Tr& fromB, const Tr& toB,
+                       float& fraction, V3& normal);
 
 }  // namespace jb

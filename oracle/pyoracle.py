@@ -61,6 +61,7 @@ def lib():
             "orc_bp_set_all": (None, [vp, _f]),
             "orc_bp_set_worlds": (None, [vp, _i32]),
             "orc_bp_destroy": (None, [vp, C.c_int, _i32]),
+            "orc_world_ray_test": (None, [vp, C.c_int, _f, _f, C.c_int16, C.c_int16, vp, _i32, _f, _f, _f]),
             "orc_bp_step": (C.c_int64, [vp, _u64, C.c_int64, C.c_int]),
             "orc_bp_get_proxies": (None, [vp, _f]),
             "orc_bp_get_quantized": (None, [vp, _i32]),
@@ -266,6 +267,21 @@ class World:
     def set_worlds(self, world):
         """World id per body: bodies of different worlds never pair (batched independent worlds)."""
         lib().orc_world_set_worlds(self.h, np.ascontiguousarray(world, np.int32))
+
+    def ray_test(self, frm, to, group=1, mask=-1, world=None):
+        """CollisionWorld.rayTest with a ClosestRayResultCallback per ray: (object, fraction, normal, point)."""
+        frm = f32(frm).reshape(-1, 3)
+        to = f32(to).reshape(-1, 3)
+        n = len(frm)
+        obj = np.zeros(n, np.int32)
+        fr = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        pt = np.zeros((n, 3), np.float32)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        lib().orc_world_ray_test(self.h, n, frm.reshape(-1), to.reshape(-1), int(group), int(mask),
+                                 None if w is None else w.ctypes.data_as(C.c_void_p), obj, fr, nrm.reshape(-1),
+                                 pt.reshape(-1))
+        return obj, fr, nrm, pt
 
     def set_order(self, order):
         o = np.ascontiguousarray(order, np.int32)

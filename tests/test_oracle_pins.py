@@ -692,3 +692,62 @@ def test_islands_sleep_and_wake():
         still = (states[k - 1] == 2) & (states[k] == 2)
         assert np.array_equal(pos[k][still], pos[k - 1][still])
     assert (states[-1, boxes] == 2).all()                           # and the pile sleeps again
+
+
+# ------------------------------------------------------------------ rayTest (CollisionWorld.java:552-591)
+def _ray_world(spec):
+    b = scenes._Builder("rays")
+    for (stype, params, pos, mass) in spec:
+        s = b.shape(stype, params)
+        b.body(s, scenes.IDENT, np.asarray(pos, np.float32), mass)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    return po.World(shapes, verts, bodies, iterations=10)
+
+
+def test_ray_closed_forms():
+    """Sphere, box face and plane hits against their closed forms (the subsimplex cast stops within 0.01 of the
+    surface: |v|^2 <= 1e-4), a miss, and the nearer of two objects winning whatever their order."""
+    w = _ray_world([(PLANE, [0, 1, 0, -2], (0, 0, 0), 0.0),       # y = -2
+                    (SPHERE, [1.0], (0, 0, 0), 1.0),
+                    (BOX, [1.0, 1.0, 1.0], (5, 0, 0), 1.0),      # implicit 0.96 + margin 0.04: faces at 4, 6
+                    (SPHERE, [0.5], (0, 0, 30), 1.0),            # farther along the last ray, lower index
+                    (SPHERE, [0.5], (0, 0, 20), 1.0)])
+    frm = np.array([[0, 5, 0], [10, 0, 0], [20, 5, 3], [50, 50, 50], [0, 0, 10]], np.float32)
+    to = np.array([[0, -5, 0], [0, 0, 0], [20, -5, 3], [60, 60, 60], [0, 0, 40]], np.float32)
+    obj, fr, nrm, pt = w.ray_test(frm, to)
+    assert obj.tolist() == [1, 2, 0, -1, 4]
+    assert abs(fr[0] - 0.4) < 1e-3 and np.allclose(nrm[0], [0, 1, 0], atol=1e-3)
+    assert abs(fr[1] - (10 - 6.0) / 10) < 1e-3 and np.allclose(nrm[1], [1, 0, 0], atol=1e-3)
+    # plane: the triangle hit is exact up to rounding; the reported normal is the unnormalised triangle normal
+    assert abs(fr[2] - 0.7) < 1e-6 and np.allclose(pt[2], [20, -2, 3], atol=1e-4)
+    assert nrm[2][1] > 0 and np.allclose(nrm[2] / np.linalg.norm(nrm[2]), [0, 1, 0], atol=1e-6)
+    assert fr[3] == 1.0
+    assert abs(fr[4] - (19.5 - 10) / 30) < 1e-3
+    assert np.allclose(pt[4], [0, 0, 19.5], atol=0.02)
+
+
+def test_ray_ties_and_filters():
+    """Equal fractions keep the first object in world order (strict <); the callback's group/mask filter
+    (needsCollision) removes static bodies (group 2) when the mask excludes them."""
+    w = _ray_world([(SPHERE, [1.0], (0, 0, 0), 1.0), (SPHERE, [1.0], (0, 0, 0), 1.0), (BOX, [4, 0.1, 4], (0, -3, 0), 0.0)])
+    frm = np.array([[0, 5, 0], [3, 5, 0]], np.float32)
+    to = np.array([[0, -5, 0], [3, -5, 0]], np.float32)
+    obj, fr, _, _ = w.ray_test(frm, to)
+    assert obj.tolist() == [0, 2]
+    obj2, fr2, _, _ = w.ray_test(frm, to, group=1, mask=np.int16(-1 ^ 2))
+    assert obj2.tolist() == [0, -1] and fr2[0] == fr[0] and fr2[1] == 1.0
+
+
+def test_ray_plane_triangle_radius_quirk():
+    """StaticPlaneShape.processAllTriangles takes the radius from aabbMin only (halfExtents.set(max).set(min)):
+    a ray whose local box has its min corner at the origin yields zero-size triangles and misses the plane."""
+    w = _ray_world([(PLANE, [0, 1, 0, 0], (0, 0, 0), 0.0)])
+    obj, fr, _, _ = w.ray_test(np.array([[0, 1, 0]], np.float32), np.array([[0, -1, 0]], np.float32))
+    assert obj.tolist() == [0] and abs(fr[0] - 0.5) < 1e-6      # box min (0,-1,0): radius 0.5, hit
+    obj, fr, _, _ = w.ray_test(np.array([[0, 1, 0]], np.float32), np.array([[0, 0, 0]], np.float32))
+    assert obj.tolist() == [-1]                                  # ends on the plane: no sign change
+    # crossing at x = 9.09, triangles of radius |(0,-1,0)|/2 around the box centre x = 5: the reference misses
+    # the plane (with the box's half extents as radius it would hit)
+    obj, _, _, _ = w.ray_test(np.array([[0, 10, 0]], np.float32), np.array([[10, -1, 0]], np.float32))
+    assert obj.tolist() == [-1]
