@@ -120,6 +120,15 @@ int jbgpu_bp_get_quantized(jbgpu_ctx* ctx, int32_t* q6);
  * (static bodies: filter group 2, mask ~2) with the shapes' AABBs. */
 int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int32_t nverts, const float* verts,
                        int32_t nbodies, const float* bodies, const int32_t* world_ids);
+/* CollisionWorld.rayTest with a ClosestRayResultCallback for each of n rays (CollisionWorld.java:552-591,
+ * rayTestSingle :260-336, ClosestRayResultCallback :697-729) against the bodies' current transforms:
+ * convex shapes by SubsimplexConvexCast, planes by their two triangles.  group/mask are the callback's
+ * filter (reference defaults 1 and -1); world ids per ray may be NULL (world 0).  Per ray: obj = body
+ * index of the closest hit or -1, fraction (1 without a hit), normal = hitNormalWorld, point =
+ * hitPointWorld.  on_device != 0: every pointer is a device pointer (rays resident in HBM). */
+int jbgpu_world_ray_test(jbgpu_ctx* ctx, int32_t n, const float* from3, const float* to3, int16_t group,
+                         int16_t mask, const int32_t* world, int32_t* obj, float* fraction, float* normal3,
+                         float* point3, int32_t on_device);
 /* stepSimulation(dt, 0) nsteps times: exactly one substep per call. */
 int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps);
 /* One step run up to and including phase p: 0 predictUnconstraintMotion, 1 AABB update +

@@ -432,7 +432,6 @@ __global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__
 // traversal record of internal node i: both children's boxes (.w lanes: world range; a leaf's world in
 // both), then (child0, child1, last0, last1) where child < 0 is leaf -child-1 and last is the highest
 // sorted leaf below the child; one visit reads the five float4 at once
-constexpr int kRecF4 = 5;
 __global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict__ child, const int* __restrict__ last,
                                                     const float* __restrict__ leafBox,
                                                     const float* __restrict__ nodeBox, float4* __restrict__ rec) {
@@ -808,6 +807,127 @@ static int bitsFor(int64_t n) {
   int b = 1;
   while ((1ll << b) <= n) b++;
   return b;
+}
+
+// ---- query tree over the current shape AABBs (world queries) -----------------------------------------
+// CollisionObject AABBs as rayTest computes them: getCollisionShape().getAabb(getWorldTransform())
+__global__ void kRayAabbs(int n, const float* __restrict__ shapes, const float* __restrict__ props,
+                          const float* __restrict__ tr, float* cand, float* aabb, int* isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Shape sh = loadShape(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * i + 6));
+  Tr t;
+  for (int k = 0; k < 9; k++) t.b.m[k] = tr[12 * i + k];
+  t.o = v3(tr[12 * i + 9], tr[12 * i + 10], tr[12 * i + 11]);
+  V3 mn, mx;
+  shapeAabb(sh, t, mn, mx);
+  ((float4*)cand)[2 * i] = make_float4(mn.x, mn.y, mn.z, 0.0f);
+  ((float4*)cand)[2 * i + 1] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+  float* A = aabb + 6 * i;
+  A[0] = mn.x; A[1] = mn.y; A[2] = mn.z; A[3] = mx.x; A[4] = mx.y; A[5] = mx.z;
+  const float e = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+  isHuge[i] = !(e < HUGE_EXTENT) ? 1 : 0;  // planes (+-1e30 boxes) are tested per world, outside the tree
+}
+
+void rayBuildTree(Ctx& c) {
+  RayTree& t = c.rt;
+  const int n = c.nb;
+  hipStream_t s = c.stream;
+  t.n = n;
+  t.cand.ensure(8 * (size_t)n);
+  t.aabb.ensure(6 * (size_t)n);
+  t.isHuge.ensure(n);
+  t.order.ensure(n);
+  t.orderAlt.ensure(n);
+  t.morton.ensure(n);
+  t.mortonAlt.ensure(n);
+  t.hugeList.ensure(n);
+  t.hugeListAlt.ensure(n);
+  t.hugeWorld.ensure(n);
+  t.hugeWorldAlt.ensure(n);
+  t.leafBox.ensure(8 * (size_t)n);
+  t.leafExact.ensure(8 * (size_t)n);
+  t.nodeBox.ensure(8 * (size_t)n);
+  t.nodeRec.ensure(4 * kRecF4 * (size_t)n);
+  t.nodeChild.ensure(2 * (size_t)n);
+  t.nodeFirst.ensure(n);
+  t.nodeLast.ensure(n);
+  t.chunkBox.ensure(8 * ((size_t)n / 31 + 64));
+  t.bounds.ensure(16 + kStatWords * kStatBlocks);
+  c.counter.ensure(8);
+  hipLaunchKernelGGL(kRayAabbs, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)c.shapes.p, (const float*)c.props.p,
+                     (const float*)c.tr.p, t.cand.p, t.aabb.p, t.isHuge.p);
+  unsigned int* bounds = t.bounds.p;
+  float* stats = (float*)(t.bounds.p + 8);
+  unsigned int* part = t.bounds.p + 16;
+  const int nparts = std::min(gridFor(n), kStatBlocks);
+  hipLaunchKernelGGL(kProxyStats, dim3(nparts), dim3(256), 0, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
+                     part);
+  hipLaunchKernelGGL(kProxyStatsFinal, dim3(1), dim3(256), 0, s, nparts, (const unsigned int*)part, bounds, stats);
+  int* hugeCount = (int*)(c.counter.p + 7);
+  size_t tbSel = 0;
+  hipcub::CountingInputIterator<int> ids(0);
+  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbSel, ids, (const int*)t.isHuge.p, t.hugeList.p, hugeCount, n, s));
+  c.cubTmp.ensure(tbSel);
+  JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbSel, ids, (const int*)t.isHuge.p, t.hugeList.p, hugeCount, n, s));
+  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
+                     (const int*)c.world.p, (const unsigned int*)bounds, t.morton.p, t.order.p);
+  int hcount = 0;
+  JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
+  size_t tmpBytes = 0;
+  hipcub::DoubleBuffer<uint64_t> dk(t.morton.p, t.mortonAlt.p);
+  hipcub::DoubleBuffer<int> dv(t.order.p, t.orderAlt.p);
+  const int keyBits = 32 + c.worldBits;
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, keyBits, s));
+  c.cubTmp.ensure(tmpBytes);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, keyBits, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  t.nHuge = hcount;
+  t.nn = n - hcount;
+  const uint64_t* code = dk.Current();
+  const int* order = dv.Current();
+  t.sortedOrder = order;
+  t.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
+  JB_CHECK(hipMemsetAsync(t.hugeRange.p, 0, 2 * ((size_t)c.maxWorld + 1) * sizeof(int), s));
+  if (t.nHuge > 0) {
+    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)t.hugeList.p,
+                       (const int*)c.world.p, t.hugeWorld.p);
+    if (c.worldBits > 0) {
+      size_t tb = 0;
+      hipcub::DoubleBuffer<int> hk(t.hugeWorld.p, t.hugeWorldAlt.p);
+      hipcub::DoubleBuffer<int> hv(t.hugeList.p, t.hugeListAlt.p);
+      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, t.nHuge, 0, c.worldBits, s));
+      c.cubTmp.ensure(tb);
+      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, t.nHuge, 0, c.worldBits, s));
+      if (hk.Current() != t.hugeWorld.p)
+        JB_CHECK(hipMemcpyAsync(t.hugeWorld.p, hk.Current(), t.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+      if (hv.Current() != t.hugeList.p)
+        JB_CHECK(hipMemcpyAsync(t.hugeList.p, hv.Current(), t.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)t.hugeWorld.p,
+                       t.hugeRange.p);
+  }
+  const int nn = t.nn;
+  if (nn >= 1)
+    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, (const float*)t.cand.p,
+                       (const int*)c.world.p, t.leafBox.p, (int)JBGPU_BP_DBVT, (const float*)t.aabb.p,
+                       (const int*)nullptr, (const int*)c.filt.p, (int4*)t.leafExact.p);
+  if (nn >= 2) {
+    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, t.nodeChild.p, t.nodeFirst.p,
+                       t.nodeLast.p);
+    const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32;
+    float* c1 = t.chunkBox.p;
+    float* c2 = c1 + 8 * (size_t)n1;
+    float* c3 = c2 + 8 * (size_t)n2;
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, t.leafBox.p, c1);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, c1, c2);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, c2, c3);
+    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, t.nodeFirst.p, t.nodeLast.p,
+                       t.leafBox.p, c1, c2, c3, t.nodeBox.p);
+    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)t.nodeChild.p,
+                       (const int*)t.nodeLast.p, (const float*)t.leafBox.p, (const float*)t.nodeBox.p,
+                       (float4*)t.nodeRec.p);
+  }
 }
 
 void bpFindPairs(Ctx& c) {

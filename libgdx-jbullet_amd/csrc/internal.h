@@ -79,6 +79,24 @@ struct Stats {
   int64_t huge = 0;
 };
 
+// LBVH over the bodies' current shape AABBs for world queries (rayTest); same layout as the broadphase tree
+constexpr int kRecF4 = 5;  // float4 per traversal record: both children's boxes, then (child0, child1, last0, last1)
+struct RayTree {
+  DBuf<float> cand, aabb, leafBox, nodeBox, nodeRec, chunkBox;
+  DBuf<int> isHuge, order, orderAlt, nodeChild, nodeFirst, nodeLast, hugeList, hugeListAlt, hugeWorld, hugeWorldAlt,
+      hugeRange, leafExact;
+  DBuf<uint64_t> morton, mortonAlt;
+  DBuf<unsigned int> bounds;
+  int n = 0, nn = 0, nHuge = 0;
+  const int* sortedOrder = nullptr;
+  // per query
+  DBuf<int> count;
+  DBuf<int64_t> offset;
+  DBuf<uint64_t> keys, keysAlt;
+  DBuf<float> rec, io;
+  DBuf<int> ioInt;
+};
+
 struct Ctx {
   hipStream_t stream = nullptr;
   std::string err;
@@ -129,6 +147,7 @@ struct Ctx {
   DBuf<float> nodeBox;   // internal nodes np*8
   DBuf<float> nodeRec;   // traversal records np*20: both children's boxes, indices and last leaves
   DBuf<int> hugeRange;   // per world: its range of the huge list
+  RayTree rt;            // world queries
   DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
@@ -258,6 +277,9 @@ void bpFindPairs(Ctx& c);                                // A3-A7: sorted canoni
 void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
 // narrowphase
 void bpDestroyProxies(Ctx& c, int n, const int32_t* uids);  // BroadphaseInterface.destroyProxy
+void rayBuildTree(Ctx& c);                               // LBVH over the current shape AABBs
+void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group, int16_t mask, const int* dWorld,
+             int* dObj, float* dFrac, float* dNormal, float* dPoint);  // CollisionWorld.rayTest, closest hit
 void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, uint64_t* removed, int64_t capRemoved,
                   int64_t* nRemoved);
 void npDispatchAll(Ctx& c);                              // A8-A16

@@ -1275,6 +1275,327 @@ __global__ __launch_bounds__(256) void kCountPoints(int64_t n, const float* __re
   }
 }
 
+
+// ------------------------------------------------------------------ rayTest (CollisionWorld.java:552-591)
+__device__ __forceinline__ int rayOutcode(V3 p, V3 h) {  // AabbUtil2.java:40-43
+  return (p.x < -h.x ? 0x01 : 0x0) | (p.x > h.x ? 0x08 : 0x0) | (p.y < -h.y ? 0x02 : 0x0) | (p.y > h.y ? 0x10 : 0x0) |
+         (p.z < -h.z ? 0x4 : 0x0) | (p.z > h.z ? 0x20 : 0x0);
+}
+// AabbUtil2.rayAabb (AabbUtil2.java:45-104): param in, entry fraction out
+__device__ bool rayAabb(V3 from, V3 to, V3 mn, V3 mx, float& param) {
+  V3 he = scl(sub(mx, mn), 0.5f);
+  V3 c = scl(add(mx, mn), 0.5f);
+  V3 source = sub(from, c), target = sub(to, c);
+  const int so = rayOutcode(source, he), tc = rayOutcode(target, he);
+  if ((so & tc) != 0x0) return false;
+  float enter = 0.0f, exit = param;
+  V3 r = sub(target, source);
+  float normSign = 1.0f;
+  int bit = 1;
+  for (int j = 0; j < 2; j++) {
+    for (int i = 0; i != 3; ++i) {
+      if ((so & bit) != 0) {
+        float l = (-coord(source, i) - coord(he, i) * normSign) / coord(r, i);
+        if (enter <= l) enter = l;
+      } else if ((tc & bit) != 0) {
+        float l = (-coord(source, i) - coord(he, i) * normSign) / coord(r, i);
+        exit = jmin(exit, l);
+      }
+      bit <<= 1;
+    }
+    normSign = -1.0f;
+  }
+  if (enter <= exit) {
+    param = enter;
+    return true;
+  }
+  return false;
+}
+// conservative segment-box test for tree nodes (double precision, box widened): never rejects a box that the
+// exact float test above accepts
+__device__ __forceinline__ bool segmentHitsBox(V3 F, V3 T, float4 mn, float4 mx) {
+  double t0 = 0.0, t1 = 1.0;
+  const float fo[3] = {F.x, F.y, F.z}, ft[3] = {T.x, T.y, T.z};
+  const float lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+  for (int a = 0; a < 3; a++) {
+    const double l = lo[a], h = hi[a];
+    const double eps = 1e-5 * (fabs(l) + fabs(h) + fabs((double)fo[a]) + fabs((double)ft[a])) + 1e-6;
+    const double o = fo[a], d = (double)ft[a] - (double)fo[a];
+    if (d == 0.0) {
+      if (o < l - eps || o > h + eps) return false;
+    } else {
+      double ta = (l - eps - o) / d, tb = (h + eps - o) / d;
+      if (ta > tb) {
+        const double x = ta;
+        ta = tb;
+        tb = x;
+      }
+      t0 = ta > t0 ? ta : t0;
+      t1 = tb < t1 ? tb : t1;
+      if (t0 > t1) return false;
+    }
+  }
+  return true;
+}
+__device__ __forceinline__ bool rayNeeds(int filt, int group, int mask) {  // RayResultCallback.needsCollision
+  const int pg = (int)(short)(filt & 0xffff), pm = (int)(short)(filt >> 16);
+  return ((pg & mask) & 0xFFFF) != 0 && ((group & pm) & 0xFFFF) != 0;
+}
+
+// candidate objects of every ray: filter and rayAabb with the callback's initial fraction (1); the count pass
+// stores the number per ray, the emit pass writes (ray << 32 | object) at the ray's offset
+constexpr int kRayStack = 64;
+template <bool Emit>
+__global__ void kRayCandidates(int nr, const float* __restrict__ from, const float* __restrict__ to,
+                               const int* __restrict__ rworld, int group, int mask, int nn,
+                               const float4* __restrict__ rec, const int4* __restrict__ leafExact,
+                               const int* __restrict__ hugeRange, const int* __restrict__ hugeList,
+                               const float* __restrict__ aabb, const int* __restrict__ filt,
+                               const int* __restrict__ wid, int* count, const int64_t* __restrict__ offset,
+                               uint64_t* keys, int* err) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const V3 F = v3(from[3 * r], from[3 * r + 1], from[3 * r + 2]), T = v3(to[3 * r], to[3 * r + 1], to[3 * r + 2]);
+  const int w = rworld ? rworld[r] : 0;
+  int cnt = 0;
+  const int64_t base = Emit ? offset[r] : 0;
+  const int room = Emit ? count[r] : 0;  // the emit pass writes no more than the count pass found
+  auto consider = [&](int o, V3 mn, V3 mx, int f) {
+    if (wid[o] != w || !rayNeeds(f, group, mask)) return;  // other worlds are other collision-object lists
+    float param = 1.0f;
+    if (!rayAabb(F, T, mn, mx, param)) return;
+    if (Emit && cnt < room) keys[base + cnt] = ((uint64_t)(uint32_t)r << 32) | (uint32_t)o;
+    cnt++;
+  };
+  for (int k = hugeRange[2 * w]; k < hugeRange[2 * w + 1]; k++) {
+    const int o = hugeList[k];
+    const float* A = aabb + 6 * o;
+    consider(o, v3(A[0], A[1], A[2]), v3(A[3], A[4], A[5]), filt[o]);
+  }
+  auto leaf = [&](int t) {
+    const int4 e0 = leafExact[2 * t], e1 = leafExact[2 * t + 1];
+    consider(e1.w, v3(__int_as_float(e0.x), __int_as_float(e0.y), __int_as_float(e0.z)),
+             v3(__int_as_float(e0.w), __int_as_float(e1.x), __int_as_float(e1.y)), e1.z);
+  };
+  if (nn == 1) {
+    leaf(0);
+  } else if (nn >= 2) {
+    int stack[kRayStack];
+    int sp = 0, node = 0;
+    while (true) {
+      const float4* R = rec + kRecF4 * node;
+      const float4 m = R[4];
+      int next = -1;
+      for (int c = 0; c < 2; c++) {
+        const int ch = __float_as_int(c == 0 ? m.x : m.y);
+        const float4 bmn = R[2 * c], bmx = R[2 * c + 1];
+        if (w < __float_as_int(bmn.w) || w > __float_as_int(bmx.w)) continue;
+        if (!segmentHitsBox(F, T, bmn, bmx)) continue;
+        if (ch < 0) {
+          leaf(-ch - 1);
+        } else if (next < 0) {
+          next = ch;
+        } else if (sp < kRayStack) {
+          stack[sp++] = ch;
+        } else {
+          atomicOr(err, 1);
+        }
+      }
+      if (next >= 0) node = next;
+      else if (sp > 0) node = stack[--sp];
+      else break;
+    }
+  }
+  if (!Emit) count[r] = cnt;
+}
+
+// planeSpace1 (TransformUtil.java:45-61)
+__device__ __forceinline__ void planeSpace1(V3 n, V3& p, V3& q) {
+  if (jabs(n.z) > 0.7071067811865475244008443621048490f) {
+    float a = n.y * n.y + n.z * n.z;
+    float k = 1.0f / jsqrt(a);
+    p = v3(0, -n.z * k, n.y * k);
+    q = v3(a * k, -n.x * p.z, n.x * p.y);
+  } else {
+    float a = n.x * n.x + n.y * n.y;
+    float k = 1.0f / jsqrt(a);
+    p = v3(-n.y * k, n.x * k, 0);
+    q = v3(-n.z * p.y, n.z * p.x, a * k);
+  }
+}
+__device__ __forceinline__ V3 interp3(V3 a, V3 b, float t) {  // VectorUtil.setInterpolate3
+  const float s = 1.0f - t;
+  return v3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
+}
+// TriangleRaycastCallback.processTriangle (TriangleRaycastCallback.java:51-124) without its `distance <
+// hitFraction` test (that one is made where the fraction is known; nothing else depends on it)
+__device__ bool rayTriangleGeom(V3 a, V3 b, V3 cc, V3 from, V3 to, float& distance, V3& normal) {
+  V3 tn = crs(sub(b, a), sub(cc, a));
+  float dist = dot(a, tn);
+  float da = dot(tn, from);
+  da -= dist;
+  float db = dot(tn, to);
+  db -= dist;
+  if (da * db >= 0.0f) return false;
+  float proj = da - db;
+  distance = da / proj;
+  float tol = len2(tn);
+  tol *= -0.0001f;
+  V3 point = interp3(from, to, distance);
+  V3 v0p = sub(a, point), v1p = sub(b, point);
+  if (!(dot(crs(v0p, v1p), tn) >= tol)) return false;
+  V3 v2p = sub(cc, point);
+  if (!(dot(crs(v1p, v2p), tn) >= tol)) return false;
+  if (!(dot(crs(v2p, v0p), tn) >= tol)) return false;
+  normal = da > 0.0f ? tn : scl(tn, -1.0f);
+  return true;
+}
+
+// SubsimplexConvexCast.calcTimeOfImpact (SubsimplexConvexCast.java:61-196) for the ray's point (SphereShape(0),
+// margin 0: support at its transform's origin) against shape sb at tb
+__device__ bool subsimplexRayCast(Simplex& ss, const Shape& sb, const float* verts, V3 F, V3 T, const Tr& tb,
+                                  float& fraction, V3& normal) {
+  ss.reset();
+  Tr fromA;
+  fromA.b = mident();
+  fromA.o = F;
+  V3 linVelA = sub(T, F), linVelB = sub(tb.o, tb.o);
+  float lambda = 0.0f;
+  Tr iA = fromA, iB = tb;
+  V3 r = sub(linVelA, linVelB);
+  V3 supA = xform(fromA, v3(0, 0, 0));
+  V3 supB = xform(tb, supportMargin(sb, verts, mulMtV(tb.b, r)));
+  V3 v = sub(supA, supB);
+  int maxIter = 32;
+  V3 n = v3(0, 0, 0);
+  float dist2 = len2(v);
+  const float epsilon = 0.0001f;
+  while ((dist2 > epsilon) && (maxIter--) != 0) {
+    supA = xform(iA, v3(0, 0, 0));
+    supB = xform(iB, supportMargin(sb, verts, mulMtV(iB.b, v)));
+    V3 w = sub(supA, supB);
+    float VdotW = dot(v, w);
+    if (lambda > 1.0f) return false;
+    if (VdotW > 0.0f) {
+      float VdotR = dot(v, r);
+      if (VdotR >= -(kFltEps * kFltEps)) return false;
+      lambda = lambda - VdotW / VdotR;
+      const float s = 1.0f - lambda;
+      iA.o = v3(s * F.x + lambda * T.x, s * F.y + lambda * T.y, s * F.z + lambda * T.z);
+      iB.o = v3(s * tb.o.x + lambda * tb.o.x, s * tb.o.y + lambda * tb.o.y, s * tb.o.z + lambda * tb.o.z);
+      w = sub(supA, supB);
+      n = v;
+    }
+    ss.addVertex(w, supA, supB);
+    const bool ok = ss.update();
+    v = ss.cV;
+    dist2 = ok ? len2(v) : 0.0f;
+  }
+  fraction = lambda;
+  normal = len2(n) >= kFltEps * kFltEps ? nor(n) : v3(0, 0, 0);
+  if (dot(normal, r) >= -0.0f) return false;  // CastResult.allowedPenetration = 0
+  return true;
+}
+
+// per candidate (sorted by ray, then object): the object's hit, independent of the ray's running fraction.
+// record: [flags (bit0 hit / first triangle, bit1 second triangle, bit2 plane), f0, n0 xyz, f1, n1 xyz, -]
+constexpr int kRayRec = 10;
+__global__ __launch_bounds__(kGjkBlock) void kRayCast(int64_t n, const uint64_t* __restrict__ keys,
+                                                      const float* __restrict__ from, const float* __restrict__ to,
+                                                      const float* __restrict__ shapes,
+                                                      const float* __restrict__ props, const float* __restrict__ tr,
+                                                      const float* __restrict__ verts, float* out) {
+  __shared__ float simplexLds[kSimplexRows * kGjkBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const int r = (int)(k >> 32), o = (int)(k & 0xffffffffu);
+  const V3 F = v3(from[3 * r], from[3 * r + 1], from[3 * r + 2]), T = v3(to[3 * r], to[3 * r + 1], to[3 * r + 2]);
+  const Shape sb = loadShape(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * o + 6));
+  const Tr tb = loadTr(tr, o);
+  float* R = out + kRayRec * i;
+  int flags = 0;
+  float f0 = 0.0f, f1 = 0.0f;
+  V3 n0 = v3(0, 0, 0), n1 = v3(0, 0, 0);
+  if (sb.type != JBGPU_SHAPE_PLANE) {
+    Simplex ss;
+    ss.lw = simplexLds + threadIdx.x;
+    V3 n;
+    if (subsimplexRayCast(ss, sb, verts, F, T, tb, f0, n) && len2(n) > 0.0001f) {
+      flags = 1;
+      n0 = nor(mulMV(mident(), n));  // castResult.normal.mul(rayFromTrans.basis), then nor()
+    }
+  } else {  // concave branch: processAllTriangles in the plane's frame (StaticPlaneShape.java:60-121)
+    flags = 4;
+    const Tr w2o = trInverse(tb);
+    const V3 fl = xform(w2o, F), tl = xform(w2o, T);
+    const V3 rmin = v3(jmin(fl.x, tl.x), jmin(fl.y, tl.y), jmin(fl.z, tl.z));
+    const V3 rmax = v3(jmax(fl.x, tl.x), jmax(fl.y, tl.y), jmax(fl.z, tl.z));
+    const float radius = len(scl(rmin, 0.5f));  // halfExtents set from aabbMax, then from aabbMin (kept)
+    const V3 center = scl(add(rmax, rmin), 0.5f);
+    V3 t0, t1;
+    planeSpace1(sb.pn, t0, t1);
+    const V3 pc = sub(center, scl(sb.pn, dot(sb.pn, center) - sb.pc));
+    const V3 a = scl(t0, radius), b = scl(t1, radius);
+    const V3 ppp = v3(pc.x + a.x + b.x, pc.y + a.y + b.y, pc.z + a.z + b.z);
+    V3 nl;
+    if (rayTriangleGeom(ppp, add(pc, sub(a, b)), sub(pc, sub(a, b)), fl, tl, f0, nl)) {
+      flags |= 1;
+      n0 = mulMV(tb.b, nl);  // addSingleResult with the normal in the object's frame
+    }
+    if (rayTriangleGeom(sub(pc, sub(a, b)), sub(pc, add(a, b)), ppp, fl, tl, f1, nl)) {
+      flags |= 2;
+      n1 = mulMV(tb.b, nl);
+    }
+  }
+  R[0] = __int_as_float(flags);
+  R[1] = f0;
+  R[2] = n0.x; R[3] = n0.y; R[4] = n0.z;
+  R[5] = f1;
+  R[6] = n1.x; R[7] = n1.y; R[8] = n1.z;
+  R[9] = 0.0f;
+}
+
+// ClosestRayResultCallback over a ray's candidates in world order: rayAabb with the running fraction, then
+// the object's hit if its fraction is below it (CollisionWorld.java:566-589, :282-298, :710-727)
+__global__ void kRayFold(int nr, const int* __restrict__ count, const int64_t* __restrict__ offset,
+                         const uint64_t* __restrict__ keys, const float* __restrict__ recs,
+                         const float* __restrict__ from, const float* __restrict__ to,
+                         const float* __restrict__ aabb, int* obj, float* frac, float* normal, float* point) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const V3 F = v3(from[3 * r], from[3 * r + 1], from[3 * r + 2]), T = v3(to[3 * r], to[3 * r + 1], to[3 * r + 2]);
+  float best = 1.0f;
+  int hit = -1;
+  V3 hn = v3(0, 0, 0), hp = v3(0, 0, 0);
+  const int64_t b = offset[r], e = b + count[r];
+  for (int64_t k = b; k < e; k++) {
+    if (best == 0.0f) break;
+    const int o = (int)(keys[k] & 0xffffffffu);
+    const float* A = aabb + 6 * o;
+    float param = best;
+    if (!rayAabb(F, T, v3(A[0], A[1], A[2]), v3(A[3], A[4], A[5]), param)) continue;
+    const float* R = recs + kRayRec * k;
+    const int flags = __float_as_int(R[0]);
+    if ((flags & 1) && R[1] < best) {
+      best = R[1];
+      hit = o;
+      hn = v3(R[2], R[3], R[4]);
+      hp = interp3(F, T, best);
+    }
+    if ((flags & 2) && R[5] < best) {  // the plane's second triangle, against the updated fraction
+      best = R[5];
+      hit = o;
+      hn = v3(R[6], R[7], R[8]);
+      hp = interp3(F, T, best);
+    }
+  }
+  obj[r] = hit;
+  frac[r] = best;
+  normal[3 * r] = hn.x; normal[3 * r + 1] = hn.y; normal[3 * r + 2] = hn.z;
+  point[3 * r] = hp.x; point[3 * r + 1] = hp.y; point[3 * r + 2] = hp.z;
+}
+
 }  // namespace
 
 int64_t npCountContacts(Ctx& c) {
@@ -1354,6 +1675,62 @@ void npCollectStats(Ctx& c) {  // after the step's final synchronisation
   c.st.epaCalls = (int64_t)c.npStatHost[0];
   c.st.epaFails = (int64_t)c.npStatHost[1];
   if (c.npStatHost[2]) c.err = "EPA workspace overflow";
+}
+
+
+void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group, int16_t mask, const int* dWorld,
+             int* dObj, float* dFrac, float* dNormal, float* dPoint) {
+  if (n <= 0) return;
+  rayBuildTree(c);
+  RayTree& t = c.rt;
+  hipStream_t s = c.stream;
+  t.count.ensure((size_t)n + 1);  // count[n] = 0 closes the scan: offset[n] is the total
+  t.offset.ensure((size_t)n + 1);
+  int* err = (int*)(c.counter.p + 6);
+  JB_CHECK(hipMemsetAsync(err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(kRayCandidates<false>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
+                     (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p, (const int*)t.hugeRange.p,
+                     (const int*)t.hugeList.p, (const float*)t.aabb.p, (const int*)c.filt.p,
+                     (const int*)c.world.p, t.count.p, (const int64_t*)nullptr, (uint64_t*)nullptr, err);
+  JB_CHECK(hipMemsetAsync(t.count.p + n, 0, sizeof(int), s));
+  size_t tb = 0;
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t.count.p, t.offset.p, n + 1, s));
+  c.cubTmp.ensure(tb);
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tb, t.count.p, t.offset.p, n + 1, s));
+  int64_t total = 0;
+  int errh = 0;
+  JB_CHECK(hipMemcpyAsync(&total, t.offset.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(&errh, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  if (errh) throw std::runtime_error("ray traversal stack overflow");
+  if (total > 0) {
+    t.keys.ensure(total);
+    t.keysAlt.ensure(total);
+    t.rec.ensure((size_t)kRayRec * total);
+    hipLaunchKernelGGL(kRayCandidates<true>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
+                       (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p,
+                       (const int*)t.hugeRange.p, (const int*)t.hugeList.p, (const float*)t.aabb.p,
+                       (const int*)c.filt.p, (const int*)c.world.p, t.count.p, (const int64_t*)t.offset.p, t.keys.p,
+                       err);
+    // world order inside every ray's segment: sort by (ray, object)
+    int bits = 32;
+    while (bits < 64 && (1ll << (bits - 32)) < (int64_t)n) bits++;
+    hipcub::DoubleBuffer<uint64_t> kk(t.keys.p, t.keysAlt.p);
+    size_t ts = 0;
+    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, ts, kk, (int)total, 0, bits, s));
+    c.cubTmp.ensure(ts);
+    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(c.cubTmp.p, ts, kk, (int)total, 0, bits, s));
+    hipLaunchKernelGGL(kRayCast, dim3(gridFor(total, kGjkBlock)), dim3(kGjkBlock), 0, s, total,
+                       (const uint64_t*)kk.Current(), dFrom, dTo, (const float*)c.shapes.p, (const float*)c.props.p,
+                       (const float*)c.tr.p, (const float*)c.verts.p, t.rec.p);
+    hipLaunchKernelGGL(kRayFold, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)t.count.p,
+                       (const int64_t*)t.offset.p, (const uint64_t*)kk.Current(), (const float*)t.rec.p, dFrom, dTo,
+                       (const float*)t.aabb.p, dObj, dFrac, dNormal, dPoint);
+  } else {
+    hipLaunchKernelGGL(kRayFold, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)t.count.p,
+                       (const int64_t*)t.offset.p, (const uint64_t*)nullptr, (const float*)nullptr, dFrom, dTo,
+                       (const float*)t.aabb.p, dObj, dFrac, dNormal, dPoint);
+  }
 }
 
 }  // namespace jbg

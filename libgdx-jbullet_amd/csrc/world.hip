@@ -590,6 +590,46 @@ int jbgpu_bp_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, int64_t cap, int64_t* n) 
   });
 }
 
+int jbgpu_world_ray_test(jbgpu_ctx* ctx, int32_t n, const float* from3, const float* to3, int16_t group,
+                         int16_t mask, const int32_t* world, int32_t* obj, float* fraction, float* normal3,
+                         float* point3, int32_t on_device) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (n < 0 || (n > 0 && (!from3 || !to3 || !obj || !fraction || !normal3 || !point3)))
+      throw std::invalid_argument("bad ray arguments");
+    if (n == 0) return;
+    if (world && !on_device)
+      for (int i = 0; i < n; i++)
+        if (world[i] < 0 || world[i] > c.maxWorld) throw std::invalid_argument("ray world id out of range");
+    if (on_device) {
+      rayTest(c, n, from3, to3, group, mask, world, obj, fraction, normal3, point3);
+      JB_CHECK(hipStreamSynchronize(c.stream));
+      return;
+    }
+    hipStream_t s = c.stream;
+    RayTree& t = c.rt;
+    t.io.ensure(13 * (size_t)n);
+    t.ioInt.ensure(2 * (size_t)n);
+    float* dFrom = t.io.p;
+    float* dTo = dFrom + 3 * (size_t)n;
+    float* dFrac = dTo + 3 * (size_t)n;
+    float* dNormal = dFrac + n;
+    float* dPoint = dNormal + 3 * (size_t)n;
+    int* dObj = t.ioInt.p;
+    int* dWorld = world ? t.ioInt.p + n : nullptr;
+    JB_CHECK(hipMemcpyAsync(dFrom, from3, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    JB_CHECK(hipMemcpyAsync(dTo, to3, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    if (world) JB_CHECK(hipMemcpyAsync(dWorld, world, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+    rayTest(c, n, dFrom, dTo, group, mask, dWorld, dObj, dFrac, dNormal, dPoint);
+    JB_CHECK(hipMemcpyAsync(obj, dObj, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(fraction, dFrac, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(normal3, dNormal, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(point3, dPoint, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
+  });
+}
+
 int jbgpu_bp_destroy_proxies(jbgpu_ctx* ctx, int32_t n, const int32_t* uids) {
   return guard(ctx, [&] {
     if (n < 0 || (n > 0 && !uids)) throw std::invalid_argument("bad proxy arguments");

@@ -57,6 +57,8 @@ SYMBOLS = {
     "jbgpu_bp_calculate_overlapping_pairs": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "jbgpu_bp_get_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64)]),
     "jbgpu_bp_destroy_proxies": (C.c_int, [_vp, C.c_int32, _vp]),
+    "jbgpu_world_ray_test": (C.c_int, [_vp, C.c_int32, _vp, _vp, C.c_int16, C.c_int16, _vp, _vp, _vp, _vp, _vp,
+                                       C.c_int32]),
     "jbgpu_bp_get_pair_deltas": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64), _vp, C.c_int64,
                                            C.POINTER(C.c_int64)]),
     "jbgpu_bp_get_proxies": (C.c_int, [_vp, _f, C.POINTER(C.c_int32)]),
@@ -267,6 +269,26 @@ class Context:
 
     def step(self, dt=1.0 / 60.0, n=1):
         self._ck(load().jbgpu_world_step(self.h, dt, int(n)), "world_step")
+
+    def ray_test(self, frm, to, group=1, mask=-1, world=None):
+        """rayTest with a closest-hit callback per ray: (body or -1, fraction, normal, point)."""
+        frm = f32(frm).reshape(-1, 3)
+        to = f32(to).reshape(-1, 3)
+        n = len(frm)
+        obj = np.zeros(n, np.int32)
+        fr = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        pt = np.zeros((n, 3), np.float32)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        self._ck(load().jbgpu_world_ray_test(self.h, n, _p(frm), _p(to), int(np.int16(group)), int(np.int16(mask)),
+                                             _p(w), _p(obj), _p(fr), _p(nrm), _p(pt), 0), "world_ray_test")
+        return obj, fr, nrm, pt
+
+    def ray_test_device(self, n, frm, to, obj, fr, nrm, pt, group=1, mask=-1):
+        """rayTest on rays and results resident in device buffers (jbgpu_buffer_alloc addresses)."""
+        v = lambda a: C.c_void_p(int(a))
+        self._ck(load().jbgpu_world_ray_test(self.h, int(n), v(frm), v(to), int(np.int16(group)), int(np.int16(mask)),
+                                             None, v(obj), v(fr), v(nrm), v(pt), 1), "world_ray_test")
 
     def step_until(self, phase, dt=1.0 / 60.0):
         self._ck(load().jbgpu_world_step_until(self.h, dt, int(phase)), "world_step_until")

@@ -249,3 +249,68 @@ def test_world_mixed_pile_100k_parity():
         _compare_world(ctx, orc, f"100k step {301 + step}")
         print(f"[100k] step {301 + step} bit-exact", flush=True)
     ctx.close()
+
+
+def _rays(rng, n, lo, hi, vertical_share=0.3):
+    """Ray segments between random points of the box [lo, hi]; a share of them vertical (axis-parallel)."""
+    lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+    a = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    b = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    v = rng.random(n) < vertical_share
+    b[v, 0], b[v, 2] = a[v, 0], a[v, 2]
+    a[v, 1], b[v, 1] = hi[1], lo[1]
+    return a, b
+
+
+def _check_rays(ctx, orc, frm, to, what, world=None, group=1, mask=-1):
+    got = ctx.ray_test(frm, to, group=group, mask=mask, world=world)
+    want = orc.ray_test(frm, to, group=group, mask=mask, world=world)
+    hits = int((want[0] >= 0).sum())
+    print(f"[rays {what}] {len(frm)} rays, {hits} hits", flush=True)
+    assert np.array_equal(got[0], want[0]), f"{what}: hit objects differ"
+    for g, w_, name in zip(got[1:], want[1:], ("fraction", "normal", "point")):
+        assert np.array_equal(g.view(np.uint32), w_.view(np.uint32)), f"{what}: {name} bits differ"
+    return hits
+
+
+@pytest.mark.parametrize("scene", ["boxes", "mixed", "batched"])
+def test_ray_test_matches_oracle(scene):
+    """CollisionWorld.rayTest (closest hit) on the device vs the oracle, from the same body state: hit body,
+    fraction, normal and point bit for bit; vertical rays included (axis-parallel segments)."""
+    rng = np.random.default_rng(17)
+    world = None
+    if scene == "boxes":
+        sc = scenes.boxes_grid(seed=2)
+        ctx, orc = _world_pair(sc, 10)
+        lo, hi = (-10, -1, -10), (10, 20, 10)
+    elif scene == "mixed":
+        sc = scenes.mixed_pile(seed=4, n=3000, bin_half=15.0)
+        ctx, orc = _world_pair(sc, 10)
+        lo, hi = (-17, -1, -17), (17, 30, 17)
+    else:
+        sc = scenes.batched_towers(seed=5, n_worlds=24)
+        ctx, orc = _world_pair(sc, 10)
+        lo, hi = (-15, -1, -15), (15, 25, 15)
+    ctx.step(n=40)
+    orc.set_bodies(ctx.bodies())
+    frm, to = _rays(rng, 3000, lo, hi)
+    if scene == "batched":
+        world = rng.integers(0, int(sc.world.max()) + 1, len(frm)).astype(np.int32)
+    hits = _check_rays(ctx, orc, frm, to, scene, world=world)
+    assert hits > len(frm) // 4
+    if scene == "mixed":  # the callback filter: static bodies (group 2) excluded
+        _check_rays(ctx, orc, frm[:500], to[:500], "mixed, static excluded", mask=np.int16(-1 ^ 2))
+    ctx.close()
+
+
+def test_ray_test_100k_pile_matches_oracle():
+    """rayTest in the 100k-body pile after 120 steps: 300 rays (the oracle tests every body per ray)."""
+    rng = np.random.default_rng(23)
+    sc = scenes.mixed_pile(seed=1, n=100_000)
+    ctx, orc = _world_pair(sc, 10)
+    ctx.step(n=120)
+    orc.set_bodies(ctx.bodies())
+    frm, to = _rays(rng, 300, (-52, -1, -52), (52, 40, 52), vertical_share=0.5)
+    hits = _check_rays(ctx, orc, frm, to, "pile 100k")
+    assert hits > 100
+    ctx.close()
