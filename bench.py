@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-broadphase", action="store_true")
+    ap.add_argument("--no-rays", action="store_true")
+    ap.add_argument("--rays", type=int, default=1 << 20)
     ap.add_argument("--bp-frames", type=int, default=20)
     ap.add_argument("--phase-steps", type=int, default=30, help="instrumented steps after the timed region")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
@@ -157,6 +159,12 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, shapes, verts, bodies, args,
                                            first_timed - 1 + args.steps + (args.phase_steps if args.phase_steps > 0 else 0))
+    if not args.no_rays:
+        try:
+            out["ray_test"] = ray_section(ctx, shapes, verts, bodies, args)
+        except Exception as e:  # an optional section must not lose the headline line
+            log(f"[bench] ray_test failed: {e!r}")
+            out["ray_test"] = {"error": repr(e)}
     ctx.close()
     if roof is not None and not args.no_traffic:
         try:
@@ -458,6 +466,51 @@ def mlp_throughput(params, norm, ev, train_samples, train_s):
             "in_solve_slots": slots, "in_solve_ms": le["mlp_ms"],
             "in_solve_tflops": None if solve is None else round(solve, 3),
             "train_tflops_wall": round(train, 3), "train_frac": round(train / BF16_DENSE_PEAK_TFLOPS, 6)}
+
+
+def ray_section(ctx, shapes, verts, bodies, args):
+    """World queries on the pile as the timed region left it: CollisionWorld.rayTest with a closest-hit callback
+    for args.rays segments between random points of the bin's volume (half of them vertical), rays and results
+    resident in device buffers; each call builds the query tree over the current AABBs.  The oracle is timed on
+    a sample of the same rays (1 core) and its results checked against the device's."""
+    rng = np.random.default_rng(args.seed)
+    n = args.rays
+    lo, hi = np.float32([-52, -1, -52]), np.float32([52, 40, 52])
+    a = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    b = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    v = rng.random(n) < 0.5
+    b[v, 0], b[v, 2] = a[v, 0], a[v, 2]
+    a[v, 1], b[v, 1] = hi[1], lo[1]
+    dev = {k: ctx.buffer_alloc(nb) for k, nb in (("f", a.nbytes), ("t", b.nbytes), ("o", 4 * n), ("fr", 4 * n),
+                                                  ("n", 12 * n), ("p", 12 * n))}
+    ctx.buffer_upload(dev["f"], a)
+    ctx.buffer_upload(dev["t"], b)
+    call = lambda: ctx.ray_test_device(n, dev["f"], dev["t"], dev["o"], dev["fr"], dev["n"], dev["p"])
+    call()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    secs = (time.perf_counter() - t0) / reps
+    for k in dev:
+        ctx.buffer_free(dev[k])
+    sample = 64
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    orc = pyoracle.World(shapes, verts, bodies, iterations=args.iterations, order=0)
+    orc.set_bodies(ctx.bodies())
+    t0 = time.perf_counter()
+    want = orc.ray_test(a[:sample], b[:sample])
+    cpu = time.perf_counter() - t0
+    got = ctx.ray_test(a[:sample], b[:sample])
+    same = bool(np.array_equal(got[0], want[0]) and all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                                                        for x, y in zip(got[1:], want[1:])))
+    log(f"[bench] ray_test: {n} rays in {1e3 * secs:.2f} ms; oracle {sample} rays in {cpu:.2f}s; agree {same}")
+    return {"rays": n, "ms_per_call": round(1e3 * secs, 3), "rays_per_s": round(n / secs, 1),
+            "hits_sample": int((want[0] >= 0).sum()), "sample": sample, "sample_matches_oracle": same,
+            "cpu_rays_per_s": round(sample / cpu, 1), "cpu_cores": 1,
+            "workload": "closest-hit rays between random points of the 104 x 41 x 104 m volume over the 100k pile "
+                        "(half vertical), after the timed region; query tree rebuilt in every call"}
 
 
 def cpu_baseline(ctx, shapes, verts, bodies, args, after_step):
