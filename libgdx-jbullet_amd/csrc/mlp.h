@@ -1,5 +1,5 @@
 // mlp.h — learned warm start for the contact solver: a 16-64-64-64-3 MLP with
-// bf16 operands and fp32 accumulation on MFMA (v_mfma_f32_16x16x16_bf16), used
+// bf16 operands and fp32 accumulation on MFMA (16x16x16 and, for inference, 16x16x32 bf16), used
 // between the solver's row build and its warm-start application.
 //
 // Input features per contact (16, fp32): normal(3), relative linear velocity at
@@ -26,8 +26,10 @@ constexpr int kMlpParams = kMlpB4 + kMlpOut;
 // normalisation: x' = (x - mean_in) * inv_std_in ; y = y' * std_out + mean_out
 // words: mean_in[16] inv_std_in[16] std_out[3] mean_out[3]
 constexpr int kMlpNormWords = 2 * kMlpIn + 2 * kMlpOut;
-// bf16 MFMA operand fragments (64 lanes x 4 values): 40 forward (W), 36 backward (W^T)
-constexpr int kMlpFwdFrags = 40, kMlpFrags = 76;
+// bf16 MFMA operand fragments (64 lanes x 4 values): 40 forward (W), 36 backward (W^T); then the inference
+// fragments of layers 2-4 for the K = 32 product (64 lanes x 8 values): 8 W2, 8 W3, 2 W4
+constexpr int kMlpFwdFrags = 40, kMlpFrags = 76, kMlpInferFrags = 18;
+constexpr int kMlpPackedWords = kMlpFrags * 256 + kMlpInferFrags * 512;
 
 void mlpSetParams(Ctx& c, const float* params, const float* norm);
 void mlpGetParams(Ctx& c, float* params, float* norm);

@@ -21,25 +21,25 @@ namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 
 constexpr int kTileFrags = 26;  // per 16-sample training tile: H0 H1x4 H2x4 H3x4 dZ1x4 dZ2x4 dZ3x4 dZ4
 
-// fp32 -> bf16, round to nearest even (finite inputs; NaN kept quiet)
-__host__ __device__ inline uint16_t toBf16(float x) {
-  uint32_t u;
-  memcpy(&u, &x, 4);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// fp32 -> bf16, round to nearest even (Inf keeps its sign, NaN stays NaN).  gfx950 converts in hardware
+// (v_cvt_pk_bf16_f32, two values per instruction); a software rounding costs ~8 VALU operations per value,
+// which made the inference kernel VALU bound.
+__device__ inline uint16_t toBf16(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// two values per conversion instruction, already in fragment order
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
-  bf16x4 r;
-  r.x = (short)toBf16(a);
-  r.y = (short)toBf16(b);
-  r.z = (short)toBf16(c);
-  r.w = (short)toBf16(d);
-  return r;
+  const bf16x2v lo = bf16x2v{(__bf16)a, (__bf16)b}, hi = bf16x2v{(__bf16)c, (__bf16)d};
+  const uint2 u = uint2{__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)};
+  return __builtin_bit_cast(bf16x4, u);
 }
 
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -78,11 +78,29 @@ __host__ __device__ inline float packedSource(const float* W, int f, int l, int 
   return W[kMlpW2 + (16 * kt + k) * kMlpHid + 16 * mt + row];
 }
 
+// Inference fragment f (K = 32 step), lane l, element i.  The B operand of K step j is the concatenation of the
+// D fragments of the previous layer's neuron tiles 2j and 2j+1, so lane group g = l/16 element i stands for
+// neuron 32j + 4g + i (i < 4) or 32j + 16 + 4g + (i - 4) (i >= 4); the A operand (weights) uses the same
+// neuron for the same (g, i).  The product pairs A and B elements by (g, i) whatever K order the hardware
+// gives them, so no further layout is assumed.
+__host__ __device__ inline float packedSource32(const float* W, int f, int l, int i) {
+  const int row = l & 15, g = l >> 4;
+  const int kk = i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4);
+  if (f < 8) return W[kMlpW2 + (16 * (f >> 1) + row) * kMlpHid + 32 * (f & 1) + kk];
+  if (f < 16) return W[kMlpW3 + (16 * ((f - 8) >> 1) + row) * kMlpHid + 32 * (f & 1) + kk];
+  return row < kMlpOut ? W[kMlpW4 + row * kMlpHid + 32 * (f - 16) + kk] : 0.0f;  // W4 padded to 16 rows
+}
+
 __global__ void kMlpPack(const float* __restrict__ W, uint16_t* __restrict__ P) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= kMlpFrags * 256) return;
-  int f = t >> 8, l = (t >> 2) & 63, i = t & 3;
-  P[t] = toBf16(packedSource(W, f, l, i));
+  if (t < kMlpFrags * 256) {
+    int f = t >> 8, l = (t >> 2) & 63, i = t & 3;
+    P[t] = toBf16(packedSource(W, f, l, i));
+  } else if (t < kMlpPackedWords) {
+    int u = t - kMlpFrags * 256;
+    int f = u >> 9, l = (u >> 3) & 63, i = u & 7;
+    P[t] = toBf16(packedSource32(W, f, l, i));
+  }
 }
 
 __device__ inline f32x4 biasInit(const float* __restrict__ b, int mt, int lane) {
@@ -90,10 +108,19 @@ __device__ inline f32x4 biasInit(const float* __restrict__ b, int mt, int lane) 
   return f32x4{p[0], p[1], p[2], p[3]};
 }
 
-__device__ inline f32x4 relu(f32x4 z) {
-  return f32x4{fmaxf(z.x, 0.0f), fmaxf(z.y, 0.0f), fmaxf(z.z, 0.0f), fmaxf(z.w, 0.0f)};
-}
+// max(z, 0) as a signed-integer max of the bit patterns (negative floats are negative integers): one
+// instruction, where fmaxf first canonicalises its operand; -0 becomes +0
+__device__ inline float relu1(float z) { return __builtin_bit_cast(float, max(__builtin_bit_cast(int, z), 0)); }
+__device__ inline f32x4 relu(f32x4 z) { return f32x4{relu1(z.x), relu1(z.y), relu1(z.z), relu1(z.w)}; }
 __device__ inline bf16x4 toB(f32x4 v) { return pack4(v.x, v.y, v.z, v.w); }
+// bf16(max(v, 0)) as max(bf16(v), 0) on the 16-bit patterns, two per instruction: rounding keeps the sign, so
+// the two agree (up to the sign of zero).  The empty asm keeps the backend from folding the 16-bit max into
+// the conversions, which splits each paired conversion in two plus a permute.
+__device__ inline bf16x4 toBRelu(f32x4 v) {
+  uint2 u = __builtin_bit_cast(uint2, toB(v));
+  asm("" : "+v"(u.x), "+v"(u.y));
+  return __builtin_bit_cast(bf16x4, __builtin_elementwise_max(__builtin_bit_cast(s16x4, u), s16x4{0, 0, 0, 0}));
+}
 
 struct Acts {  // per-lane state of one 16-contact tile (D layout)
   bf16x4 h0, h1[4], h2[4], h3[4];
@@ -146,6 +173,56 @@ __device__ inline f32x4 forward(const uint16_t* __restrict__ P, const float* __r
   return y;
 }
 
+// inference operands of one lane, loaded once per wave: the layer-1 fragments (K = 16), the K = 32 fragments of
+// layers 2-4 (80 VGPRs in all) and the bias words of the lane's output rows
+struct InferOps {
+  bf16x4 w1[4];
+  bf16x8 w[kMlpInferFrags];
+  f32x4 b[12];  // layers 1-3, tile mt of layer L at 4(L-1) + mt
+  f32x4 b4;
+};
+
+__device__ inline void loadInferOps(const uint16_t* __restrict__ P, const float* __restrict__ W, int lane,
+                                    InferOps& o) {
+#pragma unroll
+  for (int f = 0; f < 4; f++) o.w1[f] = frag(P, f, lane);
+  const uint16_t* P32 = P + kMlpFrags * 256;
+#pragma unroll
+  for (int f = 0; f < kMlpInferFrags; f++) o.w[f] = *reinterpret_cast<const bf16x8*>(P32 + ((size_t)f * 64 + lane) * 8);
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    o.b[mt] = biasInit(W + kMlpB1, mt, lane);
+    o.b[4 + mt] = biasInit(W + kMlpB2, mt, lane);
+    o.b[8 + mt] = biasInit(W + kMlpB3, mt, lane);
+  }
+  o.b4 = lane < 16 ? f32x4{W[kMlpB4], W[kMlpB4 + 1], W[kMlpB4 + 2], 0.0f} : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x4 lo, bf16x4 hi, f32x4 c) {
+  const bf16x8 b = __builtin_shufflevector(__builtin_bit_cast(bf16x4v, lo), __builtin_bit_cast(bf16x4v, hi), 0, 1, 2,
+                                           3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// forward of one 16-contact tile from register operands: the products of forward() (fp32 accumulation of the
+// same bf16 operands, in another order) with the K = 32 instruction for layers 2-4
+__device__ inline f32x4 forwardRegs(const InferOps& o, bf16x4 h0) {
+  bf16x4 h1[4], h2[4], h3[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) h1[mt] = toBRelu(mfma(o.w1[mt], h0, o.b[mt]));
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f32x4 acc = mfma32(o.w[2 * mt], h1[0], h1[1], o.b[4 + mt]);
+    h2[mt] = toBRelu(mfma32(o.w[2 * mt + 1], h1[2], h1[3], acc));
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f32x4 acc = mfma32(o.w[8 + 2 * mt], h2[0], h2[1], o.b[8 + mt]);
+    h3[mt] = toBRelu(mfma32(o.w[8 + 2 * mt + 1], h2[2], h2[3], acc));
+  }
+  return mfma32(o.w[17], h3[2], h3[3], mfma32(o.w[16], h3[0], h3[1], o.b4));
+}
+
 // input fragment of contact c (lane holds features 4(l/16)..4(l/16)+3), normalised
 __device__ inline bf16x4 inputFrag(const float* __restrict__ x, bool ok, const float* __restrict__ norm, int lane) {
   const int k = 4 * (lane >> 4);
@@ -155,37 +232,112 @@ __device__ inline bf16x4 inputFrag(const float* __restrict__ x, bool ok, const f
                (v.w - norm[k + 3]) * norm[19 + k]);
 }
 
-// inference: 256-thread blocks, one 16-contact tile per wave and iteration.  With hdr set the rows
-// of valid slots receive the predictions (fused warm start); otherwise y[n][3] is written.
+constexpr int kInferAhead = 8;  // tiles whose features a wave has in flight (HBM latency is ~10 tiles of compute)
+
+// features and slot header (validity bits 0-3, first dense row from bit 4) of one contact, loaded ahead of use
+struct InferIn {
+  float4 v;
+  int hx;
+};
+
+// branchless (a load under a lane mask makes the compiler wait for every load in flight): contacts past n read
+// contact n - 1 and are masked by the caller; features of every contact c < n are readable, slots without a
+// point included
+template <bool Fused>
+__device__ inline InferIn inferLoad(int64_t c, int64_t n, const float* __restrict__ X, const int4* __restrict__ hdr,
+                                    int k4) {
+  const int64_t cc = c < n ? c : n - 1;
+  InferIn r;
+  r.v = *reinterpret_cast<const float4*>(X + 16 * cc + k4);
+  r.hx = Fused ? hdr[cc >> 2].x : 15;
+  return r;
+}
+
+// one tile of a wave: contacts 16t..16t+15 from a ring slot, which is refilled (Refill) with the features of the
+// wave's tile kInferAhead tiles later before the products are computed
+template <bool Fused, bool Refill>
+__device__ inline void inferTile(InferIn& slot, int64_t t, int64_t n, int64_t ahead, const InferOps& o,
+                                 const float4& mean, const float4& scale, const float* __restrict__ X,
+                                 const float* __restrict__ norm, float* __restrict__ Y, const int4* __restrict__ hdr,
+                                 float* nrow, float* frow, int lane, int k4) {
+  const int64_t c = 16 * t + (lane & 15);
+  const int hx = slot.hx;
+  const bool ok = c < n && ((hx >> (int)(c & 3)) & 1) != 0;
+  // the slot is consumed before it is refilled, so the refill lands in the same registers; the invalid
+  // contacts' inputs are cleared by a mask (a select here becomes a branch)
+  const bf16x4 hv = pack4((slot.v.x - mean.x) * scale.x, (slot.v.y - mean.y) * scale.y,
+                          (slot.v.z - mean.z) * scale.z, (slot.v.w - mean.w) * scale.w);
+  const u32x2 keep = ok ? u32x2{~0u, ~0u} : u32x2{0u, 0u};
+  const bf16x4 h0 = __builtin_bit_cast(bf16x4, __builtin_bit_cast(u32x2, hv) & keep);
+  if (Refill) slot = inferLoad<Fused>(c + ahead, n, X, hdr, k4);
+  const f32x4 y = forwardRegs(o, h0);
+  if (lane < 16 && ok) {
+    float y0 = y.x * norm[32] + norm[35], y1 = y.y * norm[33] + norm[36], y2 = y.z * norm[34] + norm[37];
+    if (Fused) {  // dense rows: the slot's first row, then one row per valid point in point order
+      const int j = (int)(c & 3);
+      const size_t row = (size_t)((unsigned)hx >> 4) + __popc(hx & 15 & ((1 << j) - 1));
+      nrow[kRowWords * row + 19] = y0;
+      frow[kRowWords * (2 * row) + 19] = y1;
+      frow[kRowWords * (2 * row + 1) + 19] = y2;
+    } else {
+      Y[3 * c] = y0;
+      Y[3 * c + 1] = y1;
+      Y[3 * c + 2] = y2;
+    }
+  }
+}
+
+// inference: 256-thread blocks, one 16-contact tile per wave and iteration, operands in registers, the
+// features of the wave's next kInferAhead tiles in flight.  Fused: the rows of valid slots (hdr) receive the
+// predictions (warm start); otherwise y[n][3] is written.
+template <bool Fused>
 __global__ __launch_bounds__(256) void kMlpInfer(int64_t n, const float* __restrict__ X, const uint16_t* __restrict__ P,
                                                  const float* __restrict__ W, const float* __restrict__ norm,
                                                  float* __restrict__ Y, const int4* __restrict__ hdr, float* nrow,
                                                  float* frow) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  // tile indices are wave-uniform (scalar registers): the loop exits are scalar branches, so the compiler
+  // keeps the prefetch ring in place instead of copying it under a lane mask
+  const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
   const int64_t ntiles = (n + 15) >> 4;
-  for (int64_t t = wave; t < ntiles; t += nwaves) {
-    const int64_t c = t * 16 + (lane & 15);
-    bool ok = c < n;
-    if (hdr && ok) ok = ((hdr[c >> 2].x >> (int)(c & 3)) & 1) != 0;
-    bf16x4 h0 = inputFrag(X + 16 * c, ok, norm, lane);
-    f32x4 y = forward<false>(P, W, h0, lane, nullptr);
-    if (lane < 16 && ok) {
-      float y0 = y.x * norm[32] + norm[35], y1 = y.y * norm[33] + norm[36], y2 = y.z * norm[34] + norm[37];
-      if (hdr) {  // dense rows: the slot's first row, then one row per valid point in point order
-        const int4 h = hdr[c >> 2];
-        const int j = (int)(c & 3);
-        const size_t row = (size_t)((unsigned)h.x >> 4) + __popc(h.x & 15 & ((1 << j) - 1));
-        nrow[kRowWords * row + 19] = y0;
-        frow[kRowWords * (2 * row) + 19] = y1;
-        frow[kRowWords * (2 * row + 1) + 19] = y2;
-      } else {
-        Y[3 * c] = y0;
-        Y[3 * c + 1] = y1;
-        Y[3 * c + 2] = y2;
-      }
-    }
+  if (wave >= ntiles) return;
+  const int k4 = 4 * (lane >> 4);
+  const int lc = lane & 15;
+  InferOps o;
+  loadInferOps(P, W, lane, o);
+  const float4 mean = *reinterpret_cast<const float4*>(norm + k4);
+  const float4 scale = *reinterpret_cast<const float4*>(norm + 16 + k4);
+  // the once-per-wave operand loads complete before the ring is issued, so the loop's waits count ring loads
+  // only (the empty asm pins the normalisation loads here; invariant loads otherwise sink to their first use)
+  asm volatile("" ::"v"(mean.x), "v"(mean.y), "v"(mean.z), "v"(mean.w), "v"(scale.x), "v"(scale.y), "v"(scale.z),
+               "v"(scale.w));
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_sched_barrier(0);
+  InferIn ring[kInferAhead];
+#pragma unroll
+  for (int i = 0; i < kInferAhead; i++) {
+    ring[i] = inferLoad<Fused>(16 * ((int64_t)wave + i * nwaves) + lc, n, X, hdr, k4);
+    // issue order = slot order: the loop's wait for slot i then counts the loads issued after it (the wait
+    // instruction is static, so a slot 0 issued last would cost a full drain every iteration)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // the wave's tiles are wave, wave + nwaves, ...: whole groups of kInferAhead refill their slots; the last,
+  // partial group is a separate copy without refills, so no loop path re-enters with a fresh load in slot 0
+  const int64_t tilesOfWave = (ntiles - wave + nwaves - 1) / nwaves;
+  const int64_t ahead = 16 * (int64_t)kInferAhead * nwaves;
+  int64_t t = wave;
+  for (int64_t g = tilesOfWave / kInferAhead; g > 0; g--) {
+#pragma unroll
+    for (int i = 0; i < kInferAhead; i++, t += nwaves)
+      inferTile<Fused, true>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
+  }
+  const int rest = (int)(tilesOfWave % kInferAhead);
+#pragma unroll
+  for (int i = 0; i < kInferAhead - 1; i++, t += nwaves) {
+    if (i >= rest) break;
+    inferTile<Fused, false>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
   }
 }
 
@@ -335,12 +487,12 @@ __global__ void kFillF(float* p, int n, float v) {
 
 void ensureModel(Ctx& c) {
   c.mlpW.ensure(kMlpParams);
-  c.mlpWb.ensure((size_t)kMlpFrags * 256);
+  c.mlpWb.ensure((size_t)kMlpPackedWords);
   c.mlpNorm.ensure(kMlpNormWords);
 }
 
 void repack(Ctx& c) {
-  hipLaunchKernelGGL(kMlpPack, dim3(gridFor(kMlpFrags * 256)), dim3(256), 0, c.stream, c.mlpW.p, c.mlpWb.p);
+  hipLaunchKernelGGL(kMlpPack, dim3(gridFor(kMlpPackedWords)), dim3(256), 0, c.stream, c.mlpW.p, c.mlpWb.p);
   JB_CHECK(hipGetLastError());
 }
 
@@ -351,10 +503,10 @@ void checkNorm(const float* norm) {
     if (!(norm[32 + k] > 0.0f)) throw std::invalid_argument("output scales must be positive");
 }
 
-int inferGrid(int64_t n) {
+int inferGrid(int64_t n) {  // about as many waves as the device keeps resident (two per SIMD), four per block
   int64_t tiles = (n + 15) / 16;
   int64_t blocks = (tiles + 3) / 4;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 512));
 }
 
 }  // namespace
@@ -388,7 +540,7 @@ void mlpInfer(Ctx& c, const float* x, int64_t n, float* y) {
   c.mlpY.ensure((size_t)3 * n);
   JB_CHECK(hipMemcpyAsync(c.mlpX.p, x, 16 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
   tBegin(c, T_MLP);
-  hipLaunchKernelGGL(kMlpInfer, dim3(inferGrid(n)), dim3(256), 0, s, n, c.mlpX.p, c.mlpWb.p, c.mlpW.p, c.mlpNorm.p,
+  hipLaunchKernelGGL(kMlpInfer<false>, dim3(inferGrid(n)), dim3(256), 0, s, n, c.mlpX.p, c.mlpWb.p, c.mlpW.p, c.mlpNorm.p,
                      c.mlpY.p, nullptr, nullptr, nullptr);
   JB_CHECK(hipGetLastError());
   tEnd(c, T_MLP);
@@ -516,7 +668,7 @@ void mlpWarmStart(Ctx& c, int na) {
   if (!c.mlpReady) throw std::logic_error("no model loaded or trained");
   const int64_t n = 4 * (int64_t)na;
   tBegin(c, T_MLP);
-  hipLaunchKernelGGL(kMlpInfer, dim3(inferGrid(n)), dim3(256), 0, c.stream, n, c.features.p, c.mlpWb.p, c.mlpW.p,
+  hipLaunchKernelGGL(kMlpInfer<true>, dim3(inferGrid(n)), dim3(256), 0, c.stream, n, c.features.p, c.mlpWb.p, c.mlpW.p,
                      c.mlpNorm.p, nullptr, c.hdr.p, c.nrow.p, c.frow.p);
   JB_CHECK(hipGetLastError());
   tEnd(c, T_MLP);
