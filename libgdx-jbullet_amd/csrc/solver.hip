@@ -154,42 +154,55 @@ __global__ void kActiveBodies(int n, const int* __restrict__ list, const float* 
 
 // Jones-Plassmann rounds over the still uncoloured manifolds pend[0, m): a manifold wins a round when its
 // priority is the largest among the uncoloured manifolds of both its dynamic bodies, and then takes the
-// smallest colour free at both
+// smallest colour free at both.  Priorities carry the round in their top bits, so a later round's proposal
+// always replaces an earlier one under atomicMax and the body maxima need no reset between rounds.  The body
+// maxima are double-buffered by round parity (bodyPrio[(round & 1) * nb + body]), so that one launch can settle
+// round r (reading buffer r & 1, complete since the previous launch) and propose round r + 1 (into the other
+// buffer) for the manifolds still uncoloured: one launch per round instead of two, the same colouring.
+__device__ __forceinline__ void colourPropose(int k, int round, int2 b, const float* __restrict__ props, int nb,
+                                              unsigned long long* bodyPrio) {
+  unsigned long long* bp = bodyPrio + (size_t)(round & 1) * nb;
+  const unsigned long long p = prio(round, k);
+  if (dynamicBody(props, b.x)) atomicMax(bp + b.x, p);
+  if (dynamicBody(props, b.y)) atomicMax(bp + b.y, p);
+}
+
 __global__ void kColorPropose(int m, const int* __restrict__ pend, int round, const int2* __restrict__ ab,
-                              const float* __restrict__ props, const int* __restrict__ color,
+                              const float* __restrict__ props, const int* __restrict__ color, int nb,
                               unsigned long long* bodyPrio) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
   const int k = pend[t];
   if (color[k] >= 0) return;
-  int2 b = ab[k];
-  unsigned long long p = prio(round, k);
-  if (dynamicBody(props, b.x)) atomicMax(bodyPrio + b.x, p);
-  if (dynamicBody(props, b.y)) atomicMax(bodyPrio + b.y, p);
+  colourPropose(k, round, ab[k], props, nb, bodyPrio);
 }
 
-__global__ void kColorResolve(int m, const int* __restrict__ pend, int round, const int2* __restrict__ ab,
-                              const float* __restrict__ props, int* color, const unsigned long long* bodyPrio,
-                              unsigned long long* bodyMask) {
+// settle round `round`; with proposeNext the losers propose round + 1
+__global__ void kColorRound(int m, const int* __restrict__ pend, int round, bool proposeNext,
+                            const int2* __restrict__ ab, const float* __restrict__ props, int* color, int nb,
+                            unsigned long long* bodyPrio, unsigned long long* bodyMask) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = t < m ? pend[t] : 0;
-  if (t < m && color[k] < 0) {
-    int2 b = ab[k];
-    unsigned long long p = prio(round, k);
-    bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
-    bool win = (!da || bodyPrio[b.x] == p) && (!db || bodyPrio[b.y] == p);
-    if (win) {
-      unsigned long long used = (da ? bodyMask[b.x] : 0ull) | (db ? bodyMask[b.y] : 0ull);
-      int c;
-      if (used == ~0ull) {
-        c = 64 + round;
-      } else {
-        c = __ffsll((long long)~used) - 1;
-        if (da) bodyMask[b.x] |= (1ull << c);
-        if (db) bodyMask[b.y] |= (1ull << c);
-      }
-      color[k] = c;
+  if (t >= m) return;
+  const int k = pend[t];
+  if (color[k] >= 0) return;
+  const int2 b = ab[k];
+  const unsigned long long p = prio(round, k);
+  const unsigned long long* bp = bodyPrio + (size_t)(round & 1) * nb;
+  const bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
+  const bool win = (!da || bp[b.x] == p) && (!db || bp[b.y] == p);
+  if (win) {  // the round's winners share no dynamic body, so their mask updates do not race
+    unsigned long long used = (da ? bodyMask[b.x] : 0ull) | (db ? bodyMask[b.y] : 0ull);
+    int c;
+    if (used == ~0ull) {
+      c = 64 + round;
+    } else {
+      c = __ffsll((long long)~used) - 1;
+      if (da) bodyMask[b.x] |= (1ull << c);
+      if (db) bodyMask[b.y] |= (1ull << c);
     }
+    color[k] = c;
+  } else if (proposeNext) {
+    colourPropose(k, round + 1, b, props, nb, bodyPrio);
   }
 }
 
@@ -1072,12 +1085,12 @@ void solverSolve(Ctx& c, float dt) {
   c.colorOrder.ensure(na);
   c.colorOrderAlt.ensure(na);
   c.activeBodies.ensure(na);
-  c.bodyPrio.ensure(c.nb);
+  c.bodyPrio.ensure(2 * (size_t)c.nb);
   c.bodyMask.ensure(c.nb);
   tBegin(c, T_COLOR);
   hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
-  JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
+  JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, 2 * (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   // block partition of the dynamic bodies along this step's Morton order: interior / boundary manifolds
   const bool blocked = c.blockSize > 0 && c.sortedOrder != nullptr && c.np == c.nb;
@@ -1108,14 +1121,13 @@ void solverSolve(Ctx& c, float dt) {
   auto colourPass = [&](int m) {
     for (int batch = 0; m > 0; batch++, batches++) {
       const int rounds = batch == 0 ? firstRounds : 6;
-      for (int r = 0; r < rounds; r++, round++) {
-        hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                           (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p,
-                           c.bodyPrio.p);
-        hipLaunchKernelGGL(kColorResolve, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                           (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p,
-                           (const unsigned long long*)c.bodyPrio.p, c.bodyMask.p);
-      }
+      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                         (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p, c.nb,
+                         c.bodyPrio.p);
+      for (int r = 0; r < rounds; r++, round++)
+        hipLaunchKernelGGL(kColorRound, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                           r + 1 < rounds, (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p, c.nb,
+                           c.bodyPrio.p, c.bodyMask.p);
       hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p,
                          (const int*)c.color.p, c.pendFlag.p);
       size_t tbs = 0;
@@ -1147,7 +1159,7 @@ void solverSolve(Ctx& c, float dt) {
       JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.pendFlag.p, c.pend.p, (int*)c.counter.p, na, s));
       int m = 0;
       JB_CHECK(hipMemcpyAsync(&m, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
+      JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, 2 * (size_t)c.nb * sizeof(unsigned long long), s));
       JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
       JB_CHECK(hipStreamSynchronize(s));
       colourPass(m);
