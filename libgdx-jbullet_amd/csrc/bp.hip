@@ -451,6 +451,44 @@ __global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict
                                     __int_as_float(lst[1]));
 }
 
+// 4-wide record of internal node i from the binary records: the two children of each internal child and every
+// leaf child itself (2 to 4 entries), so a traversal step tests up to four boxes behind one dependent load and
+// the descent takes half as many steps
+__global__ __launch_bounds__(256) void kNodeRecords4(int n, const float4* __restrict__ rec, float4* __restrict__ rec4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const float4* R = rec + kRecF4 * i;
+  const float4 m = R[4];
+  int ch[4] = {0, 0, 0, 0}, lst[4] = {-1, -1, -1, -1};
+  float4* out = rec4 + kRec4F4 * i;
+  int e = 0;
+  for (int c = 0; c < 2; c++) {
+    const int id = __float_as_int(c == 0 ? m.x : m.y);
+    if (id < 0) {  // leaf child: itself
+      out[2 * e] = R[2 * c];
+      out[2 * e + 1] = R[2 * c + 1];
+      ch[e] = id;
+      lst[e] = __float_as_int(c == 0 ? m.z : m.w);
+      e++;
+    } else {  // internal child: its two children
+      const float4* Q = rec + kRecF4 * id;
+      const float4 q = Q[4];
+      for (int g = 0; g < 2; g++, e++) {
+        out[2 * e] = Q[2 * g];
+        out[2 * e + 1] = Q[2 * g + 1];
+        ch[e] = __float_as_int(g == 0 ? q.x : q.y);
+        lst[e] = __float_as_int(g == 0 ? q.z : q.w);
+      }
+    }
+  }
+  for (; e < 4; e++) {
+    out[2 * e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    out[2 * e + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  ((int4*)out)[8] = make_int4(ch[0], ch[1], ch[2], ch[3]);
+  ((int4*)out)[9] = make_int4(lst[0], lst[1], lst[2], lst[3]);
+}
+
 struct PairTest {
   int mode;
   const float* aabb;
@@ -511,11 +549,40 @@ __device__ __forceinline__ bool boxOverlap(const float4& amn, const float4& amx,
 // to the output behind one global reservation per block; only staging overflow takes the global counter.
 constexpr int kTraverseBlock = 256;
 constexpr int kStagePerBlock = 16 * kTraverseBlock;
-constexpr int kStackDepth = 64;  // traversal stack entries per thread
-constexpr int kLdsStack = 16;    // of which in LDS
+// traversal stack entries per thread: a 4-wide step pushes up to three entries and the 4-wide descent is half
+// as deep as the binary tree (whose depth the 30-bit keys and index tie-breaks keep far below 64)
+constexpr int kStackDepth = 96;
+constexpr int kLdsStack = 16;  // of which in LDS
+constexpr int kWidePairsPerProxy = 4;  // pair density from which the 4-wide traversal is used
+
+// record layouts of the two traversals: W entries per step, entry c's box at R[2c], R[2c + 1]
+template <int W>
+struct TravRec;
+template <>
+struct TravRec<2> {  // kNodeRecords: (child0, child1, last0, last1) in R[4]
+  static constexpr int F4 = kRecF4;
+  __device__ static void meta(const float4* R, int* ch, int* lst) {
+    const float4 m = R[4];
+    ch[0] = __float_as_int(m.x);
+    ch[1] = __float_as_int(m.y);
+    lst[0] = __float_as_int(m.z);
+    lst[1] = __float_as_int(m.w);
+  }
+};
+template <>
+struct TravRec<4> {  // kNodeRecords4: ids in R[8], last leaves in R[9]
+  static constexpr int F4 = kRec4F4;
+  __device__ static void meta(const float4* R, int* ch, int* lst) {
+    const int4 a = ((const int4*)R)[8], b = ((const int4*)R)[9];
+    ch[0] = a.x, ch[1] = a.y, ch[2] = a.z, ch[3] = a.w;
+    lst[0] = b.x, lst[1] = b.y, lst[2] = b.z, lst[3] = b.w;
+  }
+};
+
+template <int W>
 __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ order,
                                                             const float* __restrict__ leafBox,
-                                                            const float4* __restrict__ rec,
+                                                            const float4* __restrict__ recs,
                                                             const int4* __restrict__ leafExact, int mode, int bits,
                                                             unsigned long long* counter, uint64_t* out, int64_t cap,
                                                             int* err, uint64_t* __restrict__ stage) {
@@ -537,15 +604,17 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
     int deep[kStackDepth - kLdsStack];
     int sp = 0;
     int node = 0;
-    // descend into a hit child directly; push only the second of two hits
+    // descend into the first hit internal entry directly, push the other hit ones
     while (true) {
-      const float4* R = rec + kRecF4 * node;
-      const float4 m = R[4];
+      const float4* R = recs + TravRec<W>::F4 * node;
+      int chs[W], lsts[W];
+      TravRec<W>::meta(R, chs, lsts);
       int next = -1;
-      for (int c = 0; c < 2; c++) {
-        const int ch = __float_as_int(c == 0 ? m.x : m.y);
-        const int lst = __float_as_int(c == 0 ? m.z : m.w);
-        if (lst <= s) continue;  // only leaves after s pair with s
+#pragma unroll
+      for (int c = 0; c < W; c++) {
+        const int ch = chs[c];
+        const int lst = lsts[c];
+        if (lst <= s) continue;  // only leaves after s pair with s (unused entries have last leaf -1)
         const float4 bmn = R[2 * c], bmx = R[2 * c + 1];
         const bool worldIn = qw >= __float_as_int(bmn.w) && qw <= __float_as_int(bmx.w);
         if (!worldIn || !boxOverlap(qmn, qmx, bmn, bmx)) continue;
@@ -752,6 +821,7 @@ void ensureProxyBuffers(Ctx& c) {
   c.leafBox.ensure(8 * n);
   c.nodeBox.ensure(8 * n);
   c.nodeRec.ensure(4 * kRecF4 * (size_t)n);
+  c.nodeRec4.ensure(4 * kRec4F4 * (size_t)n);
   c.leafExact.ensure(8 * (size_t)n);
   c.nodeChild.ensure(2 * n);
   c.nodeLast.ensure(n);
@@ -934,6 +1004,10 @@ void bpFindPairs(Ctx& c) {
   const int n = c.np;
   const int bits = bitsFor((int64_t)n + 1);
   hipStream_t s = c.stream;
+  // 4-wide records halve the dependent steps of a traversal but cost ~400 B per node to build: worth it when the
+  // previous frame found several pairs per proxy (a settled pile: 10), not in sparse scenes (1M scattered boxes:
+  // 0.07), where the traversals are short.  Both traversals report the same pairs.
+  const bool wide = c.npairs >= kWidePairsPerProxy * (int64_t)n;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   // centre bounds and extent statistics (two-stage reduction)
   tBegin(c, T_BVH);
@@ -1016,6 +1090,9 @@ void bpFindPairs(Ctx& c) {
     hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)c.nodeChild.p,
                        (const int*)c.nodeLast.p, (const float*)c.leafBox.p, (const float*)c.nodeBox.p,
                        (float4*)c.nodeRec.p);
+    if (wide)
+      hipLaunchKernelGGL(kNodeRecords4, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const float4*)c.nodeRec.p,
+                         (float4*)c.nodeRec4.p);
   }
   tEnd(c, T_BVH);
   PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
@@ -1026,10 +1103,10 @@ void bpFindPairs(Ctx& c) {
     JB_CHECK(hipMemsetAsync(c.counter.p, 0, 2 * sizeof(unsigned long long), s));
     if (attempt == 0) tBegin(c, T_TRAVERSE);
     if (nn >= 2)
-      hipLaunchKernelGGL(kTraverse, dim3(gridFor(nn, kTraverseBlock)), dim3(kTraverseBlock), 0, s, nn, order,
-                         c.leafBox.p, (const float4*)c.nodeRec.p, (const int4*)c.leafExact.p, c.bpMode, bits,
-                         c.counter.p, c.rawKeys.p,
-                         c.pairCap, (int*)(c.counter.p + 2), c.pairStage.p);
+      hipLaunchKernelGGL(wide ? kTraverse<4> : kTraverse<2>, dim3(gridFor(nn, kTraverseBlock)), dim3(kTraverseBlock),
+                         0, s, nn, order, c.leafBox.p, (const float4*)(wide ? c.nodeRec4.p : c.nodeRec.p),
+                         (const int4*)c.leafExact.p, c.bpMode, bits, c.counter.p, c.rawKeys.p, c.pairCap,
+                         (int*)(c.counter.p + 2), c.pairStage.p);
     if (attempt == 0) tEnd(c, T_TRAVERSE);
     if (c.nHuge > 0)
       hipLaunchKernelGGL(kHuge, dim3(gridFor(n, kHugeBlock)), dim3(kHugeBlock), 0, s, n, (const int*)c.hugeRange.p,

@@ -81,6 +81,9 @@ struct Stats {
 
 // LBVH over the bodies' current shape AABBs for world queries (rayTest); same layout as the broadphase tree
 constexpr int kRecF4 = 5;  // float4 per traversal record: both children's boxes, then (child0, child1, last0, last1)
+// float4 per 4-wide broadphase record: the boxes of a node's grandchildren (a leaf child stands for itself), then
+// their ids and their last leaves as int4 (unused entries: last leaf -1)
+constexpr int kRec4F4 = 10;
 struct RayTree {
   DBuf<float> cand, aabb, leafBox, nodeBox, nodeRec, chunkBox;
   DBuf<int> isHuge, order, orderAlt, nodeChild, nodeFirst, nodeLast, hugeList, hugeListAlt, hugeWorld, hugeWorldAlt,
@@ -146,6 +149,7 @@ struct Ctx {
   DBuf<float> leafBox;   // sorted leaves np*8
   DBuf<float> nodeBox;   // internal nodes np*8
   DBuf<float> nodeRec;   // traversal records np*20: both children's boxes, indices and last leaves
+  DBuf<float> nodeRec4;  // 4-wide traversal records np*40 (kRec4F4 float4 per internal node)
   DBuf<int> hugeRange;   // per world: its range of the huge list
   RayTree rt;            // world queries
   DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
