@@ -58,6 +58,27 @@ struct DBuf {  // device buffer owned by a context; freed with it
   }
 };
 
+// pinned host buffer owned by a context: small device-to-host readbacks land here asynchronously (a copy into
+// pageable memory holds the calling thread until it is done, one wait per copy)
+template <typename T>
+struct HBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void ensure(size_t k) {  // callers synchronise before reusing the buffer
+    if (k <= n) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    JB_CHECK(hipHostMalloc((void**)&p, k * sizeof(T), hipHostMallocDefault));
+    n = k;
+  }
+};
+
 // EPA job workspace sizes (per queued pair)
 constexpr int kEpaFaces = 1024;
 constexpr int kEpaVerts = 300;
@@ -175,6 +196,15 @@ struct Ctx {
   DBuf<int> binRange;                        // per pair type: first sorted position and count
   unsigned long long npStatHost[3] = {0, 0, 0};  // their copy, valid after the step's final synchronisation
   bool npStatPending = false;
+  // the last solve's statistics: copied into pinned memory at its end, read after the step's final wait
+  HBuf<float> solveRo;                // residual, iterations
+  HBuf<double> solveRh;               // per-iteration (change, norm) sums
+  HBuf<unsigned long long> solveRows; // rows in total, rows of the per-colour or block sweeps
+  struct {
+    bool pending = false, tolMode = false, blockSweeps = false, sampleK = false;
+    int iters = 0;
+    size_t kUsed = 0;
+  } solveStat;
   DBuf<float> candC;  // contact candidates npairs*8
   DBuf<int> epaQueue;
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
@@ -185,6 +215,7 @@ struct Ctx {
   DBuf<float> nrow, frow;    // dense rows: normal row r, friction rows 2r and 2r+1
   DBuf<int2> activeBodies;   // per active manifold: body A, body B
   DBuf<int> colorStart;
+  HBuf<int> batchHost;  // readback of the colour batches: count, first slots, boundary flags
   // learned warm start (16-64-64-64-3 MLP, bf16 weights, fp32 accumulate)
   bool mlpReady = false;
   bool useMlp = false;       // the model predicts the warm-start impulses of every solve
@@ -289,6 +320,7 @@ void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, ui
 void npDispatchAll(Ctx& c);                              // A8-A16
 int64_t npCountContacts(Ctx& c);                         // contact points of the current manifolds
 void npCollectStats(Ctx& c);                             // EPA statistics of the last narrowphase
+void solverCollectStats(Ctx& c);                         // residual, rows and sweep timings of the last solve
 // solver
 void solverSolve(Ctx& c, float dt);                      // A18-A20
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)

@@ -50,23 +50,31 @@ __device__ __forceinline__ void putCand(float* out, int i, bool valid, V3 n, V3 
 }
 
 // ------------------------------------------------------------------ binning
+// sort key of a pair: its type (which kernel handles it) above the class of its two shape types, so that inside a
+// bin the lanes of a wave mostly run the same support functions.  Class: the other shape's type for a pair with
+// a plane, else lower convex type * 4 + higher (convex types are 0-3): 4 bits, one radix pass with 64 buckets
+constexpr int kBinClassBits = 4;
 __global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
-                         const int* __restrict__ act, int* tkey, int* idx) {
+                         const int* __restrict__ act, const float* __restrict__ shapes,
+                         const float* __restrict__ props, int* tkey, int* idx) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   int t = ptype[i];
   if (!(isActiveState(act[a]) || isActiveState(act[b]))) t = JBGPU_PAIR_NONE;  // needsCollision
-  tkey[i] = t;
+  const int sa = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
+  const int sb = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * b + 6));
+  const int cls = sa == JBGPU_SHAPE_PLANE ? sb : sb == JBGPU_SHAPE_PLANE ? sa : min(sa, sb) * 4 + max(sa, sb);
+  tkey[i] = (t << kBinClassBits) | (cls & ((1 << kBinClassBits) - 1));
   idx[i] = (int)i;
 }
 
-// first position of each pair type in the sorted type keys (types absent from the step keep -1)
+// first position of each pair type in the sorted keys (types absent from the step keep -1)
 __global__ void kBinStarts(int64_t n, const int* __restrict__ sorted, int* starts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int t = sorted[i];
-  if (i == 0 || sorted[i - 1] != t) starts[t] = (int)i;
+  const int t = sorted[i] >> kBinClassBits;
+  if (i == 0 || (sorted[i - 1] >> kBinClassBits) != t) starts[t] = (int)i;
 }
 
 // ------------------------------------------------------------------ sphere-sphere
@@ -1628,13 +1636,14 @@ void npDispatchAll(Ctx& c) {
   float* man = c.man[c.cur].p;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   tBegin(c, T_BIN);
-  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p, c.typeKey.p,
-                     c.perm.p);
+  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p,
+                     (const float*)c.shapes.p, (const float*)c.props.p, c.typeKey.p, c.perm.p);
   size_t tb = 0;
   hipcub::DoubleBuffer<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, 2, s));
+  const int keyBits = 2 + kBinClassBits;
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, keyBits, s));
   c.cubTmp.ensure(tb);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, 2, s));
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, keyBits, s));
   int* starts = (int*)(c.counter.p + 4);  // 4 ints
   JB_CHECK(hipMemsetAsync(starts, 0xff, 4 * sizeof(int), s));
   hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)kk.Current(), starts);

@@ -856,10 +856,12 @@ __global__ void kBatchFlags(int na, const unsigned long long* __restrict__ key, 
   if (p < na) flag[p] = (p == 0) || ((key[p] >> 20) != (key[p - 1] >> 20));
 }
 // 1 for the batches made of boundary slots
-__global__ void kBatchBoundary(int nbatch, const int* __restrict__ batchStart, const unsigned long long* __restrict__ key,
-                               int* out) {
+// (boundary bit, colour) batches of a solve: colour indices stay below 1024 (checked on the host)
+constexpr int kMaxBatches = 2048;
+__global__ void kBatchBoundary(const int* __restrict__ nbatch, const int* __restrict__ batchStart,
+                               const unsigned long long* __restrict__ key, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nbatch) out[i] = (int)((key[batchStart[i]] >> 52) & 1ull);
+  if (i < *nbatch) out[i] = (int)((key[batchStart[i]] >> 52) & 1ull);
 }
 // slot range of every (interior batch, block) run; batchStart holds the first slot of each batch
 __global__ void kRunBounds(int interiorEnd, int nBatchInterior, const int* __restrict__ batchStart, int nBlocks,
@@ -1183,9 +1185,10 @@ void solverSolve(Ctx& c, float dt) {
   if (dv.Current() != c.colorOrder.p)
     JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
   const unsigned long long* skey = dk.Current();
-  // batches: first slot of every (boundary bit, colour)
-  c.batchFlag.ensure(na);
-  c.colorStart.ensure(na + 1);
+  // batches: first slot of every (boundary bit, colour); count, first slots and boundary flags come back in one
+  // readback
+  c.batchFlag.ensure(na + (size_t)kMaxBatches);
+  c.colorStart.ensure(std::max<size_t>(na + 1, kMaxBatches));
   hipLaunchKernelGGL(kBatchFlags, dim3(gridFor(na)), dim3(256), 0, s, na, skey, c.batchFlag.p);
   {
     hipcub::CountingInputIterator<int> it0(0);
@@ -1195,17 +1198,19 @@ void solverSolve(Ctx& c, float dt) {
     JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na,
                                            s));
   }
-  int nbatch = 0;
-  JB_CHECK(hipMemcpyAsync(&nbatch, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
-  std::vector<int> starts(nbatch), boundary(nbatch);
-  c.batchFlag.ensure(na + (size_t)nbatch);
   int* bnd = c.batchFlag.p + na;
-  hipLaunchKernelGGL(kBatchBoundary, dim3(gridFor(nbatch)), dim3(256), 0, s, nbatch, (const int*)c.colorStart.p, skey,
-                     bnd);
-  JB_CHECK(hipMemcpyAsync(starts.data(), c.colorStart.p, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipMemcpyAsync(boundary.data(), bnd, nbatch * sizeof(int), hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(kBatchBoundary, dim3(kMaxBatches / 256), dim3(256), 0, s, (const int*)c.counter.p,
+                     (const int*)c.colorStart.p, skey, bnd);
+  c.batchHost.ensure(1 + 2 * (size_t)kMaxBatches);
+  int* hb = c.batchHost.p;
+  JB_CHECK(hipMemcpyAsync(hb, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(hb + 1, c.colorStart.p, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(hb + 1 + kMaxBatches, bnd, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
+  const int nbatch = hb[0];
+  if (nbatch < 1 || nbatch > kMaxBatches) throw std::runtime_error("colour batch count out of range");
+  const int* starts = hb + 1;
+  const int* boundary = hb + 1 + kMaxBatches;
   int nInterior = 0;
   for (int k = 0; k < nbatch; k++) {
     if (k > 0) c.colorOff.push_back(starts[k]);
@@ -1381,19 +1386,36 @@ void solverSolve(Ctx& c, float dt) {
        (int64_t)iters * 2 * (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)));
   hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
-  float ro[2] = {0, 0};
-  std::vector<double> rh(2 * (size_t)iters);
-  JB_CHECK(hipMemcpyAsync(ro, resOut, sizeof(ro), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipMemcpyAsync(rh.data(), c.resid.p, rh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  unsigned long long rowsHT[2] = {0, 0};
-  JB_CHECK(hipMemcpyAsync(rowsHT, c.rowCount.p, sizeof(rowsHT), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
+  // statistics: copied into pinned memory now, read by solverCollectStats after the step's final wait
+  c.solveRo.ensure(2);
+  c.solveRh.ensure(2 * (size_t)std::max(iters, 1));
+  c.solveRows.ensure(2);
+  JB_CHECK(hipMemcpyAsync(c.solveRo.p, resOut, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (iters > 0)
+    JB_CHECK(hipMemcpyAsync(c.solveRh.p, c.resid.p, 2 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(c.solveRows.p, c.rowCount.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  c.solveStat.pending = true;
+  c.solveStat.tolMode = tolMode;
+  c.solveStat.blockSweeps = blockSweeps;
+  c.solveStat.sampleK = sampleK;
+  c.solveStat.iters = iters;
+  c.solveStat.kUsed = kUsed;
+}
+
+void solverCollectStats(Ctx& c) {
+  if (!c.solveStat.pending) return;
+  c.solveStat.pending = false;
+  const bool tolMode = c.solveStat.tolMode, blockSweeps = c.solveStat.blockSweeps;
+  const int iters = c.solveStat.iters;
+  const float* ro = c.solveRo.p;
+  const double* rh = c.solveRh.p;
+  const unsigned long long* rowsHT = c.solveRows.p;
   c.st.rows = (int64_t)rowsHT[0];
   // rows swept per sweep by the per-colour launches (kinds 0, 1) and by the block launches (kinds 2, 3)
   const int64_t colourRows = blockSweeps ? (int64_t)(rowsHT[0] - rowsHT[1]) : (int64_t)rowsHT[1];
   const int64_t blockRows = blockSweeps ? (int64_t)rowsHT[1] : 0;
-  if (sampleK) {
-    for (size_t e = 0; e < kUsed; e += 2) {
+  if (c.solveStat.sampleK) {
+    for (size_t e = 0; e < c.solveStat.kUsed; e += 2) {
       float ms = 0.0f;
       JB_CHECK(hipEventElapsedTime(&ms, c.kEv[e], c.kEv[e + 1]));
       const int kind = c.kEvKind[e / 2];

@@ -327,6 +327,7 @@ void stepPhases(Ctx& c, float dt, int last) {
   tEnd(c, T_STEP);
   JB_CHECK(hipStreamSynchronize(s));
   npCollectStats(c);
+  solverCollectStats(c);
   if (c.timing) {
     tCollect(c);
     c.timedSteps++;
