@@ -179,6 +179,23 @@ def test_world_mixed_shapes_step_parity():
     ctx.close()
 
 
+@pytest.mark.parametrize("tail", [1 << 30, 0])
+def test_tail_sweep_paths_step_parity(tail):
+    """Resting box pyramids (four-point manifolds from the first step, ~6,000 manifolds): every colour in the
+    single-workgroup tail sweep (colours of up to ~1,000 manifolds, several passes each), and no tail at all."""
+    sc = scenes.pyramids(seed=7, n_boxes=3000, n_pyramids=6, n_spheres=0, area=60.0)
+    ctx, orc = _world_pair(sc, iterations=10)
+    ctx.solver_set_tail(tail)
+    for step in range(40):
+        _sync_and_step(ctx, orc)
+        st = ctx.stats()
+        if step % 10 == 0:
+            print(f"[tail {tail}] step {step}: manifolds {st['manifolds']} rows {st['rows']} colors {st['colors']}",
+                  flush=True)
+        _compare_world(ctx, orc, f"tail {tail} step {step}")
+    ctx.close()
+
+
 def test_batched_worlds_step_parity():
     """Independent worlds batched in one context (own floor plane each, pairs never cross worlds)."""
     sc = scenes.batched_towers(seed=3, n_worlds=24, lo=16, hi=120)
