@@ -370,6 +370,7 @@ def batched(args):
             "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
             round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
             "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
+            "last_step_epa_calls": st["epa_calls"],
             "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0}}
 
 

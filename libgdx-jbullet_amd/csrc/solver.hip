@@ -433,12 +433,11 @@ __global__ void kRowTotals(int na, int headEnd, const int* __restrict__ count, c
 }
 
 // warm-start impulses of colour c in row order: per point normal, lateral 1, lateral 2 (:659-747)
-__global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, const int4* __restrict__ hdr,
-                                                 const float* __restrict__ nrow, const float* __restrict__ frow,
-                                                 const float* __restrict__ props, float* svel) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt || !apply) return;
-  int p = off + k;
+// warm start of slot p: the applied impulses of its rows onto its two bodies (warm-start part of
+// solveGroupCacheFriendlySetup, point after point, normal row then the two friction rows)
+__device__ __forceinline__ void warmApplySlot(int p, const int4* __restrict__ hdr, const float* __restrict__ nrow,
+                                              const float* __restrict__ frow, const float* __restrict__ props,
+                                              float* svel) {
   int4 h = hdr[p];
   SBody A = loadBody(svel, props, h.y), B = loadBody(svel, props, h.z);
   const int rows = __popc(slotMask(h)), row0 = slotRow(h);
@@ -453,6 +452,28 @@ __global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, int apply, co
   }
   if (A.invMass != 0.0f) storeBody(svel, h.y, A);
   if (B.invMass != 0.0f) storeBody(svel, h.z, B);
+}
+
+__global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, const int4* __restrict__ hdr,
+                                                 const float* __restrict__ nrow, const float* __restrict__ frow,
+                                                 const float* __restrict__ props, float* svel) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < cnt) warmApplySlot(off + k, hdr, nrow, frow, props, svel);
+}
+
+// the small colours [c0, c1) in one workgroup, colour after colour (a barrier orders the workgroup's global
+// accesses: its waves share one CU and its L1)
+constexpr int kWarmTailThreads = 1024;
+__global__ __launch_bounds__(kWarmTailThreads) void kWarmApplyTail(int c0, int c1, const int* __restrict__ colorOff,
+                                                                   const int4* __restrict__ hdr,
+                                                                   const float* __restrict__ nrow,
+                                                                   const float* __restrict__ frow,
+                                                                   const float* __restrict__ props, float* svel) {
+  for (int c = c0; c < c1; c++) {
+    for (int p = colorOff[c] + (int)threadIdx.x; p < colorOff[c + 1]; p += kWarmTailThreads)
+      warmApplySlot(p, hdr, nrow, frow, props, svel);
+    __syncthreads();
+  }
 }
 
 // residual terms of one wave into one of kResidStripes (dd, nn) accumulators (res == nullptr: not tracked)
@@ -1280,13 +1301,20 @@ void solverSolve(Ctx& c, float dt) {
                      blockSweeps ? (int)c.interiorEnd : (int)c.colorOff[headColors], (const int*)c.slotCount.p,
                      (const int*)c.slotStart.p, c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
-  const int apply = (c.warmstart || c.useLearned || c.useMlp) ? 1 : 0;
-  for (int k = 0; k < c.ncolors; k++) {
-    int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-    hipLaunchKernelGGL(kWarmApply, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, apply, c.hdr.p,
-                       c.nrow.p, c.frow.p, c.props.p, c.svel.p);
+  const bool apply = c.warmstart || c.useLearned || c.useMlp;
+  int setupLaunches = 1;
+  if (apply) {  // warm start colour by colour: the head colours one launch each, the tail in one workgroup
+    for (int k = 0; k < headColors; k++) {
+      int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
+      hipLaunchKernelGGL(kWarmApply, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, c.hdr.p,
+                         c.nrow.p, c.frow.p, c.props.p, c.svel.p);
+    }
+    if (headColors < c.ncolors)
+      hipLaunchKernelGGL(kWarmApplyTail, dim3(1), dim3(kWarmTailThreads), 0, s, headColors, c.ncolors,
+                         (const int*)c.colorOffDev.p, c.hdr.p, c.nrow.p, c.frow.p, c.props.p, c.svel.p);
+    setupLaunches += headColors + (headColors < c.ncolors ? 1 : 0);
   }
-  tEnd(c, T_SOLVE_SETUP, c.ncolors + 1);
+  tEnd(c, T_SOLVE_SETUP, setupLaunches);
   // iterations
   // The residual is accumulated only where it is used: every iteration in tolerance mode, else the last one.
   const bool tolMode = c.tolerance > 0.0f;
