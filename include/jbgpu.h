@@ -62,6 +62,7 @@ typedef struct jbgpu_stats {
   float residual;       /* residual after the last iteration */
   int64_t huge_proxies; /* proxies handled outside the BVH (e.g. planes) */
   int64_t rows;         /* normal rows (contacts with distance <= 0) of the last solve */
+  int64_t epa_retries;  /* EPA jobs of the last pass that outgrew the first-pass workspace and ran again */
 } jbgpu_stats;
 
 /* ---- context ---------------------------------------------------------------- */

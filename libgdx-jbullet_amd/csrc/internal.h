@@ -85,6 +85,11 @@ constexpr int kEpaVerts = 300;
 constexpr int kEpaHe = 160;
 constexpr int kEpaFaceWords = 16;
 constexpr int kEpaSlotWords = kEpaFaces * kEpaFaceWords + kEpaVerts * 6 + kEpaHe * 4 + 8;
+// first-pass EPA workspace: small enough for several jobs per CU; jobs that outgrow it rerun with the large one
+constexpr int kEpaSmallFaces = 192;
+constexpr int kEpaSmallVerts = 96;
+constexpr int kEpaSmallSlotWords = kEpaSmallFaces * kEpaFaceWords + kEpaSmallVerts * 6 + kEpaHe * 4 + 8;
+constexpr int kEpaSmallGrid = 2048;  // first-pass EPA blocks (eight per CU)
 constexpr int kEpaGrid = 512;  // EPA blocks (LDS workspace limits them to two per CU)
 constexpr int kTimerRegions = 16;
 
@@ -95,6 +100,7 @@ constexpr int kManifoldRowWords = 4 * kContactRowWords + 4;  // + [valid mask, b
 
 struct Stats {
   int64_t pairs = 0, manifoldsActive = 0, contacts = 0, rows = 0, colors = 0, epaCalls = 0, epaFails = 0;
+  int64_t epaRetries = 0;
   int64_t iterations = 0;
   float residual = 0;
   int64_t huge = 0;
@@ -194,7 +200,7 @@ struct Ctx {
   DBuf<int> typeKey, typeKeyAlt, perm, permAlt;
   DBuf<unsigned long long> npStat;           // EPA queue length, failures, workspace overflows
   DBuf<int> binRange;                        // per pair type: first sorted position and count
-  unsigned long long npStatHost[3] = {0, 0, 0};  // their copy, valid after the step's final synchronisation
+  unsigned long long npStatHost[4] = {0, 0, 0, 0};  // their copy, valid after the step's final synchronisation
   bool npStatPending = false;
   // the last solve's statistics: copied into pinned memory at its end, read after the step's final wait
   HBuf<float> solveRo;                // residual, iterations
@@ -207,6 +213,7 @@ struct Ctx {
   } solveStat;
   DBuf<float> candC;  // contact candidates npairs*8
   DBuf<int> epaQueue;
+  DBuf<int> epaRetry;  // EPA jobs that outgrew the first-pass workspace
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
   // solver
   DBuf<float> rows;

@@ -610,12 +610,13 @@ struct Mkv {
   V3 w, r;
 };
 
-// workspace slot layout (floats): faces [kEpaFaces*16] | vertices [kEpaVerts*6] | he [kEpaHe*4]
+// workspace slot layout (floats): faces [capFace*16] | vertices [capVtx*6] | he [kEpaHe*4]
 struct EpaWork {
   float* face;
   float* vtx;
   float* he;
   int nface, nvtx, nhe;
+  int capFace, capVtx;  // capacities of this workspace
   bool overflow;
   // face fields: 0-2 v, 3-5 f, 6-8 e, 9-11 n, 12 d, 13 mark, 14 prev, 15 next
   __device__ int& fi(int f, int k) { return *reinterpret_cast<int*>(face + kEpaFaceWords * f + k); }
@@ -825,7 +826,7 @@ struct EpaSolver {
     return false;
   }
   __device__ __forceinline__ int addVertex(const Mkv& m) {
-    if (W.nvtx >= kEpaVerts) {
+    if (W.nvtx >= W.capVtx) {
       W.overflow = true;
       return -1;
     }
@@ -851,7 +852,7 @@ struct EpaSolver {
     return valid;
   }
   __device__ __forceinline__ int NewFace(int a, int b, int c) {
-    if (W.nface >= kEpaFaces) {
+    if (W.nface >= W.capFace) {
       W.overflow = true;
       return -1;
     }
@@ -1051,10 +1052,12 @@ struct EpaSolver {
 // one EPA job (pair i), run by all lanes of a wave on identical values: the polytope workspace (faces,
 // vertices, hash entries) and the solver's arrays are in LDS, the face search and hull support maps are
 // spread over the lanes; the GJK partial result is merged as in GjkPairDetector.java:263-303
+// retry != nullptr: a job that overflows this workspace writes nothing and is queued for the large one instead
+// (the algorithm does not depend on the capacities until it overflows, so the rerun gives the same result)
 __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                        const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
                        const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot,
-                       int* aux) {
+                       int* aux, int capFace, int capVtx, int* retry, unsigned long long* retryCount) {
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
   S.table = aux;
@@ -1071,9 +1074,11 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
   S.verts = verts;
   S.margin = 0.0f + EPA_accuracy;
   S.W.face = slot;
-  S.W.vtx = slot + kEpaFaces * kEpaFaceWords;
-  S.W.he = S.W.vtx + kEpaVerts * 6;
+  S.W.vtx = slot + capFace * kEpaFaceWords;
+  S.W.he = S.W.vtx + capVtx * 6;
   S.W.nface = S.W.nvtx = S.W.nhe = 0;
+  S.W.capFace = capFace;
+  S.W.capVtx = capVtx;
   S.W.overflow = false;
   const float* es = epaState + 12 * i;
   bool isValid = es[1] != 0.0f;
@@ -1090,6 +1095,10 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
     }
   }
   const bool lane0 = (threadIdx.x & 63) == 0;
+  if (S.W.overflow && retry) {
+    if (lane0) retry[atomicAdd(retryCount, 1ull)] = i;
+    return;
+  }
   if (S.W.overflow && lane0) atomicAdd(fails + 1, 1ull);
   if (ok) {
     V3 tn = sub(wB, wA);
@@ -1112,7 +1121,23 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
 }
 
 // EPA jobs in a fixed grid: every block (one wave) steps through the queue (its length read on the device)
-// until it is exhausted; all lanes run each job together, lane 0 writes its result
+// until it is exhausted; all lanes run each job together, lane 0 writes its result.  Every job first runs with
+// a small workspace (many jobs resident per CU); the few that outgrow it are queued and rerun by kEpa with the
+// large one.
+__global__ __launch_bounds__(64) void kEpaSmall(const unsigned long long* __restrict__ count,
+                                                const int* __restrict__ queue, const uint64_t* __restrict__ keys,
+                                                const float* __restrict__ shapes, const float* __restrict__ verts,
+                                                const float* __restrict__ props, const float* __restrict__ tr,
+                                                const float* __restrict__ epaState, float* cand,
+                                                unsigned long long* fails, int* retry,
+                                                unsigned long long* retryCount) {
+  __shared__ float slot[kEpaSmallSlotWords];
+  __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
+  const int jobs = (int)*count;
+  for (int k = blockIdx.x; k < jobs; k += gridDim.x)
+    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, kEpaSmallFaces,
+           kEpaSmallVerts, retry, retryCount);
+}
 __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict__ count, const int* __restrict__ queue,
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                                            const float* __restrict__ verts, const float* __restrict__ props,
@@ -1122,7 +1147,8 @@ __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict_
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
-    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux);
+    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, kEpaFaces, kEpaVerts,
+           nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------ manifold update
@@ -1622,7 +1648,7 @@ int64_t npCountContacts(Ctx& c) {
 void npDispatchAll(Ctx& c) {
   hipStream_t s = c.stream;
   const int64_t n = c.npairs;
-  c.st.epaCalls = c.st.epaFails = 0;
+  c.st.epaCalls = c.st.epaFails = c.st.epaRetries = 0;
   if (n == 0) return;
   c.typeKey.ensure(n);
   c.typeKeyAlt.ensure(n);
@@ -1666,9 +1692,13 @@ void npDispatchAll(Ctx& c) {
                      c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p, c.npStat.p);
   tEnd(c, T_GJK);
   tBegin(c, T_EPA);
-  hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
+  c.epaRetry.ensure(n);
+  hipLaunchKernelGGL(kEpaSmall, dim3(kEpaSmallGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
                      (const int*)c.epaQueue.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p,
-                     c.npStat.p + 1);
+                     c.npStat.p + 1, c.epaRetry.p, c.npStat.p + 3);
+  hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 3),
+                     (const int*)c.epaRetry.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
+                     c.candC.p, c.npStat.p + 1);
   tEnd(c, T_EPA);
   tBegin(c, T_MANIFOLD);
   hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
@@ -1683,6 +1713,7 @@ void npCollectStats(Ctx& c) {  // after the step's final synchronisation
   c.npStatPending = false;
   c.st.epaCalls = (int64_t)c.npStatHost[0];
   c.st.epaFails = (int64_t)c.npStatHost[1];
+  c.st.epaRetries = (int64_t)c.npStatHost[3];
   if (c.npStatHost[2]) c.err = "EPA workspace overflow";
 }
 

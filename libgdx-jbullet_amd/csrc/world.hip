@@ -817,6 +817,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
     out->residual = s.residual;
     out->huge_proxies = s.huge;
     out->rows = s.rows;
+    out->epa_retries = s.epaRetries;
   });
 }
 

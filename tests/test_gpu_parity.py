@@ -268,6 +268,38 @@ def test_world_mixed_pile_100k_parity():
     ctx.close()
 
 
+def test_epa_workspace_retry_step_parity():
+    """EPA jobs run first with a small LDS workspace; the few that outgrow it are rerun with the large one. The
+    100k pile has such jobs from step ~130 on (a few per ten steps): steps whose EPA pass had retries are
+    replayed by the oracle from the device state before them and must agree bit for bit."""
+    sc = scenes.mixed_pile(seed=1, n=100_000)
+    ctx, orc = _world_pair(sc, iterations=20)
+    ctx.step(n=140)
+    checked = 0
+    for step in range(141, 241):
+        b0 = ctx.bodies()
+        prox, s0 = ctx.bp_proxies()
+        k0, m0 = ctx.pairs()
+        ctx.step()
+        st = ctx.stats()
+        if st["epa_retries"] == 0:
+            continue
+        order, _ = ctx.solver_order()
+        orc.set_bodies(b0)
+        orc.set_proxies(prox, s0)
+        orc.set_pairs(k0, m0)
+        orc.set_order(order)
+        orc.step()
+        _compare_world(ctx, orc, f"retry step {step}")
+        print(f"[epa retry] step {step}: {st['epa_calls']} EPA jobs, {st['epa_retries']} rerun; bit-exact",
+              flush=True)
+        checked += 1
+        if checked == 2:
+            break
+    assert checked > 0, "no EPA job outgrew the first-pass workspace in steps 141-240"
+    ctx.close()
+
+
 def _rays(rng, n, lo, hi, vertical_share=0.3):
     """Ray segments between random points of the box [lo, hi]; a share of them vertical (axis-parallel)."""
     lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
