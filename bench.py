@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--pmc-warmup", type=int, default=300, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+        print(json.dumps({"error": "this library runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
