@@ -1054,10 +1054,11 @@ struct EpaSolver {
 // spread over the lanes; the GJK partial result is merged as in GjkPairDetector.java:263-303
 // retry != nullptr: a job that overflows this workspace writes nothing and is queued for the large one instead
 // (the algorithm does not depend on the capacities until it overflows, so the rerun gives the same result)
+template <int capFace, int capVtx>
 __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                        const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
                        const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot,
-                       int* aux, int capFace, int capVtx, int* retry, unsigned long long* retryCount) {
+                       int* aux, int* retry, unsigned long long* retryCount) {
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
   S.table = aux;
@@ -1135,8 +1136,8 @@ __global__ __launch_bounds__(64) void kEpaSmall(const unsigned long long* __rest
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
-    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, kEpaSmallFaces,
-           kEpaSmallVerts, retry, retryCount);
+    epaJob<kEpaSmallFaces, kEpaSmallVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux,
+                                           retry, retryCount);
 }
 __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict__ count, const int* __restrict__ queue,
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
@@ -1147,8 +1148,8 @@ __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict_
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
-    epaJob(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, kEpaFaces, kEpaVerts,
-           nullptr, nullptr);
+    epaJob<kEpaFaces, kEpaVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, nullptr,
+                                 nullptr);
 }
 
 // ------------------------------------------------------------------ manifold update
