@@ -1138,41 +1138,42 @@ void solverSolve(Ctx& c, float dt) {
   c.pendFlag.ensure(na);
   int round = 0;
   // the first batch's length adapts to the scene (one round shorter after a step whose first batch
-  // coloured everything, the rounds then needed after one that did not): usually one host wait per solve
+  // coloured everything, the rounds then needed after one that did not)
   const int firstRounds = std::max(12, c.colourRounds);
   int batches = 0;
+  int* remDev = (int*)(c.counter.p + 3);  // manifolds a batch left uncoloured
+  // one batch of rounds over pend[0, m): the manifolds still uncoloured are compacted into pendAlt and counted
+  // into *remDev, and the lists swap
+  auto colourBatch = [&](int m, int rounds) {
+    hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                       (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p, c.nb,
+                       c.bodyPrio.p);
+    for (int r = 0; r < rounds; r++, round++)
+      hipLaunchKernelGGL(kColorRound, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
+                         r + 1 < rounds, (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p, c.nb,
+                         c.bodyPrio.p, c.bodyMask.p);
+    hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, (const int*)c.color.p,
+                       c.pendFlag.p);
+    size_t tbs = 0;
+    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, remDev, m, s));
+    c.cubTmp.ensure(tbs);
+    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, remDev, m, s));
+    std::swap(c.pend.p, c.pendAlt.p);
+    std::swap(c.pend.n, c.pendAlt.n);
+    batches++;
+    if (round > 4000) throw std::runtime_error("colouring did not converge");
+  };
+  // batches until every manifold of pend[0, m) has a colour, with a host wait after each
   auto colourPass = [&](int m) {
-    for (int batch = 0; m > 0; batch++, batches++) {
-      const int rounds = batch == 0 ? firstRounds : 6;
-      hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                         (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p, c.nb,
-                         c.bodyPrio.p);
-      for (int r = 0; r < rounds; r++, round++)
-        hipLaunchKernelGGL(kColorRound, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
-                           r + 1 < rounds, (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p, c.nb,
-                           c.bodyPrio.p, c.bodyMask.p);
-      hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p,
-                         (const int*)c.color.p, c.pendFlag.p);
-      size_t tbs = 0;
-      JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, (int*)c.counter.p, m,
-                                             s));
-      c.cubTmp.ensure(tbs);
-      JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p,
-                                             (int*)c.counter.p, m, s));
+    for (int batch = 0; m > 0; batch++) {
+      colourBatch(m, batch == 0 ? firstRounds : 6);
       int rem = 0;
-      JB_CHECK(hipMemcpyAsync(&rem, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      JB_CHECK(hipMemcpyAsync(&rem, remDev, sizeof(int), hipMemcpyDeviceToHost, s));
       JB_CHECK(hipStreamSynchronize(s));
-      std::swap(c.pend.p, c.pendAlt.p);
-      std::swap(c.pend.n, c.pendAlt.n);
       m = rem;
-      if (round > 4000) throw std::runtime_error("colouring did not converge");
     }
   };
-  if (!blocked) {
-    hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
-    colourPass(na);
-    c.colourRounds = batches <= 1 ? firstRounds - 1 : round;
-  } else {
+  if (blocked) {
     for (int interior = 1; interior >= 0; interior--) {
       hipLaunchKernelGGL(kPassFlag, dim3(gridFor(na)), dim3(256), 0, s, na, phase, interior, c.pendFlag.p);
       hipcub::CountingInputIterator<int> it0(0);
@@ -1188,50 +1189,67 @@ void solverSolve(Ctx& c, float dt) {
       colourPass(m);
     }
   }
-  if (round >= (1 << 10) - 64) throw std::runtime_error("colour index exceeds the slot key");
-  // slot order: interior slots by (colour, block), then boundary slots by colour
+  // slot order and colour batches from the colours: interior slots by (colour, block), then boundary slots by
+  // colour; count, first slots and boundary flags come back in one readback, with the first colouring
+  // batch's uncoloured count (returned)
   c.slotKey.ensure(na);
   c.slotKeyAlt.ensure(na);
   c.permAlt.ensure(na);
-  hipLaunchKernelGGL(kSlotKeys, dim3(gridFor(na)), dim3(256), 0, s, na, phase, (const int*)c.color.p, c.slotKey.p,
-                     c.colorOrder.p);
-  hipcub::DoubleBuffer<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
-  hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
-  // key bits in use: without blocks only the colour (bits 20-29; stable sort, so equal keys keep slot order)
-  const int keyLo = blocked ? 0 : 20, keyHi = blocked ? 53 : 30;
-  size_t tb2 = 0;
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, keyLo, keyHi, s));
-  c.cubTmp.ensure(tb2);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, keyLo, keyHi, s));
-  if (dv.Current() != c.colorOrder.p)
-    JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
-  const unsigned long long* skey = dk.Current();
-  // batches: first slot of every (boundary bit, colour); count, first slots and boundary flags come back in one
-  // readback
   c.batchFlag.ensure(na + (size_t)kMaxBatches);
   c.colorStart.ensure(std::max<size_t>(na + 1, kMaxBatches));
-  hipLaunchKernelGGL(kBatchFlags, dim3(gridFor(na)), dim3(256), 0, s, na, skey, c.batchFlag.p);
-  {
+  c.batchHost.ensure(2 + 2 * (size_t)kMaxBatches);
+  int* hb = c.batchHost.p;
+  const unsigned long long* skey = nullptr;  // the sorted slot keys
+  auto order = [&]() {
+    if (round >= (1 << 10) - 64) throw std::runtime_error("colour index exceeds the slot key");
+    hipLaunchKernelGGL(kSlotKeys, dim3(gridFor(na)), dim3(256), 0, s, na, phase, (const int*)c.color.p, c.slotKey.p,
+                       c.colorOrder.p);
+    hipcub::DoubleBuffer<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
+    hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
+    // key bits in use: without blocks only the colour (bits 20-29; stable sort, so equal keys keep slot order)
+    const int keyLo = blocked ? 0 : 20, keyHi = blocked ? 53 : 30;
+    size_t tb2 = 0;
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, keyLo, keyHi, s));
+    c.cubTmp.ensure(tb2);
+    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, keyLo, keyHi, s));
+    if (dv.Current() != c.colorOrder.p)
+      JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
+    skey = dk.Current();
+    hipLaunchKernelGGL(kBatchFlags, dim3(gridFor(na)), dim3(256), 0, s, na, skey, c.batchFlag.p);
     hipcub::CountingInputIterator<int> it0(0);
     size_t tbs = 0;
     JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na, s));
     c.cubTmp.ensure(tbs);
     JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na,
                                            s));
+    int* bnd = c.batchFlag.p + na;
+    hipLaunchKernelGGL(kBatchBoundary, dim3(kMaxBatches / 256), dim3(256), 0, s, (const int*)c.counter.p,
+                       (const int*)c.colorStart.p, skey, bnd);
+    JB_CHECK(hipMemcpyAsync(hb, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(hb + 1, remDev, sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(hb + 2, c.colorStart.p, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(hb + 2 + kMaxBatches, bnd, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
+    return hb[1];
+  };
+  if (!blocked) {
+    // no host wait after the first batch: its uncoloured count comes back with the batches; in the rare case
+    // that it is not zero, the remaining batches run (waiting after each) and the order is made again
+    hipLaunchKernelGGL(kIota, dim3(gridFor(na)), dim3(256), 0, s, na, c.pend.p);
+    colourBatch(na, firstRounds);
+    const int rem = order();
+    if (rem > 0) {
+      colourPass(rem);
+      order();
+    }
+    c.colourRounds = batches <= 1 ? firstRounds - 1 : round;
+  } else {
+    order();
   }
-  int* bnd = c.batchFlag.p + na;
-  hipLaunchKernelGGL(kBatchBoundary, dim3(kMaxBatches / 256), dim3(256), 0, s, (const int*)c.counter.p,
-                     (const int*)c.colorStart.p, skey, bnd);
-  c.batchHost.ensure(1 + 2 * (size_t)kMaxBatches);
-  int* hb = c.batchHost.p;
-  JB_CHECK(hipMemcpyAsync(hb, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipMemcpyAsync(hb + 1, c.colorStart.p, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipMemcpyAsync(hb + 1 + kMaxBatches, bnd, kMaxBatches * sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
   const int nbatch = hb[0];
   if (nbatch < 1 || nbatch > kMaxBatches) throw std::runtime_error("colour batch count out of range");
-  const int* starts = hb + 1;
-  const int* boundary = hb + 1 + kMaxBatches;
+  const int* starts = hb + 2;
+  const int* boundary = hb + 2 + kMaxBatches;
   int nInterior = 0;
   for (int k = 0; k < nbatch; k++) {
     if (k > 0) c.colorOff.push_back(starts[k]);
