@@ -609,19 +609,31 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
       const float4* R = recs + TravRec<W>::F4 * node;
       int chs[W], lsts[W];
       TravRec<W>::meta(R, chs, lsts);
-      int next = -1;
+      // box tests of all entries first; then the exact-test records of the hit leaves, loaded together (one
+      // round trip for up to W leaves instead of one per leaf), then the exact tests; then the descent
+      bool hit[W];
+      bool anyLeaf = false;
 #pragma unroll
       for (int c = 0; c < W; c++) {
-        const int ch = chs[c];
-        const int lst = lsts[c];
-        if (lst <= s) continue;  // only leaves after s pair with s (unused entries have last leaf -1)
         const float4 bmn = R[2 * c], bmx = R[2 * c + 1];
         const bool worldIn = qw >= __float_as_int(bmn.w) && qw <= __float_as_int(bmx.w);
-        if (!worldIn || !boxOverlap(qmn, qmx, bmn, bmx)) continue;
-        if (ch < 0) {
-          const int4 e0 = leafExact[2 * (-ch - 1)], e1 = leafExact[2 * (-ch - 1) + 1];
-          const int pb = e1.w;
-          if (leafExactOverlap(mode, qe0, qe1, e0, e1)) {
+        // only leaves after s pair with s (unused entries have last leaf -1)
+        hit[c] = lsts[c] > s && worldIn && boxOverlap(qmn, qmx, bmn, bmx);
+        anyLeaf |= hit[c] && chs[c] < 0;
+      }
+      if (anyLeaf) {
+        int4 e0[W], e1[W];
+#pragma unroll
+        for (int c = 0; c < W; c++) {  // entries that are not hit leaves read leaf 0's record, unused
+          const int li = hit[c] && chs[c] < 0 ? -chs[c] - 1 : 0;
+          e0[c] = leafExact[2 * li];
+          e1[c] = leafExact[2 * li + 1];
+        }
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+          if (!(hit[c] && chs[c] < 0)) continue;
+          const int pb = e1[c].w;
+          if (leafExactOverlap(mode, qe0, qe1, e0[c], e1[c])) {
             const uint64_t lo = (uint64_t)min(pa, pb) + 1, hi = (uint64_t)max(pa, pb) + 1;
             const unsigned slot = atomicAdd(&sCount, 1u);
             if (slot < (unsigned)kStagePerBlock)
@@ -629,7 +641,14 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
             else
               emit(pa, pb, bits, counter, out, cap);
           }
-        } else if (next < 0) {
+        }
+      }
+      int next = -1;
+#pragma unroll
+      for (int c = 0; c < W; c++) {
+        const int ch = chs[c];
+        if (!hit[c] || ch < 0) continue;
+        if (next < 0) {
           next = ch;
         } else if (sp < kLdsStack) {
           sStack[sp++][threadIdx.x] = ch;
