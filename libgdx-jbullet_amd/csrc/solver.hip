@@ -331,6 +331,13 @@ __global__ void kSlotRows(int na, const int* __restrict__ corder, const int* __r
   count[p] = k;
 }
 
+// a 20-word row from registers to its 16-byte-aligned place (rows are 80 B)
+__device__ __forceinline__ void storeRow(float* dst, const float* r) {
+#pragma unroll
+  for (int k = 0; k < RW / 4; k++)
+    reinterpret_cast<float4*>(dst)[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
 // rows of every slot, stored densely from the slot's first row (rowStart, exclusive prefix sum of the
 // slot row counts): normal row r at nrow[r], its two friction rows at frow[2r], frow[2r+1]
 // (solveGroupCacheFriendlySetup :494-747 without the impulse application); the warm-start impulse of each
@@ -355,7 +362,8 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
     float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
     if (!(P[15] <= 0.0f)) continue;
     mask |= 1 << j;
-    float* N = nrow + (size_t)RW * row;
+    float N[RW];  // the row is assembled in registers and stored as five 16-byte words (scalar stores of
+                  // neighbouring slots' rows, 80 B apart, would touch a cache line per lane each)
     V3 normal = ld3(P + 12);
     V3 rp1 = sub(ld3(P + 6), o0);
     V3 rp2 = sub(ld3(P + 9), o1);
@@ -392,6 +400,7 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       F[12] = P[18]; F[13] = P[19]; F[14] = P[20]; F[15] = friction;
     }
     N[19] = SP.useLearned ? learned[3 * slot] : (SP.warm ? P[18] * SP.warmFactor : 0.0f);
+    storeRow(nrow + (size_t)RW * row, N);
     // lateral friction directions (:678-708)
     if (ldi(P + 27) == 0) {
       V3 d1v = sub(vv, scl(normal, rel_vel));
@@ -407,9 +416,7 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       st3(P + 24, d2v);
       sti(P + 27, 1);
     }
-    float* F1 = frow + (size_t)RW * (2 * row);
-    float* F2 = frow + (size_t)RW * (2 * row + 1);
-    row++;
+    float F1[RW], F2[RW];
     frictionRow(F1, ld3(P + 21), rp1, rp2, I0, I1, im0, im1, friction);
     frictionRow(F2, ld3(P + 24), rp1, rp2, I0, I1, im0, im1, friction);
     if (SP.useLearned) {
@@ -419,6 +426,9 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       F1[19] = P[19] * SP.warmFactor;
       F2[19] = P[20] * SP.warmFactor;
     }
+    storeRow(frow + (size_t)RW * (2 * row), F1);
+    storeRow(frow + (size_t)RW * (2 * row + 1), F2);
+    row++;
   }
   hdr[p] = make_int4(mask | (row0 << 4), b0, b1, pi);
 }
