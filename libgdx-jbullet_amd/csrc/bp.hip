@@ -768,10 +768,11 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
                        int64_t nOld, const uint64_t* __restrict__ keysOld, const float* __restrict__ manOld,
                        const float* __restrict__ shapes, const float* __restrict__ props, int* ptype) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+  const bool live = i < np;  // every lane stays to the end: the point copy below is done by the whole wave
+  if (!live) i = np - 1;
   uint64_t p = packed[i];
   uint64_t key = ((p >> bits) << 32) | (p & ((1ull << bits) - 1ull));
-  keysNew[i] = key;
+  if (live) keysNew[i] = key;
   int a = (int)(key >> 32) - 1, b = (int)(key & 0xffffffffu) - 1;
   int ta = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
   int tb = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * b + 6));
@@ -780,26 +781,41 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
   else if (ta == JBGPU_SHAPE_PLANE && tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_NONE;
   else if (ta == JBGPU_SHAPE_PLANE || tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_CONVEX_PLANE;
   else type = JBGPU_PAIR_CONVEX_CONVEX;
-  ptype[i] = type;
   int64_t lo = 0, hi = nOld;
   while (lo < hi) {
     int64_t mid = (lo + hi) >> 1;
     if (keysOld[mid] < key) lo = mid + 1; else hi = mid;
   }
-  const bool found = lo < nOld && keysOld[lo] == key;
+  const bool found = live && lo < nOld && keysOld[lo] == key;
   // only the header and the live points travel (slots past the point count are never read)
   float4* dst = (float4*)(manNew + (size_t)JBGPU_MANIFOLD_WORDS * i);
+  int nf4 = 0;  // 16-byte words of live points behind the header
   if (found) {
-    const float4* src = (const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * lo);
-    float4 h = src[0];
+    const float4 h = ((const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * lo))[0];
     dst[0] = h;
-    int words = JBGPU_POINT_WORDS * min(max(__builtin_bit_cast(int, h.x), 0), 4);
-    for (int k = 0; k < words / 4; k++) dst[1 + k] = src[1 + k];
-  } else {
+    nf4 = JBGPU_POINT_WORDS / 4 * min(max(__builtin_bit_cast(int, h.x), 0), 4);
+  } else if (live) {
     int b0 = a, b1 = b;
     if (type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE) { b0 = b; b1 = a; }
     dst[0] = make_float4(__builtin_bit_cast(float, 0), __builtin_bit_cast(float, b0), __builtin_bit_cast(float, b1),
                          __builtin_bit_cast(float, type));
+  }
+  if (live) ptype[i] = type;
+  // the points of the wave's carried records, two records per pass (lanes 0-31 and 32-63, 512 B each), so every
+  // load and store instruction covers contiguous memory instead of 64 records 528 B apart
+  uint64_t todo = __ballot(nf4 > 0);
+  const int lane = threadIdx.x & 63, k = lane & 31;
+  while (todo) {
+    const int r0 = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const int r1 = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+    if (r1 >= 0) todo &= todo - 1;
+    const int r = lane < 32 ? r0 : r1;
+    const int n = __shfl(nf4, r < 0 ? r0 : r);
+    const long long so = __shfl((long long)lo, r < 0 ? r0 : r), dn = __shfl((long long)i, r < 0 ? r0 : r);
+    if (r >= 0 && k < n)
+      ((float4*)(manNew + (size_t)JBGPU_MANIFOLD_WORDS * dn))[1 + k] =
+          ((const float4*)(manOld + (size_t)JBGPU_MANIFOLD_WORDS * so))[1 + k];
   }
 }
 
