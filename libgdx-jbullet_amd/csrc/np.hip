@@ -1271,27 +1271,34 @@ __global__ void kManifold(int64_t npairs, const uint64_t* __restrict__ keys, con
   // refreshContactPoints (PersistentManifold.java:312-372) if the manifold has points
   int n = ldi(m);
   if (n == 0) return;
+  // a point in 16-byte words (records and points are 16-byte aligned): [0] localA, localB.x; [1] localB.yz,
+  // worldA.xy; [2] worldA.z, worldB; [3] normal, distance; [7] lifetime and three spare words. The keep/drop
+  // test of point k reads only its own refreshed values, and removing a point k moves a point of a higher index
+  // (already tested) into k, so the tests are taken in the refresh pass and the removals follow, from the top
+  unsigned int drop = 0;
   for (int k = n - 1; k >= 0; k--) {
-    float* Q = P + JBGPU_POINT_WORDS * k;
-    V3 wa = xform(t0, ld3(Q + 0));
-    V3 wb = xform(t1, ld3(Q + 3));
-    st3(Q + 6, wa);
-    st3(Q + 9, wb);
-    Q[15] = dot(sub(wa, wb), ld3(Q + 12));
-    sti(Q + 28, ldi(Q + 28) + 1);
-  }
-  for (int k = n - 1; k >= 0; k--) {
-    float* Q = P + JBGPU_POINT_WORDS * k;
-    float d = Q[15];
+    float4* Q = reinterpret_cast<float4*>(P + JBGPU_POINT_WORDS * k);
+    const float4 q0 = Q[0], q1 = Q[1], q3 = Q[3], q7 = Q[7];
+    const V3 wa = xform(t0, v3(q0.x, q0.y, q0.z));
+    const V3 wb = xform(t1, v3(q0.w, q1.x, q1.y));
+    const V3 nrm = v3(q3.x, q3.y, q3.z);
+    const float d = dot(sub(wa, wb), nrm);
+    Q[1] = make_float4(q1.x, q1.y, wa.x, wa.y);
+    Q[2] = make_float4(wa.z, wb.x, wb.y, wb.z);
+    Q[3] = make_float4(q3.x, q3.y, q3.z, d);
+    Q[7] = make_float4(__builtin_bit_cast(float, __builtin_bit_cast(int, q7.x) + 1), q7.y, q7.z, q7.w);
+    bool remove;
     if (!(d <= kContactThreshold)) {
-      removePoint(m, k);
+      remove = true;
     } else {
-      V3 proj = sub(ld3(Q + 6), scl(ld3(Q + 12), d));
-      V3 pd = sub(ld3(Q + 9), proj);
-      float d2 = dot(pd, pd);
-      if (d2 > kContactThreshold * kContactThreshold) removePoint(m, k);
+      const V3 proj = sub(wa, scl(nrm, d));
+      const V3 pd = sub(wb, proj);
+      remove = dot(pd, pd) > kContactThreshold * kContactThreshold;
     }
+    if (remove) drop |= 1u << k;
   }
+  for (int k = n - 1; k >= 0; k--)
+    if ((drop >> k) & 1u) removePoint(m, k);
 }
 
 // contact points over all manifolds (per-block sums, one atomic per block); run only when statistics are read
