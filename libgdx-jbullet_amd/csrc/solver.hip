@@ -359,14 +359,19 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
   float im0 = props[8 * b0], im1 = props[8 * b1];
   int mask = 0;
   for (int j = 0; j < n; j++) {
-    float* P = m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j;
-    if (!(P[15] <= 0.0f)) continue;
+    // the point in 16-byte words: [1] localB.yz, worldA.xy; [2] worldA.z, worldB; [3] normal, distance;
+    // [4] friction, restitution, impulse, lateral impulse 1; [5] lateral impulse 2, lateral direction 1;
+    // [6] lateral direction 2, the directions' flag
+    float4* P = reinterpret_cast<float4*>(m + JBGPU_MANIFOLD_HDR + JBGPU_POINT_WORDS * j);
+    const float4 q3 = P[3];
+    if (!(q3.w <= 0.0f)) continue;
+    const float4 q1 = P[1], q2 = P[2], q4 = P[4], q5 = P[5], q6 = P[6];
     mask |= 1 << j;
     float N[RW];  // the row is assembled in registers and stored as five 16-byte words (scalar stores of
                   // neighbouring slots' rows, 80 B apart, would touch a cache line per lane each)
-    V3 normal = ld3(P + 12);
-    V3 rp1 = sub(ld3(P + 6), o0);
-    V3 rp2 = sub(ld3(P + 9), o1);
+    V3 normal = v3(q3.x, q3.y, q3.z);
+    V3 rp1 = sub(v3(q1.z, q1.w, q2.x), o0);
+    V3 rp2 = sub(v3(q2.y, q2.z, q2.w), o1);
     V3 angA = mulMV(I0, crs(rp1, normal));
     V3 angB = mulMV(I1, crs(rp2, normal));
     float d0 = im0 + dot(normal, crs(angA, rp1));
@@ -381,9 +386,9 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
     V3 vel2 = add(crs(av1, rp2), lv1);
     V3 vv = sub(vel1, vel2);
     float rel_vel = dot(normal, vv);
-    float pen = jmin(P[15] + SP.linearSlop, 0.0f);
-    float friction = P[16];
-    float rest = P[17] * -rel_vel;
+    float pen = jmin(q3.w + SP.linearSlop, 0.0f);
+    float friction = q4.x;
+    float rest = q4.y * -rel_vel;
     if (rest <= 0.0f) rest = 0.0f;
     float penVel = -pen / SP.dt;
     if (rest > penVel) pen = 0.0f;
@@ -396,13 +401,14 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       F[0] = normal.x; F[1] = normal.y; F[2] = normal.z;
       F[3] = vv.x; F[4] = vv.y; F[5] = vv.z;
       F[6] = av0.x - av1.x; F[7] = av0.y - av1.y; F[8] = av0.z - av1.z;
-      F[9] = P[15]; F[10] = im0; F[11] = im1;
-      F[12] = P[18]; F[13] = P[19]; F[14] = P[20]; F[15] = friction;
+      F[9] = q3.w; F[10] = im0; F[11] = im1;
+      F[12] = q4.z; F[13] = q4.w; F[14] = q5.x; F[15] = friction;
     }
-    N[19] = SP.useLearned ? learned[3 * slot] : (SP.warm ? P[18] * SP.warmFactor : 0.0f);
+    N[19] = SP.useLearned ? learned[3 * slot] : (SP.warm ? q4.z * SP.warmFactor : 0.0f);
     storeRow(nrow + (size_t)RW * row, N);
     // lateral friction directions (:678-708)
-    if (ldi(P + 27) == 0) {
+    V3 dir1 = v3(q5.y, q5.z, q5.w), dir2 = v3(q6.x, q6.y, q6.z);
+    if (__builtin_bit_cast(int, q6.w) == 0) {
       V3 d1v = sub(vv, scl(normal, rel_vel));
       float lat = len2(d1v);
       V3 d2v;
@@ -412,19 +418,20 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
       } else {
         planeSpace1(normal, d1v, d2v);
       }
-      st3(P + 21, d1v);
-      st3(P + 24, d2v);
-      sti(P + 27, 1);
+      P[5] = make_float4(q5.x, d1v.x, d1v.y, d1v.z);
+      P[6] = make_float4(d2v.x, d2v.y, d2v.z, __builtin_bit_cast(float, 1));
+      dir1 = d1v;
+      dir2 = d2v;
     }
     float F1[RW], F2[RW];
-    frictionRow(F1, ld3(P + 21), rp1, rp2, I0, I1, im0, im1, friction);
-    frictionRow(F2, ld3(P + 24), rp1, rp2, I0, I1, im0, im1, friction);
+    frictionRow(F1, dir1, rp1, rp2, I0, I1, im0, im1, friction);
+    frictionRow(F2, dir2, rp1, rp2, I0, I1, im0, im1, friction);
     if (SP.useLearned) {
       F1[19] = learned[3 * slot + 1];
       F2[19] = learned[3 * slot + 2];
     } else if (SP.warm) {
-      F1[19] = P[19] * SP.warmFactor;
-      F2[19] = P[20] * SP.warmFactor;
+      F1[19] = q4.w * SP.warmFactor;
+      F2[19] = q5.x * SP.warmFactor;
     }
     storeRow(frow + (size_t)RW * (2 * row), F1);
     storeRow(frow + (size_t)RW * (2 * row + 1), F2);
