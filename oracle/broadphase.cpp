@@ -162,7 +162,8 @@ void Broadphase::pairsBrute(std::vector<uint64_t>& out) const {
       if (overlap(i, j) && filterPass(proxies[i], proxies[j])) out.push_back(pairKey(i + 1, j + 1));
 }
 
-struct CellEntry {
+struct CellEntry {  // proxies of different worlds never pair, so the grid is per world
+  int world;
   uint64_t cell;
   int idx;
 };
@@ -212,10 +213,24 @@ void Broadphase::pairs(std::vector<uint64_t>& out) const {
            cz = ((double)box[i].mx.z - box[i].mn.z) / cell + 1;
     if (cx * cy * cz > 4096.0) huge[i] = 1;
   }
-  // huge proxies (e.g. StaticPlaneShape's +-1e30 box) are tested against everyone
+  // huge proxies (e.g. StaticPlaneShape's +-1e30 box) are tested against every proxy of their world
+  std::vector<int> byWorld(n);
+  for (int i = 0; i < n; i++) byWorld[i] = i;
+  std::stable_sort(byWorld.begin(), byWorld.end(),
+                   [&](int a, int b) { return proxies[a].world < proxies[b].world; });
+  std::vector<int> worldStart(n), worldEnd(n);  // per proxy: its world's range in byWorld
+  for (int s0 = 0, e0 = 0; s0 < n; s0 = e0) {
+    for (e0 = s0 + 1; e0 < n && proxies[byWorld[e0]].world == proxies[byWorld[s0]].world; e0++) {
+    }
+    for (int k = s0; k < e0; k++) {
+      worldStart[byWorld[k]] = s0;
+      worldEnd[byWorld[k]] = e0;
+    }
+  }
   for (int i = 0; i < n; i++) {
     if (!huge[i]) continue;
-    for (int j = 0; j < n; j++) {
+    for (int k = worldStart[i]; k < worldEnd[i]; k++) {
+      const int j = byWorld[k];
       if (j == i || (huge[j] && j < i)) continue;  // huge-huge pairs once
       int a = std::min(i, j), b = std::max(i, j);
       if (overlap(a, b) && filterPass(proxies[a], proxies[b])) out.push_back(pairKey(a + 1, b + 1));
@@ -233,15 +248,17 @@ void Broadphase::pairs(std::vector<uint64_t>& out) const {
     int64_t z0 = cidx(box[i].mn.z), z1 = cidx(box[i].mx.z);
     for (int64_t x = x0; x <= x1; x++)
       for (int64_t y = y0; y <= y1; y++)
-        for (int64_t z = z0; z <= z1; z++) ent.push_back({((uint64_t)x << 42) | ((uint64_t)y << 21) | (uint64_t)z, i});
+        for (int64_t z = z0; z <= z1; z++)
+          ent.push_back({proxies[i].world, ((uint64_t)x << 42) | ((uint64_t)y << 21) | (uint64_t)z, i});
   }
   std::sort(ent.begin(), ent.end(), [](const CellEntry& a, const CellEntry& b) {
+    if (a.world != b.world) return a.world < b.world;
     return a.cell < b.cell || (a.cell == b.cell && a.idx < b.idx);
   });
   size_t s = 0;
   while (s < ent.size()) {
     size_t e = s + 1;
-    while (e < ent.size() && ent[e].cell == ent[s].cell) e++;
+    while (e < ent.size() && ent[e].cell == ent[s].cell && ent[e].world == ent[s].world) e++;
     for (size_t u = s; u < e; u++) {
       for (size_t v = u + 1; v < e; v++) {
         int a = ent[u].idx, b = ent[v].idx;  // a < b by sort order

@@ -2,6 +2,7 @@
 // (through ctypes) and by bench.py's cpu_baseline leg.  Record layouts are the
 // ones documented in include/jbgpu_format.h.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -402,6 +403,21 @@ int orc_world_step_until(void* h, float dt, int phase) {
   w->stepCount++;
   return 0;
 }
+// one phase of the step alone: 0 predictUnconstraintMotion, 1 updateAabbs + calculateOverlappingPairs,
+// 2 dispatchAllCollisionPairs, 3 calculateSimulationIslands + solveConstraints, 4 integrateTransforms +
+// updateActivationState (DiscreteDynamicsWorld.java:356-400)
+int orc_world_phase(void* h, float dt, int phase) {
+  World* w = (World*)h;
+  switch (phase) {
+    case 0: w->predictUnconstraintMotion(dt); break;
+    case 1: w->updateAabbs(); w->calculateOverlappingPairs(); break;
+    case 2: w->dispatchAllCollisionPairs(); break;
+    case 3: w->calculateSimulationIslands(); w->solveConstraints(); break;
+    case 4: w->integrateTransforms(dt); w->updateActivationState(dt); w->stepCount++; break;
+    default: return -1;
+  }
+  return 0;
+}
 void orc_world_get_bodies(void* h, float* out) {
   World* w = (World*)h;
   for (size_t i = 0; i < w->bodies.size(); i++) bodyToRecord(w->bodies[i], out + i * JBGPU_BODY_WORDS);
@@ -445,7 +461,8 @@ void orc_world_set_proxies(void* h, const float* rec, int stageCurrent) {
   w->bp.stageCurrent = stageCurrent;
 }
 int orc_world_stage_current(void* h) { return ((World*)h)->bp.stageCurrent; }
-// stats: [iterations, residual, rows, frictionRows, gjkIters, epaCalls, epaFails]
+// stats: [iterations, residual, rows, frictionRows, gjkIters, epaCalls, epaFails, step residual]; the step
+// residual is ||dl|| / max(1, ||l||) of the last iteration over the rows of every island solved in the step
 void orc_world_stats(void* h, float* out) {
   World* w = (World*)h;
   out[0] = (float)w->lastSolve.iterations;
@@ -455,6 +472,56 @@ void orc_world_stats(void* h, float* out) {
   out[4] = (float)w->gjk.gjkIters;
   out[5] = (float)w->gjk.epaCalls;
   out[6] = (float)w->gjk.epaFails;
+  const double nrm = std::sqrt(w->lastSolve.finalNN);
+  out[7] = (float)(std::sqrt(w->lastSolve.finalDD) / (nrm > 1.0 ? nrm : 1.0));
+}
+// the permutation of 0..n-1 after `rounds` shuffles of one pool drawn from new Random(seed)
+void orc_shuffle(int64_t seed, int n, int rounds, int32_t* out) {
+  JavaRandom rng(seed);
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  for (int r = 0; r < rounds; r++) shuffleOrder(order, rng);
+  for (int i = 0; i < n; i++) out[i] = order[i];
+}
+struct OrcSolverInfo {
+  float timeStep, erp, warmstartingFactor, tolerance;
+  int32_t numIterations, maxIterations, solverMode, shuffle;
+  int64_t seed;
+};
+// SequentialImpulseConstraintSolver.solveGroup alone on caller records: nb bodies (world inertia recomputed from
+// each transform; islandTag 0 for dynamic bodies, -1 for static ones, as calculateSimulationIslands leaves them),
+// nm manifolds solved in the given order; bodies and manifolds written back.  out: [iterations, residual].
+void orc_solve_group(int nb, float* bodies, int nm, float* manifolds, const OrcSolverInfo* si, float* out) {
+  std::vector<Body> bs(nb);
+  for (int i = 0; i < nb; i++) {
+    bs[i] = bodyFromRecord(bodies + (size_t)i * JBGPU_BODY_WORDS);
+    bs[i].islandTag = bs[i].isStatic() ? -1 : 0;
+    bs[i].companionId = -1;
+    updateInertiaTensor(bs[i]);
+  }
+  std::vector<Manifold> ms(nm);
+  std::vector<int> types(nm);
+  std::vector<Manifold*> mp(nm);
+  for (int k = 0; k < nm; k++) {
+    ms[k] = manifoldFrom(manifolds + (size_t)k * JBGPU_MANIFOLD_WORDS);
+    types[k] = ldi(manifolds + (size_t)k * JBGPU_MANIFOLD_WORDS + 3);
+    mp[k] = &ms[k];
+  }
+  SolverInfo info;
+  info.timeStep = si->timeStep;
+  info.erp = si->erp;
+  info.warmstartingFactor = si->warmstartingFactor;
+  info.tolerance = si->tolerance;
+  info.numIterations = si->numIterations;
+  info.maxIterations = si->maxIterations;
+  info.solverMode = si->solverMode;
+  JavaRandom rng(si->seed);
+  SolverStats st;
+  solveGroup(bs, mp, info, si->shuffle != 0, rng, &st, nullptr, nullptr);
+  for (int i = 0; i < nb; i++) bodyToRecord(bs[i], bodies + (size_t)i * JBGPU_BODY_WORDS);
+  for (int k = 0; k < nm; k++) manifoldTo(ms[k], types[k], manifolds + (size_t)k * JBGPU_MANIFOLD_WORDS);
+  out[0] = (float)st.iterations;
+  out[1] = st.residual;
 }
 int orc_world_residual_history(void* h, float* out, int cap) {
   World* w = (World*)h;

@@ -7,13 +7,14 @@
 // it follows (paths relative to src/com/bulletphysics/).
 //
 // Parity pinning (what fixes each part, since the reference has no tests):
-//   jrandom          KATs of java.util.Random                 tests/test_oracle_basics.py
-//   jmath / libGDX   orthonormality, known rotations          tests/test_oracle_basics.py
+//   jrandom          KATs of java.util.Random, the shuffle permutation   tests/test_oracle_pins.py
+//   jmath / libGDX   orthonormality, known rotations                     tests/test_oracle_pins.py
 //   broadphase       O(N^2) brute force, closed-form quantised overlap, invariants
 //   sphere/plane     closed forms
 //   GJK / EPA        double-precision brute-force distance/depth on boxes & spheres
-//   manifold         hand-computed add/replace/evict/refresh sequences
-//   solver           resting-box impulse m*g*dt, head-on e=0, cone/complementarity
+//   manifold         hand-computed add/replace/evict (sortCachedPoints) and refresh sequences
+//   solver           resting-box impulse m*g*dt, head-on e=0, cone/complementarity, hand-computed
+//                    warm start (0.85 factor, Lateral1 written twice, directions kept per point lifetime)
 //   integrator       exact free fall, constant-omega rotation
 // libGDX arithmetic (Quaternion.setFromMatrix / nor) is restated from libGDX's
 // documented behaviour; its bit parity against a libGDX build is unpinned.
@@ -160,6 +161,7 @@ struct SolverInfo {
 struct SolverStats {
   int iterations = 0;
   float residual = 0.0f;
+  double finalDD = 0.0, finalNN = 0.0;  // last-iteration sums of (dl)^2 and l^2 over every island solved
   int rows = 0, frictionRows = 0;
   std::vector<float> residualHistory;
 };
@@ -180,6 +182,9 @@ struct ImpulseInit { // learned warm start: override of the warm-start impulses 
   const float* values = nullptr;  // [row][3] normal, lat1, lat2 (already scaled)
 };
 
+// The row-order shuffle of one pool: j = 0..n-1 swaps order[j] with order[Random.nextInt(j + 1)]
+// (SequentialImpulseConstraintSolver.java:805-821, randInt2 :138-149).
+void shuffleOrder(std::vector<int>& order, JavaRandom& rng);
 // One solveGroup over an ordered list of manifolds (SequentialImpulseConstraintSolver.java:954-969).
 // shuffle: apply the Random shuffle at iterations with (iter & 7) == 0.
 void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,

@@ -31,6 +31,12 @@ class WorldCfg(C.Structure):
                 ("tolerance", C.c_float), ("maxIterations", C.c_int32), ("solverMode", C.c_int32)]
 
 
+class SolverInfo(C.Structure):
+    _fields_ = [("timeStep", C.c_float), ("erp", C.c_float), ("warmstartingFactor", C.c_float),
+                ("tolerance", C.c_float), ("numIterations", C.c_int32), ("maxIterations", C.c_int32),
+                ("solverMode", C.c_int32), ("shuffle", C.c_int32), ("seed", C.c_int64)]
+
+
 _lib = None
 
 
@@ -88,6 +94,9 @@ def lib():
             "orc_world_get_features": (C.c_int64, [_f, C.c_int64]),
             "orc_world_clear_features": (None, []),
             "orc_world_time_steps": (C.c_double, [vp, C.c_float, C.c_int]),
+            "orc_shuffle": (None, [C.c_int64, C.c_int, C.c_int, _i32]),
+            "orc_world_phase": (C.c_int, [vp, C.c_float, C.c_int]),
+            "orc_solve_group": (None, [C.c_int, _f, C.c_int, _f, C.POINTER(SolverInfo), _f]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -180,6 +189,27 @@ def random_bounded(seed, bounds):
     return out
 
 
+def shuffle(seed, n, rounds=1):
+    """Permutation of 0..n-1 after `rounds` row-order shuffles drawn from new Random(seed)."""
+    out = np.zeros(max(n, 1), np.int32)
+    lib().orc_shuffle(int(seed), int(n), int(rounds), out)
+    return out[:n]
+
+
+def solve_group(bodies, manifolds, dt=1.0 / 60.0, iterations=10, erp=0.2, warm_factor=0.85, warm=True,
+                shuffle=False, seed=0, tolerance=0.0, max_iterations=100):
+    """solveGroup on caller records (manifolds solved in the given order); returns (bodies, manifolds,
+    iterations, residual) after the solve."""
+    b = f32(bodies).reshape(-1, BODY_WORDS).copy()
+    m = f32(manifolds).reshape(-1, MANIFOLD_WORDS).copy()
+    si = SolverInfo(np.float32(dt), erp, warm_factor, tolerance, iterations, max_iterations,
+                    (1 if shuffle else 0) | 8 | (4 if warm else 0), int(shuffle), seed)
+    out = np.zeros(2, np.float32)
+    lib().orc_solve_group(len(b), b.reshape(-1), len(m), m.reshape(-1) if len(m) else np.zeros(1, np.float32),
+                          C.byref(si), out)
+    return b, m, int(out[0]), float(out[1])
+
+
 # ----------------------------------------------------------------- broadphase
 class Broadphase:
     def __init__(self, mode, wmin=None, wmax=None):
@@ -261,6 +291,11 @@ class World:
     def step(self, dt=1.0 / 60.0, n=1):
         lib().orc_world_step(self.h, dt, n)
 
+    def phase(self, p, dt=1.0 / 60.0):
+        """One phase of the step alone (0 predict, 1 AABBs + pairs, 2 narrowphase, 3 islands + solve,
+        4 integrate + activation)."""
+        lib().orc_world_phase(self.h, dt, int(p))
+
     def step_until(self, phase, dt=1.0 / 60.0):
         lib().orc_world_step_until(self.h, dt, phase)
 
@@ -322,7 +357,8 @@ class World:
         out = np.zeros(8, np.float32)
         lib().orc_world_stats(self.h, out)
         return dict(iterations=int(out[0]), residual=float(out[1]), rows=int(out[2]), friction_rows=int(out[3]),
-                    gjk_iters=int(out[4]), epa_calls=int(out[5]), epa_fails=int(out[6]))
+                    gjk_iters=int(out[4]), epa_calls=int(out[5]), epa_fails=int(out[6]),
+                    step_residual=float(out[7]))
 
     def residual_history(self):
         out = np.zeros(1024, np.float32)

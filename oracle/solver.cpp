@@ -64,6 +64,15 @@ static SolverRow makeFrictionRow(V3 axis, int ida, int idb, int fi, const MPoint
   return r;
 }
 
+void shuffleOrder(std::vector<int>& order, JavaRandom& rng) {  // :805-821, one pool
+  for (int j = 0; j < (int)order.size(); ++j) {
+    int tmp = order[j];
+    int swapi = rng.nextInt(j + 1);
+    order[j] = order[swapi];
+    order[swapi] = tmp;
+  }
+}
+
 void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,
                 JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features) {
   std::vector<SolverBody> sb;
@@ -188,21 +197,12 @@ void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, co
   std::vector<float> prev(rows.size() + frows.size());
   int done = 0;
   float residual = 0.0f;
+  double lastDD = 0.0, lastNN = 0.0;  // the last iteration's sums (whole-step residual over all islands)
   if (st) st->residualHistory.clear();
   for (int it = 0; it < iters; it++) {
-    if (shuffle && (info.solverMode & 1) && ((it & 7) == 0)) {
-      for (int j = 0; j < (int)order.size(); ++j) {
-        int tmp = order[j];
-        int swapi = rng.nextInt(j + 1);
-        order[j] = order[swapi];
-        order[swapi] = tmp;
-      }
-      for (int j = 0; j < (int)forder.size(); ++j) {
-        int tmp = forder[j];
-        int swapi = rng.nextInt(j + 1);
-        forder[j] = forder[swapi];
-        forder[swapi] = tmp;
-      }
+    if (shuffle && (info.solverMode & 1) && ((it & 7) == 0)) {  // normal rows first, then friction rows
+      shuffleOrder(order, rng);
+      shuffleOrder(forder, rng);
     }
     for (size_t i = 0; i < rows.size(); i++) prev[i] = rows[i].applied;
     for (size_t i = 0; i < frows.size(); i++) prev[rows.size() + i] = frows[i].applied;
@@ -262,6 +262,8 @@ void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, co
     }
     double nrm = std::sqrt(nn);
     residual = (float)(std::sqrt(dd) / (nrm > 1.0 ? nrm : 1.0));
+    lastDD = dd;
+    lastNN = nn;
     if (st) st->residualHistory.push_back(residual);
     if (info.tolerance > 0.0f && residual <= info.tolerance) break;
   }
@@ -281,6 +283,8 @@ void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, co
   if (st) {
     st->iterations += done;
     st->residual = residual;
+    st->finalDD += lastDD;
+    st->finalNN += lastNN;
     st->rows += (int)rows.size();
     st->frictionRows += (int)frows.size();
   }
