@@ -130,7 +130,10 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
 int jbgpu_world_ray_test(jbgpu_ctx* ctx, int32_t n, const float* from3, const float* to3, int16_t group,
                          int16_t mask, const int32_t* world, int32_t* obj, float* fraction, float* normal3,
                          float* point3, int32_t on_device);
-/* stepSimulation(dt, 0) nsteps times: exactly one substep per call. */
+/* stepSimulation(dt, 0) nsteps times: exactly one substep per call.  JBGPU_ERR_CAPACITY with an "EPA workspace
+ * overflow" message reports that a pair's GJK-EPA outgrew its workspace in the step just taken (that pair got no
+ * contact point); the step has been applied and the world state has advanced, the following steps are not
+ * affected. */
 int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps);
 /* One step run up to and including phase p: 0 predictUnconstraintMotion, 1 AABB update +
  * calculateOverlappingPairs, 2 dispatchAllCollisionPairs, 3 solveConstraints, 4 full step. */
@@ -140,10 +143,42 @@ int jbgpu_world_set_bodies(jbgpu_ctx* ctx, const float* bodies);
 int jbgpu_world_num_pairs(jbgpu_ctx* ctx, int64_t* n);
 /* pairs (sorted keys) with their persistent manifold records (JBGPU_MANIFOLD_WORDS each) */
 int jbgpu_world_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, float* manifolds);
+/* Replaces the pair cache (the persistent manifolds of HashedOverlappingPairCache's pairs).  The records are
+ * checked on the host first (jbgpu_check_pair_records); a rejected call changes nothing. */
 int jbgpu_world_set_pairs(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds);
+/* Host-only check of n pair keys and manifold records for a world of nbodies bodies: keys strictly increasing
+ * with 1 <= uid_lo < uid_hi <= nbodies (uid = body index + 1), point count in [0, 4], pair type in [0, 3],
+ * manifold bodies equal to the key's two bodies (either order).  JBGPU_ERR_ARG with the reason in
+ * jbgpu_last_error(NULL, ...) otherwise.  Needs no device. */
+int jbgpu_check_pair_records(int32_t nbodies, int64_t n, const uint64_t* keys, const float* manifolds);
 int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out);
 
+/* ---- Dispatcher ----------------------------------------------------------------------- */
+/* CollisionDispatcher.dispatchAllCollisionPairs(pairCache, dispatchInfo, dispatcher)
+ * (collision/broadphase/Dispatcher.java:58, collision/dispatch/CollisionDispatcher.java:252-257) on the caller's
+ * pair cache: n pairs (sorted keys uid_lo << 32 | uid_hi with uid = body index + 1, and their persistent
+ * manifold records), the bodies as last set by jbgpu_world_set_bodies (or as the world's own steps left them).
+ * Every pair is run through its collision algorithm (sphere-sphere, convex-plane, GJK/EPA; by shape types) and
+ * its manifold refreshed and updated (ManifoldResult.addContactPoint / refreshContactPoints); the updated
+ * records are written to manifolds_out and become the context's pair cache (jbgpu_world_get_pairs).  Records
+ * are checked first (jbgpu_check_pair_records); a record with points must carry the dispatcher's header (body0
+ * the lower uid, except a convex-plane pair whose lower uid is the plane, which lists the convex body first;
+ * type as JBGPU_PAIR_*), a record without points gets that header written.  A rejected call changes
+ * nothing. */
+int jbgpu_np_dispatch(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds_in,
+                      float* manifolds_out);
+
 /* ---- ConstraintSolver -------------------------------------------------------------- */
+/* The solver seam as DiscreteDynamicsWorld.solveConstraints drives it (dynamics/DiscreteDynamicsWorld.java:
+ * 636-643): prepareSolve(numBodies, numManifolds) once (ConstraintSolver.java:41), solveGroup once per awake
+ * island (ConstraintSolver.java:46), allSolved once (ConstraintSolver.java:49).  Here solveGroup only records the
+ * island's manifolds (pair indices into the context's pair cache); allSolved colours and solves every recorded
+ * manifold in one pass (islands share no dynamic body, so solving them together in colour order equals solving
+ * each alone in that order) and leaves the velocities in the bodies (jbgpu_world_get_bodies) and the impulses in
+ * the manifolds (jbgpu_world_get_pairs).  jbgpu_solver_get_order then gives the manifold order used. */
+int jbgpu_solver_prepare(jbgpu_ctx* ctx, int32_t num_bodies, int64_t num_manifolds);
+int jbgpu_solver_add_island(jbgpu_ctx* ctx, int64_t n, const int32_t* pair_index);
+int jbgpu_solver_all_solved(jbgpu_ctx* ctx, float dt);
 /* Manifold order of the last solve: pair indices, colour batch after colour batch; a
  * SequentialImpulseConstraintSolver fed this manifold order without shuffling produces
  * the same velocities and impulses.  offsets[k]..offsets[k+1] is colour k (may be NULL). */

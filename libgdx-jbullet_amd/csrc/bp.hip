@@ -764,6 +764,32 @@ __global__ void kHugeWorlds(int nh, const int* __restrict__ list, const int* __r
 }
 
 // remap: unpack sorted keys, carry manifolds of surviving pairs, initialise new ones
+// collision algorithm of a pair (DefaultCollisionConfiguration.getCollisionAlgorithmCreateFunc,
+// collision/dispatch/DefaultCollisionConfiguration.java:149-213); planeFirst: a convex-plane pair whose lower
+// uid is the plane (its manifold lists the convex body first)
+__device__ __forceinline__ int pairTypeOf(int a, int b, const float* __restrict__ shapes,
+                                          const float* __restrict__ props, bool& planeFirst) {
+  const int ta = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
+  const int tb = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * b + 6));
+  int type;
+  if (ta == JBGPU_SHAPE_SPHERE && tb == JBGPU_SHAPE_SPHERE) type = JBGPU_PAIR_SPHERE_SPHERE;
+  else if (ta == JBGPU_SHAPE_PLANE && tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_NONE;
+  else if (ta == JBGPU_SHAPE_PLANE || tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_CONVEX_PLANE;
+  else type = JBGPU_PAIR_CONVEX_CONVEX;
+  planeFirst = type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE;
+  return type;
+}
+
+// pair types of a caller-supplied pair set (the remap computes them for the pairs the broadphase finds)
+__global__ void kPairTypes(int64_t np, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
+                           const float* __restrict__ props, int* ptype) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint64_t key = keys[i];
+  bool planeFirst;
+  ptype[i] = pairTypeOf((int)(key >> 32) - 1, (int)(key & 0xffffffffu) - 1, shapes, props, planeFirst);
+}
+
 __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits, uint64_t* keysNew, float* manNew,
                        int64_t nOld, const uint64_t* __restrict__ keysOld, const float* __restrict__ manOld,
                        const float* __restrict__ shapes, const float* __restrict__ props, int* ptype) {
@@ -774,13 +800,8 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
   uint64_t key = ((p >> bits) << 32) | (p & ((1ull << bits) - 1ull));
   if (live) keysNew[i] = key;
   int a = (int)(key >> 32) - 1, b = (int)(key & 0xffffffffu) - 1;
-  int ta = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * a + 6));
-  int tb = ldi(shapes + JBGPU_SHAPE_WORDS * ldi(props + 8 * b + 6));
-  int type;
-  if (ta == JBGPU_SHAPE_SPHERE && tb == JBGPU_SHAPE_SPHERE) type = JBGPU_PAIR_SPHERE_SPHERE;
-  else if (ta == JBGPU_SHAPE_PLANE && tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_NONE;
-  else if (ta == JBGPU_SHAPE_PLANE || tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_CONVEX_PLANE;
-  else type = JBGPU_PAIR_CONVEX_CONVEX;
+  bool planeFirst;
+  const int type = pairTypeOf(a, b, shapes, props, planeFirst);
   int64_t lo = 0, hi = nOld;
   while (lo < hi) {
     int64_t mid = (lo + hi) >> 1;
@@ -796,7 +817,7 @@ __global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits
     nf4 = JBGPU_POINT_WORDS / 4 * min(max(__builtin_bit_cast(int, h.x), 0), 4);
   } else if (live) {
     int b0 = a, b1 = b;
-    if (type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE) { b0 = b; b1 = a; }
+    if (planeFirst) { b0 = b; b1 = a; }
     dst[0] = make_float4(__builtin_bit_cast(float, 0), __builtin_bit_cast(float, b0), __builtin_bit_cast(float, b1),
                          __builtin_bit_cast(float, type));
   }
@@ -1219,12 +1240,17 @@ static int64_t compactKeys(Ctx& c, uint64_t* keys, int64_t n, const int* flag) {
 
 void bpDestroyProxies(Ctx& c, int n, const int32_t* uids) {
   std::vector<int> u(uids, uids + n);
+  // every uid is checked (range, already destroyed, repeated in this call) before any state changes, so a
+  // rejected call leaves the host and device state as they were
+  std::vector<char> seen(c.np, 0);
   for (int i = 0; i < n; i++) {
     if (u[i] < 1 || u[i] > c.np) throw std::invalid_argument("proxy uid out of range");
     if (c.proxyDead[u[i] - 1]) throw std::invalid_argument("proxy already destroyed");
-    c.proxyDead[u[i] - 1] = 1;
+    if (seen[u[i] - 1]) throw std::invalid_argument("proxy uid repeated in one call");
+    seen[u[i] - 1] = 1;
   }
   if (n == 0) return;
+  for (int i = 0; i < n; i++) c.proxyDead[u[i] - 1] = 1;
   hipStream_t s = c.stream;
   c.counter.ensure(8);
   c.deltaFlag.ensure(n);
@@ -1287,6 +1313,14 @@ void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, ui
     }
     if (cnt) *cnt = k;
   }
+}
+
+void bpPairTypes(Ctx& c) {
+  c.ptype.ensure(std::max<int64_t>(c.npairs, 1));
+  if (c.npairs > 0)
+    hipLaunchKernelGGL(kPairTypes, dim3(gridFor(c.npairs)), dim3(256), 0, c.stream, c.npairs,
+                       (const uint64_t*)c.keys[c.cur].p, (const float*)c.shapes.p, (const float*)c.props.p, c.ptype.p);
+  JB_CHECK(hipGetLastError());
 }
 
 void bpRemapManifolds(Ctx& c) {

@@ -202,6 +202,12 @@ struct Ctx {
   DBuf<int> binRange;                        // per pair type: first sorted position and count
   unsigned long long npStatHost[4] = {0, 0, 0, 0};  // their copy, valid after the step's final synchronisation
   bool npStatPending = false;
+  bool epaOverflow = false;  // the last narrowphase pass overflowed an EPA workspace
+  std::vector<int> shapeTypeHost;  // host copies: type per shape and per body (canonical manifold headers)
+  std::vector<int> bodyShapeType;
+  // ConstraintSolver seam: islands recorded between prepareSolve and allSolved
+  bool solveBatchOpen = false;
+  std::vector<int> islandManifolds;
   // the last solve's statistics: copied into pinned memory at its end, read after the step's final wait
   HBuf<float> solveRo;                // residual, iterations
   HBuf<double> solveRh;               // per-iteration (change, norm) sums
@@ -317,6 +323,7 @@ void bpUpdateFromBodies(Ctx& c);                        // A1 + A2 (+ collide st
 void bpUpdateFromAabbs(Ctx& c, const float* dAabb6);    // setAabb for all proxies from a device array
 void bpFindPairs(Ctx& c);                                // A3-A7: sorted canonical keys in keys[cur]
 void bpRemapManifolds(Ctx& c);                           // A5: pair persistence
+void bpPairTypes(Ctx& c);                                // pair types of a pair set set by the caller
 // narrowphase
 void bpDestroyProxies(Ctx& c, int n, const int32_t* uids);  // BroadphaseInterface.destroyProxy
 void rayBuildTree(Ctx& c);                               // LBVH over the current shape AABBs
@@ -329,7 +336,9 @@ int64_t npCountContacts(Ctx& c);                         // contact points of th
 void npCollectStats(Ctx& c);                             // EPA statistics of the last narrowphase
 void solverCollectStats(Ctx& c);                         // residual, rows and sweep timings of the last solve
 // solver
-void solverSolve(Ctx& c, float dt);                      // A18-A20
+// A18-A20; explicit: the manifolds to solve (pair indices, the islands a caller recorded) instead of the
+// step's own selection (activity, islands, rows)
+void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList = nullptr);
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)
 // integration
 void worldPredict(Ctx& c, float dt);

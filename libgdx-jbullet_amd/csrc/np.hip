@@ -1393,12 +1393,19 @@ __global__ void kRayCandidates(int nr, const float* __restrict__ from, const flo
                                const float4* __restrict__ rec, const int4* __restrict__ leafExact,
                                const int* __restrict__ hugeRange, const int* __restrict__ hugeList,
                                const float* __restrict__ aabb, const int* __restrict__ filt,
-                               const int* __restrict__ wid, int* count, const int64_t* __restrict__ offset,
-                               uint64_t* keys, int* err) {
+                               const int* __restrict__ wid, int maxWorld, int* count,
+                               const int64_t* __restrict__ offset, uint64_t* keys, int* err) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
   const V3 F = v3(from[3 * r], from[3 * r + 1], from[3 * r + 2]), T = v3(to[3 * r], to[3 * r + 1], to[3 * r + 2]);
   const int w = rworld ? rworld[r] : 0;
+  if (w < 0 || w > maxWorld) {  // a world id the proxies never had: no candidates, and the call reports it
+    if (!Emit) {
+      count[r] = 0;
+      atomicOr(err, 2);
+    }
+    return;
+  }
   int cnt = 0;
   const int64_t base = Emit ? offset[r] : 0;
   const int room = Emit ? count[r] : 0;  // the emit pass writes no more than the count pass found
@@ -1722,7 +1729,7 @@ void npCollectStats(Ctx& c) {  // after the step's final synchronisation
   c.st.epaCalls = (int64_t)c.npStatHost[0];
   c.st.epaFails = (int64_t)c.npStatHost[1];
   c.st.epaRetries = (int64_t)c.npStatHost[3];
-  if (c.npStatHost[2]) c.err = "EPA workspace overflow";
+  c.epaOverflow = c.npStatHost[2] != 0;  // this pass only: the next step starts clean
 }
 
 
@@ -1739,7 +1746,7 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
   hipLaunchKernelGGL(kRayCandidates<false>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
                      (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p, (const int*)t.hugeRange.p,
                      (const int*)t.hugeList.p, (const float*)t.aabb.p, (const int*)c.filt.p,
-                     (const int*)c.world.p, t.count.p, (const int64_t*)nullptr, (uint64_t*)nullptr, err);
+                     (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)nullptr, (uint64_t*)nullptr, err);
   JB_CHECK(hipMemsetAsync(t.count.p + n, 0, sizeof(int), s));
   size_t tb = 0;
   JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t.count.p, t.offset.p, n + 1, s));
@@ -1750,7 +1757,8 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
   JB_CHECK(hipMemcpyAsync(&total, t.offset.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipMemcpyAsync(&errh, err, sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
-  if (errh) throw std::runtime_error("ray traversal stack overflow");
+  if (errh & 2) throw std::invalid_argument("ray world id out of range (no such world among the proxies)");
+  if (errh & 1) throw std::runtime_error("ray traversal stack overflow");
   if (total > 0) {
     t.keys.ensure(total);
     t.keysAlt.ensure(total);
@@ -1758,8 +1766,8 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
     hipLaunchKernelGGL(kRayCandidates<true>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
                        (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p,
                        (const int*)t.hugeRange.p, (const int*)t.hugeList.p, (const float*)t.aabb.p,
-                       (const int*)c.filt.p, (const int*)c.world.p, t.count.p, (const int64_t*)t.offset.p, t.keys.p,
-                       err);
+                       (const int*)c.filt.p, (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)t.offset.p,
+                       t.keys.p, err);
     // world order inside every ray's segment: sort by (ray, object)
     int bits = 32;
     while (bits < 64 && (1ll << (bits - 32)) < (int64_t)n) bits++;

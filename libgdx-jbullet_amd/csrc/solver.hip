@@ -1059,7 +1059,7 @@ void solverGetImpulses(Ctx& c, float* out, int64_t nslots) {
   JB_CHECK(hipStreamSynchronize(s));
 }
 
-void solverSolve(Ctx& c, float dt) {
+void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   hipStream_t s = c.stream;
   const int64_t np = c.npairs;
   c.nActive = 0;
@@ -1077,6 +1077,12 @@ void solverSolve(Ctx& c, float dt) {
   c.mActive.ensure(np);
   c.counter.ensure(8);
   const int* island = nullptr;
+  int na = 0;
+  if (explicitList) {  // the caller's islands: exactly these manifolds, in the caller's order
+    na = (int)explicitList->size();
+    c.mActive.ensure(std::max(na, 1));
+    if (na) JB_CHECK(hipMemcpyAsync(c.mActive.p, explicitList->data(), (size_t)na * sizeof(int), hipMemcpyHostToDevice, s));
+  } else {
   if (!c.deactivationDisabled) {  // islands decide sleeping; with deactivation disabled every body stays awake
     const int nb = c.nb;
     c.island.ensure(nb);
@@ -1111,9 +1117,9 @@ void solverSolve(Ctx& c, float dt) {
   c.cubTmp.ensure(tb);
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));
-  int na = 0;
   JB_CHECK(hipMemcpyAsync(&na, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
+  }
   c.nActive = na;
   if (na == 0) return;
   if (c.useMlp && !c.mlpReady) throw std::logic_error("learned warm start enabled without a model");
@@ -1133,7 +1139,7 @@ void solverSolve(Ctx& c, float dt) {
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, 2 * (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
   // block partition of the dynamic bodies along this step's Morton order: interior / boundary manifolds
-  const bool blocked = c.blockSize > 0 && c.sortedOrder != nullptr && c.np == c.nb;
+  const bool blocked = !explicitList && c.blockSize > 0 && c.sortedOrder != nullptr && c.np == c.nb;
   const int B = c.blockSize;
   const int* phase = nullptr;
   c.nBlocks = 0;

@@ -258,6 +258,8 @@ void bodiesToDevice(Ctx& c, const float* bodies) {
     int si = ldi(r + 27);
     if (si < 0 || si >= c.ns) throw std::invalid_argument("body shape index out of range");
   }
+  c.bodyShapeType.resize(n);
+  for (int i = 0; i < n; i++) c.bodyShapeType[i] = c.shapeTypeHost[ldi(bodies + (size_t)JBGPU_BODY_WORDS * i + 27)];
   hipStream_t s = c.stream;
   upload(c.tr, tr.data(), tr.size(), s);
   upload(c.vel, vel.data(), vel.size(), s);
@@ -302,6 +304,75 @@ void bodiesFromDevice(Ctx& c, float* bodies) {
   }
 }
 
+// Host check of caller-supplied pair keys and manifold records before any of them reaches a kernel: keys
+// strictly increasing with 1 <= uid_lo < uid_hi <= nbodies; per record a point count in [0, 4], a pair type in
+// [0, 3] and the key's two bodies (in either order: a convex-plane record lists the convex body first).
+void checkPairRecords(int nbodies, int64_t n, const uint64_t* keys, const float* man) {
+  if (n < 0) throw std::invalid_argument("negative pair count");
+  if (n == 0) return;
+  if (!keys || !man) throw std::invalid_argument("null pair arrays");
+  for (int64_t i = 0; i < n; i++) {
+    const uint64_t k = keys[i];
+    const int64_t lo = (int64_t)(k >> 32), hi = (int64_t)(k & 0xffffffffu);
+    if (lo < 1 || hi <= lo || hi > nbodies)
+      throw std::invalid_argument("pair " + std::to_string(i) + ": key uids out of range or not uid_lo < uid_hi");
+    if (i > 0 && keys[i - 1] >= k) throw std::invalid_argument("pair keys not strictly increasing at " + std::to_string(i));
+    const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * i;
+    const int cnt = ldi(m), b0 = ldi(m + 1), b1 = ldi(m + 2), type = ldi(m + 3);
+    if (cnt < 0 || cnt > 4) throw std::invalid_argument("pair " + std::to_string(i) + ": point count not in [0, 4]");
+    if (type < 0 || type > JBGPU_PAIR_NONE)
+      throw std::invalid_argument("pair " + std::to_string(i) + ": pair type not in [0, 3]");
+    const int a = (int)lo - 1, b = (int)hi - 1;
+    if (!((b0 == a && b1 == b) || (b0 == b && b1 == a)))
+      throw std::invalid_argument("pair " + std::to_string(i) + ": manifold bodies do not match the key");
+  }
+}
+
+// The manifold header the dispatcher keeps for a pair (count, body0, body1, type): bodies in uid order, except a
+// convex-plane pair whose lower uid is the plane, which lists the convex body first (the device's remap and the
+// oracle build the same header).  Records of a caller's pair set for the narrowphase seam must carry it: a
+// record with points gets ERR_ARG otherwise (its points' A/B sides would be swapped); a record without points
+// gets it written here.
+void canonicalHeaders(const Ctx& c, int64_t n, const uint64_t* keys, float* man) {
+  for (int64_t i = 0; i < n; i++) {
+    const int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+    const int ta = c.bodyShapeType[a], tb = c.bodyShapeType[b];
+    int type;
+    if (ta == JBGPU_SHAPE_SPHERE && tb == JBGPU_SHAPE_SPHERE) type = JBGPU_PAIR_SPHERE_SPHERE;
+    else if (ta == JBGPU_SHAPE_PLANE && tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_NONE;
+    else if (ta == JBGPU_SHAPE_PLANE || tb == JBGPU_SHAPE_PLANE) type = JBGPU_PAIR_CONVEX_PLANE;
+    else type = JBGPU_PAIR_CONVEX_CONVEX;
+    const bool planeFirst = type == JBGPU_PAIR_CONVEX_PLANE && ta == JBGPU_SHAPE_PLANE;
+    const int b0 = planeFirst ? b : a, b1 = planeFirst ? a : b;
+    float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * i;
+    if (ldi(m) > 0) {
+      if (ldi(m + 1) != b0 || ldi(m + 2) != b1 || ldi(m + 3) != type)
+        throw std::invalid_argument("pair " + std::to_string(i) +
+                                    ": manifold header is not the dispatcher's (body order or pair type)");
+    } else {
+      sti(m + 1, b0);
+      sti(m + 2, b1);
+      sti(m + 3, type);
+    }
+  }
+}
+
+// the pair set (sorted keys with their manifolds) into the context's current pair cache
+void uploadPairs(Ctx& c, int64_t n, const uint64_t* keys, const float* manifolds) {
+  c.keys[c.cur].ensure(std::max<int64_t>(n, 1));
+  c.man[c.cur].ensure((size_t)JBGPU_MANIFOLD_WORDS * std::max<int64_t>(n, 1));
+  if (n) {
+    JB_CHECK(hipMemcpy(c.keys[c.cur].p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    JB_CHECK(hipMemcpy(c.man[c.cur].p, manifolds, (size_t)n * JBGPU_MANIFOLD_WORDS * sizeof(float),
+                       hipMemcpyHostToDevice));
+  }
+  c.npairs = n;
+  c.npairsOld = 0;
+  c.st.pairs = n;
+  bpPairTypes(c);
+  JB_CHECK(hipStreamSynchronize(c.stream));
+}
+
 void stepPhases(Ctx& c, float dt, int last) {
   hipStream_t s = c.stream;
   tBegin(c, T_STEP);
@@ -332,7 +403,7 @@ void stepPhases(Ctx& c, float dt, int last) {
     tCollect(c);
     c.timedSteps++;
   }
-  if (!c.err.empty() && c.err == "EPA workspace overflow") throw std::length_error(c.err);
+  if (c.epaOverflow) throw std::length_error("EPA workspace overflow in this step (the step has been applied)");
 }
 
 }  // namespace
@@ -712,6 +783,8 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     upload(c.shapes, shapes, (size_t)JBGPU_SHAPE_WORDS * nshapes, c.stream);
     std::vector<float> vz(3, 0.0f);
     upload(c.verts, nverts > 0 ? verts : vz.data(), nverts > 0 ? 3 * (size_t)nverts : 3, c.stream);
+    c.shapeTypeHost.resize(nshapes);
+    for (int k = 0; k < nshapes; k++) c.shapeTypeHost[k] = ldi(shapes + (size_t)JBGPU_SHAPE_WORDS * k);
     bodiesToDevice(c, bodies);
     // addRigidBody -> addCollisionObject: proxy with the shape's AABB (no contact threshold)
     std::vector<float> box(6 * (size_t)nbodies);
@@ -792,16 +865,84 @@ int jbgpu_world_get_pairs(jbgpu_ctx* ctx, uint64_t* keys, float* manifolds) {
 int jbgpu_world_set_pairs(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds) {
   return guard(ctx, [&] {
     Ctx& c = ctx->c;
-    if (n < 0) throw std::invalid_argument("negative pair count");
-    c.keys[c.cur].ensure(std::max<int64_t>(n, 1));
-    c.man[c.cur].ensure((size_t)JBGPU_MANIFOLD_WORDS * std::max<int64_t>(n, 1));
-    if (n) {
-      JB_CHECK(hipMemcpy(c.keys[c.cur].p, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice));
-      JB_CHECK(hipMemcpy(c.man[c.cur].p, manifolds, (size_t)n * JBGPU_MANIFOLD_WORDS * sizeof(float),
-                         hipMemcpyHostToDevice));
-    }
-    c.npairs = n;
+    requireWorld(c);
+    checkPairRecords(c.nb, n, keys, manifolds);
+    uploadPairs(c, n, keys, manifolds);
   });
+}
+
+// ---- Dispatcher seam ---------------------------------------------------------------
+int jbgpu_np_dispatch(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds_in,
+                      float* manifolds_out) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (n > 0 && !manifolds_out) throw std::invalid_argument("null output manifolds");
+    checkPairRecords(c.nb, n, keys, manifolds_in);
+    std::vector<float> man(manifolds_in, manifolds_in + (size_t)JBGPU_MANIFOLD_WORDS * (n > 0 ? n : 0));
+    canonicalHeaders(c, n, keys, man.data());
+    uploadPairs(c, n, keys, man.data());
+    npDispatchAll(c);
+    JB_CHECK(hipStreamSynchronize(c.stream));
+    npCollectStats(c);
+    if (n)
+      JB_CHECK(hipMemcpy(manifolds_out, c.man[c.cur].p, (size_t)n * JBGPU_MANIFOLD_WORDS * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    if (c.epaOverflow)
+      throw std::length_error("EPA workspace overflow in this pass (the manifolds have been updated)");
+  });
+}
+
+// ---- ConstraintSolver seam -----------------------------------------------------------
+int jbgpu_solver_prepare(jbgpu_ctx* ctx, int32_t num_bodies, int64_t num_manifolds) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (num_bodies != c.nb) throw std::invalid_argument("prepareSolve: body count differs from the world's");
+    if (num_manifolds < 0 || num_manifolds != c.npairs)
+      throw std::invalid_argument("prepareSolve: manifold count differs from the pair cache's");
+    c.islandManifolds.clear();
+    c.solveBatchOpen = true;
+  });
+}
+
+int jbgpu_solver_add_island(jbgpu_ctx* ctx, int64_t n, const int32_t* pair_index) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (!c.solveBatchOpen) throw std::logic_error("solveGroup outside prepareSolve / allSolved");
+    if (n < 0 || (n > 0 && !pair_index)) throw std::invalid_argument("bad island arguments");
+    for (int64_t i = 0; i < n; i++)
+      if (pair_index[i] < 0 || pair_index[i] >= c.npairs) throw std::invalid_argument("manifold index out of range");
+    c.islandManifolds.insert(c.islandManifolds.end(), pair_index, pair_index + n);
+  });
+}
+
+int jbgpu_solver_all_solved(jbgpu_ctx* ctx, float dt) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (!c.solveBatchOpen) throw std::logic_error("allSolved without prepareSolve");
+    if (!(dt > 0.0f)) throw std::invalid_argument("bad time step");
+    c.solveBatchOpen = false;
+    std::vector<char> seen(c.npairs, 0);
+    for (int k : c.islandManifolds) {
+      if (seen[k]) throw std::invalid_argument("a manifold was recorded in two islands");
+      seen[k] = 1;
+    }
+    solverSolve(c, dt, &c.islandManifolds);
+    JB_CHECK(hipStreamSynchronize(c.stream));
+    solverCollectStats(c);
+  });
+}
+
+int jbgpu_check_pair_records(int32_t nbodies, int64_t n, const uint64_t* keys, const float* manifolds) {
+  try {
+    if (nbodies < 0) throw std::invalid_argument("negative body count");
+    checkPairRecords(nbodies, n, keys, manifolds);
+    return JBGPU_OK;
+  } catch (const std::exception& e) {
+    g_createError = e.what();
+    return JBGPU_ERR_ARG;
+  }
 }
 
 int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {

@@ -74,6 +74,11 @@ SYMBOLS = {
     "jbgpu_world_get_pairs": (C.c_int, [_vp, _vp, _vp]),
     "jbgpu_world_set_pairs": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "jbgpu_world_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+    "jbgpu_check_pair_records": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp]),
+    "jbgpu_np_dispatch": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp]),
+    "jbgpu_solver_prepare": (C.c_int, [_vp, C.c_int32, C.c_int64]),
+    "jbgpu_solver_add_island": (C.c_int, [_vp, C.c_int64, _vp]),
+    "jbgpu_solver_all_solved": (C.c_int, [_vp, C.c_float]),
     "jbgpu_solver_get_order": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64), _vp, C.POINTER(C.c_int32)]),
     "jbgpu_solver_set_warmstart": (C.c_int, [_vp, _vp, C.c_int64, C.c_int32]),
     "jbgpu_solver_enable_features": (C.c_int, [_vp, C.c_int32]),
@@ -165,6 +170,20 @@ def shape_inertia(rec, mass, verts=None):
     if rc:
         raise JbgpuError(f"jbgpu_shape_local_inertia failed ({rc})")
     return out
+
+
+def check_pair_records(nbodies, keys, manifolds):
+    """jbgpu_check_pair_records: raises JbgpuError with the library's reason if a record is malformed."""
+    k = np.ascontiguousarray(keys, np.uint64)
+    m = f32(manifolds).reshape(-1)
+    if m.size != len(k) * MANIFOLD_WORDS:
+        raise ValueError("one manifold record per key")
+    L = load()
+    rc = L.jbgpu_check_pair_records(int(nbodies), len(k), _p(k), _p(m))
+    if rc:
+        buf = C.create_string_buffer(256)
+        L.jbgpu_last_error(None, buf, 256)
+        raise JbgpuError(f"jbgpu_check_pair_records failed ({rc}): {buf.value.decode()}")
 
 
 class Context:
@@ -285,11 +304,13 @@ class Context:
                                              _p(w), _p(obj), _p(fr), _p(nrm), _p(pt), 0), "world_ray_test")
         return obj, fr, nrm, pt
 
-    def ray_test_device(self, n, frm, to, obj, fr, nrm, pt, group=1, mask=-1):
-        """rayTest on rays and results resident in device buffers (jbgpu_buffer_alloc addresses)."""
+    def ray_test_device(self, n, frm, to, obj, fr, nrm, pt, group=1, mask=-1, world=None):
+        """rayTest on rays and results resident in device buffers (jbgpu_buffer_alloc addresses); world: device
+        address of n int32 world ids or None."""
         v = lambda a: C.c_void_p(int(a))
         self._ck(load().jbgpu_world_ray_test(self.h, int(n), v(frm), v(to), int(np.int16(group)), int(np.int16(mask)),
-                                             None, v(obj), v(fr), v(nrm), v(pt), 1), "world_ray_test")
+                                             None if world is None else v(world), v(obj), v(fr), v(nrm), v(pt), 1),
+                 "world_ray_test")
 
     def step_until(self, phase, dt=1.0 / 60.0):
         self._ck(load().jbgpu_world_step_until(self.h, dt, int(phase)), "world_step_until")
@@ -320,7 +341,28 @@ class Context:
         self._ck(load().jbgpu_world_stats(self.h, C.byref(s)), "world_stats")
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
+    # ---- Dispatcher
+    def np_dispatch(self, keys, manifolds):
+        """dispatchAllCollisionPairs on the given pair cache (sorted keys + manifold records): updated manifolds."""
+        k = np.ascontiguousarray(keys, np.uint64)
+        m = f32(manifolds).reshape(-1)
+        if m.size != len(k) * MANIFOLD_WORDS:
+            raise ValueError("one manifold record per key")
+        out = np.zeros(max(len(k), 1) * MANIFOLD_WORDS, np.float32)
+        self._ck(load().jbgpu_np_dispatch(self.h, len(k), _p(k), _p(m), _p(out)), "np_dispatch")
+        return out.reshape(-1, MANIFOLD_WORDS)[:len(k)]
+
     # ---- ConstraintSolver
+    def solver_prepare(self, num_bodies, num_manifolds):
+        self._ck(load().jbgpu_solver_prepare(self.h, int(num_bodies), int(num_manifolds)), "solver_prepare")
+
+    def solver_add_island(self, pair_index):
+        ix = np.ascontiguousarray(pair_index, np.int32)
+        self._ck(load().jbgpu_solver_add_island(self.h, len(ix), _p(ix)), "solver_add_island")
+
+    def solver_all_solved(self, dt=1.0 / 60.0):
+        self._ck(load().jbgpu_solver_all_solved(self.h, float(dt)), "solver_all_solved")
+
     def solver_order(self):
         n = C.c_int64(0)
         nc = C.c_int32(0)

@@ -213,7 +213,9 @@ def aabb_cloud_step(c, h, v, dt=np.float32(1.0 / 60.0)):
 
 
 def batched_towers(seed=1, n_worlds=4096, lo=16, hi=512, spacing=3.0, world_gap=0.0):
-    """Many independent worlds, each a few towers of spheres and boxes on its own floor."""
+    """Many independent worlds, each a few towers of spheres and boxes on its own floor.  Sizes as in the mixed
+    pile: sphere radius log-uniform 0.1-1 m, box half extents U(0.1, 1) m per axis, density U(500, 2000);
+    spheres and boxes 4:3; towers of height U{4..20} on a `spacing` grid, eight towers per row."""
     rng = np.random.default_rng(seed)
     b = _Builder("batched_towers")
     plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
@@ -230,12 +232,12 @@ def batched_towers(seed=1, n_worlds=4096, lo=16, hi=512, spacing=3.0, world_gap=
             y = 0.0
             for _ in range(min(height, nb - placed)):
                 if rng.random() < 4.0 / 7.0:
-                    r = float(np.exp(rng.uniform(np.log(0.1), np.log(1.0)))) * 0.5
+                    r = float(np.exp(rng.uniform(np.log(0.1), np.log(1.0))))
                     s = b.shape(SPHERE, [r])
                     ext = r
                     m = rng.uniform(500, 2000) * 4.0 / 3.0 * np.pi * r ** 3
                 else:
-                    h = rng.uniform(0.1, 1.0, 3) * 0.5
+                    h = rng.uniform(0.1, 1.0, 3)
                     s = b.shape(BOX, list(h))
                     ext = h[1]
                     m = rng.uniform(500, 2000) * 8 * h[0] * h[1] * h[2]
@@ -298,6 +300,23 @@ def pyramids(seed=1, n_boxes=64_000, n_pyramids=32, n_spheres=2000, area=200.0):
                        np.float32)
         b.body(s, IDENT, pos, np.float32(1000.0 * 4.0 / 3.0 * np.pi * r ** 3))
     return b.done(dt=1 / 60, iterations=100, steps=200)
+
+
+def aabb_overflow(nan_body=False):
+    """AABB overflow (CollisionWorld.updateSingleAabb, CollisionWorld.java:211-227): a dynamic box whose AABB
+    extent squared exceeds 1e12 (half extents 8e5 x 0.5 x 0.5) is taken out of the simulation
+    (DISABLE_SIMULATION), beside a 1 m box falling onto the floor; optionally a box with a NaN origin, whose
+    AABB test fails the same way."""
+    b = _Builder("aabb_overflow")
+    plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
+    b.body(plane, IDENT, np.zeros(3, np.float32), 0.0)
+    box = b.shape(BOX, [0.5, 0.5, 0.5])
+    b.body(box, IDENT, np.array([0.0, 3.0, 0.0], np.float32), 1.0)
+    big = b.shape(BOX, [8e5, 0.5, 0.5])
+    b.body(big, IDENT, np.array([0.0, 20.0, 0.0], np.float32), 1.0)
+    if nan_body:
+        b.body(box, IDENT, np.array([np.nan, 5.0, 0.0], np.float32), 1.0)
+    return b.done(dt=1 / 60, iterations=10, steps=30)
 
 
 def sleep_and_wake(n=3, layers=2, drop_height=30.0):

@@ -88,3 +88,53 @@ def test_scene_records_match_oracle(gen):
     s2, v2, b2 = scenes.build_records(sc, po)
     for x, y in ((s1, s2), (v1, v2), (b1, b2)):
         assert x.shape == y.shape and np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def _pair_records():
+    """Three well-formed pairs of a 5-body world as the device keeps them: keys (uid_lo << 32 | uid_hi),
+    manifold headers (count, body0, body1, type); the last is a convex-plane pair listing the convex body first."""
+    keys = np.array([(1 << 32) | 2, (1 << 32) | 4, (3 << 32) | 5], np.uint64)
+    man = np.zeros((3, jbgpu.MANIFOLD_WORDS), np.float32)
+    hdr = np.array([[2, 0, 1, 2], [0, 0, 3, 0], [1, 4, 2, 1]], np.int32)
+    man[:, :4] = hdr.view(np.float32)
+    return keys, man
+
+
+def test_pair_record_check_accepts_well_formed_records():
+    keys, man = _pair_records()
+    jbgpu.check_pair_records(5, keys, man)
+    jbgpu.check_pair_records(5, keys[:0], man[:0])
+
+
+@pytest.mark.parametrize("case,reason", [
+    ("count5", "point count"), ("count_neg", "point count"), ("type", "pair type"), ("body", "bodies do not match"),
+    ("unsorted", "strictly increasing"), ("dup", "strictly increasing"), ("uid_range", "out of range"),
+    ("uid_zero", "out of range"), ("self", "uid_lo < uid_hi")])
+def test_pair_record_check_rejects_each_bad_record(case, reason):
+    """Every malformed record a caller could hand to jbgpu_world_set_pairs is refused on the host, before any
+    of it reaches a kernel (a point count above 4 or a foreign body index would index past device buffers)."""
+    keys, man = _pair_records()
+    h = man[:, :4].view(np.int32)
+    if case == "count5":
+        h[0, 0] = 5
+    elif case == "count_neg":
+        h[1, 0] = -1
+    elif case == "type":
+        h[2, 3] = 7
+    elif case == "body":
+        h[1, 2] = 2
+    elif case == "unsorted":
+        keys = keys[[1, 0, 2]].copy()
+        man = man[[1, 0, 2]].copy()
+    elif case == "dup":
+        keys[1] = keys[0]
+        man[1] = man[0]
+    elif case == "uid_range":
+        keys[2] = (3 << 32) | 6
+    elif case == "uid_zero":
+        keys[0] = 2
+        h[0, 1:3] = [-1, 1]
+    elif case == "self":
+        keys[2] = (5 << 32) | 5
+    with pytest.raises(jbgpu.JbgpuError, match=reason):
+        jbgpu.check_pair_records(5, keys, man)
