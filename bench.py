@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "libgdx-jbullet_amd"))
 
 METRIC = "body-steps/s @100k bodies; solver iters to 1e-4 residual, learned vs plain"
+DROP_STEPS = 600         # the pile's drop: the timed window ends at its last step when the steps fit
+TUNE_CANDIDATES = ((2048, 0), (512, 0), (0, 0), (0, 256), (0, 512), (0, 1024))
+TUNE_STEPS = 8
+TRACE_MARK_BYTES = 48 << 10  # marker copy around the timed window (a kCopyProbe of this size)
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per unit of work (compulsory traffic, each array touched once)
@@ -41,8 +45,8 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bodies", type=int, default=100_000)
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1)
@@ -62,6 +66,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--sweep-shape", default=None, metavar="BLOCKS,TAIL",
                     help="fix the sweep launch shape instead of autotuning it (e.g. 0,256)")
+    ap.add_argument("--trace-window", action="store_true",
+                    help="launch a marker copy right before and after the timed steps (for --summarize-trace)")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="summarise the kernels between the two window markers of a rocprofv3 kernel trace")
     ap.add_argument("--section", choices=("batched", "learned", "broadphase_1m"),
                     help="run only this secondary section and print its JSON (for profiling it alone)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -78,6 +86,8 @@ def main():
     import jbgpu
     from jbgpu import scenes
 
+    if args.summarize_trace:
+        return summarize_trace(args.summarize_trace, args.steps)
     if args.pmc_child:
         return pmc_child(args)
     if args.section:
@@ -91,12 +101,17 @@ def main():
     ctx = jbgpu.Context(num_iterations=args.iterations)
     ctx.world_create(shapes, verts, bodies)
     dt = 1.0 / 60.0
+    # The timed window is fixed by the scene, not by --warmup: the untimed settle phase runs the drop so that
+    # settle + autotune + warmup + timed steps end at step DROP_STEPS (the last steps of the 600-step drop, the
+    # settled pile) whenever they fit.
+    tune_steps = 0 if args.sweep_shape else len(TUNE_CANDIDATES) * (TUNE_STEPS + 1)
+    settle = max(0, DROP_STEPS - tune_steps - args.warmup - args.steps)
     t = time.time()
-    for k in range(args.warmup):
+    for k in range(settle):
         ctx.step(dt)
-        if (k + 1) % 50 == 0:
+        if (k + 1) % 100 == 0:
             st = ctx.stats()
-            log(f"[bench] warmup {k + 1}/{args.warmup}: pairs {st['pairs']} manifolds {st['manifolds']} "
+            log(f"[bench] settle {k + 1}/{settle}: pairs {st['pairs']} manifolds {st['manifolds']} "
                 f"colors {st['colors']} epa {st['epa_calls']} ({time.time() - t:.1f}s)")
     if args.sweep_shape:
         blocks, tail = (int(v) for v in args.sweep_shape.split(","))
@@ -105,15 +120,25 @@ def main():
         tune = {"block_size": blocks, "tail_threshold": tail, "ms_per_step": None, "steps": 0}
     else:
         tune = autotune(ctx, dt)
-    first_timed = args.warmup + tune["steps"] + 1  # the pile's step number of the first timed step
+    ctx.step(dt, n=args.warmup)
+    first_timed = settle + tune["steps"] + args.warmup + 1  # the pile's step number of the first timed step
+    window = f"steps {first_timed}-{first_timed + args.steps - 1} of the drop"
+    # the state at the start of the window: the CPU baseline and the residual comparison start from it
+    snap = dict(bodies=ctx.bodies(), proxies=ctx.bp_proxies(), pairs=ctx.pairs())
+    st0 = ctx.stats()
+    log(f"[bench] window {window}: pairs {st0['pairs']} manifolds {st0['manifolds']} colors {st0['colors']}")
     # timed region: only the sampled per-launch sweep events (the roofline's kernel durations) are recorded
     ctx.timing_kernels(args.kernel_every)
+    if args.trace_window:
+        ctx.measure_copy(TRACE_MARK_BYTES, 1)  # kCopyProbe of a marker size: the window's first dispatch
     t0 = time.perf_counter()
     for k in range(args.steps):
         ctx.step(dt)
         if (k + 1) % 50 == 0:
             log(f"[bench] timed {k + 1}/{args.steps} ({time.perf_counter() - t0:.1f}s)")
     t1 = time.perf_counter()
+    if args.trace_window:
+        ctx.measure_copy(TRACE_MARK_BYTES, 1)  # and its last
     secs = t1 - t0
     st = ctx.stats()
     value = args.bodies * args.steps / secs
@@ -144,11 +169,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"mixed_pile: {args.bodies} convex bodies (spheres/boxes/capsules/hulls) in a walled "
-                               f"100x100 m bin, {args.iterations} solver iterations, dt 1/60, steps "
-                               f"{first_timed}-{first_timed + args.steps - 1} of the drop",
+        "config": {"workload": f"mixed_pile: {args.bodies} convex bodies (spheres/boxes/capsules/hulls) dropped in a "
+                               f"walled 100x100 m bin, {args.iterations} solver iterations, dt 1/60; timed: {window} "
+                               f"(untimed before it: {settle} settle, {tune['steps']} autotune, {args.warmup} warmup "
+                               f"steps)",
                    "bodies": args.bodies, "solver_iterations": args.iterations, "broadphase": "dbvt",
-                   "solver_order": "colour batches"},
+                   "solver_order": "colour batches", "first_timed_step": first_timed,
+                   "window_start_state": {k: st0[k] for k in ("pairs", "contacts", "manifolds", "colors")}},
         "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in tim.items()},
         "phase_roofline": phase_roofline(tim, nsteps, ctx.stats(), args),
         "phase_note": f"event-instrumented pass of {nsteps} steps after the timed region",
@@ -156,9 +183,18 @@ def main():
         "roofline": roof,
         "autotune": tune,
     }
+    # the first window step again from the snapshot, on the device (colour order): its final residual beside the
+    # reference order's from the same state (the CPU baseline's first step)
+    ctx.set_bodies(snap["bodies"])
+    ctx.bp_set_proxies(*snap["proxies"])
+    ctx.set_pairs(*snap["pairs"])
+    ctx.step(dt)
+    rst = ctx.stats()
+    out["residual"] = {"step": first_timed, "colour_order": float(rst["residual"]), "iterations": args.iterations,
+                       "definition": "||l_k - l_(k-1)|| / max(1, ||l_k||) over every normal and friction row after "
+                                     "the last iteration, both from the state before the step"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, shapes, verts, bodies, args,
-                                           first_timed - 1 + args.steps + (args.phase_steps if args.phase_steps > 0 else 0))
+        out["cpu_baseline"] = cpu_baseline(shapes, verts, bodies, snap, args, first_timed, out["residual"])
     if not args.no_rays:
         try:
             out["ray_test"] = ray_section(ctx, shapes, verts, bodies, args)
@@ -299,11 +335,11 @@ def phase_roofline(tim, nsteps, st, args):
     return out
 
 
-def autotune(ctx, dt, steps=8):
+def autotune(ctx, dt, steps=TUNE_STEPS):
     """Launch shaping of the sweeps (results are bit-identical for every choice): the block size of the
     block-partitioned sweeps (0: one launch per colour) and, without blocks, the largest colour swept by the
     single-workgroup tail launch.  Each candidate runs a few untimed warmup steps."""
-    cands = ((2048, 0), (512, 0), (0, 0), (0, 256), (0, 512), (0, 1024))
+    cands = TUNE_CANDIDATES
     ms = {}
     for blocks, tail in cands:
         ctx.solver_set_blocks(blocks)
@@ -514,28 +550,72 @@ def ray_section(ctx, shapes, verts, bodies, args):
                         "(half vertical), after the timed region; query tree rebuilt in every call"}
 
 
-def cpu_baseline(ctx, shapes, verts, bodies, args, after_step):
-    """The oracle (C++ restatement of the Java path, 1 core) stepping the same pile from the GPU's current state."""
+def cpu_baseline(shapes, verts, bodies, snap, args, first_timed, residual):
+    """The oracle (C++ restatement of the Java path, 1 core) stepping the same pile from the state at the start of
+    the timed window, in the reference's island order with the Random(0) shuffle: the first steps of the same
+    window, as many as fit in --cpu-seconds.  The first step's final residual is the reference order's residual
+    for the window's first step (the device's colour-order residual from the same state is beside it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.lib()
     orc = pyoracle.World(shapes, verts, bodies, iterations=args.iterations, order=0)
-    orc.set_bodies(ctx.bodies())
-    prox, s0 = ctx.bp_proxies()
-    orc.set_proxies(prox, s0)
-    k, m = ctx.pairs()
-    orc.set_pairs(k, m)
+    orc.set_bodies(snap["bodies"])
+    orc.set_proxies(*snap["proxies"])
+    orc.set_pairs(*snap["pairs"])
     n = 0
     secs = 0.0
-    while secs < args.cpu_seconds and n < 50:
+    res = []
+    while secs < args.cpu_seconds and n < args.steps:
         secs += orc.time_steps(1)
+        res.append(orc.stats()["step_residual"])
         n += 1
-        log(f"[bench] cpu baseline step {n}: {secs:.1f}s")
+        log(f"[bench] cpu baseline step {first_timed + n - 1}: {secs:.1f}s, residual {res[-1]:.3e}")
+    residual["reference_order"] = float(res[0])
+    residual["reference_order_per_step"] = [float(r) for r in res]
     v = args.bodies * n / secs
     return {"value": round(v, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the same {args.bodies}-body pile continued from the GPU state after step "
-                      f"{after_step}, single-threaded C++ restatement of the Java path "
-                      f"(reference island order, Random(0) shuffle), host {os.cpu_count()} cores, 1 used"}
+            "sample": f"steps {first_timed}-{first_timed + n - 1} (the first {n} of the timed window) of the same "
+                      f"{args.bodies}-body pile from the device state at the window's start; single-threaded C++ "
+                      f"restatement of the Java path (reference island order, Random(0) shuffle), host "
+                      f"{os.cpu_count()} cores, 1 used"}
+
+
+def summarize_trace(d, steps):
+    """Per-kernel summary of the dispatches between the two window markers (kCopyProbe of TRACE_MARK_BYTES,
+    launched by --trace-window right before and after the timed steps) in a rocprofv3 kernel trace (csv)."""
+    import csv
+    files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("kernel_trace.csv")]
+    if not files:
+        raise SystemExit(f"no kernel_trace.csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    # each marker is one jbgpu_measure_copy call of TRACE_MARK_BYTES: a warm-up and a timed kCopyProbe dispatch;
+    # the window lies between the second dispatch of the first marker and the first of the second (the
+    # copy-ceiling probe after the window copies 1 GiB, far longer than a marker)
+    marks = [i for i, r in enumerate(rows) if "kCopyProbe" in r["Kernel_Name"]
+             and int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) < 200_000]
+    if len(marks) < 4:
+        raise SystemExit("window markers not found")
+    a, b = marks[1], marks[2]
+    win = [r for r in rows[a + 1:b] if not r["Kernel_Name"].startswith("__amd_rocclr")]
+    span_ms = (int(rows[b]["Start_Timestamp"]) - int(rows[a]["End_Timestamp"])) / 1e6
+    agg = {}
+    for r in win:
+        name = r["Kernel_Name"].split("(")[0]
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3  # us
+        e = agg.setdefault(name, [0, 0.0])
+        e[0] += 1
+        e[1] += dur
+    total = sum(v[1] for v in agg.values())
+    print(f"window: {len(win)} dispatches over {steps} steps, {span_ms:.3f} ms between the markers "
+          f"({span_ms / steps:.3f} ms/step wall), kernel time {total / 1e3:.3f} ms ({total / 1e3 / steps:.3f} ms/step)")
+    print(f"{'kernel':48s} {'calls':>7s} {'calls/step':>10s} {'total_ms':>9s} {'ms/step':>8s} {'avg_us':>8s} {'%':>6s}")
+    for name, (n, us) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"{name[:48]:48s} {n:7d} {n / steps:10.1f} {us / 1e3:9.3f} {us / 1e3 / steps:8.3f} {us / n:8.2f} "
+              f"{100 * us / total:6.2f}")
 
 
 if __name__ == "__main__":

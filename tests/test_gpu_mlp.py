@@ -144,3 +144,48 @@ def test_learned_warm_start_solver_parity():
     print(f"[learned parity] {checked} contacts checked", flush=True)
     assert checked > 1000
     ctx.close()
+
+
+def test_learned_warm_start_parity_on_the_pyramid_scene():
+    """The learned warm-start configuration at its full size: 64k boxes in 32 pyramids and walls plus 2,000
+    falling spheres.  60 device steps with the model driving the warm start, then two steps checked bit for bit
+    against the oracle fed the model's impulses (features bit-exact too)."""
+    sc = scenes.pyramids(seed=1)
+    shapes, verts, bodies = scenes.build_records(sc, jbgpu)
+    print(f"[learned C5] {sc.n_bodies} bodies", flush=True)
+    ctx = jbgpu.Context(num_iterations=10)
+    ctx.world_create(shapes, verts, bodies)
+    orc = pyoracle.World(shapes, verts, bodies, iterations=10, order=jbgpu.ORDER_EXPLICIT)
+    p, _, _ = _model(13, last_scale=0.05)
+    norm = np.concatenate([np.zeros(16), np.ones(16), [0.5, 0.05, 0.05], [0.1, 0.0, 0.0]]).astype(np.float32)
+    ctx.mlp_set_params(p, norm)
+    ctx.solver_use_mlp(True)
+    ctx.step(n=60)
+    ctx.solver_enable_features(True)
+    orc.enable_features(True)
+    for step in range(2):
+        b0 = ctx.bodies()
+        prox, s0 = ctx.bp_proxies()
+        k0, m0 = ctx.pairs()
+        ctx.step()
+        order, _ = ctx.solver_order()
+        na = len(order)
+        feats = ctx.solver_features(4 * na)
+        _, man = ctx.pairs()
+        mm = man[order]
+        npts = mm[:, 0].view(np.int32)
+        dist = mm[:, 4:].reshape(na, 4, 32)[:, :, 15]
+        valid = ((np.arange(4)[None, :] < npts[:, None]) & (dist <= 0)).reshape(-1)
+        pred = ctx.mlp_infer(np.where(valid[:, None], feats, 0.0))
+        orc.set_bodies(b0)
+        orc.set_proxies(prox, s0)
+        orc.set_pairs(k0, m0)
+        orc.set_order(order)
+        orc.set_learned_init(pred[valid])
+        orc.step()
+        fo = orc.features()
+        assert np.array_equal(fo.view(np.uint32), feats[valid].view(np.uint32)), f"features step {61 + step}"
+        bg, bo = ctx.bodies(), orc.bodies()
+        assert np.array_equal(bg.view(np.uint32), bo.view(np.uint32)), f"bodies step {61 + step}"
+        print(f"[learned C5] step {61 + step}: {int(valid.sum())} contacts, bit-exact", flush=True)
+    ctx.close()
