@@ -228,6 +228,14 @@ int jbgpu_mlp_set_params(jbgpu_ctx* ctx, const float* params, const float* norm)
 int jbgpu_mlp_get_params(jbgpu_ctx* ctx, float* params, float* norm);
 /* y[n][3] = model(x[n][16]) (host arrays). */
 int jbgpu_mlp_infer(jbgpu_ctx* ctx, const float* x, int64_t n, float* y);
+/* jbgpu_mlp_infer without any device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The features and predictions are staged through a fixed buffer the library keeps on every device
+ * it runs on until the process ends (JBGPU_MLP_STATIC_MAX contacts, 4.98 MB); at most JBGPU_MLP_STATIC_MAX
+ * contacts per call (JBGPU_ERR_CAPACITY beyond).  Calls that overlap on one GPU (several threads or contexts)
+ * race on that buffer.  Same results as jbgpu_mlp_infer. */
+#define JBGPU_MLP_STATIC_MAX 65536
+int jbgpu_mlp_infer_static(jbgpu_ctx* ctx, const float* x, int64_t n, float* y);
 /* Gradient (JBGPU_MLP_PARAMS floats) of mean((y' - t')^2) over the n x 3 normalised outputs. */
 int jbgpu_mlp_grad(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, float* grad, float* loss);
 /* Adam training from He-uniform weights (seeded), normalisation taken from the data;

@@ -129,6 +129,7 @@ struct RayTree {
 
 struct Ctx {
   hipStream_t stream = nullptr;
+  int device = 0;  // the HIP device current when the context was created
   std::string err;
   // configuration
   int bpMode = JBGPU_BP_DBVT;

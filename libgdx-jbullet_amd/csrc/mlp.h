@@ -35,6 +35,11 @@ void mlpSetParams(Ctx& c, const float* params, const float* norm);
 void mlpGetParams(Ctx& c, float* params, float* norm);
 // predictions for n feature rows (host arrays)
 void mlpInfer(Ctx& c, const float* x, int64_t n, float* y);
+// the same through a fixed per-device staging buffer (no device allocation); n <= kStaticInferMax.
+// mlpStaticInit looks the buffer's device address up for the current device (once per device).
+constexpr int64_t kStaticInferMax = 65536;
+void mlpStaticInit();
+void mlpInferStatic(Ctx& c, const float* x, int64_t n, float* y);
 // gradient of the mean squared error (normalised outputs) at the current parameters
 void mlpGrad(Ctx& c, const float* x, const float* y, int64_t n, float* grad, float* loss);
 // Adam training on the device from host data; normalisation from the data; loss per epoch

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 
 #include "mlp.h"
@@ -547,6 +548,52 @@ void mlpInfer(Ctx& c, const float* x, int64_t n, float* y) {
   JB_CHECK(hipMemcpyAsync(y, c.mlpY.p, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
   if (c.timing) tCollect(c);  // outside a step: the kernel's timing region is read here
+}
+
+// ---- inference without device allocation ---------------------------------------------------------
+// Staging of jbgpu_mlp_infer_static: features (16 floats) then predictions (3 floats) of at most
+// kStaticInferMax contacts, 4.98 MB, held by every device the library runs on until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: kernels and copies use the device address that
+// hipGetSymbolAddress returns for the current device (kept per device in g_scratchDev).
+static __device__ __align__(256) float scratch[kStaticInferMax * (kMlpIn + kMlpOut)];
+namespace {
+std::mutex g_scratchMu;
+std::vector<float*> g_scratchDev;  // device address of `scratch` per device id (nullptr: not looked up yet)
+}  // namespace
+
+void mlpStaticInit() {
+  int dev = 0;
+  JB_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if ((int)g_scratchDev.size() <= dev) g_scratchDev.resize((size_t)dev + 1, nullptr);
+  if (g_scratchDev[dev]) return;
+  void* p = nullptr;
+  JB_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratchDev[dev] = (float*)p;
+}
+
+void mlpInferStatic(Ctx& c, const float* x, int64_t n, float* y) {
+  if (!c.mlpReady) throw std::logic_error("no model loaded or trained");
+  if (n < 0 || (n > 0 && (!x || !y))) throw std::invalid_argument("bad feature arrays");
+  if (n > kStaticInferMax)
+    throw std::length_error("jbgpu_mlp_infer_static takes at most " + std::to_string(kStaticInferMax) +
+                            " contacts per call");
+  if (n == 0) return;
+  float* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (c.device < (int)g_scratchDev.size()) base = g_scratchDev[c.device];
+  }
+  if (!base) throw std::logic_error("static inference buffer not set up on this device");
+  float* dX = base;                                  // n x 16 features
+  float* dY = base + (size_t)kMlpIn * kStaticInferMax;  // n x 3 predictions
+  hipStream_t s = c.stream;
+  JB_CHECK(hipMemcpyAsync(dX, x, (size_t)kMlpIn * n * sizeof(float), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(kMlpInfer<false>, dim3(inferGrid(n)), dim3(256), 0, s, n, (const float*)dX, c.mlpWb.p,
+                     c.mlpW.p, c.mlpNorm.p, dY, nullptr, nullptr, nullptr);
+  JB_CHECK(hipGetLastError());
+  JB_CHECK(hipMemcpyAsync(y, dY, (size_t)kMlpOut * n * sizeof(float), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
 }
 
 namespace {

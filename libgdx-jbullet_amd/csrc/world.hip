@@ -472,8 +472,16 @@ int jbgpu_ctx_create(const jbgpu_config* cfg, jbgpu_ctx** out) {
   c.warmstart = cfg->warmstart;
   c.erp = cfg->erp;
   c.warmFactor = cfg->warmstart_factor;
-  if (hipStreamCreate(&c.stream) != hipSuccess) {
+  if (hipGetDevice(&c.device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess) {
     g_createError = "hipStreamCreate failed";
+    delete ctx;
+    return JBGPU_ERR_HIP;
+  }
+  try {
+    mlpStaticInit();  // the device address of the static inference buffer, once per device
+  } catch (const std::exception& e) {
+    g_createError = e.what();
+    (void)hipStreamDestroy(c.stream);
     delete ctx;
     return JBGPU_ERR_HIP;
   }
@@ -1057,6 +1065,7 @@ int jbgpu_solver_residual_history(jbgpu_ctx* ctx, float* out, int32_t cap, int32
 // ---- learned warm start ----------------------------------------------------------------
 static_assert(jbg::kTimerRegions == JBGPU_TIMER_REGIONS, "timer regions");
 static_assert(jbg::kMlpParams == JBGPU_MLP_PARAMS && jbg::kMlpNormWords == JBGPU_MLP_NORM_WORDS, "MLP layout");
+static_assert(jbg::kStaticInferMax == JBGPU_MLP_STATIC_MAX, "static inference limit");
 
 int jbgpu_mlp_set_params(jbgpu_ctx* ctx, const float* params, const float* norm) {
   return guard(ctx, [&] { mlpSetParams(ctx->c, params, norm); });
@@ -1068,6 +1077,10 @@ int jbgpu_mlp_get_params(jbgpu_ctx* ctx, float* params, float* norm) {
 
 int jbgpu_mlp_infer(jbgpu_ctx* ctx, const float* x, int64_t n, float* y) {
   return guard(ctx, [&] { mlpInfer(ctx->c, x, n, y); });
+}
+
+int jbgpu_mlp_infer_static(jbgpu_ctx* ctx, const float* x, int64_t n, float* y) {
+  return guard(ctx, [&] { mlpInferStatic(ctx->c, x, n, y); });
 }
 
 int jbgpu_mlp_grad(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, float* grad, float* loss) {

@@ -22,6 +22,7 @@ TIMER_REGIONS = 16
 TIMER_NAMES = ["predict", "setaabb", "traverse", "gjk", "manifold", "solver_setup", "solver_iter", "integrate",
                "step", "epa", "remap", "mlp", "bvh", "keysort", "bin", "color"]
 MLP_PARAMS, MLP_NORM_WORDS = 9603, 38
+MLP_STATIC_MAX = 65536
 
 # every exported symbol of include/jbgpu.h: (restype, argtypes)
 _f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -92,6 +93,7 @@ SYMBOLS = {
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
     "jbgpu_mlp_get_params": (C.c_int, [_vp, _f, _f]),
     "jbgpu_mlp_infer": (C.c_int, [_vp, _f, C.c_int64, _f]),
+    "jbgpu_mlp_infer_static": (C.c_int, [_vp, _f, C.c_int64, _f]),
     "jbgpu_mlp_grad": (C.c_int, [_vp, _f, _f, C.c_int64, _f, C.POINTER(C.c_float)]),
     "jbgpu_mlp_train": (C.c_int, [_vp, _f, _f, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _vp]),
     "jbgpu_solver_use_mlp": (C.c_int, [_vp, C.c_int32]),
@@ -414,6 +416,15 @@ class Context:
         y = np.zeros((max(len(x), 1), 3), np.float32)
         self._ck(load().jbgpu_mlp_infer(self.h, x.reshape(-1) if len(x) else np.zeros(16, np.float32), len(x),
                                         y.reshape(-1)), "mlp_infer")
+        return y[:len(x)]
+
+    def mlp_infer_static(self, x):
+        """mlp_infer through the library's fixed per-device staging buffer (no device allocation; at most
+        MLP_STATIC_MAX contacts)."""
+        x = f32(x).reshape(-1, 16)
+        y = np.zeros((max(len(x), 1), 3), np.float32)
+        self._ck(load().jbgpu_mlp_infer_static(self.h, x.reshape(-1) if len(x) else np.zeros(16, np.float32),
+                                               len(x), y.reshape(-1)), "mlp_infer_static")
         return y[:len(x)]
 
     def mlp_grad(self, x, t):
