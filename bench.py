@@ -383,9 +383,12 @@ def roofline(ks, args):
 
 
 def batched(args):
-    """Batched independent worlds: towers of spheres and boxes, own floor plane per world, 10 iterations."""
+    """Batched independent worlds: towers of spheres and boxes (sizes as in the mixed pile), own floor plane per
+    world, 10 iterations.  The world -> workgroup mapping of the solve is autotuned first (untimed steps), then
+    --batched-steps steps are timed.  CPU baseline: the oracle stepping the first worlds of the same state (as
+    4096 separate worlds would be stepped one after another), about --cpu-seconds of work."""
     import jbgpu
-    from jbgpu import scenes
+    from jbgpu import autotune, scenes
     t = time.time()
     sc = scenes.batched_towers(seed=args.seed, n_worlds=args.batched_worlds)
     shapes, verts, bodies = scenes.build_records(sc, jbgpu)
@@ -393,7 +396,11 @@ def batched(args):
     log(f"[bench] batched: {args.batched_worlds} worlds, {nb} dynamic bodies, built in {time.time() - t:.1f}s")
     ctx = jbgpu.Context(num_iterations=10)
     ctx.world_create(shapes, verts, bodies, world_ids=sc.world)
-    ctx.step(n=max(args.batched_steps // 2, 1))
+    warm = max(args.batched_steps // 2, 1)
+    ctx.step(n=warm)
+    tune = autotune.world_blocks(ctx, log=log)
+    first = warm + tune["steps"] + 1
+    snap = dict(bodies=ctx.bodies(), proxies=ctx.bp_proxies(), pairs=ctx.pairs())
     ctx.timing(True)
     t0 = time.perf_counter()
     ctx.step(n=args.batched_steps)
@@ -401,13 +408,56 @@ def batched(args):
     st = ctx.stats()
     regions, nsteps = ctx.timing_get()
     ctx.close()
-    return {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": args.batched_steps,
-            "warmup": max(args.batched_steps // 2, 1), "ms_per_step": round(1e3 * secs / args.batched_steps, 3),
-            "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
-            round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
-            "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
-            "last_step_epa_calls": st["epa_calls"],
-            "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0}}
+    out = {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": args.batched_steps,
+           "timed_window": f"steps {first}-{first + args.batched_steps - 1}",
+           "ms_per_step": round(1e3 * secs / args.batched_steps, 3),
+           "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
+           round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
+           "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
+           "last_step_epa_calls": st["epa_calls"], "autotune": tune,
+           "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0},
+           "phase_note": "event timing on (events between phases) during the timed steps"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = batched_cpu_baseline(shapes, verts, bodies, sc.world, snap, args, first)
+    return out
+
+
+def batched_cpu_baseline(shapes, verts, bodies, world, snap, args, first):
+    """The oracle (1 core) stepping whole worlds of the batched scene from the device state at the start of the
+    timed window: worlds 0, 1, 2, ... (one DiscreteDynamicsWorld each, stepped one after another), as many as
+    fit in about --cpu-seconds, each for two steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    world = np.asarray(world)
+    secs, body_steps, worlds = 0.0, 0, 0
+    keys, man = snap["pairs"]
+    lo = (keys >> np.uint64(32)).astype(np.int64) - 1
+    prox, s0 = snap["proxies"]
+    for w in range(int(world.max()) + 1):
+        if secs >= args.cpu_seconds:
+            break
+        idx = np.nonzero(world == w)[0]
+        remap = np.full(len(world), -1, np.int64)
+        remap[idx] = np.arange(len(idx))
+        sel = np.nonzero(world[lo] == w)[0]
+        k = keys[sel]
+        a = remap[(k >> np.uint64(32)).astype(np.int64) - 1] + 1
+        b = remap[(k & np.uint64(0xffffffff)).astype(np.int64) - 1] + 1
+        m = man[sel].copy()
+        hv = m[:, 1:3].view(np.int32)
+        hv[:] = remap[hv]
+        orc = pyoracle.World(shapes, verts, bodies[idx], iterations=10, order=0)
+        orc.set_bodies(snap["bodies"][idx])
+        orc.set_proxies(prox[idx], s0)
+        orc.set_pairs((a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64), m)
+        secs += orc.time_steps(2)
+        body_steps += 2 * int((bodies[idx, 18] != 0).sum())
+        worlds += 1
+    log(f"[bench] batched cpu baseline: {worlds} worlds, {body_steps} body-steps in {secs:.1f}s")
+    return {"value": round(body_steps / secs, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": f"worlds 0-{worlds - 1} of the same scene, each stepped 2 steps (steps {first}-{first + 1}) "
+                      f"as its own world from the device state at the timed window's start; single-threaded C++ "
+                      f"restatement of the Java path (reference island order, Random(0) shuffle per world)"}
 
 
 def broadphase_1m(args):
@@ -603,8 +653,9 @@ def summarize_trace(d, steps):
     win = [r for r in rows[a + 1:b] if not r["Kernel_Name"].startswith("__amd_rocclr")]
     span_ms = (int(rows[b]["Start_Timestamp"]) - int(rows[a]["End_Timestamp"])) / 1e6
     agg = {}
+    import re
     for r in win:
-        name = r["Kernel_Name"].split("(")[0]
+        name = re.sub(r"^void ", "", r["Kernel_Name"].replace("(anonymous namespace)::", "")).split("(")[0]
         dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3  # us
         e = agg.setdefault(name, [0, 0.0])
         e[0] += 1
@@ -612,9 +663,9 @@ def summarize_trace(d, steps):
     total = sum(v[1] for v in agg.values())
     print(f"window: {len(win)} dispatches over {steps} steps, {span_ms:.3f} ms between the markers "
           f"({span_ms / steps:.3f} ms/step wall), kernel time {total / 1e3:.3f} ms ({total / 1e3 / steps:.3f} ms/step)")
-    print(f"{'kernel':48s} {'calls':>7s} {'calls/step':>10s} {'total_ms':>9s} {'ms/step':>8s} {'avg_us':>8s} {'%':>6s}")
+    print(f"{'kernel':64s} {'calls':>7s} {'calls/step':>10s} {'total_ms':>9s} {'ms/step':>8s} {'avg_us':>8s} {'%':>6s}")
     for name, (n, us) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
-        print(f"{name[:48]:48s} {n:7d} {n / steps:10.1f} {us / 1e3:9.3f} {us / 1e3 / steps:8.3f} {us / n:8.2f} "
+        print(f"{name[:64]:64s} {n:7d} {n / steps:10.1f} {us / 1e3:9.3f} {us / 1e3 / steps:8.3f} {us / n:8.2f} "
               f"{100 * us / total:6.2f}")
 
 

@@ -164,7 +164,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out);
  * are checked first (jbgpu_check_pair_records); a record with points must carry the dispatcher's header (body0
  * the lower uid, except a convex-plane pair whose lower uid is the plane, which lists the convex body first;
  * type as JBGPU_PAIR_*), a record without points gets that header written.  A rejected call changes
- * nothing. */
+ * nothing.  manifolds_out may be manifolds_in. */
 int jbgpu_np_dispatch(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds_in,
                       float* manifolds_out);
 
@@ -205,6 +205,15 @@ int jbgpu_solver_set_params(jbgpu_ctx* ctx, float tolerance, int32_t max_iterati
  * manifolds inside a block are swept by one workgroup per block with the block's velocities in LDS,
  * colour after colour, the others by one launch per colour.  0 (the default) disables blocks. */
 int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size);
+/* Batched worlds (no reference counterpart; results are identical for every value): with bodies_per_block > 0
+ * every fixed-iteration solve runs one workgroup of `threads` threads (64, 128, 256 or 512) per block of whole
+ * worlds (consecutive world ids, at most bodies_per_block dynamic bodies, in [16, 1024]; a larger world forms a
+ * block of its own, up to 1024 dynamic bodies): the workgroup colours the block's manifolds, applies the warm
+ * start and runs every iteration with the block's velocities in LDS, so a solve is one launch.  The manifold
+ * order (jbgpu_solver_get_order) is then block after block, colour after colour inside a block, reported as one
+ * batch.  0 (the default) solves with one launch per colour batch.  Tolerance-driven solves (tolerance > 0)
+ * always use the per-colour launches. */
+int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads);
 /* ContactSolverInfo.warmstartingFactor for the following solves (0.85 by default). */
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor);
 /* Launch shaping of the colour sweeps (no reference counterpart; results are identical for every
@@ -242,7 +251,10 @@ int jbgpu_mlp_grad(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, fl
  * loss_per_epoch may be NULL. */
 int jbgpu_mlp_train(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
                     float lr, uint64_t seed, float* loss_per_epoch);
-/* on != 0: every solve takes its warm-start impulses from the model (needs a model). */
+/* mode 1: every solve takes the warm-start impulses of every contact from the model; mode 2: only contacts
+ * without cached impulses (all three zero, e.g. points new in this step) take the model's, the others keep the
+ * warm-start factor times their cached impulses as SequentialImpulseConstraintSolver does; 0: off.  Needs a
+ * model. */
 int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 
 /* ---- measurement ---------------------------------------------------------------------- */

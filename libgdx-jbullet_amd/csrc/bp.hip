@@ -158,6 +158,15 @@ __global__ void kSetAabbArray(BpParams P, int n, const float* __restrict__ in6, 
   writeCandidate(P, i, aabb, q, cand, isHuge);
 }
 
+// a pair query with no setAabb since the last one (e.g. right after createProxy): the stored boxes stay, the
+// collide's stage rotation still runs (DbvtBroadphase.java:96-113), the candidate boxes come from the state
+__global__ void kKeepAabbs(BpParams P, int n, int* stage, float* leaf, float* aabb, int* q, float* cand, int* isHuge) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  setAabbDev(P, i, v3(0, 0, 0), v3(0, 0, 0), false, stage, leaf, aabb, q);
+  writeCandidate(P, i, aabb, q, cand, isHuge);
+}
+
 __device__ __forceinline__ unsigned int fOrd(float f) {  // order-preserving float -> uint
   unsigned int u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -917,6 +926,7 @@ void bpUpdateFromBodies(Ctx& c) {
                      c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
   tEnd(c, T_SETAABB);
   c.stageCurrent = P.newStage;
+  c.rotationPending = true;
 }
 
 void bpUpdateFromAabbs(Ctx& c, const float* dAabb6) {
@@ -927,6 +937,7 @@ void bpUpdateFromAabbs(Ctx& c, const float* dAabb6) {
                      c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
   tEnd(c, T_SETAABB);
   c.stageCurrent = P.newStage;
+  c.rotationPending = true;
 }
 
 static int bitsFor(int64_t n) {
@@ -1064,6 +1075,14 @@ void bpFindPairs(Ctx& c) {
   // previous frame found several pairs per proxy (a settled pile: 10), not in sparse scenes (1M scattered boxes:
   // 0.07), where the traversals are short.  Both traversals report the same pairs.
   const bool wide = c.npairs >= kWidePairsPerProxy * (int64_t)n;
+  if (!c.rotationPending) {  // no setAabb since the last pair query: the boxes stay, the stages rotate
+    ensureProxyBuffers(c);
+    BpParams P = makeParams(c);
+    hipLaunchKernelGGL(kKeepAabbs, dim3(gridFor(n)), dim3(256), 0, s, P, n, c.stage.p, c.leaf.p, c.aabb.p, c.q.p,
+                       c.cand.p, c.isHuge.p);
+    c.stageCurrent = P.newStage;
+  }
+  c.rotationPending = false;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   // centre bounds and extent statistics (two-stage reduction)
   tBegin(c, T_BVH);

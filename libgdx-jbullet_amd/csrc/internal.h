@@ -159,6 +159,7 @@ struct Ctx {
   // broadphase
   int np = 0;  // proxies
   int stageCurrent = 0;
+  bool rotationPending = false;  // a setAabb pass ran the collide's stage rotation for the next pair query
   DBuf<int> stage;
   DBuf<float> leaf, aabb;  // np*6
   DBuf<int> filt;          // group | mask << 16
@@ -206,6 +207,21 @@ struct Ctx {
   bool epaOverflow = false;  // the last narrowphase pass overflowed an EPA workspace
   std::vector<int> shapeTypeHost;  // host copies: type per shape and per body (canonical manifold headers)
   std::vector<int> bodyShapeType;
+  std::vector<int> worldHost;       // host copies: world id and static flag per body (world-block tables)
+  std::vector<char> bodyStaticHost;
+  // batched worlds solved one workgroup per block of whole worlds (the whole solve in one launch)
+  int wbBodies = 0;      // at most this many dynamic bodies per block (0: off)
+  int wbThreads = 256;   // workgroup size of the block solve
+  bool wbReady = false;  // the tables below match wbBodies and the bodies
+  int wbBlocks = 0, wbMaxBodies = 0;
+  DBuf<int> wbBodyList;   // dynamic bodies, block after block
+  DBuf<int> wbBodyStart;  // per block: first entry of wbBodyList (wbBlocks + 1 entries)
+  DBuf<int> wbBodyBlock;  // per body: its block, -1 for static bodies
+  DBuf<int> wbBodyLocal;  // per body: position in its block, -1 for static bodies
+  DBuf<int> wbKey, wbKeyAlt, wbIdx, wbSlotStart, wbColour, wbSolveSlot;
+  DBuf<int> wbInfo;       // [0] error flags, [1] most colours in a block
+  HBuf<int> wbInfoHost;
+  bool wbLastSolve = false;
   // ConstraintSolver seam: islands recorded between prepareSolve and allSolved
   bool solveBatchOpen = false;
   std::vector<int> islandManifolds;
@@ -233,6 +249,7 @@ struct Ctx {
   // learned warm start (16-64-64-64-3 MLP, bf16 weights, fp32 accumulate)
   bool mlpReady = false;
   bool useMlp = false;       // the model predicts the warm-start impulses of every solve
+  bool mlpNewOnly = false;   // ... of the contacts without cached impulses only
   DBuf<float> mlpW;          // fp32 master parameters
   DBuf<uint16_t> mlpWb;      // bf16 MFMA operand fragments (forward W, backward W^T)
   DBuf<float> mlpNorm;       // input / output normalisation
@@ -347,5 +364,23 @@ void worldIntegrate(Ctx& c, float dt);
 void worldActivation(Ctx& c, float dt);
 
 inline int gridFor(int64_t n, int block = 256) { return (int)((n + block - 1) / block); }
+
+// device-wide primitives (prim.hip): the element count is read from *count (device memory) or, when count is
+// null, taken from hostCount; launch shapes depend only on the capacity cap (count clamped to it), so the
+// launches can be captured in a hipGraph.  scratch: primScratchInts(cap) ints of device memory.
+int primTiles(int cap);
+size_t primScratchInts(int cap, int keyBytes = 8);
+// out[i] = in[0] + ... + in[i - 1] for i < n; *total (device, may be null) = sum of in[0, n)
+void primExclusiveScan(hipStream_t s, const int* in, int* out, const int* count, int hostCount, int cap, int* scratch,
+                       int* total);
+// out[0, m) = the i < n with flag[i] != 0 in index order (vals[i] instead of i when vals is not null);
+// *selectedCount (device) = m
+void primSelectFlagged(hipStream_t s, const int* flag, const int* vals, int* out, const int* count, int hostCount,
+                       int cap, int* scratch, int* selectedCount);
+// stable LSD radix sort of the first n keys (and int values, when v0 is not null) by bits [bitLo, bitHi) rounded
+// up to whole 8-bit digits (the keys' bits above that must be zero); returns true when the result is in (k1, v1)
+template <class K>
+bool primRadixSort(hipStream_t s, K* k0, int* v0, K* k1, int* v1, const int* count, int hostCount, int cap, int bitLo,
+                   int bitHi, int* scratch);
 
 }  // namespace jbg

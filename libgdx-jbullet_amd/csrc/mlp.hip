@@ -256,7 +256,9 @@ __device__ inline InferIn inferLoad(int64_t c, int64_t n, const float* __restric
 
 // one tile of a wave: contacts 16t..16t+15 from a ring slot, which is refilled (Refill) with the features of the
 // wave's tile kInferAhead tiles later before the products are computed
-template <bool Fused, bool Refill>
+// NewOnly (fused): only contacts without cached impulses (features 12-14 all zero, e.g. points new this step)
+// take the prediction; the others keep the scaled cached impulses already in their rows
+template <bool Fused, bool Refill, bool NewOnly>
 __device__ inline void inferTile(InferIn& slot, int64_t t, int64_t n, int64_t ahead, const InferOps& o,
                                  const float4& mean, const float4& scale, const float* __restrict__ X,
                                  const float* __restrict__ norm, float* __restrict__ Y, const int4* __restrict__ hdr,
@@ -270,9 +272,14 @@ __device__ inline void inferTile(InferIn& slot, int64_t t, int64_t n, int64_t ah
                           (slot.v.z - mean.z) * scale.z, (slot.v.w - mean.w) * scale.w);
   const u32x2 keep = ok ? u32x2{~0u, ~0u} : u32x2{0u, 0u};
   const bf16x4 h0 = __builtin_bit_cast(bf16x4, __builtin_bit_cast(u32x2, hv) & keep);
+  int fresh = 1;
+  if (NewOnly) {  // lanes 48-63 hold features 12-15 of the tile's contacts: cached impulses, friction
+    const int none = (slot.v.x == 0.0f && slot.v.y == 0.0f && slot.v.z == 0.0f) ? 1 : 0;
+    fresh = __shfl(none, (lane & 15) + 48);
+  }
   if (Refill) slot = inferLoad<Fused>(c + ahead, n, X, hdr, k4);
   const f32x4 y = forwardRegs(o, h0);
-  if (lane < 16 && ok) {
+  if (lane < 16 && ok && fresh) {
     float y0 = y.x * norm[32] + norm[35], y1 = y.y * norm[33] + norm[36], y2 = y.z * norm[34] + norm[37];
     if (Fused) {  // dense rows: the slot's first row, then one row per valid point in point order
       const int j = (int)(c & 3);
@@ -291,7 +298,7 @@ __device__ inline void inferTile(InferIn& slot, int64_t t, int64_t n, int64_t ah
 // inference: 256-thread blocks, one 16-contact tile per wave and iteration, operands in registers, the
 // features of the wave's next kInferAhead tiles in flight.  Fused: the rows of valid slots (hdr) receive the
 // predictions (warm start); otherwise y[n][3] is written.
-template <bool Fused>
+template <bool Fused, bool NewOnly = false>
 __global__ __launch_bounds__(256) void kMlpInfer(int64_t n, const float* __restrict__ X, const uint16_t* __restrict__ P,
                                                  const float* __restrict__ W, const float* __restrict__ norm,
                                                  float* __restrict__ Y, const int4* __restrict__ hdr, float* nrow,
@@ -332,13 +339,13 @@ __global__ __launch_bounds__(256) void kMlpInfer(int64_t n, const float* __restr
   for (int64_t g = tilesOfWave / kInferAhead; g > 0; g--) {
 #pragma unroll
     for (int i = 0; i < kInferAhead; i++, t += nwaves)
-      inferTile<Fused, true>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
+      inferTile<Fused, true, NewOnly>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
   }
   const int rest = (int)(tilesOfWave % kInferAhead);
 #pragma unroll
   for (int i = 0; i < kInferAhead - 1; i++, t += nwaves) {
     if (i >= rest) break;
-    inferTile<Fused, false>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
+    inferTile<Fused, false, NewOnly>(ring[i], t, n, ahead, o, mean, scale, X, norm, Y, hdr, nrow, frow, lane, k4);
   }
 }
 
@@ -715,8 +722,12 @@ void mlpWarmStart(Ctx& c, int na) {
   if (!c.mlpReady) throw std::logic_error("no model loaded or trained");
   const int64_t n = 4 * (int64_t)na;
   tBegin(c, T_MLP);
-  hipLaunchKernelGGL(kMlpInfer<true>, dim3(inferGrid(n)), dim3(256), 0, c.stream, n, c.features.p, c.mlpWb.p, c.mlpW.p,
-                     c.mlpNorm.p, nullptr, c.hdr.p, c.nrow.p, c.frow.p);
+  if (c.mlpNewOnly)
+    hipLaunchKernelGGL((kMlpInfer<true, true>), dim3(inferGrid(n)), dim3(256), 0, c.stream, n, c.features.p, c.mlpWb.p,
+                       c.mlpW.p, c.mlpNorm.p, nullptr, c.hdr.p, c.nrow.p, c.frow.p);
+  else
+    hipLaunchKernelGGL((kMlpInfer<true, false>), dim3(inferGrid(n)), dim3(256), 0, c.stream, n, c.features.p, c.mlpWb.p,
+                       c.mlpW.p, c.mlpNorm.p, nullptr, c.hdr.p, c.nrow.p, c.frow.p);
   JB_CHECK(hipGetLastError());
   tEnd(c, T_MLP);
 }

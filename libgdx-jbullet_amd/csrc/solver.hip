@@ -741,8 +741,9 @@ __device__ __forceinline__ void normalRound(int q, bool mine, const float* N, co
   quadBcast<L>(B);
 }
 // normal rows of slot p, lane q of its quad
+template <class Bodies>
 __device__ __forceinline__ void normalQuad(int p, int q, const SolverParams& SP, const int4* __restrict__ hdr,
-                                           float* nrow, const GlobalBodies& io, double& dd, double& nn) {
+                                           float* nrow, const Bodies& io, double& dd, double& nn) {
   const int4 h = hdr[p];
   const int nr = __popc(slotMask(h));
   const bool mine = q < nr;
@@ -769,8 +770,9 @@ __device__ __forceinline__ void frictionRound(int q, bool mine, const float* F, 
   quadBcast<L>(B);
 }
 // friction rows of slot p, lane q of its quad (the point's two lateral rows in order)
+template <class Bodies>
 __device__ __forceinline__ void frictionQuad(int p, int q, const SolverParams& SP, const int4* __restrict__ hdr,
-                                             const float* __restrict__ nrow, float* frow, const GlobalBodies& io,
+                                             const float* __restrict__ nrow, float* frow, const Bodies& io,
                                              double& dd, double& nn) {
   (void)SP;
   const int4 h = hdr[p];
@@ -979,6 +981,189 @@ __global__ __launch_bounds__(kBlockThreads) void kSweepBlocks(int nBatches, int 
   accumulate(dd, nn, res);
 }
 
+// ---- batched worlds: one workgroup per block of whole worlds ----------------------------------------------
+// Worlds never share a body, so a block made of whole worlds is solved on its own: the workgroup keeps the
+// block's dynamic bodies in LDS, colours the block's manifolds itself (first free colour, slot order), then
+// applies the warm start and runs every iteration colour after colour with workgroup barriers in between, and
+// finally writes the velocities back.  The whole solve is one launch.  The order used (block after block,
+// colour after colour inside a block, slot order inside a colour) is written to colorOrder; a sequential solver
+// fed that manifold order gives the same bits (blocks are independent, a colour's manifolds share no dynamic
+// body).
+constexpr int kWbMaxColours = 128;
+struct WorldLdsBodies {  // a block's dynamic bodies in LDS (lv, invMass, av, -), static bodies from svel
+  float* lds;
+  const int* __restrict__ local;  // per body: position in its block, -1 for static bodies
+  const float* __restrict__ svel;
+  const float* __restrict__ props;
+  __device__ void load(int p, const int4& h, SBody& A, SBody& B) const {
+    (void)p;
+    const int la = local[h.y], lb = local[h.z];
+    A = la >= 0 ? LdsBodies::get(lds + 8 * la) : loadBody(svel, props, h.y);
+    B = lb >= 0 ? LdsBodies::get(lds + 8 * lb) : loadBody(svel, props, h.z);
+  }
+  __device__ void store(int p, const int4& h, const SBody& A, const SBody& B) const {
+    (void)p;
+    const int la = local[h.y], lb = local[h.z];
+    if (la >= 0) LdsBodies::put(lds + 8 * la, A);
+    if (lb >= 0) LdsBodies::put(lds + 8 * lb, B);
+  }
+};
+
+// warm start of slot p onto bodies held by io (the warm-start part of solveGroupCacheFriendlySetup, :659-747)
+template <class Bodies>
+__device__ __forceinline__ void warmApplySlotIo(int p, const int4* __restrict__ hdr, const float* __restrict__ nrow,
+                                                const float* __restrict__ frow, const Bodies& io) {
+  const int4 h = hdr[p];
+  SBody A, B;
+  io.load(p, h, A, B);
+  const int rows = __popc(slotMask(h)), row0 = slotRow(h);
+  for (int q = 0; q < rows; q++) {
+    for (int r = 0; r < 3; r++) {
+      const float* R = r == 0 ? nrow + (size_t)RW * (row0 + q) : frow + (size_t)RW * (2 * (row0 + q) + r - 1);
+      V3 nrm = ld3(R);
+      float ap = R[19];
+      applyImpulse(A, scl(nrm, A.invMass), ld3(R + 9), ap);
+      applyImpulse(B, scl(nrm, B.invMass), ld3(R + 12), -ap);
+    }
+  }
+  io.store(p, h, A, B);
+}
+
+// blockOf[k] for active manifold k: the block of its dynamic body (both bodies of a manifold with two dynamic
+// bodies are in one world, hence in one block)
+__global__ void kManifoldBlocks(int na, const int2* __restrict__ ab, const int* __restrict__ bodyBlock, int* key,
+                                int* idx, int* info) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= na) return;
+  const int2 b = ab[k];
+  const int ba = bodyBlock[b.x], bb = bodyBlock[b.y];
+  if (ba >= 0 && bb >= 0 && ba != bb) atomicOr(info, 4);  // a manifold across two blocks: not worlds
+  key[k] = ba >= 0 ? ba : bb;
+  idx[k] = k;
+}
+// slotStart[b] = first slot of block b in the block-sorted slot order (slotStart[nBlocks] = na)
+__global__ void kBlockSlotStarts(int na, int nBlocks, const int* __restrict__ key, int* slotStart) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > na) return;
+  const int lo = p == 0 ? -1 : key[p - 1];
+  const int hi = p == na ? nBlocks - 1 : key[p];
+  for (int b = lo + 1; b <= hi; b++) slotStart[b] = p;
+  if (p == na && lo < nBlocks) slotStart[nBlocks] = na;
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void kSolveWorldBlocks(int iters, SolverParams SP, int warm,
+                                                       const int* __restrict__ bodyList,
+                                                       const int* __restrict__ bodyStart,
+                                                       const int* __restrict__ bodyLocal,
+                                                       const int* __restrict__ slotStart,
+                                                       const int* __restrict__ slotManifold, int* slotColour,
+                                                       int* solveSlot, int* colorOrder, const int4* __restrict__ hdr,
+                                                       float* nrow, float* frow, const float* __restrict__ props,
+                                                       const float* __restrict__ svel, float* vel, double* res,
+                                                       int* info) {
+  extern __shared__ float lds[];  // [nbod][8] bodies, then [nbod][2] colour masks (64-bit words)
+  __shared__ int colourStart[kWbMaxColours + 1];
+  __shared__ int count[kWbMaxColours];
+  __shared__ int nColours;
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int q0 = bodyStart[b], nbod = bodyStart[b + 1] - q0;
+  const int s0 = slotStart[b], ns = slotStart[b + 1] - s0;
+  if (ns == 0) return;  // nothing touches this block's bodies: their velocities stay as they are
+  unsigned long long* masks = reinterpret_cast<unsigned long long*>(lds + 8 * nbod);
+  for (int i = t; i < nbod; i += T) {
+    const float4* v = reinterpret_cast<const float4*>(svel + 8 * bodyList[q0 + i]);
+    reinterpret_cast<float4*>(lds)[2 * i] = v[0];  // .w: inverse mass
+    reinterpret_cast<float4*>(lds)[2 * i + 1] = v[1];
+    masks[2 * i] = 0ull;
+    masks[2 * i + 1] = 0ull;
+  }
+  __syncthreads();
+  // colouring: first free colour of the slot's dynamic bodies, slot after slot (one thread; the slots' body
+  // positions are loaded eight at a time ahead of the dependent mask updates)
+  if (t == 0) {
+    for (int c = 0; c < kWbMaxColours; c++) count[c] = 0;
+    int ncol = 0, overflow = 0;
+    for (int i0 = 0; i0 < ns; i0 += 8) {
+      int la[8], lb[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int4 h = i0 + k < ns ? hdr[s0 + i0 + k] : make_int4(0, 0, 0, 0);
+        la[k] = i0 + k < ns ? bodyLocal[h.y] : -1;
+        lb[k] = i0 + k < ns ? bodyLocal[h.z] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (i0 + k >= ns) break;
+        unsigned long long u0 = 0ull, u1 = 0ull;
+        if (la[k] >= 0) { u0 |= masks[2 * la[k]]; u1 |= masks[2 * la[k] + 1]; }
+        if (lb[k] >= 0) { u0 |= masks[2 * lb[k]]; u1 |= masks[2 * lb[k] + 1]; }
+        int c;
+        if (~u0) c = __ffsll((long long)~u0) - 1;
+        else if (~u1) c = 64 + __ffsll((long long)~u1) - 1;
+        else { c = kWbMaxColours - 1; overflow = 1; }
+        const unsigned long long bit = 1ull << (c & 63);
+        if (la[k] >= 0) masks[2 * la[k] + (c >> 6)] |= bit;
+        if (lb[k] >= 0) masks[2 * lb[k] + (c >> 6)] |= bit;
+        slotColour[s0 + i0 + k] = c;
+        count[c]++;
+        ncol = max(ncol, c + 1);
+      }
+    }
+    int acc = 0;
+    for (int c = 0; c < ncol; c++) {
+      colourStart[c] = acc;
+      acc += count[c];
+    }
+    colourStart[ncol] = acc;
+    // stable placement: position of every slot in its colour, in slot order
+    for (int c = 0; c < ncol; c++) count[c] = colourStart[c];
+    for (int i = 0; i < ns; i++) {
+      const int pos = count[slotColour[s0 + i]]++;
+      solveSlot[s0 + pos] = s0 + i;
+    }
+    nColours = ncol;
+    if (overflow) atomicOr(info, 2);
+    atomicMax(info + 1, ncol);
+  }
+  __syncthreads();
+  const int ncol = nColours;
+  for (int j = t; j < ns; j += T) colorOrder[s0 + j] = slotManifold[solveSlot[s0 + j]];
+  const WorldLdsBodies io{lds, bodyLocal, svel, props};
+  if (warm) {
+    for (int c = 0; c < ncol; c++) {
+      for (int j = colourStart[c] + t; j < colourStart[c + 1]; j += T) warmApplySlotIo(solveSlot[s0 + j], hdr, nrow, frow, io);
+      __syncthreads();
+    }
+  }
+  double dd = 0.0, nn = 0.0;
+  for (int it = 0; it < iters; it++) {
+    double dd1 = 0.0, nn1 = 0.0;
+    for (int c = 0; c < ncol; c++) {
+      for (int j = colourStart[c] + (t >> 2); j < colourStart[c + 1]; j += T / 4)
+        normalQuad(solveSlot[s0 + j], t & 3, SP, hdr, nrow, io, dd1, nn1);
+      __syncthreads();
+    }
+    for (int c = 0; c < ncol; c++) {
+      for (int j = colourStart[c] + (t >> 2); j < colourStart[c + 1]; j += T / 4)
+        frictionQuad(solveSlot[s0 + j], t & 3, SP, hdr, nrow, frow, io, dd1, nn1);
+      __syncthreads();
+    }
+    if (it == iters - 1) {
+      dd = dd1;
+      nn = nn1;
+    }
+  }
+  for (int i = t; i < nbod; i += T) {
+    const float4 lv = reinterpret_cast<const float4*>(lds)[2 * i];
+    const float4 av = reinterpret_cast<const float4*>(lds)[2 * i + 1];
+    float4* d = reinterpret_cast<float4*>(vel + 8 * bodyList[q0 + i]);
+    d[0] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+    d[1] = make_float4(av.x, av.y, av.z, 0.0f);
+  }
+  accumulate(dd, nn, res);
+}
+
 // one wave: sum the stripes of iteration `it` into hist[2*it..], then r = ||dl|| / max(1, ||l||)
 __global__ __launch_bounds__(64) void kResidual(int it, float tol, const double* __restrict__ stripes, double* hist,
                                                 int* done, float* out) {
@@ -1059,6 +1244,176 @@ void solverGetImpulses(Ctx& c, float* out, int64_t nslots) {
   JB_CHECK(hipStreamSynchronize(s));
 }
 
+namespace {
+// Block tables of the world-block solve: dynamic bodies grouped by world (world id, then index), whole worlds
+// packed into blocks of at most wbBodies dynamic bodies in world order (a larger world forms a block of its
+// own, up to the LDS limit of 1024 bodies).
+void worldBlocksPrepare(Ctx& c) {
+  const int nb = c.nb, B = c.wbBodies;
+  std::vector<int> ids;
+  for (int i = 0; i < nb; i++)
+    if (!c.bodyStaticHost[i]) ids.push_back(i);
+  std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return c.worldHost[a] < c.worldHost[b]; });
+  std::vector<int> start(1, 0), block(nb, -1), local(nb, -1);
+  int cur = 0, maxBodies = 0;
+  for (size_t i = 0; i < ids.size();) {
+    size_t j = i;
+    while (j < ids.size() && c.worldHost[ids[j]] == c.worldHost[ids[i]]) j++;
+    const int w = (int)(j - i);
+    if (w > 1024) throw std::length_error("a world has more than 1024 dynamic bodies: too many for one block");
+    if (cur > 0 && cur + w > B) {
+      start.push_back((int)i);
+      cur = 0;
+    }
+    cur += w;
+    i = j;
+  }
+  start.push_back((int)ids.size());
+  const int nBlocks = (int)start.size() - 1;
+  for (int b = 0; b < nBlocks; b++) {
+    maxBodies = std::max(maxBodies, start[b + 1] - start[b]);
+    for (int q = start[b]; q < start[b + 1]; q++) {
+      block[ids[q]] = b;
+      local[ids[q]] = q - start[b];
+    }
+  }
+  hipStream_t s = c.stream;
+  c.wbBodyList.ensure(std::max<size_t>(ids.size(), 1));
+  c.wbBodyStart.ensure(start.size());
+  c.wbBodyBlock.ensure(nb);
+  c.wbBodyLocal.ensure(nb);
+  if (!ids.empty()) JB_CHECK(hipMemcpyAsync(c.wbBodyList.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.wbBodyStart.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.wbBodyBlock.p, block.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipMemcpyAsync(c.wbBodyLocal.p, local.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  c.wbBlocks = nBlocks;
+  c.wbMaxBodies = maxBodies;
+  c.wbReady = true;
+}
+
+int bitsForBlocks(int n) {
+  int b = 1;
+  while ((1 << b) <= n) b++;
+  return b;
+}
+
+// the whole solve of batched worlds: manifolds grouped by block, rows built in that order, one launch per solve
+void solveWorldBlocks(Ctx& c, float dt, int na) {
+  hipStream_t s = c.stream;
+  float* man = c.man[c.cur].p;
+  if (!c.wbReady) worldBlocksPrepare(c);
+  const int nBlocks = c.wbBlocks;
+  tBegin(c, T_COLOR);
+  c.activeBodies.ensure(na);
+  c.wbKey.ensure(na);
+  c.wbKeyAlt.ensure(na);
+  c.wbIdx.ensure(na);
+  c.colorOrderAlt.ensure(na);
+  c.wbSlotStart.ensure((size_t)nBlocks + 1);
+  c.wbInfo.ensure(4);
+  JB_CHECK(hipMemsetAsync(c.wbInfo.p, 0, 4 * sizeof(int), s));
+  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
+  hipLaunchKernelGGL(kManifoldBlocks, dim3(gridFor(na)), dim3(256), 0, s, na, (const int2*)c.activeBodies.p,
+                     (const int*)c.wbBodyBlock.p, c.wbKey.p, c.wbIdx.p, c.wbInfo.p);
+  // slot order: active manifolds by block (stable: pair-key order inside a block)
+  hipcub::DoubleBuffer<int> dk(c.wbKey.p, c.wbKeyAlt.p), dv(c.wbIdx.p, c.colorOrderAlt.p);
+  size_t tb = 0;
+  const int keyBits = bitsForBlocks(nBlocks);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dk, dv, na, 0, keyBits, s));
+  c.cubTmp.ensure(tb);
+  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, dk, dv, na, 0, keyBits, s));
+  const int* slotManifold = dv.Current();  // slot p -> active manifold
+  hipLaunchKernelGGL(kBlockSlotStarts, dim3(gridFor((int64_t)na + 1)), dim3(256), 0, s, na, nBlocks,
+                     (const int*)dk.Current(), c.wbSlotStart.p);
+  tEnd(c, T_COLOR);
+  // setup: rows of every slot (the warm start is applied inside the block solve)
+  tBegin(c, T_SOLVE_SETUP);
+  c.hdr.ensure(na);
+  c.nrow.ensure((size_t)NW * na);
+  c.frow.ensure((size_t)FW * na);
+  c.svel.ensure((size_t)8 * c.nb);
+  hipLaunchKernelGGL(kSolverBodies, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, (const float*)c.vel.p,
+                     (const float*)c.props.p, c.svel.p);
+  const bool learnedHere = c.useMlp;
+  SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.useMlp) ? 1 : 0,
+                  (c.dumpFeatures || learnedHere) ? 1 : 0, 0.0f};
+  if (SP.dumpFeatures) c.features.ensure((size_t)16 * 4 * na);
+  if ((int64_t)na * 4 >= (1ll << 27)) throw std::length_error("too many active manifolds for the slot header");
+  c.slotCount.ensure(na);
+  c.slotStart.ensure(na);
+  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, slotManifold, c.mActive.p, man,
+                     c.slotCount.p);
+  size_t tbScan = 0;
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbScan, c.slotCount.p, c.slotStart.p, na, s));
+  c.cubTmp.ensure(tbScan);
+  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tbScan, c.slotCount.p, c.slotStart.p, na, s));
+  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, slotManifold, c.mActive.p, SP, man,
+                     c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
+                     SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
+  c.rowCount.ensure(2);
+  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na, na, (const int*)c.slotCount.p, (const int*)c.slotStart.p,
+                     c.rowCount.p);
+  if (learnedHere) mlpWarmStart(c, na);
+  tEnd(c, T_SOLVE_SETUP);
+  // the block solve: colouring, warm start, every iteration, velocities
+  const int iters = c.numIterations;
+  const size_t histWords = 2 * (size_t)iters + 4, stripeWords = 2 * (size_t)kResidStripes * iters;
+  c.resid.ensure(histWords + stripeWords);
+  c.done.ensure(4);
+  JB_CHECK(hipMemsetAsync(c.resid.p, 0, (histWords + stripeWords) * sizeof(double), s));
+  JB_CHECK(hipMemsetAsync(c.done.p, 0, 4 * sizeof(int), s));
+  float* resOut = (float*)(c.counter.p + 6);
+  JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
+  double* res = iters > 0 ? c.resid.p + histWords + 2 * (size_t)kResidStripes * (iters - 1) : nullptr;
+  c.wbColour.ensure(na);
+  c.wbSolveSlot.ensure(na);
+  c.colorOrder.ensure(na);
+  const bool apply = c.warmstart || c.useLearned || c.useMlp;
+  const size_t ldsBytes = (size_t)std::max(c.wbMaxBodies, 1) * (8 * sizeof(float) + 2 * sizeof(unsigned long long));
+  tBegin(c, T_SOLVE_ITER);
+#define JB_WB_LAUNCH(T)                                                                                           \
+  hipLaunchKernelGGL(kSolveWorldBlocks<T>, dim3(nBlocks), dim3(T), ldsBytes, s, iters, SP, apply ? 1 : 0,         \
+                     (const int*)c.wbBodyList.p, (const int*)c.wbBodyStart.p, (const int*)c.wbBodyLocal.p,        \
+                     (const int*)c.wbSlotStart.p, slotManifold, c.wbColour.p, c.wbSolveSlot.p, c.colorOrder.p,     \
+                     (const int4*)c.hdr.p, c.nrow.p, c.frow.p, (const float*)c.props.p, (const float*)c.svel.p,    \
+                     c.vel.p, res, c.wbInfo.p)
+  switch (c.wbThreads) {
+    case 64: JB_WB_LAUNCH(64); break;
+    case 128: JB_WB_LAUNCH(128); break;
+    case 512: JB_WB_LAUNCH(512); break;
+    default: JB_WB_LAUNCH(256); break;
+  }
+#undef JB_WB_LAUNCH
+  JB_CHECK(hipGetLastError());
+  if (iters > 0)
+    hipLaunchKernelGGL(kResidual, dim3(1), dim3(64), 0, s, iters - 1, 0.0f, res, c.resid.p, c.done.p, resOut);
+  tEnd(c, T_SOLVE_ITER, iters > 0 ? 2 : 1);
+  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
+  // the order is reported as one batch (block after block, colour after colour inside each block)
+  c.colorOff.assign({0, (int64_t)na});
+  c.ncolors = 1;
+  c.headColors = 1;
+  // statistics: read after the step's final wait
+  c.solveRo.ensure(2);
+  c.solveRh.ensure(2 * (size_t)std::max(iters, 1));
+  c.solveRows.ensure(2);
+  c.wbInfoHost.ensure(4);
+  JB_CHECK(hipMemcpyAsync(c.solveRo.p, resOut, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (iters > 0)
+    JB_CHECK(hipMemcpyAsync(c.solveRh.p, c.resid.p, 2 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(c.solveRows.p, c.rowCount.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(c.wbInfoHost.p, c.wbInfo.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+  c.solveStat.pending = true;
+  c.solveStat.tolMode = false;
+  c.solveStat.blockSweeps = false;
+  c.solveStat.sampleK = false;
+  c.solveStat.iters = iters;
+  c.solveStat.kUsed = 0;
+  c.wbLastSolve = true;
+}
+}  // namespace
+
 void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   hipStream_t s = c.stream;
   const int64_t np = c.npairs;
@@ -1121,7 +1476,15 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   JB_CHECK(hipStreamSynchronize(s));
   }
   c.nActive = na;
+  c.wbLastSolve = false;
   if (na == 0) return;
+  if (!explicitList && c.wbBodies > 0 && !(c.tolerance > 0.0f)) {  // batched worlds: one launch per solve
+    if (c.useMlp && !c.mlpReady) throw std::logic_error("learned warm start enabled without a model");
+    if (!c.useMlp && c.useLearned && c.learnedSlots < 4 * (int64_t)na)
+      throw std::length_error("learned warm start has fewer slots than 4 x active manifolds");
+    solveWorldBlocks(c, dt, na);
+    return;
+  }
   if (c.useMlp && !c.mlpReady) throw std::logic_error("learned warm start enabled without a model");
   if (!c.useMlp && c.useLearned && c.learnedSlots < 4 * (int64_t)na)
     throw std::length_error("learned warm start has fewer slots than 4 x active manifolds");
@@ -1498,6 +1861,12 @@ void solverCollectStats(Ctx& c) {
   }
   c.st.residual = ro[0];
   c.st.iterations = (int64_t)ro[1];
+  if (c.wbLastSolve) {
+    const int* wi = c.wbInfoHost.p;
+    c.st.colors = wi[1];
+    if (wi[0] & 2) throw std::length_error("a block needed more than 128 colours");
+    if (wi[0] & 4) throw std::logic_error("a manifold joins two world blocks (bodies of different worlds)");
+  }
   for (int k = tolMode ? 0 : std::max(iters - 1, 0); k < (int)c.st.iterations; k++) {
     double nrm = std::sqrt(rh[2 * k + 1]);
     c.residHist.push_back((float)(std::sqrt(rh[2 * k]) / (nrm > 1.0 ? nrm : 1.0)));

@@ -259,7 +259,13 @@ void bodiesToDevice(Ctx& c, const float* bodies) {
     if (si < 0 || si >= c.ns) throw std::invalid_argument("body shape index out of range");
   }
   c.bodyShapeType.resize(n);
-  for (int i = 0; i < n; i++) c.bodyShapeType[i] = c.shapeTypeHost[ldi(bodies + (size_t)JBGPU_BODY_WORDS * i + 27)];
+  c.bodyStaticHost.resize(n);
+  for (int i = 0; i < n; i++) {
+    const float* r = bodies + (size_t)JBGPU_BODY_WORDS * i;
+    c.bodyShapeType[i] = c.shapeTypeHost[ldi(r + 27)];
+    c.bodyStaticHost[i] = (ldi(r + 28) & JBGPU_BODY_STATIC) != 0;
+  }
+  c.wbReady = false;
   hipStream_t s = c.stream;
   upload(c.tr, tr.data(), tr.size(), s);
   upload(c.vel, vel.data(), vel.size(), s);
@@ -817,6 +823,9 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     // hull AABBs read the hull points through the shape record only (host copy above uses records)
     c.stageCurrent = 0;
     uploadProxies(c, nbodies, box.data(), group.data(), mask.data(), world_ids, true);
+    c.worldHost.assign(nbodies, 0);
+    if (world_ids) c.worldHost.assign(world_ids, world_ids + nbodies);
+    c.wbReady = false;
     c.haveWorld = true;
   });
 }
@@ -1040,6 +1049,19 @@ int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size) {
   });
 }
 
+int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads) {
+  return guard(ctx, [&] {
+    if (bodies_per_block != 0 && (bodies_per_block < 16 || bodies_per_block > 1024))
+      throw std::invalid_argument("bodies per block must be 0 or in [16, 1024]");
+    if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+      throw std::invalid_argument("threads per block must be 64, 128, 256 or 512");
+    Ctx& c = ctx->c;
+    c.wbBodies = bodies_per_block;
+    c.wbThreads = threads;
+    c.wbReady = false;
+  });
+}
+
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor) {
   return guard(ctx, [&] {
     if (!std::isfinite(factor)) throw std::invalid_argument("warm-start factor must be finite");
@@ -1094,8 +1116,10 @@ int jbgpu_mlp_train(jbgpu_ctx* ctx, const float* x, const float* t, int64_t n, i
 
 int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on) {
   return guard(ctx, [&] {
+    if (on < 0 || on > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
     if (on && !ctx->c.mlpReady) throw std::logic_error("no model loaded or trained");
     ctx->c.useMlp = on != 0;
+    ctx->c.mlpNewOnly = on == 2;
   });
 }
 

@@ -88,6 +88,7 @@ SYMBOLS = {
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_tail": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_solver_set_blocks": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_solver_set_world_blocks": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_warmstart_factor": (C.c_int, [_vp, C.c_float]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
@@ -364,6 +365,11 @@ class Context:
 
     def solver_all_solved(self, dt=1.0 / 60.0):
         self._ck(load().jbgpu_solver_all_solved(self.h, float(dt)), "solver_all_solved")
+
+    def solver_set_world_blocks(self, bodies_per_block, threads=256):
+        """One workgroup per block of whole worlds (at most bodies_per_block dynamic bodies; 0 = off)."""
+        self._ck(load().jbgpu_solver_set_world_blocks(self.h, int(bodies_per_block), int(threads)),
+                 "solver_set_world_blocks")
 
     def solver_order(self):
         n = C.c_int64(0)

@@ -30,9 +30,10 @@ def _world(sc, iterations):
     return ctx
 
 
-def collect(seeds, scene_kw, settle=90, samples=6, every=5, tol=1e-12, cap=100, log=None):
+def collect(seeds, scene_kw, settle=90, samples=6, every=5, tol=1e-12, cap=100, new_only=False, log=None):
     """Features and final impulses of solves run to ``tol`` with at most ``cap`` iterations (by default the
-    evaluation's regime: every step runs the full ``cap`` iterations)."""
+    evaluation's regime: every step runs the full ``cap`` iterations).  new_only: only the contacts that enter the
+    solve without cached impulses (the ones the model warm-starts in mode 2)."""
     xs, ys = [], []
     for seed in seeds:
         ctx = _world(scenes.pyramids(seed=seed, **scene_kw), 10)
@@ -45,7 +46,10 @@ def collect(seeds, scene_kw, settle=90, samples=6, every=5, tol=1e-12, cap=100, 
             if len(order) == 0:
                 continue
             v = valid_slots(ctx, order)
-            xs.append(ctx.solver_features(4 * len(order))[v])
+            f = ctx.solver_features(4 * len(order))
+            if new_only:
+                v &= (f[:, 12] == 0) & (f[:, 13] == 0) & (f[:, 14] == 0)
+            xs.append(f[v])
             ys.append(ctx.solver_impulses(4 * len(order))[v])
         ctx.close()
         if log:
@@ -61,19 +65,21 @@ def _iterations_to(hist, tol, cap):
     return int(below[0]) + 1 if len(below) else cap
 
 
-def evaluate(params, norm, seed, scene_kw, settle=90, steps=30, cap=100, modes=("plain", "learned", "factor_1")):
+def evaluate(params, norm, seed, scene_kw, settle=90, steps=30, cap=100,
+             modes=("plain", "learned", "learned_new", "factor_1")):
     """Every solve of `steps` steps runs `cap` iterations with the residual recorded after each one, from the same
-    settled state for every mode: plain (0.85 x cached impulses, as the reference), learned (the model), factor_1
-    (1.0 x cached impulses).  Reported per mode: mean iterations until the residual first drops to each threshold
-    (cap when it never does), the residual after 20 iterations, and the solve time per iteration."""
+    settled state for every mode: plain (0.85 x cached impulses, as the reference), learned (the model for every
+    contact), learned_new (the model for the contacts without cached impulses, 0.85 x cached for the others),
+    factor_1 (1.0 x cached impulses).  Reported per mode: mean iterations until the residual first drops to each
+    threshold (cap when it never does), the residual after 20 iterations, and the solve time per iteration."""
     out = {}
     for mode in modes:
         ctx = _world(scenes.pyramids(seed=seed, **scene_kw), 10)
         ctx.solver_set_params(tolerance=1e-12, max_iterations=cap, num_iterations=0)
         ctx.step(n=settle)
-        if mode == "learned":
+        if mode in ("learned", "learned_new"):
             ctx.mlp_set_params(params, norm)
-            ctx.solver_use_mlp(True)
+            ctx.solver_use_mlp(1 if mode == "learned" else 2)
         if mode == "factor_1":
             ctx.solver_set_warmstart_factor(1.0)
         ctx.timing(True)
@@ -94,8 +100,11 @@ def evaluate(params, norm, seed, scene_kw, settle=90, steps=30, cap=100, modes=(
                          setup_ms=round(setup_ms, 4), mlp_ms=round(reg["mlp"][0] / max(nsteps, 1), 4),
                          contacts=int(st["contacts"]), manifolds=int(st["manifolds"]))
         ctx.close()
-    pl, le = out["plain"]["iterations_to"], out["learned"]["iterations_to"]
-    out["iteration_ratio"] = {t: le[t] / max(pl[t], 1e-9) for t in pl}
+    pl = out["plain"]["iterations_to"]
+    for mode in modes:
+        if mode.startswith("learned"):
+            le = out[mode]["iterations_to"]
+            out[f"iteration_ratio_{mode}"] = {t: le[t] / max(pl[t], 1e-9) for t in pl}
     return out
 
 

@@ -108,6 +108,11 @@ def test_failed_destroy_changes_nothing():
     orc.set_all(boxes)
     ctx.bp_calculate_overlapping_pairs()
     assert np.array_equal(ctx.bp_pairs(), orc.step())
+    # the first pair query came with no setAabb: the stored boxes stayed and the stages rotated as in the oracle
+    pg, sg = ctx.bp_proxies()
+    po_ = orc.proxies()
+    assert sg == pyoracle.lib().orc_bp_stage_current(orc.h)
+    assert np.array_equal(pg[:, :13].view(np.uint32), po_[:, :13].view(np.uint32))
     ctx.close()
 
 

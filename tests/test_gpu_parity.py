@@ -365,13 +365,33 @@ def test_ray_test_100k_pile_matches_oracle():
     ctx.close()
 
 
-def test_batched_4096_worlds_step_parity():
+@pytest.mark.parametrize("bodies,threads", [(16, 64), (256, 256), (1024, 512)])
+def test_batched_world_blocks_step_parity(bodies, threads):
+    """The batched solve with one workgroup per block of whole worlds (colouring, warm start and all iterations
+    in one launch): bit-exact against the oracle fed the block-by-block manifold order, for blocks of one small
+    world (16 bodies: every world is a block), of several worlds and of many."""
+    sc = scenes.batched_towers(seed=3, n_worlds=24, lo=16, hi=120)
+    ctx, orc = _world_pair(sc, iterations=10)
+    ctx.solver_set_world_blocks(bodies, threads)
+    for step in range(60):
+        _sync_and_step(ctx, orc)
+        st = ctx.stats()
+        if step % 20 == 0:
+            print(f"[world blocks {bodies}/{threads}] step {step}: manifolds {st['manifolds']} rows {st['rows']} "
+                  f"colours {st['colors']} residual {st['residual']:.3e}", flush=True)
+        _compare_world(ctx, orc, f"world blocks {bodies} step {step}")
+    ctx.close()
+
+
+@pytest.mark.parametrize("world_blocks", [0, 512])
+def test_batched_4096_worlds_step_parity(world_blocks):
     """The batched-worlds configuration at its full size: 4,096 worlds of log-uniform 16-512 bodies (~588k
     bodies, sizes as in the mixed pile), towers on their own floors, 10 iterations.  50 device-only steps, then
     two steps checked bit for bit against the oracle stepping all worlds from the same state."""
     sc = scenes.batched_towers(seed=1, n_worlds=4096)
-    print(f"[batched 4096] {sc.n_bodies} bodies", flush=True)
+    print(f"[batched 4096] {sc.n_bodies} bodies, world blocks {world_blocks}", flush=True)
     ctx, orc = _world_pair(sc, iterations=10)
+    ctx.solver_set_world_blocks(world_blocks, 256)
     ctx.step(n=50)
     st = ctx.stats()
     print(f"[batched 4096] after 50 steps: pairs {st['pairs']} manifolds {st['manifolds']} colors {st['colors']}",
