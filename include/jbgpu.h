@@ -212,8 +212,16 @@ int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size);
  * start and runs every iteration with the block's velocities in LDS, so a solve is one launch.  The manifold
  * order (jbgpu_solver_get_order) is then block after block, colour after colour inside a block, reported as one
  * batch.  0 (the default) solves with one launch per colour batch.  Tolerance-driven solves (tolerance > 0)
- * always use the per-colour launches. */
-int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads);
+ * always use the per-colour launches.  pipelined != 0: while a colour is swept, the rows of the next colour are
+ * already being loaded (more registers per thread, fewer waves per SIMD). */
+int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads, int32_t pipelined);
+/* Colouring of the manifolds into batches (no reference counterpart): Jones-Plassmann rounds with hashed
+ * priorities (the default) or priorities largest degree first (degree_first != 0; a manifold's degree is the
+ * number of manifolds on its dynamic bodies), then recolour_passes iterated-greedy passes (classes from the last
+ * colour to the first, each manifold moved to the smallest colour free at its bodies; 0 by default).  Fewer
+ * colours would mean fewer sweep launches.  The order changes (jbgpu_solver_get_order reports it); every order is
+ * a valid sequential order of the reference solver. */
+int jbgpu_solver_set_colouring(jbgpu_ctx* ctx, int32_t degree_first, int32_t recolour_passes);
 /* ContactSolverInfo.warmstartingFactor for the following solves (0.85 by default). */
 int jbgpu_solver_set_warmstart_factor(jbgpu_ctx* ctx, float factor);
 /* Launch shaping of the colour sweeps (no reference counterpart; results are identical for every

@@ -212,6 +212,7 @@ struct Ctx {
   // batched worlds solved one workgroup per block of whole worlds (the whole solve in one launch)
   int wbBodies = 0;      // at most this many dynamic bodies per block (0: off)
   int wbThreads = 256;   // workgroup size of the block solve
+  bool wbPipelined = false;  // rows of the next colour loaded while a colour is swept (more registers)
   bool wbReady = false;  // the tables below match wbBodies and the bodies
   int wbBlocks = 0, wbMaxBodies = 0;
   DBuf<int> wbBodyList;   // dynamic bodies, block after block
@@ -219,6 +220,7 @@ struct Ctx {
   DBuf<int> wbBodyBlock;  // per body: its block, -1 for static bodies
   DBuf<int> wbBodyLocal;  // per body: position in its block, -1 for static bodies
   DBuf<int> wbKey, wbKeyAlt, wbIdx, wbSlotStart, wbColour, wbSolveSlot;
+  DBuf<int4> wbSlotInfo;  // per solve position: mask | first row << 4, bodies, packed LDS positions
   DBuf<int> wbInfo;       // [0] error flags, [1] most colours in a block
   HBuf<int> wbInfoHost;
   bool wbLastSolve = false;
@@ -263,6 +265,9 @@ struct Ctx {
   DBuf<unsigned long long> bodyPrio;
   DBuf<unsigned long long> bodyMask;
   DBuf<int> color, colorAlt;
+  bool colourByDegree = false;  // Jones-Plassmann priorities largest degree first (false: hashed only)
+  int recolourPasses = 0;      // iterated-greedy passes over the colour classes after the colouring
+  DBuf<int> bodyDeg, mDeg;     // manifolds per dynamic body, per manifold the sum over its bodies
   DBuf<int> pend, pendAlt, pendFlag;  // uncoloured manifolds between colouring batches
   DBuf<uint64_t> colorKey, colorKeyAlt;
   DBuf<int> colorOrder, colorOrderAlt;

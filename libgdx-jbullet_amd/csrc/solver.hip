@@ -140,8 +140,28 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 __device__ __forceinline__ bool dynamicBody(const float* props, int i) {
   return (ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) == 0;
 }
-__device__ __forceinline__ unsigned long long prio(int round, int k) {
-  return ((unsigned long long)round << 52) | ((unsigned long long)(hash32(k) >> 12) << 32) | (unsigned long long)k;
+// round (bits 52-63), then the manifold's degree when colouring largest-degree-first (bits 40-51: manifolds
+// sharing one of its dynamic bodies, 0 otherwise), then a hash of its index, then the index (ties)
+__device__ __forceinline__ unsigned long long prio(int round, int k, const int* __restrict__ deg) {
+  const unsigned long long d = deg ? (unsigned long long)min(deg[k], 4095) : 0ull;
+  return ((unsigned long long)round << 52) | (d << 40) | ((unsigned long long)(hash32(k) >> 24) << 32) |
+         (unsigned long long)k;
+}
+
+// largest-degree-first priorities: manifolds per dynamic body, then per manifold the sum over its bodies
+__global__ void kBodyDegree(int n, const int2* __restrict__ ab, const float* __restrict__ props, int* bodyDeg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 b = ab[k];
+  if (dynamicBody(props, b.x)) atomicAdd(bodyDeg + b.x, 1);
+  if (dynamicBody(props, b.y)) atomicAdd(bodyDeg + b.y, 1);
+}
+__global__ void kManifoldDegree(int n, const int2* __restrict__ ab, const float* __restrict__ props,
+                                const int* __restrict__ bodyDeg, int* deg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int2 b = ab[k];
+  deg[k] = (dynamicBody(props, b.x) ? bodyDeg[b.x] : 0) + (dynamicBody(props, b.y) ? bodyDeg[b.y] : 0);
 }
 
 // body pair of active manifold k
@@ -160,33 +180,33 @@ __global__ void kActiveBodies(int n, const int* __restrict__ list, const float* 
 // round r (reading buffer r & 1, complete since the previous launch) and propose round r + 1 (into the other
 // buffer) for the manifolds still uncoloured: one launch per round instead of two, the same colouring.
 __device__ __forceinline__ void colourPropose(int k, int round, int2 b, const float* __restrict__ props, int nb,
-                                              unsigned long long* bodyPrio) {
+                                              unsigned long long* bodyPrio, const int* __restrict__ deg) {
   unsigned long long* bp = bodyPrio + (size_t)(round & 1) * nb;
-  const unsigned long long p = prio(round, k);
+  const unsigned long long p = prio(round, k, deg);
   if (dynamicBody(props, b.x)) atomicMax(bp + b.x, p);
   if (dynamicBody(props, b.y)) atomicMax(bp + b.y, p);
 }
 
 __global__ void kColorPropose(int m, const int* __restrict__ pend, int round, const int2* __restrict__ ab,
                               const float* __restrict__ props, const int* __restrict__ color, int nb,
-                              unsigned long long* bodyPrio) {
+                              unsigned long long* bodyPrio, const int* __restrict__ deg) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
   const int k = pend[t];
   if (color[k] >= 0) return;
-  colourPropose(k, round, ab[k], props, nb, bodyPrio);
+  colourPropose(k, round, ab[k], props, nb, bodyPrio, deg);
 }
 
 // settle round `round`; with proposeNext the losers propose round + 1
 __global__ void kColorRound(int m, const int* __restrict__ pend, int round, bool proposeNext,
                             const int2* __restrict__ ab, const float* __restrict__ props, int* color, int nb,
-                            unsigned long long* bodyPrio, unsigned long long* bodyMask) {
+                            unsigned long long* bodyPrio, unsigned long long* bodyMask, const int* __restrict__ deg) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
   const int k = pend[t];
   if (color[k] >= 0) return;
   const int2 b = ab[k];
-  const unsigned long long p = prio(round, k);
+  const unsigned long long p = prio(round, k, deg);
   const unsigned long long* bp = bodyPrio + (size_t)(round & 1) * nb;
   const bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
   const bool win = (!da || bp[b.x] == p) && (!db || bp[b.y] == p);
@@ -202,8 +222,27 @@ __global__ void kColorRound(int m, const int* __restrict__ pend, int round, bool
     }
     color[k] = c;
   } else if (proposeNext) {
-    colourPropose(k, round + 1, b, props, nb, bodyPrio);
+    colourPropose(k, round + 1, b, props, nb, bodyPrio, deg);
   }
+}
+
+// one colour class of the iterated-greedy pass: slots [off, off + cnt) of the colour-sorted order
+__global__ void kRecolour(int off, int cnt, const int* __restrict__ order, const int2* __restrict__ ab,
+                          const float* __restrict__ props, int* color, unsigned long long* bodyMask) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int k = order[off + t];
+  const int c = color[k];
+  if (c >= 64) return;  // colours past the masks stay where they are
+  const int2 b = ab[k];
+  const bool da = dynamicBody(props, b.x), db = dynamicBody(props, b.y);
+  const unsigned long long bit = 1ull << c;
+  const unsigned long long used = ((da ? bodyMask[b.x] : 0ull) | (db ? bodyMask[b.y] : 0ull)) & ~bit;
+  const int nc = __ffsll((long long)~used) - 1;
+  if (nc >= c) return;
+  color[k] = nc;
+  if (da) bodyMask[b.x] = (bodyMask[b.x] & ~bit) | (1ull << nc);
+  if (db) bodyMask[b.y] = (bodyMask[b.y] & ~bit) | (1ull << nc);
 }
 
 __global__ void kUncoloured(int m, const int* __restrict__ pend, const int* __restrict__ color, int* flag) {
@@ -1051,7 +1090,74 @@ __global__ void kBlockSlotStarts(int na, int nBlocks, const int* __restrict__ ke
   if (p == na && lo < nBlocks) slotStart[nBlocks] = na;
 }
 
-template <int T>
+// Per solve position of a block: (valid-point mask | first row << 4, body A, body B, LDS positions of A and B
+// packed as 16-bit fields, 0xffff for a static body).  One 16-byte load names everything a sweep step needs.
+__device__ __forceinline__ int infoLocal(const int4& i, int which) {
+  const int v = which ? ((unsigned)i.w >> 16) : (i.w & 0xffff);
+  return v == 0xffff ? -1 : v;
+}
+struct WorldInfoBodies {  // bodies of a solve position: LDS positions from the packed info, static ones from svel
+  float* lds;
+  const float* __restrict__ svel;
+  const float* __restrict__ props;
+  __device__ void load(const int4& i, SBody& A, SBody& B) const {
+    const int la = infoLocal(i, 0), lb = infoLocal(i, 1);
+    A = la >= 0 ? LdsBodies::get(lds + 8 * la) : loadBody(svel, props, i.y);
+    B = lb >= 0 ? LdsBodies::get(lds + 8 * lb) : loadBody(svel, props, i.z);
+  }
+  __device__ void store(const int4& i, const SBody& A, const SBody& B) const {
+    const int la = infoLocal(i, 0), lb = infoLocal(i, 1);
+    if (la >= 0) LdsBodies::put(lds + 8 * la, A);
+    if (lb >= 0) LdsBodies::put(lds + 8 * lb, B);
+  }
+};
+// the rows a quad lane holds for one solve position (loaded ahead of their colour's step)
+struct NormalStage {
+  float N[RW];
+};
+struct FrictionStage {
+  float F[2 * RW];
+  float total;
+};
+__device__ __forceinline__ void stageNormal(const int4& i, int q, const float* __restrict__ nrow, NormalStage& st) {
+  if (q < __popc(i.x & 15)) loadRow(st.N, nrow + (size_t)RW * (((unsigned)i.x >> 4) + q));
+}
+__device__ __forceinline__ void stageFriction(const int4& i, int q, const float* __restrict__ nrow,
+                                              const float* __restrict__ frow, FrictionStage& st) {
+  if (q < __popc(i.x & 15)) {
+    const int r = (int)((unsigned)i.x >> 4) + q;
+    loadRow(st.F, frow + (size_t)RW * (2 * r));
+    loadRow(st.F + RW, frow + (size_t)RW * (2 * r + 1));
+    st.total = nrow[(size_t)RW * r + 19] + 0.0f;  // appliedImpulse + appliedPushImpulse (0)
+  }
+}
+// normalQuad / frictionQuad on staged rows
+__device__ __forceinline__ void normalQuadStaged(const int4& i, int q, const SolverParams& SP, const NormalStage& st,
+                                                 float* nrow, const WorldInfoBodies& io, double& dd, double& nn) {
+  const bool mine = q < __popc(i.x & 15);
+  float* out = nrow + (size_t)RW * (((unsigned)i.x >> 4) + q) + 19;
+  SBody A, B;
+  io.load(i, A, B);
+  normalRound<0>(q, mine, st.N, SP, A, B, out, dd, nn);
+  normalRound<1>(q, mine, st.N, SP, A, B, out, dd, nn);
+  normalRound<2>(q, mine, st.N, SP, A, B, out, dd, nn);
+  normalRound<3>(q, mine, st.N, SP, A, B, out, dd, nn);
+  if (q == 0) io.store(i, A, B);
+}
+__device__ __forceinline__ void frictionQuadStaged(const int4& i, int q, const FrictionStage& st, float* frow,
+                                                   const WorldInfoBodies& io, double& dd, double& nn) {
+  const bool mine = q < __popc(i.x & 15);
+  float* rows = frow + (size_t)RW * (2 * (((unsigned)i.x >> 4) + q));
+  SBody A, B;
+  io.load(i, A, B);
+  frictionRound<0>(q, mine, st.F, st.total, A, B, rows, dd, nn);
+  frictionRound<1>(q, mine, st.F, st.total, A, B, rows, dd, nn);
+  frictionRound<2>(q, mine, st.F, st.total, A, B, rows, dd, nn);
+  frictionRound<3>(q, mine, st.F, st.total, A, B, rows, dd, nn);
+  if (q == 0) io.store(i, A, B);
+}
+
+template <int T, bool Pipelined>
 __global__ __launch_bounds__(T) void kSolveWorldBlocks(int iters, SolverParams SP, int warm,
                                                        const int* __restrict__ bodyList,
                                                        const int* __restrict__ bodyStart,
@@ -1059,9 +1165,9 @@ __global__ __launch_bounds__(T) void kSolveWorldBlocks(int iters, SolverParams S
                                                        const int* __restrict__ slotStart,
                                                        const int* __restrict__ slotManifold, int* slotColour,
                                                        int* solveSlot, int* colorOrder, const int4* __restrict__ hdr,
-                                                       float* nrow, float* frow, const float* __restrict__ props,
-                                                       const float* __restrict__ svel, float* vel, double* res,
-                                                       int* info) {
+                                                       int4* sinfo, float* nrow, float* frow,
+                                                       const float* __restrict__ props, const float* __restrict__ svel,
+                                                       float* vel, double* res, int* info) {
   extern __shared__ float lds[];  // [nbod][8] bodies, then [nbod][2] colour masks (64-bit words)
   __shared__ int colourStart[kWbMaxColours + 1];
   __shared__ int count[kWbMaxColours];
@@ -1128,26 +1234,101 @@ __global__ __launch_bounds__(T) void kSolveWorldBlocks(int iters, SolverParams S
   }
   __syncthreads();
   const int ncol = nColours;
-  for (int j = t; j < ns; j += T) colorOrder[s0 + j] = slotManifold[solveSlot[s0 + j]];
+  // solve positions: manifold order for the caller, packed slot info for the sweeps
+  for (int j = t; j < ns; j += T) {
+    const int p = solveSlot[s0 + j];
+    colorOrder[s0 + j] = slotManifold[p];
+    const int4 h = hdr[p];
+    const int la = bodyLocal[h.y], lb = bodyLocal[h.z];
+    sinfo[s0 + j] = make_int4(h.x, h.y, h.z, (la & 0xffff) | ((lb & 0xffff) << 16));
+  }
+  __syncthreads();
   const WorldLdsBodies io{lds, bodyLocal, svel, props};
+  const WorldInfoBodies iio{lds, svel, props};
   if (warm) {
     for (int c = 0; c < ncol; c++) {
       for (int j = colourStart[c] + t; j < colourStart[c + 1]; j += T) warmApplySlotIo(solveSlot[s0 + j], hdr, nrow, frow, io);
       __syncthreads();
     }
   }
+  // Sweeps, software-pipelined over the colours: while colour c is swept, the quad's slot info of colour c + 2
+  // and its rows of colour c + 1 are already on their way (rows do not depend on velocities; a colour's rows are
+  // written only by its own step).  Quads past the first round of a large colour load as they go.
+  const int q = t & 3, g = t >> 2;
+  auto infoAt = [&](int c) -> int4 {
+    if (c >= ncol) return make_int4(0, 0, 0, -1);
+    const int j = colourStart[c] + g;
+    return j < colourStart[c + 1] ? sinfo[s0 + j] : make_int4(0, 0, 0, -1);
+  };
   double dd = 0.0, nn = 0.0;
-  for (int it = 0; it < iters; it++) {
+  for (int it = 0; it < iters && !Pipelined; it++) {  // one step per colour, each loading its own rows
     double dd1 = 0.0, nn1 = 0.0;
     for (int c = 0; c < ncol; c++) {
-      for (int j = colourStart[c] + (t >> 2); j < colourStart[c + 1]; j += T / 4)
-        normalQuad(solveSlot[s0 + j], t & 3, SP, hdr, nrow, io, dd1, nn1);
+      for (int j = colourStart[c] + g; j < colourStart[c + 1]; j += T / 4) {
+        const int4 ij = sinfo[s0 + j];
+        NormalStage sj;
+        stageNormal(ij, q, nrow, sj);
+        normalQuadStaged(ij, q, SP, sj, nrow, iio, dd1, nn1);
+      }
       __syncthreads();
     }
     for (int c = 0; c < ncol; c++) {
-      for (int j = colourStart[c] + (t >> 2); j < colourStart[c + 1]; j += T / 4)
-        frictionQuad(solveSlot[s0 + j], t & 3, SP, hdr, nrow, frow, io, dd1, nn1);
+      for (int j = colourStart[c] + g; j < colourStart[c + 1]; j += T / 4) {
+        const int4 ij = sinfo[s0 + j];
+        FrictionStage sj;
+        stageFriction(ij, q, nrow, frow, sj);
+        frictionQuadStaged(ij, q, sj, frow, iio, dd1, nn1);
+      }
       __syncthreads();
+    }
+    if (it == iters - 1) {
+      dd = dd1;
+      nn = nn1;
+    }
+  }
+  for (int it = 0; it < iters && Pipelined; it++) {
+    double dd1 = 0.0, nn1 = 0.0;
+    {
+      int4 i0 = infoAt(0), i1 = infoAt(1);
+      NormalStage cur;
+      stageNormal(i0, q, nrow, cur);
+      for (int c = 0; c < ncol; c++) {
+        NormalStage nxt;
+        stageNormal(i1, q, nrow, nxt);
+        const int4 i2 = infoAt(c + 2);
+        if ((i0.x & 15) != 0) normalQuadStaged(i0, q, SP, cur, nrow, iio, dd1, nn1);
+        for (int j = colourStart[c] + g + T / 4; j < colourStart[c + 1]; j += T / 4) {
+          const int4 ij = sinfo[s0 + j];
+          NormalStage sj;
+          stageNormal(ij, q, nrow, sj);
+          normalQuadStaged(ij, q, SP, sj, nrow, iio, dd1, nn1);
+        }
+        __syncthreads();
+        cur = nxt;
+        i0 = i1;
+        i1 = i2;
+      }
+    }
+    {
+      int4 i0 = infoAt(0), i1 = infoAt(1);
+      FrictionStage cur;
+      stageFriction(i0, q, nrow, frow, cur);
+      for (int c = 0; c < ncol; c++) {
+        FrictionStage nxt;
+        stageFriction(i1, q, nrow, frow, nxt);
+        const int4 i2 = infoAt(c + 2);
+        if ((i0.x & 15) != 0) frictionQuadStaged(i0, q, cur, frow, iio, dd1, nn1);
+        for (int j = colourStart[c] + g + T / 4; j < colourStart[c + 1]; j += T / 4) {
+          const int4 ij = sinfo[s0 + j];
+          FrictionStage sj;
+          stageFriction(ij, q, nrow, frow, sj);
+          frictionQuadStaged(ij, q, sj, frow, iio, dd1, nn1);
+        }
+        __syncthreads();
+        cur = nxt;
+        i0 = i1;
+        i1 = i2;
+      }
     }
     if (it == iters - 1) {
       dd = dd1;
@@ -1368,21 +1549,31 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   double* res = iters > 0 ? c.resid.p + histWords + 2 * (size_t)kResidStripes * (iters - 1) : nullptr;
   c.wbColour.ensure(na);
   c.wbSolveSlot.ensure(na);
+  c.wbSlotInfo.ensure(na);
   c.colorOrder.ensure(na);
   const bool apply = c.warmstart || c.useLearned || c.useMlp;
   const size_t ldsBytes = (size_t)std::max(c.wbMaxBodies, 1) * (8 * sizeof(float) + 2 * sizeof(unsigned long long));
   tBegin(c, T_SOLVE_ITER);
-#define JB_WB_LAUNCH(T)                                                                                           \
-  hipLaunchKernelGGL(kSolveWorldBlocks<T>, dim3(nBlocks), dim3(T), ldsBytes, s, iters, SP, apply ? 1 : 0,         \
+#define JB_WB_LAUNCH(T, P)                                                                                        \
+  hipLaunchKernelGGL((kSolveWorldBlocks<T, P>), dim3(nBlocks), dim3(T), ldsBytes, s, iters, SP, apply ? 1 : 0,    \
                      (const int*)c.wbBodyList.p, (const int*)c.wbBodyStart.p, (const int*)c.wbBodyLocal.p,        \
                      (const int*)c.wbSlotStart.p, slotManifold, c.wbColour.p, c.wbSolveSlot.p, c.colorOrder.p,     \
-                     (const int4*)c.hdr.p, c.nrow.p, c.frow.p, (const float*)c.props.p, (const float*)c.svel.p,    \
-                     c.vel.p, res, c.wbInfo.p)
-  switch (c.wbThreads) {
-    case 64: JB_WB_LAUNCH(64); break;
-    case 128: JB_WB_LAUNCH(128); break;
-    case 512: JB_WB_LAUNCH(512); break;
-    default: JB_WB_LAUNCH(256); break;
+                     (const int4*)c.hdr.p, c.wbSlotInfo.p, c.nrow.p, c.frow.p, (const float*)c.props.p,            \
+                     (const float*)c.svel.p, c.vel.p, res, c.wbInfo.p)
+  if (c.wbPipelined) {
+    switch (c.wbThreads) {
+      case 64: JB_WB_LAUNCH(64, true); break;
+      case 128: JB_WB_LAUNCH(128, true); break;
+      case 512: JB_WB_LAUNCH(512, true); break;
+      default: JB_WB_LAUNCH(256, true); break;
+    }
+  } else {
+    switch (c.wbThreads) {
+      case 64: JB_WB_LAUNCH(64, false); break;
+      case 128: JB_WB_LAUNCH(128, false); break;
+      case 512: JB_WB_LAUNCH(512, false); break;
+      default: JB_WB_LAUNCH(256, false); break;
+    }
   }
 #undef JB_WB_LAUNCH
   JB_CHECK(hipGetLastError());
@@ -1498,6 +1689,17 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.bodyMask.ensure(c.nb);
   tBegin(c, T_COLOR);
   hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
+  const int* deg = nullptr;  // largest-degree-first priorities (null: hashed priorities only)
+  if (c.colourByDegree) {
+    c.bodyDeg.ensure(c.nb);
+    c.mDeg.ensure(na);
+    JB_CHECK(hipMemsetAsync(c.bodyDeg.p, 0, (size_t)c.nb * sizeof(int), s));
+    hipLaunchKernelGGL(kBodyDegree, dim3(gridFor(na)), dim3(256), 0, s, na, (const int2*)c.activeBodies.p,
+                       (const float*)c.props.p, c.bodyDeg.p);
+    hipLaunchKernelGGL(kManifoldDegree, dim3(gridFor(na)), dim3(256), 0, s, na, (const int2*)c.activeBodies.p,
+                       (const float*)c.props.p, (const int*)c.bodyDeg.p, c.mDeg.p);
+    deg = c.mDeg.p;
+  }
   JB_CHECK(hipMemsetAsync(c.color.p, 0xff, (size_t)na * sizeof(int), s));
   JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, 2 * (size_t)c.nb * sizeof(unsigned long long), s));
   JB_CHECK(hipMemsetAsync(c.bodyMask.p, 0, (size_t)c.nb * sizeof(unsigned long long), s));
@@ -1533,11 +1735,11 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   auto colourBatch = [&](int m, int rounds) {
     hipLaunchKernelGGL(kColorPropose, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
                        (const int2*)c.activeBodies.p, (const float*)c.props.p, (const int*)c.color.p, c.nb,
-                       c.bodyPrio.p);
+                       c.bodyPrio.p, deg);
     for (int r = 0; r < rounds; r++, round++)
       hipLaunchKernelGGL(kColorRound, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, round,
                          r + 1 < rounds, (const int2*)c.activeBodies.p, (const float*)c.props.p, c.color.p, c.nb,
-                         c.bodyPrio.p, c.bodyMask.p);
+                         c.bodyPrio.p, c.bodyMask.p, deg);
     hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, (const int*)c.color.p,
                        c.pendFlag.p);
     size_t tbs = 0;
@@ -1629,6 +1831,19 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
       order();
     }
     c.colourRounds = batches <= 1 ? firstRounds - 1 : round;
+    for (int pass = 0; pass < c.recolourPasses; pass++) {
+      // iterated greedy: the colour classes from the last to the first, every manifold of a class moved to the
+      // smallest colour free at its bodies (a class shares no dynamic body, so its moves do not race); the
+      // colour count never grows and the empty classes disappear when the order is made again
+      const int nb0 = hb[0];
+      std::vector<int> st(hb + 2, hb + 2 + nb0);
+      st.push_back(na);
+      for (int k = nb0 - 1; k >= 0; k--)
+        hipLaunchKernelGGL(kRecolour, dim3(gridFor(st[k + 1] - st[k])), dim3(256), 0, s, st[k], st[k + 1] - st[k],
+                           (const int*)c.colorOrder.p, (const int2*)c.activeBodies.p, (const float*)c.props.p,
+                           c.color.p, c.bodyMask.p);
+      order();
+    }
   } else {
     order();
   }

@@ -1049,7 +1049,7 @@ int jbgpu_solver_set_blocks(jbgpu_ctx* ctx, int32_t block_size) {
   });
 }
 
-int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads) {
+int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int32_t threads, int32_t pipelined) {
   return guard(ctx, [&] {
     if (bodies_per_block != 0 && (bodies_per_block < 16 || bodies_per_block > 1024))
       throw std::invalid_argument("bodies per block must be 0 or in [16, 1024]");
@@ -1058,7 +1058,16 @@ int jbgpu_solver_set_world_blocks(jbgpu_ctx* ctx, int32_t bodies_per_block, int3
     Ctx& c = ctx->c;
     c.wbBodies = bodies_per_block;
     c.wbThreads = threads;
+    c.wbPipelined = pipelined != 0;
     c.wbReady = false;
+  });
+}
+
+int jbgpu_solver_set_colouring(jbgpu_ctx* ctx, int32_t degree_first, int32_t recolour_passes) {
+  return guard(ctx, [&] {
+    if (recolour_passes < 0 || recolour_passes > 8) throw std::invalid_argument("recolour passes must be in [0, 8]");
+    ctx->c.colourByDegree = degree_first != 0;
+    ctx->c.recolourPasses = recolour_passes;
   });
 }
 

@@ -88,7 +88,8 @@ SYMBOLS = {
     "jbgpu_solver_set_params": (C.c_int, [_vp, C.c_float, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_tail": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_solver_set_blocks": (C.c_int, [_vp, C.c_int32]),
-    "jbgpu_solver_set_world_blocks": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "jbgpu_solver_set_world_blocks": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
+    "jbgpu_solver_set_colouring": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "jbgpu_solver_set_warmstart_factor": (C.c_int, [_vp, C.c_float]),
     "jbgpu_solver_residual_history": (C.c_int, [_vp, _f, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_mlp_set_params": (C.c_int, [_vp, _f, _f]),
@@ -366,10 +367,16 @@ class Context:
     def solver_all_solved(self, dt=1.0 / 60.0):
         self._ck(load().jbgpu_solver_all_solved(self.h, float(dt)), "solver_all_solved")
 
-    def solver_set_world_blocks(self, bodies_per_block, threads=256):
-        """One workgroup per block of whole worlds (at most bodies_per_block dynamic bodies; 0 = off)."""
-        self._ck(load().jbgpu_solver_set_world_blocks(self.h, int(bodies_per_block), int(threads)),
+    def solver_set_world_blocks(self, bodies_per_block, threads=256, pipelined=False):
+        """One workgroup per block of whole worlds (at most bodies_per_block dynamic bodies; 0 = off);
+        pipelined: rows of the next colour loaded while a colour is swept."""
+        self._ck(load().jbgpu_solver_set_world_blocks(self.h, int(bodies_per_block), int(threads), int(pipelined)),
                  "solver_set_world_blocks")
+
+    def solver_set_colouring(self, degree_first=False, recolour_passes=0):
+        """Colouring priorities (largest degree first or hashed) and iterated-greedy recolouring passes."""
+        self._ck(load().jbgpu_solver_set_colouring(self.h, int(bool(degree_first)), int(recolour_passes)),
+                 "solver_set_colouring")
 
     def solver_order(self):
         n = C.c_int64(0)

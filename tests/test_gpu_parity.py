@@ -166,9 +166,13 @@ def test_world_boxes_grid_step_parity(blocks, tail):
     ctx.close()
 
 
-def test_world_mixed_shapes_step_parity():
+@pytest.mark.parametrize("degree_first,recolour", [(1, 0), (0, 0), (1, 2)])
+def test_world_mixed_shapes_step_parity(degree_first, recolour):
+    """Mixed pile (all shape pairs, EPA), under each colouring: largest-degree-first or hashed Jones-Plassmann
+    priorities, with and without iterated-greedy recolouring passes (each gives its own valid order)."""
     sc = scenes.mixed_pile(seed=5, n=3000, bin_half=12.0)
     ctx, orc = _world_pair(sc, iterations=20)
+    ctx.solver_set_colouring(degree_first, recolour)
     for step in range(120):
         _sync_and_step(ctx, orc)
         st = ctx.stats()
@@ -365,19 +369,20 @@ def test_ray_test_100k_pile_matches_oracle():
     ctx.close()
 
 
-@pytest.mark.parametrize("bodies,threads", [(16, 64), (256, 256), (1024, 512)])
-def test_batched_world_blocks_step_parity(bodies, threads):
+@pytest.mark.parametrize("bodies,threads,pipelined", [(16, 64, 0), (256, 256, 0), (1024, 512, 0), (16, 64, 1),
+                                                      (256, 256, 1), (1024, 512, 1)])
+def test_batched_world_blocks_step_parity(bodies, threads, pipelined):
     """The batched solve with one workgroup per block of whole worlds (colouring, warm start and all iterations
     in one launch): bit-exact against the oracle fed the block-by-block manifold order, for blocks of one small
     world (16 bodies: every world is a block), of several worlds and of many."""
     sc = scenes.batched_towers(seed=3, n_worlds=24, lo=16, hi=120)
     ctx, orc = _world_pair(sc, iterations=10)
-    ctx.solver_set_world_blocks(bodies, threads)
+    ctx.solver_set_world_blocks(bodies, threads, pipelined)
     for step in range(60):
         _sync_and_step(ctx, orc)
         st = ctx.stats()
         if step % 20 == 0:
-            print(f"[world blocks {bodies}/{threads}] step {step}: manifolds {st['manifolds']} rows {st['rows']} "
+            print(f"[world blocks {bodies}/{threads}/{pipelined}] step {step}: manifolds {st['manifolds']} rows {st['rows']} "
                   f"colours {st['colors']} residual {st['residual']:.3e}", flush=True)
         _compare_world(ctx, orc, f"world blocks {bodies} step {step}")
     ctx.close()
@@ -391,7 +396,7 @@ def test_batched_4096_worlds_step_parity(world_blocks):
     sc = scenes.batched_towers(seed=1, n_worlds=4096)
     print(f"[batched 4096] {sc.n_bodies} bodies, world blocks {world_blocks}", flush=True)
     ctx, orc = _world_pair(sc, iterations=10)
-    ctx.solver_set_world_blocks(world_blocks, 256)
+    ctx.solver_set_world_blocks(world_blocks, 256, 1)
     ctx.step(n=50)
     st = ctx.stats()
     print(f"[batched 4096] after 50 steps: pairs {st['pairs']} manifolds {st['manifolds']} colors {st['colors']}",
