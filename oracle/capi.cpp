@@ -403,6 +403,31 @@ int orc_world_step_until(void* h, float dt, int phase) {
   w->stepCount++;
   return 0;
 }
+// Point2PointConstraint(rbA, rbB, pivotInA, pivotInB) added with DiscreteDynamicsWorld.addConstraint; bodyB -1:
+// TypedConstraint's fixed body (pivotB a world point); settings per joint (tau, damping, impulseClamp) or null
+void orc_world_add_p2p(void* h, int n, const int32_t* bodyA, const int32_t* bodyB, const float* pivotA,
+                       const float* pivotB, const float* settings) {
+  World* w = (World*)h;
+  for (int i = 0; i < n; i++) {
+    Joint j{};
+    j.bodyA = bodyA[i];
+    j.bodyB = bodyB[i];
+    j.pivotA = ld3(pivotA + 3 * i);
+    j.pivotB = ld3(pivotB + 3 * i);
+    j.tau = settings ? settings[3 * i] : 0.3f;
+    j.damping = settings ? settings[3 * i + 1] : 1.0f;
+    j.impulseClamp = settings ? settings[3 * i + 2] : 0.0f;
+    w->joints.push_back(j);
+  }
+}
+void orc_world_set_joint_order(void* h, int n, const int32_t* order) {
+  ((World*)h)->explicitJointOrder.assign(order, order + n);
+}
+// appliedImpulse of every joint after the last solve (the sum over its three axes and the iterations)
+void orc_world_joint_impulses(void* h, float* out) {
+  World* w = (World*)h;
+  for (size_t i = 0; i < w->joints.size(); i++) out[i] = w->joints[i].applied;
+}
 // one phase of the step alone: 0 predictUnconstraintMotion, 1 updateAabbs + calculateOverlappingPairs,
 // 2 dispatchAllCollisionPairs, 3 calculateSimulationIslands + solveConstraints, 4 integrateTransforms +
 // updateActivationState (DiscreteDynamicsWorld.java:356-400)

@@ -182,13 +182,27 @@ struct ImpulseInit { // learned warm start: override of the warm-start impulses 
   const float* values = nullptr;  // [row][3] normal, lat1, lat2 (already scaled)
 };
 
+// Point2PointConstraint (dynamics/constraintsolver/Point2PointConstraint.java): a ball socket between pivotA in
+// bodyA's frame and pivotB in bodyB's frame; bodyB == -1 is TypedConstraint's fixed body (static, identity
+// transform, zero velocity), pivotB then a world point.  adiag / applied are the per-solve state of buildJacobian.
+struct Joint {
+  int bodyA, bodyB;
+  V3 pivotA, pivotB;
+  float tau = 0.3f, damping = 1.0f, impulseClamp = 0.0f;  // ConstraintSetting (:202-206)
+  float adiag[3];
+  float applied;
+};
 // The row-order shuffle of one pool: j = 0..n-1 swaps order[j] with order[Random.nextInt(j + 1)]
 // (SequentialImpulseConstraintSolver.java:805-821, randInt2 :138-149).
 void shuffleOrder(std::vector<int>& order, JavaRandom& rng);
 // One solveGroup over an ordered list of manifolds (SequentialImpulseConstraintSolver.java:954-969).
 // shuffle: apply the Random shuffle at iterations with (iter & 7) == 0.
 void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,
-                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features);
+                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features,
+                std::vector<Joint*>* joints = nullptr);
+// Point2PointConstraint.buildJacobian (:71-107) and solveConstraint (:109-177) on the bodies' own velocities
+void jointBuildJacobian(Joint& j, const std::vector<Body>& bodies);
+void jointSolve(Joint& j, std::vector<Body>& bodies, float timeStep);
 
 // ---- islands (collision/dispatch/SimulationIslandManager.java, UnionFind.java)
 struct UnionFind {
@@ -229,6 +243,8 @@ struct World {
   SolverStats lastSolve;
   GjkStats gjk;
   std::vector<int> explicitOrder;  // manifold order for JBGPU_ORDER_EXPLICIT (indices into pairs)
+  std::vector<Joint> joints;       // DiscreteDynamicsWorld.addConstraint, in creation order
+  std::vector<int> explicitJointOrder;  // joint order for JBGPU_ORDER_EXPLICIT
   const float* learnedInit = nullptr;
   std::vector<float>* features = nullptr;
   long stepCount = 0;

@@ -96,6 +96,9 @@ def lib():
             "orc_world_time_steps": (C.c_double, [vp, C.c_float, C.c_int]),
             "orc_shuffle": (None, [C.c_int64, C.c_int, C.c_int, _i32]),
             "orc_world_phase": (C.c_int, [vp, C.c_float, C.c_int]),
+            "orc_world_add_p2p": (None, [vp, C.c_int, _i32, _i32, _f, _f, vp]),
+            "orc_world_set_joint_order": (None, [vp, C.c_int, _i32]),
+            "orc_world_joint_impulses": (None, [vp, _f]),
             "orc_solve_group": (None, [C.c_int, _f, C.c_int, _f, C.POINTER(SolverInfo), _f]),
         }
         for name, (res, args) in sig.items():
@@ -290,6 +293,24 @@ class World:
 
     def step(self, dt=1.0 / 60.0, n=1):
         lib().orc_world_step(self.h, dt, n)
+
+    def add_point2point(self, body_a, body_b, pivot_a, pivot_b, settings=None):
+        """Point2PointConstraint per row (body_b -1: the fixed world body, pivot_b a world point)."""
+        a = np.ascontiguousarray(body_a, np.int32)
+        b = np.ascontiguousarray(body_b, np.int32)
+        self._settings = None if settings is None else f32(settings).reshape(-1)
+        lib().orc_world_add_p2p(self.h, len(a), a, b, f32(pivot_a).reshape(-1), f32(pivot_b).reshape(-1),
+                                _ptr(self._settings))
+        self.nj = getattr(self, "nj", 0) + len(a)
+
+    def set_joint_order(self, order):
+        o = np.ascontiguousarray(order, np.int32)
+        lib().orc_world_set_joint_order(self.h, len(o), o if len(o) else np.zeros(1, np.int32))
+
+    def joint_impulses(self):
+        out = np.zeros(max(getattr(self, "nj", 0), 1), np.float32)
+        lib().orc_world_joint_impulses(self.h, out)
+        return out[:getattr(self, "nj", 0)]
 
     def phase(self, p, dt=1.0 / 60.0):
         """One phase of the step alone (0 predict, 1 AABBs + pairs, 2 narrowphase, 3 islands + solve,

@@ -64,6 +64,72 @@ static SolverRow makeFrictionRow(V3 axis, int ida, int idb, int fi, const MPoint
   return r;
 }
 
+// ---- Point2PointConstraint ------------------------------------------------------------------------------
+namespace {
+Body g_fixed = [] {  // TypedConstraint.getFixed(): new RigidBody(0, null, null), setMassProps(0, (0, 0, 0))
+  Body b{};
+  b.tr.b = m3ident();
+  b.tr.o = v3(0, 0, 0);
+  b.invInertiaWorld = mscale(m3ident(), v3(0, 0, 0));
+  b.flags = JBGPU_BODY_STATIC;
+  return b;
+}();
+}  // namespace
+
+void jointBuildJacobian(Joint& j, const std::vector<Body>& bodies) {  // Point2PointConstraint.java:71-107
+  j.applied = 0.0f;
+  const Body& A = bodies[j.bodyA];
+  const Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : g_fixed;
+  for (int i = 0; i < 3; i++) {
+    V3 normal = v3(0, 0, 0);
+    setcoord(normal, i, 1.0f);
+    const M3 world2A = transpose(A.tr.b), world2B = transpose(B.tr.b);
+    const V3 rel1 = sub(xform(A.tr, j.pivotA), A.tr.o);
+    const V3 rel2 = sub(xform(B.tr, j.pivotB), B.tr.o);
+    // JacobianEntry.init, two bodies (JacobianEntry.java:56-73)
+    const V3 aJ = mulMV(world2A, crs(rel1, normal));
+    const V3 bJ = mulMV(world2B, crs(rel2, scl(normal, -1.0f)));
+    const V3 m0 = mulc(A.invInertiaLocal, aJ), m1 = mulc(B.invInertiaLocal, bJ);
+    j.adiag[i] = A.invMass + dot(m0, aJ) + B.invMass + dot(m1, bJ);
+  }
+}
+
+// RigidBody.applyImpulse (RigidBody.java:414-428) with linear and angular factors 1
+static void applyImpulseRB(Body& b, V3 impulse, V3 relPos) {
+  if (b.invMass == 0.0f) return;
+  const V3 scaled = v3(impulse.x * 1.0f, impulse.y * 1.0f, impulse.z * 1.0f);  // applyCentralImpulse :395-402
+  b.lv = add(b.lv, scl(scaled, b.invMass));
+  V3 t = crs(relPos, v3(impulse.x * 1.0f, impulse.y * 1.0f, impulse.z * 1.0f));
+  t = v3(t.x * 1.0f, t.y * 1.0f, t.z * 1.0f);
+  b.av = add(b.av, mulMV(b.invInertiaWorld, t));  // applyTorqueImpulse :405-411
+}
+
+void jointSolve(Joint& j, std::vector<Body>& bodies, float timeStep) {  // Point2PointConstraint.java:109-177
+  Body fixed = g_fixed;
+  Body& A = bodies[j.bodyA];
+  Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : fixed;
+  const V3 pivotAInW = xform(A.tr, j.pivotA), pivotBInW = xform(B.tr, j.pivotB);
+  for (int i = 0; i < 3; i++) {
+    V3 normal = v3(0, 0, 0);
+    setcoord(normal, i, 1.0f);
+    const float jacDiagABInv = 1.0f / j.adiag[i];
+    const V3 rel1 = sub(pivotAInW, A.tr.o), rel2 = sub(pivotBInW, B.tr.o);
+    const V3 vel1 = add(crs(A.av, rel1), A.lv), vel2 = add(crs(B.av, rel2), B.lv);  // getVelocityInLocalPoint
+    const V3 vel = sub(vel1, vel2);
+    const float rel_vel = dot(normal, vel);
+    const float depth = -dot(sub(pivotAInW, pivotBInW), normal);
+    float impulse = depth * j.tau / timeStep * jacDiagABInv - j.damping * rel_vel * jacDiagABInv;
+    if (j.impulseClamp > 0.0f) {
+      if (impulse < -j.impulseClamp) impulse = -j.impulseClamp;
+      if (impulse > j.impulseClamp) impulse = j.impulseClamp;
+    }
+    j.applied += impulse;
+    const V3 iv = scl(normal, impulse);
+    applyImpulseRB(A, iv, sub(pivotAInW, A.tr.o));
+    applyImpulseRB(B, scl(iv, -1.0f), sub(pivotBInW, B.tr.o));
+  }
+}
+
 void shuffleOrder(std::vector<int>& order, JavaRandom& rng) {  // :805-821, one pool
   for (int j = 0; j < (int)order.size(); ++j) {
     int tmp = order[j];
@@ -74,7 +140,8 @@ void shuffleOrder(std::vector<int>& order, JavaRandom& rng) {  // :805-821, one 
 }
 
 void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, const SolverInfo& info, bool shuffle,
-                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features) {
+                JavaRandom& rng, SolverStats* st, const float* learnedInit, std::vector<float>* features,
+                std::vector<Joint*>* joints) {
   std::vector<SolverBody> sb;
   std::vector<SolverRow> rows, frows;
   std::vector<int> companion(bodies.size(), -1);
@@ -188,6 +255,8 @@ void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, co
       rows.push_back(r);
     }
   }
+  if (joints)  // buildJacobian of every constraint after the contact rows (:757-763)
+    for (Joint* jt : *joints) jointBuildJacobian(*jt, bodies);
   std::vector<int> order(rows.size()), forder(frows.size());
   for (size_t i = 0; i < rows.size(); i++) order[i] = (int)i;
   for (size_t i = 0; i < frows.size(); i++) forder[i] = (int)i;
@@ -206,6 +275,22 @@ void solveGroup(std::vector<Body>& bodies, std::vector<Manifold*>& manifolds, co
     }
     for (size_t i = 0; i < rows.size(); i++) prev[i] = rows[i].applied;
     for (size_t i = 0; i < frows.size(); i++) prev[rows.size() + i] = frows[i].applied;
+    if (joints) {  // constraints first, on the rigid bodies' velocities (:823-842)
+      for (Joint* jt : *joints) {
+        const int ids[2] = {jt->bodyA, jt->bodyB};
+        for (int bi : ids)  // SolverBody.writebackVelocity of a body that has a solver body
+          if (bi >= 0 && bodies[bi].islandTag >= 0 && companion[bi] >= 0 && sb[companion[bi]].invMass != 0.0f) {
+            bodies[bi].lv = sb[companion[bi]].lv;
+            bodies[bi].av = sb[companion[bi]].av;
+          }
+        jointSolve(*jt, bodies, info.timeStep);
+        for (int bi : ids)  // SolverBody.readVelocity
+          if (bi >= 0 && bodies[bi].islandTag >= 0 && companion[bi] >= 0 && sb[companion[bi]].invMass != 0.0f) {
+            sb[companion[bi]].lv = bodies[bi].lv;
+            sb[companion[bi]].av = bodies[bi].av;
+          }
+      }
+    }
     for (int j = 0; j < (int)order.size(); j++) {  // resolveSingleCollisionCombinedCacheFriendly (:234-284)
       SolverRow& c = rows[order[j]];
       SolverBody& A = sb[c.bodyA];

@@ -222,6 +222,12 @@ void World::calculateSimulationIslands() {  // DiscreteDynamicsWorld.java:649-67
     int a = (int)(ps.key >> 32) - 1, b = (int)(ps.key & 0xffffffffu) - 1;
     if (!bodies[a].isStatic() && !bodies[b].isStatic()) uf.unite(bodies[a].islandTag, bodies[b].islandTag);
   }
+  for (const Joint& j : joints) {  // constraints join islands (DiscreteDynamicsWorld.java:655-669)
+    if (j.bodyB < 0) continue;       // TypedConstraint's fixed body is static
+    const Body& a = bodies[j.bodyA];
+    const Body& b = bodies[j.bodyB];
+    if (!a.isStatic() && !b.isStatic() && (a.isActive() || b.isActive())) uf.unite(a.islandTag, b.islandTag);
+  }
   for (int i = 0; i < n; i++) {  // storeIslandActivationState (:92-110)
     if (!bodies[i].isStatic()) {
       bodies[i].islandTag = uf.find(i);
@@ -267,9 +273,20 @@ void World::solveConstraints() {  // DiscreteDynamicsWorld.java:620-647 -> build
   if (cfg.solverOrder == JBGPU_ORDER_EXPLICIT) {  // island sleep states above, then the caller's manifold order
     std::vector<Manifold*> ms;
     for (int idx : explicitOrder) ms.push_back(&pairs[idx].m);
-    solveGroup(bodies, ms, info, false, rng, &lastSolve, learnedInit, features);
+    std::vector<Joint*> js;
+    for (int idx : explicitJointOrder) js.push_back(&joints[idx]);
+    if (!ms.empty() || !js.empty()) solveGroup(bodies, ms, info, false, rng, &lastSolve, learnedInit, features, &js);
     return;
   }
+  // sortedConstraints: the constraints quicksorted by island (DiscreteDynamicsWorld.java:621-629)
+  std::vector<Joint*> sj;
+  for (Joint& j : joints) sj.push_back(&j);
+  auto jointIsland = [&](const Joint* j) {  // getConstraintIslandId (:550-556)
+    const int a = bodies[j->bodyA].islandTag;
+    return a >= 0 ? a : (j->bodyB >= 0 ? bodies[j->bodyB].islandTag : -1);
+  };
+  jquickSort(sj, [&](const Joint* l, const Joint* r) { return jointIsland(l) < jointIsland(r); });
+  const int numJoints = (int)sj.size();
   std::vector<Manifold*> im;
   for (auto& ps : pairs) {
     Manifold& m = ps.m;
@@ -300,9 +317,22 @@ void World::solveConstraints() {  // DiscreteDynamicsWorld.java:620-647 -> build
         numIslandManifolds = em - sm;
       }
     }
-    if (!islandSleeping && numIslandManifolds > 0) {
-      std::vector<Manifold*> ms(im.begin() + startIdx, im.begin() + startIdx + numIslandManifolds);
-      solveGroup(bodies, ms, info, cfg.solverOrder == JBGPU_ORDER_REFERENCE, rng, &lastSolve, learnedInit, features);
+    // the island's constraints (InplaceSolverIslandCallback.processIsland, DiscreteDynamicsWorld.java:586-612)
+    int startC = -1, numC = 0, ci = 0;
+    for (; ci < numJoints; ci++)
+      if (jointIsland(sj[ci]) == islandId) {
+        startC = ci;
+        break;
+      }
+    for (; ci < numJoints; ci++)
+      if (jointIsland(sj[ci]) == islandId) numC++;
+    if (!islandSleeping && numIslandManifolds + numC > 0) {
+      std::vector<Manifold*> ms;
+      if (numIslandManifolds > 0) ms.assign(im.begin() + startIdx, im.begin() + startIdx + numIslandManifolds);
+      std::vector<Joint*> js;
+      if (numC > 0) js.assign(sj.begin() + startC, sj.begin() + startC + numC);
+      solveGroup(bodies, ms, info, cfg.solverOrder == JBGPU_ORDER_REFERENCE, rng, &lastSolve, learnedInit, features,
+                 &js);
     }
     if (numIslandManifolds != 0) sm = em;
   }

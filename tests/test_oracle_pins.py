@@ -934,3 +934,172 @@ def test_aabb_overflow_disables_simulation():
     for _ in range(10):
         v = v + np.float32(1.0) * dt * g  # invMass * dt * (gravity force = g / invMass)
     assert abs(b10[1, 13] - v) < 1e-5
+
+
+# ------------------------------------------------------------------ Point2PointConstraint
+def _pendulum(theta0=0.1, L=1.0, r=0.1, pivot=(0.0, 10.0, 0.0), iterations=10):
+    """A sphere (radius r, mass 1) hanging from a world point by a Point2PointConstraint whose pivot lies on
+    the sphere's body at distance L from its centre (a physical pendulum), released at theta0 from rest; the
+    floor is 50 m below, out of reach."""
+    b = scenes._Builder("pendulum")
+    pl = b.shape(PLANE, [0, 1, 0, -50.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    s = b.shape(SPHERE, [r])
+    pos = np.array([pivot[0] + L * np.sin(theta0), pivot[1] - L * np.cos(theta0), pivot[2]], np.float32)
+    b.body(s, scenes.IDENT, pos, 1.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=iterations, order=0)
+    w.add_point2point([1], [-1], np.float32(pivot) - pos, np.float32(pivot))
+    return w, np.float32(pivot) - pos
+
+
+def test_point2point_pendulum_period_closed_form():
+    """Physical pendulum: period 2 pi sqrt(I_p / (m g L)) with I_p = 2/5 m r^2 + m L^2 (parallel axes),
+    times (1 + theta0^2 / 16) for the amplitude; the simulated period (zero crossings over about five periods at
+    dt 1/60, 10 iterations, Baumgarte tau 0.3) lies within 2% of it, and the pivot stays within 1% of L of its
+    world point (the constraint holds)."""
+    L, r, th = 1.0, 0.1, 0.1
+    w, pa = _pendulum(th, L, r)
+    xs, drift = [], 0.0
+    for _ in range(660):
+        w.step()
+        b = w.bodies()[1]
+        R = b[:9].reshape(3, 3).astype(np.float64)
+        pw = R @ pa.astype(np.float64) + b[9:12]
+        drift = max(drift, np.linalg.norm(pw - np.array([0.0, 10.0, 0.0])))
+        xs.append(float(b[9]))
+    xs = np.array(xs)
+    t = (np.arange(len(xs)) + 1) / 60.0
+    cross = [t[i] + (t[i + 1] - t[i]) * xs[i] / (xs[i] - xs[i + 1]) for i in range(len(xs) - 1)
+             if xs[i] > 0 >= xs[i + 1] or xs[i] < 0 <= xs[i + 1]]
+    period = 2.0 * np.mean(np.diff(cross))
+    want = 2 * np.pi * np.sqrt((0.4 * r * r + L * L) / (9.81 * L)) * (1 + th * th / 16)
+    assert len(cross) >= 9
+    assert abs(period / want - 1) < 0.02, (period, want)
+    assert drift < 0.01 * L, drift
+
+
+def test_point2point_joined_bodies_fall_together():
+    """Two boxes joined at a shared point in free fall: gravity alone keeps the joint closed, so the solver's
+    impulses stay near zero and the pivots stay together while the pair falls freely (v_y = -9.81 t)."""
+    b = scenes._Builder("pair")
+    pl = b.shape(PLANE, [0, 1, 0, -500.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    bx = b.shape(BOX, [0.5, 0.5, 0.5])
+    b.body(bx, scenes.IDENT, np.array([0, 10, 0], np.float32), 1.0)
+    b.body(bx, scenes.IDENT, np.array([1.0, 10, 0], np.float32), 2.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=10, order=0)
+    w.add_point2point([1], [2], [[0.5, 0, 0]], [[-0.5, 0, 0]])
+    for k in range(60):
+        w.step()
+    bb = w.bodies()
+    assert abs(float(bb[1, 13]) + 9.81) < 1e-3 and abs(float(bb[2, 13]) + 9.81) < 1e-3
+    assert abs(float(bb[2, 9] - bb[1, 9]) - 1.0) < 1e-4
+    assert np.abs(w.joint_impulses()).max() < 1e-4
+
+
+def test_point2point_hanging_chain_carries_the_weights():
+    """world <- box 1 (1 kg) <- box 2 (2 kg), both joints vertical: at rest each joint's impulse per step is the
+    weight it carries times dt, with the sign of the push on its body A (box 1 in both): joint 1 lifts box 1 by
+    (1 + 2) 9.81 / 60 = 0.4905, joint 2 pulls it down by 2 * 9.81 / 60 = 0.327; within 2%, and the chain hangs
+    still (both boxes within 1 cm of where they started)."""
+    b = scenes._Builder("chain")
+    pl = b.shape(PLANE, [0, 1, 0, -50.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    bx = b.shape(BOX, [0.4, 0.4, 0.4])
+    b.body(bx, scenes.IDENT, np.array([0, 9, 0], np.float32), 1.0)
+    b.body(bx, scenes.IDENT, np.array([0, 8, 0], np.float32), 2.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=10, order=0)
+    w.add_point2point([1, 1], [-1, 2], [[0, 0.5, 0], [0, -0.5, 0]], [[0, 9.5, 0], [0, 0.5, 0]])
+    for _ in range(120):
+        w.step()
+    bb = w.bodies()
+    assert np.abs(bb[1, 9:12] - [0, 9, 0]).max() < 0.01 and np.abs(bb[2, 9:12] - [0, 8, 0]).max() < 0.01
+    imp = w.joint_impulses()
+    assert abs(imp[0] / (3 * 9.81 / 60) - 1) < 0.02 and abs(imp[1] / (-2 * 9.81 / 60) - 1) < 0.02, imp
+
+
+def test_point2point_holds_a_body_in_contact():
+    """A box resting on the ground, pinned at its centre to that world point and given 1 m/s sideways: the
+    joint runs on the rigid-body velocities between the contact rows' solver bodies' write-back and read-back
+    (SolverBody.writebackVelocity / readVelocity), so it holds the box while it touches the ground; friction
+    alone would let it slide v^2 / (2 mu g) = 0.1 m."""
+    b = scenes._Builder("pinned")
+    pl = b.shape(PLANE, [0, 1, 0, 0.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    bx = b.shape(BOX, [0.5, 0.5, 0.5])
+    b.body(bx, scenes.IDENT, np.array([0, 0.5, 0], np.float32), 1.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    bodies = np.asarray(bodies, np.float32).reshape(-1, po.BODY_WORDS).copy()
+    bodies[1, 12] = 1.0
+    w = po.World(shapes, verts, bodies, iterations=10, order=0)
+    w.add_point2point([1], [-1], [[0, 0, 0]], [[0, 0.5, 0]])
+    for _ in range(60):
+        w.step()
+        assert abs(float(w.bodies()[1, 9])) < 0.01
+    assert len(w.pairs()[0]) == 1                        # still touching the ground
+
+
+def test_point2point_joins_islands_for_sleep():
+    """A heavy box at rest on a ledge with a small box hanging from it as a pendulum, swinging beside the ledge:
+    the joint puts both in one simulation island (DiscreteDynamicsWorld.calculateSimulationIslands), so the
+    resting box comes to want deactivation after 2 s but the island never sleeps while the pendulum swings."""
+    b = scenes._Builder("ledge")
+    lb = b.shape(BOX, [2.0, 1.0, 2.0])
+    b.body(lb, scenes.IDENT, np.array([0, 1, 0], np.float32), 0.0)
+    big = b.shape(BOX, [0.5, 0.5, 0.5])
+    b.body(big, scenes.IDENT, np.array([1.5, 2.5, 0], np.float32), 100.0)
+    small = b.shape(BOX, [0.1, 0.1, 0.1])
+    b.body(small, scenes.IDENT, np.array([2.3, 1.5, 0], np.float32), 0.1)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    bodies = np.asarray(bodies, np.float32).reshape(-1, po.BODY_WORDS).copy()
+    bodies[2, 14] = 2.0
+    w = po.World(shapes, verts, bodies, iterations=10, order=0, deactivation_disabled=False)
+    w.add_point2point([1], [2], [[0.8, 0, 0]], [[0, 1.0, 0]])
+    states = []
+    for _ in range(300):
+        w.step()
+        states.append(w.bodies()[:, 29].view(np.int32).copy())
+    states = np.array(states)
+    assert (states[:, 1:] != 2).all()                    # nobody sleeps
+    assert (states[150:, 1] == 3).any()                  # the resting box wants to, after 2 s at rest
+    assert (states[:, 2] != 3).all()                     # the swinging box never does
+
+
+def test_point2point_pair_on_the_ground_balances_momentum():
+    """Two boxes (1 and 2 kg) on the ground, joined between their facing sides: the joint's impulses are internal,
+    so over steps a+1..b the ground's normal impulses (on the points solved, distance <= 0; the others keep an
+    older step's value) equal the weight's impulse plus the pair's momentum change,
+    sum = 3 * 9.81 / 60 * (b - a) + p_y(b) - p_y(a), to 1e-5.  The joint rows run between the contact rows on the
+    same velocities; a joint that ran on stale rigid-body velocities would wipe the contacts' earlier iterations
+    and their stored impulses would pile up about one weight per iteration."""
+    b = scenes._Builder("pair_on_ground")
+    pl = b.shape(PLANE, [0, 1, 0, 0.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    bx = b.shape(BOX, [0.5, 0.5, 0.5])
+    b.body(bx, scenes.IDENT, np.array([0, 0.5, 0], np.float32), 1.0)
+    b.body(bx, scenes.IDENT, np.array([1.1, 0.5, 0], np.float32), 2.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    w = po.World(shapes, verts, bodies, iterations=10, order=0)
+    w.add_point2point([1], [2], [[0.55, 0, 0]], [[-0.55, 0, 0]])
+    totals, p = [], []
+    for k in range(300):
+        w.step()
+        keys, man = w.pairs()
+        hdr = man[:, :4].view(np.int32)                  # points, body 0, body 1, pair type; 32 words a point
+        assert hdr[(hdr[:, 1] != 0) & (hdr[:, 2] != 0), 0].sum() == 0    # the boxes do not touch
+        totals.append(sum(float(m[4 + 32 * j + 18]) for m, h in zip(man, hdr) for j in range(h[0])
+                          if m[4 + 32 * j + 15] <= 0.0))
+        bb = w.bodies()
+        p.append(1.0 * float(bb[1, 13]) + 2.0 * float(bb[2, 13]))
+    a, b_ = 20, 299
+    want = 3 * 9.81 / 60 * (b_ - a) + p[b_] - p[a]
+    assert abs(sum(totals[a + 1:b_ + 1]) / want - 1) < 1e-5, (sum(totals[a + 1:b_ + 1]), want)
