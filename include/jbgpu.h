@@ -152,6 +152,25 @@ int jbgpu_world_set_pairs(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const
  * jbgpu_last_error(NULL, ...) otherwise.  Needs no device. */
 int jbgpu_check_pair_records(int32_t nbodies, int64_t n, const uint64_t* keys, const float* manifolds);
 int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out);
+/* DiscreteDynamicsWorld.addConstraint (DiscreteDynamicsWorld.java:393-400) of n Point2PointConstraints
+ * (dynamics/constraintsolver/Point2PointConstraint.java:35-69): a ball socket between pivot_a (3 floats each, in
+ * body A's frame) and pivot_b (in body B's frame); body_b -1 is TypedConstraint's fixed body (static, identity
+ * transform), pivot_b then a world point.  settings: per constraint (tau, damping, impulseClamp) of its
+ * ConstraintSetting, or NULL for (0.3, 1, 0).  Constraint ids follow creation order from 0.  Every step solves
+ * the constraints of the awake islands before the contact rows in each iteration (buildJacobian, solveConstraint;
+ * SequentialImpulseConstraintSolver.java:757-763, 823-842), and they join islands
+ * (DiscreteDynamicsWorld.calculateSimulationIslands :655-669).  Bodies out of range, or body A equal to body B,
+ * give JBGPU_ERR_ARG and change nothing. */
+int jbgpu_world_add_point2point(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
+                                const float* pivot_a, const float* pivot_b, const float* settings);
+int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n);
+/* appliedImpulse of every constraint (TypedConstraint.getAppliedImpulse): the sum over the last solve that
+ * processed it of its impulses on all three axes and iterations; 0 before its first solve. */
+int jbgpu_world_joint_impulses(jbgpu_ctx* ctx, float* out);
+/* The constraints the last step solved, in solve order (joint colour after joint colour, creation order inside
+ * a colour): a SequentialImpulseConstraintSolver given this constraint order (and the manifold order of
+ * jbgpu_solver_get_order) produces the same velocities and impulses.  *n = their count; at most cap written. */
+int jbgpu_world_joint_order(jbgpu_ctx* ctx, int32_t* order, int32_t cap, int32_t* n);
 
 /* ---- Dispatcher ----------------------------------------------------------------------- */
 /* CollisionDispatcher.dispatchAllCollisionPairs(pairCache, dispatchInfo, dispatcher)

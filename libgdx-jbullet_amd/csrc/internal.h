@@ -93,6 +93,11 @@ constexpr int kEpaSmallGrid = 2048;  // first-pass EPA blocks (eight per CU)
 constexpr int kEpaGrid = 512;  // EPA blocks (LDS workspace limits them to two per CU)
 constexpr int kTimerRegions = 16;
 
+// Point2Point constraint record: body A, body B (-1: the fixed body), pivot in A, pivot in B (a world point
+// for the fixed body), tau, damping, impulse clamp, 0
+constexpr int kJointWords = 12;
+constexpr int kJointStateWords = 32;
+
 // solver rows per manifold: 4 contacts x (normal row + 2 friction rows)
 constexpr int kRowWords = 20;   // normal(3) rel1xn(3) rel2xn(3) angA(3) angB(3) jacInv pen rest fric applied
 constexpr int kContactRowWords = 3 * kRowWords;
@@ -209,6 +214,18 @@ struct Ctx {
   std::vector<int> bodyShapeType;
   std::vector<int> worldHost;       // host copies: world id and static flag per body (world-block tables)
   std::vector<char> bodyStaticHost;
+  std::vector<float> bodyInvMassHost;
+  // Point2Point constraints (DiscreteDynamicsWorld.addConstraint): records in creation order, their solve order
+  // (greedy joint colours: joints of one colour share no body that a joint impulse moves) and per-solve state
+  int nj = 0;
+  std::vector<float> jointHost;       // kJointWords per joint
+  DBuf<float> joints;                 // the same on the device
+  DBuf<float> jointState;             // kJointStateWords per joint: 1 / diagonal, applied impulse, arms, inertia
+  DBuf<int> jointOrder;               // joint indices, colour after colour (creation order inside a colour)
+  DBuf<int> jointSolve;               // per joint: solved in the last solve (its island awake)
+  std::vector<int> jointColourOff;    // colour k = jointOrder[off[k], off[k + 1])
+  DBuf<int> jointColourOffDev;
+  bool jointOrderReady = false;       // the order matches the joints and the bodies' static flags
   // batched worlds solved one workgroup per block of whole worlds (the whole solve in one launch)
   int wbBodies = 0;      // at most this many dynamic bodies per block (0: off)
   int wbThreads = 256;   // workgroup size of the block solve
@@ -363,6 +380,9 @@ void solverCollectStats(Ctx& c);                         // residual, rows and s
 // step's own selection (activity, islands, rows)
 void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList = nullptr);
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots);  // per slot final (normal, lat1, lat2)
+void jointsChanged(Ctx& c);                                  // joints or bodies changed: new solve order
+void jointsImpulses(Ctx& c, float* out);                     // appliedImpulse per joint
+int jointsOrderSolved(Ctx& c, int32_t* order, int cap);      // joints solved in the last solve, in order
 // integration
 void worldPredict(Ctx& c, float dt);
 void worldIntegrate(Ctx& c, float dt);

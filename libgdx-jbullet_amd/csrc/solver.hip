@@ -1411,6 +1411,178 @@ __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const
   }
 }
 
+
+// ---- Point2PointConstraint (dynamics/constraintsolver/Point2PointConstraint.java) ---------------------------
+// One thread per constraint.  The constraints are solved on the solver body records: the reference copies a
+// body's solver velocities into the rigid body before each constraint and back after it (SolverBody.
+// writebackVelocity / readVelocity, SequentialImpulseConstraintSolver.java:823-842), and a body without solver
+// rows keeps its rigid-body velocities, which its solver record here equals.  The fixed body (bodyB == -1,
+// TypedConstraint.getFixed) has the identity transform, zero velocity, zero inverse mass and inertia.
+__device__ __forceinline__ int jointBodyA(const float* R) { return ldi(R); }
+__device__ __forceinline__ int jointBodyB(const float* R) { return ldi(R + 1); }
+__device__ __forceinline__ Tr bodyTr(const float* __restrict__ tr, int i) {
+  Tr t;
+  if (i < 0) {
+    t.b = mident();
+    t.o = v3(0.0f, 0.0f, 0.0f);
+    return t;
+  }
+  for (int k = 0; k < 9; k++) t.b.m[k] = tr[12 * i + k];
+  t.o = v3(tr[12 * i + 9], tr[12 * i + 10], tr[12 * i + 11]);
+  return t;
+}
+__device__ __forceinline__ V3 unitAxis(int i) {  // new Vector3f(0, 0, 0) with coordinate i set to 1
+  V3 n = v3(0.0f, 0.0f, 0.0f);
+  if (i == 0) n.x = 1.0f;
+  else if (i == 1) n.y = 1.0f;
+  else n.z = 1.0f;
+  return n;
+}
+// the island merge of a constraint (DiscreteDynamicsWorld.calculateSimulationIslands :655-669): two
+// non-static bodies, one of them active
+__global__ void kIslandHookJoints(int nj, const float* __restrict__ rec, const float* __restrict__ props,
+                                  const int* __restrict__ act, int* parent, int* changed) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const float* R = rec + kJointWords * j;
+  const int a = jointBodyA(R), b = jointBodyB(R);
+  if (b < 0 || isStaticBody(props, a) || isStaticBody(props, b)) return;
+  if (!isActiveState(act[a]) && !isActiveState(act[b])) return;
+  int ra = islandRoot(parent, a), rb = islandRoot(parent, b);
+  if (ra == rb) return;
+  atomicMin(parent + max(ra, rb), min(ra, rb));
+  *changed = 1;
+}
+// constraints solved this step: the island of body A (of body B when A is static; getConstraintIslandId
+// :550-556) is processed, i.e. it has no inactive body (buildAndProcessIslands :283-289); without islands
+// (deactivation disabled) its non-static bodies are active
+__global__ void kJointMark(int nj, const float* __restrict__ rec, const float* __restrict__ props,
+                           const int* __restrict__ act, const int* __restrict__ island,
+                           const int* __restrict__ inactive, int* solve) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  const float* R = rec + kJointWords * j;
+  const int a = jointBodyA(R), b = jointBodyB(R);
+  const bool sa = isStaticBody(props, a), sb = b < 0 || isStaticBody(props, b);
+  int f = 0;
+  if (!(sa && sb)) {
+    if (island) f = !inactive[island[sa ? b : a]];
+    else f = (sa || isActiveState(act[a])) && (sb || isActiveState(act[b]));
+  }
+  solve[j] = f;
+}
+// buildJacobian (Point2PointConstraint.java:71-107, JacobianEntry.java:56-73) with appliedImpulse = 0; the
+// state keeps what solveConstraint recomputes from the unchanged transforms: 1 / diagonal per axis, the applied
+// impulse, the pivots' arms rel1 / rel2, pivotAInW - pivotBInW, and both bodies' world inverse inertia
+__global__ void kJointBuild(int nj, const float* __restrict__ rec, const int* __restrict__ solve,
+                            const float* __restrict__ tr, const float* __restrict__ props, float* st) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj || !solve[j]) return;
+  const float* R = rec + kJointWords * j;
+  float* S = st + kJointStateWords * j;
+  const int a = jointBodyA(R), b = jointBodyB(R);
+  const Tr ta = bodyTr(tr, a), tb = bodyTr(tr, b);
+  const V3 pivotAInW = xform(ta, ld3(R + 2)), pivotBInW = xform(tb, ld3(R + 5));
+  const V3 rel1 = sub(pivotAInW, ta.o), rel2 = sub(pivotBInW, tb.o);
+  const float imA = props[8 * a], imB = b >= 0 ? props[8 * b] : 0.0f;
+  const V3 ilA = v3(props[8 * a + 1], props[8 * a + 2], props[8 * a + 3]);
+  const V3 ilB = b >= 0 ? v3(props[8 * b + 1], props[8 * b + 2], props[8 * b + 3]) : v3(0.0f, 0.0f, 0.0f);
+  const M3 world2A = transpose(ta.b), world2B = transpose(tb.b);
+  for (int i = 0; i < 3; i++) {
+    const V3 normal = unitAxis(i);
+    const V3 aJ = mulMV(world2A, crs(rel1, normal));
+    const V3 bJ = mulMV(world2B, crs(rel2, scl(normal, -1.0f)));
+    const V3 m0 = mulc(ilA, aJ), m1 = mulc(ilB, bJ);
+    const float diag = imA + dot(m0, aJ) + imB + dot(m1, bJ);
+    S[i] = 1.0f / diag;
+  }
+  S[3] = 0.0f;
+  st3(S + 4, rel1);
+  st3(S + 7, rel2);
+  st3(S + 10, sub(pivotAInW, pivotBInW));
+  S[13] = 0.0f;
+  const M3 IA = invInertiaWorld(tr, props, a);
+  M3 IB;
+  if (b >= 0) IB = invInertiaWorld(tr, props, b);
+  else
+    for (int k = 0; k < 9; k++) IB.m[k] = 0.0f;
+  for (int k = 0; k < 9; k++) {
+    S[14 + k] = IA.m[k];
+    S[23 + k] = IB.m[k];
+  }
+}
+// RigidBody.applyImpulse (RigidBody.java:414-428, linear and angular factors 1): applyCentralImpulse, then
+// applyTorqueImpulse(relPos x impulse) through the world inverse inertia
+__device__ __forceinline__ void applyImpulseRB(SBody& b, V3 impulse, V3 relPos, const float* I) {
+  if (b.invMass == 0.0f) return;
+  b.lv = add(b.lv, scl(impulse, b.invMass));
+  M3 m;
+  for (int k = 0; k < 9; k++) m.m[k] = I[k];
+  b.av = add(b.av, mulMV(m, crs(relPos, impulse)));
+}
+// solveConstraint (Point2PointConstraint.java:109-177): per axis the positional error (tau / dt) and the
+// damped relative velocity along it, clamped when impulseClamp > 0
+__device__ void jointSolve(int j, float dt, const float* __restrict__ rec, float* st, float* svel) {
+  const float* R = rec + kJointWords * j;
+  float* S = st + kJointStateWords * j;
+  const int a = jointBodyA(R), b = jointBodyB(R);
+  const float tau = R[8], damping = R[9], clamp = R[10];
+  SBody A = loadBody(svel, nullptr, a), B;
+  if (b >= 0) B = loadBody(svel, nullptr, b);
+  else {
+    B.lv = v3(0.0f, 0.0f, 0.0f);
+    B.av = v3(0.0f, 0.0f, 0.0f);
+    B.invMass = 0.0f;
+  }
+  const V3 rel1 = ld3(S + 4), rel2 = ld3(S + 7), dpiv = ld3(S + 10);
+  float applied = S[3];
+  for (int i = 0; i < 3; i++) {
+    const V3 normal = unitAxis(i);
+    const float jacDiagABInv = S[i];
+    const V3 vel1 = add(crs(A.av, rel1), A.lv), vel2 = add(crs(B.av, rel2), B.lv);  // getVelocityInLocalPoint
+    const V3 vel = sub(vel1, vel2);
+    const float rel_vel = dot(normal, vel);
+    const float depth = -dot(dpiv, normal);
+    float impulse = depth * tau / dt * jacDiagABInv - damping * rel_vel * jacDiagABInv;
+    if (clamp > 0.0f) {
+      if (impulse < -clamp) impulse = -clamp;
+      if (impulse > clamp) impulse = clamp;
+    }
+    applied += impulse;
+    const V3 iv = scl(normal, impulse);
+    applyImpulseRB(A, iv, rel1, S + 14);
+    applyImpulseRB(B, scl(iv, -1.0f), rel2, S + 23);
+  }
+  S[3] = applied;
+  if (A.invMass != 0.0f) storeBody(svel, a, A);
+  if (b >= 0 && B.invMass != 0.0f) storeBody(svel, b, B);
+}
+// one joint colour per launch: joints [off, off + cnt) of the order
+__global__ void kJointSweep(int off, int cnt, float dt, const int* __restrict__ order, const int* __restrict__ solve,
+                            const float* __restrict__ rec, float* st, float* svel, const int* __restrict__ done) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt || (done && *done)) return;
+  const int j = order[off + k];
+  if (solve[j]) jointSolve(j, dt, rec, st, svel);
+}
+// every joint colour in one workgroup, colour after colour, separated by workgroup barriers
+constexpr int kJointTailThreads = 1024;
+__global__ __launch_bounds__(kJointTailThreads) void kJointSweepAll(int ncol, const int* __restrict__ colourOff,
+                                                                    float dt, const int* __restrict__ order,
+                                                                    const int* __restrict__ solve,
+                                                                    const float* __restrict__ rec, float* st,
+                                                                    float* svel, const int* __restrict__ done) {
+  if (done && *done) return;
+  for (int c = 0; c < ncol; c++) {
+    const int a = colourOff[c], e = colourOff[c + 1];
+    for (int k = a + (int)threadIdx.x; k < e; k += kJointTailThreads) {
+      const int j = order[k];
+      if (solve[j]) jointSolve(j, dt, rec, st, svel);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 void solverGetImpulses(Ctx& c, float* out, int64_t nslots) {
@@ -1424,6 +1596,111 @@ void solverGetImpulses(Ctx& c, float* out, int64_t nslots) {
   JB_CHECK(hipMemcpyAsync(out, c.mlpY.p, 3 * (size_t)nslots * sizeof(float), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
 }
+
+void jointsChanged(Ctx& c) { c.jointOrderReady = false; }
+
+// The joints' solve order: greedy colours in creation order (a joint takes the smallest colour that no earlier
+// joint on one of its moving bodies has; a body moves when it is not static or has an inverse mass), then colour
+// after colour.  Solving the colours one after the other is the sequential solver fed this joint order.
+static void jointsMakeOrder(Ctx& c) {
+  if (c.jointOrderReady) return;
+  const int nj = c.nj;
+  std::vector<int> colour(nj);
+  std::vector<std::vector<int>> used(c.nb);
+  auto moves = [&](int b) { return b >= 0 && (!c.bodyStaticHost[b] || c.bodyInvMassHost[b] != 0.0f); };
+  auto taken = [&](int b, int k) {
+    if (!moves(b)) return false;
+    for (int u : used[b])
+      if (u == k) return true;
+    return false;
+  };
+  int ncol = 0;
+  for (int j = 0; j < nj; j++) {
+    const int a = ldi(&c.jointHost[(size_t)kJointWords * j]), b = ldi(&c.jointHost[(size_t)kJointWords * j + 1]);
+    int k = 0;
+    while (taken(a, k) || taken(b, k)) k++;
+    colour[j] = k;
+    if (moves(a)) used[a].push_back(k);
+    if (moves(b)) used[b].push_back(k);
+    ncol = std::max(ncol, k + 1);
+  }
+  c.jointColourOff.assign(ncol + 1, 0);
+  for (int j = 0; j < nj; j++) c.jointColourOff[colour[j] + 1]++;
+  for (int k = 0; k < ncol; k++) c.jointColourOff[k + 1] += c.jointColourOff[k];
+  std::vector<int> order(std::max(nj, 1)), fill(c.jointColourOff.begin(), c.jointColourOff.end() - 1);
+  for (int j = 0; j < nj; j++) order[fill[colour[j]]++] = j;
+  c.jointOrder.ensure(std::max(nj, 1));
+  c.jointColourOffDev.ensure(ncol + 1);
+  c.jointSolve.ensure(std::max(nj, 1));
+  JB_CHECK(hipMemcpy(c.jointOrder.p, order.data(), (size_t)nj * sizeof(int), hipMemcpyHostToDevice));
+  JB_CHECK(hipMemcpy(c.jointColourOffDev.p, c.jointColourOff.data(), (ncol + 1) * sizeof(int),
+                     hipMemcpyHostToDevice));
+  JB_CHECK(hipMemset(c.jointSolve.p, 0, (size_t)std::max(nj, 1) * sizeof(int)));  // none solved since the change
+  c.jointOrderReady = true;
+}
+
+void jointsImpulses(Ctx& c, float* out) {
+  if (c.nj == 0) return;
+  jointsMakeOrder(c);
+  std::vector<float> st((size_t)kJointStateWords * c.nj);
+  JB_CHECK(hipMemcpy(st.data(), c.jointState.p, st.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int j = 0; j < c.nj; j++) out[j] = st[(size_t)kJointStateWords * j + 3];
+}
+
+int jointsOrderSolved(Ctx& c, int32_t* out, int cap) {
+  if (c.nj == 0) return 0;
+  jointsMakeOrder(c);
+  std::vector<int> order(c.nj), solve(c.nj);
+  JB_CHECK(hipMemcpy(order.data(), c.jointOrder.p, (size_t)c.nj * sizeof(int), hipMemcpyDeviceToHost));
+  JB_CHECK(hipMemcpy(solve.data(), c.jointSolve.p, (size_t)c.nj * sizeof(int), hipMemcpyDeviceToHost));
+  int n = 0;
+  for (int k = 0; k < c.nj; k++)
+    if (solve[order[k]]) {
+      if (out && n < cap) out[n] = order[k];
+      n++;
+    }
+  return n;
+}
+
+namespace {
+// the step's joint work, in the order the solve needs it
+struct JointPass {
+  Ctx& c;
+  bool on;
+  void hook(int* parent, int* changed) {  // island merges, inside the island labelling loop
+    if (on)
+      hipLaunchKernelGGL(kIslandHookJoints, dim3(gridFor(c.nj)), dim3(256), 0, c.stream, c.nj, (const float*)c.joints.p,
+                         (const float*)c.props.p, (const int*)c.act.p, parent, changed);
+  }
+  void mark(const int* island, const int* inactive) {  // which joints this solve processes
+    if (on)
+      hipLaunchKernelGGL(kJointMark, dim3(gridFor(c.nj)), dim3(256), 0, c.stream, c.nj, (const float*)c.joints.p,
+                         (const float*)c.props.p, (const int*)c.act.p, island, inactive, c.jointSolve.p);
+  }
+  void build() {
+    if (on)
+      hipLaunchKernelGGL(kJointBuild, dim3(gridFor(c.nj)), dim3(256), 0, c.stream, c.nj, (const float*)c.joints.p,
+                         (const int*)c.jointSolve.p, (const float*)c.tr.p, (const float*)c.props.p, c.jointState.p);
+  }
+  int sweep(float dt, const int* done) {  // one iteration's constraints; returns the launches
+    if (!on) return 0;
+    const int ncol = (int)c.jointColourOff.size() - 1;
+    if (c.nj <= 4 * kJointTailThreads) {
+      hipLaunchKernelGGL(kJointSweepAll, dim3(1), dim3(kJointTailThreads), 0, c.stream, ncol,
+                         (const int*)c.jointColourOffDev.p, dt, (const int*)c.jointOrder.p,
+                         (const int*)c.jointSolve.p, (const float*)c.joints.p, c.jointState.p, c.svel.p, done);
+      return 1;
+    }
+    for (int k = 0; k < ncol; k++) {
+      const int cnt = c.jointColourOff[k + 1] - c.jointColourOff[k];
+      hipLaunchKernelGGL(kJointSweep, dim3(gridFor(cnt)), dim3(256), 0, c.stream, c.jointColourOff[k], cnt, dt,
+                         (const int*)c.jointOrder.p, (const int*)c.jointSolve.p, (const float*)c.joints.p,
+                         c.jointState.p, c.svel.p, done);
+    }
+    return ncol;
+  }
+};
+}  // namespace
 
 namespace {
 // Block tables of the world-block solve: dynamic bodies grouped by world (world id, then index), whole worlds
@@ -1605,6 +1882,27 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
 }
 }  // namespace
 
+// A solve with constraints and no contact rows (solveGroup with numManifolds == 0): the constraints' iterations
+// on the solver records, then the velocities back into the bodies.  In tolerance mode the residual over the
+// (absent) contact rows is 0, so one iteration runs.
+static void solveJointsOnly(Ctx& c, float dt, JointPass& jp) {
+  hipStream_t s = c.stream;
+  c.svel.ensure((size_t)8 * c.nb);
+  tBegin(c, T_SOLVE_SETUP);
+  hipLaunchKernelGGL(kSolverBodies, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, (const float*)c.vel.p,
+                     (const float*)c.props.p, c.svel.p);
+  jp.build();
+  tEnd(c, T_SOLVE_SETUP, 2);
+  const int iters = c.tolerance > 0.0f ? 1 : c.numIterations;
+  tBegin(c, T_SOLVE_ITER);
+  int launches = 0;
+  for (int itn = 0; itn < iters; itn++) launches += jp.sweep(dt, nullptr);
+  tEnd(c, T_SOLVE_ITER, launches);
+  hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
+  JB_CHECK(hipGetLastError());
+  c.st.iterations = iters;
+}
+
 void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   hipStream_t s = c.stream;
   const int64_t np = c.npairs;
@@ -1616,7 +1914,10 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.st.iterations = 0;
   c.st.residual = 0.0f;
   c.residHist.clear();
-  if (np == 0) return;
+  // the world's constraints are solved with its steps (a caller's island list carries manifolds only)
+  JointPass jp{c, c.nj > 0 && !explicitList};
+  if (jp.on) jointsMakeOrder(c);
+  if (np == 0 && !jp.on) return;
   float* man = c.man[c.cur].p;
   // active manifolds (stable compaction keeps pair-key order)
   c.flags.ensure(np);
@@ -1639,8 +1940,10 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
     hipLaunchKernelGGL(kIslandInit, dim3(gridFor(nb)), dim3(256), 0, s, nb, c.island.p);
     for (int round = 0;; round++) {
       JB_CHECK(hipMemsetAsync(changed, 0, sizeof(int), s));
-      hipLaunchKernelGGL(kIslandHook, dim3(gridFor(np)), dim3(256), 0, s, np, (const uint64_t*)c.keys[c.cur].p,
-                         (const float*)c.props.p, c.island.p, changed);
+      if (np > 0)
+        hipLaunchKernelGGL(kIslandHook, dim3(gridFor(np)), dim3(256), 0, s, np, (const uint64_t*)c.keys[c.cur].p,
+                           (const float*)c.props.p, c.island.p, changed);
+      jp.hook(c.island.p, changed);
       hipLaunchKernelGGL(kIslandCompress, dim3(gridFor(nb)), dim3(256), 0, s, nb, c.island.p);
       int h = 0;
       JB_CHECK(hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1655,21 +1958,29 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
                        (const float*)c.props.p, (const int*)awake, c.act.p, inactive);
     island = c.island.p;
   }
-  hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p, island,
-                     island ? (const int*)(c.islandFlag.p + c.nb) : nullptr, c.flags.p);
-  size_t tb = 0;
-  hipcub::CountingInputIterator<int> it(0);
-  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));
-  c.cubTmp.ensure(tb);
-  JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
-  JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));
-  JB_CHECK(hipMemcpyAsync(&na, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  JB_CHECK(hipStreamSynchronize(s));
+  jp.mark(island, island ? (const int*)(c.islandFlag.p + c.nb) : nullptr);
+  if (np > 0) {
+    hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p,
+                       island, island ? (const int*)(c.islandFlag.p + c.nb) : nullptr, c.flags.p);
+    size_t tb = 0;
+    hipcub::CountingInputIterator<int> it(0);
+    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));
+    c.cubTmp.ensure(tb);
+    JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
+    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np,
+                                           s));
+    JB_CHECK(hipMemcpyAsync(&na, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
+  }
   }
   c.nActive = na;
   c.wbLastSolve = false;
-  if (na == 0) return;
-  if (!explicitList && c.wbBodies > 0 && !(c.tolerance > 0.0f)) {  // batched worlds: one launch per solve
+  if (na == 0) {
+    if (jp.on) solveJointsOnly(c, dt, jp);
+    return;
+  }
+  // batched worlds: one launch per solve (scenes without constraints)
+  if (!explicitList && c.wbBodies > 0 && !(c.tolerance > 0.0f) && !jp.on) {
     if (c.useMlp && !c.mlpReady) throw std::logic_error("learned warm start enabled without a model");
     if (!c.useMlp && c.useLearned && c.learnedSlots < 4 * (int64_t)na)
       throw std::length_error("learned warm start has fewer slots than 4 x active manifolds");
@@ -1882,6 +2193,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.svel.ensure((size_t)8 * c.nb);
   hipLaunchKernelGGL(kSolverBodies, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, (const float*)c.vel.p,
                      (const float*)c.props.p, c.svel.p);
+  jp.build();  // buildJacobian of the constraints after the contact rows (:757-763)
   const bool learnedHere = c.useMlp;
   SolverParams SP{dt, c.erp, c.warmFactor, c.linearSlop, c.warmstart, (c.useLearned && !c.useMlp) ? 1 : 0,
                   (c.dumpFeatures || learnedHere) ? 1 : 0, c.tolerance};
@@ -1988,9 +2300,11 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
     }
   };
   tBegin(c, T_SOLVE_ITER);
+  int jointLaunches = 0;
   for (int itn = 0; itn < iters; itn++) {
     const bool track = tolMode || itn == iters - 1;
     double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
+    jointLaunches += jp.sweep(dt, done);  // the constraints first in every iteration (:823-842)
     sweepBlocks(false, res);
     for (int k = firstColor; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
@@ -2030,7 +2344,8 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   }
   JB_CHECK(hipGetLastError());
   tEnd(c, T_SOLVE_ITER,
-       (int64_t)iters * 2 * (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)));
+       (int64_t)iters * 2 * (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)) +
+           jointLaunches);
   hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
   // statistics: copied into pinned memory now, read by solverCollectStats after the step's final wait

@@ -260,12 +260,15 @@ void bodiesToDevice(Ctx& c, const float* bodies) {
   }
   c.bodyShapeType.resize(n);
   c.bodyStaticHost.resize(n);
+  c.bodyInvMassHost.resize(n);
   for (int i = 0; i < n; i++) {
     const float* r = bodies + (size_t)JBGPU_BODY_WORDS * i;
     c.bodyShapeType[i] = c.shapeTypeHost[ldi(r + 27)];
     c.bodyStaticHost[i] = (ldi(r + 28) & JBGPU_BODY_STATIC) != 0;
+    c.bodyInvMassHost[i] = r[18];
   }
   c.wbReady = false;
+  jointsChanged(c);
   hipStream_t s = c.stream;
   upload(c.tr, tr.data(), tr.size(), s);
   upload(c.vel, vel.data(), vel.size(), s);
@@ -794,6 +797,8 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     c.ns = nshapes;
     c.nv = nverts;
     c.nb = nbodies;
+    c.nj = 0;  // a new world has no constraints
+    c.jointHost.clear();
     upload(c.shapes, shapes, (size_t)JBGPU_SHAPE_WORDS * nshapes, c.stream);
     std::vector<float> vz(3, 0.0f);
     upload(c.verts, nverts > 0 ? verts : vz.data(), nverts > 0 ? 3 * (size_t)nverts : 3, c.stream);
@@ -845,6 +850,68 @@ int jbgpu_world_step_until(jbgpu_ctx* ctx, float dt, int32_t phase) {
     requireWorld(c);
     if (phase < 0 || phase > 4) throw std::invalid_argument("phase must be 0..4");
     stepPhases(c, dt, phase);
+  });
+}
+
+int jbgpu_world_add_point2point(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
+                                const float* pivot_a, const float* pivot_b, const float* settings) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (n < 0 || (n > 0 && (!body_a || !body_b || !pivot_a || !pivot_b)))
+      throw std::invalid_argument("bad constraint arguments");
+    if ((int64_t)c.nj + n > (1 << 30)) throw std::length_error("too many constraints");
+    for (int i = 0; i < n; i++) {
+      const int a = body_a[i], b = body_b[i];
+      if (a < 0 || a >= c.nb) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A out of range");
+      if (b < -1 || b >= c.nb)
+        throw std::invalid_argument("constraint " + std::to_string(i) + ": body B out of range (-1: the fixed body)");
+      if (a == b) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A is body B");
+    }
+    if (n == 0) return;
+    const int nj = c.nj + n;
+    for (int i = 0; i < n; i++) {
+      float r[kJointWords] = {};
+      sti(r, body_a[i]);
+      sti(r + 1, body_b[i]);
+      for (int k = 0; k < 3; k++) {
+        r[2 + k] = pivot_a[3 * i + k];
+        r[5 + k] = pivot_b[3 * i + k];
+      }
+      r[8] = settings ? settings[3 * i] : 0.3f;  // ConstraintSetting defaults (Point2PointConstraint.java:202-206)
+      r[9] = settings ? settings[3 * i + 1] : 1.0f;
+      r[10] = settings ? settings[3 * i + 2] : 0.0f;
+      c.jointHost.insert(c.jointHost.end(), r, r + kJointWords);
+    }
+    c.joints.ensureKeep((size_t)kJointWords * nj);
+    c.jointState.ensureKeep((size_t)kJointStateWords * nj);
+    JB_CHECK(hipMemcpy(c.joints.p + (size_t)kJointWords * c.nj, c.jointHost.data() + (size_t)kJointWords * c.nj,
+                       (size_t)kJointWords * n * sizeof(float), hipMemcpyHostToDevice));
+    JB_CHECK(hipMemset(c.jointState.p + (size_t)kJointStateWords * c.nj, 0,
+                       (size_t)kJointStateWords * n * sizeof(float)));  // appliedImpulse starts at 0
+    c.nj = nj;
+    jointsChanged(c);
+  });
+}
+
+int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n) {
+  return guard(ctx, [&] {
+    if (n) *n = ctx->c.nj;
+  });
+}
+
+int jbgpu_world_joint_impulses(jbgpu_ctx* ctx, float* out) {
+  return guard(ctx, [&] {
+    if (ctx->c.nj > 0 && !out) throw std::invalid_argument("null output");
+    jointsImpulses(ctx->c, out);
+  });
+}
+
+int jbgpu_world_joint_order(jbgpu_ctx* ctx, int32_t* order, int32_t cap, int32_t* n) {
+  return guard(ctx, [&] {
+    if (cap < 0) throw std::invalid_argument("negative capacity");
+    const int m = jointsOrderSolved(ctx->c, order, cap);
+    if (n) *n = m;
   });
 }
 

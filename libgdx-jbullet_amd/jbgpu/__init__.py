@@ -75,6 +75,10 @@ SYMBOLS = {
     "jbgpu_world_get_pairs": (C.c_int, [_vp, _vp, _vp]),
     "jbgpu_world_set_pairs": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "jbgpu_world_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
+    "jbgpu_world_add_point2point": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]),
+    "jbgpu_world_num_joints": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "jbgpu_world_joint_impulses": (C.c_int, [_vp, _vp]),
+    "jbgpu_world_joint_order": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_check_pair_records": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp]),
     "jbgpu_np_dispatch": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp]),
     "jbgpu_solver_prepare": (C.c_int, [_vp, C.c_int32, C.c_int64]),
@@ -339,6 +343,38 @@ class Context:
         k = np.ascontiguousarray(keys, np.uint64)
         m = f32(man).reshape(-1)
         self._ck(load().jbgpu_world_set_pairs(self.h, len(k), _p(k), _p(m)), "world_set_pairs")
+
+    def add_point2point(self, body_a, body_b, pivot_a, pivot_b, settings=None):
+        """addConstraint of Point2PointConstraints; body_b -1 is the fixed body (pivot_b a world point);
+        settings rows (tau, damping, impulse clamp) or None for (0.3, 1, 0)."""
+        a = np.ascontiguousarray(body_a, np.int32).reshape(-1)
+        b = np.ascontiguousarray(body_b, np.int32).reshape(-1)
+        pa = f32(pivot_a).reshape(-1)
+        pb = f32(pivot_b).reshape(-1)
+        if not (len(b) == len(a) and pa.size == 3 * len(a) and pb.size == 3 * len(a)):
+            raise ValueError("body_a, body_b, pivot_a (n x 3), pivot_b (n x 3) disagree")
+        st = None if settings is None else f32(settings).reshape(-1)
+        if st is not None and st.size != 3 * len(a):
+            raise ValueError("settings must be n x 3")
+        self._ck(load().jbgpu_world_add_point2point(self.h, len(a), _p(a), _p(b), _p(pa), _p(pb), _p(st)),
+                 "world_add_point2point")
+
+    def num_joints(self):
+        n = C.c_int32(0)
+        self._ck(load().jbgpu_world_num_joints(self.h, C.byref(n)), "world_num_joints")
+        return n.value
+
+    def joint_impulses(self):
+        out = np.zeros(max(self.num_joints(), 1), np.float32)
+        self._ck(load().jbgpu_world_joint_impulses(self.h, _p(out)), "world_joint_impulses")
+        return out[:self.num_joints()]
+
+    def joint_order(self):
+        n = C.c_int32(0)
+        self._ck(load().jbgpu_world_joint_order(self.h, None, 0, C.byref(n)), "world_joint_order")
+        out = np.zeros(max(n.value, 1), np.int32)
+        self._ck(load().jbgpu_world_joint_order(self.h, _p(out), len(out), C.byref(n)), "world_joint_order")
+        return out[:n.value]
 
     def stats(self):
         s = Stats()

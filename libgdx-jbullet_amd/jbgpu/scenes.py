@@ -21,6 +21,8 @@ Scenes (sizes as in the benchmark configurations):
                         boxes 4:3, towers of height 4-20 on a 3 m grid, own floor each.
   * ``pyramids``        boxes (half-extents U(0.2, 1)) in pyramids and walls plus
                         falling spheres (learned warm-start workload).
+  * ``chains``          hanging chains of boxes joined by Point2Point constraints, with
+                        boxes dropped onto them.
 """
 from dataclasses import dataclass, field
 
@@ -332,6 +334,43 @@ def sleep_and_wake(n=3, layers=2, drop_height=30.0):
     sphere = b.shape(SPHERE, [0.4])
     b.body(sphere, IDENT, np.array([(n - 1) * 0.75, drop_height, (n - 1) * 0.75], np.float32), 1.0)
     return b.done(dt=1 / 60, iterations=10, steps=400)
+
+
+def chains(seed=1, nx=8, nz=8, links=8, spacing=0.9, drops=200):
+    """Chains of boxes joined by Point2Point constraints: nx x nz chains of `links` links (half extents
+    0.15 x 0.2 x 0.15, 0.5 m apart), each hung from a fixed world point; every other chain reaches the floor, the
+    rest hang free; `drops` boxes (half-extent 0.2-0.4) fall onto them from 2-8 m above.  The links start with
+    random horizontal velocities (meta["velocity"]), so chains swing into their neighbours.  meta["joints"] =
+    (body_a, body_b, pivot_a, pivot_b), body_b -1 for the fixed body."""
+    rng = np.random.default_rng(seed)
+    b = _Builder("chains")
+    plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
+    b.body(plane, IDENT, np.zeros(3, np.float32), 0.0)
+    link = b.shape(BOX, [0.15, 0.2, 0.15])
+    ja, jb, pa, pb = [], [], [], []
+    vel = [np.zeros(3, np.float32)]
+    for i in range(nx):
+        for k in range(nz):
+            x, z = (i - (nx - 1) / 2.0) * spacing, (k - (nz - 1) / 2.0) * spacing
+            top = links * 0.5 + (0.0 if (i + k) % 2 == 0 else 0.6)  # the even chains' last link touches the floor
+            for m in range(links):
+                idx = len(b.bs)
+                b.body(link, IDENT, np.array([x, top - 0.25 - 0.5 * m, z], np.float32), 1.0)
+                vel.append(np.array([rng.uniform(-1, 1), 0.0, rng.uniform(-1, 1)], np.float32))
+                if m == 0:
+                    ja.append(idx), jb.append(-1), pa.append([0, 0.25, 0]), pb.append([x, top, z])
+                else:
+                    ja.append(idx - 1), jb.append(idx), pa.append([0, -0.25, 0]), pb.append([0, 0.25, 0])
+    ext = (nx * spacing / 2.0, nz * spacing / 2.0)
+    for _ in range(drops):
+        h = rng.uniform(0.2, 0.4, 3).astype(np.float32)
+        bx = b.shape(BOX, list(h))
+        pos = np.array([rng.uniform(-ext[0], ext[0]), links * 0.5 + rng.uniform(2.0, 8.0),
+                        rng.uniform(-ext[1], ext[1])], np.float32)
+        b.body(bx, _yaw_basis(rng.uniform(0, 2 * np.pi, 1))[0], pos, np.float32(8.0 * h.prod() * 200.0))
+        vel.append(np.zeros(3, np.float32))
+    joints = (np.array(ja, np.int32), np.array(jb, np.int32), np.array(pa, np.float32), np.array(pb, np.float32))
+    return b.done(dt=1 / 60, iterations=10, steps=300, joints=joints, velocity=np.array(vel, np.float32))
 
 
 def build_records(scene, builder, gravity=GRAVITY):
