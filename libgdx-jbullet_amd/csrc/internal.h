@@ -307,6 +307,8 @@ struct Ctx {
   int headColors = 0;
   DBuf<double> resid;  // per iteration: dd, nn
   DBuf<int> done;
+  HBuf<int> doneHost;   // tolerance mode: the done flag read back every few iterations
+  int tolCheckEvery = 4;  // iterations between those reads (0: never; every iteration's kernels are launched)
   DBuf<float> learned;  // per-contact learned warm-start impulses (npairs*4*3)
   bool useLearned = false;
   int64_t learnedSlots = 0;

@@ -2301,6 +2301,8 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   };
   tBegin(c, T_SOLVE_ITER);
   int jointLaunches = 0;
+  int launchedIters = iters;
+  if (tolMode) c.doneHost.ensure(1);
   for (int itn = 0; itn < iters; itn++) {
     const bool track = tolMode || itn == iters - 1;
     double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
@@ -2341,10 +2343,21 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
                          (const float*)c.props.p, c.svel.p, res, (const int*)done);
     if (track)
       hipLaunchKernelGGL(kResidual, dim3(1), dim3(64), 0, s, itn, c.tolerance, res, c.resid.p, c.done.p, resOut);
+    // tolerance mode: every few iterations the host reads the done flag and stops launching once the residual
+    // is reached (the kernels of later iterations would only read the flag and exit)
+    if (tolMode && c.tolCheckEvery > 0 && (itn + 1) % c.tolCheckEvery == 0 && itn + 1 < iters) {
+      JB_CHECK(hipMemcpyAsync(c.doneHost.p, c.done.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      JB_CHECK(hipStreamSynchronize(s));
+      if (*c.doneHost.p) {
+        launchedIters = itn + 1;
+        break;
+      }
+    }
   }
   JB_CHECK(hipGetLastError());
   tEnd(c, T_SOLVE_ITER,
-       (int64_t)iters * 2 * (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)) +
+       (int64_t)launchedIters * 2 *
+               (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)) +
            jointLaunches);
   hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);

@@ -69,17 +69,21 @@ def _step_and_compare(ctx, orc, tag):
     return jorder
 
 
-@pytest.mark.parametrize("mode", ["colours", "blocks", "world_blocks"])
+@pytest.mark.parametrize("mode", ["colours", "blocks", "world_blocks", "tolerance"])
 def test_chains_step_parity(mode):
     """64 chains of 8 links (every other one on the floor), 200 boxes dropped onto them; 150 steps.  'blocks':
     block-partitioned contact sweeps; 'world_blocks': the one-launch batched-worlds solve is asked for, and the
-    step falls back to the colour sweeps because the world has constraints."""
+    step falls back to the colour sweeps because the world has constraints; 'tolerance': iterate to a residual
+    of 1e-3 (at most 40 iterations), the constraints stopping with the contact rows."""
     sc = scenes.chains(seed=3)
     ctx, orc = _pair(sc)
     if mode == "blocks":
         ctx.solver_set_blocks(128)
     elif mode == "world_blocks":
         ctx.solver_set_world_blocks(256)
+    elif mode == "tolerance":
+        ctx.solver_set_params(tolerance=1e-3, max_iterations=40, num_iterations=10)
+        orc.set_solver(jbgpu.ORDER_EXPLICIT, 1e-3, 40, 10)
     nj = ctx.num_joints()
     assert nj == 512
     for step in range(150):

@@ -166,6 +166,25 @@ def test_world_boxes_grid_step_parity(blocks, tail):
     ctx.close()
 
 
+def test_tolerance_mode_step_parity():
+    """Tolerance mode (iterate until ||dlambda|| / max(1, ||lambda||) <= 1e-3, at most 60 iterations): the host
+    stops launching sweeps once the device's residual check has passed; the iteration count and the state are
+    those of the oracle run with the same tolerance in the device's order."""
+    sc = scenes.boxes_grid(seed=2)
+    ctx, orc = _world_pair(sc)
+    ctx.solver_set_params(tolerance=1e-3, max_iterations=60, num_iterations=10)
+    orc.set_solver(jbgpu.ORDER_EXPLICIT, 1e-3, 60, 10)
+    early = 0
+    for step in range(120):
+        _sync_and_step(ctx, orc)
+        _compare_world(ctx, orc, f"tolerance step {step}")
+        it_d, it_o = ctx.stats()["iterations"], orc.stats()["iterations"]
+        assert it_d == it_o, (step, it_d, it_o)
+        early += 0 < it_d < 60
+    assert early > 0  # some solves converged before the cap
+    ctx.close()
+
+
 @pytest.mark.parametrize("degree_first,recolour", [(1, 0), (0, 0), (1, 2)])
 def test_world_mixed_shapes_step_parity(degree_first, recolour):
     """Mixed pile (all shape pairs, EPA), under each colouring: largest-degree-first or hashed Jones-Plassmann
