@@ -420,6 +420,53 @@ void orc_world_add_p2p(void* h, int n, const int32_t* bodyA, const int32_t* body
     w->joints.push_back(j);
   }
 }
+// HingeConstraints from the solver frames (rbAFrame, rbBFrame as the constructors leave them, 12 floats each:
+// basis row-major, origin); bodyB -1 for the fixed body; params per hinge (lowerLimit, upperLimit, limitSoftness,
+// biasFactor, relaxationFactor, angularOnly, enableAngularMotor, motorTargetVelocity, maxMotorImpulse) or null
+void orc_world_add_hinge(void* h, int n, const int32_t* bodyA, const int32_t* bodyB, const float* frameA,
+                         const float* frameB, const float* params) {
+  World* w = (World*)h;
+  for (int i = 0; i < n; i++) {
+    Joint j{};
+    j.type = 1;
+    j.bodyA = bodyA[i];
+    j.bodyB = bodyB[i];
+    j.frameA = trFrom(frameA + 12 * i);
+    j.frameB = trFrom(frameB + 12 * i);
+    if (params) {
+      const float* p = params + 9 * i;
+      j.lowerLimit = p[0];
+      j.upperLimit = p[1];
+      j.limitSoftness = p[2];
+      j.biasFactor = p[3];
+      j.relaxationFactor = p[4];
+      j.angularOnly = p[5] != 0.0f;
+      j.enableMotor = p[6] != 0.0f;
+      j.motorTarget = p[7];
+      j.maxMotorImpulse = p[8];
+    }
+    w->joints.push_back(j);
+  }
+}
+// the constructors' frames (HingeConstraint.java:84-238) from body A's transform trA (12 floats):
+// fromFrames = 0: pivot/axis constructors (fixedB: (rbA, pivotInA, axisInA)); 1: frame constructors
+void orc_hinge_frames(int fromFrames, int fixedB, const float* trA, const float* pivotA, const float* pivotB,
+                      const float* axisA, const float* axisB, const float* inA, const float* inB, float* outA,
+                      float* outB) {
+  Tr fa, fb;
+  if (fromFrames)
+    hingeFramesFromFrames(trFrom(trA), trFrom(inA), fixedB ? trFrom(inA) : trFrom(inB), fixedB != 0, fa, fb);
+  else
+    hingeFramesFromAxes(trFrom(trA), ld3(pivotA), fixedB ? v3(0, 0, 0) : ld3(pivotB), ld3(axisA),
+                        fixedB ? v3(0, 0, 0) : ld3(axisB), fixedB != 0, fa, fb);
+  trTo(fa, outA);
+  trTo(fb, outB);
+}
+float orc_atan2_fast(float y, float x) { return atan2Fast(y, x); }
+float orc_world_hinge_angle(void* h, int j) {
+  World* w = (World*)h;
+  return hingeAngle(w->joints[j], w->bodies);
+}
 void orc_world_set_joint_order(void* h, int n, const int32_t* order) {
   ((World*)h)->explicitJointOrder.assign(order, order + n);
 }

@@ -185,13 +185,33 @@ struct ImpulseInit { // learned warm start: override of the warm-start impulses 
 // Point2PointConstraint (dynamics/constraintsolver/Point2PointConstraint.java): a ball socket between pivotA in
 // bodyA's frame and pivotB in bodyB's frame; bodyB == -1 is TypedConstraint's fixed body (static, identity
 // transform, zero velocity), pivotB then a world point.  adiag / applied are the per-solve state of buildJacobian.
+// HingeConstraint (dynamics/constraintsolver/HingeConstraint.java): frameA / frameB are rbAFrame / rbBFrame as
+// the constructors leave them (B's third column already flipped); the linear part is three JacobianEntry rows
+// along linAxis (buildJacobian), the angular part a torque on the orthogonal relative angular velocity and the
+// axis misalignment, plus an optional limit and motor about the hinge axis.
 struct Joint {
+  int type = 0;  // 0 Point2PointConstraint, 1 HingeConstraint
   int bodyA, bodyB;
   V3 pivotA, pivotB;
   float tau = 0.3f, damping = 1.0f, impulseClamp = 0.0f;  // ConstraintSetting (:202-206)
   float adiag[3];
   float applied;
+  Tr frameA, frameB;
+  float lowerLimit = 1e30f, upperLimit = -1e30f, limitSoftness = 0.9f, biasFactor = 0.3f, relaxationFactor = 1.0f;
+  bool angularOnly = false, enableMotor = false;
+  float motorTarget = 0.0f, maxMotorImpulse = 0.0f;
+  V3 linAxis[3];
+  float kHinge = 0.0f, correction = 0.0f, limitSign = 0.0f, accLimitImpulse = 0.0f;
+  bool solveLimit = false;
 };
+// The HingeConstraint constructors' frames from the body A transform at construction:
+// (rbA, rbB, pivotInA, pivotInB, axisInA, axisInB) (HingeConstraint.java:84-130), and with fixedB
+// (rbA, pivotInA, axisInA) (:132-178, pivotB / axisB unused).
+void hingeFramesFromAxes(const Tr& trA, V3 pivotA, V3 pivotB, V3 axisA, V3 axisB, bool fixedB, Tr& frameA, Tr& frameB);
+// (rbA, rbB, rbAFrame, rbBFrame) (:180-210) and, with fixedB, (rbA, rbAFrame) (:212-238).
+void hingeFramesFromFrames(const Tr& trA, const Tr& inA, const Tr& inB, bool fixedB, Tr& frameA, Tr& frameB);
+float hingeAngle(const Joint& j, const std::vector<Body>& bodies);  // getHingeAngle (:530-549)
+float atan2Fast(float y, float x);                                   // ScalarUtil.atan2Fast (ScalarUtil.java:43-57)
 // The row-order shuffle of one pool: j = 0..n-1 swaps order[j] with order[Random.nextInt(j + 1)]
 // (SequentialImpulseConstraintSolver.java:805-821, randInt2 :138-149).
 void shuffleOrder(std::vector<int>& order, JavaRandom& rng);

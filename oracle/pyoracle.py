@@ -97,6 +97,10 @@ def lib():
             "orc_shuffle": (None, [C.c_int64, C.c_int, C.c_int, _i32]),
             "orc_world_phase": (C.c_int, [vp, C.c_float, C.c_int]),
             "orc_world_add_p2p": (None, [vp, C.c_int, _i32, _i32, _f, _f, vp]),
+            "orc_world_add_hinge": (None, [vp, C.c_int, _i32, _i32, _f, _f, vp]),
+            "orc_hinge_frames": (None, [C.c_int, C.c_int, _f, _f, _f, _f, _f, _f, _f, _f, _f]),
+            "orc_atan2_fast": (C.c_float, [C.c_float, C.c_float]),
+            "orc_world_hinge_angle": (C.c_float, [vp, C.c_int]),
             "orc_world_set_joint_order": (None, [vp, C.c_int, _i32]),
             "orc_world_joint_impulses": (None, [vp, _f]),
             "orc_solve_group": (None, [C.c_int, _f, C.c_int, _f, C.POINTER(SolverInfo), _f]),
@@ -264,6 +268,32 @@ class Broadphase:
 
 
 # ----------------------------------------------------------------- world
+def hinge_frames(tr_a, pivot_a, axis_a, pivot_b=None, axis_b=None):
+    """rbAFrame / rbBFrame of HingeConstraint(rbA, rbB, pivotInA, pivotInB, axisInA, axisInB), or of
+    HingeConstraint(rbA, pivotInA, axisInA) when pivot_b is None (B the fixed body); tr_a = body A's 12 words."""
+    fa, fb = np.zeros(12, np.float32), np.zeros(12, np.float32)
+    fixed = pivot_b is None
+    z = np.zeros(3, np.float32)
+    lib().orc_hinge_frames(0, int(fixed), f32(tr_a), f32(pivot_a), z if fixed else f32(pivot_b), f32(axis_a),
+                           z if fixed else f32(axis_b), np.zeros(12, np.float32), np.zeros(12, np.float32), fa, fb)
+    return fa, fb
+
+
+def hinge_frames_from_frames(tr_a, frame_a, frame_b=None):
+    """The frame constructors: HingeConstraint(rbA, rbB, rbAFrame, rbBFrame), or (rbA, rbAFrame) when frame_b is
+    None."""
+    fa, fb = np.zeros(12, np.float32), np.zeros(12, np.float32)
+    fixed = frame_b is None
+    z = np.zeros(3, np.float32)
+    lib().orc_hinge_frames(1, int(fixed), f32(tr_a), z, z, z, z, f32(frame_a),
+                           f32(frame_a) if fixed else f32(frame_b), fa, fb)
+    return fa, fb
+
+
+def atan2_fast(y, x):
+    return float(lib().orc_atan2_fast(np.float32(y), np.float32(x)))
+
+
 class World:
     def __init__(self, shapes, verts, bodies, bp_mode=0, iterations=10, order=0, deactivation_disabled=True,
                  wmin=(-1000, -1000, -1000), wmax=(1000, 1000, 1000), tolerance=0.0, max_iterations=100,
@@ -302,6 +332,19 @@ class World:
         lib().orc_world_add_p2p(self.h, len(a), a, b, f32(pivot_a).reshape(-1), f32(pivot_b).reshape(-1),
                                 _ptr(self._settings))
         self.nj = getattr(self, "nj", 0) + len(a)
+
+    def add_hinge(self, body_a, body_b, frame_a, frame_b, params=None):
+        """HingeConstraints from the solver frames (n x 12: basis row-major, origin); params rows
+        (lower, upper, softness, bias, relaxation, angular_only, motor_on, motor_target, max_motor) or None."""
+        a = np.ascontiguousarray(body_a, np.int32)
+        b = np.ascontiguousarray(body_b, np.int32)
+        fa = f32(frame_a).reshape(-1)
+        fb = f32(frame_b).reshape(-1)
+        self._hinge_params = None if params is None else f32(params).reshape(-1)
+        lib().orc_world_add_hinge(self.h, len(a), a, b, fa, fb, _ptr(self._hinge_params))
+
+    def hinge_angle(self, j):
+        return float(lib().orc_world_hinge_angle(self.h, int(j)))
 
     def set_joint_order(self, order):
         o = np.ascontiguousarray(order, np.int32)

@@ -76,7 +76,14 @@ Body g_fixed = [] {  // TypedConstraint.getFixed(): new RigidBody(0, null, null)
 }();
 }  // namespace
 
+static void hingeBuildJacobian(Joint& j, const std::vector<Body>& bodies);
+static void hingeSolve(Joint& j, std::vector<Body>& bodies, float timeStep);
+
 void jointBuildJacobian(Joint& j, const std::vector<Body>& bodies) {  // Point2PointConstraint.java:71-107
+  if (j.type == 1) {
+    hingeBuildJacobian(j, bodies);
+    return;
+  }
   j.applied = 0.0f;
   const Body& A = bodies[j.bodyA];
   const Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : g_fixed;
@@ -105,6 +112,10 @@ static void applyImpulseRB(Body& b, V3 impulse, V3 relPos) {
 }
 
 void jointSolve(Joint& j, std::vector<Body>& bodies, float timeStep) {  // Point2PointConstraint.java:109-177
+  if (j.type == 1) {
+    hingeSolve(j, bodies, timeStep);
+    return;
+  }
   Body fixed = g_fixed;
   Body& A = bodies[j.bodyA];
   Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : fixed;
@@ -127,6 +138,226 @@ void jointSolve(Joint& j, std::vector<Body>& bodies, float timeStep) {  // Point
     const V3 iv = scl(normal, impulse);
     applyImpulseRB(A, iv, sub(pivotAInW, A.tr.o));
     applyImpulseRB(B, scl(iv, -1.0f), sub(pivotBInW, B.tr.o));
+  }
+}
+
+// ---- HingeConstraint -----------------------------------------------------------------------------------
+static V3 column(const M3& m, int c) { return v3(m.m[c], m.m[3 + c], m.m[6 + c]); }  // MatrixUtil.getColumn
+static void setRow(M3& m, int r, float x, float y, float z) {                        // MatrixUtil.setRow
+  m.m[3 * r] = x;
+  m.m[3 * r + 1] = y;
+  m.m[3 * r + 2] = z;
+}
+static Q shortestArcQuat(V3 v0, V3 v1) {  // QuaternionUtil.java:49-68
+  const V3 c = crs(v0, v1);
+  const float d = dot(v0, v1);
+  if ((double)d < -1.0 + (double)1.19209290e-07f) return q4(0.0f, 1.0f, 0.0f, 0.0f);  // a double comparison
+  const float s = (float)std::sqrt((double)((1.0f + d) * 2.0f));
+  const float rs = 1.0f / s;
+  return q4(c.x * rs, c.y * rs, c.z * rs, s * 0.5f);
+}
+static V3 quatRotate(Q r, V3 v) {  // QuaternionUtil.java:70-90: (r * v) * conj(r)
+  Q q = q4(r.w * v.x + r.y * v.z - r.z * v.y, r.w * v.y + r.z * v.x - r.x * v.z, r.w * v.z + r.x * v.y - r.y * v.x,
+           -r.x * v.x - r.y * v.y - r.z * v.z);
+  q = qmul(q, q4(-r.x, -r.y, -r.z, r.w));
+  return v3(q.x, q.y, q.z);
+}
+
+void hingeFramesFromAxes(const Tr& trA, V3 pivotA, V3 pivotB, V3 axisA, V3 axisB, bool fixedB, Tr& frameA,
+                         Tr& frameB) {
+  const float eps = 1.19209290e-07f;
+  V3 a1 = column(trA.b, 0), a2;
+  if (!fixedB) {  // HingeConstraint.java:84-130
+    const float projection = dot(axisA, a1);
+    if (projection >= 1.0f - eps) {
+      a1 = scl(column(trA.b, 2), -1.0f);
+      a2 = column(trA.b, 1);
+    } else if (projection <= -1.0f + eps) {
+      a1 = column(trA.b, 2);
+      a2 = column(trA.b, 1);
+    } else {
+      a2 = crs(axisA, a1);
+      a1 = crs(a2, axisA);
+    }
+  } else {  // :132-178, B is the fixed body: the axis in B is -axisA in world space
+    const float projection = dot(a1, axisA);
+    if (projection > eps) {
+      a1 = scl(a1, projection);
+      a1 = sub(a1, axisA);
+    } else {
+      a1 = column(trA.b, 1);
+    }
+    a2 = crs(axisA, a1);
+    axisB = mulMV(trA.b, scl(axisA, -1.0f));  // Vector3.mul(Matrix3)
+  }
+  frameA.o = pivotA;
+  setRow(frameA.b, 0, a1.x, a2.x, axisA.x);
+  setRow(frameA.b, 1, a1.y, a2.y, axisA.y);
+  setRow(frameA.b, 2, a1.z, a2.z, axisA.z);
+  const Q arc = shortestArcQuat(axisA, axisB);
+  const V3 b1 = quatRotate(arc, a1);
+  const V3 b2 = crs(axisB, b1);
+  if (!fixedB) {
+    frameB.o = pivotB;
+    setRow(frameB.b, 0, b1.x, b2.x, -axisB.x);
+    setRow(frameB.b, 1, b1.y, b2.y, -axisB.y);
+    setRow(frameB.b, 2, b1.z, b2.z, -axisB.z);
+  } else {
+    frameB.o = xform(trA, pivotA);
+    setRow(frameB.b, 0, b1.x, b2.x, axisB.x);
+    setRow(frameB.b, 1, b1.y, b2.y, axisB.y);
+    setRow(frameB.b, 2, b1.z, b2.z, axisB.z);
+  }
+}
+
+void hingeFramesFromFrames(const Tr& trA, const Tr& inA, const Tr& inB, bool fixedB, Tr& frameA, Tr& frameB) {
+  frameA = inA;
+  frameB = fixedB ? inA : inB;  // (rbA, rbAFrame): B's frame starts as A's (:212-238)
+  frameB.b.m[2] *= -1.0f;       // flip axis: M02, M12, M22
+  frameB.b.m[5] *= -1.0f;
+  frameB.b.m[8] *= -1.0f;
+  if (fixedB) frameB.o = xform(trA, inA.o);
+}
+
+float atan2Fast(float y, float x) {  // ScalarUtil.java:43-57
+  const float simdPi = 6.283185307179586232f * 0.5f;
+  const float coeff_1 = simdPi / 4.0f;
+  const float coeff_2 = 3.0f * coeff_1;
+  const float abs_y = jabs(y);
+  float angle;
+  if (x >= 0.0f) {
+    const float r = (x - abs_y) / (x + abs_y);
+    angle = coeff_1 - coeff_1 * r;
+  } else {
+    const float r = (x + abs_y) / (abs_y - x);
+    angle = coeff_2 - coeff_1 * r;
+  }
+  return (y < 0.0f) ? -angle : angle;
+}
+
+float hingeAngle(const Joint& j, const std::vector<Body>& bodies) {  // HingeConstraint.java:530-549
+  const Body& A = bodies[j.bodyA];
+  const Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : g_fixed;
+  const V3 ref0 = mulMV(A.tr.b, column(j.frameA.b, 0));
+  const V3 ref1 = mulMV(A.tr.b, column(j.frameA.b, 1));
+  const V3 swing = mulMV(B.tr.b, column(j.frameB.b, 1));
+  return atan2Fast(dot(swing, ref0), dot(swing, ref1));
+}
+
+// RigidBody.computeAngularImpulseDenominator (RigidBody.java:534-541): axis . (I_world^T axis)
+static float angularDenominator(const Body& b, V3 axis) { return dot(axis, mulMtV(b.invInertiaWorld, axis)); }
+// RigidBody.applyTorqueImpulse (RigidBody.java:405-411): no inverse-mass test
+static void applyTorqueImpulse(Body& b, V3 torque) { b.av = add(b.av, mulMV(b.invInertiaWorld, torque)); }
+
+static void hingeBuildJacobian(Joint& j, const std::vector<Body>& bodies) {  // HingeConstraint.java:240-333
+  const Body& A = bodies[j.bodyA];
+  const Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : g_fixed;
+  j.applied = 0.0f;
+  if (!j.angularOnly) {
+    const V3 pivotAInW = xform(A.tr, j.frameA.o), pivotBInW = xform(B.tr, j.frameB.o);
+    const V3 relPos = sub(pivotBInW, pivotAInW);
+    if (len2(relPos) > 1.19209290e-07f) j.linAxis[0] = nor(relPos);
+    else j.linAxis[0] = v3(1.0f, 0.0f, 0.0f);
+    planeSpace1(j.linAxis[0], j.linAxis[1], j.linAxis[2]);
+    for (int i = 0; i < 3; i++) {  // JacobianEntry.init (JacobianEntry.java:56-73)
+      const M3 world2A = transpose(A.tr.b), world2B = transpose(B.tr.b);
+      const V3 rel1 = sub(pivotAInW, A.tr.o), rel2 = sub(pivotBInW, B.tr.o);
+      const V3 n = j.linAxis[i];
+      const V3 aJ = mulMV(world2A, crs(rel1, n));
+      const V3 bJ = mulMV(world2B, crs(rel2, scl(n, -1.0f)));
+      const V3 m0 = mulc(A.invInertiaLocal, aJ), m1 = mulc(B.invInertiaLocal, bJ);
+      j.adiag[i] = A.invMass + dot(m0, aJ) + B.invMass + dot(m1, bJ);
+    }
+  }
+  // the angular JacobianEntries (jacAng) are built by the reference but never read by solveConstraint
+  const float angle = hingeAngle(j, bodies);
+  j.correction = 0.0f;
+  j.limitSign = 0.0f;
+  j.solveLimit = false;
+  j.accLimitImpulse = 0.0f;
+  if (j.lowerLimit < j.upperLimit) {
+    if (angle <= j.lowerLimit * j.limitSoftness) {
+      j.correction = j.lowerLimit - angle;
+      j.limitSign = 1.0f;
+      j.solveLimit = true;
+    } else if (angle >= j.upperLimit * j.limitSoftness) {
+      j.correction = j.upperLimit - angle;
+      j.limitSign = -1.0f;
+      j.solveLimit = true;
+    }
+  }
+  const V3 axisA = mulMV(A.tr.b, column(j.frameA.b, 2));
+  j.kHinge = 1.0f / (angularDenominator(A, axisA) + angularDenominator(B, axisA));
+}
+
+static void hingeSolve(Joint& j, std::vector<Body>& bodies, float timeStep) {  // HingeConstraint.java:335-520
+  Body fixed = g_fixed;
+  Body& A = bodies[j.bodyA];
+  Body& B = j.bodyB >= 0 ? bodies[j.bodyB] : fixed;
+  const V3 pivotAInW = xform(A.tr, j.frameA.o), pivotBInW = xform(B.tr, j.frameB.o);
+  const float tau = 0.3f;
+  if (!j.angularOnly) {  // linear part: the relative velocity is taken once, before the three rows
+    const V3 rel1 = sub(pivotAInW, A.tr.o), rel2 = sub(pivotBInW, B.tr.o);
+    const V3 vel = sub(add(crs(A.av, rel1), A.lv), add(crs(B.av, rel2), B.lv));
+    for (int i = 0; i < 3; i++) {
+      const V3 n = j.linAxis[i];
+      const float jacDiagABInv = 1.0f / j.adiag[i];
+      const float rel_vel = dot(n, vel);
+      const float depth = -dot(sub(pivotAInW, pivotBInW), n);
+      const float impulse = depth * tau / timeStep * jacDiagABInv - rel_vel * jacDiagABInv;
+      j.applied += impulse;
+      const V3 iv = scl(n, impulse);
+      applyImpulseRB(A, iv, sub(pivotAInW, A.tr.o));
+      applyImpulseRB(B, scl(iv, -1.0f), sub(pivotBInW, B.tr.o));
+    }
+  }
+  // angular part
+  const V3 axisA = mulMV(A.tr.b, column(j.frameA.b, 2));
+  const V3 axisB = mulMV(B.tr.b, column(j.frameB.b, 2));
+  const V3 angVelA = A.av, angVelB = B.av;
+  const V3 aroundA = scl(axisA, dot(axisA, angVelA));
+  const V3 aroundB = scl(axisB, dot(axisB, angVelB));
+  V3 velrelOrthog = sub(sub(angVelA, aroundA), sub(angVelB, aroundB));
+  {
+    const float relaxation = 1.0f;
+    if (len(velrelOrthog) > 0.00001f) {
+      const V3 n = nor(velrelOrthog);
+      const float denom = angularDenominator(A, n) + angularDenominator(B, n);
+      velrelOrthog = scl(velrelOrthog, (1.0f / denom) * j.relaxationFactor);
+    }
+    V3 angularError = crs(axisA, axisB);
+    angularError = scl(angularError, -1.0f);
+    angularError = scl(angularError, 1.0f / timeStep);
+    if (len(angularError) > 0.00001f) {
+      const V3 n2 = nor(angularError);
+      const float denom2 = angularDenominator(A, n2) + angularDenominator(B, n2);
+      angularError = scl(angularError, (1.0f / denom2) * relaxation);
+    }
+    applyTorqueImpulse(A, add(scl(velrelOrthog, -1.0f), angularError));
+    applyTorqueImpulse(B, sub(velrelOrthog, angularError));
+    if (j.solveLimit) {  // on the angular velocities read before this constraint's torques
+      const float amplitude =
+          (dot(sub(angVelB, angVelA), axisA) * j.relaxationFactor + j.correction * (1.0f / timeStep) * j.biasFactor) *
+          j.limitSign;
+      float impulseMag = amplitude * j.kHinge;
+      const float temp = j.accLimitImpulse;
+      j.accLimitImpulse = jmax(j.accLimitImpulse + impulseMag, 0.0f);
+      impulseMag = j.accLimitImpulse - temp;
+      const V3 impulse = scl(axisA, impulseMag * j.limitSign);
+      applyTorqueImpulse(A, impulse);
+      applyTorqueImpulse(B, scl(impulse, -1.0f));
+    }
+  }
+  if (j.enableMotor) {
+    const V3 angularLimit = v3(0.0f, 0.0f, 0.0f);
+    const float projRelVel = dot(sub(aroundA, aroundB), axisA);
+    const float motor_relvel = j.motorTarget - projRelVel;
+    const float unclipped = j.kHinge * motor_relvel;
+    float clipped = unclipped > j.maxMotorImpulse ? j.maxMotorImpulse : unclipped;
+    clipped = clipped < -j.maxMotorImpulse ? -j.maxMotorImpulse : clipped;
+    const V3 motorImp = scl(axisA, clipped);
+    applyTorqueImpulse(A, add(motorImp, angularLimit));
+    applyTorqueImpulse(B, sub(scl(motorImp, -1.0f), angularLimit));
   }
 }
 

@@ -1103,3 +1103,180 @@ def test_point2point_pair_on_the_ground_balances_momentum():
     a, b_ = 20, 299
     want = 3 * 9.81 / 60 * (b_ - a) + p[b_] - p[a]
     assert abs(sum(totals[a + 1:b_ + 1]) / want - 1) < 1e-5, (sum(totals[a + 1:b_ + 1]), want)
+
+
+# ------------------------------------------------------------------ HingeConstraint
+def test_atan2_fast_error_bound_and_exact_octants():
+    """ScalarUtil.atan2Fast is the first-order octant approximation: exact at the eight multiples of 45 degrees and
+    within 0.072 rad of atan2 everywhere (the approximation's known worst case is ~0.0711 rad, near 22.5 degrees
+    from an axis); the sign follows y and the quadrant x."""
+    q = np.float32(np.pi / 4)
+    for (y, x), want in {(1, 1): q, (1, 0): 2 * q, (1, -1): 3 * q, (0, -1): 4 * q, (-1, -1): -3 * q, (-1, 0): -2 * q,
+                         (-1, 1): -q, (0, 1): 0.0}.items():
+        assert abs(po.atan2_fast(y, x) - want) <= 1e-6, (y, x, po.atan2_fast(y, x), want)
+    rng = np.random.default_rng(3)
+    th = rng.uniform(-np.pi, np.pi, 2000)
+    got = np.array([po.atan2_fast(np.sin(t), np.cos(t)) for t in th])
+    err = np.abs(np.angle(np.exp(1j * (got - th))))
+    assert err.max() < 0.072 and err.max() > 0.06, err.max()
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def test_hinge_frames_of_the_constructors():
+    """The constructors' frames (HingeConstraint.java:84-238) for a body turned 30 degrees about y: A's frame has
+    the hinge axis (in A) as its third column and the pivot as its origin; its first two columns are the world x
+    axis of body A made orthogonal to the axis by two cross products and, as in the reference, not renormalised:
+    both have length s = sqrt(1 - (axis . x_A)^2), so the basis is diag(s^2, s^2, 1)-orthogonal with determinant
+    s^2.  B's third column is -axisInB (flipped), and for aligned axes the hinge angle at construction is 0.  The
+    fixed-body forms put B's frame in world space: origin = A's transform of the pivot, third column =
+    -(A's basis * axis)."""
+    R = _rot([0, 1, 0], np.pi / 6).astype(np.float32)
+    tr = np.concatenate([R.reshape(-1), [1.0, 2.0, 3.0]]).astype(np.float32)
+    axis = np.array([0.0, 0.6, 0.8], np.float32)
+    fa, fb = po.hinge_frames(tr, [0.5, 0, 0], axis, [-0.5, 0, 0], axis)
+    Ba, Bb = fa[:9].reshape(3, 3).astype(np.float64), fb[:9].reshape(3, 3).astype(np.float64)
+    s2 = 1.0 - float(axis @ R[:, 0]) ** 2
+    assert np.allclose(Ba.T @ Ba, np.diag([s2, s2, 1.0]), atol=1e-6) and abs(np.linalg.det(Ba) - s2) < 1e-6
+    assert np.allclose(Ba[:, 2], axis, atol=1e-7) and np.allclose(fa[9:], [0.5, 0, 0])
+    assert np.allclose(Bb[:, 2], -axis, atol=1e-7) and np.allclose(fb[9:], [-0.5, 0, 0])
+    # B's first two columns are A's carried along the (identity) shortest arc, the second as axis x first
+    assert np.allclose(Bb[:, 0], Ba[:, 0], atol=1e-6) and np.allclose(Bb[:, 1], np.cross(axis, Ba[:, 0]), atol=1e-6)
+    fa2, fb2 = po.hinge_frames(tr, [0.0, 0.5, 0.0], axis)
+    assert np.allclose(fb2[9:], R @ np.float32([0, 0.5, 0]) + tr[9:], atol=1e-6)
+    assert np.allclose(fb2[:9].reshape(3, 3)[:, 2], -(R @ axis), atol=1e-6)
+    # frame constructors: B's third column flipped; fixed body: B = A's frame, origin moved to world space
+    F = np.concatenate([_rot([1, 2, 3], 0.7).reshape(-1), [0, 0.5, 0]]).astype(np.float32)
+    ga, gb = po.hinge_frames_from_frames(tr, F, F)
+    assert np.array_equal(ga, F) and np.array_equal(gb[[0, 1, 3, 4, 6, 7]], F[[0, 1, 3, 4, 6, 7]])
+    assert np.array_equal(gb[[2, 5, 8]], -F[[2, 5, 8]])
+    ga, gb = po.hinge_frames_from_frames(tr, F)
+    assert np.allclose(gb[9:], R @ F[9:] + tr[9:], atol=1e-6)
+
+
+def _hinge_world(L=1.0, half=(0.1, 0.1, 0.1), axis=(0.0, 0.0, 1.0), w0=(0.0, 0.0, 0.0), params=None, tilt=0.0,
+                 iterations=10, frames=False):
+    """A box of 1 kg hinged to the world (the fixed body) at a pivot L above its centre, about `axis` (in the
+    box's frame, which starts aligned with the world), released at `tilt` radians about the axis with angular
+    velocity w0 and the matching centre velocity w0 x (centre - pivot); the floor is far below.  frames: built with
+    the (rbA, rbAFrame) constructor (frame = identity basis at the pivot; axis z), else (rbA, pivotInA, axisInA)."""
+    b = scenes._Builder("hinge")
+    pl = b.shape(PLANE, [0, 1, 0, -50.0])
+    b.body(pl, scenes.IDENT, np.zeros(3, np.float32), 0.0)
+    bx = b.shape(BOX, list(half))
+    pivot = np.array([0.0, 10.0, 0.0])
+    Rt = _rot(axis, tilt)
+    pos = pivot + Rt @ np.array([0.0, -L, 0.0])
+    b.body(bx, Rt.astype(np.float32).reshape(-1), pos.astype(np.float32), 1.0)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    bodies = np.asarray(bodies, np.float32).reshape(-1, po.BODY_WORDS).copy()
+    bodies[1, 15:18] = w0
+    bodies[1, 12:15] = np.cross(np.float64(w0), pos - pivot)
+    w = po.World(shapes, verts, bodies, iterations=iterations, order=0)
+    if frames:
+        F = np.concatenate([np.eye(3).reshape(-1), [0.0, L, 0.0]]).astype(np.float32)
+        fa, fb = po.hinge_frames_from_frames(bodies[1, :12], F)
+    else:
+        fa, fb = po.hinge_frames(bodies[1, :12], [0.0, L, 0.0], np.float32(axis))
+    w.add_hinge([1], [-1], fa[None], fb[None], None if params is None else np.float32(params)[None])
+    return w
+
+
+def test_hinge_pendulum_period_and_plane():
+    """A box pendulum on a hinge about z: period 2 pi sqrt(I_p / (m g L)) (I_p = box inertia about z + m L^2) times
+    (1 + theta0^2 / 16) within 2%; the motion stays in the hinge plane (|v_z|, |w_x|, |w_y| below 1e-3) and the
+    pivot within 1% of L of its world point."""
+    L, h, th = 1.0, 0.1, 0.15
+    w = _hinge_world(L=L, half=(h, h, h), tilt=th)
+    xs, off = [], 0.0
+    for _ in range(660):
+        w.step()
+        b = w.bodies()[1]
+        R = b[:9].reshape(3, 3).astype(np.float64)
+        off = max(off, np.linalg.norm(R @ [0, L, 0] + b[9:12] - [0, 10, 0]))
+        assert abs(b[14]) < 1e-3 and abs(b[15]) < 1e-3 and abs(b[16]) < 1e-3, b[12:18]
+        xs.append(float(b[9]))
+    xs = np.array(xs)
+    t = (np.arange(len(xs)) + 1) / 60.0
+    cross = [t[i] + (t[i + 1] - t[i]) * xs[i] / (xs[i] - xs[i + 1]) for i in range(len(xs) - 1)
+             if xs[i] * xs[i + 1] < 0]
+    period = 2.0 * np.mean(np.diff(cross))
+    Ip = (2 * h) ** 2 * 2 / 12.0 + L * L
+    want = 2 * np.pi * np.sqrt(Ip / (9.81 * L)) * (1 + th * th / 16)
+    assert len(cross) >= 9 and abs(period / want - 1) < 0.02, (period, want)
+    assert off < 0.01 * L, off
+
+
+def test_hinge_removes_off_axis_spin_and_keeps_axis_spin():
+    """A wheel hinged at its centre about z, spun at 5 rad/s about z and 3 rad/s about x: gravity exerts no torque
+    about the pivot, the hinge takes out the x spin within the first step and leaves the z spin at 5 rad/s (to
+    1e-4) for two seconds."""
+    w = _hinge_world(L=0.0, half=(0.5, 0.5, 0.1), w0=(3.0, 0.0, 5.0))
+    for k in range(120):
+        w.step()
+        b = w.bodies()[1]
+        assert abs(b[15]) < 1e-3 and abs(b[16]) < 1e-3, (k, b[15:18])
+        assert abs(b[17] - 5.0) < 1e-4, (k, b[17])
+
+
+def test_hinge_limit_holds_the_angle():
+    """A pendulum on a hinge (frame constructor, hinge angle 0 at rest) with limits [-0.3, 0.3] rad (softness 0.9),
+    kicked at 4 rad/s: unlimited it swings past 1 rad; with the limit its orientation about the axis stays below
+    0.3 rad and reaches 0.2.  The limit acts on atan2Fast's angle, which reads 0.3 at a true 0.232 rad
+    (r = (1 - 0.3 / (pi / 4)) solves (cos - sin) / (cos + sin) = r), and the bias lets it overshoot a little."""
+    w = _hinge_world(L=1.0, w0=(0.0, 0.0, 4.0), params=(-0.3, 0.3, 0.9, 0.3, 1.0, 0, 0, 0, 0), frames=True)
+    assert w.hinge_angle(0) == 0.0
+    free = _hinge_world(L=1.0, w0=(0.0, 0.0, 4.0), frames=True)
+    amax = fmax = 0.0
+    for _ in range(120):
+        w.step()
+        free.step()
+        R = w.bodies()[1, :9].reshape(3, 3)
+        amax = max(amax, abs(float(np.arctan2(R[1, 0], R[0, 0]))))
+        R = free.bodies()[1, :9].reshape(3, 3)
+        fmax = max(fmax, abs(float(np.arctan2(R[1, 0], R[0, 0]))))
+    assert fmax > 1.0, fmax
+    assert 0.2 < amax < 0.3, amax
+
+
+def test_hinge_motor_reaches_its_target():
+    """A wheel hinged at its centre with the angular motor on (target 2 rad/s): with a large impulse cap it spins
+    at 2 rad/s (to 1e-4) after the first step; with a cap of 0.001 per iteration its spin after one step of 10
+    iterations is at most 10 * 0.001 / I_zz (the motor impulse is clipped each iteration)."""
+    w = _hinge_world(L=0.0, half=(0.5, 0.5, 0.1), params=(1e30, -1e30, 0.9, 0.3, 1.0, 0, 1, 2.0, 100.0))
+    for _ in range(3):
+        w.step()
+    assert abs(w.bodies()[1, 17] - 2.0) < 1e-4, w.bodies()[1, 15:18]
+    w = _hinge_world(L=0.0, half=(0.5, 0.5, 0.1), params=(1e30, -1e30, 0.9, 0.3, 1.0, 0, 1, 2.0, 0.001))
+    w.step()
+    Izz = (1.0 + 1.0) / 12.0
+    assert 0.0 < w.bodies()[1, 17] <= 10 * 0.001 / Izz * 1.0001, w.bodies()[1, 17]
+
+
+def test_hinge_angle_of_the_fixed_body_axis_constructor_is_pi():
+    """(rbA, pivotInA, axisInA) with A unrotated and the axis along z: B's axis is -z, so the shortest arc from z
+    to -z takes the "pick any vector" branch, the quaternion (0, 1, 0, 0), which turns A's reference axis y into
+    itself; B's swing axis is then (-z) x y = +x, A's second reference axis z x y = -x, and the hinge angle at
+    rest is atan2Fast(0, -1) = pi (as in the reference, so limits about 0 act at once)."""
+    w = _hinge_world(L=1.0)
+    assert w.hinge_angle(0) == np.float32(np.pi)
+
+
+def test_hinge_realigns_a_tilted_axis():
+    """A wheel hinged at its centre about z whose body is then tilted 0.2 rad about x: the angular error term
+    (-(axisA x axisB) / dt, scaled by the inverse angular mass) turns the axle back; after one second the body's
+    z axis is within 0.01 rad of the world's, and it never tilts further than at the start."""
+    w = _hinge_world(L=0.0, half=(0.5, 0.5, 0.1))
+    b = w.bodies()
+    b[1, :9] = _rot([1, 0, 0], 0.2).astype(np.float32).reshape(-1)
+    w.set_bodies(b)
+    for _ in range(60):
+        w.step()
+        tilt = np.arccos(min(1.0, float(w.bodies()[1, :9].reshape(3, 3)[2, 2])))
+        assert tilt < 0.2 + 1e-4, tilt
+    assert tilt < 0.01, tilt
