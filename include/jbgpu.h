@@ -163,6 +163,24 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out);
  * give JBGPU_ERR_ARG and change nothing. */
 int jbgpu_world_add_point2point(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
                                 const float* pivot_a, const float* pivot_b, const float* settings);
+/* DiscreteDynamicsWorld.addConstraint of n HingeConstraints (dynamics/constraintsolver/HingeConstraint.java):
+ * frame_a / frame_b are rbAFrame / rbBFrame as the constructors leave them (12 floats each: basis row-major, then
+ * origin; B's third column already flipped; for the fixed body, body_b -1, B's frame is in world space), e.g. from
+ * jbgpu_hinge_frames.  params per hinge: (lowerLimit, upperLimit, limitSoftness, biasFactor, relaxationFactor,
+ * angularOnly, enableAngularMotor, motorTargetVelocity, maxMotorImpulse) as setLimit / setAngularOnly /
+ * enableAngularMotor set them (:566-583), or NULL for a free hinge (1e30, -1e30, 0.9, 0.3, 1, 0, 0, 0, 0).
+ * Solved with the other constraints (buildJacobian :240-333, solveConstraint :335-520); ids continue the
+ * constraint numbering; errors as jbgpu_world_add_point2point. */
+int jbgpu_world_add_hinge(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
+                          const float* frame_a, const float* frame_b, const float* params);
+/* Host-only: the frames HingeConstraint's constructors build (HingeConstraint.java:84-238) from body A's current
+ * transform tr_a (12 floats).  from_frames 0: the pivot/axis forms (rbA, rbB, pivotInA, pivotInB, axisInA,
+ * axisInB), or with fixed_b (rbA, pivotInA, axisInA; pivot_b / axis_b unused); from_frames 1: the frame forms
+ * (rbA, rbB, rbAFrame, rbBFrame), or with fixed_b (rbA, rbAFrame; frame_b_in unused).  Writes frame_a and
+ * frame_b for jbgpu_world_add_hinge.  Needs no device. */
+int jbgpu_hinge_frames(int32_t from_frames, int32_t fixed_b, const float* tr_a, const float* pivot_a,
+                       const float* pivot_b, const float* axis_a, const float* axis_b, const float* frame_a_in,
+                       const float* frame_b_in, float* frame_a, float* frame_b);
 int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n);
 /* appliedImpulse of every constraint (TypedConstraint.getAppliedImpulse): the sum over the last solve that
  * processed it of its impulses on all three axes and iterations; 0 before its first solve. */

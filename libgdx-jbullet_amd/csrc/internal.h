@@ -93,10 +93,18 @@ constexpr int kEpaSmallGrid = 2048;  // first-pass EPA blocks (eight per CU)
 constexpr int kEpaGrid = 512;  // EPA blocks (LDS workspace limits them to two per CU)
 constexpr int kTimerRegions = 16;
 
-// Point2Point constraint record: body A, body B (-1: the fixed body), pivot in A, pivot in B (a world point
-// for the fixed body), tau, damping, impulse clamp, 0
-constexpr int kJointWords = 12;
-constexpr int kJointStateWords = 32;
+// Constraint record: type (0 Point2PointConstraint, 1 HingeConstraint), body A, body B (-1: the fixed body), 0,
+// then per type
+//   point2point: pivot in A [4, 7), pivot in B (a world point for the fixed body) [7, 10), tau, damping, clamp
+//   hinge: rbAFrame [4, 16) and rbBFrame [16, 28) (basis row-major, origin; B's third column flipped), lower and
+//          upper limit, limit softness, bias factor, relaxation factor, angularOnly, enableAngularMotor, motor
+//          target velocity, max motor impulse [28, 37)
+constexpr int kJointWords = 40;
+constexpr int kJointP2P = 0, kJointHinge = 1;
+// per-solve state: 1 / diagonal of the three linear rows, appliedImpulse, the pivots' arms, pivotAInW -
+// pivotBInW, both bodies' world inverse inertia [14, 32); hinge: linear axes [32, 41), world hinge axes of A and
+// B [41, 47), kHinge, limit correction, limit sign, limit on, accumulated limit impulse [47, 52)
+constexpr int kJointStateWords = 64;
 
 // solver rows per manifold: 4 contacts x (normal row + 2 friction rows)
 constexpr int kRowWords = 20;   // normal(3) rel1xn(3) rel2xn(3) angA(3) angB(3) jacInv pen rest fric applied

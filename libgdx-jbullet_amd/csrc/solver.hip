@@ -1418,8 +1418,9 @@ __global__ void kCopyVel(int n, const float* __restrict__ src, float* dst, const
 // writebackVelocity / readVelocity, SequentialImpulseConstraintSolver.java:823-842), and a body without solver
 // rows keeps its rigid-body velocities, which its solver record here equals.  The fixed body (bodyB == -1,
 // TypedConstraint.getFixed) has the identity transform, zero velocity, zero inverse mass and inertia.
-__device__ __forceinline__ int jointBodyA(const float* R) { return ldi(R); }
-__device__ __forceinline__ int jointBodyB(const float* R) { return ldi(R + 1); }
+__device__ __forceinline__ int jointType(const float* R) { return ldi(R); }
+__device__ __forceinline__ int jointBodyA(const float* R) { return ldi(R + 1); }
+__device__ __forceinline__ int jointBodyB(const float* R) { return ldi(R + 2); }
 __device__ __forceinline__ Tr bodyTr(const float* __restrict__ tr, int i) {
   Tr t;
   if (i < 0) {
@@ -1471,36 +1472,45 @@ __global__ void kJointMark(int nj, const float* __restrict__ rec, const float* _
   }
   solve[j] = f;
 }
-// buildJacobian (Point2PointConstraint.java:71-107, JacobianEntry.java:56-73) with appliedImpulse = 0; the
-// state keeps what solveConstraint recomputes from the unchanged transforms: 1 / diagonal per axis, the applied
-// impulse, the pivots' arms rel1 / rel2, pivotAInW - pivotBInW, and both bodies' world inverse inertia
+__device__ __forceinline__ V3 column(const M3& m, int c) { return v3(m.m[c], m.m[3 + c], m.m[6 + c]); }
+__device__ __forceinline__ float atan2Fast(float y, float x) {  // ScalarUtil.java:43-57
+  const float simdPi = 6.283185307179586232f * 0.5f;
+  const float coeff_1 = simdPi / 4.0f;
+  const float coeff_2 = 3.0f * coeff_1;
+  const float abs_y = jabs(y);
+  float angle;
+  if (x >= 0.0f) {
+    const float r = (x - abs_y) / (x + abs_y);
+    angle = coeff_1 - coeff_1 * r;
+  } else {
+    const float r = (x + abs_y) / (abs_y - x);
+    angle = coeff_2 - coeff_1 * r;
+  }
+  return (y < 0.0f) ? -angle : angle;
+}
+__device__ __forceinline__ M3 loadM3(const float* p) {
+  M3 m;
+  for (int k = 0; k < 9; k++) m.m[k] = p[k];
+  return m;
+}
+// RigidBody.computeAngularImpulseDenominator (RigidBody.java:534-541): axis . (I_world^T axis)
+__device__ __forceinline__ float angularDenominator(const M3& I, V3 axis) { return dot(axis, mulMtV(I, axis)); }
+// buildJacobian with appliedImpulse = 0: Point2PointConstraint.java:71-107 (linear rows along the world axes),
+// HingeConstraint.java:240-333 (linear rows along the pivot separation and its plane space, the limit state from
+// getHingeAngle :530-549, kHinge); JacobianEntry.java:56-73.  The state keeps what solveConstraint recomputes from
+// the unchanged transforms: 1 / diagonal per row, the pivots' arms, pivotAInW - pivotBInW, both bodies' world
+// inverse inertia and, for a hinge, the world hinge axes
 __global__ void kJointBuild(int nj, const float* __restrict__ rec, const int* __restrict__ solve,
                             const float* __restrict__ tr, const float* __restrict__ props, float* st) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nj || !solve[j]) return;
   const float* R = rec + kJointWords * j;
   float* S = st + kJointStateWords * j;
-  const int a = jointBodyA(R), b = jointBodyB(R);
+  const int type = jointType(R), a = jointBodyA(R), b = jointBodyB(R);
   const Tr ta = bodyTr(tr, a), tb = bodyTr(tr, b);
-  const V3 pivotAInW = xform(ta, ld3(R + 2)), pivotBInW = xform(tb, ld3(R + 5));
-  const V3 rel1 = sub(pivotAInW, ta.o), rel2 = sub(pivotBInW, tb.o);
   const float imA = props[8 * a], imB = b >= 0 ? props[8 * b] : 0.0f;
   const V3 ilA = v3(props[8 * a + 1], props[8 * a + 2], props[8 * a + 3]);
   const V3 ilB = b >= 0 ? v3(props[8 * b + 1], props[8 * b + 2], props[8 * b + 3]) : v3(0.0f, 0.0f, 0.0f);
-  const M3 world2A = transpose(ta.b), world2B = transpose(tb.b);
-  for (int i = 0; i < 3; i++) {
-    const V3 normal = unitAxis(i);
-    const V3 aJ = mulMV(world2A, crs(rel1, normal));
-    const V3 bJ = mulMV(world2B, crs(rel2, scl(normal, -1.0f)));
-    const V3 m0 = mulc(ilA, aJ), m1 = mulc(ilB, bJ);
-    const float diag = imA + dot(m0, aJ) + imB + dot(m1, bJ);
-    S[i] = 1.0f / diag;
-  }
-  S[3] = 0.0f;
-  st3(S + 4, rel1);
-  st3(S + 7, rel2);
-  st3(S + 10, sub(pivotAInW, pivotBInW));
-  S[13] = 0.0f;
   const M3 IA = invInertiaWorld(tr, props, a);
   M3 IB;
   if (b >= 0) IB = invInertiaWorld(tr, props, b);
@@ -1510,6 +1520,62 @@ __global__ void kJointBuild(int nj, const float* __restrict__ rec, const int* __
     S[14 + k] = IA.m[k];
     S[23 + k] = IB.m[k];
   }
+  S[3] = 0.0f;
+  S[13] = 0.0f;
+  const bool hinge = type == kJointHinge;
+  const bool linear = !hinge || R[33] == 0.0f;  // a hinge with angularOnly has no linear rows
+  if (linear) {
+    // the pivots: a hinge's frame origins, a point-to-point's pivots
+    const V3 pivotAInW = xform(ta, ld3(R + (hinge ? 13 : 4))), pivotBInW = xform(tb, ld3(R + (hinge ? 25 : 7)));
+    const V3 rel1 = sub(pivotAInW, ta.o), rel2 = sub(pivotBInW, tb.o);
+    V3 n[3];
+    if (hinge) {
+      const V3 relPos = sub(pivotBInW, pivotAInW);
+      n[0] = len2(relPos) > 1.19209290e-07f ? nor(relPos) : v3(1.0f, 0.0f, 0.0f);
+      planeSpace1(n[0], n[1], n[2]);
+    } else {
+      for (int i = 0; i < 3; i++) n[i] = unitAxis(i);
+    }
+    const M3 world2A = transpose(ta.b), world2B = transpose(tb.b);
+    for (int i = 0; i < 3; i++) {
+      const V3 aJ = mulMV(world2A, crs(rel1, n[i]));
+      const V3 bJ = mulMV(world2B, crs(rel2, scl(n[i], -1.0f)));
+      const V3 m0 = mulc(ilA, aJ), m1 = mulc(ilB, bJ);
+      const float diag = imA + dot(m0, aJ) + imB + dot(m1, bJ);
+      S[i] = 1.0f / diag;
+      if (hinge) st3(S + 32 + 3 * i, n[i]);
+    }
+    st3(S + 4, rel1);
+    st3(S + 7, rel2);
+    st3(S + 10, sub(pivotAInW, pivotBInW));
+  }
+  if (!hinge) return;
+  const M3 fa = loadM3(R + 4), fb = loadM3(R + 16);
+  // getHingeAngle: B's second frame axis against A's first two, in world space
+  const V3 ref0 = mulMV(ta.b, column(fa, 0)), ref1 = mulMV(ta.b, column(fa, 1));
+  const V3 swing = mulMV(tb.b, column(fb, 1));
+  const float angle = atan2Fast(dot(swing, ref0), dot(swing, ref1));
+  const float lower = R[28], upper = R[29], softness = R[30];
+  float correction = 0.0f, limitSign = 0.0f, solveLimit = 0.0f;
+  if (lower < upper) {
+    if (angle <= lower * softness) {
+      correction = lower - angle;
+      limitSign = 1.0f;
+      solveLimit = 1.0f;
+    } else if (angle >= upper * softness) {
+      correction = upper - angle;
+      limitSign = -1.0f;
+      solveLimit = 1.0f;
+    }
+  }
+  const V3 axisA = mulMV(ta.b, column(fa, 2)), axisB = mulMV(tb.b, column(fb, 2));
+  st3(S + 41, axisA);
+  st3(S + 44, axisB);
+  S[47] = 1.0f / (angularDenominator(IA, axisA) + angularDenominator(IB, axisA));
+  S[48] = correction;
+  S[49] = limitSign;
+  S[50] = solveLimit;
+  S[51] = 0.0f;
 }
 // RigidBody.applyImpulse (RigidBody.java:414-428, linear and angular factors 1): applyCentralImpulse, then
 // applyTorqueImpulse(relPos x impulse) through the world inverse inertia
@@ -1520,20 +1586,16 @@ __device__ __forceinline__ void applyImpulseRB(SBody& b, V3 impulse, V3 relPos, 
   for (int k = 0; k < 9; k++) m.m[k] = I[k];
   b.av = add(b.av, mulMV(m, crs(relPos, impulse)));
 }
-// solveConstraint (Point2PointConstraint.java:109-177): per axis the positional error (tau / dt) and the
-// damped relative velocity along it, clamped when impulseClamp > 0
-__device__ void jointSolve(int j, float dt, const float* __restrict__ rec, float* st, float* svel) {
-  const float* R = rec + kJointWords * j;
-  float* S = st + kJointStateWords * j;
-  const int a = jointBodyA(R), b = jointBodyB(R);
-  const float tau = R[8], damping = R[9], clamp = R[10];
-  SBody A = loadBody(svel, nullptr, a), B;
-  if (b >= 0) B = loadBody(svel, nullptr, b);
-  else {
-    B.lv = v3(0.0f, 0.0f, 0.0f);
-    B.av = v3(0.0f, 0.0f, 0.0f);
-    B.invMass = 0.0f;
-  }
+// RigidBody.applyTorqueImpulse (RigidBody.java:405-411): no inverse-mass test
+__device__ __forceinline__ void applyTorqueImpulse(SBody& b, V3 torque, const float* I) {
+  M3 m;
+  for (int k = 0; k < 9; k++) m.m[k] = I[k];
+  b.av = add(b.av, mulMV(m, torque));
+}
+// Point2PointConstraint.solveConstraint (Point2PointConstraint.java:109-177): per world axis the positional error
+// (tau / dt) and the damped relative velocity along it, clamped when impulseClamp > 0
+__device__ __forceinline__ void p2pSolve(const float* R, float* S, float dt, SBody& A, SBody& B) {
+  const float tau = R[10], damping = R[11], clamp = R[12];
   const V3 rel1 = ld3(S + 4), rel2 = ld3(S + 7), dpiv = ld3(S + 10);
   float applied = S[3];
   for (int i = 0; i < 3; i++) {
@@ -1554,6 +1616,93 @@ __device__ void jointSolve(int j, float dt, const float* __restrict__ rec, float
     applyImpulseRB(B, scl(iv, -1.0f), rel2, S + 23);
   }
   S[3] = applied;
+}
+// HingeConstraint.solveConstraint (HingeConstraint.java:335-520): the linear rows on the relative pivot velocity
+// taken once before them; then, on the angular velocities read after them, the torque that removes the relative
+// angular velocity orthogonal to the hinge axis and the one that realigns the axes, the limit (accumulated impulse
+// kept >= 0) and the motor (clipped to its maximum impulse, not accumulated)
+__device__ __forceinline__ void hingeSolve(const float* R, float* S, float dt, SBody& A, SBody& B) {
+  const float tau = 0.3f;
+  const float* IA = S + 14;
+  const float* IB = S + 23;
+  if (R[33] == 0.0f) {
+    const V3 rel1 = ld3(S + 4), rel2 = ld3(S + 7), dpiv = ld3(S + 10);
+    const V3 vel = sub(add(crs(A.av, rel1), A.lv), add(crs(B.av, rel2), B.lv));
+    float applied = S[3];
+    for (int i = 0; i < 3; i++) {
+      const V3 n = ld3(S + 32 + 3 * i);
+      const float jacDiagABInv = S[i];
+      const float rel_vel = dot(n, vel);
+      const float depth = -dot(dpiv, n);
+      const float impulse = depth * tau / dt * jacDiagABInv - rel_vel * jacDiagABInv;
+      applied += impulse;
+      const V3 iv = scl(n, impulse);
+      applyImpulseRB(A, iv, rel1, IA);
+      applyImpulseRB(B, scl(iv, -1.0f), rel2, IB);
+    }
+    S[3] = applied;
+  }
+  const V3 axisA = ld3(S + 41), axisB = ld3(S + 44);
+  const float relaxationFactor = R[32];
+  const M3 mA = loadM3(IA), mB = loadM3(IB);
+  const V3 angVelA = A.av, angVelB = B.av;
+  const V3 aroundA = scl(axisA, dot(axisA, angVelA));
+  const V3 aroundB = scl(axisB, dot(axisB, angVelB));
+  V3 velrelOrthog = sub(sub(angVelA, aroundA), sub(angVelB, aroundB));
+  const float relaxation = 1.0f;
+  if (len(velrelOrthog) > 0.00001f) {
+    const V3 n = nor(velrelOrthog);
+    const float denom = angularDenominator(mA, n) + angularDenominator(mB, n);
+    velrelOrthog = scl(velrelOrthog, (1.0f / denom) * relaxationFactor);
+  }
+  V3 angularError = crs(axisA, axisB);
+  angularError = scl(angularError, -1.0f);
+  angularError = scl(angularError, 1.0f / dt);
+  if (len(angularError) > 0.00001f) {
+    const V3 n2 = nor(angularError);
+    const float denom2 = angularDenominator(mA, n2) + angularDenominator(mB, n2);
+    angularError = scl(angularError, (1.0f / denom2) * relaxation);
+  }
+  applyTorqueImpulse(A, add(scl(velrelOrthog, -1.0f), angularError), IA);
+  applyTorqueImpulse(B, sub(velrelOrthog, angularError), IB);
+  if (S[50] != 0.0f) {
+    const float amplitude =
+        (dot(sub(angVelB, angVelA), axisA) * relaxationFactor + S[48] * (1.0f / dt) * R[31]) * S[49];
+    float impulseMag = amplitude * S[47];
+    const float temp = S[51];
+    S[51] = jmax(S[51] + impulseMag, 0.0f);
+    impulseMag = S[51] - temp;
+    const V3 impulse = scl(axisA, impulseMag * S[49]);
+    applyTorqueImpulse(A, impulse, IA);
+    applyTorqueImpulse(B, scl(impulse, -1.0f), IB);
+  }
+  if (R[34] != 0.0f) {
+    const V3 angularLimit = v3(0.0f, 0.0f, 0.0f);
+    const float projRelVel = dot(sub(aroundA, aroundB), axisA);
+    const float motor_relvel = R[35] - projRelVel;
+    const float unclipped = S[47] * motor_relvel;
+    const float maxImp = R[36];
+    float clipped = unclipped > maxImp ? maxImp : unclipped;
+    clipped = clipped < -maxImp ? -maxImp : clipped;
+    const V3 motorImp = scl(axisA, clipped);
+    applyTorqueImpulse(A, add(motorImp, angularLimit), IA);
+    applyTorqueImpulse(B, sub(scl(motorImp, -1.0f), angularLimit), IB);
+  }
+}
+__device__ void jointSolve(int j, float dt, const float* __restrict__ rec, float* st, float* svel) {
+  const float* R = rec + kJointWords * j;
+  float* S = st + kJointStateWords * j;
+  const int a = jointBodyA(R), b = jointBodyB(R);
+  SBody A = loadBody(svel, nullptr, a), B;
+  if (b >= 0) B = loadBody(svel, nullptr, b);
+  else {
+    B.lv = v3(0.0f, 0.0f, 0.0f);
+    B.av = v3(0.0f, 0.0f, 0.0f);
+    B.invMass = 0.0f;
+  }
+  if (jointType(R) == kJointHinge) hingeSolve(R, S, dt, A, B);
+  else p2pSolve(R, S, dt, A, B);
+  // a body without inverse mass keeps its record (the reference's read-back skips it as well)
   if (A.invMass != 0.0f) storeBody(svel, a, A);
   if (b >= 0 && B.invMass != 0.0f) storeBody(svel, b, B);
 }
@@ -1616,7 +1765,7 @@ static void jointsMakeOrder(Ctx& c) {
   };
   int ncol = 0;
   for (int j = 0; j < nj; j++) {
-    const int a = ldi(&c.jointHost[(size_t)kJointWords * j]), b = ldi(&c.jointHost[(size_t)kJointWords * j + 1]);
+    const int a = ldi(&c.jointHost[(size_t)kJointWords * j + 1]), b = ldi(&c.jointHost[(size_t)kJointWords * j + 2]);
     int k = 0;
     while (taken(a, k) || taken(b, k)) k++;
     colour[j] = k;

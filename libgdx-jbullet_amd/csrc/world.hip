@@ -853,6 +853,117 @@ int jbgpu_world_step_until(jbgpu_ctx* ctx, float dt, int32_t phase) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// bodies of n new constraints: A in range, B in range or -1 (the fixed body), A != B; all checked before any change
+void checkJointBodies(Ctx& c, int n, const int32_t* body_a, const int32_t* body_b) {
+  if ((int64_t)c.nj + n > (1 << 30)) throw std::length_error("too many constraints");
+  for (int i = 0; i < n; i++) {
+    const int a = body_a[i], b = body_b[i];
+    if (a < 0 || a >= c.nb) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A out of range");
+    if (b < -1 || b >= c.nb)
+      throw std::invalid_argument("constraint " + std::to_string(i) + ": body B out of range (-1: the fixed body)");
+    if (a == b) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A is body B");
+  }
+}
+// the last n host records to the device, appliedImpulse 0 (TypedConstraint's initial value)
+void appendJoints(Ctx& c, int n) {
+  if (n == 0) return;
+  const int nj = c.nj + n;
+  c.joints.ensureKeep((size_t)kJointWords * nj);
+  c.jointState.ensureKeep((size_t)kJointStateWords * nj);
+  JB_CHECK(hipMemcpy(c.joints.p + (size_t)kJointWords * c.nj, c.jointHost.data() + (size_t)kJointWords * c.nj,
+                     (size_t)kJointWords * n * sizeof(float), hipMemcpyHostToDevice));
+  JB_CHECK(hipMemset(c.jointState.p + (size_t)kJointStateWords * c.nj, 0, (size_t)kJointStateWords * n * sizeof(float)));
+  c.nj = nj;
+  jointsChanged(c);
+}
+Tr trFrom12(const float* p) {
+  Tr t;
+  for (int k = 0; k < 9; k++) t.b.m[k] = p[k];
+  t.o = v3(p[9], p[10], p[11]);
+  return t;
+}
+void trTo12(const Tr& t, float* p) {
+  for (int k = 0; k < 9; k++) p[k] = t.b.m[k];
+  p[9] = t.o.x;
+  p[10] = t.o.y;
+  p[11] = t.o.z;
+}
+V3 columnOf(const M3& m, int c) { return v3(m.m[c], m.m[3 + c], m.m[6 + c]); }  // MatrixUtil.getColumn
+void setRowOf(M3& m, int r, float x, float y, float z) {                         // MatrixUtil.setRow
+  m.m[3 * r] = x;
+  m.m[3 * r + 1] = y;
+  m.m[3 * r + 2] = z;
+}
+// HingeConstraint's constructors (HingeConstraint.java:84-238), host side: the frames the solver uses
+void hingeFramesFromAxes(const Tr& trA, V3 pivotA, V3 pivotB, V3 axisA, V3 axisB, bool fixedB, Tr& frameA,
+                         Tr& frameB) {
+  const float eps = 1.19209290e-07f;  // BulletGlobals.SIMD_EPSILON
+  V3 a1 = columnOf(trA.b, 0), a2;
+  if (!fixedB) {
+    const float projection = dot(axisA, a1);
+    if (projection >= 1.0f - eps) {
+      a1 = scl(columnOf(trA.b, 2), -1.0f);
+      a2 = columnOf(trA.b, 1);
+    } else if (projection <= -1.0f + eps) {
+      a1 = columnOf(trA.b, 2);
+      a2 = columnOf(trA.b, 1);
+    } else {  // not renormalised: both axes have length sin(angle between axisA and A's x axis)
+      a2 = crs(axisA, a1);
+      a1 = crs(a2, axisA);
+    }
+  } else {
+    const float projection = dot(a1, axisA);
+    if (projection > eps) {
+      a1 = scl(a1, projection);
+      a1 = sub(a1, axisA);
+    } else {
+      a1 = columnOf(trA.b, 1);
+    }
+    a2 = crs(axisA, a1);
+    axisB = mulMV(trA.b, scl(axisA, -1.0f));  // the axis in the fixed body: -axisA in world space
+  }
+  frameA.o = pivotA;
+  setRowOf(frameA.b, 0, a1.x, a2.x, axisA.x);
+  setRowOf(frameA.b, 1, a1.y, a2.y, axisA.y);
+  setRowOf(frameA.b, 2, a1.z, a2.z, axisA.z);
+  // QuaternionUtil.shortestArcQuat (QuaternionUtil.java:49-68; its antiparallel test is a double comparison),
+  // then quatRotate (:78-90): (q * v) * conj(q)
+  const V3 cr = crs(axisA, axisB);
+  const float d = dot(axisA, axisB);
+  Q4 arc;
+  if ((double)d < -1.0 + (double)eps) {
+    arc = Q4{0.0f, 1.0f, 0.0f, 0.0f};
+  } else {
+    const float sq = (float)std::sqrt((double)((1.0f + d) * 2.0f));
+    const float rs = 1.0f / sq;
+    arc = Q4{cr.x * rs, cr.y * rs, cr.z * rs, sq * 0.5f};
+  }
+  Q4 q{arc.w * a1.x + arc.y * a1.z - arc.z * a1.y, arc.w * a1.y + arc.z * a1.x - arc.x * a1.z,
+       arc.w * a1.z + arc.x * a1.y - arc.y * a1.x, -arc.x * a1.x - arc.y * a1.y - arc.z * a1.z};
+  q = qmul(q, Q4{-arc.x, -arc.y, -arc.z, arc.w});
+  const V3 b1 = v3(q.x, q.y, q.z);
+  const V3 b2 = crs(axisB, b1);
+  // the two-body form flips B's axis column; the fixed-body form keeps it (it is -axisA already)
+  frameB.o = fixedB ? xform(trA, pivotA) : pivotB;
+  setRowOf(frameB.b, 0, b1.x, b2.x, fixedB ? axisB.x : -axisB.x);
+  setRowOf(frameB.b, 1, b1.y, b2.y, fixedB ? axisB.y : -axisB.y);
+  setRowOf(frameB.b, 2, b1.z, b2.z, fixedB ? axisB.z : -axisB.z);
+}
+void hingeFramesFromFrames(const Tr& trA, const Tr& inA, const Tr& inB, bool fixedB, Tr& frameA, Tr& frameB) {
+  frameA = inA;
+  frameB = fixedB ? inA : inB;
+  frameB.b.m[2] *= -1.0f;  // flip axis: M02, M12, M22
+  frameB.b.m[5] *= -1.0f;
+  frameB.b.m[8] *= -1.0f;
+  if (fixedB) frameB.o = xform(trA, inA.o);
+}
+}  // namespace
+
+extern "C" {
+
 int jbgpu_world_add_point2point(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
                                 const float* pivot_a, const float* pivot_b, const float* settings) {
   return guard(ctx, [&] {
@@ -861,37 +972,76 @@ int jbgpu_world_add_point2point(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a
     if (n < 0 || (n > 0 && (!body_a || !body_b || !pivot_a || !pivot_b)))
       throw std::invalid_argument("bad constraint arguments");
     if ((int64_t)c.nj + n > (1 << 30)) throw std::length_error("too many constraints");
-    for (int i = 0; i < n; i++) {
-      const int a = body_a[i], b = body_b[i];
-      if (a < 0 || a >= c.nb) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A out of range");
-      if (b < -1 || b >= c.nb)
-        throw std::invalid_argument("constraint " + std::to_string(i) + ": body B out of range (-1: the fixed body)");
-      if (a == b) throw std::invalid_argument("constraint " + std::to_string(i) + ": body A is body B");
-    }
-    if (n == 0) return;
-    const int nj = c.nj + n;
+    checkJointBodies(c, n, body_a, body_b);
     for (int i = 0; i < n; i++) {
       float r[kJointWords] = {};
-      sti(r, body_a[i]);
-      sti(r + 1, body_b[i]);
+      sti(r, kJointP2P);
+      sti(r + 1, body_a[i]);
+      sti(r + 2, body_b[i]);
       for (int k = 0; k < 3; k++) {
-        r[2 + k] = pivot_a[3 * i + k];
-        r[5 + k] = pivot_b[3 * i + k];
+        r[4 + k] = pivot_a[3 * i + k];
+        r[7 + k] = pivot_b[3 * i + k];
       }
-      r[8] = settings ? settings[3 * i] : 0.3f;  // ConstraintSetting defaults (Point2PointConstraint.java:202-206)
-      r[9] = settings ? settings[3 * i + 1] : 1.0f;
-      r[10] = settings ? settings[3 * i + 2] : 0.0f;
+      r[10] = settings ? settings[3 * i] : 0.3f;  // ConstraintSetting defaults (Point2PointConstraint.java:202-206)
+      r[11] = settings ? settings[3 * i + 1] : 1.0f;
+      r[12] = settings ? settings[3 * i + 2] : 0.0f;
       c.jointHost.insert(c.jointHost.end(), r, r + kJointWords);
     }
-    c.joints.ensureKeep((size_t)kJointWords * nj);
-    c.jointState.ensureKeep((size_t)kJointStateWords * nj);
-    JB_CHECK(hipMemcpy(c.joints.p + (size_t)kJointWords * c.nj, c.jointHost.data() + (size_t)kJointWords * c.nj,
-                       (size_t)kJointWords * n * sizeof(float), hipMemcpyHostToDevice));
-    JB_CHECK(hipMemset(c.jointState.p + (size_t)kJointStateWords * c.nj, 0,
-                       (size_t)kJointStateWords * n * sizeof(float)));  // appliedImpulse starts at 0
-    c.nj = nj;
-    jointsChanged(c);
+    appendJoints(c, n);
   });
+}
+
+int jbgpu_world_add_hinge(jbgpu_ctx* ctx, int32_t n, const int32_t* body_a, const int32_t* body_b,
+                          const float* frame_a, const float* frame_b, const float* params) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (n < 0 || (n > 0 && (!body_a || !body_b || !frame_a || !frame_b)))
+      throw std::invalid_argument("bad constraint arguments");
+    checkJointBodies(c, n, body_a, body_b);
+    for (int i = 0; i < n; i++) {
+      float r[kJointWords] = {};
+      sti(r, kJointHinge);
+      sti(r + 1, body_a[i]);
+      sti(r + 2, body_b[i]);
+      for (int k = 0; k < 12; k++) {
+        r[4 + k] = frame_a[12 * i + k];
+        r[16 + k] = frame_b[12 * i + k];
+      }
+      // free hinge defaults (HingeConstraint.java:123-129): no limit, softness 0.9, bias 0.3, relaxation 1
+      const float def[9] = {1e30f, -1e30f, 0.9f, 0.3f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      for (int k = 0; k < 9; k++) r[28 + k] = params ? params[9 * i + k] : def[k];
+      r[33] = r[33] != 0.0f ? 1.0f : 0.0f;
+      r[34] = r[34] != 0.0f ? 1.0f : 0.0f;
+      c.jointHost.insert(c.jointHost.end(), r, r + kJointWords);
+    }
+    appendJoints(c, n);
+  });
+}
+
+int jbgpu_hinge_frames(int32_t from_frames, int32_t fixed_b, const float* tr_a, const float* pivot_a,
+                       const float* pivot_b, const float* axis_a, const float* axis_b, const float* frame_a_in,
+                       const float* frame_b_in, float* frame_a, float* frame_b) {
+  try {
+    if (!tr_a || !frame_a || !frame_b) throw std::invalid_argument("null argument");
+    Tr ta = trFrom12(tr_a), fa, fb;
+    if (from_frames) {
+      if (!frame_a_in || (!fixed_b && !frame_b_in)) throw std::invalid_argument("null frame");
+      const Tr ia = trFrom12(frame_a_in);
+      hingeFramesFromFrames(ta, ia, fixed_b ? ia : trFrom12(frame_b_in), fixed_b != 0, fa, fb);
+    } else {
+      if (!pivot_a || !axis_a || (!fixed_b && (!pivot_b || !axis_b))) throw std::invalid_argument("null pivot or axis");
+      const V3 z = v3(0.0f, 0.0f, 0.0f);
+      hingeFramesFromAxes(ta, ld3(pivot_a), fixed_b ? z : ld3(pivot_b), ld3(axis_a), fixed_b ? z : ld3(axis_b),
+                          fixed_b != 0, fa, fb);
+    }
+    trTo12(fa, frame_a);
+    trTo12(fb, frame_b);
+    return JBGPU_OK;
+  } catch (const std::exception& e) {
+    g_createError = e.what();
+    return JBGPU_ERR_ARG;
+  }
 }
 
 int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n) {

@@ -76,6 +76,8 @@ SYMBOLS = {
     "jbgpu_world_set_pairs": (C.c_int, [_vp, C.c_int64, _vp, _vp]),
     "jbgpu_world_stats": (C.c_int, [_vp, C.POINTER(Stats)]),
     "jbgpu_world_add_point2point": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]),
+    "jbgpu_world_add_hinge": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]),
+    "jbgpu_hinge_frames": (C.c_int, [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jbgpu_world_num_joints": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "jbgpu_world_joint_impulses": (C.c_int, [_vp, _vp]),
     "jbgpu_world_joint_order": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
@@ -178,6 +180,30 @@ def shape_inertia(rec, mass, verts=None):
     if rc:
         raise JbgpuError(f"jbgpu_shape_local_inertia failed ({rc})")
     return out
+
+
+def hinge_frames(tr_a, pivot_a=None, axis_a=None, pivot_b=None, axis_b=None, frame_a=None, frame_b=None):
+    """The solver frames of a HingeConstraint built from body A's transform tr_a (12 floats) as its constructors
+    build them: pivot_a / axis_a (and pivot_b / axis_b for two bodies; None: the fixed body), or frame_a (and
+    frame_b; None: the fixed body).  Host-only.  Returns (frame_a, frame_b), 12 floats each."""
+    fa, fb = np.zeros(12, np.float32), np.zeros(12, np.float32)
+    tr = f32(tr_a).reshape(-1)
+    if frame_a is not None:
+        ia = f32(frame_a).reshape(-1)
+        ib = None if frame_b is None else f32(frame_b).reshape(-1)
+        rc = load().jbgpu_hinge_frames(1, int(ib is None), _p(tr), None, None, None, None, _p(ia), _p(ib), _p(fa),
+                                       _p(fb))
+    else:
+        pa, xa = f32(pivot_a).reshape(-1), f32(axis_a).reshape(-1)
+        pb = None if pivot_b is None else f32(pivot_b).reshape(-1)
+        xb = None if axis_b is None else f32(axis_b).reshape(-1)
+        rc = load().jbgpu_hinge_frames(0, int(pb is None), _p(tr), _p(pa), _p(pb), _p(xa), _p(xb), None, None,
+                                       _p(fa), _p(fb))
+    if rc:
+        buf = C.create_string_buffer(256)
+        load().jbgpu_last_error(None, buf, 256)
+        raise JbgpuError(f"hinge_frames failed ({rc}): {buf.value.decode()}")
+    return fa, fb
 
 
 def check_pair_records(nbodies, keys, manifolds):
@@ -358,6 +384,21 @@ class Context:
             raise ValueError("settings must be n x 3")
         self._ck(load().jbgpu_world_add_point2point(self.h, len(a), _p(a), _p(b), _p(pa), _p(pb), _p(st)),
                  "world_add_point2point")
+
+    def add_hinge(self, body_a, body_b, frame_a, frame_b, params=None):
+        """addConstraint of HingeConstraints from their solver frames (n x 12, see hinge_frames); params rows
+        (lower, upper, softness, bias, relaxation, angular_only, motor_on, motor_target, max_motor) or None."""
+        a = np.ascontiguousarray(body_a, np.int32).reshape(-1)
+        b = np.ascontiguousarray(body_b, np.int32).reshape(-1)
+        fa = f32(frame_a).reshape(-1)
+        fb = f32(frame_b).reshape(-1)
+        if not (len(b) == len(a) and fa.size == 12 * len(a) and fb.size == 12 * len(a)):
+            raise ValueError("body_a, body_b, frame_a (n x 12), frame_b (n x 12) disagree")
+        pr = None if params is None else f32(params).reshape(-1)
+        if pr is not None and pr.size != 9 * len(a):
+            raise ValueError("params must be n x 9")
+        self._ck(load().jbgpu_world_add_hinge(self.h, len(a), _p(a), _p(b), _p(fa), _p(fb), _p(pr)),
+                 "world_add_hinge")
 
     def num_joints(self):
         n = C.c_int32(0)

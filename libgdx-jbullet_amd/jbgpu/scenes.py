@@ -23,6 +23,8 @@ Scenes (sizes as in the benchmark configurations):
                         falling spheres (learned warm-start workload).
   * ``chains``          hanging chains of boxes joined by Point2Point constraints, with
                         boxes dropped onto them.
+  * ``hinges``          hinged chains, doors (limits, motors) and motor-driven wheels on
+                        static posts, with boxes dropped onto them.
 """
 from dataclasses import dataclass, field
 
@@ -371,6 +373,77 @@ def chains(seed=1, nx=8, nz=8, links=8, spacing=0.9, drops=200):
         vel.append(np.zeros(3, np.float32))
     joints = (np.array(ja, np.int32), np.array(jb, np.int32), np.array(pa, np.float32), np.array(pb, np.float32))
     return b.done(dt=1 / 60, iterations=10, steps=300, joints=joints, velocity=np.array(vel, np.float32))
+
+
+def hinges(seed=1, n_chains=12, links=5, n_doors=24, n_wheels=12, drops=120):
+    """Hinged mechanisms on a floor: chains of boxes hinged end to end about z (the top link to the world),
+    doors hinged to the world about a vertical edge (resting on the floor, half of them limited to +-1 rad, a
+    quarter driven by a motor), wheels hinged at their centre to static posts (motors on, some angular-only), and
+    `drops` boxes falling onto all of it; random initial spins.  meta["hinges"]: one tuple per hinge,
+    (body_a, body_b, pivot_a, axis_a, pivot_b, axis_b, frame_a, params): pivot/axis constructor when frame_a is
+    None (pivot_b None: the fixed body), else the frame constructor (rbA, rbAFrame) with B the fixed body;
+    params (9 floats) or None.  meta["velocity"], meta["spin"]: initial linear / angular velocities."""
+    rng = np.random.default_rng(seed)
+    b = _Builder("hinges")
+    plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
+    b.body(plane, IDENT, np.zeros(3, np.float32), 0.0)
+    vel, spin, hs = [np.zeros(3)], [np.zeros(3)], []
+    z = np.array([0.0, 0.0, 1.0], np.float32)
+    link = b.shape(BOX, [0.15, 0.2, 0.1])
+    for c in range(n_chains):
+        x = (c - (n_chains - 1) / 2.0) * 1.1
+        top = links * 0.5 + 0.3 * (c % 2)
+        for m in range(links):
+            idx = len(b.bs)
+            b.body(link, IDENT, np.array([x, top - 0.25 - 0.5 * m, -6.0], np.float32), 1.0)
+            vel.append(np.zeros(3))
+            spin.append(np.array([0.0, 0.0, rng.uniform(-2, 2)]))
+            if m == 0:
+                hs.append((idx, -1, [0, 0.25, 0], z, None, None, None, None))
+            else:
+                hs.append((idx - 1, idx, [0, -0.25, 0], z, [0, 0.25, 0], z, None, None))
+    door = b.shape(BOX, [0.4, 0.6, 0.05])
+    y = np.array([0.0, 1.0, 0.0], np.float32)
+    for d in range(n_doors):
+        idx = len(b.bs)
+        pos = np.array([(d % 6) * 1.6 - 4.0, 0.6, (d // 6) * 1.6 - 1.0], np.float32)
+        b.body(door, IDENT, pos, 4.0)
+        vel.append(np.zeros(3))
+        spin.append(np.array([0.0, rng.uniform(-3, 3), 0.0]))
+        params = None
+        if d % 2 == 0:
+            params = [-1.0, 1.0, 0.9, 0.3, 1.0, 0.0, 0.0, 0.0, 0.0]
+        if d % 4 == 1:
+            params = [1e30, -1e30, 0.9, 0.3, 1.0, 0.0, 1.0, rng.uniform(-2, 2), 0.5]
+        if d % 3 == 0:  # the frame constructor: the hinge angle starts at 0, so the limit is about the rest pose
+            F = np.concatenate([np.array([1, 0, 0, 0, 0, -1, 0, 1, 0], np.float32), [-0.4, 0.0, 0.0]])
+            hs.append((idx, -1, None, None, None, None, F.astype(np.float32), params))
+        else:
+            hs.append((idx, -1, [-0.4, 0.0, 0.0], y, None, None, None, params))
+    wheel = b.shape(BOX, [0.35, 0.35, 0.08])
+    post = b.shape(BOX, [0.05, 0.05, 0.05])
+    for w in range(n_wheels):
+        pos = np.array([(w % 6) * 1.6 - 4.0, 0.45, 5.0 + (w // 6) * 1.2], np.float32)
+        pidx = len(b.bs)
+        b.body(post, IDENT, pos + np.array([0, 0, -0.3], np.float32), 0.0)
+        vel.append(np.zeros(3))
+        spin.append(np.zeros(3))
+        idx = len(b.bs)
+        b.body(wheel, IDENT, pos, 2.0)
+        vel.append(np.zeros(3))
+        spin.append(np.array([0.0, 0.0, rng.uniform(-4, 4)]))
+        params = [1e30, -1e30, 0.9, 0.3, 1.0, float(w % 3 == 0), 1.0, rng.uniform(-5, 5), 0.3]
+        hs.append((idx, pidx, [0, 0, 0], z, [0, 0, 0.3], z, None, params))
+    ext = 6.0
+    for _ in range(drops):
+        h = rng.uniform(0.15, 0.3, 3).astype(np.float32)
+        bx = b.shape(BOX, list(h))
+        p = np.array([rng.uniform(-ext, ext), rng.uniform(3.0, 8.0), rng.uniform(-7.0, 7.0)], np.float32)
+        b.body(bx, _yaw_basis(rng.uniform(0, 2 * np.pi, 1))[0], p, np.float32(8.0 * h.prod() * 300.0))
+        vel.append(np.zeros(3))
+        spin.append(np.zeros(3))
+    return b.done(dt=1 / 60, iterations=10, steps=300, hinges=hs, velocity=np.array(vel, np.float32),
+                  spin=np.array(spin, np.float32))
 
 
 def build_records(scene, builder, gravity=GRAVITY):

@@ -342,6 +342,7 @@ class World:
         fb = f32(frame_b).reshape(-1)
         self._hinge_params = None if params is None else f32(params).reshape(-1)
         lib().orc_world_add_hinge(self.h, len(a), a, b, fa, fb, _ptr(self._hinge_params))
+        self.nj = getattr(self, "nj", 0) + len(a)
 
     def hinge_angle(self, j):
         return float(lib().orc_world_hinge_angle(self.h, int(j)))

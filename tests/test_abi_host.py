@@ -138,3 +138,43 @@ def test_pair_record_check_rejects_each_bad_record(case, reason):
         keys[2] = (5 << 32) | 5
     with pytest.raises(jbgpu.JbgpuError, match=reason):
         jbgpu.check_pair_records(5, keys, man)
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]], np.float32)
+
+
+def test_hinge_frames_match_oracle():
+    """jbgpu_hinge_frames (host side, no device) builds the same frames as the oracle's restatement of the
+    HingeConstraint constructors, bit for bit: random transforms and axes, the axis equal to / opposite to A's x
+    axis (the constructor's two special cases), antiparallel A and B axes (the shortest arc's fallback), and the
+    frame forms."""
+    rng = np.random.default_rng(7)
+    for k in range(200):
+        R = _random_rotation(rng) if k % 4 else np.eye(3, dtype=np.float32)
+        tr = np.concatenate([R.reshape(-1), rng.uniform(-5, 5, 3)]).astype(np.float32)
+        ax = rng.normal(size=3).astype(np.float32)
+        ax /= np.linalg.norm(ax)
+        if k % 10 == 1:
+            ax = R[:, 0].copy()
+        elif k % 10 == 2:
+            ax = -R[:, 0]
+        bx = -ax if k % 10 == 3 else ax
+        pa, pb = rng.uniform(-1, 1, 3).astype(np.float32), rng.uniform(-1, 1, 3).astype(np.float32)
+        got = jbgpu.hinge_frames(tr, pa, ax, pb, bx)
+        want = po.hinge_frames(tr, pa, ax, pb, bx)
+        assert all(np.array_equal(g.view(np.uint32), w.view(np.uint32)) for g, w in zip(got, want)), k
+        got = jbgpu.hinge_frames(tr, pa, ax)
+        want = po.hinge_frames(tr, pa, ax)
+        assert all(np.array_equal(g.view(np.uint32), w.view(np.uint32)) for g, w in zip(got, want)), k
+        F = np.concatenate([_random_rotation(rng).reshape(-1), pa]).astype(np.float32)
+        G = np.concatenate([_random_rotation(rng).reshape(-1), pb]).astype(np.float32)
+        for fb in (G, None):
+            got = jbgpu.hinge_frames(tr, frame_a=F, frame_b=fb)
+            want = po.hinge_frames_from_frames(tr, F, fb)
+            assert all(np.array_equal(g.view(np.uint32), w.view(np.uint32)) for g, w in zip(got, want)), k

@@ -160,3 +160,54 @@ def test_add_point2point_argument_errors():
     ctx.step()
     assert len(ctx.joint_order()) == nj + 1
     ctx.close()
+
+
+def _hinge_rows(sc, bodies):
+    """The scene's hinges as (body_a, body_b, frame_a, frame_b, params) arrays; frames from the constructors'
+    host helper on body A's initial transform."""
+    A, B, FA, FB, P = [], [], [], [], []
+    free = [1e30, -1e30, 0.9, 0.3, 1.0, 0.0, 0.0, 0.0, 0.0]
+    for (a, b, pa, xa, pb, xb, F, params) in sc.meta["hinges"]:
+        tr = bodies[a, :12]
+        if F is not None:
+            fa, fb = jbgpu.hinge_frames(tr, frame_a=F)
+        elif pb is None:
+            fa, fb = jbgpu.hinge_frames(tr, pa, xa)
+        else:
+            fa, fb = jbgpu.hinge_frames(tr, pa, xa, pb, xb)
+        A.append(a), B.append(b), FA.append(fa), FB.append(fb), P.append(free if params is None else params)
+    return (np.array(A, np.int32), np.array(B, np.int32), np.array(FA), np.array(FB), np.array(P, np.float32))
+
+
+@pytest.mark.parametrize("deactivation", [False, True])
+def test_hinges_step_parity(deactivation):
+    """Hinged chains, doors (limits, motors, both constructor forms), motor wheels on static posts (some
+    angular-only) and falling boxes, with Point2Point constraints joining some of the falling boxes in pairs
+    (both constraint types in one colouring); 180 steps bit-exact, constraint impulses included."""
+    sc = scenes.hinges(seed=2)
+    shapes, verts, bodies = scenes.build_records(sc, jbgpu)
+    bodies = np.asarray(bodies, np.float32).reshape(-1, jbgpu.BODY_WORDS).copy()
+    bodies[:, 12:15] = sc.meta["velocity"]
+    bodies[:, 15:18] = sc.meta["spin"]
+    ctx = jbgpu.Context(num_iterations=10, deactivation_disabled=int(not deactivation))
+    ctx.world_create(shapes, verts, bodies)
+    orc = pyoracle.World(shapes, verts, bodies, iterations=10, order=jbgpu.ORDER_EXPLICIT,
+                         deactivation_disabled=not deactivation)
+    a, b, fa, fb, params = _hinge_rows(sc, bodies)
+    ctx.add_hinge(a, b, fa, fb, params)
+    orc.add_hinge(a, b, fa, fb, params)
+    drops = np.arange(len(bodies) - 40, len(bodies), 2, dtype=np.int32)
+    pa = np.tile(np.float32([0.3, 0.0, 0.0]), (len(drops), 1))
+    ctx.add_point2point(drops, drops + 1, pa, -pa)
+    orc.add_point2point(drops, drops + 1, pa, -pa)
+    nj = ctx.num_joints()
+    assert nj == len(a) + len(drops)
+    for step in range(180):
+        jorder = _step_and_compare(ctx, orc, f"hinges step {step}")
+        if not deactivation:
+            assert len(jorder) == nj
+        if step % 30 == 0:
+            st = ctx.stats()
+            print(f"[hinges deact={deactivation}] step {step}: manifolds {st['manifolds']} solved constraints "
+                  f"{len(jorder)}/{nj} max |impulse| {np.abs(ctx.joint_impulses()).max():.3f}", flush=True)
+    ctx.close()
