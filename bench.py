@@ -70,6 +70,8 @@ def main():
                     help="launch a marker copy right before and after the timed steps (for --summarize-trace)")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="summarise the kernels between the two window markers of a rocprofv3 kernel trace")
+    ap.add_argument("--world-blocks", default=None, metavar="BODIES,THREADS,PIPELINED",
+                    help="fix the batched solve's world -> workgroup mapping instead of autotuning it (e.g. 512,256,0)")
     ap.add_argument("--section", choices=("batched", "learned", "broadphase_1m"),
                     help="run only this secondary section and print its JSON (for profiling it alone)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -398,13 +400,23 @@ def batched(args):
     ctx.world_create(shapes, verts, bodies, world_ids=sc.world)
     warm = max(args.batched_steps // 2, 1)
     ctx.step(n=warm)
-    tune = autotune.world_blocks(ctx, log=log)
+    if args.world_blocks:  # fixed mapping (profiling: the profiler distorts the autotuner's timings)
+        cand = tuple(int(x) for x in args.world_blocks.split(","))
+        ctx.solver_set_world_blocks(*cand)
+        ctx.step()
+        tune = {"bodies_per_block": cand[0], "threads": cand[1], "pipelined": bool(cand[2]), "fixed": True, "steps": 1}
+    else:
+        tune = autotune.world_blocks(ctx, log=log)
     first = warm + tune["steps"] + 1
     snap = dict(bodies=ctx.bodies(), proxies=ctx.bp_proxies(), pairs=ctx.pairs())
-    ctx.timing(True)
+    ctx.timing(not args.trace_window)
+    if args.trace_window:
+        ctx.measure_copy(TRACE_MARK_BYTES, 1)  # window markers for --summarize-trace
     t0 = time.perf_counter()
     ctx.step(n=args.batched_steps)
     secs = time.perf_counter() - t0
+    if args.trace_window:
+        ctx.measure_copy(TRACE_MARK_BYTES, 1)
     st = ctx.stats()
     regions, nsteps = ctx.timing_get()
     ctx.close()
@@ -481,11 +493,16 @@ def broadphase_1m(args):
             ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
             ctx.bp_calculate_overlapping_pairs()
         pairs = 0
+        mark = args.trace_window and name == "dbvt"  # window markers for --summarize-trace (DBVT frames)
+        if mark:
+            ctx.measure_copy(TRACE_MARK_BYTES, 1)
         t0 = time.perf_counter()
         for k in range(3, args.bp_frames):
             ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
             pairs += ctx.bp_calculate_overlapping_pairs()
         secs = time.perf_counter() - t0
+        if mark:
+            ctx.measure_copy(TRACE_MARK_BYTES, 1)
         steps = args.bp_frames - 3
         ctx.buffer_free(dev)
         ctx.close()

@@ -63,6 +63,7 @@ typedef struct jbgpu_stats {
   int64_t huge_proxies; /* proxies handled outside the BVH (e.g. planes) */
   int64_t rows;         /* normal rows (contacts with distance <= 0) of the last solve */
   int64_t epa_retries;  /* EPA jobs of the last pass that outgrew the first-pass workspace and ran again */
+  int64_t epa_huge;     /* ... of those, the ones that also outgrew the large LDS workspace (global-memory tier) */
 } jbgpu_stats;
 
 /* ---- context ---------------------------------------------------------------- */

@@ -91,6 +91,12 @@ constexpr int kEpaSmallVerts = 96;
 constexpr int kEpaSmallSlotWords = kEpaSmallFaces * kEpaFaceWords + kEpaSmallVerts * 6 + kEpaHe * 4 + 8;
 constexpr int kEpaSmallGrid = 2048;  // first-pass EPA blocks (eight per CU)
 constexpr int kEpaGrid = 512;  // EPA blocks (LDS workspace limits them to two per CU)
+// last EPA tier (jobs that outgrow the large workspace): workspaces in global memory, one per block
+constexpr int kEpaHugeFaces = 16384;
+constexpr int kEpaHugeStack = 4096;
+constexpr int kEpaHugeGrid = 16;
+constexpr int kEpaHugeSlotWords = kEpaHugeFaces * kEpaFaceWords + kEpaVerts * 6 + kEpaHe * 4 + 8;
+constexpr int kEpaHugeAuxInts = 64 + 2 * kEpaHugeStack + 5 * 6;
 constexpr int kTimerRegions = 16;
 
 // Constraint record: type (0 Point2PointConstraint, 1 HingeConstraint), body A, body B (-1: the fixed body), 0,
@@ -113,7 +119,7 @@ constexpr int kManifoldRowWords = 4 * kContactRowWords + 4;  // + [valid mask, b
 
 struct Stats {
   int64_t pairs = 0, manifoldsActive = 0, contacts = 0, rows = 0, colors = 0, epaCalls = 0, epaFails = 0;
-  int64_t epaRetries = 0;
+  int64_t epaRetries = 0, epaHuge = 0;
   int64_t iterations = 0;
   float residual = 0;
   int64_t huge = 0;
@@ -215,7 +221,7 @@ struct Ctx {
   DBuf<int> typeKey, typeKeyAlt, perm, permAlt;
   DBuf<unsigned long long> npStat;           // EPA queue length, failures, workspace overflows
   DBuf<int> binRange;                        // per pair type: first sorted position and count
-  unsigned long long npStatHost[4] = {0, 0, 0, 0};  // their copy, valid after the step's final synchronisation
+  unsigned long long npStatHost[5] = {0, 0, 0, 0, 0};  // their copy, valid after the step's final synchronisation
   bool npStatPending = false;
   bool epaOverflow = false;  // the last narrowphase pass overflowed an EPA workspace
   std::vector<int> shapeTypeHost;  // host copies: type per shape and per body (canonical manifold headers)
@@ -264,6 +270,9 @@ struct Ctx {
   DBuf<float> candC;  // contact candidates npairs*8
   DBuf<int> epaQueue;
   DBuf<int> epaRetry;  // EPA jobs that outgrew the first-pass workspace
+  DBuf<int> epaRetryHuge;    // ... and then the large one
+  DBuf<float> epaHugeWork;   // the last tier's workspaces (kEpaHugeGrid x kEpaHugeSlotWords)
+  DBuf<int> epaHugeAux;
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
   // solver
   DBuf<float> rows;

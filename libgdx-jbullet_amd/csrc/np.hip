@@ -635,7 +635,8 @@ struct EpaSolver {
   // shared by the wave's lanes (all of them run the job in lockstep on identical values): LDS
   int* table;     // 64 hash heads
   Mkv* simplex;   // 5 vertices
-  int* horizon;   // BuildHorizon stack: 64 faces, then 64 edges
+  int* horizon;   // BuildHorizon stack: capStack faces, then capStack edges
+  int capStack;
   V3 ray;
   int order, iterations;
   bool failed;
@@ -897,7 +898,7 @@ struct EpaSolver {
   __device__ __forceinline__ int BuildHorizon(int markid, int w, int f0, int e0, int* cf, int* ff) {
     const int mod3[5] = {0, 1, 2, 0, 1};
     int* stkF = horizon;
-    int* stkE = horizon + 64;
+    int* stkE = horizon + capStack;
     int sp = 0;
     int ne = 0;
     stkF[sp] = f0;
@@ -927,7 +928,7 @@ struct EpaSolver {
         int e2 = mod3[e + 2];
         Detach(f);
         W.fi(f, 13) = markid;
-        if (sp + 2 > 64) {
+        if (sp + 2 > capStack) {
           W.overflow = true;
           return ne;
         }
@@ -1050,11 +1051,12 @@ struct EpaSolver {
 };
 
 // one EPA job (pair i), run by all lanes of a wave on identical values: the polytope workspace (faces,
-// vertices, hash entries) and the solver's arrays are in LDS, the face search and hull support maps are
-// spread over the lanes; the GJK partial result is merged as in GjkPairDetector.java:263-303
-// retry != nullptr: a job that overflows this workspace writes nothing and is queued for the large one instead
-// (the algorithm does not depend on the capacities until it overflows, so the rerun gives the same result)
-template <int capFace, int capVtx>
+// vertices, hash entries) and the solver's arrays (hash heads, horizon stack of capStack entries, simplex: aux) are
+// in LDS (in global memory for the last tier), the face search and hull support maps are spread over the lanes;
+// the GJK partial result is merged as in GjkPairDetector.java:263-303
+// retry != nullptr: a job that overflows this workspace writes nothing and is queued for the next larger one
+// instead (the algorithm does not depend on the capacities until it overflows, so the rerun gives the same result)
+template <int capFace, int capVtx, int capStack = 64>
 __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                        const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
                        const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot,
@@ -1063,7 +1065,8 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
   EpaSolver S;
   S.table = aux;
   S.horizon = aux + 64;
-  S.simplex = reinterpret_cast<Mkv*>(aux + 192);
+  S.capStack = capStack;
+  S.simplex = reinterpret_cast<Mkv*>(aux + 64 + 2 * capStack);
   S.sh[0] = bodyShape(shapes, props, a);
   S.sh[1] = bodyShape(shapes, props, b);
   Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
@@ -1143,13 +1146,30 @@ __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict_
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                                            const float* __restrict__ verts, const float* __restrict__ props,
                                            const float* __restrict__ tr, const float* __restrict__ epaState,
-                                           float* cand, unsigned long long* fails) {
+                                           float* cand, unsigned long long* fails, int* retry,
+                                           unsigned long long* retryCount) {
   __shared__ float slot[kEpaSlotWords];
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
-    epaJob<kEpaFaces, kEpaVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, nullptr,
-                                 nullptr);
+    epaJob<kEpaFaces, kEpaVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, retry,
+                                 retryCount);
+}
+// the last tier: the rare jobs that outgrow the large LDS workspace (long EPA runs on detailed hulls: up to 256
+// iterations, each adding a horizon's worth of faces; or a horizon deeper than 64), rerun with a workspace in
+// global memory, one per block: kEpaHugeFaces faces and a kEpaHugeStack-entry horizon stack
+__global__ __launch_bounds__(64) void kEpaHuge(const unsigned long long* __restrict__ count,
+                                               const int* __restrict__ queue, const uint64_t* __restrict__ keys,
+                                               const float* __restrict__ shapes, const float* __restrict__ verts,
+                                               const float* __restrict__ props, const float* __restrict__ tr,
+                                               const float* __restrict__ epaState, float* cand,
+                                               unsigned long long* fails, float* work, int* auxWork) {
+  float* slot = work + (size_t)kEpaHugeSlotWords * blockIdx.x;
+  int* aux = auxWork + (size_t)kEpaHugeAuxInts * blockIdx.x;
+  const int jobs = (int)*count;
+  for (int k = blockIdx.x; k < jobs; k += gridDim.x)
+    epaJob<kEpaHugeFaces, kEpaVerts, kEpaHugeStack>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails,
+                                                    slot, aux, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------ manifold update
@@ -1694,9 +1714,10 @@ void npDispatchAll(Ctx& c) {
   hipLaunchKernelGGL(kBinRanges, dim3(1), dim3(64), 0, s, n, (const int*)starts, range);
   tEnd(c, T_BIN);
   const int* list = vv.Current();
-  // EPA queue length, failures, workspace overflows: read back with the step's final synchronisation
-  c.npStat.ensure(4);
-  JB_CHECK(hipMemsetAsync(c.npStat.p, 0, 4 * sizeof(unsigned long long), s));
+  // EPA queue length, failures, workspace overflows, large-tier and last-tier queue lengths: read back with the
+  // step's final synchronisation
+  c.npStat.ensure(5);
+  JB_CHECK(hipMemsetAsync(c.npStat.p, 0, 5 * sizeof(unsigned long long), s));
   hipLaunchKernelGGL(kSphereSphere, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_SPHERE_SPHERE,
                      list, keys, c.shapes.p, c.props.p, c.tr.p, c.candC.p);
   hipLaunchKernelGGL(kConvexPlane, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_CONVEX_PLANE,
@@ -1711,9 +1732,15 @@ void npDispatchAll(Ctx& c) {
   hipLaunchKernelGGL(kEpaSmall, dim3(kEpaSmallGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
                      (const int*)c.epaQueue.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p,
                      c.npStat.p + 1, c.epaRetry.p, c.npStat.p + 3);
+  c.epaRetryHuge.ensure(n);
+  c.epaHugeWork.ensure((size_t)kEpaHugeGrid * kEpaHugeSlotWords);
+  c.epaHugeAux.ensure((size_t)kEpaHugeGrid * kEpaHugeAuxInts);
   hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 3),
                      (const int*)c.epaRetry.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
-                     c.candC.p, c.npStat.p + 1);
+                     c.candC.p, c.npStat.p + 1, c.epaRetryHuge.p, c.npStat.p + 4);
+  hipLaunchKernelGGL(kEpaHuge, dim3(kEpaHugeGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 4),
+                     (const int*)c.epaRetryHuge.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
+                     c.candC.p, c.npStat.p + 1, c.epaHugeWork.p, c.epaHugeAux.p);
   tEnd(c, T_EPA);
   tBegin(c, T_MANIFOLD);
   hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
@@ -1729,6 +1756,7 @@ void npCollectStats(Ctx& c) {  // after the step's final synchronisation
   c.st.epaCalls = (int64_t)c.npStatHost[0];
   c.st.epaFails = (int64_t)c.npStatHost[1];
   c.st.epaRetries = (int64_t)c.npStatHost[3];
+  c.st.epaHuge = (int64_t)c.npStatHost[4];
   c.epaOverflow = c.npStatHost[2] != 0;  // this pass only: the next step starts clean
 }
 

@@ -1193,6 +1193,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
     out->huge_proxies = s.huge;
     out->rows = s.rows;
     out->epa_retries = s.epaRetries;
+    out->epa_huge = s.epaHuge;
   });
 }
 

@@ -42,7 +42,7 @@ class Stats(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64), ("manifolds", C.c_int64), ("colors", C.c_int64),
                 ("epa_calls", C.c_int64), ("epa_fails", C.c_int64), ("iterations", C.c_int64),
                 ("residual", C.c_float), ("huge_proxies", C.c_int64), ("rows", C.c_int64),
-                ("epa_retries", C.c_int64)]
+                ("epa_retries", C.c_int64), ("epa_huge", C.c_int64)]
 
 
 SYMBOLS = {

@@ -323,6 +323,32 @@ def test_epa_workspace_retry_step_parity():
     ctx.close()
 
 
+def test_epa_last_tier_step_parity():
+    """EPA jobs that outgrow the large LDS workspace too (a long run on detailed hulls: more than 1024 faces or a
+    horizon deeper than 64) are rerun with a workspace in global memory.  The 100k pile meets one in its first 400
+    steps: found on a device-only run, the same step of a second run is replayed by the oracle from the device
+    state before it and must agree bit for bit (the device run is deterministic: same inputs, same bits)."""
+    sc = scenes.mixed_pile(seed=1, n=100_000)
+    ctx, orc = _world_pair(sc, iterations=20)
+    found = None
+    for step in range(1, 401):
+        ctx.step()
+        st = ctx.stats()
+        if st["epa_huge"] > 0:
+            found = step
+            print(f"[epa last tier] step {step}: {st['epa_calls']} EPA jobs, {st['epa_retries']} rerun, "
+                  f"{st['epa_huge']} in the last tier", flush=True)
+            break
+    ctx.close()
+    assert found is not None, "no EPA job reached the last workspace tier in 400 steps"
+    ctx, orc = _world_pair(sc, iterations=20)
+    ctx.step(n=found - 1)
+    _sync_and_step(ctx, orc)
+    assert ctx.stats()["epa_huge"] > 0
+    _compare_world(ctx, orc, f"last-tier step {found}")
+    ctx.close()
+
+
 def _rays(rng, n, lo, hi, vertical_share=0.3):
     """Ray segments between random points of the box [lo, hi]; a share of them vertical (axis-parallel)."""
     lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
