@@ -206,6 +206,13 @@ int jbgpu_world_joint_order(jbgpu_ctx* ctx, int32_t* order, int32_t cap, int32_t
 int jbgpu_np_dispatch(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const float* manifolds_in,
                       float* manifolds_out);
 
+/* EPA workspace tiers (no reference counterpart: the reference's GjkEpaSolver grows its lists): every EPA job
+ * runs first with a small LDS workspace, a job that outgrows it reruns with a large LDS one, and one that
+ * outgrows that too with a global-memory one (16,384 faces).  These set the face capacities the two LDS tiers
+ * use (defaults and maxima 192 and 1024; at least 16): they move work between the tiers and never change a
+ * result.  JBGPU_ERR_ARG outside the ranges. */
+int jbgpu_np_set_epa_capacity(jbgpu_ctx* ctx, int32_t small_faces, int32_t large_faces);
+
 /* ---- ConstraintSolver -------------------------------------------------------------- */
 /* The solver seam as DiscreteDynamicsWorld.solveConstraints drives it (dynamics/DiscreteDynamicsWorld.java:
  * 636-643): prepareSolve(numBodies, numManifolds) once (ConstraintSolver.java:41), solveGroup once per awake

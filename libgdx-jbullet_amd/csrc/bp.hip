@@ -15,7 +15,6 @@
 // or the quantised edge test), the group/mask filter
 // (HashedOverlappingPairCache.java:179-188) and the world-id filter, so the
 // sorted (uid_lo, uid_hi) list equals the reference's overlapping-pair set.
-#include <hipcub/hipcub.hpp>
 
 #include "internal.h"
 #include "shapes.h"
@@ -1002,22 +1001,15 @@ void rayBuildTree(Ctx& c) {
                      part);
   hipLaunchKernelGGL(kProxyStatsFinal, dim3(1), dim3(256), 0, s, nparts, (const unsigned int*)part, bounds, stats);
   int* hugeCount = (int*)(c.counter.p + 7);
-  size_t tbSel = 0;
-  hipcub::CountingInputIterator<int> ids(0);
-  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbSel, ids, (const int*)t.isHuge.p, t.hugeList.p, hugeCount, n, s));
-  c.cubTmp.ensure(tbSel);
-  JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbSel, ids, (const int*)t.isHuge.p, t.hugeList.p, hugeCount, n, s));
+  primSelectIndex(c, (const int*)t.isHuge.p, n, t.hugeList.p, hugeCount);
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
                      (const int*)c.world.p, (const unsigned int*)bounds, t.morton.p, t.order.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
-  size_t tmpBytes = 0;
-  hipcub::DoubleBuffer<uint64_t> dk(t.morton.p, t.mortonAlt.p);
-  hipcub::DoubleBuffer<int> dv(t.order.p, t.orderAlt.p);
+  Db<uint64_t> dk(t.morton.p, t.mortonAlt.p);
+  Db<int> dv(t.order.p, t.orderAlt.p);
   const int keyBits = 32 + c.worldBits;
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, keyBits, s));
-  c.cubTmp.ensure(tmpBytes);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, keyBits, s));
+  primSortPairs(c, dk, dv, n, 0, keyBits);
   JB_CHECK(hipStreamSynchronize(s));
   t.nHuge = hcount;
   t.nn = n - hcount;
@@ -1030,12 +1022,9 @@ void rayBuildTree(Ctx& c) {
     hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)t.hugeList.p,
                        (const int*)c.world.p, t.hugeWorld.p);
     if (c.worldBits > 0) {
-      size_t tb = 0;
-      hipcub::DoubleBuffer<int> hk(t.hugeWorld.p, t.hugeWorldAlt.p);
-      hipcub::DoubleBuffer<int> hv(t.hugeList.p, t.hugeListAlt.p);
-      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, t.nHuge, 0, c.worldBits, s));
-      c.cubTmp.ensure(tb);
-      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, t.nHuge, 0, c.worldBits, s));
+      Db<int> hk(t.hugeWorld.p, t.hugeWorldAlt.p);
+      Db<int> hv(t.hugeList.p, t.hugeListAlt.p);
+      primSortPairs(c, hk, hv, t.nHuge, 0, c.worldBits);
       if (hk.Current() != t.hugeWorld.p)
         JB_CHECK(hipMemcpyAsync(t.hugeWorld.p, hk.Current(), t.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
       if (hv.Current() != t.hugeList.p)
@@ -1099,24 +1088,16 @@ void bpFindPairs(Ctx& c) {
   hipLaunchKernelGGL(kMarkLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, kLargeCap,
                      (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
   int* hugeCount = (int*)(c.counter.p + 1);
-  size_t tbSel = 0;
-  hipcub::CountingInputIterator<int> ids(0);
-  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbSel, ids, (const int*)c.isHuge.p, c.hugeList.p, hugeCount, n, s));
-  c.cubTmp.ensure(tbSel);
-  JB_CHECK(
-      hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbSel, ids, (const int*)c.isHuge.p, c.hugeList.p, hugeCount, n, s));
+  primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, hugeCount);
   // Morton codes (huge proxies carry all ones and sort last)
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
                      (const unsigned int*)bounds, c.morton.p, c.order.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
-  size_t tmpBytes = 0;
-  hipcub::DoubleBuffer<uint64_t> dk(c.morton.p, c.mortonAlt.p);
-  hipcub::DoubleBuffer<int> dv(c.order.p, c.orderAlt.p);
+  Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
+  Db<int> dv(c.order.p, c.orderAlt.p);
   const int keyBits = 32 + c.worldBits;  // one world: the 32-bit Morton sort
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, dk, dv, n, 0, keyBits, s));
-  c.cubTmp.ensure(tmpBytes);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tmpBytes, dk, dv, n, 0, keyBits, s));
+  primSortPairs(c, dk, dv, n, 0, keyBits);
   JB_CHECK(hipStreamSynchronize(s));
   c.nHuge = hcount;
   c.st.huge = c.nHuge;
@@ -1128,12 +1109,9 @@ void bpFindPairs(Ctx& c) {
     hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)c.hugeList.p,
                        (const int*)c.world.p, c.hugeWorld.p);
     if (c.worldBits > 0) {
-      size_t tb = 0;
-      hipcub::DoubleBuffer<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
-      hipcub::DoubleBuffer<int> hv(c.hugeList.p, c.hugeListAlt.p);
-      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hv, c.nHuge, 0, c.worldBits, s));
-      c.cubTmp.ensure(tb);
-      JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, hk, hv, c.nHuge, 0, c.worldBits, s));
+      Db<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
+      Db<int> hv(c.hugeList.p, c.hugeListAlt.p);
+      primSortPairs(c, hk, hv, c.nHuge, 0, c.worldBits);
       if (hk.Current() != c.hugeWorld.p)
         JB_CHECK(hipMemcpyAsync(c.hugeWorld.p, hk.Current(), c.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
       if (hv.Current() != c.hugeList.p)
@@ -1202,11 +1180,8 @@ void bpFindPairs(Ctx& c) {
   // sort the packed keys (2*bits significant bits)
   tBegin(c, T_KEYSORT);
   if (c.npairs > 1) {
-    size_t tb = 0;
-    hipcub::DoubleBuffer<uint64_t> kk(c.rawKeys.p, c.rawKeysAlt.p);
-    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, kk, (int)c.npairs, 0, 2 * bits, s));
-    c.cubTmp.ensure(tb);
-    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(c.cubTmp.p, tb, kk, (int)c.npairs, 0, 2 * bits, s));
+    Db<uint64_t> kk(c.rawKeys.p, c.rawKeysAlt.p);
+    primSortKeys(c, kk, (int)c.npairs, 0, 2 * bits);
     if (kk.Current() != c.rawKeys.p)
       JB_CHECK(hipMemcpyAsync(c.rawKeys.p, kk.Current(), c.npairs * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
   }
@@ -1246,10 +1221,7 @@ static int64_t compactKeys(Ctx& c, uint64_t* keys, int64_t n, const int* flag) {
   hipStream_t s = c.stream;
   c.deltaKeys.ensure(n);
   int* cnt = (int*)(c.counter.p + 7);
-  size_t tb = 0;
-  JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, keys, flag, c.deltaKeys.p, cnt, (int)n, s));
-  c.cubTmp.ensure(tb);
-  JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, keys, flag, c.deltaKeys.p, cnt, (int)n, s));
+  primSelectValues<uint64_t>(c, keys, flag, (int)n, c.deltaKeys.p, cnt);
   int k = 0;
   JB_CHECK(hipMemcpyAsync(&k, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
   JB_CHECK(hipStreamSynchronize(s));
@@ -1313,11 +1285,7 @@ void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, ui
       if (m > 0) {
         hipLaunchKernelGGL(kKeyAbsent, dim3(gridFor(n)), dim3(256), 0, s, n, a, m, b, c.deltaFlag.p);
         int* dc = (int*)(c.counter.p + 7);
-        size_t tb = 0;
-        JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, a, (const int*)c.deltaFlag.p, c.deltaKeys.p, dc, (int)n, s));
-        c.cubTmp.ensure(tb);
-        JB_CHECK(
-            hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, a, (const int*)c.deltaFlag.p, c.deltaKeys.p, dc, (int)n, s));
+        primSelectValues<uint64_t>(c, a, (const int*)c.deltaFlag.p, (int)n, c.deltaKeys.p, dc);
         int kk = 0;
         JB_CHECK(hipMemcpyAsync(&kk, dc, sizeof(int), hipMemcpyDeviceToHost, s));
         JB_CHECK(hipStreamSynchronize(s));

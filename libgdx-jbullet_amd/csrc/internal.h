@@ -210,7 +210,8 @@ struct Ctx {
   DBuf<uint64_t> rawKeys, rawKeysAlt;
   DBuf<uint64_t> pairStage;  // per traversal block: staged pair keys
   DBuf<unsigned long long> counter;  // misc device counters
-  DBuf<uint8_t> cubTmp;
+  DBuf<int> primTmp;  // scratch of the device-wide scans, compactions and sorts (prim.hip)
+  DBuf<int> primWide;
   int64_t pairCap = 0;
   // pairs and manifolds (double buffered for the remap)
   int cur = 0;
@@ -271,6 +272,7 @@ struct Ctx {
   DBuf<int> epaQueue;
   DBuf<int> epaRetry;  // EPA jobs that outgrew the first-pass workspace
   DBuf<int> epaRetryHuge;    // ... and then the large one
+  int epaSmallFaces = kEpaSmallFaces, epaLargeFaces = kEpaFaces;  // face capacities the two LDS tiers use
   DBuf<float> epaHugeWork;   // the last tier's workspaces (kEpaHugeGrid x kEpaHugeSlotWords)
   DBuf<int> epaHugeAux;
   DBuf<float> epaState;  // npairs*12 GJK state for EPA merge
@@ -426,5 +428,27 @@ void primSelectFlagged(hipStream_t s, const int* flag, const int* vals, int* out
 template <class K>
 bool primRadixSort(hipStream_t s, K* k0, int* v0, K* k1, int* v1, const int* count, int hostCount, int cap, int bitLo,
                    int bitHi, int* scratch);
+// The step's forms (host-known count, scratch held by the context, work on the context stream).  A sort works on a
+// pair of buffers and leaves its result in either: cur names it afterwards, alt the other one.
+template <class T>
+struct Db {
+  T* cur;
+  T* alt;
+  Db(T* a, T* b) : cur(a), alt(b) {}
+  T* Current() const { return cur; }
+};
+// stable sort of n keys (with int values) by bits [bitLo, bitHi); keys compare as unsigned words
+template <class K>
+void primSortPairs(Ctx& c, Db<K>& k, Db<int>& v, int n, int bitLo, int bitHi);
+template <class K>
+void primSortKeys(Ctx& c, Db<K>& k, int n, int bitLo, int bitHi);
+// out[0, m) = the i < n with flag[i] != 0, in index order; *count (device) = m
+void primSelectIndex(Ctx& c, const int* flag, int n, int* out, int* count);
+// out[0, m) = vals[i] for the i < n with flag[i] != 0, in index order; *count (device) = m
+template <class T>
+void primSelectValues(Ctx& c, const T* vals, const int* flag, int n, T* out, int* count);
+// out[i] = in[0] + ... + in[i - 1], i < n (the sums must fit in an int; primScanWide writes them as int64)
+void primScan(Ctx& c, const int* in, int* out, int n);
+void primScanWide(Ctx& c, const int* in, int64_t* out, int n);
 
 }  // namespace jbg

@@ -15,7 +15,6 @@
 // a queue of the pairs that need it, each with a face/vertex pool in a global
 // workspace slot.  Candidates (valid, normal on B, point on B, depth) are
 // written per pair and folded into the manifold by a final per-pair kernel.
-#include <hipcub/hipcub.hpp>
 
 #include "internal.h"
 #include "shapes.h"
@@ -1056,11 +1055,13 @@ struct EpaSolver {
 // the GJK partial result is merged as in GjkPairDetector.java:263-303
 // retry != nullptr: a job that overflows this workspace writes nothing and is queued for the next larger one
 // instead (the algorithm does not depend on the capacities until it overflows, so the rerun gives the same result)
+// faceLimit (<= capFace) lowers the face capacity this tier uses; a job that needs more moves to the next tier,
+// so the limits shape the work between tiers and never the results
 template <int capFace, int capVtx, int capStack = 64>
 __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                        const float* __restrict__ verts, const float* __restrict__ props, const float* __restrict__ tr,
                        const float* __restrict__ epaState, float* cand, unsigned long long* fails, float* slot,
-                       int* aux, int* retry, unsigned long long* retryCount) {
+                       int* aux, int* retry, unsigned long long* retryCount, int faceLimit = capFace) {
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
   EpaSolver S;
   S.table = aux;
@@ -1081,7 +1082,7 @@ __device__ void epaJob(int i, const uint64_t* __restrict__ keys, const float* __
   S.W.vtx = slot + capFace * kEpaFaceWords;
   S.W.he = S.W.vtx + capVtx * 6;
   S.W.nface = S.W.nvtx = S.W.nhe = 0;
-  S.W.capFace = capFace;
+  S.W.capFace = faceLimit < capFace ? faceLimit : capFace;
   S.W.capVtx = capVtx;
   S.W.overflow = false;
   const float* es = epaState + 12 * i;
@@ -1134,26 +1135,26 @@ __global__ __launch_bounds__(64) void kEpaSmall(const unsigned long long* __rest
                                                 const float* __restrict__ props, const float* __restrict__ tr,
                                                 const float* __restrict__ epaState, float* cand,
                                                 unsigned long long* fails, int* retry,
-                                                unsigned long long* retryCount) {
+                                                unsigned long long* retryCount, int faceLimit) {
   __shared__ float slot[kEpaSmallSlotWords];
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
     epaJob<kEpaSmallFaces, kEpaSmallVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux,
-                                           retry, retryCount);
+                                           retry, retryCount, faceLimit);
 }
 __global__ __launch_bounds__(64) void kEpa(const unsigned long long* __restrict__ count, const int* __restrict__ queue,
                                            const uint64_t* __restrict__ keys, const float* __restrict__ shapes,
                                            const float* __restrict__ verts, const float* __restrict__ props,
                                            const float* __restrict__ tr, const float* __restrict__ epaState,
                                            float* cand, unsigned long long* fails, int* retry,
-                                           unsigned long long* retryCount) {
+                                           unsigned long long* retryCount, int faceLimit) {
   __shared__ float slot[kEpaSlotWords];
   __shared__ int aux[192 + 5 * 6];  // hash heads, horizon stack, simplex
   const int jobs = (int)*count;
   for (int k = blockIdx.x; k < jobs; k += gridDim.x)
     epaJob<kEpaFaces, kEpaVerts>(queue[k], keys, shapes, verts, props, tr, epaState, cand, fails, slot, aux, retry,
-                                 retryCount);
+                                 retryCount, faceLimit);
 }
 // the last tier: the rare jobs that outgrow the large LDS workspace (long EPA runs on detailed hulls: up to 256
 // iterations, each adding a horizon's worth of faces; or a horizon deeper than 64), rerun with a workspace in
@@ -1699,12 +1700,9 @@ void npDispatchAll(Ctx& c) {
   tBegin(c, T_BIN);
   hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p,
                      (const float*)c.shapes.p, (const float*)c.props.p, c.typeKey.p, c.perm.p);
-  size_t tb = 0;
-  hipcub::DoubleBuffer<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
+  Db<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
   const int keyBits = 2 + kBinClassBits;
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kk, vv, (int)n, 0, keyBits, s));
-  c.cubTmp.ensure(tb);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, kk, vv, (int)n, 0, keyBits, s));
+  primSortPairs(c, kk, vv, (int)n, 0, keyBits);
   int* starts = (int*)(c.counter.p + 4);  // 4 ints
   JB_CHECK(hipMemsetAsync(starts, 0xff, 4 * sizeof(int), s));
   hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)kk.Current(), starts);
@@ -1731,13 +1729,13 @@ void npDispatchAll(Ctx& c) {
   c.epaRetry.ensure(n);
   hipLaunchKernelGGL(kEpaSmall, dim3(kEpaSmallGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
                      (const int*)c.epaQueue.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p,
-                     c.npStat.p + 1, c.epaRetry.p, c.npStat.p + 3);
+                     c.npStat.p + 1, c.epaRetry.p, c.npStat.p + 3, c.epaSmallFaces);
   c.epaRetryHuge.ensure(n);
   c.epaHugeWork.ensure((size_t)kEpaHugeGrid * kEpaHugeSlotWords);
   c.epaHugeAux.ensure((size_t)kEpaHugeGrid * kEpaHugeAuxInts);
   hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 3),
                      (const int*)c.epaRetry.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
-                     c.candC.p, c.npStat.p + 1, c.epaRetryHuge.p, c.npStat.p + 4);
+                     c.candC.p, c.npStat.p + 1, c.epaRetryHuge.p, c.npStat.p + 4, c.epaLargeFaces);
   hipLaunchKernelGGL(kEpaHuge, dim3(kEpaHugeGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 4),
                      (const int*)c.epaRetryHuge.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
                      c.candC.p, c.npStat.p + 1, c.epaHugeWork.p, c.epaHugeAux.p);
@@ -1776,10 +1774,7 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
                      (const int*)t.hugeList.p, (const float*)t.aabb.p, (const int*)c.filt.p,
                      (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)nullptr, (uint64_t*)nullptr, err);
   JB_CHECK(hipMemsetAsync(t.count.p + n, 0, sizeof(int), s));
-  size_t tb = 0;
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, t.count.p, t.offset.p, n + 1, s));
-  c.cubTmp.ensure(tb);
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tb, t.count.p, t.offset.p, n + 1, s));
+  primScanWide(c, t.count.p, t.offset.p, n + 1);
   int64_t total = 0;
   int errh = 0;
   JB_CHECK(hipMemcpyAsync(&total, t.offset.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
@@ -1799,11 +1794,9 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
     // world order inside every ray's segment: sort by (ray, object)
     int bits = 32;
     while (bits < 64 && (1ll << (bits - 32)) < (int64_t)n) bits++;
-    hipcub::DoubleBuffer<uint64_t> kk(t.keys.p, t.keysAlt.p);
-    size_t ts = 0;
-    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, ts, kk, (int)total, 0, bits, s));
-    c.cubTmp.ensure(ts);
-    JB_CHECK(hipcub::DeviceRadixSort::SortKeys(c.cubTmp.p, ts, kk, (int)total, 0, bits, s));
+    if (total >= (int64_t)1 << 31) throw std::length_error("too many ray candidates in one call");
+    Db<uint64_t> kk(t.keys.p, t.keysAlt.p);
+    primSortKeys(c, kk, (int)total, 0, bits);
     hipLaunchKernelGGL(kRayCast, dim3(gridFor(total, kGjkBlock)), dim3(kGjkBlock), 0, s, total,
                        (const uint64_t*)kk.Current(), dFrom, dTo, (const float*)c.shapes.p, (const float*)c.props.p,
                        (const float*)c.tr.p, (const float*)c.verts.p, t.rec.p);

@@ -9,6 +9,8 @@
 // tile ranks its keys chunk by chunk (256 keys per chunk, in thread order): lanes of a wave holding the same
 // digit are found with eight ballots, the waves' counts are combined through LDS, and a per-digit running
 // offset carries the order from chunk to chunk.  Equal digits keep their input order, so the sort is stable.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace jbg {
@@ -16,8 +18,8 @@ namespace jbg {
 namespace {
 
 constexpr int kPT = 256;                // threads per block
-constexpr int kPerThread = 16;          // items per thread and tile
-constexpr int kTile = kPT * kPerThread; // items per tile (4096)
+constexpr int kPerThread = 4;           // items per thread and tile
+constexpr int kTile = kPT * kPerThread; // items per tile (1024: a 100k-key sort still spreads over ~100 blocks)
 constexpr int kRadix = 256;
 constexpr int kScanTop = 1024;          // threads of the single-block scan of the tile sums
 
@@ -113,10 +115,10 @@ __global__ __launch_bounds__(kPT) void kScanTiles(const int* __restrict__ in, co
 
 // ---- compaction ----------------------------------------------------------------------------------------------
 // selected[ tileOffset + rank ] = i (or vals[i]) for every i < n with flag[i] != 0, in index order
-template <bool Vals>
-__global__ __launch_bounds__(kPT) void kSelectTiles(const int* __restrict__ flag, const int* __restrict__ vals,
+template <class T, bool Vals>
+__global__ __launch_bounds__(kPT) void kSelectTiles(const int* __restrict__ flag, const T* __restrict__ vals,
                                                     const int* __restrict__ count, int hostCount, int cap,
-                                                    const int* __restrict__ tileOffset, int* __restrict__ out) {
+                                                    const int* __restrict__ tileOffset, T* __restrict__ out) {
   __shared__ int tile[kTile];
   __shared__ int waveSums[kPT / 64];
   __shared__ int total;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(kPT) void kSelectTiles(const int* __restrict__ flag
   __syncthreads();
   for (int k = threadIdx.x; k < kTile && base + k < n; k += kPT) {
     const int pos = tile[k];
-    if (pos >= 0) out[pos] = Vals ? vals[base + k] : base + k;
+    if (pos >= 0) out[pos] = Vals ? vals[base + k] : (T)(base + k);
   }
 }
 
@@ -157,21 +159,23 @@ __global__ __launch_bounds__(kPT) void kFlagTileSums(const int* __restrict__ fla
 }
 
 // ---- radix sort ----------------------------------------------------------------------------------------------
+// the digit of a key: bits [shift, shift + 8) under mask (the last digit of a bit range may be narrower)
 template <class K>
-__device__ __forceinline__ int digitOf(K k, int shift) {
-  return (int)((k >> shift) & (K)(kRadix - 1));
+__device__ __forceinline__ int digitOf(K k, int shift, int mask) {
+  return (int)((k >> shift) & (K)mask);
 }
 
 // hist[d * nTiles + tile] = keys of the tile with digit d
 template <class K>
 __global__ __launch_bounds__(kPT) void kRadixHist(const K* __restrict__ keys, const int* __restrict__ count,
-                                                  int hostCount, int cap, int shift, int* __restrict__ hist) {
+                                                  int hostCount, int cap, int shift, int mask, int* __restrict__ hist) {
   __shared__ int h[kRadix];
   const int n = countOf(count, hostCount, cap);
   const int base = blockIdx.x * kTile;
   h[threadIdx.x] = 0;
   __syncthreads();
-  for (int i = base + (int)threadIdx.x; i < base + kTile && i < n; i += kPT) atomicAdd(&h[digitOf(keys[i], shift)], 1);
+  for (int i = base + (int)threadIdx.x; i < base + kTile && i < n; i += kPT)
+    atomicAdd(&h[digitOf(keys[i], shift, mask)], 1);
   __syncthreads();
   hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
@@ -181,7 +185,7 @@ template <class K, bool Vals>
 __global__ __launch_bounds__(kPT) void kRadixScatter(const K* __restrict__ kin, const int* __restrict__ vin,
                                                      K* __restrict__ kout, int* __restrict__ vout,
                                                      const int* __restrict__ count, int hostCount, int cap, int shift,
-                                                     const int* __restrict__ offs) {
+                                                     int mask, const int* __restrict__ offs) {
   __shared__ int running[kRadix];
   __shared__ int cnt[kPT / 64][kRadix];
   const int n = countOf(count, hostCount, cap);
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(kPT) void kRadixScatter(const K* __restrict__ kin, 
     const bool valid = i < n;
     K k = valid ? kin[i] : (K)0;
     const int v = (Vals && valid) ? vin[i] : 0;
-    const int d = digitOf(k, shift);
+    const int d = digitOf(k, shift, mask);
     unsigned long long m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -233,7 +237,8 @@ int primTiles(int cap) { return (std::max(cap, 1) + kTile - 1) / kTile; }
 size_t primScratchInts(int cap, int keyBytes) {
   (void)keyBytes;
   const int tiles = primTiles(cap);
-  return (size_t)kRadix * tiles + (size_t)tiles + 64;
+  const size_t table = (size_t)kRadix * tiles;  // the radix sort's digit table, then its scan's tile sums
+  return table + (size_t)primTiles((int)std::min<size_t>(table, 1u << 30)) + (size_t)tiles + 64;
 }
 
 void primExclusiveScan(hipStream_t s, const int* in, int* out, const int* count, int hostCount, int cap, int* scratch,
@@ -250,11 +255,11 @@ void primSelectFlagged(hipStream_t s, const int* flag, const int* vals, int* out
   hipLaunchKernelGGL(kFlagTileSums, dim3(tiles), dim3(kPT), 0, s, flag, count, hostCount, cap, scratch);
   hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, tiles, scratch, selectedCount);
   if (vals)
-    hipLaunchKernelGGL(kSelectTiles<true>, dim3(tiles), dim3(kPT), 0, s, flag, vals, count, hostCount, cap,
+    hipLaunchKernelGGL((kSelectTiles<int, true>), dim3(tiles), dim3(kPT), 0, s, flag, vals, count, hostCount, cap,
                        (const int*)scratch, out);
   else
-    hipLaunchKernelGGL(kSelectTiles<false>, dim3(tiles), dim3(kPT), 0, s, flag, (const int*)nullptr, count, hostCount,
-                       cap, (const int*)scratch, out);
+    hipLaunchKernelGGL((kSelectTiles<int, false>), dim3(tiles), dim3(kPT), 0, s, flag, (const int*)nullptr, count,
+                       hostCount, cap, (const int*)scratch, out);
 }
 
 template <class K>
@@ -262,27 +267,35 @@ bool primRadixSort(hipStream_t s, K* k0, int* v0, K* k1, int* v1, const int* cou
                    int bitHi, int* scratch) {
   const int tiles = primTiles(cap);
   int* hist = scratch;
-  int* top = scratch + (size_t)kRadix * tiles;
+  int* top = scratch + (size_t)kRadix * tiles;  // tile sums of the table's scan
   bool alt = false;
   for (int shift = bitLo; shift < bitHi; shift += 8) {
     K* ki = alt ? k1 : k0;
     K* ko = alt ? k0 : k1;
     int* vi = alt ? v1 : v0;
     int* vo = alt ? v0 : v1;
-    hipLaunchKernelGGL(kRadixHist<K>, dim3(tiles), dim3(kPT), 0, s, (const K*)ki, count, hostCount, cap, shift, hist);
+    const int mask = (1 << std::min(8, bitHi - shift)) - 1;  // bits at or above bitHi take no part
+    hipLaunchKernelGGL(kRadixHist<K>, dim3(tiles), dim3(kPT), 0, s, (const K*)ki, count, hostCount, cap, shift, mask,
+                       hist);
+    // digit-major table (digit, tile): one exclusive scan gives every tile's first position per digit (a small
+    // table in one single-block launch, a large one as tile sums, their scan and the tiles)
     const int entries = kRadix * tiles;
-    const int scanTiles = primTiles(entries);
-    hipLaunchKernelGGL(kTileSums, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
-                       entries, top);
-    hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, scanTiles, top, (int*)nullptr);
-    hipLaunchKernelGGL(kScanTiles, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
-                       entries, (const int*)top, hist);
+    if (entries <= 4 * kScanTop) {
+      hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, entries, hist, (int*)nullptr);
+    } else {
+      const int scanTiles = primTiles(entries);
+      hipLaunchKernelGGL(kTileSums, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
+                         entries, top);
+      hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, scanTiles, top, (int*)nullptr);
+      hipLaunchKernelGGL(kScanTiles, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
+                         entries, (const int*)top, hist);
+    }
     if (v0)
       hipLaunchKernelGGL((kRadixScatter<K, true>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
-                         count, hostCount, cap, shift, (const int*)hist);
+                         count, hostCount, cap, shift, mask, (const int*)hist);
     else
       hipLaunchKernelGGL((kRadixScatter<K, false>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)nullptr,
-                         ko, (int*)nullptr, count, hostCount, cap, shift, (const int*)hist);
+                         ko, (int*)nullptr, count, hostCount, cap, shift, mask, (const int*)hist);
     alt = !alt;
   }
   return alt;
@@ -292,5 +305,74 @@ template bool primRadixSort<uint32_t>(hipStream_t, uint32_t*, int*, uint32_t*, i
                                       int*);
 template bool primRadixSort<uint64_t>(hipStream_t, uint64_t*, int*, uint64_t*, int*, const int*, int, int, int, int,
                                       int*);
+
+// ---- the context-level forms used by the step: host-known counts, scratch in the context ----------------------
+static int* primScratch(Ctx& c, int cap) {
+  c.primTmp.ensure(primScratchInts(cap) + 64);
+  return c.primTmp.p;
+}
+
+template <class K>
+void primSortPairs(Ctx& c, Db<K>& k, Db<int>& v, int n, int bitLo, int bitHi) {
+  if (n <= 1 || bitHi <= bitLo) return;
+  using U = typename std::conditional<sizeof(K) == 8, uint64_t, uint32_t>::type;  // keys sort as unsigned words
+  const bool alt = primRadixSort<U>(c.stream, (U*)k.cur, v.cur, (U*)k.alt, v.alt, nullptr, n, n, bitLo, bitHi,
+                                    primScratch(c, n));
+  if (alt) {
+    std::swap(k.cur, k.alt);
+    std::swap(v.cur, v.alt);
+  }
+}
+template <class K>
+void primSortKeys(Ctx& c, Db<K>& k, int n, int bitLo, int bitHi) {
+  if (n <= 1 || bitHi <= bitLo) return;
+  using U = typename std::conditional<sizeof(K) == 8, uint64_t, uint32_t>::type;
+  const bool alt = primRadixSort<U>(c.stream, (U*)k.cur, nullptr, (U*)k.alt, nullptr, nullptr, n, n, bitLo, bitHi,
+                                    primScratch(c, n));
+  if (alt) std::swap(k.cur, k.alt);
+}
+template void primSortPairs<uint64_t>(Ctx&, Db<uint64_t>&, Db<int>&, int, int, int);
+template void primSortPairs<unsigned long long>(Ctx&, Db<unsigned long long>&, Db<int>&, int, int, int);
+template void primSortPairs<int>(Ctx&, Db<int>&, Db<int>&, int, int, int);
+template void primSortKeys<uint64_t>(Ctx&, Db<uint64_t>&, int, int, int);
+
+void primSelectIndex(Ctx& c, const int* flag, int n, int* out, int* count) {
+  if (n <= 0) {
+    JB_CHECK(hipMemsetAsync(count, 0, sizeof(int), c.stream));
+    return;
+  }
+  primSelectFlagged(c.stream, flag, nullptr, out, nullptr, n, n, primScratch(c, n), count);
+}
+template <class T>
+void primSelectValues(Ctx& c, const T* vals, const int* flag, int n, T* out, int* count) {
+  if (n <= 0) {
+    JB_CHECK(hipMemsetAsync(count, 0, sizeof(int), c.stream));
+    return;
+  }
+  const int tiles = primTiles(n);
+  int* scratch = primScratch(c, n);
+  hipLaunchKernelGGL(kFlagTileSums, dim3(tiles), dim3(kPT), 0, c.stream, flag, (const int*)nullptr, n, n, scratch);
+  hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, c.stream, tiles, scratch, count);
+  hipLaunchKernelGGL((kSelectTiles<T, true>), dim3(tiles), dim3(kPT), 0, c.stream, flag, vals, (const int*)nullptr, n,
+                     n, (const int*)scratch, out);
+}
+template void primSelectValues<uint64_t>(Ctx&, const uint64_t*, const int*, int, uint64_t*, int*);
+template void primSelectValues<int>(Ctx&, const int*, const int*, int, int*, int*);
+
+void primScan(Ctx& c, const int* in, int* out, int n) {
+  if (n <= 0) return;
+  primExclusiveScan(c.stream, in, out, nullptr, n, n, primScratch(c, n), nullptr);
+}
+
+__global__ void kWiden(int n, const int* __restrict__ in, int64_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+void primScanWide(Ctx& c, const int* in, int64_t* out, int n) {
+  if (n <= 0) return;
+  c.primWide.ensure(n);
+  primExclusiveScan(c.stream, in, c.primWide.p, nullptr, n, n, primScratch(c, n), nullptr);
+  hipLaunchKernelGGL(kWiden, dim3(gridFor(n)), dim3(256), 0, c.stream, n, (const int*)c.primWide.p, out);
+}
 
 }  // namespace jbg

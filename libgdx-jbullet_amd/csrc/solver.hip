@@ -22,7 +22,6 @@
 // per-colour warm-start application (impulses are applied in manifold order).
 #include <cmath>
 
-#include <hipcub/hipcub.hpp>
 
 #include "internal.h"
 #include <algorithm>
@@ -140,10 +139,13 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 __device__ __forceinline__ bool dynamicBody(const float* props, int i) {
   return (ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) == 0;
 }
-// round (bits 52-63), then the manifold's degree when colouring largest-degree-first (bits 40-51: manifolds
-// sharing one of its dynamic bodies, 0 otherwise), then a hash of its index, then the index (ties)
+// round (bits 52-63), then a 20-bit hash of the manifold's index, then the index (ties); colouring
+// largest-degree-first, the manifold's degree (manifolds sharing one of its dynamic bodies) takes bits 40-51 and
+// the hash keeps 8 bits (a hash with only 8 bits leaves long runs of index-ordered priorities, which colour worse)
 __device__ __forceinline__ unsigned long long prio(int round, int k, const int* __restrict__ deg) {
-  const unsigned long long d = deg ? (unsigned long long)min(deg[k], 4095) : 0ull;
+  if (!deg)
+    return ((unsigned long long)round << 52) | ((unsigned long long)(hash32(k) >> 12) << 32) | (unsigned long long)k;
+  const unsigned long long d = (unsigned long long)min(deg[k], 4095);
   return ((unsigned long long)round << 52) | (d << 40) | ((unsigned long long)(hash32(k) >> 24) << 32) |
          (unsigned long long)k;
 }
@@ -1924,12 +1926,9 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   hipLaunchKernelGGL(kManifoldBlocks, dim3(gridFor(na)), dim3(256), 0, s, na, (const int2*)c.activeBodies.p,
                      (const int*)c.wbBodyBlock.p, c.wbKey.p, c.wbIdx.p, c.wbInfo.p);
   // slot order: active manifolds by block (stable: pair-key order inside a block)
-  hipcub::DoubleBuffer<int> dk(c.wbKey.p, c.wbKeyAlt.p), dv(c.wbIdx.p, c.colorOrderAlt.p);
-  size_t tb = 0;
+  Db<int> dk(c.wbKey.p, c.wbKeyAlt.p), dv(c.wbIdx.p, c.colorOrderAlt.p);
   const int keyBits = bitsForBlocks(nBlocks);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, dk, dv, na, 0, keyBits, s));
-  c.cubTmp.ensure(tb);
-  JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb, dk, dv, na, 0, keyBits, s));
+  primSortPairs(c, dk, dv, na, 0, keyBits);
   const int* slotManifold = dv.Current();  // slot p -> active manifold
   hipLaunchKernelGGL(kBlockSlotStarts, dim3(gridFor((int64_t)na + 1)), dim3(256), 0, s, na, nBlocks,
                      (const int*)dk.Current(), c.wbSlotStart.p);
@@ -1951,10 +1950,7 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   c.slotStart.ensure(na);
   hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, slotManifold, c.mActive.p, man,
                      c.slotCount.p);
-  size_t tbScan = 0;
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbScan, c.slotCount.p, c.slotStart.p, na, s));
-  c.cubTmp.ensure(tbScan);
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tbScan, c.slotCount.p, c.slotStart.p, na, s));
+  primScan(c, c.slotCount.p, c.slotStart.p, na);
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, slotManifold, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
                      SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
@@ -2111,13 +2107,8 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if (np > 0) {
     hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p,
                        island, island ? (const int*)(c.islandFlag.p + c.nb) : nullptr, c.flags.p);
-    size_t tb = 0;
-    hipcub::CountingInputIterator<int> it(0);
-    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np, s));
-    c.cubTmp.ensure(tb);
     JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
-    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tb, it, c.flags.p, c.mActive.p, (int*)c.counter.p, (int)np,
-                                           s));
+    primSelectIndex(c, c.flags.p, (int)np, c.mActive.p, (int*)c.counter.p);
     JB_CHECK(hipMemcpyAsync(&na, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
     JB_CHECK(hipStreamSynchronize(s));
   }
@@ -2202,10 +2193,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
                          c.bodyPrio.p, c.bodyMask.p, deg);
     hipLaunchKernelGGL(kUncoloured, dim3(gridFor(m)), dim3(256), 0, s, m, (const int*)c.pend.p, (const int*)c.color.p,
                        c.pendFlag.p);
-    size_t tbs = 0;
-    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, remDev, m, s));
-    c.cubTmp.ensure(tbs);
-    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, c.pend.p, c.pendFlag.p, c.pendAlt.p, remDev, m, s));
+    primSelectValues<int>(c, c.pend.p, c.pendFlag.p, m, c.pendAlt.p, remDev);
     std::swap(c.pend.p, c.pendAlt.p);
     std::swap(c.pend.n, c.pendAlt.n);
     batches++;
@@ -2224,11 +2212,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if (blocked) {
     for (int interior = 1; interior >= 0; interior--) {
       hipLaunchKernelGGL(kPassFlag, dim3(gridFor(na)), dim3(256), 0, s, na, phase, interior, c.pendFlag.p);
-      hipcub::CountingInputIterator<int> it0(0);
-      size_t tbs = 0;
-      JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, it0, c.pendFlag.p, c.pend.p, (int*)c.counter.p, na, s));
-      c.cubTmp.ensure(tbs);
-      JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.pendFlag.p, c.pend.p, (int*)c.counter.p, na, s));
+      primSelectIndex(c, c.pendFlag.p, na, c.pend.p, (int*)c.counter.p);
       int m = 0;
       JB_CHECK(hipMemcpyAsync(&m, c.counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
       JB_CHECK(hipMemsetAsync(c.bodyPrio.p, 0, 2 * (size_t)c.nb * sizeof(unsigned long long), s));
@@ -2252,24 +2236,16 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
     if (round >= (1 << 10) - 64) throw std::runtime_error("colour index exceeds the slot key");
     hipLaunchKernelGGL(kSlotKeys, dim3(gridFor(na)), dim3(256), 0, s, na, phase, (const int*)c.color.p, c.slotKey.p,
                        c.colorOrder.p);
-    hipcub::DoubleBuffer<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
-    hipcub::DoubleBuffer<int> dv(c.colorOrder.p, c.permAlt.p);
+    Db<unsigned long long> dk(c.slotKey.p, c.slotKeyAlt.p);
+    Db<int> dv(c.colorOrder.p, c.permAlt.p);
     // key bits in use: without blocks only the colour (bits 20-29; stable sort, so equal keys keep slot order)
     const int keyLo = blocked ? 0 : 20, keyHi = blocked ? 53 : 30;
-    size_t tb2 = 0;
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dk, dv, na, keyLo, keyHi, s));
-    c.cubTmp.ensure(tb2);
-    JB_CHECK(hipcub::DeviceRadixSort::SortPairs(c.cubTmp.p, tb2, dk, dv, na, keyLo, keyHi, s));
+    primSortPairs(c, dk, dv, na, keyLo, keyHi);
     if (dv.Current() != c.colorOrder.p)
       JB_CHECK(hipMemcpyAsync(c.colorOrder.p, dv.Current(), (size_t)na * sizeof(int), hipMemcpyDeviceToDevice, s));
     skey = dk.Current();
     hipLaunchKernelGGL(kBatchFlags, dim3(gridFor(na)), dim3(256), 0, s, na, skey, c.batchFlag.p);
-    hipcub::CountingInputIterator<int> it0(0);
-    size_t tbs = 0;
-    JB_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na, s));
-    c.cubTmp.ensure(tbs);
-    JB_CHECK(hipcub::DeviceSelect::Flagged(c.cubTmp.p, tbs, it0, c.batchFlag.p, c.colorStart.p, (int*)c.counter.p, na,
-                                           s));
+    primSelectIndex(c, c.batchFlag.p, na, c.colorStart.p, (int*)c.counter.p);
     int* bnd = c.batchFlag.p + na;
     hipLaunchKernelGGL(kBatchBoundary, dim3(kMaxBatches / 256), dim3(256), 0, s, (const int*)c.counter.p,
                        (const int*)c.colorStart.p, skey, bnd);
@@ -2352,10 +2328,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.slotStart.ensure(na);
   hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p, c.mActive.p, man,
                      c.slotCount.p);
-  size_t tbScan = 0;
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tbScan, c.slotCount.p, c.slotStart.p, na, s));
-  c.cubTmp.ensure(tbScan);
-  JB_CHECK(hipcub::DeviceScan::ExclusiveSum(c.cubTmp.p, tbScan, c.slotCount.p, c.slotStart.p, na, s));
+  primScan(c, c.slotCount.p, c.slotStart.p, na);
   hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
                      SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);

@@ -1044,6 +1044,16 @@ int jbgpu_hinge_frames(int32_t from_frames, int32_t fixed_b, const float* tr_a, 
   }
 }
 
+int jbgpu_np_set_epa_capacity(jbgpu_ctx* ctx, int32_t small_faces, int32_t large_faces) {
+  return guard(ctx, [&] {
+    if (small_faces < 16 || small_faces > kEpaSmallFaces || large_faces < 16 || large_faces > kEpaFaces)
+      throw std::invalid_argument("EPA face capacities must lie in [16, " + std::to_string(kEpaSmallFaces) + "] and [16, " +
+                                  std::to_string(kEpaFaces) + "]");
+    ctx->c.epaSmallFaces = small_faces;
+    ctx->c.epaLargeFaces = large_faces;
+  });
+}
+
 int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n) {
   return guard(ctx, [&] {
     if (n) *n = ctx->c.nj;

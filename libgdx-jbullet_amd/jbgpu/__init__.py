@@ -79,6 +79,7 @@ SYMBOLS = {
     "jbgpu_world_add_hinge": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp]),
     "jbgpu_hinge_frames": (C.c_int, [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jbgpu_world_num_joints": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "jbgpu_np_set_epa_capacity": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "jbgpu_world_joint_impulses": (C.c_int, [_vp, _vp]),
     "jbgpu_world_joint_order": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_check_pair_records": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp]),
@@ -399,6 +400,10 @@ class Context:
             raise ValueError("params must be n x 9")
         self._ck(load().jbgpu_world_add_hinge(self.h, len(a), _p(a), _p(b), _p(fa), _p(fb), _p(pr)),
                  "world_add_hinge")
+
+    def np_set_epa_capacity(self, small_faces=192, large_faces=1024):
+        """Face capacities of the two LDS EPA tiers (work moves between tiers; results never change)."""
+        self._ck(load().jbgpu_np_set_epa_capacity(self.h, int(small_faces), int(large_faces)), "np_set_epa_capacity")
 
     def num_joints(self):
         n = C.c_int32(0)

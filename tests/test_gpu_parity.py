@@ -323,29 +323,26 @@ def test_epa_workspace_retry_step_parity():
     ctx.close()
 
 
-def test_epa_last_tier_step_parity():
-    """EPA jobs that outgrow the large LDS workspace too (a long run on detailed hulls: more than 1024 faces or a
-    horizon deeper than 64) are rerun with a workspace in global memory.  The 100k pile meets one in its first 400
-    steps: found on a device-only run, the same step of a second run is replayed by the oracle from the device
-    state before it and must agree bit for bit (the device run is deterministic: same inputs, same bits)."""
-    sc = scenes.mixed_pile(seed=1, n=100_000)
+@pytest.mark.parametrize("small,large", [(16, 16), (16, 1024), (24, 48)])
+def test_epa_tiers_step_parity(small, large):
+    """EPA jobs run in a small LDS workspace, then a large LDS one, then a global-memory one, each job moving on
+    only when it outgrows the last.  With the LDS tiers' face capacities lowered (16 faces: nearly every job moves
+    on), the mixed pile's EPA jobs go through all three tiers; the steps stay bit-exact against the oracle (the
+    capacities move work, they never change a result)."""
+    sc = scenes.mixed_pile(seed=5, n=3000, bin_half=12.0)
     ctx, orc = _world_pair(sc, iterations=20)
-    found = None
-    for step in range(1, 401):
-        ctx.step()
+    ctx.np_set_epa_capacity(small, large)
+    retried = huge = 0
+    for step in range(90):
+        _sync_and_step(ctx, orc)
+        _compare_world(ctx, orc, f"EPA tiers {small}/{large} step {step}")
         st = ctx.stats()
-        if st["epa_huge"] > 0:
-            found = step
-            print(f"[epa last tier] step {step}: {st['epa_calls']} EPA jobs, {st['epa_retries']} rerun, "
-                  f"{st['epa_huge']} in the last tier", flush=True)
-            break
-    ctx.close()
-    assert found is not None, "no EPA job reached the last workspace tier in 400 steps"
-    ctx, orc = _world_pair(sc, iterations=20)
-    ctx.step(n=found - 1)
-    _sync_and_step(ctx, orc)
-    assert ctx.stats()["epa_huge"] > 0
-    _compare_world(ctx, orc, f"last-tier step {found}")
+        retried += st["epa_retries"]
+        huge += st["epa_huge"]
+    print(f"[epa tiers {small}/{large}] {retried} jobs rerun in the large tier, {huge} in the global one", flush=True)
+    assert retried > 0
+    if large == 16:
+        assert huge > 0
     ctx.close()
 
 
