@@ -376,6 +376,55 @@ void orc_world_ray_test(void* h, int n, const float* from3, const float* to3, in
     st3(point3 + 3 * i, r.point);
   }
 }
+// convexSweepTest with a ClosestConvexResultCallback per query: cast shape = shape index of the world, from / to
+// 12 floats each (basis row-major, origin)
+void orc_world_convex_sweep(void* h, int n, int shape, const float* from12, const float* to12, int16_t group,
+                            int16_t mask, const int32_t* world, int32_t* obj, float* frac, float* normal3,
+                            float* point3) {
+  World* w = (World*)h;
+  for (int i = 0; i < n; i++) {
+    RayHit r = convexSweepClosest(*w, w->shapes[shape], trFrom(from12 + 12 * i), trFrom(to12 + 12 * i), group, mask,
+                                  world ? world[i] : 0);
+    obj[i] = r.object;
+    frac[i] = r.fraction;
+    st3(normal3 + 3 * i, r.normal);
+    st3(point3 + 3 * i, r.point);
+  }
+}
+// CollisionObject.setCcdMotionThreshold / setCcdSweptSphereRadius of bodies
+void orc_world_set_ccd(void* h, int n, const int32_t* body, const float* threshold, const float* radius) {
+  World* w = (World*)h;
+  for (int i = 0; i < n; i++) {
+    w->bodies[body[i]].ccdMotionThreshold = threshold[i];
+    w->bodies[body[i]].ccdSweptSphereRadius = radius[i];
+  }
+}
+// TransformUtil.calculateVelocity and CollisionShape.calculateTemporalAabb (of a shape record) for the pins
+void orc_calculate_velocity(const float* t0, const float* t1, float dt, float* lin3, float* ang3) {
+  V3 l, a;
+  calculateVelocity(trFrom(t0), trFrom(t1), dt, l, a);
+  st3(lin3, l);
+  st3(ang3, a);
+}
+void orc_temporal_aabb(const float* rec, const float* tr12, const float* lin3, const float* ang3, float dt, float* mn3,
+                       float* mx3) {
+  V3 mn, mx;
+  temporalAabb(shapeFromRecord(rec), trFrom(tr12), ld3(lin3), ld3(ang3), dt, mn, mx);
+  st3(mn3, mn);
+  st3(mx3, mx);
+}
+// GjkConvexCast of shape record a from fromA to toA against shape record b at tb (hull points hv, may be null)
+int orc_gjk_convex_cast(const float* recA, const float* fromA, const float* toA, const float* recB, const float* tb,
+                        const float* hv, float allowedPenetration, float* out) {
+  V3 n = v3(0, 0, 0), p = v3(0, 0, 0);
+  float f = 1.0f;
+  const bool hit = gjkConvexCast(shapeFromRecord(recA), trFrom(fromA), trFrom(toA), shapeFromRecord(recB), trFrom(tb),
+                                 (const V3*)hv, allowedPenetration, f, n, p);
+  out[0] = f;
+  st3(out + 1, n);
+  st3(out + 4, p);
+  return hit ? 1 : 0;
+}
 void orc_world_set_worlds(void* h, const int32_t* world) {
   World* w = (World*)h;
   for (size_t i = 0; i < w->bp.proxies.size(); i++) w->bp.proxies[i].world = world[i];

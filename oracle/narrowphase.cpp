@@ -306,7 +306,7 @@ struct Voronoi {
 
 // ============================ GjkPairDetector ==============================
 ContactCandidate gjkPairDetector(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb, const V3* hv,
-                                 float maxDist2, GjkStats* st) {
+                                 float maxDist2, GjkStats* st, bool penetrationSolver) {
   ContactCandidate out{};
   const float REL_ERROR2 = 1.0e-6f;  // GjkPairDetector.java:44
   float distance = 0.0f;
@@ -394,7 +394,7 @@ ContactCandidate gjkPairDetector(const Shape& sa, const Tr& ta, const Shape& sb,
     }
   }
   bool catchDegeneratePenetrationCase = (degenerateSimplex != 0 && ((distance + margin) < 0.01f));  // :259-260
-  if (checkPenetration && (!isValid || catchDegeneratePenetrationCase)) {  // :263-303
+  if (penetrationSolver && checkPenetration && (!isValid || catchDegeneratePenetrationCase)) {  // :263-303
     V3 tA, tB;
     int it = 0;
     if (st) st->epaCalls++;
@@ -1138,7 +1138,7 @@ static void planeSpace1(V3 n, V3& p, V3& q) {  // TransformUtil.java:45-61
   }
 }
 
-static V3 interp3(V3 a, V3 b, float t) {  // VectorUtil.setInterpolate3
+V3 interp3(V3 a, V3 b, float t) {  // VectorUtil.setInterpolate3
   float s = 1.0f - t;
   return v3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
 }

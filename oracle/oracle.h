@@ -63,10 +63,12 @@ struct Body {
   int activation;      // CollisionObject.activationState1
   float deactivationTime;
   int islandTag, companionId;
+  float ccdMotionThreshold = 0.0f, ccdSweptSphereRadius = 0.0f;  // CollisionObject.java:83-86 (0: no CCD)
   bool isStatic() const { return (flags & JBGPU_BODY_STATIC) != 0; }
   bool isActive() const { return activation != JBGPU_ISLAND_SLEEPING && activation != JBGPU_DISABLE_SIMULATION; }
 };
 void updateInertiaTensor(Body& b);  // RigidBody.java:449-459
+Tr integrateTransform(const Tr& cur, V3 lv, V3 av, float dt);  // TransformUtil.integrateTransform (:64-103)
 
 // ---- broadphase -----------------------------------------------------------
 struct Aabb {
@@ -137,8 +139,9 @@ struct GjkStats {
 };
 ContactCandidate sphereSphere(const Shape& s0, const Tr& t0, const Shape& s1, const Tr& t1, bool& touching);
 ContactCandidate convexPlane(const Shape& convex, const V3* hv, const Tr& tc, const Shape& plane, const Tr& tp);
+// penetrationSolver false: GjkPairDetector without a penetration depth solver (the convex casts' use)
 ContactCandidate gjkPairDetector(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb, const V3* hv,
-                                 float maxDist2, GjkStats* st);
+                                 float maxDist2, GjkStats* st, bool penetrationSolver = true);
 bool gjkEpaPenDepth(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb, const V3* hv, V3& wA, V3& wB,
                     int* epaIters);
 // ManifoldResult.addContactPoint (ManifoldResult.java:92-157)
@@ -286,6 +289,22 @@ struct World {
 
 // CollisionWorld.rayTest with a ClosestRayResultCallback over the bodies of `world`
 RayHit rayTestClosest(const World& w, V3 from, V3 to, int16_t group, int16_t mask, int world);
+// CollisionWorld.convexSweepTest with a ClosestConvexResultCallback over the bodies of `world` (RayHit fields:
+// hitCollisionObject, closestHitFraction, hitNormalWorld, hitPointWorld).  Static planes are skipped: the
+// reference's objectQuerySingle has no caster for them (it throws).
+RayHit convexSweepClosest(const World& w, const Shape& castShape, const Tr& from, const Tr& to, int16_t group,
+                          int16_t mask, int world);
+// GjkConvexCast.calcTimeOfImpact (GjkConvexCast.java:67-193) for a static B (fromB == toB)
+bool gjkConvexCast(const Shape& sa, const Tr& fromA, const Tr& toA, const Shape& sb, const Tr& tb, const V3* hv,
+                   float allowedPenetration, float& fraction, V3& normal, V3& hitPoint);
+// TransformUtil.calculateVelocity (TransformUtil.java:105-115, calculateDiffAxisAngle :117-151)
+void calculateVelocity(const Tr& t0, const Tr& t1, float timeStep, V3& linVel, V3& angVel);
+// CollisionShape.calculateTemporalAabb (CollisionShape.java:76-122) with getAngularMotionDisc (:44-72)
+void temporalAabb(const Shape& s, const Tr& cur, V3 linvel, V3 angvel, float timeStep, V3& mn, V3& mx);
+V3 interp3(V3 a, V3 b, float t);  // VectorUtil.setInterpolate3
+// the CCD motion clamp of integrateTransforms (DiscreteDynamicsWorld.java:693-725) for body i with predicted
+// transform `pred`: the clamped transform (pred itself when no sweep hit counts)
+Tr ccdClamp(World& w, int i, const Tr& pred, float dt);
 bool rayAabb(V3 from, V3 to, V3 mn, V3 mx, float& param, V3& normal);
 bool subsimplexRayCast(const Shape& sb, const V3* hv, const Tr& fromA, const Tr& toA, const Tr& fromB, const Tr& toB,
                        float& fraction, V3& normal);

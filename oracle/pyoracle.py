@@ -98,6 +98,12 @@ def lib():
             "orc_world_phase": (C.c_int, [vp, C.c_float, C.c_int]),
             "orc_world_add_p2p": (None, [vp, C.c_int, _i32, _i32, _f, _f, vp]),
             "orc_world_add_hinge": (None, [vp, C.c_int, _i32, _i32, _f, _f, vp]),
+            "orc_world_convex_sweep": (None, [vp, C.c_int, C.c_int, _f, _f, C.c_int16, C.c_int16, vp, _i32, _f, _f,
+                                              _f]),
+            "orc_world_set_ccd": (None, [vp, C.c_int, _i32, _f, _f]),
+            "orc_calculate_velocity": (None, [_f, _f, C.c_float, _f, _f]),
+            "orc_temporal_aabb": (None, [_f, _f, _f, _f, C.c_float, _f, _f]),
+            "orc_gjk_convex_cast": (C.c_int, [_f, _f, _f, _f, _f, vp, C.c_float, _f]),
             "orc_hinge_frames": (None, [C.c_int, C.c_int, _f, _f, _f, _f, _f, _f, _f, _f, _f]),
             "orc_atan2_fast": (C.c_float, [C.c_float, C.c_float]),
             "orc_world_hinge_angle": (C.c_float, [vp, C.c_int]),
@@ -290,6 +296,31 @@ def hinge_frames_from_frames(tr_a, frame_a, frame_b=None):
     return fa, fb
 
 
+def calculate_velocity(t0, t1, dt):
+    """TransformUtil.calculateVelocity: (linear, angular) velocity taking transform t0 to t1 (12 floats) in dt."""
+    lv, av = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_calculate_velocity(f32(t0), f32(t1), np.float32(dt), lv, av)
+    return lv, av
+
+
+def temporal_aabb(rec, tr, lin, ang, dt):
+    """CollisionShape.calculateTemporalAabb of a shape record at tr (12 floats) moving with (lin, ang) for dt."""
+    mn, mx = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_temporal_aabb(f32(rec), f32(tr), f32(lin), f32(ang), np.float32(dt), mn, mx)
+    return mn, mx
+
+
+def gjk_convex_cast(rec_a, from_a, to_a, rec_b, tr_b, hull=None, allowed_penetration=0.04):
+    """GjkConvexCast.calcTimeOfImpact of shape record a moving from_a -> to_a against b at rest at tr_b:
+    (hit, fraction, normal, point)."""
+    out = np.zeros(7, np.float32)
+    hv = None if hull is None else f32(hull).reshape(-1)
+    hit = lib().orc_gjk_convex_cast(f32(rec_a), f32(from_a), f32(to_a), f32(rec_b), f32(tr_b),
+                                    None if hv is None else hv.ctypes.data_as(C.c_void_p),
+                                    np.float32(allowed_penetration), out)
+    return bool(hit), float(out[0]), out[1:4].copy(), out[4:7].copy()
+
+
 def atan2_fast(y, x):
     return float(lib().orc_atan2_fast(np.float32(y), np.float32(x)))
 
@@ -382,6 +413,29 @@ class World:
                                  None if w is None else w.ctypes.data_as(C.c_void_p), obj, fr, nrm.reshape(-1),
                                  pt.reshape(-1))
         return obj, fr, nrm, pt
+
+    def convex_sweep(self, shape, frm, to, group=1, mask=-1, world=None):
+        """convexSweepTest of the world's shape `shape` from each transform of frm to to (n x 12) with a
+        ClosestConvexResultCallback: (object, fraction, normal, point)."""
+        frm = f32(frm).reshape(-1, 12)
+        to = f32(to).reshape(-1, 12)
+        n = len(frm)
+        obj = np.zeros(n, np.int32)
+        fr = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        pt = np.zeros((n, 3), np.float32)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        lib().orc_world_convex_sweep(self.h, n, int(shape), frm.reshape(-1), to.reshape(-1), int(group), int(mask),
+                                     None if w is None else w.ctypes.data_as(C.c_void_p), obj, fr, nrm.reshape(-1),
+                                     pt.reshape(-1))
+        return obj, fr, nrm, pt
+
+    def set_ccd(self, bodies, threshold, radius):
+        """setCcdMotionThreshold / setCcdSweptSphereRadius of the given bodies."""
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        t = np.broadcast_to(f32(threshold), b.shape).copy()
+        r = np.broadcast_to(f32(radius), b.shape).copy()
+        lib().orc_world_set_ccd(self.h, len(b), b, t, r)
 
     def set_order(self, order):
         o = np.ascontiguousarray(order, np.int32)

@@ -27,7 +27,7 @@ void updateInertiaTensor(Body& b) {  // RigidBody.java:449-459
 }
 
 // TransformUtil.integrateTransform (TransformUtil.java:64-103)
-static Tr integrateTransform(const Tr& cur, V3 lv, V3 av, float dt) {
+Tr integrateTransform(const Tr& cur, V3 lv, V3 av, float dt) {
   Tr p;
   p.o.x = cur.o.x + dt * lv.x;
   p.o.y = cur.o.y + dt * lv.y;
@@ -339,9 +339,16 @@ void World::solveConstraints() {  // DiscreteDynamicsWorld.java:620-647 -> build
 }
 
 void World::integrateTransforms(float dt) {  // DiscreteDynamicsWorld.java:679-739
-  for (auto& b : bodies) {
+  for (size_t i = 0; i < bodies.size(); i++) {
+    Body& b = bodies[i];
     if (b.isActive() && !b.isStatic()) {
-      b.tr = integrateTransform(b.tr, b.lv, b.av, dt);
+      Tr pred = integrateTransform(b.tr, b.lv, b.av, dt);
+      // CCD motion clamp (:693-725): a convex body moving more than its threshold sweeps a sphere against the
+      // world (bodies earlier in the list already at their new transforms) and stops at the first hit
+      const float sq = b.ccdMotionThreshold * b.ccdMotionThreshold;  // getCcdSquareMotionThreshold
+      const float squareMotion = len2(sub(pred.o, b.tr.o));
+      if (sq != 0.0f && sq < squareMotion && shapes[b.shape].type != JBGPU_SHAPE_PLANE) pred = ccdClamp(*this, (int)i, pred, dt);
+      b.tr = pred;
       updateInertiaTensor(b);  // proceedToTransform -> setCenterOfMassTransform (RigidBody.java:344-354)
     }
   }

@@ -1280,3 +1280,179 @@ def test_hinge_realigns_a_tilted_axis():
         tilt = np.arccos(min(1.0, float(w.bodies()[1, :9].reshape(3, 3)[2, 2])))
         assert tilt < 0.2 + 1e-4, tilt
     assert tilt < 0.01, tilt
+
+
+# ------------------------------------------------------------------ convex sweeps and CCD
+def _tr12(o, R=None):
+    return np.concatenate([(np.eye(3) if R is None else np.asarray(R)).reshape(-1), o]).astype(np.float32)
+
+
+def test_calculate_velocity_closed_form():
+    """TransformUtil.calculateVelocity: from A to R(axis, angle)·A over dt the angular velocity w is a rotation
+    vector of R: rot(w/|w|, |w| dt) = R with |w| dt in [0, 2pi]; while the trace of R is not negative the difference
+    quaternion comes out with w >= 0 and the vector is axis·angle/dt itself (beyond that it may be the equivalent
+    turn 2pi - angle about -axis, as the reference gives no sign to the quaternion). The linear velocity is the
+    origin difference over dt."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        ang = rng.uniform(0.1, 2.5)
+        A = random_rot(rng)
+        o0, o1 = rng.uniform(-5, 5, 3), rng.uniform(-5, 5, 3)
+        dt = 0.25
+        lv, av = po.calculate_velocity(_tr12(o0, A), _tr12(o1, rot(a, ang) @ A), dt)
+        assert np.allclose(lv, (np.float32(o1) - np.float32(o0)) / dt, rtol=1e-6, atol=1e-5)
+        th = float(np.linalg.norm(av)) * dt
+        assert 0.0 <= th <= 2 * np.pi + 1e-5
+        assert np.abs(rot(av / np.linalg.norm(av), th) - rot(a, ang)).max() < 5e-5
+        if 1 + 2 * np.cos(ang) >= 1e-3:
+            assert np.abs(av * dt - a * ang).max() < 5e-5, (av * dt, a * ang)
+    lv, av = po.calculate_velocity(_tr12([0, 0, 0]), _tr12([1, 0, 0]), 1.0)
+    assert np.array_equal(av, [0, 0, 0])  # no rotation: the axis falls back to x with angle 0
+
+
+def test_temporal_aabb_closed_form():
+    """CollisionShape.calculateTemporalAabb: the shape's AABB, its max side grown by positive and its min side by
+    negative linear motion, then both by |w| dt times the angular motion disc (half the diagonal of the AABB at the
+    identity plus the distance of its centre)."""
+    box = po.make_shape(BOX, [1.0, 2.0, 3.0])
+    mn, mx = po.temporal_aabb(box, _tr12([1, 1, 1]), [2.0, -4.0, 0.0], [0.0, 3.0, 4.0], 0.5)
+    am = 5.0 * np.sqrt(1 + 4 + 9) * 0.5
+    assert np.allclose(mn, np.array([0, -1, -2]) + [0, -2, 0] - am, atol=1e-5)
+    assert np.allclose(mx, np.array([2, 3, 4]) + [1, 0, 0] + am, atol=1e-5)
+    # a hull off its origin: the disc adds the distance of the AABB centre to the half diagonal
+    pts = np.array([[x, y, z] for x in (2, 4) for y in (-1, 1) for z in (-1, 1)], np.float32)
+    hull = po.make_shape(HULL, [0, 8], pts)
+    a = po.aabb(hull, _tr12([0, 0, 0])).astype(np.float64)
+    lo, hi = a[:3], a[3:]
+    disc = np.linalg.norm(hi - lo) * 0.5 + np.linalg.norm((hi + lo) * 0.5)
+    mn, mx = po.temporal_aabb(hull, _tr12([0, 0, 0]), [0.0, 0.0, 0.0], [0.0, 0.0, 2.0], 1.0)
+    assert np.allclose(mn, lo - 2 * disc, atol=1e-5) and np.allclose(mx, hi + 2 * disc, atol=1e-5)
+
+
+def test_gjk_convex_cast_closed_forms():
+    """GjkConvexCast: a sphere (r 0.5) swept along x from 0 to 10 meets a box of half-extent 1 centred at x = 5 when
+    its centre reaches 3.5: fraction 0.35, normal -x, contact point on the box face (4, 0, 0); swept away, or past
+    the box with a lateral offset larger than the sum of the half sizes, it reports nothing; against a sphere of
+    radius 1 at 5 the fraction is (5 - 1.5) / 10."""
+    sph = po.make_shape(SPHERE, [0.5])
+    box = po.make_shape(BOX, [1.0, 1.0, 1.0])
+    hit, f, n, p = po.gjk_convex_cast(sph, _tr12([0, 0, 0]), _tr12([10, 0, 0]), box, _tr12([5, 0, 0]))
+    assert hit and abs(f - 0.35) < 1e-4 and np.allclose(n, [-1, 0, 0], atol=1e-5) and np.allclose(p, [4, 0, 0],
+                                                                                                    atol=2e-3)
+    assert not po.gjk_convex_cast(sph, _tr12([0, 0, 0]), _tr12([-10, 0, 0]), box, _tr12([5, 0, 0]))[0]
+    assert not po.gjk_convex_cast(sph, _tr12([0, 1.6, 0]), _tr12([10, 1.6, 0]), box, _tr12([5, 0, 0]))[0]
+    # starting within the cast's 0.001 tolerance of the face: moving in hits at fraction 0, moving away (or
+    # along the face) does not, as the motion along the normal is not below -allowedPenetration
+    hit, f, n, p = po.gjk_convex_cast(sph, _tr12([3.5 - 5e-4, 0, 0]), _tr12([5.5, 0, 0]), box, _tr12([5, 0, 0]))
+    assert hit and f == 0.0 and np.allclose(n, [-1, 0, 0], atol=1e-5)
+    assert not po.gjk_convex_cast(sph, _tr12([3.5 - 5e-4, 0, 0]), _tr12([1.5, 0, 0]), box, _tr12([5, 0, 0]))[0]
+    assert not po.gjk_convex_cast(sph, _tr12([3.5 - 5e-4, 0, 0]), _tr12([3.5 - 5e-4, 0.5, 0]), box,
+                                  _tr12([5, 0, 0]))[0]
+    big = po.make_shape(SPHERE, [1.0])
+    hit, f, n, p = po.gjk_convex_cast(sph, _tr12([0, 0, 0]), _tr12([10, 0, 0]), big, _tr12([5, 0, 0]))
+    assert hit and abs(f - 0.35) < 1e-4
+
+
+def _sweep_world():
+    """Static boxes of half-extent 1 at x = 5, 8 and 11 (the middle one in filter group 2 only as every static
+    body), a dynamic box above the line, no floor plane."""
+    b = scenes._Builder("sweep")
+    bx = b.shape(BOX, [1.0, 1.0, 1.0])
+    for x in (11.0, 5.0, 8.0):
+        b.body(bx, scenes.IDENT, np.array([x, 0, 0], np.float32), 0.0)
+    b.body(bx, scenes.IDENT, np.array([6.5, 5, 0], np.float32), 1.0)
+    sph = b.shape(SPHERE, [0.5])
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    return po.World(shapes, verts, bodies), sph
+
+
+def test_convex_sweep_closest_hit_and_filters():
+    """convexSweepTest with a closest-hit callback: the sphere swept from x = 0 to 20 stops at the nearest box (list
+    order does not matter: the box at 5 is the second object), fraction 0.175; a callback whose mask excludes the
+    static group sees nothing; a sweep above the boxes passes below the dynamic box only when it clears it."""
+    w, sph = _sweep_world()
+    obj, fr, n, p = w.convex_sweep(sph, _tr12([0, 0, 0])[None], _tr12([20, 0, 0])[None])
+    assert obj[0] == 1 and abs(fr[0] - 0.175) < 1e-4 and np.allclose(n[0], [-1, 0, 0], atol=1e-5)
+    obj, fr, n, p = w.convex_sweep(sph, _tr12([0, 0, 0])[None], _tr12([20, 0, 0])[None], group=1, mask=1)
+    assert obj[0] == -1 and fr[0] == 1.0
+    obj, fr, n, p = w.convex_sweep(sph, _tr12([0, 5, 0])[None], _tr12([20, 5, 0])[None])
+    assert obj[0] == 3 and abs(fr[0] - (6.5 - 1 - 0.5) / 20) < 1e-4
+    obj, fr, n, p = w.convex_sweep(sph, _tr12([0, 6.6, 0])[None], _tr12([20, 6.6, 0])[None])
+    assert obj[0] == -1
+
+
+def test_ccd_stops_a_fast_sphere_at_a_thin_wall():
+    """A sphere (r 0.1) at 300 m/s towards a static wall 0.1 m thick 2.5 m away moves 5 m per step: without CCD it
+    passes through the wall in the first step; with a motion threshold of 0.1 and a swept sphere of radius 0.1 the
+    step is clamped at the time of impact (the sphere's centre just short of the wall face at 2.45 - 0.1) and the
+    sphere never gets past the wall."""
+    def world(ccd):
+        b = scenes._Builder("wall")
+        wall = b.shape(BOX, [0.05, 2.0, 2.0])
+        b.body(wall, scenes.IDENT, np.array([2.5, 0, 0], np.float32), 0.0)
+        s = b.shape(SPHERE, [0.1])
+        b.body(s, scenes.IDENT, np.array([0, 0, 0], np.float32), 1.0)
+        sc = b.done()
+        shapes, verts, bodies = scenes.build_records(sc, po)
+        bodies = np.asarray(bodies, np.float32).reshape(-1, po.BODY_WORDS).copy()
+        bodies[1, 12] = 300.0
+        w = po.World(shapes, verts, bodies)
+        if ccd:
+            w.set_ccd([1], 0.1, 0.1)
+        return w
+    w = world(False)
+    w.step()
+    assert w.bodies()[1, 9] > 4.0
+    w = world(True)
+    w.step()
+    x = float(w.bodies()[1, 9])
+    assert 2.3 < x < 2.35 + 1e-3, x
+    for _ in range(30):
+        w.step()
+        assert w.bodies()[1, 9] < 2.45, w.bodies()[1, 9]
+
+
+def _ccd_world(bodies_spec, ccd_bodies, radius):
+    """bodies_spec: (shape type, params, x, vx, mass) per body in list order, no floor."""
+    b = scenes._Builder("ccd")
+    for st, par, x, vx, mass in bodies_spec:
+        s = b.shape(st, par)
+        b.body(s, scenes.IDENT, np.array([x, 0, 0], np.float32), mass)
+    sc = b.done()
+    shapes, verts, bodies = scenes.build_records(sc, po)
+    bodies = np.asarray(bodies, np.float32).reshape(-1, po.BODY_WORDS).copy()
+    for i, spec in enumerate(bodies_spec):
+        bodies[i, 12] = spec[3]
+    w = po.World(shapes, verts, bodies)
+    w.set_ccd(ccd_bodies, 0.1, radius)
+    return w
+
+
+def test_ccd_skips_bodies_it_already_touches():
+    """Two boxes (half 0.5) 0.01 apart, both at 300 m/s along x: the pair has a contact point (within the 0.02
+    breaking threshold) and no impulse, so the rear box's sweep skips the front box and both move 5 m; the same
+    boxes 0.03 apart have no contact point, the sweep meets the front box at its old place and clamps the rear box
+    to about 0.03 m of travel."""
+    for gap, far in ((0.01, True), (0.03, False)):
+        w = _ccd_world([(BOX, [0.5, 0.5, 0.5], 0.0, 300.0, 1.0), (BOX, [0.5, 0.5, 0.5], 1.0 + gap, 300.0, 1.0)],
+                       [0], 0.5)
+        w.step()
+        x = float(w.bodies()[0, 9])
+        assert (x > 4.9) if far else (0.0 < x < 0.05), (gap, x)
+
+
+def test_ccd_sweeps_earlier_bodies_at_their_new_place():
+    """integrateTransforms moves the bodies one after another: a CCD sphere behind a sphere that moves away at the
+    same speed passes freely when the front sphere comes first in the body list (already moved), and stops at
+    the front sphere's old place (centres 1 m apart: a fifth of its 5 m step) when it comes later."""
+    sp = (SPHERE, [0.5])
+    w = _ccd_world([sp + (2.0, 300.0, 1.0), sp + (0.0, 300.0, 1.0)], [1], 0.5)
+    w.step()
+    assert float(w.bodies()[1, 9]) > 4.9
+    w = _ccd_world([sp + (0.0, 300.0, 1.0), sp + (2.0, 300.0, 1.0)], [0], 0.5)
+    w.step()
+    x = float(w.bodies()[0, 9])
+    assert abs(x - 1.0) < 0.01, x
