@@ -131,6 +131,29 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
 int jbgpu_world_ray_test(jbgpu_ctx* ctx, int32_t n, const float* from3, const float* to3, int16_t group,
                          int16_t mask, const int32_t* world, int32_t* obj, float* fraction, float* normal3,
                          float* point3, int32_t on_device);
+/* CollisionWorld.convexSweepTest with a ClosestConvexResultCallback for each of n queries of one convex shape
+ * (CollisionWorld.java:596-646, objectQuerySingle's convex branch :392-424, ClosestConvexResultCallback
+ * :766-810): cast_shape is a shape record (JBGPU_SHAPE_WORDS floats in host memory; sphere, box, capsule or a
+ * hull over the world's point table) swept from from12 to to12 (12 floats per query: row-major basis, then
+ * origin) against the bodies' current transforms by GjkConvexCast with allowedCcdPenetration 0.04.  Planes are
+ * never hit (the reference has no caster for a StaticPlaneShape and throws, :455-485).  group/mask/world as for
+ * jbgpu_world_ray_test.  Per query: obj = body index
+ * of the closest hit or -1, fraction (1 without a hit), normal = hitNormalWorld (unit), point = hitPointWorld.
+ * on_device != 0: every pointer but cast_shape is a device pointer. */
+int jbgpu_world_convex_sweep_test(jbgpu_ctx* ctx, const float* cast_shape, int32_t n, const float* from12,
+                                  const float* to12, int16_t group, int16_t mask, const int32_t* world, int32_t* obj,
+                                  float* fraction, float* normal3, float* point3, int32_t on_device);
+/* CollisionObject.setCcdMotionThreshold / setCcdSweptSphereRadius (CollisionObject.java:280-300) of n bodies
+ * (all 0 when the world is created: no continuous collision detection).  In every step's integrateTransforms
+ * (DiscreteDynamicsWorld.java:679-739) a convex, active, non-static body whose predicted step moves its origin
+ * further than its threshold sweeps a sphere of its swept radius from its old to its predicted transform
+ * (ClosestNotMeConvexResultCallback, :1129-1198: not itself, its own proxy filter, no body it already has contact
+ * points with, no hit for motion along or away from the normal); a hit above 0.0001 of the step re-integrates
+ * it over that fraction of the step.  Bodies are moved in index order, so a body's sweep sees the bodies before
+ * it at their new transforms.  Thresholds and radii must be finite, radii not negative (JBGPU_ERR_ARG, nothing
+ * changed). */
+int jbgpu_world_set_ccd(jbgpu_ctx* ctx, int32_t n, const int32_t* bodies, const float* motion_threshold,
+                        const float* swept_sphere_radius);
 /* stepSimulation(dt, 0) nsteps times: exactly one substep per call.  JBGPU_ERR_CAPACITY with an "EPA workspace
  * overflow" message reports that a pair's GJK-EPA outgrew its workspace in the step just taken (that pair got no
  * contact point); the step has been applied and the world state has advanced, the following steps are not

@@ -144,6 +144,10 @@ struct RayTree {
   DBuf<uint64_t> keys, keysAlt;
   DBuf<float> rec, io;
   DBuf<int> ioInt;
+  // convex sweeps: per query the origins of from and to, then the cast shape's AABB over the sweep (12 floats),
+  // and the cast shape record
+  DBuf<float> sweep, sweepShape;
+  float sweepShapeHost[JBGPU_SHAPE_WORDS] = {};
 };
 
 struct Ctx {
@@ -200,6 +204,12 @@ struct Ctx {
   DBuf<float> nodeRec4;  // 4-wide traversal records np*40 (kRec4F4 float4 per internal node)
   DBuf<int> hugeRange;   // per world: its range of the huge list
   RayTree rt;            // world queries
+  // continuous collision detection: per body (motion threshold, swept sphere radius); the bodies with a
+  // threshold other than 0 in index order; transforms before integration while it runs
+  DBuf<float> ccd, trOld;
+  DBuf<int> ccdList;
+  std::vector<float> ccdHost;
+  int nccd = 0;
   DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
@@ -390,6 +400,11 @@ void bpDestroyProxies(Ctx& c, int n, const int32_t* uids);  // BroadphaseInterfa
 void rayBuildTree(Ctx& c);                               // LBVH over the current shape AABBs
 void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group, int16_t mask, const int* dWorld,
              int* dObj, float* dFrac, float* dNormal, float* dPoint);  // CollisionWorld.rayTest, closest hit
+// CollisionWorld.convexSweepTest with a closest-hit callback for n queries of one convex shape (a shape record
+// in host memory); from/to are 12-float transforms
+void sweepTest(Ctx& c, int n, const float* castShape, const float* dFrom, const float* dTo, int16_t group,
+               int16_t mask, const int* dWorld, int* dObj, float* dFrac, float* dNormal, float* dPoint);
+void ccdClamp(Ctx& c, float dt);  // integrateTransforms' motion clamping, after the integrate kernel
 void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, uint64_t* removed, int64_t capRemoved,
                   int64_t* nRemoved);
 void npDispatchAll(Ctx& c);                              // A8-A16

@@ -15,6 +15,11 @@
 // a queue of the pairs that need it, each with a face/vertex pool in a global
 // workspace slot.  Candidates (valid, normal on B, point on B, depth) are
 // written per pair and folded into the manifold by a final per-pair kernel.
+// World queries live here too: rayTest, convexSweepTest (GjkConvexCast over the
+// query tree's candidates) and the continuous-collision motion clamp of
+// integrateTransforms.
+
+#include <cstring>
 
 #include "internal.h"
 #include "shapes.h"
@@ -283,11 +288,13 @@ __device__ bool tetClosest(V3 p, V3 a, V3 b, V3 c, V3 d, Sub& fr) {
 }
 
 // Vertex storage of the simplex (w, p, q of up to 5 vertices) in LDS, [row][thread] with row = 15 * array
-// + 3 * vertex + component: indexed by the vertex count, it would otherwise sit in scratch memory
+// + 3 * vertex + component and a row stride of S (the block size): indexed by the vertex count, it would
+// otherwise sit in scratch memory
 constexpr int kGjkBlock = 128;
 constexpr int kSimplexRows = 45;
 
-struct Simplex {
+template <int S>
+struct SimplexT {
   int nv;
   float* lw;  // this thread's column of the LDS block
   V3 cP1, cP2, cV, lastW;
@@ -295,14 +302,14 @@ struct Simplex {
   Sub bc;
 
   __device__ V3 get(int arr, int i) const {
-    const float* p = lw + (15 * arr + 3 * i) * kGjkBlock;
-    return v3(p[0], p[kGjkBlock], p[2 * kGjkBlock]);
+    const float* p = lw + (15 * arr + 3 * i) * S;
+    return v3(p[0], p[S], p[2 * S]);
   }
   __device__ void set(int arr, int i, V3 v) {
-    float* p = lw + (15 * arr + 3 * i) * kGjkBlock;
+    float* p = lw + (15 * arr + 3 * i) * S;
     p[0] = v.x;
-    p[kGjkBlock] = v.y;
-    p[2 * kGjkBlock] = v.z;
+    p[S] = v.y;
+    p[2 * S] = v.z;
   }
   __device__ V3 W(int i) const { return get(0, i); }
   __device__ V3 P(int i) const { return get(1, i); }
@@ -425,23 +432,21 @@ struct Simplex {
     return validClosest;
   }
 };
+using Simplex = SimplexT<kGjkBlock>;
 
 // ------------------------------------------------------------------ GJK pair detector
-// epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
-__global__ __launch_bounds__(kGjkBlock) void kGjk(const int* __restrict__ range, const int* __restrict__ list, const uint64_t* __restrict__ keys,
-                                            const float* __restrict__ shapes, const float* __restrict__ verts,
-                                            const float* __restrict__ props, const float* __restrict__ tr,
-                                            float* cand, float* epaState, int* epaQueue,
-                                            unsigned long long* epaCount) {
-  __shared__ float simplexLds[kSimplexRows * kGjkBlock];
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= range[1]) return;
-  int i = list[range[0] + k];
-  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
-  Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
-  Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
-  float md = sa.margin + sb.margin + kContactThreshold;  // ConvexConvexAlgorithm.java:122-123
-  md *= md;
+// GjkPairDetector.getClosestPoints up to the penetration test (GjkPairDetector.java:76-260): the closest points
+// of a and b (cores plus margins) in the frame shifted by off = (ta.o + tb.o) / 2, md the maximum distance
+// squared; checkPenetration and the degenerate-simplex code decide whether a penetration solver would run
+struct GjkOut {
+  bool isValid, checkPenetration;
+  int degen;
+  float distance, margin;
+  V3 normalInB, pointOnA, pointOnB, off;
+};
+template <int S>
+__device__ __forceinline__ void gjkClosest(const Shape& sa, const Tr& ta, const Shape& sb, const Tr& tb,
+                                           const float* __restrict__ verts, float md, SimplexT<S>& vs, GjkOut& out) {
   const float REL_ERROR2 = 1.0e-6f;
   float distance = 0.0f;
   V3 normalInB = v3(0, 0, 0), pointOnA = v3(0, 0, 0), pointOnB = v3(0, 0, 0);
@@ -456,8 +461,6 @@ __global__ __launch_bounds__(kGjkBlock) void kGjk(const int* __restrict__ range,
   int degen = 0;
   float sqd = kInf;
   float margin = marginA + marginB;
-  Simplex vs;
-  vs.lw = simplexLds + threadIdx.x;
   vs.reset();
   for (;;) {
     V3 sepA = mulMtV(ta.b, scl(sep, -1.0f));
@@ -527,22 +530,53 @@ __global__ __launch_bounds__(kGjkBlock) void kGjk(const int* __restrict__ range,
       isValid = true;
     }
   }
-  bool catchDegen = (degen != 0 && ((distance + margin) < 0.01f));
+  out.isValid = isValid;
+  out.checkPenetration = checkPenetration;
+  out.degen = degen;
+  out.distance = distance;
+  out.margin = margin;
+  out.normalInB = normalInB;
+  out.pointOnA = pointOnA;
+  out.pointOnB = pointOnB;
+  out.off = off;
+}
+
+// epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
+// epaState per pair: [needEpa, isValid, distance, pA xyz, pB xyz, nB xyz]
+__global__ __launch_bounds__(kGjkBlock) void kGjk(const int* __restrict__ range, const int* __restrict__ list, const uint64_t* __restrict__ keys,
+                                            const float* __restrict__ shapes, const float* __restrict__ verts,
+                                            const float* __restrict__ props, const float* __restrict__ tr,
+                                            float* cand, float* epaState, int* epaQueue,
+                                            unsigned long long* epaCount) {
+  __shared__ float simplexLds[kSimplexRows * kGjkBlock];
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= range[1]) return;
+  int i = list[range[0] + k];
+  int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
+  Shape sa = bodyShape(shapes, props, a), sb = bodyShape(shapes, props, b);
+  Tr ta = loadTr(tr, a), tb = loadTr(tr, b);
+  float md = sa.margin + sb.margin + kContactThreshold;  // ConvexConvexAlgorithm.java:122-123
+  md *= md;
+  Simplex vs;
+  vs.lw = simplexLds + threadIdx.x;
+  GjkOut g;
+  gjkClosest(sa, ta, sb, tb, verts, md, vs, g);
+  bool catchDegen = (g.degen != 0 && ((g.distance + g.margin) < 0.01f));
   float* es = epaState + 12 * i;
-  if (checkPenetration && (!isValid || catchDegen)) {
+  if (g.checkPenetration && (!g.isValid || catchDegen)) {
     es[0] = 1.0f;
-    es[1] = isValid ? 1.0f : 0.0f;
-    es[2] = distance;
-    st3(es + 3, pointOnA);
-    st3(es + 6, pointOnB);
-    st3(es + 9, normalInB);
+    es[1] = g.isValid ? 1.0f : 0.0f;
+    es[2] = g.distance;
+    st3(es + 3, g.pointOnA);
+    st3(es + 6, g.pointOnB);
+    st3(es + 9, g.normalInB);
     unsigned long long q = atomicAdd(epaCount, 1ull);
     epaQueue[q] = i;
     putCand(cand, i, false, v3(0, 0, 0), v3(0, 0, 0), 0.0f);
     return;
   }
   es[0] = 0.0f;
-  putCand(cand, i, isValid, normalInB, add(pointOnB, off), distance);
+  putCand(cand, i, g.isValid, g.normalInB, add(g.pointOnB, g.off), g.distance);
 }
 
 // ------------------------------------------------------------------ GJK-EPA (GjkEpaSolver)
@@ -1406,10 +1440,14 @@ __device__ __forceinline__ bool rayNeeds(int filt, int group, int mask) {  // Ra
 }
 
 // candidate objects of every ray: filter and rayAabb with the callback's initial fraction (1); the count pass
-// stores the number per ray, the emit pass writes (ray << 32 | object) at the ray's offset
+// stores the number per ray, the emit pass writes (ray << 32 | object) at the ray's offset.  With `expand`
+// (min xyz, max xyz: a convex sweep's cast-shape AABB) every object box is widened by it first, as
+// AabbUtil2.aabbExpand does (node boxes alike: float addition keeps them around their leaves).  A query's
+// from, to and expand values start at stride * query.
 constexpr int kRayStack = 64;
 template <bool Emit>
 __global__ void kRayCandidates(int nr, const float* __restrict__ from, const float* __restrict__ to,
+                               const float* __restrict__ expand, int stride,
                                const int* __restrict__ rworld, int group, int mask, int nn,
                                const float4* __restrict__ rec, const int4* __restrict__ leafExact,
                                const int* __restrict__ hugeRange, const int* __restrict__ hugeList,
@@ -1418,7 +1456,15 @@ __global__ void kRayCandidates(int nr, const float* __restrict__ from, const flo
                                const int64_t* __restrict__ offset, uint64_t* keys, int* err) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
-  const V3 F = v3(from[3 * r], from[3 * r + 1], from[3 * r + 2]), T = v3(to[3 * r], to[3 * r + 1], to[3 * r + 2]);
+  const float* fp = from + (size_t)stride * r;
+  const float* tp = to + (size_t)stride * r;
+  const V3 F = v3(fp[0], fp[1], fp[2]), T = v3(tp[0], tp[1], tp[2]);
+  V3 emn = v3(0, 0, 0), emx = v3(0, 0, 0);
+  if (expand) {
+    const float* ep = expand + (size_t)stride * r;
+    emn = v3(ep[0], ep[1], ep[2]);
+    emx = v3(ep[3], ep[4], ep[5]);
+  }
   const int w = rworld ? rworld[r] : 0;
   if (w < 0 || w > maxWorld) {  // a world id the proxies never had: no candidates, and the call reports it
     if (!Emit) {
@@ -1432,6 +1478,10 @@ __global__ void kRayCandidates(int nr, const float* __restrict__ from, const flo
   const int room = Emit ? count[r] : 0;  // the emit pass writes no more than the count pass found
   auto consider = [&](int o, V3 mn, V3 mx, int f) {
     if (wid[o] != w || !rayNeeds(f, group, mask)) return;  // other worlds are other collision-object lists
+    if (expand) {
+      mn = add(mn, emn);
+      mx = add(mx, emx);
+    }
     float param = 1.0f;
     if (!rayAabb(F, T, mn, mx, param)) return;
     if (Emit && cnt < room) keys[base + cnt] = ((uint64_t)(uint32_t)r << 32) | (uint32_t)o;
@@ -1458,8 +1508,12 @@ __global__ void kRayCandidates(int nr, const float* __restrict__ from, const flo
       int next = -1;
       for (int c = 0; c < 2; c++) {
         const int ch = __float_as_int(c == 0 ? m.x : m.y);
-        const float4 bmn = R[2 * c], bmx = R[2 * c + 1];
+        float4 bmn = R[2 * c], bmx = R[2 * c + 1];
         if (w < __float_as_int(bmn.w) || w > __float_as_int(bmx.w)) continue;
+        if (expand) {
+          bmn.x += emn.x; bmn.y += emn.y; bmn.z += emn.z;
+          bmx.x += emx.x; bmx.y += emx.y; bmx.z += emx.z;
+        }
         if (!segmentHitsBox(F, T, bmn, bmx)) continue;
         if (ch < 0) {
           leaf(-ch - 1);
@@ -1666,6 +1720,305 @@ __global__ void kRayFold(int nr, const int* __restrict__ count, const int64_t* _
   point[3 * r] = hp.x; point[3 * r + 1] = hp.y; point[3 * r + 2] = hp.z;
 }
 
+// ------------------------------------------------------------------ convex sweeps (CollisionWorld.java:596-646)
+// MatrixUtil.invert (MatrixUtil.java:366-394): cofactors over the determinant
+__device__ __forceinline__ float cofac(const M3& m, int r1, int c1, int r2, int c2) {
+  return m.m[3 * r1 + c1] * m.m[3 * r2 + c2] - m.m[3 * r1 + c2] * m.m[3 * r2 + c1];
+}
+__device__ M3 invertM(const M3& a) {
+  const float cx = cofac(a, 1, 1, 2, 2), cy = cofac(a, 1, 2, 2, 0), cz = cofac(a, 1, 0, 2, 1);
+  const float det = a.m[0] * cx + a.m[1] * cy + a.m[2] * cz;
+  const float s = 1.0f / det;
+  M3 r;
+  r.m[0] = cx * s;
+  r.m[1] = cofac(a, 0, 2, 2, 1) * s;
+  r.m[2] = cofac(a, 0, 1, 1, 2) * s;
+  r.m[3] = cy * s;
+  r.m[4] = cofac(a, 0, 0, 2, 2) * s;
+  r.m[5] = cofac(a, 0, 2, 1, 0) * s;
+  r.m[6] = cz * s;
+  r.m[7] = cofac(a, 0, 1, 2, 0) * s;
+  r.m[8] = cofac(a, 0, 0, 1, 1) * s;
+  return r;
+}
+// TransformUtil.calculateVelocity (TransformUtil.java:105-140): origin difference over the step; the rotation
+// basis1 * basis0^-1 as a normalised quaternion gives angle 2 acos(w) about its vector part (x when that vanishes)
+__device__ void calcVelocity(const Tr& t0, const Tr& t1, float timeStep, V3& lin, V3& ang) {
+  lin = scl(sub(t1.o, t0.o), 1.0f / timeStep);
+  const Q4 d = qnor(qFromMatrix(mulMM(t1.b, invertM(t0.b))));
+  const float angle = 2.0f * (float)acos((double)d.w);
+  V3 axis = v3(d.x, d.y, d.z);
+  const float l = len2(axis);
+  if (l < kFltEps * kFltEps)
+    axis = v3(1.0f, 0.0f, 0.0f);
+  else
+    axis = scl(axis, 1.0f / (float)sqrt((double)l));
+  ang = scl(axis, angle / timeStep);
+}
+// CollisionShape.calculateTemporalAabb and getAngularMotionDisc (CollisionShape.java:65-100): the AABB grown on
+// the side of the linear motion, then on all sides by |angvel| dt times the half diagonal of the identity AABB
+// plus the distance of its centre
+__device__ void temporalAabb(const Shape& sh, const Tr& cur, V3 lv, V3 av, float timeStep, V3& mn, V3& mx) {
+  shapeAabb(sh, cur, mn, mx);
+  const V3 lm = scl(lv, timeStep);
+  if (lm.x > 0.0f) mx.x += lm.x; else mn.x += lm.x;
+  if (lm.y > 0.0f) mx.y += lm.y; else mn.y += lm.y;
+  if (lm.z > 0.0f) mx.z += lm.z; else mn.z += lm.z;
+  Tr id;
+  id.b = mident();
+  id.o = v3(0, 0, 0);
+  V3 a0, a1;
+  shapeAabb(sh, id, a0, a1);
+  const float disc = len(sub(a1, a0)) * 0.5f + len(scl(add(a0, a1), 0.5f));
+  const float am = len(av) * disc * timeStep;
+  mn = sub(mn, v3(am, am, am));
+  mx = add(mx, v3(am, am, am));
+}
+// the cast shape's AABB over a sweep (CollisionWorld.java:608-617): velocities over a unit step, the shape at
+// from's rotation (through a quaternion) and the origin
+__device__ void castAabb(const Shape& cs, const Tr& from, const Tr& to, V3& mn, V3& mx) {
+  V3 lv, av;
+  calcVelocity(from, to, 1.0f, lv, av);
+  Tr R;
+  R.b = mFromQ(qFromMatrix(from.b));
+  R.o = v3(0, 0, 0);
+  temporalAabb(cs, R, lv, av, 1.0f, mn, mx);
+}
+__device__ __forceinline__ Shape sphereShape(float r) {  // SphereShape(r): margin = radius
+  Shape s;
+  s.type = JBGPU_SHAPE_SPHERE;
+  s.margin = r * 1.0f;
+  s.dims = v3(r, 0, 0);
+  s.vfirst = s.vcount = 0;
+  s.lmin = s.lmax = s.pn = v3(0, 0, 0);
+  s.pc = 0.0f;
+  return s;
+}
+
+// GjkConvexCast.calcTimeOfImpact (GjkConvexCast.java:67-175) for a moving against b at rest: conservative
+// advancement along the GJK normal (no penetration solver, unlimited distance) while the distance exceeds 0.001,
+// at most 32 steps; the point collector keeps the smallest distance over all its calls (PointCollector.java)
+template <int S>
+__device__ bool gjkConvexCast(const Shape& sa, const Tr& fromA, const Tr& toA, const Shape& sb, const Tr& tb,
+                              const float* __restrict__ verts, float allowedPenetration, SimplexT<S>& vs,
+                              float& fraction, V3& normal, V3& hitPoint) {
+  const V3 r = sub(sub(toA.o, fromA.o), sub(tb.o, tb.o));
+  const float radius = 0.001f;
+  float lambda = 0.0f, lastLambda = 0.0f;
+  int numIter = 0;
+  bool pcHas = false;
+  float pcDist = 1e30f;
+  V3 pcN = v3(0, 0, 0), pcP = v3(0, 0, 0);
+  auto collect = [&](const GjkOut& g) {
+    if (g.isValid && g.distance < pcDist) {
+      pcHas = true;
+      pcN = g.normalInB;
+      pcP = add(g.pointOnB, g.off);
+      pcDist = g.distance;
+    }
+  };
+  Tr ia = fromA, ib = tb;
+  GjkOut g;
+  gjkClosest(sa, ia, sb, ib, verts, kInf, vs, g);
+  collect(g);
+  if (!pcHas) return false;
+  V3 c = pcP, n = pcN;
+  float dist = pcDist;
+  while (dist > radius) {
+    numIter++;
+    if (numIter > 32) return false;
+    const float dLambda = dist / dot(r, n);
+    lambda = lambda - dLambda;
+    if (lambda > 1.0f) return false;
+    if (lambda < 0.0f) return false;
+    if (lambda <= lastLambda) return false;
+    lastLambda = lambda;
+    ia.o = interp3(fromA.o, toA.o, lambda);
+    ib.o = interp3(tb.o, tb.o, lambda);
+    gjkClosest(sa, ia, sb, ib, verts, kInf, vs, g);
+    collect(g);
+    if (!pcHas) return false;
+    if (pcDist < 0.0f) {
+      fraction = lastLambda;
+      normal = pcN;
+      hitPoint = pcP;
+      return true;
+    }
+    c = pcP;
+    n = pcN;
+    dist = pcDist;
+  }
+  if (dot(n, r) >= -allowedPenetration) return false;  // no time of impact for motion away from the normal
+  fraction = lambda;
+  normal = n;
+  hitPoint = c;
+  return true;
+}
+
+// per query: the origins of from and to, and the cast shape's AABB over the sweep
+__global__ void kSweepPrep(int nq, const float* __restrict__ castRec, const float* __restrict__ from,
+                           const float* __restrict__ to, float* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const Shape cs = loadShape(castRec);
+  const Tr f = loadTr(from, q), t = loadTr(to, q);
+  V3 mn, mx;
+  castAabb(cs, f, t, mn, mx);
+  float* o = out + 12 * (size_t)q;
+  st3(o, f.o);
+  st3(o + 3, t.o);
+  st3(o + 6, mn);
+  st3(o + 9, mx);
+}
+
+// per candidate (sorted by query, then object): objectQuerySingle's convex branch (CollisionWorld.java:392-424)
+// up to the fraction test: the cast with allowedCcdPenetration 0.04 and a normal longer than 0.01, normalised.
+// Planes are not convex and have no caster there: never a hit.  record: [hit, fraction, normal xyz, point xyz]
+constexpr int kSweepRec = 8;
+__global__ __launch_bounds__(kGjkBlock) void kSweepCast(int64_t n, const uint64_t* __restrict__ keys,
+                                                        const float* __restrict__ castRec,
+                                                        const float* __restrict__ from, const float* __restrict__ to,
+                                                        const float* __restrict__ shapes,
+                                                        const float* __restrict__ props, const float* __restrict__ tr,
+                                                        const float* __restrict__ verts, float* out) {
+  __shared__ float simplexLds[kSimplexRows * kGjkBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const int q = (int)(k >> 32), o = (int)(k & 0xffffffffu);
+  const Shape cs = loadShape(castRec), sb = bodyShape(shapes, props, o);
+  float* R = out + kSweepRec * i;
+  float fr = 1.0f;
+  V3 nrm = v3(0, 0, 0), pt = v3(0, 0, 0);
+  bool hit = false;
+  if (sb.type != JBGPU_SHAPE_PLANE) {
+    Simplex vs;
+    vs.lw = simplexLds + threadIdx.x;
+    if (gjkConvexCast(cs, loadTr(from, q), loadTr(to, q), sb, loadTr(tr, o), verts, 0.04f, vs, fr, nrm, pt) &&
+        len2(nrm) > 0.0001f) {
+      hit = true;
+      nrm = nor(nrm);
+    }
+  }
+  R[0] = hit ? 1.0f : 0.0f;
+  R[1] = fr;
+  st3(R + 2, nrm);
+  st3(R + 5, pt);
+}
+
+// ClosestConvexResultCallback over a query's candidates in world order: a hit below the running fraction
+// replaces it (CollisionWorld.java:766-810)
+__global__ void kSweepFold(int nq, const int* __restrict__ count, const int64_t* __restrict__ offset,
+                           const uint64_t* __restrict__ keys, const float* __restrict__ recs, int* obj, float* frac,
+                           float* normal, float* point) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  float best = 1.0f;
+  int hit = -1;
+  V3 hn = v3(0, 0, 0), hp = v3(0, 0, 0);
+  const int64_t b = offset[q], e = b + count[q];
+  for (int64_t k = b; k < e; k++) {
+    const float* R = recs + kSweepRec * k;
+    if (R[0] != 0.0f && R[1] < best) {
+      best = R[1];
+      hit = (int)(keys[k] & 0xffffffffu);
+      hn = v3(R[2], R[3], R[4]);
+      hp = v3(R[5], R[6], R[7]);
+    }
+  }
+  obj[q] = hit;
+  frac[q] = best;
+  st3(normal + 3 * (size_t)q, hn);
+  st3(point + 3 * (size_t)q, hp);
+}
+
+// ------------------------------------------------------------------ CCD motion clamping
+// DiscreteDynamicsWorld.integrateTransforms (DiscreteDynamicsWorld.java:679-739) for the bodies with a motion
+// threshold, in body order, after the integrate kernel moved every body: a body whose predicted step exceeds its
+// threshold sweeps a sphere of its swept radius from its old to its predicted origin through the world as it is
+// at that point of the loop (earlier bodies at their new transforms, later ones at their old), with
+// ClosestNotMeConvexResultCallback (:1129-1198): not itself, its own filter, no body it already has contact
+// points with, no hit for motion along or away from the normal.  A hit above 0.0001 of the step re-integrates
+// the body over that fraction of the step.  One workgroup walks the list; its threads split the objects.
+constexpr int kCcdBlock = 512;
+__device__ __forceinline__ bool pairHasPoints(int a, int b, int64_t np, const uint64_t* __restrict__ keys,
+                                              const float* __restrict__ man) {
+  const uint64_t lo = (uint64_t)min(a, b) + 1, hi = (uint64_t)max(a, b) + 1;
+  const uint64_t key = (lo << 32) | hi;
+  int64_t l = 0, e = np;
+  while (l < e) {
+    const int64_t m = (l + e) / 2;
+    if (keys[m] < key) l = m + 1; else e = m;
+  }
+  return l < np && keys[l] == key && ldi(man + (size_t)JBGPU_MANIFOLD_WORDS * l) > 0;
+}
+__global__ __launch_bounds__(kCcdBlock) void kCcd(int nlist, const int* __restrict__ list,
+                                                  const float* __restrict__ ccd, int nb, float dt,
+                                                  const float* __restrict__ trOld, float* tr,
+                                                  const float* __restrict__ vel, const float* __restrict__ props,
+                                                  const int* __restrict__ act, const float* __restrict__ shapes,
+                                                  const float* __restrict__ verts, const int* __restrict__ filt,
+                                                  const int* __restrict__ wid, int64_t np,
+                                                  const uint64_t* __restrict__ pkeys, const float* __restrict__ man) {
+  __shared__ float simplexLds[kSimplexRows * kCcdBlock];
+  __shared__ float red[kCcdBlock / 64];
+  SimplexT<kCcdBlock> vs;
+  vs.lw = simplexLds + threadIdx.x;
+  for (int k = 0; k < nlist; k++) {
+    const int i = list[k];
+    if ((ldi(props + 8 * i + 7) & JBGPU_BODY_STATIC) != 0 || !isActiveState(act[i])) continue;  // not integrated
+    const Shape si = bodyShape(shapes, props, i);
+    if (si.type == JBGPU_SHAPE_PLANE) continue;  // only convex shapes are clamped
+    const Tr from = loadTr(trOld, i), pred = loadTr(tr, i);
+    const float thr = ccd[2 * i];
+    const float sq = thr * thr;
+    const float squareMotion = len2(sub(pred.o, from.o));
+    if (!(sq != 0.0f && sq < squareMotion)) continue;
+    const Shape cast = sphereShape(ccd[2 * i + 1]);
+    V3 cmn, cmx;
+    castAabb(cast, from, pred, cmn, cmx);
+    const V3 relVel = sub(sub(pred.o, from.o), v3(0, 0, 0));
+    const int fi = filt[i];
+    const int group = (int)(short)(fi & 0xffff), mask = (int)(short)(fi >> 16), w = wid[i];
+    float best = 1.0f;
+    for (int j = threadIdx.x; j < nb; j += kCcdBlock) {
+      if (j == i || wid[j] != w || !rayNeeds(filt[j], group, mask)) continue;
+      const Tr tj = loadTr(j < i ? (const float*)tr : trOld, j);
+      const Shape sj = bodyShape(shapes, props, j);
+      V3 mn, mx;
+      shapeAabb(sj, tj, mn, mx);
+      mn = add(mn, cmn);
+      mx = add(mx, cmx);
+      float lambda = 1.0f;
+      if (!rayAabb(from.o, pred.o, mn, mx, lambda)) continue;
+      if (sj.type == JBGPU_SHAPE_PLANE) continue;
+      if (pairHasPoints(i, j, np, pkeys, man)) continue;  // needsResponse holds: i is not static
+      float fr;
+      V3 nrm, pt;
+      if (!gjkConvexCast(cast, from, pred, sj, tj, verts, 0.04f, vs, fr, nrm, pt)) continue;
+      if (!(len2(nrm) > 0.0001f)) continue;
+      if (dot(nor(nrm), relVel) >= -0.0f) continue;
+      if (fr < best) best = fr;
+    }
+    for (int o = 32; o > 0; o >>= 1) best = fminf(best, __shfl_xor(best, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int q = 1; q < kCcdBlock / 64; q++) m = fminf(m, red[q]);
+      if (m < 1.0f && m > 0.0001f) {  // hasHit and the closestHitFraction test
+        const Tr r = integrateTr(from, ld3(vel + 8 * i), ld3(vel + 8 * i + 4), dt * m);
+        float* T = tr + 12 * i;
+        for (int q = 0; q < 9; q++) T[q] = r.b.m[q];
+        T[9] = r.o.x;
+        T[10] = r.o.y;
+        T[11] = r.o.z;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 int64_t npCountContacts(Ctx& c) {
@@ -1769,7 +2122,8 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
   t.offset.ensure((size_t)n + 1);
   int* err = (int*)(c.counter.p + 6);
   JB_CHECK(hipMemsetAsync(err, 0, sizeof(int), s));
-  hipLaunchKernelGGL(kRayCandidates<false>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
+  hipLaunchKernelGGL(kRayCandidates<false>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, (const float*)nullptr, 3,
+                     dWorld, (int)group,
                      (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p, (const int*)t.hugeRange.p,
                      (const int*)t.hugeList.p, (const float*)t.aabb.p, (const int*)c.filt.p,
                      (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)nullptr, (uint64_t*)nullptr, err);
@@ -1786,7 +2140,8 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
     t.keys.ensure(total);
     t.keysAlt.ensure(total);
     t.rec.ensure((size_t)kRayRec * total);
-    hipLaunchKernelGGL(kRayCandidates<true>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, dWorld, (int)group,
+    hipLaunchKernelGGL(kRayCandidates<true>, dim3(gridFor(n)), dim3(256), 0, s, n, dFrom, dTo, (const float*)nullptr, 3,
+                       dWorld, (int)group,
                        (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p,
                        (const int*)t.hugeRange.p, (const int*)t.hugeList.p, (const float*)t.aabb.p,
                        (const int*)c.filt.p, (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)t.offset.p,
@@ -1808,6 +2163,70 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
                        (const int64_t*)t.offset.p, (const uint64_t*)nullptr, (const float*)nullptr, dFrom, dTo,
                        (const float*)t.aabb.p, dObj, dFrac, dNormal, dPoint);
   }
+}
+
+void sweepTest(Ctx& c, int n, const float* castShape, const float* dFrom, const float* dTo, int16_t group,
+               int16_t mask, const int* dWorld, int* dObj, float* dFrac, float* dNormal, float* dPoint) {
+  if (n <= 0) return;
+  rayBuildTree(c);
+  RayTree& t = c.rt;
+  hipStream_t s = c.stream;
+  std::memcpy(t.sweepShapeHost, castShape, sizeof(t.sweepShapeHost));
+  t.sweepShape.ensure(JBGPU_SHAPE_WORDS);
+  JB_CHECK(hipMemcpyAsync(t.sweepShape.p, t.sweepShapeHost, sizeof(t.sweepShapeHost), hipMemcpyHostToDevice, s));
+  t.sweep.ensure(12 * (size_t)n);
+  hipLaunchKernelGGL(kSweepPrep, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)t.sweepShape.p, dFrom, dTo,
+                     t.sweep.p);
+  const float* F = t.sweep.p;
+  t.count.ensure((size_t)n + 1);
+  t.offset.ensure((size_t)n + 1);
+  int* err = (int*)(c.counter.p + 6);
+  JB_CHECK(hipMemsetAsync(err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(kRayCandidates<false>, dim3(gridFor(n)), dim3(256), 0, s, n, F, F + 3, F + 6, 12, dWorld,
+                     (int)group, (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p,
+                     (const int*)t.hugeRange.p, (const int*)t.hugeList.p, (const float*)t.aabb.p,
+                     (const int*)c.filt.p, (const int*)c.world.p, c.maxWorld, t.count.p, (const int64_t*)nullptr,
+                     (uint64_t*)nullptr, err);
+  JB_CHECK(hipMemsetAsync(t.count.p + n, 0, sizeof(int), s));
+  primScanWide(c, t.count.p, t.offset.p, n + 1);
+  int64_t total = 0;
+  int errh = 0;
+  JB_CHECK(hipMemcpyAsync(&total, t.offset.p + n, sizeof(total), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(&errh, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipStreamSynchronize(s));
+  if (errh & 2) throw std::invalid_argument("sweep world id out of range (no such world among the proxies)");
+  if (errh & 1) throw std::runtime_error("sweep traversal stack overflow");
+  if (total >= (int64_t)1 << 31) throw std::length_error("too many sweep candidates in one call");
+  const uint64_t* keys = nullptr;
+  if (total > 0) {
+    t.keys.ensure(total);
+    t.keysAlt.ensure(total);
+    t.rec.ensure((size_t)kSweepRec * total);
+    hipLaunchKernelGGL(kRayCandidates<true>, dim3(gridFor(n)), dim3(256), 0, s, n, F, F + 3, F + 6, 12, dWorld,
+                       (int)group, (int)mask, t.nn, (const float4*)t.nodeRec.p, (const int4*)t.leafExact.p,
+                       (const int*)t.hugeRange.p, (const int*)t.hugeList.p, (const float*)t.aabb.p,
+                       (const int*)c.filt.p, (const int*)c.world.p, c.maxWorld, t.count.p,
+                       (const int64_t*)t.offset.p, t.keys.p, err);
+    int bits = 32;
+    while (bits < 64 && (1ll << (bits - 32)) < (int64_t)n) bits++;
+    Db<uint64_t> kk(t.keys.p, t.keysAlt.p);
+    primSortKeys(c, kk, (int)total, 0, bits);
+    keys = kk.Current();
+    hipLaunchKernelGGL(kSweepCast, dim3(gridFor(total, kGjkBlock)), dim3(kGjkBlock), 0, s, total, keys,
+                       (const float*)t.sweepShape.p, dFrom, dTo, (const float*)c.shapes.p, (const float*)c.props.p,
+                       (const float*)c.tr.p, (const float*)c.verts.p, t.rec.p);
+  }
+  hipLaunchKernelGGL(kSweepFold, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)t.count.p,
+                     (const int64_t*)t.offset.p, keys, (const float*)t.rec.p, dObj, dFrac, dNormal, dPoint);
+}
+
+void ccdClamp(Ctx& c, float dt) {
+  if (c.nccd == 0) return;
+  hipLaunchKernelGGL(kCcd, dim3(1), dim3(kCcdBlock), 0, c.stream, c.nccd, (const int*)c.ccdList.p,
+                     (const float*)c.ccd.p, c.nb, dt, (const float*)c.trOld.p, c.tr.p, (const float*)c.vel.p,
+                     (const float*)c.props.p, (const int*)c.act.p, (const float*)c.shapes.p, (const float*)c.verts.p,
+                     (const int*)c.filt.p, (const int*)c.world.p, c.npairs, (const uint64_t*)c.keys[c.cur].p,
+                     (const float*)c.man[c.cur].p);
 }
 
 }  // namespace jbg

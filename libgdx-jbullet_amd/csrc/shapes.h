@@ -169,4 +169,24 @@ JBD void shapeAabb(const Shape& s, const Tr& t, V3& mn, V3& mx) {
   }
 }
 
+// TransformUtil.integrateTransform (linearmath/TransformUtil.java:64-103): the origin advanced by lv dt, the
+// orientation by the exponential map of av dt (angle limited to a quarter turn per step), renormalised
+JBD Tr integrateTr(const Tr& t, V3 lv, V3 av, float dt) {
+  const float ANGULAR_MOTION_THRESHOLD = 0.5f * kHalfPi;
+  Tr r;
+  r.o = v3(t.o.x + dt * lv.x, t.o.y + dt * lv.y, t.o.z + dt * lv.z);
+  float fAngle = len(av);
+  if (fAngle * dt > ANGULAR_MOTION_THRESHOLD) fAngle = ANGULAR_MOTION_THRESHOLD / dt;
+  V3 axis;
+  if (fAngle < 0.001f)
+    axis = scl(av, 0.5f * dt - (dt * dt * dt) * (0.020833333333f) * fAngle * fAngle);
+  else
+    axis = scl(av, (float)sin((double)(0.5f * fAngle * dt)) / fAngle);
+  Q4 dorn = Q4{axis.x, axis.y, axis.z, (float)cos((double)(fAngle * dt * 0.5f))};
+  Q4 orn0 = qFromMatrix(t.b);
+  Q4 pq = qnor(qmul(dorn, orn0));
+  r.b = mFromQ(pq);
+  return r;
+}
+
 }  // namespace jbg

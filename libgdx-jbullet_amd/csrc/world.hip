@@ -24,7 +24,6 @@ namespace {
 
 std::string g_createError;
 
-constexpr float ANGULAR_MOTION_THRESHOLD = 0.5f * kHalfPi;
 constexpr float MAX_ANGVEL = kHalfPi;
 
 __global__ void kPredict(int n, float dt, const float* __restrict__ tr, const float* __restrict__ props,
@@ -74,25 +73,14 @@ __global__ void kIntegrate(int n, float dt, float* tr, const float* __restrict__
   int a = act[i];
   if (a == JBGPU_ISLAND_SLEEPING || a == JBGPU_DISABLE_SIMULATION) return;
   float* T = tr + 12 * i;
-  V3 lv = ld3(vel + 8 * i), av = ld3(vel + 8 * i + 4);
-  V3 o = v3(T[9] + dt * lv.x, T[10] + dt * lv.y, T[11] + dt * lv.z);
-  float fAngle = len(av);
-  if (fAngle * dt > ANGULAR_MOTION_THRESHOLD) fAngle = ANGULAR_MOTION_THRESHOLD / dt;
-  V3 axis;
-  if (fAngle < 0.001f)
-    axis = scl(av, 0.5f * dt - (dt * dt * dt) * (0.020833333333f) * fAngle * fAngle);
-  else
-    axis = scl(av, (float)sin((double)(0.5f * fAngle * dt)) / fAngle);
-  Q4 dorn = Q4{axis.x, axis.y, axis.z, (float)cos((double)(fAngle * dt * 0.5f))};
-  M3 b;
-  for (int k = 0; k < 9; k++) b.m[k] = T[k];
-  Q4 orn0 = qFromMatrix(b);
-  Q4 pq = qnor(qmul(dorn, orn0));
-  M3 nb = mFromQ(pq);
-  for (int k = 0; k < 9; k++) T[k] = nb.m[k];
-  T[9] = o.x;
-  T[10] = o.y;
-  T[11] = o.z;
+  Tr t;
+  for (int k = 0; k < 9; k++) t.b.m[k] = T[k];
+  t.o = v3(T[9], T[10], T[11]);
+  const Tr r = integrateTr(t, ld3(vel + 8 * i), ld3(vel + 8 * i + 4), dt);
+  for (int k = 0; k < 9; k++) T[k] = r.b.m[k];
+  T[9] = r.o.x;
+  T[10] = r.o.y;
+  T[11] = r.o.z;
 }
 
 // plain device copy of a known size (bandwidth ceiling and counter scale reference)
@@ -398,8 +386,13 @@ void stepPhases(Ctx& c, float dt, int last) {
   if (last >= 3) solverSolve(c, dt);
   if (last >= 4) {
     tBegin(c, T_INTEGRATE);
+    if (c.nccd > 0) {  // the motion clamp sweeps against the transforms before integration
+      c.trOld.ensure(12 * (size_t)c.nb);
+      JB_CHECK(hipMemcpyAsync(c.trOld.p, c.tr.p, 12 * (size_t)c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
     hipLaunchKernelGGL(kIntegrate, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.act.p,
                        c.vel.p);
+    ccdClamp(c, dt);
     tEnd(c, T_INTEGRATE);
     hipLaunchKernelGGL(kActivation, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.deactivationDisabled,
                        c.props.p, c.act.p, c.deact.p, c.vel.p);
@@ -799,6 +792,8 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     c.nb = nbodies;
     c.nj = 0;  // a new world has no constraints
     c.jointHost.clear();
+    c.nccd = 0;  // and no body with continuous collision detection
+    c.ccdHost.assign(2 * (size_t)nbodies, 0.0f);
     upload(c.shapes, shapes, (size_t)JBGPU_SHAPE_WORDS * nshapes, c.stream);
     std::vector<float> vz(3, 0.0f);
     upload(c.verts, nverts > 0 ? verts : vz.data(), nverts > 0 ? 3 * (size_t)nverts : 3, c.stream);
@@ -1072,6 +1067,79 @@ int jbgpu_world_joint_order(jbgpu_ctx* ctx, int32_t* order, int32_t cap, int32_t
     if (cap < 0) throw std::invalid_argument("negative capacity");
     const int m = jointsOrderSolved(ctx->c, order, cap);
     if (n) *n = m;
+  });
+}
+
+int jbgpu_world_set_ccd(jbgpu_ctx* ctx, int32_t n, const int32_t* bodies, const float* motion_threshold,
+                        const float* swept_sphere_radius) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (n < 0 || (n > 0 && (!bodies || !motion_threshold || !swept_sphere_radius)))
+      throw std::invalid_argument("bad CCD arguments");
+    for (int k = 0; k < n; k++) {
+      if (bodies[k] < 0 || bodies[k] >= c.nb) throw std::invalid_argument("CCD body index out of range");
+      if (!std::isfinite(motion_threshold[k]) || !std::isfinite(swept_sphere_radius[k]) ||
+          swept_sphere_radius[k] < 0.0f)
+        throw std::invalid_argument("CCD motion threshold and swept sphere radius must be finite, the radius >= 0");
+    }
+    for (int k = 0; k < n; k++) {
+      c.ccdHost[2 * (size_t)bodies[k]] = motion_threshold[k];
+      c.ccdHost[2 * (size_t)bodies[k] + 1] = swept_sphere_radius[k];
+    }
+    std::vector<int> list;
+    for (int i = 0; i < c.nb; i++)
+      if (c.ccdHost[2 * (size_t)i] != 0.0f) list.push_back(i);
+    c.nccd = (int)list.size();
+    upload(c.ccd, c.ccdHost.data(), c.ccdHost.size(), c.stream);
+    upload(c.ccdList, list.data(), list.size(), c.stream);
+    JB_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+int jbgpu_world_convex_sweep_test(jbgpu_ctx* ctx, const float* cast_shape, int32_t n, const float* from12,
+                                  const float* to12, int16_t group, int16_t mask, const int32_t* world, int32_t* obj,
+                                  float* fraction, float* normal3, float* point3, int32_t on_device) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    requireWorld(c);
+    if (!cast_shape || n < 0 || (n > 0 && (!from12 || !to12 || !obj || !fraction || !normal3 || !point3)))
+      throw std::invalid_argument("bad sweep arguments");
+    const Shape cs = loadShape(cast_shape);
+    if (cs.type != JBGPU_SHAPE_SPHERE && cs.type != JBGPU_SHAPE_BOX && cs.type != JBGPU_SHAPE_CAPSULE &&
+        cs.type != JBGPU_SHAPE_HULL)
+      throw std::invalid_argument("the cast shape must be convex (sphere, box, capsule or hull)");
+    if (cs.type == JBGPU_SHAPE_HULL && (cs.vfirst < 0 || cs.vcount <= 0 || cs.vfirst + cs.vcount > c.nv))
+      throw std::invalid_argument("cast hull points out of the world's point table");
+    if (n == 0) return;
+    if (world && !on_device)
+      for (int i = 0; i < n; i++)
+        if (world[i] < 0 || world[i] > c.maxWorld) throw std::invalid_argument("sweep world id out of range");
+    if (on_device) {
+      sweepTest(c, n, cast_shape, from12, to12, group, mask, world, obj, fraction, normal3, point3);
+      JB_CHECK(hipStreamSynchronize(c.stream));
+      return;
+    }
+    hipStream_t s = c.stream;
+    RayTree& t = c.rt;
+    t.io.ensure(31 * (size_t)n);
+    t.ioInt.ensure(2 * (size_t)n);
+    float* dFrom = t.io.p;
+    float* dTo = dFrom + 12 * (size_t)n;
+    float* dFrac = dTo + 12 * (size_t)n;
+    float* dNormal = dFrac + n;
+    float* dPoint = dNormal + 3 * (size_t)n;
+    int* dObj = t.ioInt.p;
+    int* dWorld = world ? t.ioInt.p + n : nullptr;
+    JB_CHECK(hipMemcpyAsync(dFrom, from12, 12 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    JB_CHECK(hipMemcpyAsync(dTo, to12, 12 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+    if (world) JB_CHECK(hipMemcpyAsync(dWorld, world, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+    sweepTest(c, n, cast_shape, dFrom, dTo, group, mask, dWorld, dObj, dFrac, dNormal, dPoint);
+    JB_CHECK(hipMemcpyAsync(obj, dObj, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(fraction, dFrac, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(normal3, dNormal, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipMemcpyAsync(point3, dPoint, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
   });
 }
 

@@ -61,6 +61,9 @@ SYMBOLS = {
     "jbgpu_bp_destroy_proxies": (C.c_int, [_vp, C.c_int32, _vp]),
     "jbgpu_world_ray_test": (C.c_int, [_vp, C.c_int32, _vp, _vp, C.c_int16, C.c_int16, _vp, _vp, _vp, _vp, _vp,
                                        C.c_int32]),
+    "jbgpu_world_convex_sweep_test": (C.c_int, [_vp, _f, C.c_int32, _vp, _vp, C.c_int16, C.c_int16, _vp, _vp, _vp,
+                                                _vp, _vp, C.c_int32]),
+    "jbgpu_world_set_ccd": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     "jbgpu_bp_get_pair_deltas": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(C.c_int64), _vp, C.c_int64,
                                            C.POINTER(C.c_int64)]),
     "jbgpu_bp_get_proxies": (C.c_int, [_vp, _f, C.POINTER(C.c_int32)]),
@@ -346,6 +349,31 @@ class Context:
         self._ck(load().jbgpu_world_ray_test(self.h, int(n), v(frm), v(to), int(np.int16(group)), int(np.int16(mask)),
                                              None if world is None else v(world), v(obj), v(fr), v(nrm), v(pt), 1),
                  "world_ray_test")
+
+    def convex_sweep(self, shape, frm, to, group=1, mask=-1, world=None):
+        """convexSweepTest of a convex shape record with a closest-hit callback per query (from/to: 12-float
+        transforms, basis rows then origin): (body or -1, fraction, normal, point)."""
+        frm = f32(frm).reshape(-1, 12)
+        to = f32(to).reshape(-1, 12)
+        n = len(frm)
+        if len(to) != n:
+            raise ValueError("from and to differ in length")
+        obj = np.zeros(n, np.int32)
+        fr = np.zeros(n, np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        pt = np.zeros((n, 3), np.float32)
+        w = None if world is None else np.ascontiguousarray(world, np.int32)
+        self._ck(load().jbgpu_world_convex_sweep_test(self.h, f32(shape).reshape(-1), n, _p(frm), _p(to),
+                                                      int(np.int16(group)), int(np.int16(mask)), _p(w), _p(obj),
+                                                      _p(fr), _p(nrm), _p(pt), 0), "world_convex_sweep_test")
+        return obj, fr, nrm, pt
+
+    def set_ccd(self, bodies, threshold, radius):
+        """setCcdMotionThreshold / setCcdSweptSphereRadius of the given bodies (scalars or one value per body)."""
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        t = np.broadcast_to(f32(threshold), b.shape).copy()
+        r = np.broadcast_to(f32(radius), b.shape).copy()
+        self._ck(load().jbgpu_world_set_ccd(self.h, len(b), _p(b), _p(t), _p(r)), "world_set_ccd")
 
     def step_until(self, phase, dt=1.0 / 60.0):
         self._ck(load().jbgpu_world_step_until(self.h, dt, int(phase)), "world_step_until")

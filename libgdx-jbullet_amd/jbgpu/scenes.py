@@ -446,6 +446,58 @@ def hinges(seed=1, n_chains=12, links=5, n_doors=24, n_wheels=12, drops=120):
                   spin=np.array(spin, np.float32))
 
 
+def projectiles(seed=1, lanes=6, shots=40, speed=(60.0, 300.0)):
+    """Fast small bodies fired at thin walls, for continuous collision detection.  Per lane (2.5 m apart along
+    z): a static wall 0.1 m thick at x = 8, a dynamic plank 0.1 m thick standing at x = 14 and a stack of three
+    boxes at x = 20, on a floor.  `shots` projectiles (spheres of radius 0.1-0.15, boxes of half-extent 0.1,
+    tetrahedral hulls of 0.15) start at x = -2 to 0, 0.4-2 m up, aimed along +x at 60-300 m/s (2-5 m per 1/60 s
+    step, far more than the wall or plank is thick) with a little spread, some spinning; a few are fired
+    downwards at the floor.  meta["velocity"], meta["angular"]: initial velocities; meta["ccd"] = (bodies,
+    motion threshold, swept sphere radius) for the projectiles."""
+    rng = np.random.default_rng(seed)
+    b = _Builder("projectiles")
+    plane = b.shape(PLANE, [0.0, 1.0, 0.0, 0.0])
+    b.body(plane, IDENT, np.zeros(3, np.float32), 0.0)
+    wall = b.shape(BOX, [0.05, 2.0, 1.0])
+    plank = b.shape(BOX, [0.05, 1.0, 0.8])
+    block = b.shape(BOX, [0.4, 0.4, 0.4])
+    for k in range(lanes):
+        z = (k - (lanes - 1) / 2.0) * 2.5
+        b.body(wall, IDENT, np.array([8.0, 2.0, z], np.float32), 0.0)
+        b.body(plank, IDENT, np.array([14.0, 1.0, z], np.float32), 2.0)
+        for m in range(3):
+            b.body(block, IDENT, np.array([20.0, 0.4 + 0.8 * m, z], np.float32), 5.0)
+    nstat = len(b.bs)
+    vel = [np.zeros(3, np.float32)] * nstat
+    ang = [np.zeros(3, np.float32)] * nstat
+    sph = [b.shape(SPHERE, [r]) for r in (0.1, 0.12, 0.15)]
+    cube = b.shape(BOX, [0.1, 0.1, 0.1])
+    tet = b.hull(np.array([[0.15, 0.15, 0.15], [-0.15, -0.15, 0.15], [-0.15, 0.15, -0.15], [0.15, -0.15, -0.15]],
+                          np.float32))
+    ccd_bodies, thr, rad = [], [], []
+    span = (lanes - 1) / 2.0 * 2.5 + 0.5
+    for i in range(shots):
+        kind = i % 5
+        shape = sph[i % 3] if kind < 3 else (cube if kind == 3 else tet)
+        pos = np.array([rng.uniform(-2.0, 0.0), rng.uniform(0.4, 2.0), rng.uniform(-span, span)], np.float32)
+        v = np.float32(rng.uniform(*speed))
+        if i % 8 == 7:  # fired down at the floor
+            d = np.array([rng.uniform(0.2, 0.6), -1.0, rng.uniform(-0.2, 0.2)], np.float32)
+            pos[1] = rng.uniform(3.0, 6.0)
+        else:
+            d = np.array([1.0, rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05)], np.float32)
+        d = d / np.float32(np.linalg.norm(d))
+        ccd_bodies.append(len(b.bs))
+        b.body(shape, _quat_basis(_random_quats(rng, 1))[0] if kind >= 3 else IDENT, pos, 0.2)
+        vel.append((d * v).astype(np.float32))
+        ang.append((rng.normal(size=3) * 20.0).astype(np.float32) if i % 3 == 0 else np.zeros(3, np.float32))
+        thr.append(0.05)
+        rad.append(0.08)
+    ccd = (np.array(ccd_bodies, np.int32), np.array(thr, np.float32), np.array(rad, np.float32))
+    return b.done(dt=1 / 60, iterations=10, steps=90, velocity=np.array(vel, np.float32),
+                  angular=np.array(ang, np.float32), ccd=ccd)
+
+
 def build_records(scene, builder, gravity=GRAVITY):
     """Shape and body records for ``scene``.
 

@@ -549,6 +549,8 @@ void orc_world_set_bodies(void* h, const float* in) {
     Body b = bodyFromRecord(in + i * JBGPU_BODY_WORDS);
     b.islandTag = w->bodies[i].islandTag;
     b.companionId = w->bodies[i].companionId;
+    b.ccdMotionThreshold = w->bodies[i].ccdMotionThreshold;  // not part of the record
+    b.ccdSweptSphereRadius = w->bodies[i].ccdSweptSphereRadius;
     updateInertiaTensor(b);
     w->bodies[i] = b;
   }
