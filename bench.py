@@ -387,8 +387,9 @@ def roofline(ks, args):
 def batched(args):
     """Batched independent worlds: towers of spheres and boxes (sizes as in the mixed pile), own floor plane per
     world, 10 iterations.  The world -> workgroup mapping of the solve is autotuned first (untimed steps), then
-    --batched-steps steps are timed.  CPU baseline: the oracle stepping the first worlds of the same state (as
-    4096 separate worlds would be stepped one after another), about --cpu-seconds of work."""
+    --batched-steps steps run with phase timing, as many as ordinary launches, and as many timed as replayed
+    hipGraphs (the headline).  CPU baseline: the oracle stepping the first worlds of the same state (as 4096
+    separate worlds would be stepped one after another), about --cpu-seconds of work."""
     import jbgpu
     from jbgpu import autotune, scenes
     t = time.time()
@@ -409,26 +410,42 @@ def batched(args):
         tune = autotune.world_blocks(ctx, log=log)
     first = warm + tune["steps"] + 1
     snap = dict(bodies=ctx.bodies(), proxies=ctx.bp_proxies(), pairs=ctx.pairs())
+    n = args.batched_steps
+    # 1. ordinary steps with event timing between the phases (the phase breakdown)
     ctx.timing(not args.trace_window)
+    ctx.step(n=n)
+    st_phases = ctx.stats()
+    regions, nsteps = ctx.timing_get()
+    ctx.timing(False)
+    # 2. ordinary steps, no events
+    t0 = time.perf_counter()
+    ctx.step(n=n)
+    ordinary_secs = time.perf_counter() - t0
+    # 3. the step as replayed hipGraphs (the configuration's launch form): two untimed steps capture both graphs
+    ctx.set_step_graph(True)
+    ctx.step(n=2)
+    graph_first = first + 2 * n + 2
     if args.trace_window:
         ctx.measure_copy(TRACE_MARK_BYTES, 1)  # window markers for --summarize-trace
     t0 = time.perf_counter()
-    ctx.step(n=args.batched_steps)
+    ctx.step(n=n)
     secs = time.perf_counter() - t0
     if args.trace_window:
         ctx.measure_copy(TRACE_MARK_BYTES, 1)
     st = ctx.stats()
-    regions, nsteps = ctx.timing_get()
     ctx.close()
-    out = {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": args.batched_steps,
-           "timed_window": f"steps {first}-{first + args.batched_steps - 1}",
-           "ms_per_step": round(1e3 * secs / args.batched_steps, 3),
-           "body_steps_per_s": round(nb * args.batched_steps / secs, 1), "worlds_steps_per_s":
-           round(args.batched_worlds * args.batched_steps / secs, 1), "last_step_pairs": st["pairs"],
-           "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
-           "last_step_epa_calls": st["epa_calls"], "autotune": tune,
+    out = {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": n, "launch_form": "hipGraph replay",
+           "timed_window": f"steps {graph_first}-{graph_first + n - 1}",
+           "ms_per_step": round(1e3 * secs / n, 3),
+           "body_steps_per_s": round(nb * n / secs, 1), "worlds_steps_per_s":
+           round(args.batched_worlds * n / secs, 1),
+           "ordinary_launches": {"ms_per_step": round(1e3 * ordinary_secs / n, 3),
+                                 "timed_window": f"steps {first + n}-{first + 2 * n - 1}"},
+           "last_step_pairs": st["pairs"], "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
+           "last_step_epa_calls": st["epa_calls"], "graph_redos": st["graph_redos"], "autotune": tune,
            "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0},
-           "phase_note": "event timing on (events between phases) during the timed steps"}
+           "phase_note": f"ordinary launches with event timing between the phases, steps {first}-{first + n - 1} "
+                         f"({st_phases['pairs']} pairs at the last)"}
     if not args.no_cpu:
         out["cpu_baseline"] = batched_cpu_baseline(shapes, verts, bodies, sc.world, snap, args, first)
     return out

@@ -64,6 +64,8 @@ typedef struct jbgpu_stats {
   int64_t rows;         /* normal rows (contacts with distance <= 0) of the last solve */
   int64_t epa_retries;  /* EPA jobs of the last pass that outgrew the first-pass workspace and ran again */
   int64_t epa_huge;     /* ... of those, the ones that also outgrew the large LDS workspace (global-memory tier) */
+  int64_t graph_redos;  /* graph-captured steps that outgrew their pair capacity and were finished on the ordinary
+                           path (since the context was created) */
 } jbgpu_stats;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -159,6 +161,14 @@ int jbgpu_world_set_ccd(jbgpu_ctx* ctx, int32_t n, const int32_t* bodies, const 
  * contact point); the step has been applied and the world state has advanced, the following steps are not
  * affected. */
 int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps);
+/* enable != 0: jbgpu_world_step runs each step as a replayed hipGraph (no counterpart in the reference: a launch
+ * form of the same stepSimulation).  Every launch of the captured step is shaped by capacities (bodies, a pair
+ * capacity of 1.5x the last pair count + 4096) with the per-step counts kept on the device, so the steps of one
+ * call run without host waits; results are bit-identical to the ordinary step.  A step that finds more pairs than
+ * the capacity is finished on the ordinary path, the capacity raised and the step captured again.  Needs the
+ * world-block solve (jbgpu_solver_set_world_blocks), deactivation disabled, no constraints, no tolerance mode,
+ * no learned warm start and no phase timing (JBGPU_ERR_ARG naming what is missing). */
+int jbgpu_world_set_step_graph(jbgpu_ctx* ctx, int32_t enable);
 /* One step run up to and including phase p: 0 predictUnconstraintMotion, 1 AABB update +
  * calculateOverlappingPairs, 2 dispatchAllCollisionPairs, 3 solveConstraints, 4 full step. */
 int jbgpu_world_step_until(jbgpu_ctx* ctx, float dt, int32_t phase);

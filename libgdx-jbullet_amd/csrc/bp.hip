@@ -121,9 +121,9 @@ __device__ void writeCandidate(const BpParams& P, int i, const float* aabb, cons
 
 __global__ void kSetAabbBodies(BpParams P, int n, const float* __restrict__ shapes, const float* __restrict__ tr,
                                const float* __restrict__ props, int* act, int* stage, float* leaf, float* aabb, int* q,
-                               float* cand, int* isHuge) {
+                               float* cand, int* isHuge, const int* __restrict__ abortFlag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (abortFlag && *abortFlag)) return;
   int a = act[i];
   bool active = (a != JBGPU_ISLAND_SLEEPING) && (a != JBGPU_DISABLE_SIMULATION);
   bool update = false;
@@ -323,9 +323,10 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
 // sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
 // leafExact[s]: the exact-test data of sorted leaf s in one 32-byte record: the stored AABB (Dbvt,
 // Simple) or the quantised box (AxisSweep3) as 6 words, the group/mask word and the proxy index
-__global__ void kGatherLeaves(int n, const int* __restrict__ order, const float* __restrict__ cand,
+__global__ void kGatherLeaves(int n, const int* __restrict__ dn, const int* __restrict__ order, const float* __restrict__ cand,
                               const int* __restrict__ world, float* leafBox, int mode, const float* __restrict__ aabb,
                               const int* __restrict__ q, const int* __restrict__ filt, int4* __restrict__ leafExact) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int o = order[s];
@@ -349,7 +350,8 @@ __device__ __forceinline__ int deltaK(const uint64_t* code, int n, int i, int j)
 }
 
 // Karras (2012) hierarchy: internal node i in [0, n-2]; child < 0 encodes leaf -(child+1)
-__global__ void kBuild(int n, const uint64_t* __restrict__ code, int* child, int* first, int* last) {
+__global__ void kBuild(int n, const int* __restrict__ dn, const uint64_t* __restrict__ code, int* child, int* first, int* last) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int d = (deltaK(code, n, i, i + 1) - deltaK(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -381,7 +383,8 @@ __global__ void kBuild(int n, const uint64_t* __restrict__ code, int* child, int
 }
 
 // union of 32 consecutive boxes (8 floats each) per output box; the last group may be partial
-__global__ __launch_bounds__(256) void kChunk(int n, const float* __restrict__ src, float* __restrict__ dst) {
+__global__ __launch_bounds__(256) void kChunk(int n, const int* __restrict__ dn, const float* __restrict__ src, float* __restrict__ dst) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const float inf = __builtin_huge_valf();
   float4 mn = make_float4(inf, inf, inf, 0.0f), mx = make_float4(-inf, -inf, -inf, 0.0f);
@@ -414,10 +417,11 @@ struct BoxAcc {
 
 // internal node boxes: the union of the node's sorted-leaf range [first, last], assembled from
 // leaves at the ragged ends and from full 32-, 1024- and 32768-leaf chunk boxes in between
-__global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__ first, const int* __restrict__ last,
+__global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__ dn, const int* __restrict__ first, const int* __restrict__ last,
                                                   const float* __restrict__ leafBox, const float* __restrict__ c1,
                                                   const float* __restrict__ c2, const float* __restrict__ c3,
                                                   float* __restrict__ nodeBox) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const float inf = __builtin_huge_valf();
@@ -440,9 +444,10 @@ __global__ __launch_bounds__(256) void kNodeBoxes(int n, const int* __restrict__
 // traversal record of internal node i: both children's boxes (.w lanes: world range; a leaf's world in
 // both), then (child0, child1, last0, last1) where child < 0 is leaf -child-1 and last is the highest
 // sorted leaf below the child; one visit reads the five float4 at once
-__global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict__ child, const int* __restrict__ last,
+__global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict__ dn, const int* __restrict__ child, const int* __restrict__ last,
                                                     const float* __restrict__ leafBox,
                                                     const float* __restrict__ nodeBox, float4* __restrict__ rec) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int ch[2], lst[2];
@@ -462,7 +467,8 @@ __global__ __launch_bounds__(256) void kNodeRecords(int n, const int* __restrict
 // 4-wide record of internal node i from the binary records: the two children of each internal child and every
 // leaf child itself (2 to 4 entries), so a traversal step tests up to four boxes behind one dependent load and
 // the descent takes half as many steps
-__global__ __launch_bounds__(256) void kNodeRecords4(int n, const float4* __restrict__ rec, float4* __restrict__ rec4) {
+__global__ __launch_bounds__(256) void kNodeRecords4(int n, const int* __restrict__ dn, const float4* __restrict__ rec, float4* __restrict__ rec4) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const float4* R = rec + kRecF4 * i;
@@ -588,12 +594,13 @@ struct TravRec<4> {  // kNodeRecords4: ids in R[8], last leaves in R[9]
 };
 
 template <int W>
-__global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ order,
+__global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __restrict__ dn, const int* __restrict__ order,
                                                             const float* __restrict__ leafBox,
                                                             const float4* __restrict__ recs,
                                                             const int4* __restrict__ leafExact, int mode, int bits,
                                                             unsigned long long* counter, uint64_t* out, int64_t cap,
                                                             int* err, uint64_t* __restrict__ stage) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   __shared__ unsigned int sCount;
   __shared__ unsigned long long sBase;
   // the top kLdsStack entries of every thread's stack live in LDS ([depth][thread]: no bank conflicts),
@@ -688,7 +695,8 @@ __global__ __launch_bounds__(kTraverseBlock) void kTraverse(int n, const int* __
 
 // per world w: range[2w], range[2w + 1] = its part of the huge list (sorted by world, then index); the
 // table is cleared to empty ranges beforehand
-__global__ void kHugeRanges(int nh, const int* __restrict__ hugeWorld, int* range) {
+__global__ void kHugeRanges(int nh, const int* __restrict__ dn, const int* __restrict__ hugeWorld, int* range) {
+  if (dn) nh = *dn;  // graph-captured steps: the count on the device
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nh) return;
   const int w = hugeWorld[k];
@@ -766,8 +774,16 @@ __global__ void kMarkLarge(int n, float factor, int cap, const float* __restrict
   if (isLarge(i, factor, cand, stats)) isHuge[i] = 2;
 }
 
-__global__ void kHugeWorlds(int nh, const int* __restrict__ list, const int* __restrict__ world, int* wk) {
+__global__ void kHugeWorlds(int nh, const int* __restrict__ dn, int* leafCounts, const int* __restrict__ list, const int* __restrict__ world, int* wk) {
+  const int np = nh;
+  if (dn) nh = *dn;  // graph-captured steps: the count on the device
   int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leafCounts && k == 0) {  // graph-captured steps: tree leaves (proxies - huge) and the chunk counts
+    const int nn = np - nh;
+    leafCounts[0] = nn;
+    leafCounts[1] = (nn + 31) / 32;
+    leafCounts[2] = (leafCounts[1] + 31) / 32;
+  }
   if (k < nh) wk[k] = world[list[k]];
 }
 
@@ -798,9 +814,13 @@ __global__ void kPairTypes(int64_t np, const uint64_t* __restrict__ keys, const 
   ptype[i] = pairTypeOf((int)(key >> 32) - 1, (int)(key & 0xffffffffu) - 1, shapes, props, planeFirst);
 }
 
-__global__ void kRemap(int64_t np, const uint64_t* __restrict__ packed, int bits, uint64_t* keysNew, float* manNew,
-                       int64_t nOld, const uint64_t* __restrict__ keysOld, const float* __restrict__ manOld,
+__global__ void kRemap(int64_t np, const int* __restrict__ dnp, const uint64_t* __restrict__ packed, int bits,
+                       uint64_t* keysNew, float* manNew, int64_t nOld, const int* __restrict__ dnOld,
+                       const uint64_t* __restrict__ keysOld, const float* __restrict__ manOld,
                        const float* __restrict__ shapes, const float* __restrict__ props, int* ptype) {
+  if (dnp) np = *dnp;  // graph-captured steps: both counts on the device
+  if (dnOld) nOld = *dnOld;
+  if ((int64_t)blockIdx.x * blockDim.x >= np) return;  // whole blocks past the pairs (np = 0 included)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = i < np;  // every lane stays to the end: the point copy below is done by the whole wave
   if (!live) i = np - 1;
@@ -922,7 +942,7 @@ void bpUpdateFromBodies(Ctx& c) {
   BpParams P = makeParams(c);
   tBegin(c, T_SETAABB);
   hipLaunchKernelGGL(kSetAabbBodies, dim3(gridFor(c.np)), dim3(256), 0, c.stream, P, c.np, c.shapes.p, c.tr.p,
-                     c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
+                     c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)nullptr);
   tEnd(c, T_SETAABB);
   c.stageCurrent = P.newStage;
   c.rotationPending = true;
@@ -1019,7 +1039,7 @@ void rayBuildTree(Ctx& c) {
   t.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
   JB_CHECK(hipMemsetAsync(t.hugeRange.p, 0, 2 * ((size_t)c.maxWorld + 1) * sizeof(int), s));
   if (t.nHuge > 0) {
-    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)t.hugeList.p,
+    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)nullptr, (int*)nullptr, (const int*)t.hugeList.p,
                        (const int*)c.world.p, t.hugeWorld.p);
     if (c.worldBits > 0) {
       Db<int> hk(t.hugeWorld.p, t.hugeWorldAlt.p);
@@ -1030,27 +1050,27 @@ void rayBuildTree(Ctx& c) {
       if (hv.Current() != t.hugeList.p)
         JB_CHECK(hipMemcpyAsync(t.hugeList.p, hv.Current(), t.nHuge * sizeof(int), hipMemcpyDeviceToDevice, s));
     }
-    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)t.hugeWorld.p,
+    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(t.nHuge)), dim3(256), 0, s, t.nHuge, (const int*)nullptr, (const int*)t.hugeWorld.p,
                        t.hugeRange.p);
   }
   const int nn = t.nn;
   if (nn >= 1)
-    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, (const float*)t.cand.p,
+    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, (const float*)t.cand.p,
                        (const int*)c.world.p, t.leafBox.p, (int)JBGPU_BP_DBVT, (const float*)t.aabb.p,
                        (const int*)nullptr, (const int*)c.filt.p, (int4*)t.leafExact.p);
   if (nn >= 2) {
-    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, t.nodeChild.p, t.nodeFirst.p,
+    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, code, t.nodeChild.p, t.nodeFirst.p,
                        t.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32;
     float* c1 = t.chunkBox.p;
     float* c2 = c1 + 8 * (size_t)n1;
     float* c3 = c2 + 8 * (size_t)n2;
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, t.leafBox.p, c1);
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, c1, c2);
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, c2, c3);
-    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, t.nodeFirst.p, t.nodeLast.p,
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, t.leafBox.p, c1);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, (const int*)nullptr, c1, c2);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, (const int*)nullptr, c2, c3);
+    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, t.nodeFirst.p, t.nodeLast.p,
                        t.leafBox.p, c1, c2, c3, t.nodeBox.p);
-    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)t.nodeChild.p,
+    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, (const int*)t.nodeChild.p,
                        (const int*)t.nodeLast.p, (const float*)t.leafBox.p, (const float*)t.nodeBox.p,
                        (float4*)t.nodeRec.p);
   }
@@ -1106,7 +1126,7 @@ void bpFindPairs(Ctx& c) {
   c.sortedOrder = order;
   const int nn = n - c.nHuge;  // leaves in the tree
   if (c.nHuge > 0) {  // per-world lookup: the index-ordered list sorted stably by world, i.e. by (world, index)
-    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)c.hugeList.p,
+    hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)nullptr, (int*)nullptr, (const int*)c.hugeList.p,
                        (const int*)c.world.p, c.hugeWorld.p);
     if (c.worldBits > 0) {
       Db<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
@@ -1121,30 +1141,30 @@ void bpFindPairs(Ctx& c) {
   if (c.nHuge > 0) {
     c.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
     JB_CHECK(hipMemsetAsync(c.hugeRange.p, 0, 2 * ((size_t)c.maxWorld + 1) * sizeof(int), s));
-    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)c.hugeWorld.p,
+    hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(c.nHuge)), dim3(256), 0, s, c.nHuge, (const int*)nullptr, (const int*)c.hugeWorld.p,
                        c.hugeRange.p);
   }
   if (nn >= 2) {
-    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, order, c.cand.p, c.world.p,
+    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, c.cand.p, c.world.p,
                        c.leafBox.p, c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p,
                        (int4*)c.leafExact.p);
-    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, code, c.nodeChild.p, c.nodeFirst.p,
+    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, code, c.nodeChild.p, c.nodeFirst.p,
                        c.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32, n3 = (n2 + 31) / 32;
     float* c1 = c.chunkBox.p;
     float* c2 = c1 + 8 * (size_t)n1;
     float* c3 = c2 + 8 * (size_t)n2;
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, c.leafBox.p, c1);
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, c1, c2);
-    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, c2, c3);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, c.leafBox.p, c1);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, (const int*)nullptr, c1, c2);
+    hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, (const int*)nullptr, c2, c3);
     (void)n3;
-    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, c.nodeFirst.p, c.nodeLast.p,
+    hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, c.nodeFirst.p, c.nodeLast.p,
                        c.leafBox.p, c1, c2, c3, c.nodeBox.p);
-    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)c.nodeChild.p,
+    hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, (const int*)c.nodeChild.p,
                        (const int*)c.nodeLast.p, (const float*)c.leafBox.p, (const float*)c.nodeBox.p,
                        (float4*)c.nodeRec.p);
     if (wide)
-      hipLaunchKernelGGL(kNodeRecords4, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const float4*)c.nodeRec.p,
+      hipLaunchKernelGGL(kNodeRecords4, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, (const float4*)c.nodeRec.p,
                          (float4*)c.nodeRec4.p);
   }
   tEnd(c, T_BVH);
@@ -1157,7 +1177,7 @@ void bpFindPairs(Ctx& c) {
     if (attempt == 0) tBegin(c, T_TRAVERSE);
     if (nn >= 2)
       hipLaunchKernelGGL(wide ? kTraverse<4> : kTraverse<2>, dim3(gridFor(nn, kTraverseBlock)), dim3(kTraverseBlock),
-                         0, s, nn, order, c.leafBox.p, (const float4*)(wide ? c.nodeRec4.p : c.nodeRec.p),
+                         0, s, nn, (const int*)nullptr, order, c.leafBox.p, (const float4*)(wide ? c.nodeRec4.p : c.nodeRec.p),
                          (const int4*)c.leafExact.p, c.bpMode, bits, c.counter.p, c.rawKeys.p, c.pairCap,
                          (int*)(c.counter.p + 2), c.pairStage.p);
     if (attempt == 0) tEnd(c, T_TRAVERSE);
@@ -1327,11 +1347,131 @@ void bpRemapManifolds(Ctx& c) {
   c.ptype.ensure(std::max<int64_t>(c.npairs, 1));
   tBegin(c, T_REMAP);
   if (c.npairs > 0)
-    hipLaunchKernelGGL(kRemap, dim3(gridFor(c.npairs)), dim3(256), 0, c.stream, c.npairs, c.rawKeys.p, bits,
-                       c.keys[nxt].p, c.man[nxt].p, c.npairsOld, c.keys[c.cur].p, c.man[c.cur].p, c.shapes.p,
+    hipLaunchKernelGGL(kRemap, dim3(gridFor(c.npairs)), dim3(256), 0, c.stream, c.npairs, (const int*)nullptr, c.rawKeys.p, bits,
+                       c.keys[nxt].p, c.man[nxt].p, c.npairsOld, (const int*)nullptr, c.keys[c.cur].p, c.man[c.cur].p, c.shapes.p,
                        c.props.p, c.ptype.p);
   tEnd(c, T_REMAP);
   c.cur = nxt;
+}
+
+// ---- the broadphase of a graph-captured step --------------------------------------------------------
+// Every launch is shaped by the proxy count and the pair capacity; the huge-proxy, leaf and pair counts stay on
+// the device (Ctx::gc); the counters, the huge ranges and the narrowphase and solver flags are cleared by one
+// fill at the start of the step.  A pair set larger than the capacity, or a traversal stack overflow, raises the sticky
+// abort flag and leaves the step's pair count 0, so nothing after the broadphase changes the world; the host
+// then redoes that step on the ordinary path.
+namespace {
+__global__ void kGraphPairsFound(const unsigned long long* __restrict__ counter, int64_t cap, int* gc) {
+  const unsigned long long found = counter[0];
+  const int err = ((const int*)(counter + 2))[0];
+  if (gc[kGcAbort] || err || (int64_t)found > cap) {
+    gc[kGcAbort] = 1;
+    gc[kGcPairs] = 0;
+  } else {
+    gc[kGcPairs] = (int)found;
+  }
+}
+}  // namespace
+
+void bpGraphReserve(Ctx& c) {
+  ensureProxyBuffers(c);
+  const size_t cap = (size_t)std::max<int64_t>(c.pairCap, 1);
+  c.rawKeys.ensure(cap);
+  c.rawKeysAlt.ensure(cap);
+  // the pair sets of the last two steps (keys and manifolds) carry over: grown, not replaced
+  c.keys[0].ensureKeep(cap);
+  c.keys[1].ensureKeep(cap);
+  c.man[0].ensureKeep((size_t)JBGPU_MANIFOLD_WORDS * cap);
+  c.man[1].ensureKeep((size_t)JBGPU_MANIFOLD_WORDS * cap);
+  c.ptype.ensure(cap);
+  c.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
+  c.gc.ensure(kGcWords);
+  primReserve(c, (int)std::max<int64_t>(c.np, c.pairCap));
+}
+
+const uint64_t* bpStepGraph(Ctx& c) {
+  const int n = c.np;
+  const int bits = bitsFor((int64_t)n + 1);
+  hipStream_t s = c.stream;
+  int* gc = c.gc.p;
+  const bool wide = c.npairs >= kWidePairsPerProxy * (int64_t)n;
+  BpParams P = makeParams(c);
+  hipLaunchKernelGGL(kSetAabbBodies, dim3(gridFor(n)), dim3(256), 0, s, P, n, c.shapes.p, c.tr.p, c.props.p, c.act.p,
+                     c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)(gc + kGcAbort));
+  c.stageCurrent = P.newStage;
+  c.rotationPending = false;
+  unsigned int* bounds = (unsigned int*)c.bounds.p;
+  float* stats = c.bounds.p + 8;
+  unsigned int* part = (unsigned int*)c.bounds.p + 16;
+  const int nparts = std::min(gridFor(n), kStatBlocks);
+  hipLaunchKernelGGL(kProxyStats, dim3(nparts), dim3(256), 0, s, n, (const float*)c.cand.p, (const int*)c.isHuge.p,
+                     part);
+  hipLaunchKernelGGL(kProxyStatsFinal, dim3(1), dim3(256), 0, s, nparts, (const unsigned int*)part, bounds, stats);
+  hipLaunchKernelGGL(kCountLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, (const float*)c.cand.p,
+                     stats, (const int*)c.isHuge.p);
+  hipLaunchKernelGGL(kMarkLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, kLargeCap,
+                     (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
+  primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, gc + kGcHuge);
+  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
+                     (const unsigned int*)bounds, c.morton.p, c.order.p);
+  Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
+  Db<int> dv(c.order.p, c.orderAlt.p);
+  primSortPairs(c, dk, dv, n, 0, 32 + c.worldBits);
+  const uint64_t* code = dk.Current();
+  const int* order = dv.Current();
+  c.sortedOrder = order;
+  // huge proxies by world (and the tree's leaf counts)
+  hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)(gc + kGcHuge), gc + kGcLeaves,
+                     (const int*)c.hugeList.p, (const int*)c.world.p, c.hugeWorld.p);
+  Db<int> hk(c.hugeWorld.p, c.hugeWorldAlt.p);
+  Db<int> hv(c.hugeList.p, c.hugeListAlt.p);
+  if (c.worldBits > 0) primSortPairsDev(c, hk, hv, (const int*)(gc + kGcHuge), n, 0, c.worldBits);
+  hipLaunchKernelGGL(kHugeRanges, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)(gc + kGcHuge),
+                     (const int*)hk.Current(), c.hugeRange.p);
+  // the tree over the remaining leaves (chunk levels at the offsets of a full proxy set)
+  const int* nl = gc + kGcLeaves;
+  const int n1 = (n + 31) / 32, n2 = (n1 + 31) / 32;
+  float* c1 = c.chunkBox.p;
+  float* c2 = c1 + 8 * (size_t)n1;
+  float* c3 = c2 + 8 * (size_t)n2;
+  hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(n)), dim3(256), 0, s, n, nl, order, c.cand.p, c.world.p, c.leafBox.p,
+                     c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p, (int4*)c.leafExact.p);
+  hipLaunchKernelGGL(kBuild, dim3(gridFor(n)), dim3(256), 0, s, n, nl, code, c.nodeChild.p, c.nodeFirst.p,
+                     c.nodeLast.p);
+  hipLaunchKernelGGL(kChunk, dim3(gridFor(n)), dim3(256), 0, s, n, nl, c.leafBox.p, c1);
+  hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, (const int*)(gc + kGcChunk1), c1, c2);
+  hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, (const int*)(gc + kGcChunk2), c2, c3);
+  hipLaunchKernelGGL(kNodeBoxes, dim3(gridFor(n)), dim3(256), 0, s, n, nl, c.nodeFirst.p, c.nodeLast.p, c.leafBox.p,
+                     c1, c2, c3, c.nodeBox.p);
+  hipLaunchKernelGGL(kNodeRecords, dim3(gridFor(n)), dim3(256), 0, s, n, nl, (const int*)c.nodeChild.p,
+                     (const int*)c.nodeLast.p, (const float*)c.leafBox.p, (const float*)c.nodeBox.p,
+                     (float4*)c.nodeRec.p);
+  if (wide)
+    hipLaunchKernelGGL(kNodeRecords4, dim3(gridFor(n)), dim3(256), 0, s, n, nl, (const float4*)c.nodeRec.p,
+                       (float4*)c.nodeRec4.p);
+  // pairs: traversal, then every proxy against its world's huge proxies
+  PairTest T{c.bpMode, c.aabb.p, c.q.p, c.filt.p, c.world.p};
+  hipLaunchKernelGGL(wide ? kTraverse<4> : kTraverse<2>, dim3(gridFor(n, kTraverseBlock)), dim3(kTraverseBlock), 0, s,
+                     n, nl, order, c.leafBox.p, (const float4*)(wide ? c.nodeRec4.p : c.nodeRec.p),
+                     (const int4*)c.leafExact.p, c.bpMode, bits, c.counter.p, c.rawKeys.p, c.pairCap,
+                     (int*)(c.counter.p + 2), c.pairStage.p);
+  hipLaunchKernelGGL(kHuge, dim3(gridFor(n, kHugeBlock)), dim3(kHugeBlock), 0, s, n, (const int*)c.hugeRange.p,
+                     (const int*)hv.Current(), c.isHuge.p, c.cand.p, T, bits, c.counter.p, c.rawKeys.p, c.pairCap);
+  hipLaunchKernelGGL(kGraphPairsFound, dim3(1), dim3(1), 0, s, (const unsigned long long*)c.counter.p, c.pairCap, gc);
+  Db<uint64_t> kk(c.rawKeys.p, c.rawKeysAlt.p);
+  primSortKeysDev(c, kk, (const int*)(gc + kGcPairs), (int)c.pairCap, 0, 2 * bits);
+  return kk.Current();
+}
+
+void bpRemapGraph(Ctx& c, const uint64_t* sorted) {
+  const int bits = bitsFor((int64_t)c.np + 1);
+  const int nxt = 1 - c.cur;
+  int* gc = c.gc.p;
+  hipLaunchKernelGGL(kRemap, dim3(gridFor(c.pairCap)), dim3(256), 0, c.stream, c.pairCap, (const int*)(gc + kGcPairs),
+                     sorted, bits, c.keys[nxt].p, c.man[nxt].p, (int64_t)0, (const int*)(gc + kGcPairCount + c.cur),
+                     (const uint64_t*)c.keys[c.cur].p, (const float*)c.man[c.cur].p, (const float*)c.shapes.p,
+                     (const float*)c.props.p, c.ptype.p);
+  c.cur = nxt;  // the end-of-step kernel records the new set's count
 }
 
 }  // namespace jbg

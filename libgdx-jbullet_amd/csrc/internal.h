@@ -19,6 +19,10 @@ namespace jbg {
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// bumped by every device-buffer (re)allocation: a captured graph holds buffer addresses and is valid only while
+// this stays what it was at capture
+inline uint64_t gAllocEpoch = 0;
+
 template <class T>
 struct DBuf {  // device buffer owned by a context; freed with it
   T* p = nullptr;
@@ -37,6 +41,7 @@ struct DBuf {  // device buffer owned by a context; freed with it
     p = nullptr;
     JB_CHECK(hipMalloc((void**)&p, cap * sizeof(T)));
     n = cap;
+    gAllocEpoch++;
   }
   void ensureKeep(size_t k) {  // as ensure, keeping the first n elements
     if (k <= n) return;
@@ -50,6 +55,7 @@ struct DBuf {  // device buffer owned by a context; freed with it
     }
     p = q;
     n = cap;
+    gAllocEpoch++;
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -78,6 +84,37 @@ struct HBuf {
     n = k;
   }
 };
+
+// an instantiated graph, destroyed with its owner
+struct GraphExec {
+  hipGraphExec_t e = nullptr;
+  std::vector<int64_t> key;  // what the capture baked in (see stepGraph)
+  int stageAfter = 0, curAfter = 0;  // the host's Dbvt stage and pair buffer after one replay
+  GraphExec() = default;
+  GraphExec(const GraphExec&) = delete;
+  GraphExec& operator=(const GraphExec&) = delete;
+  ~GraphExec() { reset(); }
+  void reset() {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+    key.clear();
+  }
+};
+
+// device-side counts of a graph-captured step (int slots of Ctx::gc)
+constexpr int kGcHuge = 0;       // huge proxies
+constexpr int kGcLeaves = 1;     // tree leaves (proxies - huge), then the 32- and 1024-leaf chunk counts
+constexpr int kGcChunk1 = 2;
+constexpr int kGcChunk2 = 3;
+constexpr int kGcPairs = 4;      // pairs found this step (0 once aborted)
+constexpr int kGcAbort = 5;      // sticky: a step found more pairs than the capacity (or overflowed a stack)
+constexpr int kGcDone = 6;       // steps completed since the host cleared it
+constexpr int kGcActive = 7;     // active manifolds of the solve
+constexpr int kGcPairCount = 8;  // pairs held by keys[0] and keys[1]
+constexpr int kGcEpaOverflow = 10;  // sticky: an EPA job outgrew every workspace tier
+constexpr int kGcWbErr = 11;        // sticky: error bits of the world-block solves
+constexpr int kGcWords = 16;
+constexpr int kResidStripes = 64;  // solver residual accumulators (per-wave stripes of (dd, nn) sums)
 
 // EPA job workspace sizes (per queued pair)
 constexpr int kEpaFaces = 1024;
@@ -120,6 +157,7 @@ constexpr int kManifoldRowWords = 4 * kContactRowWords + 4;  // + [valid mask, b
 struct Stats {
   int64_t pairs = 0, manifoldsActive = 0, contacts = 0, rows = 0, colors = 0, epaCalls = 0, epaFails = 0;
   int64_t epaRetries = 0, epaHuge = 0;
+  int64_t graphRedos = 0;
   int64_t iterations = 0;
   float residual = 0;
   int64_t huge = 0;
@@ -210,6 +248,12 @@ struct Ctx {
   DBuf<int> ccdList;
   std::vector<float> ccdHost;
   int nccd = 0;
+  // graph-captured steps: the per-step counts on the device, their readback, one graph per (Dbvt stage, pair
+  // buffer) parity
+  DBuf<int> gc;
+  HBuf<int> gcHost;
+  GraphExec graphs[4];
+  bool stepGraph = false;  // jbgpu_world_step replays captured graphs
   DBuf<int> leafExact;   // np*8: exact-test box, group/mask and proxy index of each sorted leaf
   DBuf<int> nodeChild;   // np*2
   DBuf<int> nodeLast;    // np
@@ -404,7 +448,32 @@ void rayTest(Ctx& c, int n, const float* dFrom, const float* dTo, int16_t group,
 // in host memory); from/to are 12-float transforms
 void sweepTest(Ctx& c, int n, const float* castShape, const float* dFrom, const float* dTo, int16_t group,
                int16_t mask, const int* dWorld, int* dObj, float* dFrac, float* dNormal, float* dPoint);
-void ccdClamp(Ctx& c, float dt);  // integrateTransforms' motion clamping, after the integrate kernel
+void ccdClamp(Ctx& c, float dt, const int* abortFlag = nullptr, const int* dnp = nullptr);  // integrateTransforms' motion clamping
+// graph-captured step (world.hip drives it): capacity-shaped launches with counts on the device
+void graphFill(hipStream_t s, void* p, size_t words, uint32_t v);  // fill kernel (no memset node in a graph)
+struct FillList {  // several fills in one launch
+  static constexpr int kMax = 12;
+  uint32_t* p[kMax];
+  uint32_t n[kMax], v[kMax];
+  int count = 0;
+  void add(void* q, size_t words, uint32_t value) {
+    if (count >= kMax) throw std::logic_error("fill list full");
+    p[count] = (uint32_t*)q;
+    n[count] = (uint32_t)words;
+    v[count] = value;
+    count++;
+  }
+};
+void graphFillList(hipStream_t s, const FillList& f);
+void bpGraphReserve(Ctx& c);
+const uint64_t* bpStepGraph(Ctx& c);  // AABB update, tree, traversal, key sort; returns the sorted keys
+void bpRemapGraph(Ctx& c, const uint64_t* sorted);
+void npGraphReserve(Ctx& c);
+void npDispatchGraph(Ctx& c);
+void npStatsCopy(Ctx& c);  // the narrowphase statistics into pinned memory (after a graph launch)
+void solverGraphReserve(Ctx& c);
+void solverSolveGraph(Ctx& c, float dt);
+void solverStatsCopy(Ctx& c);
 void bpPairDeltas(Ctx& c, uint64_t* added, int64_t capAdded, int64_t* nAdded, uint64_t* removed, int64_t capRemoved,
                   int64_t* nRemoved);
 void npDispatchAll(Ctx& c);                              // A8-A16
@@ -465,5 +534,14 @@ void primSelectValues(Ctx& c, const T* vals, const int* flag, int n, T* out, int
 // out[i] = in[0] + ... + in[i - 1], i < n (the sums must fit in an int; primScanWide writes them as int64)
 void primScan(Ctx& c, const int* in, int* out, int n);
 void primScanWide(Ctx& c, const int* in, int64_t* out, int n);
+// the same with the element count in device memory (*count, at most cap): launches shaped by cap only, as a
+// captured graph needs; the sorted data ends in k.cur / v.cur (which buffer that is depends on cap and the bits)
+void primReserve(Ctx& c, int cap);  // scratch for counts up to cap, allocated outside a capture
+template <class K>
+void primSortPairsDev(Ctx& c, Db<K>& k, Db<int>& v, const int* count, int cap, int bitLo, int bitHi);
+template <class K>
+void primSortKeysDev(Ctx& c, Db<K>& k, const int* count, int cap, int bitLo, int bitHi);
+void primSelectIndexDev(Ctx& c, const int* flag, const int* count, int cap, int* out, int* selected);
+void primScanDev(Ctx& c, const int* in, int* out, const int* count, int cap);
 
 }  // namespace jbg

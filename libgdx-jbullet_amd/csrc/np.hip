@@ -58,9 +58,10 @@ __device__ __forceinline__ void putCand(float* out, int i, bool valid, V3 n, V3 
 // bin the lanes of a wave mostly run the same support functions.  Class: the other shape's type for a pair with
 // a plane, else lower convex type * 4 + higher (convex types are 0-3): 4 bits, one radix pass with 64 buckets
 constexpr int kBinClassBits = 4;
-__global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
+__global__ void kBinKeys(int64_t n, const int* __restrict__ dn, const int* __restrict__ ptype, const uint64_t* __restrict__ keys,
                          const int* __restrict__ act, const float* __restrict__ shapes,
                          const float* __restrict__ props, int* tkey, int* idx) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
@@ -74,7 +75,8 @@ __global__ void kBinKeys(int64_t n, const int* __restrict__ ptype, const uint64_
 }
 
 // first position of each pair type in the sorted keys (types absent from the step keep -1)
-__global__ void kBinStarts(int64_t n, const int* __restrict__ sorted, int* starts) {
+__global__ void kBinStarts(int64_t n, const int* __restrict__ dn, const int* __restrict__ sorted, int* starts) {
+  if (dn) n = *dn;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int t = sorted[i] >> kBinClassBits;
@@ -83,7 +85,8 @@ __global__ void kBinStarts(int64_t n, const int* __restrict__ sorted, int* start
 
 // ------------------------------------------------------------------ sphere-sphere
 // per pair type t: range[2t] first sorted position, range[2t + 1] count (types absent from the sort: 0, 0)
-__global__ void kBinRanges(int64_t n, const int* __restrict__ starts, int* range) {
+__global__ void kBinRanges(int64_t n, const int* __restrict__ dn, const int* __restrict__ starts, int* range) {
+  if (dn) n = *dn;
   const int t = threadIdx.x;
   if (t >= 4) return;
   int end = (int)n;
@@ -1234,9 +1237,10 @@ __device__ void removePoint(float* m, int idx) {  // PersistentManifold.java:259
   sti(m, n - 1);
 }
 
-__global__ void kManifold(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ ptype,
+__global__ void kManifold(int64_t npairs, const int* __restrict__ dn, const uint64_t* __restrict__ keys, const int* __restrict__ ptype,
                           const int* __restrict__ act, const float* __restrict__ props, const float* __restrict__ tr,
                           const float* __restrict__ cand, float* man) {
+  if (dn) npairs = *dn;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   int a = (int)(keys[i] >> 32) - 1, b = (int)(keys[i] & 0xffffffffu) - 1;
@@ -1958,9 +1962,12 @@ __global__ __launch_bounds__(kCcdBlock) void kCcd(int nlist, const int* __restri
                                                   const float* __restrict__ vel, const float* __restrict__ props,
                                                   const int* __restrict__ act, const float* __restrict__ shapes,
                                                   const float* __restrict__ verts, const int* __restrict__ filt,
-                                                  const int* __restrict__ wid, int64_t np,
-                                                  const uint64_t* __restrict__ pkeys, const float* __restrict__ man) {
+                                                  const int* __restrict__ wid, int64_t np, const int* __restrict__ dnp,
+                                                  const uint64_t* __restrict__ pkeys, const float* __restrict__ man,
+                                                  const int* __restrict__ abortFlag) {
   __shared__ float simplexLds[kSimplexRows * kCcdBlock];
+  if (abortFlag && *abortFlag) return;  // an aborted graph-captured step changes nothing
+  if (dnp) np = *dnp;
   __shared__ float red[kCcdBlock / 64];
   SimplexT<kCcdBlock> vs;
   vs.lw = simplexLds + threadIdx.x;
@@ -2051,18 +2058,18 @@ void npDispatchAll(Ctx& c) {
   float* man = c.man[c.cur].p;
   JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
   tBegin(c, T_BIN);
-  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, c.ptype.p, keys, c.act.p,
+  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)nullptr, c.ptype.p, keys, c.act.p,
                      (const float*)c.shapes.p, (const float*)c.props.p, c.typeKey.p, c.perm.p);
   Db<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
   const int keyBits = 2 + kBinClassBits;
   primSortPairs(c, kk, vv, (int)n, 0, keyBits);
   int* starts = (int*)(c.counter.p + 4);  // 4 ints
   JB_CHECK(hipMemsetAsync(starts, 0xff, 4 * sizeof(int), s));
-  hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)kk.Current(), starts);
+  hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)nullptr, (const int*)kk.Current(), starts);
   // per-type ranges stay on the device: each type's kernel covers every pair position and reads its range
   c.binRange.ensure(8);
   int* range = c.binRange.p;
-  hipLaunchKernelGGL(kBinRanges, dim3(1), dim3(64), 0, s, n, (const int*)starts, range);
+  hipLaunchKernelGGL(kBinRanges, dim3(1), dim3(64), 0, s, n, (const int*)nullptr, (const int*)starts, range);
   tEnd(c, T_BIN);
   const int* list = vv.Current();
   // EPA queue length, failures, workspace overflows, large-tier and last-tier queue lengths: read back with the
@@ -2094,10 +2101,69 @@ void npDispatchAll(Ctx& c) {
                      c.candC.p, c.npStat.p + 1, c.epaHugeWork.p, c.epaHugeAux.p);
   tEnd(c, T_EPA);
   tBegin(c, T_MANIFOLD);
-  hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
+  hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)nullptr, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
                      c.candC.p, man);
   tEnd(c, T_MANIFOLD);
   JB_CHECK(hipMemcpyAsync(c.npStatHost, c.npStat.p, sizeof(c.npStatHost), hipMemcpyDeviceToHost, s));
+  c.npStatPending = true;
+}
+
+// the narrowphase of a graph-captured step: launches shaped by the pair capacity, the pair count on the device
+void npGraphReserve(Ctx& c) {
+  const size_t n = (size_t)std::max<int64_t>(c.pairCap, 1);
+  c.typeKey.ensure(n);
+  c.typeKeyAlt.ensure(n);
+  c.perm.ensure(n);
+  c.permAlt.ensure(n);
+  c.candC.ensure(8 * n);
+  c.epaState.ensure(12 * n);
+  c.epaQueue.ensure(n);
+  c.epaRetry.ensure(n);
+  c.epaRetryHuge.ensure(n);
+  c.epaHugeWork.ensure((size_t)kEpaHugeGrid * kEpaHugeSlotWords);
+  c.epaHugeAux.ensure((size_t)kEpaHugeGrid * kEpaHugeAuxInts);
+  c.counter.ensure(8);
+  c.binRange.ensure(8);
+  c.npStat.ensure(5);
+}
+
+void npDispatchGraph(Ctx& c) {
+  hipStream_t s = c.stream;
+  const int64_t n = c.pairCap;
+  const int* dn = c.gc.p + kGcPairs;
+  const uint64_t* keys = c.keys[c.cur].p;
+  float* man = c.man[c.cur].p;
+  hipLaunchKernelGGL(kBinKeys, dim3(gridFor(n)), dim3(256), 0, s, n, dn, c.ptype.p, keys, c.act.p,
+                     (const float*)c.shapes.p, (const float*)c.props.p, c.typeKey.p, c.perm.p);
+  Db<int> kk(c.typeKey.p, c.typeKeyAlt.p), vv(c.perm.p, c.permAlt.p);
+  primSortPairsDev(c, kk, vv, dn, (int)n, 0, 2 + kBinClassBits);
+  int* starts = (int*)(c.counter.p + 4);  // -1 from the step's first fill
+  hipLaunchKernelGGL(kBinStarts, dim3(gridFor(n)), dim3(256), 0, s, n, dn, (const int*)kk.Current(), starts);
+  int* range = c.binRange.p;
+  hipLaunchKernelGGL(kBinRanges, dim3(1), dim3(64), 0, s, n, dn, (const int*)starts, range);
+  const int* list = vv.Current();
+  hipLaunchKernelGGL(kSphereSphere, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_SPHERE_SPHERE,
+                     list, keys, c.shapes.p, c.props.p, c.tr.p, c.candC.p);
+  hipLaunchKernelGGL(kConvexPlane, dim3(gridFor(n)), dim3(256), 0, s, (const int*)range + 2 * JBGPU_PAIR_CONVEX_PLANE,
+                     list, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.candC.p);
+  hipLaunchKernelGGL(kGjk, dim3(gridFor(n, kGjkBlock)), dim3(kGjkBlock), 0, s,
+                     (const int*)range + 2 * JBGPU_PAIR_CONVEX_CONVEX, list, keys, c.shapes.p, c.verts.p, c.props.p,
+                     c.tr.p, c.candC.p, c.epaState.p, c.epaQueue.p, c.npStat.p);
+  hipLaunchKernelGGL(kEpaSmall, dim3(kEpaSmallGrid), dim3(64), 0, s, (const unsigned long long*)c.npStat.p,
+                     (const int*)c.epaQueue.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p, c.candC.p,
+                     c.npStat.p + 1, c.epaRetry.p, c.npStat.p + 3, c.epaSmallFaces);
+  hipLaunchKernelGGL(kEpa, dim3(kEpaGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 3),
+                     (const int*)c.epaRetry.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
+                     c.candC.p, c.npStat.p + 1, c.epaRetryHuge.p, c.npStat.p + 4, c.epaLargeFaces);
+  hipLaunchKernelGGL(kEpaHuge, dim3(kEpaHugeGrid), dim3(64), 0, s, (const unsigned long long*)(c.npStat.p + 4),
+                     (const int*)c.epaRetryHuge.p, keys, c.shapes.p, c.verts.p, c.props.p, c.tr.p, c.epaState.p,
+                     c.candC.p, c.npStat.p + 1, c.epaHugeWork.p, c.epaHugeAux.p);
+  hipLaunchKernelGGL(kManifold, dim3(gridFor(n)), dim3(256), 0, s, n, dn, keys, c.ptype.p, c.act.p, c.props.p, c.tr.p,
+                     c.candC.p, man);
+}
+
+void npStatsCopy(Ctx& c) {
+  JB_CHECK(hipMemcpyAsync(c.npStatHost, c.npStat.p, sizeof(c.npStatHost), hipMemcpyDeviceToHost, c.stream));
   c.npStatPending = true;
 }
 
@@ -2220,13 +2286,13 @@ void sweepTest(Ctx& c, int n, const float* castShape, const float* dFrom, const 
                      (const int64_t*)t.offset.p, keys, (const float*)t.rec.p, dObj, dFrac, dNormal, dPoint);
 }
 
-void ccdClamp(Ctx& c, float dt) {
+void ccdClamp(Ctx& c, float dt, const int* abortFlag, const int* dnp) {
   if (c.nccd == 0) return;
   hipLaunchKernelGGL(kCcd, dim3(1), dim3(kCcdBlock), 0, c.stream, c.nccd, (const int*)c.ccdList.p,
                      (const float*)c.ccd.p, c.nb, dt, (const float*)c.trOld.p, c.tr.p, (const float*)c.vel.p,
                      (const float*)c.props.p, (const int*)c.act.p, (const float*)c.shapes.p, (const float*)c.verts.p,
-                     (const int*)c.filt.p, (const int*)c.world.p, c.npairs, (const uint64_t*)c.keys[c.cur].p,
-                     (const float*)c.man[c.cur].p);
+                     (const int*)c.filt.p, (const int*)c.world.p, c.npairs, dnp, (const uint64_t*)c.keys[c.cur].p,
+                     (const float*)c.man[c.cur].p, abortFlag);
 }
 
 }  // namespace jbg

@@ -364,6 +364,63 @@ void primScan(Ctx& c, const int* in, int* out, int n) {
   primExclusiveScan(c.stream, in, out, nullptr, n, n, primScratch(c, n), nullptr);
 }
 
+// ---- device-count forms (graph-captured steps) -----------------------------------------------------------------
+void primReserve(Ctx& c, int cap) { (void)primScratch(c, std::max(cap, 1)); }
+
+template <class K>
+void primSortPairsDev(Ctx& c, Db<K>& k, Db<int>& v, const int* count, int cap, int bitLo, int bitHi) {
+  if (bitHi <= bitLo) return;
+  cap = std::max(cap, 1);
+  using U = typename std::conditional<sizeof(K) == 8, uint64_t, uint32_t>::type;
+  const bool alt = primRadixSort<U>(c.stream, (U*)k.cur, v.cur, (U*)k.alt, v.alt, count, 0, cap, bitLo, bitHi,
+                                    primScratch(c, cap));
+  if (alt) {
+    std::swap(k.cur, k.alt);
+    std::swap(v.cur, v.alt);
+  }
+}
+template <class K>
+void primSortKeysDev(Ctx& c, Db<K>& k, const int* count, int cap, int bitLo, int bitHi) {
+  if (bitHi <= bitLo) return;
+  cap = std::max(cap, 1);
+  using U = typename std::conditional<sizeof(K) == 8, uint64_t, uint32_t>::type;
+  const bool alt = primRadixSort<U>(c.stream, (U*)k.cur, nullptr, (U*)k.alt, nullptr, count, 0, cap, bitLo, bitHi,
+                                    primScratch(c, cap));
+  if (alt) std::swap(k.cur, k.alt);
+}
+template void primSortPairsDev<uint64_t>(Ctx&, Db<uint64_t>&, Db<int>&, const int*, int, int, int);
+template void primSortPairsDev<int>(Ctx&, Db<int>&, Db<int>&, const int*, int, int, int);
+template void primSortKeysDev<uint64_t>(Ctx&, Db<uint64_t>&, const int*, int, int, int);
+
+void primSelectIndexDev(Ctx& c, const int* flag, const int* count, int cap, int* out, int* selected) {
+  cap = std::max(cap, 1);
+  primSelectFlagged(c.stream, flag, nullptr, out, count, 0, cap, primScratch(c, cap), selected);
+}
+void primScanDev(Ctx& c, const int* in, int* out, const int* count, int cap) {
+  cap = std::max(cap, 1);
+  primExclusiveScan(c.stream, in, out, count, 0, cap, primScratch(c, cap), nullptr);
+}
+
+__global__ void kFill32(size_t n, uint32_t* __restrict__ p, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void kFillList(FillList f) {
+  for (int e = 0; e < f.count; e++)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < f.n[e]; i += gridDim.x * blockDim.x) f.p[e][i] = f.v[e];
+}
+void graphFillList(hipStream_t s, const FillList& f) {
+  if (f.count == 0) return;
+  uint32_t most = 0;
+  for (int e = 0; e < f.count; e++) most = std::max(most, f.n[e]);
+  const int grid = (int)std::min<uint32_t>((most + 255) / 256, 256);
+  hipLaunchKernelGGL(kFillList, dim3(std::max(grid, 1)), dim3(256), 0, s, f);
+}
+void graphFill(hipStream_t s, void* p, size_t words, uint32_t v) {
+  if (words == 0) return;
+  const int grid = (int)std::min<size_t>((words + 255) / 256, 1024);
+  hipLaunchKernelGGL(kFill32, dim3(grid), dim3(256), 0, s, words, (uint32_t*)p, v);
+}
+
 __global__ void kWiden(int n, const int* __restrict__ in, int64_t* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[i];

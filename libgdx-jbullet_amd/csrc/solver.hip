@@ -107,9 +107,10 @@ __global__ void kIslandApply(int n, const int* __restrict__ parent, const float*
 
 // manifolds solved this step: response needed, not both bodies asleep, island without inactive bodies
 // (island == nullptr: deactivation disabled, every island is awake)
-__global__ void kMarkActive(int64_t npairs, const float* __restrict__ man, const int* __restrict__ ptype,
+__global__ void kMarkActive(int64_t npairs, const int* __restrict__ dn, const float* __restrict__ man, const int* __restrict__ ptype,
                             const int* __restrict__ act, const float* __restrict__ props,
                             const int* __restrict__ island, const int* __restrict__ inactive, int* flag) {
+  if (dn) npairs = *dn;  // graph-captured steps: the count on the device
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * i;
@@ -167,7 +168,8 @@ __global__ void kManifoldDegree(int n, const int2* __restrict__ ab, const float*
 }
 
 // body pair of active manifold k
-__global__ void kActiveBodies(int n, const int* __restrict__ list, const float* __restrict__ man, int2* ab) {
+__global__ void kActiveBodies(int n, const int* __restrict__ dn, const int* __restrict__ list, const float* __restrict__ man, int2* ab) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[k];
@@ -360,8 +362,9 @@ __device__ __forceinline__ void planeSpace1(V3 n, V3& p, V3& q) {  // TransformU
 // rows of every slot (solveGroupCacheFriendlySetup :494-747 without the impulse application);
 // the warm-start impulse of each row is left in its applied word
 // contact rows of each slot: points with distance <= 0 (the rows solveGroupCacheFriendlySetup creates)
-__global__ void kSlotRows(int na, const int* __restrict__ corder, const int* __restrict__ list,
+__global__ void kSlotRows(int na, const int* __restrict__ dn, const int* __restrict__ corder, const int* __restrict__ list,
                           const float* __restrict__ man, int* count) {
+  if (dn) na = *dn;  // graph-captured steps: the count on the device
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= na) return;
   const float* m = man + (size_t)JBGPU_MANIFOLD_WORDS * list[corder[p]];
@@ -383,10 +386,11 @@ __device__ __forceinline__ void storeRow(float* dst, const float* r) {
 // slot row counts): normal row r at nrow[r], its two friction rows at frow[2r], frow[2r+1]
 // (solveGroupCacheFriendlySetup :494-747 without the impulse application); the warm-start impulse of each
 // row is left in its applied word
-__global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __restrict__ list, SolverParams SP,
+__global__ void kBuildRows(int na, const int* __restrict__ dn, const int* __restrict__ corder, const int* __restrict__ list, SolverParams SP,
                            float* man, const float* __restrict__ tr, const float* __restrict__ vel,
                            const float* __restrict__ props, const int* __restrict__ rowStart, int4* hdr, float* nrow,
                            float* frow, const float* __restrict__ learned, float* features) {
+  if (dn) na = *dn;  // graph-captured steps: the count on the device
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= na) return;
   int pi = list[corder[p]];
@@ -482,8 +486,9 @@ __global__ void kBuildRows(int na, const int* __restrict__ corder, const int* __
 }
 
 // normal rows of the solve, read off the rows' prefix sum: out[0] all slots, out[1] slots below headEnd
-__global__ void kRowTotals(int na, int headEnd, const int* __restrict__ count, const int* __restrict__ start,
-                           unsigned long long* out) {
+__global__ void kRowTotals(int na, const int* __restrict__ dn, int headEnd, const int* __restrict__ count,
+                           const int* __restrict__ start, unsigned long long* out) {
+  if (dn) na = headEnd = *dn;  // graph-captured steps (world blocks: one batch)
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const unsigned long long all = na > 0 ? (unsigned long long)(start[na - 1] + count[na - 1]) : 0ull;
   out[0] = all;
@@ -535,7 +540,6 @@ __global__ __launch_bounds__(kWarmTailThreads) void kWarmApplyTail(int c0, int c
 }
 
 // residual terms of one wave into one of kResidStripes (dd, nn) accumulators (res == nullptr: not tracked)
-constexpr int kResidStripes = 64;
 __device__ __forceinline__ void accumulate(double dd, double nn, double* res) {
   if (!res) return;
   for (int off = 32; off > 0; off >>= 1) {
@@ -1072,8 +1076,9 @@ __device__ __forceinline__ void warmApplySlotIo(int p, const int4* __restrict__ 
 
 // blockOf[k] for active manifold k: the block of its dynamic body (both bodies of a manifold with two dynamic
 // bodies are in one world, hence in one block)
-__global__ void kManifoldBlocks(int na, const int2* __restrict__ ab, const int* __restrict__ bodyBlock, int* key,
+__global__ void kManifoldBlocks(int na, const int* __restrict__ dn, const int2* __restrict__ ab, const int* __restrict__ bodyBlock, int* key,
                                 int* idx, int* info) {
+  if (dn) na = *dn;  // graph-captured steps: the count on the device
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= na) return;
   const int2 b = ab[k];
@@ -1083,7 +1088,8 @@ __global__ void kManifoldBlocks(int na, const int2* __restrict__ ab, const int* 
   idx[k] = k;
 }
 // slotStart[b] = first slot of block b in the block-sorted slot order (slotStart[nBlocks] = na)
-__global__ void kBlockSlotStarts(int na, int nBlocks, const int* __restrict__ key, int* slotStart) {
+__global__ void kBlockSlotStarts(int na, const int* __restrict__ dn, int nBlocks, const int* __restrict__ key, int* slotStart) {
+  if (dn) na = *dn;  // graph-captured steps: the count on the device
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > na) return;
   const int lo = p == 0 ? -1 : key[p - 1];
@@ -1169,8 +1175,10 @@ __global__ __launch_bounds__(T) void kSolveWorldBlocks(int iters, SolverParams S
                                                        int* solveSlot, int* colorOrder, const int4* __restrict__ hdr,
                                                        int4* sinfo, float* nrow, float* frow,
                                                        const float* __restrict__ props, const float* __restrict__ svel,
-                                                       float* vel, double* res, int* info) {
+                                                       float* vel, double* res, int* info,
+                                                       const int* __restrict__ abortFlag) {
   extern __shared__ float lds[];  // [nbod][8] bodies, then [nbod][2] colour masks (64-bit words)
+  if (abortFlag && *abortFlag) return;  // an aborted graph-captured step changes nothing
   __shared__ int colourStart[kWbMaxColours + 1];
   __shared__ int count[kWbMaxColours];
   __shared__ int nColours;
@@ -1366,8 +1374,9 @@ __global__ __launch_bounds__(64) void kResidual(int it, float tol, const double*
   if (tol > 0.0f && r <= tol) *done = 1;
 }
 
-__global__ void kWriteback(int n, const int4* __restrict__ hdr, const float* __restrict__ nrow,
+__global__ void kWriteback(int n, const int* __restrict__ dn, const int4* __restrict__ hdr, const float* __restrict__ nrow,
                            const float* __restrict__ frow, float* man) {
+  if (dn) n = *dn;  // graph-captured steps: the count on the device
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   int4 h = hdr[p];
@@ -1907,10 +1916,16 @@ int bitsForBlocks(int n) {
   return b;
 }
 
-// the whole solve of batched worlds: manifolds grouped by block, rows built in that order, one launch per solve
-void solveWorldBlocks(Ctx& c, float dt, int na) {
+// the whole solve of batched worlds: manifolds grouped by block, rows built in that order, one launch per solve.
+// dna: the graph-captured form (na is then the capacity, *dna the active manifolds; no host-side statistics)
+void solveWorldBlocks(Ctx& c, float dt, int na, const int* dna = nullptr) {
   hipStream_t s = c.stream;
   float* man = c.man[c.cur].p;
+  const bool graph = dna != nullptr;
+  const int* abortFlag = graph ? c.gc.p + kGcAbort : nullptr;
+  auto clear = [&](void* p, size_t bytes) {  // a captured step clears these with its first fill
+    if (!graph) JB_CHECK(hipMemsetAsync(p, 0, bytes, s));
+  };
   if (!c.wbReady) worldBlocksPrepare(c);
   const int nBlocks = c.wbBlocks;
   tBegin(c, T_COLOR);
@@ -1921,16 +1936,19 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   c.colorOrderAlt.ensure(na);
   c.wbSlotStart.ensure((size_t)nBlocks + 1);
   c.wbInfo.ensure(4);
-  JB_CHECK(hipMemsetAsync(c.wbInfo.p, 0, 4 * sizeof(int), s));
-  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
-  hipLaunchKernelGGL(kManifoldBlocks, dim3(gridFor(na)), dim3(256), 0, s, na, (const int2*)c.activeBodies.p,
+  clear(c.wbInfo.p, 4 * sizeof(int));
+  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, dna, c.mActive.p, man, c.activeBodies.p);
+  hipLaunchKernelGGL(kManifoldBlocks, dim3(gridFor(na)), dim3(256), 0, s, na, dna, (const int2*)c.activeBodies.p,
                      (const int*)c.wbBodyBlock.p, c.wbKey.p, c.wbIdx.p, c.wbInfo.p);
   // slot order: active manifolds by block (stable: pair-key order inside a block)
   Db<int> dk(c.wbKey.p, c.wbKeyAlt.p), dv(c.wbIdx.p, c.colorOrderAlt.p);
   const int keyBits = bitsForBlocks(nBlocks);
-  primSortPairs(c, dk, dv, na, 0, keyBits);
+  if (graph)
+    primSortPairsDev(c, dk, dv, dna, na, 0, keyBits);
+  else
+    primSortPairs(c, dk, dv, na, 0, keyBits);
   const int* slotManifold = dv.Current();  // slot p -> active manifold
-  hipLaunchKernelGGL(kBlockSlotStarts, dim3(gridFor((int64_t)na + 1)), dim3(256), 0, s, na, nBlocks,
+  hipLaunchKernelGGL(kBlockSlotStarts, dim3(gridFor((int64_t)na + 1)), dim3(256), 0, s, na, dna, nBlocks,
                      (const int*)dk.Current(), c.wbSlotStart.p);
   tEnd(c, T_COLOR);
   // setup: rows of every slot (the warm start is applied inside the block solve)
@@ -1948,14 +1966,17 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   if ((int64_t)na * 4 >= (1ll << 27)) throw std::length_error("too many active manifolds for the slot header");
   c.slotCount.ensure(na);
   c.slotStart.ensure(na);
-  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, slotManifold, c.mActive.p, man,
+  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, dna, slotManifold, c.mActive.p, man,
                      c.slotCount.p);
-  primScan(c, c.slotCount.p, c.slotStart.p, na);
-  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, slotManifold, c.mActive.p, SP, man,
+  if (graph)
+    primScanDev(c, c.slotCount.p, c.slotStart.p, dna, na);
+  else
+    primScan(c, c.slotCount.p, c.slotStart.p, na);
+  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, dna, slotManifold, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
                      SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
   c.rowCount.ensure(2);
-  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na, na, (const int*)c.slotCount.p, (const int*)c.slotStart.p,
+  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na, dna, na, (const int*)c.slotCount.p, (const int*)c.slotStart.p,
                      c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);
   tEnd(c, T_SOLVE_SETUP);
@@ -1964,10 +1985,10 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   const size_t histWords = 2 * (size_t)iters + 4, stripeWords = 2 * (size_t)kResidStripes * iters;
   c.resid.ensure(histWords + stripeWords);
   c.done.ensure(4);
-  JB_CHECK(hipMemsetAsync(c.resid.p, 0, (histWords + stripeWords) * sizeof(double), s));
-  JB_CHECK(hipMemsetAsync(c.done.p, 0, 4 * sizeof(int), s));
+  clear(c.resid.p, (histWords + stripeWords) * sizeof(double));
+  clear(c.done.p, 4 * sizeof(int));
   float* resOut = (float*)(c.counter.p + 6);
-  JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
+  clear(resOut, 2 * sizeof(float));
   double* res = iters > 0 ? c.resid.p + histWords + 2 * (size_t)kResidStripes * (iters - 1) : nullptr;
   c.wbColour.ensure(na);
   c.wbSolveSlot.ensure(na);
@@ -1981,7 +2002,7 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
                      (const int*)c.wbBodyList.p, (const int*)c.wbBodyStart.p, (const int*)c.wbBodyLocal.p,        \
                      (const int*)c.wbSlotStart.p, slotManifold, c.wbColour.p, c.wbSolveSlot.p, c.colorOrder.p,     \
                      (const int4*)c.hdr.p, c.wbSlotInfo.p, c.nrow.p, c.frow.p, (const float*)c.props.p,            \
-                     (const float*)c.svel.p, c.vel.p, res, c.wbInfo.p)
+                     (const float*)c.svel.p, c.vel.p, res, c.wbInfo.p, abortFlag)
   if (c.wbPipelined) {
     switch (c.wbThreads) {
       case 64: JB_WB_LAUNCH(64, true); break;
@@ -2002,11 +2023,13 @@ void solveWorldBlocks(Ctx& c, float dt, int na) {
   if (iters > 0)
     hipLaunchKernelGGL(kResidual, dim3(1), dim3(64), 0, s, iters - 1, 0.0f, res, c.resid.p, c.done.p, resOut);
   tEnd(c, T_SOLVE_ITER, iters > 0 ? 2 : 1);
-  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
+  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, dna, c.hdr.p, c.nrow.p, c.frow.p, man);
   // the order is reported as one batch (block after block, colour after colour inside each block)
   c.colorOff.assign({0, (int64_t)na});
   c.ncolors = 1;
   c.headColors = 1;
+  c.wbLastSolve = true;
+  if (graph) return;  // statistics: solverStatsCopy after the graph's launch
   // statistics: read after the step's final wait
   c.solveRo.ensure(2);
   c.solveRh.ensure(2 * (size_t)std::max(iters, 1));
@@ -2105,7 +2128,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   }
   jp.mark(island, island ? (const int*)(c.islandFlag.p + c.nb) : nullptr);
   if (np > 0) {
-    hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, man, c.ptype.p, c.act.p, c.props.p,
+    hipLaunchKernelGGL(kMarkActive, dim3(gridFor(np)), dim3(256), 0, s, np, (const int*)nullptr, man, c.ptype.p, c.act.p, c.props.p,
                        island, island ? (const int*)(c.islandFlag.p + c.nb) : nullptr, c.flags.p);
     JB_CHECK(hipMemsetAsync(c.counter.p, 0, 8 * sizeof(unsigned long long), s));
     primSelectIndex(c, c.flags.p, (int)np, c.mActive.p, (int*)c.counter.p);
@@ -2139,7 +2162,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.bodyPrio.ensure(2 * (size_t)c.nb);
   c.bodyMask.ensure(c.nb);
   tBegin(c, T_COLOR);
-  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, c.mActive.p, man, c.activeBodies.p);
+  hipLaunchKernelGGL(kActiveBodies, dim3(gridFor(na)), dim3(256), 0, s, na, (const int*)nullptr, c.mActive.p, man, c.activeBodies.p);
   const int* deg = nullptr;  // largest-degree-first priorities (null: hashed priorities only)
   if (c.colourByDegree) {
     c.bodyDeg.ensure(c.nb);
@@ -2326,10 +2349,10 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if ((int64_t)na * 4 >= (1ll << 27)) throw std::length_error("too many active manifolds for the slot header");
   c.slotCount.ensure(na);
   c.slotStart.ensure(na);
-  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, c.colorOrder.p, c.mActive.p, man,
+  hipLaunchKernelGGL(kSlotRows, dim3(gridFor(na)), dim3(256), 0, s, na, (const int*)nullptr, c.colorOrder.p, c.mActive.p, man,
                      c.slotCount.p);
   primScan(c, c.slotCount.p, c.slotStart.p, na);
-  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, c.colorOrder.p, c.mActive.p, SP, man,
+  hipLaunchKernelGGL(kBuildRows, dim3(gridFor(na, 128)), dim3(128), 0, s, na, (const int*)nullptr, c.colorOrder.p, c.mActive.p, SP, man,
                      c.tr.p, c.vel.p, c.props.p, (const int*)c.slotStart.p, c.hdr.p, c.nrow.p, c.frow.p,
                      SP.useLearned ? c.learned.p : nullptr, SP.dumpFeatures ? c.features.p : nullptr);
   // per-colour launches: with blocks, the boundary colours [firstColor, ncolors); without, colours
@@ -2350,7 +2373,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.rowCount.ensure(2);
   // rows: all slots, and the slots swept by the timed per-colour launches' counterpart (block sweeps: the
   // interior slots; otherwise the head colours)
-  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na,
+  hipLaunchKernelGGL(kRowTotals, dim3(1), dim3(64), 0, s, na, (const int*)nullptr,
                      blockSweeps ? (int)c.interiorEnd : (int)c.colorOff[headColors], (const int*)c.slotCount.p,
                      (const int*)c.slotStart.p, c.rowCount.p);
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
@@ -2481,7 +2504,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
        (int64_t)launchedIters * 2 *
                (headColors - firstColor + (headColors < c.ncolors ? 1 : 0) + (blockSweeps ? 1 : 0)) +
            jointLaunches);
-  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, c.hdr.p, c.nrow.p, c.frow.p, man);
+  hipLaunchKernelGGL(kWriteback, dim3(gridFor(na)), dim3(256), 0, s, na, (const int*)nullptr, c.hdr.p, c.nrow.p, c.frow.p, man);
   hipLaunchKernelGGL(kCopyVel, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.svel.p, c.vel.p, c.props.p);
   // statistics: copied into pinned memory now, read by solverCollectStats after the step's final wait
   c.solveRo.ensure(2);
@@ -2536,6 +2559,71 @@ void solverCollectStats(Ctx& c) {
     double nrm = std::sqrt(rh[2 * k + 1]);
     c.residHist.push_back((float)(std::sqrt(rh[2 * k]) / (nrm > 1.0 ? nrm : 1.0)));
   }
+}
+
+// ---- the solve of a graph-captured step: batched worlds, one block solve, counts on the device ------------
+void solverGraphReserve(Ctx& c) {
+  if (!c.wbReady) worldBlocksPrepare(c);
+  const size_t cap = (size_t)std::max<int64_t>(c.pairCap, 1);
+  const int iters = c.numIterations;
+  c.flags.ensure(cap);
+  c.mActive.ensure(cap);
+  c.counter.ensure(8);
+  c.activeBodies.ensure(cap);
+  c.wbKey.ensure(cap);
+  c.wbKeyAlt.ensure(cap);
+  c.wbIdx.ensure(cap);
+  c.colorOrderAlt.ensure(cap);
+  c.wbSlotStart.ensure((size_t)c.wbBlocks + 1);
+  c.wbInfo.ensure(4);
+  c.hdr.ensure(cap);
+  c.nrow.ensure((size_t)NW * cap);
+  c.frow.ensure((size_t)FW * cap);
+  c.svel.ensure((size_t)8 * c.nb);
+  c.slotCount.ensure(cap);
+  c.slotStart.ensure(cap);
+  c.rowCount.ensure(2);
+  c.resid.ensure(2 * (size_t)iters + 4 + 2 * (size_t)kResidStripes * iters);
+  c.done.ensure(4);
+  c.wbColour.ensure(cap);
+  c.wbSolveSlot.ensure(cap);
+  c.wbSlotInfo.ensure(cap);
+  c.colorOrder.ensure(cap);
+  c.solveRo.ensure(2);
+  c.solveRh.ensure(2 * (size_t)std::max(iters, 1));
+  c.solveRows.ensure(2);
+  c.wbInfoHost.ensure(4);
+  primReserve(c, (int)cap);
+  if ((int64_t)cap * 4 >= (1ll << 27)) throw std::length_error("pair capacity too large for the slot header");
+}
+
+void solverSolveGraph(Ctx& c, float dt) {
+  hipStream_t s = c.stream;
+  const int cap = (int)c.pairCap;
+  int* gc = c.gc.p;
+  float* man = c.man[c.cur].p;
+  c.nActive = 0;
+  hipLaunchKernelGGL(kMarkActive, dim3(gridFor(cap)), dim3(256), 0, s, (int64_t)cap, (const int*)(gc + kGcPairs), man,
+                     c.ptype.p, c.act.p, c.props.p, (const int*)nullptr, (const int*)nullptr, c.flags.p);
+  primSelectIndexDev(c, c.flags.p, gc + kGcPairs, cap, c.mActive.p, gc + kGcActive);
+  solveWorldBlocks(c, dt, cap, gc + kGcActive);
+}
+
+void solverStatsCopy(Ctx& c) {
+  hipStream_t s = c.stream;
+  const int iters = c.numIterations;
+  JB_CHECK(hipMemcpyAsync(c.solveRo.p, (float*)(c.counter.p + 6), 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (iters > 0)
+    JB_CHECK(hipMemcpyAsync(c.solveRh.p, c.resid.p, 2 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(c.solveRows.p, c.rowCount.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  JB_CHECK(hipMemcpyAsync(c.wbInfoHost.p, c.wbInfo.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+  c.solveStat.pending = true;
+  c.solveStat.tolMode = false;
+  c.solveStat.blockSweeps = false;
+  c.solveStat.sampleK = false;
+  c.solveStat.iters = iters;
+  c.solveStat.kUsed = 0;
+  c.wbLastSolve = true;
 }
 
 }  // namespace jbg

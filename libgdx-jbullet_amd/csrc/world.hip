@@ -27,9 +27,10 @@ std::string g_createError;
 constexpr float MAX_ANGVEL = kHalfPi;
 
 __global__ void kPredict(int n, float dt, const float* __restrict__ tr, const float* __restrict__ props,
-                         const float* __restrict__ grav, const int* __restrict__ act, float* vel) {
+                         const float* __restrict__ grav, const int* __restrict__ act, float* vel,
+                         const int* __restrict__ abortFlag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (abortFlag && *abortFlag)) return;  // an aborted graph-captured step changes nothing
   const float* pr = props + 8 * i;
   if ((ldi(pr + 7) & JBGPU_BODY_STATIC) != 0) return;
   int a = act[i];
@@ -65,9 +66,9 @@ __global__ void kPredict(int n, float dt, const float* __restrict__ tr, const fl
 }
 
 __global__ void kIntegrate(int n, float dt, float* tr, const float* __restrict__ props, const int* __restrict__ act,
-                           const float* __restrict__ vel) {
+                           const float* __restrict__ vel, const int* __restrict__ abortFlag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (abortFlag && *abortFlag)) return;
   const float* pr = props + 8 * i;
   if ((ldi(pr + 7) & JBGPU_BODY_STATIC) != 0) return;
   int a = act[i];
@@ -90,9 +91,9 @@ __global__ __launch_bounds__(256) void kCopyProbe(int64_t n4, const float4* __re
 }
 
 __global__ void kActivation(int n, float dt, int deactDisabled, const float* __restrict__ props, int* act,
-                            float* deact, float* vel) {
+                            float* deact, float* vel, const int* __restrict__ abortFlag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (abortFlag && *abortFlag)) return;
   int a = act[i];
   float t = deact[i];
   float* v = vel + 8 * i;
@@ -375,7 +376,7 @@ void stepPhases(Ctx& c, float dt, int last) {
   tBegin(c, T_STEP);
   tBegin(c, T_PREDICT);
   hipLaunchKernelGGL(kPredict, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.grav.p, c.act.p,
-                     c.vel.p);
+                     c.vel.p, (const int*)nullptr);
   tEnd(c, T_PREDICT);
   if (last >= 1) {
     bpUpdateFromBodies(c);
@@ -391,11 +392,11 @@ void stepPhases(Ctx& c, float dt, int last) {
       JB_CHECK(hipMemcpyAsync(c.trOld.p, c.tr.p, 12 * (size_t)c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
     hipLaunchKernelGGL(kIntegrate, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.act.p,
-                       c.vel.p);
+                       c.vel.p, (const int*)nullptr);
     ccdClamp(c, dt);
     tEnd(c, T_INTEGRATE);
     hipLaunchKernelGGL(kActivation, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.deactivationDisabled,
-                       c.props.p, c.act.p, c.deact.p, c.vel.p);
+                       c.props.p, c.act.p, c.deact.p, c.vel.p, (const int*)nullptr);
   }
   tEnd(c, T_STEP);
   JB_CHECK(hipStreamSynchronize(s));
@@ -406,6 +407,220 @@ void stepPhases(Ctx& c, float dt, int last) {
     c.timedSteps++;
   }
   if (c.epaOverflow) throw std::length_error("EPA workspace overflow in this step (the step has been applied)");
+}
+
+// ---- graph-captured steps ---------------------------------------------------------------------------------
+// A step whose every launch is shaped by capacities (proxies, pair capacity) with the per-step counts on the
+// device, captured once per (Dbvt stage, pair buffer) parity and replayed: no host wait inside a step and none
+// between the steps of one call.  The broadphase runs before the velocity prediction (it reads transforms and
+// activation only, the prediction writes velocities only).  A step that finds more pairs than the capacity
+// raises a sticky abort flag after its AABB update and changes nothing else; every later replay of the call is
+// a no-op, and the host finishes that step on the ordinary path (which raises the capacity) and captures again.
+// end of a captured step: the new pair set's count, the step counted, sticky EPA overflow and block-solve errors
+__global__ void kGraphStepEnd(int* gc, int nxt, const unsigned long long* __restrict__ npStat,
+                              const int* __restrict__ wbInfo) {
+  if (!gc[kGcAbort]) {
+    gc[kGcPairCount + nxt] = gc[kGcPairs];
+    gc[kGcDone]++;
+  }
+  if (npStat[2]) gc[kGcEpaOverflow] = 1;
+  if (wbInfo[0]) gc[kGcWbErr] |= wbInfo[0];
+}
+
+std::string graphUnsupported(const Ctx& c) {
+  if (c.wbBodies <= 0) return "graph-captured steps need the world-block solve (jbgpu_solver_set_world_blocks)";
+  if (c.nj > 0) return "graph-captured steps take no constraints";
+  if (!c.deactivationDisabled) return "graph-captured steps need deactivation disabled";
+  if (c.tolerance > 0.0f) return "graph-captured steps run a fixed iteration count (no tolerance mode)";
+  if (c.useMlp || c.useLearned || c.dumpFeatures) return "graph-captured steps take no learned warm start";
+  if (c.timing) return "graph-captured steps are not timed per phase";
+  return "";
+}
+
+int64_t fbits64(float f) { return (int64_t)__builtin_bit_cast(uint32_t, f); }
+
+// everything a capture bakes into its launches
+std::vector<int64_t> graphKey(const Ctx& c, float dt) {
+  return {(int64_t)gAllocEpoch, c.stageCurrent, c.cur, c.np, c.nb, c.pairCap, c.nccd, c.numIterations,
+          fbits64(c.erp), fbits64(c.warmFactor), fbits64(c.linearSlop), c.warmstart, c.wbBodies, c.wbThreads,
+          c.wbPipelined ? 1 : 0, c.wbBlocks, c.wbMaxBodies, c.npairs >= 4 * (int64_t)c.np ? 1 : 0, c.bpMode, c.worldBits, c.maxWorld,
+          c.epaSmallFaces, c.epaLargeFaces, c.deactivationDisabled, fbits64(dt)};
+}
+
+void stepGraphBody(Ctx& c, float dt) {
+  hipStream_t s = c.stream;
+  const int* abortFlag = c.gc.p + kGcAbort;
+  {  // every counter and flag region the step accumulates into, cleared in one launch
+    const int iters = c.numIterations;
+    FillList f;
+    f.add(c.counter.p, 8, 0u);             // pair counter, traversal stack flag
+    f.add(c.counter.p + 4, 4, 0xffffffffu);  // first sorted position per pair type
+    f.add(c.counter.p + 6, 2, 0u);          // residual readout
+    f.add(c.hugeRange.p, 2 * ((size_t)c.maxWorld + 1), 0u);
+    f.add(c.npStat.p, 10, 0u);
+    f.add(c.wbInfo.p, 4, 0u);
+    f.add(c.resid.p, 2 * (2 * (size_t)iters + 4 + 2 * (size_t)kResidStripes * iters), 0u);
+    f.add(c.done.p, 4, 0u);
+    graphFillList(s, f);
+  }
+  const uint64_t* sorted = bpStepGraph(c);
+  hipLaunchKernelGGL(kPredict, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.grav.p, c.act.p,
+                     c.vel.p, abortFlag);
+  bpRemapGraph(c, sorted);
+  npDispatchGraph(c);
+  solverSolveGraph(c, dt);
+  if (c.nccd > 0)
+    JB_CHECK(hipMemcpyAsync(c.trOld.p, c.tr.p, 12 * (size_t)c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(kIntegrate, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.act.p, c.vel.p,
+                     abortFlag);
+  ccdClamp(c, dt, abortFlag, c.gc.p + kGcPairs);
+  hipLaunchKernelGGL(kActivation, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.deactivationDisabled, c.props.p,
+                     c.act.p, c.deact.p, c.vel.p, abortFlag);
+  hipLaunchKernelGGL(kGraphStepEnd, dim3(1), dim3(1), 0, s, c.gc.p, c.cur, (const unsigned long long*)c.npStat.p,
+                     (const int*)c.wbInfo.p);
+}
+
+// the rest of a step whose AABBs are set (a graph-captured step that gave up in its pair search), ordinary path
+void stepAfterAabbs(Ctx& c, float dt) {
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(kPredict, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.grav.p, c.act.p,
+                     c.vel.p, (const int*)nullptr);
+  c.rotationPending = true;
+  bpFindPairs(c);
+  bpRemapManifolds(c);
+  npDispatchAll(c);
+  solverSolve(c, dt);
+  if (c.nccd > 0) {
+    c.trOld.ensure(12 * (size_t)c.nb);
+    JB_CHECK(hipMemcpyAsync(c.trOld.p, c.tr.p, 12 * (size_t)c.nb * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(kIntegrate, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.tr.p, c.props.p, c.act.p, c.vel.p,
+                     (const int*)nullptr);
+  ccdClamp(c, dt);
+  hipLaunchKernelGGL(kActivation, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, dt, c.deactivationDisabled, c.props.p,
+                     c.act.p, c.deact.p, c.vel.p, (const int*)nullptr);
+  JB_CHECK(hipStreamSynchronize(s));
+  npCollectStats(c);
+  solverCollectStats(c);
+}
+
+void stepGraph(Ctx& c, float dt, int nsteps) {
+  const std::string why = graphUnsupported(c);
+  if (!why.empty()) throw std::invalid_argument(why);
+  hipStream_t s = c.stream;
+  bool epaOverflow = false;
+  int done = 0;
+  while (done < nsteps) {
+    if (c.pairCap == 0) {  // no pair search yet: the first step sets the pair count the capacity starts from
+      stepPhases(c, dt, 4);
+      done++;
+      continue;
+    }
+    // the pair capacity launches are shaped by: 1.5x the last pair count + 4096, rounded up to a sixteenth of a
+    // power of two; kept while it stays within [want, 1.25 want], so small changes do not capture again
+    const int64_t want = c.npairs + c.npairs / 2 + 4096;
+    if (c.pairCap < want || c.pairCap > want + want / 4) {
+      int64_t g = 1;
+      while (g * 16 < want) g *= 2;
+      c.pairCap = (want + g - 1) / g * g;
+    }
+    if (c.pairCap >= (1ll << 31) / 8) throw std::length_error("pair capacity too large for a graph-captured step");
+    c.gc.ensure(kGcWords);
+    c.gcHost.ensure(kGcWords);
+    bpGraphReserve(c);
+    npGraphReserve(c);
+    solverGraphReserve(c);
+    if (c.nccd > 0) c.trOld.ensure(12 * (size_t)c.nb);
+    int* h = c.gcHost.p;
+    std::fill(h, h + kGcWords, 0);
+    h[kGcPairCount + c.cur] = (int)c.npairs;
+    h[kGcPairCount + 1 - c.cur] = (int)c.npairsOld;
+    JB_CHECK(hipMemcpyAsync(c.gc.p, h, kGcWords * sizeof(int), hipMemcpyHostToDevice, s));
+    JB_CHECK(hipStreamSynchronize(s));  // h is read back into below
+    // host state before each replay (stage, pair buffer), and after the last
+    std::vector<std::pair<int, int>> state;
+    const int todo = nsteps - done;
+    for (int k = 0; k < todo; k++) {
+      state.emplace_back(c.stageCurrent, c.cur);
+      GraphExec& g = c.graphs[2 * (c.stageCurrent & 1) + c.cur];
+      const std::vector<int64_t> key = graphKey(c, dt);
+      if (!g.e || g.key != key) {
+        g.reset();
+        const int st = c.stageCurrent, cu = c.cur;
+        const bool rp = c.rotationPending;
+        const uint64_t epoch = gAllocEpoch;
+        hipGraph_t graph = nullptr;
+        JB_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          stepGraphBody(c, dt);
+        } catch (...) {
+          hipGraph_t junk = nullptr;
+          (void)hipStreamEndCapture(s, &junk);
+          if (junk) (void)hipGraphDestroy(junk);
+          c.stageCurrent = st;
+          c.cur = cu;
+          c.rotationPending = rp;
+          throw;
+        }
+        JB_CHECK(hipStreamEndCapture(s, &graph));
+        g.stageAfter = c.stageCurrent;
+        g.curAfter = c.cur;
+        c.stageCurrent = st;
+        c.cur = cu;
+        c.rotationPending = rp;
+        if (gAllocEpoch != epoch) {
+          (void)hipGraphDestroy(graph);
+          throw std::logic_error("a device buffer grew while a step was being captured");
+        }
+        const hipError_t e = hipGraphInstantiate(&g.e, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          g.e = nullptr;
+          throw std::runtime_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        g.key = key;
+      }
+      JB_CHECK(hipGraphLaunch(g.e, s));
+      c.stageCurrent = g.stageAfter;
+      c.cur = g.curAfter;
+      c.rotationPending = false;
+    }
+    state.emplace_back(c.stageCurrent, c.cur);
+    npStatsCopy(c);
+    solverStatsCopy(c);
+    JB_CHECK(hipMemcpyAsync(h, c.gc.p, kGcWords * sizeof(int), hipMemcpyDeviceToHost, s));
+    JB_CHECK(hipStreamSynchronize(s));
+    const int ok = h[kGcDone];
+    epaOverflow = epaOverflow || h[kGcEpaOverflow] != 0;
+    if (h[kGcWbErr] & 2) throw std::length_error("a block needed more than 128 colours");
+    if (h[kGcWbErr] & 4) throw std::logic_error("a manifold joins two world blocks (bodies of different worlds)");
+    if (h[kGcAbort]) {
+      // replay `ok` set its AABBs and gave up in its pair search; the replays after it did nothing
+      c.stageCurrent = state[ok + 1].first;
+      c.cur = state[ok].second;
+      c.npairs = h[kGcPairCount + c.cur];
+      c.npairsOld = h[kGcPairCount + 1 - c.cur];
+      c.npStatPending = false;
+      c.solveStat.pending = false;
+      c.st.graphRedos++;
+      stepAfterAabbs(c, dt);
+      epaOverflow = epaOverflow || c.epaOverflow;
+      done += ok + 1;
+      continue;
+    }
+    if (ok != todo) throw std::logic_error("a graph-captured step neither completed nor aborted");
+    done += ok;
+    c.npairs = h[kGcPairCount + c.cur];
+    c.npairsOld = h[kGcPairCount + 1 - c.cur];
+    c.st.pairs = c.npairs;
+    c.nHuge = h[kGcHuge];
+    c.st.huge = c.nHuge;
+    c.nActive = h[kGcActive];
+    c.colorOff.assign({0, (int64_t)c.nActive});
+    npCollectStats(c);
+    solverCollectStats(c);
+  }
+  if (epaOverflow) throw std::length_error("EPA workspace overflow in these steps (the steps have been applied)");
 }
 
 }  // namespace
@@ -794,6 +1009,7 @@ int jbgpu_world_create(jbgpu_ctx* ctx, int32_t nshapes, const float* shapes, int
     c.jointHost.clear();
     c.nccd = 0;  // and no body with continuous collision detection
     c.ccdHost.assign(2 * (size_t)nbodies, 0.0f);
+    for (GraphExec& g : c.graphs) g.reset();  // captured steps of an earlier world
     upload(c.shapes, shapes, (size_t)JBGPU_SHAPE_WORDS * nshapes, c.stream);
     std::vector<float> vz(3, 0.0f);
     upload(c.verts, nverts > 0 ? verts : vz.data(), nverts > 0 ? 3 * (size_t)nverts : 3, c.stream);
@@ -835,7 +1051,25 @@ int jbgpu_world_step(jbgpu_ctx* ctx, float dt, int32_t nsteps) {
     Ctx& c = ctx->c;
     requireWorld(c);
     if (!(dt > 0.0f) || nsteps < 0) throw std::invalid_argument("bad step arguments");
+    if (c.stepGraph) {
+      stepGraph(c, dt, nsteps);
+      return;
+    }
     for (int i = 0; i < nsteps; i++) stepPhases(c, dt, 4);
+  });
+}
+
+int jbgpu_world_set_step_graph(jbgpu_ctx* ctx, int32_t enable) {
+  return guard(ctx, [&] {
+    Ctx& c = ctx->c;
+    if (enable) {
+      requireWorld(c);
+      const std::string why = graphUnsupported(c);
+      if (!why.empty()) throw std::invalid_argument(why);
+    }
+    c.stepGraph = enable != 0;
+    if (!enable)
+      for (GraphExec& g : c.graphs) g.reset();
   });
 }
 
@@ -1272,6 +1506,7 @@ int jbgpu_world_stats(jbgpu_ctx* ctx, jbgpu_stats* out) {
     out->rows = s.rows;
     out->epa_retries = s.epaRetries;
     out->epa_huge = s.epaHuge;
+    out->graph_redos = s.graphRedos;
   });
 }
 

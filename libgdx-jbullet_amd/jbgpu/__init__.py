@@ -42,7 +42,7 @@ class Stats(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("contacts", C.c_int64), ("manifolds", C.c_int64), ("colors", C.c_int64),
                 ("epa_calls", C.c_int64), ("epa_fails", C.c_int64), ("iterations", C.c_int64),
                 ("residual", C.c_float), ("huge_proxies", C.c_int64), ("rows", C.c_int64),
-                ("epa_retries", C.c_int64), ("epa_huge", C.c_int64)]
+                ("epa_retries", C.c_int64), ("epa_huge", C.c_int64), ("graph_redos", C.c_int64)]
 
 
 SYMBOLS = {
@@ -71,6 +71,7 @@ SYMBOLS = {
     "jbgpu_bp_get_quantized": (C.c_int, [_vp, _i32]),
     "jbgpu_world_create": (C.c_int, [_vp, C.c_int32, _f, C.c_int32, _f, C.c_int32, _f, _vp]),
     "jbgpu_world_step": (C.c_int, [_vp, C.c_float, C.c_int32]),
+    "jbgpu_world_set_step_graph": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_world_step_until": (C.c_int, [_vp, C.c_float, C.c_int32]),
     "jbgpu_world_get_bodies": (C.c_int, [_vp, _f]),
     "jbgpu_world_set_bodies": (C.c_int, [_vp, _f]),
@@ -327,6 +328,10 @@ class Context:
 
     def step(self, dt=1.0 / 60.0, n=1):
         self._ck(load().jbgpu_world_step(self.h, dt, int(n)), "world_step")
+
+    def set_step_graph(self, enable=True):
+        """Run jbgpu_world_step as replayed hipGraphs (batched worlds with the world-block solve)."""
+        self._ck(load().jbgpu_world_set_step_graph(self.h, 1 if enable else 0), "world_set_step_graph")
 
     def ray_test(self, frm, to, group=1, mask=-1, world=None):
         """rayTest with a closest-hit callback per ray: (body or -1, fraction, normal, point)."""
