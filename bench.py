@@ -447,6 +447,17 @@ def batched(args):
         ctx.measure_copy(TRACE_MARK_BYTES, 1)
     same = bool(np.array_equal(ctx.bodies().view(np.uint32), ordinary_end.view(np.uint32)))
     st = ctx.stats()
+    # the rest of the configuration's run (its step count), graph replays
+    rest_out = None
+    rest = int(sc.meta.get("steps", 0)) - (first + n - 1)
+    if graph and rest > 0:
+        t0 = time.perf_counter()
+        ctx.step(n=rest)
+        rest_secs = time.perf_counter() - t0
+        rest_out = {"timed_window": f"steps {first + n}-{first + n + rest - 1}",
+                    "ms_per_step": round(1e3 * rest_secs / rest, 3),
+                    "body_steps_per_s": round(nb * rest / rest_secs, 1), "last_step_pairs": ctx.stats()["pairs"],
+                    "graph_redos": ctx.stats()["graph_redos"]}
     ctx.close()
     out = {"worlds": args.batched_worlds, "dynamic_bodies": nb, "steps": n,
            "launch_form": "hipGraph replay" if graph else "ordinary launches (profiled window)",
@@ -458,6 +469,7 @@ def batched(args):
                                  "end_state_bit_identical": same},
            "last_step_pairs": st["pairs"], "last_step_colors": st["colors"], "last_step_manifolds": st["manifolds"],
            "last_step_epa_calls": st["epa_calls"], "graph_redos": st["graph_redos"], "autotune": tune,
+           "rest_of_run": rest_out,
            "phase_ms_per_step": {k: round(v[0] / max(nsteps, 1), 3) for k, v in regions.items() if v[1] > 0},
            "phase_note": f"ordinary launches with event timing between the phases, the same window "
                          f"({st_phases['pairs']} pairs at the last)"}

@@ -8,8 +8,8 @@ colour's rows ahead (more registers, fewer resident waves), against the per-colo
 """
 import time
 
-WORLD_BLOCK_CANDIDATES = ((0, 256, 0), (128, 128, 0), (256, 256, 0), (512, 256, 0), (1024, 512, 0),
-                          (128, 128, 1), (256, 256, 1), (512, 256, 1), (1024, 512, 1))
+WORLD_BLOCK_CANDIDATES = ((0, 256, 0), (128, 128, 0), (256, 128, 0), (256, 256, 0), (512, 128, 0), (512, 256, 0),
+                          (1024, 256, 0), (1024, 512, 0), (128, 128, 1), (256, 256, 1), (512, 256, 1), (1024, 512, 1))
 
 
 def world_blocks(ctx, dt=1.0 / 60.0, steps=4, candidates=WORLD_BLOCK_CANDIDATES, log=None):

@@ -150,13 +150,14 @@ def _compare_world(ctx, orc, tag):
 
 @pytest.mark.parametrize("blocks,tail", [(2048, 0), (256, 0), (0, 1024), (0, 0)])
 def test_world_boxes_grid_step_parity(blocks, tail):
-    """Launch shapes of the sweeps: blocks = bodies per block of the block-partitioned sweep (0: none);
-    tail: trailing colours of at most this many manifolds swept in one workgroup (0: one launch each)."""
+    """The 1000-box configuration for its 300 steps under each launch shape of the sweeps: blocks = bodies per
+    block of the block-partitioned sweep (0: none); tail: trailing colours of at most this many manifolds swept in
+    one workgroup (0: one launch each)."""
     sc = scenes.boxes_grid(seed=1)
     ctx, orc = _world_pair(sc)
     ctx.solver_set_blocks(blocks)
     ctx.solver_set_tail(tail)
-    for step in range(300 if blocks == 2048 else 90):  # the 1000-box configuration runs 300 steps
+    for step in range(300):  # the 1000-box configuration's 300 steps, under every sweep shape
         _sync_and_step(ctx, orc)
         st = ctx.stats()
         if step % 15 == 0:
