@@ -265,6 +265,7 @@ struct Ctx {
   DBuf<uint64_t> pairStage;  // per traversal block: staged pair keys
   DBuf<unsigned long long> counter;  // misc device counters
   DBuf<int> primTmp;  // scratch of the device-wide scans, compactions and sorts (prim.hip)
+  HBuf<int> primErrHost;  // the sorts' error word, read back after a step
   DBuf<int> primWide;
   int64_t pairCap = 0;
   // pairs and manifolds (double buffered for the remap)
@@ -537,6 +538,8 @@ void primScanWide(Ctx& c, const int* in, int64_t* out, int n);
 // the same with the element count in device memory (*count, at most cap): launches shaped by cap only, as a
 // captured graph needs; the sorted data ends in k.cur / v.cur (which buffer that is depends on cap and the bits)
 void primReserve(Ctx& c, int cap);  // scratch for counts up to cap, allocated outside a capture
+void primErrCopy(Ctx& c);   // the sorts' error word into pinned memory (asynchronous)
+void primErrCheck(Ctx& c);  // after the stream's wait: throws if a sort's look-back gave up
 template <class K>
 void primSortPairsDev(Ctx& c, Db<K>& k, Db<int>& v, const int* count, int cap, int bitLo, int bitHi);
 template <class K>

@@ -4,11 +4,14 @@
 // has a shape that depends only on the capacity: no host wait for the count, no memset, and a sequence of
 // these launches can be captured once in a hipGraph and replayed for any count up to the capacity.
 //
-// Radix sort: 8-bit digits, least significant first, three launches per digit: per-tile digit histograms
-// (LDS atomics), one exclusive scan over the digit-major histogram table, and a stable scatter in which every
-// tile ranks its keys chunk by chunk (256 keys per chunk, in thread order): lanes of a wave holding the same
-// digit are found with eight ballots, the waves' counts are combined through LDS, and a per-digit running
-// offset carries the order from chunk to chunk.  Equal digits keep their input order, so the sort is stable.
+// Radix sort: 8-bit digits, least significant first.  One launch counts the digits of every pass at once (and
+// clears the look-back state), one scans those counts; then one launch per digit: each workgroup takes the next
+// tile by an atomic ticket, publishes its per-digit counts, adds up the earlier tiles' counts by decoupled
+// look-back (a tile's inclusive prefix ends the walk), and scatters its keys stably, chunk by chunk (256 keys per
+// chunk, in thread order): lanes of a wave holding the same digit are found with eight ballots, the waves' counts
+// are combined through LDS, and a per-digit running offset carries the order from chunk to chunk.  Equal digits
+// keep their input order, so the sort is stable.  A tile only ever waits for tiles with smaller tickets, which
+// were taken by workgroups already running; the wait is bounded all the same and sets an error word.
 #include <type_traits>
 
 #include "internal.h"
@@ -22,6 +25,18 @@ constexpr int kPerThread = 4;           // items per thread and tile
 constexpr int kTile = kPT * kPerThread; // items per tile (1024: a 100k-key sort still spreads over ~100 blocks)
 constexpr int kRadix = 256;
 constexpr int kScanTop = 1024;          // threads of the single-block scan of the tile sums
+// sort header at the start of the scratch (fixed offsets; kept zero between sorts where noted)
+constexpr int kMaxPasses = 8;
+constexpr int kHdrHist = 0;                            // digit counts per pass (zero between sorts)
+constexpr int kHdrOfs = kMaxPasses * kRadix;           // their exclusive scan
+constexpr int kHdrTicket = 2 * kMaxPasses * kRadix;    // tile tickets per pass
+constexpr int kHdrErr = kHdrTicket + 16;               // sticky: a look-back wait gave up
+constexpr int kSortHdr = kHdrErr + 16;
+constexpr uint32_t kStAgg = 1u << 30, kStIncl = 2u << 30, kStMask = (1u << 30) - 1u;  // look-back status word
+// keys per thread of a digit pass: 8 (2048-key tiles: half the tiles and look-back steps) for large sorts, 4 for
+// small ones (more workgroups)
+inline int sweepPer(int cap) { return cap >= (1 << 18) ? 8 : 4; }
+inline int sweepTiles(int cap) { return (std::max(cap, 1) + kPT * sweepPer(cap) - 1) / (kPT * sweepPer(cap)); }
 
 __device__ __forceinline__ int countOf(const int* count, int hostCount, int cap) {
   const int n = count ? *count : hostCount;
@@ -165,44 +180,114 @@ __device__ __forceinline__ int digitOf(K k, int shift, int mask) {
   return (int)((k >> shift) & (K)mask);
 }
 
-// hist[d * nTiles + tile] = keys of the tile with digit d
+// digit counts of every pass (shift bitLo + 8p) over all keys into hdr[kHdrHist]; clears the look-back status
 template <class K>
-__global__ __launch_bounds__(kPT) void kRadixHist(const K* __restrict__ keys, const int* __restrict__ count,
-                                                  int hostCount, int cap, int shift, int mask, int* __restrict__ hist) {
-  __shared__ int h[kRadix];
+__global__ __launch_bounds__(kPT) void kDigitHist(const K* __restrict__ keys, const int* __restrict__ count,
+                                                  int hostCount, int cap, int bitLo, int bitHi, int* hdr,
+                                                  uint32_t* __restrict__ status, size_t statusWords) {
+  __shared__ int h[kMaxPasses][kRadix];
   const int n = countOf(count, hostCount, cap);
-  const int base = blockIdx.x * kTile;
-  h[threadIdx.x] = 0;
+  const int t = threadIdx.x;
+  const int passes = (bitHi - bitLo + 7) / 8;
+  for (int p = 0; p < passes; p++) h[p][t] = 0;
   __syncthreads();
-  for (int i = base + (int)threadIdx.x; i < base + kTile && i < n; i += kPT)
-    atomicAdd(&h[digitOf(keys[i], shift, mask)], 1);
+  for (int i = blockIdx.x * kPT + t; i < n; i += gridDim.x * kPT) {
+    const K k = keys[i];
+    for (int p = 0; p < passes; p++) {
+      const int shift = bitLo + 8 * p;
+      atomicAdd(&h[p][digitOf(k, shift, (1 << min(8, bitHi - shift)) - 1)], 1);
+    }
+  }
   __syncthreads();
-  hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+  for (int p = 0; p < passes; p++)
+    if (h[p][t]) atomicAdd(&hdr[kHdrHist + p * kRadix + t], h[p][t]);
+  for (size_t w = (size_t)blockIdx.x * kPT + t; w < statusWords; w += (size_t)gridDim.x * kPT) status[w] = 0u;
 }
 
-// stable scatter of one tile: chunks of kPT keys in thread order
-template <class K, bool Vals>
-__global__ __launch_bounds__(kPT) void kRadixScatter(const K* __restrict__ kin, const int* __restrict__ vin,
-                                                     K* __restrict__ kout, int* __restrict__ vout,
-                                                     const int* __restrict__ count, int hostCount, int cap, int shift,
-                                                     int mask, const int* __restrict__ offs) {
+// exclusive scan of every pass's digit counts into hdr[kHdrOfs]; counts and tickets left zero
+__global__ __launch_bounds__(kPT) void kDigitScan(int passes, int* hdr) {
+  __shared__ int ws[kPT / 64];
+  __shared__ int tot;
+  const int t = threadIdx.x;
+  for (int p = 0; p < passes; p++) {
+    const int v = hdr[kHdrHist + p * kRadix + t];
+    hdr[kHdrOfs + p * kRadix + t] = blockExclusive<kPT>(v, ws, &tot);
+    hdr[kHdrHist + p * kRadix + t] = 0;
+    __syncthreads();
+  }
+  if (t < kMaxPasses) hdr[kHdrTicket + t] = 0;
+}
+
+// one digit pass: the tile of this workgroup's ticket, its keys in registers, look-back, stable scatter
+template <class K, bool Vals, int Per>
+__global__ __launch_bounds__(kPT) void kOneSweep(const K* __restrict__ kin, const int* __restrict__ vin,
+                                                 K* __restrict__ kout, int* __restrict__ vout,
+                                                 const int* __restrict__ count, int hostCount, int cap, int shift,
+                                                 int mask, int pass, int* hdr, uint32_t* status) {
+  __shared__ int sTile;
+  __shared__ int tileCount[kRadix];
   __shared__ int running[kRadix];
   __shared__ int cnt[kPT / 64][kRadix];
   const int n = countOf(count, hostCount, cap);
-  const int base = blockIdx.x * kTile;
-  if (base >= n) return;
+  constexpr int kTileKeys = kPT * Per;
+  const int used = (n + kTileKeys - 1) / kTileKeys;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  running[t] = offs[t * gridDim.x + blockIdx.x];
+  if (t == 0) sTile = atomicAdd(&hdr[kHdrTicket + pass], 1);
+  tileCount[t] = 0;
 #pragma unroll
   for (int q = 0; q < kPT / 64; q++) cnt[q][t] = 0;
   __syncthreads();
+  const int tile = sTile;
+  if (tile >= used) return;  // the whole workgroup: no tile left
+  const int base = tile * kTileKeys;
+  K key[Per];
+  int val[Per], dig[Per];
+#pragma unroll
+  for (int q = 0; q < Per; q++) {
+    const int i = base + q * kPT + t;
+    const bool ok = i < n;
+    key[q] = ok ? kin[i] : (K)0;
+    val[q] = (Vals && ok) ? vin[i] : 0;
+    dig[q] = digitOf(key[q], shift, mask);
+    if (ok) atomicAdd(&tileCount[dig[q]], 1);
+  }
+  __syncthreads();
+  // status[pass][tile][digit]: the tile's count at once (its inclusive prefix for tile 0), then the inclusive
+  // prefix once the earlier tiles' counts are summed
+  uint32_t* st = status + (size_t)pass * gridDim.x * kRadix;
+  const int mine = tileCount[t];
+  __hip_atomic_store(st + (size_t)tile * kRadix + t, (tile == 0 ? kStIncl : kStAgg) | (uint32_t)mine,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int prefix = 0;
+  if (tile > 0) {
+    int j = tile - 1;
+    unsigned spins = 0;
+    while (j >= 0) {
+      const uint32_t sv = __hip_atomic_load(st + (size_t)j * kRadix + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t f = sv & ~kStMask;
+      if (f == 0u) {
+        if (++spins > (1u << 24)) {
+          atomicOr(&hdr[kHdrErr], 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += (int)(sv & kStMask);
+      if (f == kStIncl) break;
+      j--;
+    }
+    __hip_atomic_store(st + (size_t)tile * kRadix + t, kStIncl | (uint32_t)(prefix + mine), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  running[t] = hdr[kHdrOfs + pass * kRadix + t] + prefix;
+  __syncthreads();
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (int c0 = base; c0 < base + kTile && c0 < n; c0 += kPT) {
-    const int i = c0 + t;
-    const bool valid = i < n;
-    K k = valid ? kin[i] : (K)0;
-    const int v = (Vals && valid) ? vin[i] : 0;
-    const int d = digitOf(k, shift, mask);
+#pragma unroll
+  for (int q = 0; q < Per; q++) {
+    if (base + q * kPT >= n) break;  // the same for the whole workgroup
+    const bool valid = base + q * kPT + t < n;
+    const int d = dig[q];
     unsigned long long m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -214,16 +299,16 @@ __global__ __launch_bounds__(kPT) void kRadixScatter(const K* __restrict__ kin, 
     __syncthreads();
     if (valid) {
       int pos = running[d] + rank;
-      for (int q = 0; q < w; q++) pos += cnt[q][d];
-      kout[pos] = k;
-      if (Vals) vout[pos] = v;
+      for (int qq = 0; qq < w; qq++) pos += cnt[qq][d];
+      kout[pos] = key[q];
+      if (Vals) vout[pos] = val[q];
     }
     __syncthreads();
     int add = 0;
 #pragma unroll
-    for (int q = 0; q < kPT / 64; q++) {
-      add += cnt[q][t];
-      cnt[q][t] = 0;
+    for (int qq = 0; qq < kPT / 64; qq++) {
+      add += cnt[qq][t];
+      cnt[qq][t] = 0;
     }
     running[t] += add;
     __syncthreads();
@@ -234,11 +319,11 @@ __global__ __launch_bounds__(kPT) void kRadixScatter(const K* __restrict__ kin, 
 
 int primTiles(int cap) { return (std::max(cap, 1) + kTile - 1) / kTile; }
 
+// scratch: the sort header, the scans' tile sums (tiles + 64), the sort's look-back status (passes x tiles x 256)
 size_t primScratchInts(int cap, int keyBytes) {
   (void)keyBytes;
   const int tiles = primTiles(cap);
-  const size_t table = (size_t)kRadix * tiles;  // the radix sort's digit table, then its scan's tile sums
-  return table + (size_t)primTiles((int)std::min<size_t>(table, 1u << 30)) + (size_t)tiles + 64;
+  return (size_t)kSortHdr + (size_t)tiles + 64 + (size_t)kMaxPasses * sweepTiles(cap) * kRadix;
 }
 
 void primExclusiveScan(hipStream_t s, const int* in, int* out, const int* count, int hostCount, int cap, int* scratch,
@@ -265,37 +350,38 @@ void primSelectFlagged(hipStream_t s, const int* flag, const int* vals, int* out
 template <class K>
 bool primRadixSort(hipStream_t s, K* k0, int* v0, K* k1, int* v1, const int* count, int hostCount, int cap, int bitLo,
                    int bitHi, int* scratch) {
-  const int tiles = primTiles(cap);
-  int* hist = scratch;
-  int* top = scratch + (size_t)kRadix * tiles;  // tile sums of the table's scan
+  if (bitHi <= bitLo) return false;
+  if (cap >= (1 << 30)) throw std::length_error("radix sort of 2^30 keys or more");
+  const int tiles = sweepTiles(cap);
+  const int passes = (bitHi - bitLo + 7) / 8;
+  if (passes > kMaxPasses) throw std::logic_error("radix sort of more than 64 bits");
+  int* hdr = scratch;
+  uint32_t* status = (uint32_t*)(scratch + kSortHdr + primTiles(cap) + 64);
+  const size_t statusWords = (size_t)passes * tiles * kRadix;
+  hipLaunchKernelGGL(kDigitHist<K>, dim3(std::min(primTiles(cap), 512)), dim3(kPT), 0, s, (const K*)k0, count, hostCount,
+                     cap, bitLo, bitHi, hdr, status, statusWords);
+  hipLaunchKernelGGL(kDigitScan, dim3(1), dim3(kPT), 0, s, passes, hdr);
   bool alt = false;
-  for (int shift = bitLo; shift < bitHi; shift += 8) {
+  for (int p = 0; p < passes; p++) {
+    const int shift = bitLo + 8 * p;
+    const int mask = (1 << std::min(8, bitHi - shift)) - 1;  // bits at or above bitHi take no part
     K* ki = alt ? k1 : k0;
     K* ko = alt ? k0 : k1;
     int* vi = alt ? v1 : v0;
     int* vo = alt ? v0 : v1;
-    const int mask = (1 << std::min(8, bitHi - shift)) - 1;  // bits at or above bitHi take no part
-    hipLaunchKernelGGL(kRadixHist<K>, dim3(tiles), dim3(kPT), 0, s, (const K*)ki, count, hostCount, cap, shift, mask,
-                       hist);
-    // digit-major table (digit, tile): one exclusive scan gives every tile's first position per digit (a small
-    // table in one single-block launch, a large one as tile sums, their scan and the tiles)
-    const int entries = kRadix * tiles;
-    if (entries <= 4 * kScanTop) {
-      hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, entries, hist, (int*)nullptr);
-    } else {
-      const int scanTiles = primTiles(entries);
-      hipLaunchKernelGGL(kTileSums, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
-                         entries, top);
-      hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, s, scanTiles, top, (int*)nullptr);
-      hipLaunchKernelGGL(kScanTiles, dim3(scanTiles), dim3(kPT), 0, s, (const int*)hist, (const int*)nullptr, entries,
-                         entries, (const int*)top, hist);
-    }
-    if (v0)
-      hipLaunchKernelGGL((kRadixScatter<K, true>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
-                         count, hostCount, cap, shift, mask, (const int*)hist);
+    const bool wide = sweepPer(cap) == 8;
+    if (v0 && wide)
+      hipLaunchKernelGGL((kOneSweep<K, true, 8>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
+                         count, hostCount, cap, shift, mask, p, hdr, status);
+    else if (v0)
+      hipLaunchKernelGGL((kOneSweep<K, true, 4>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
+                         count, hostCount, cap, shift, mask, p, hdr, status);
+    else if (wide)
+      hipLaunchKernelGGL((kOneSweep<K, false, 8>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)nullptr, ko,
+                         (int*)nullptr, count, hostCount, cap, shift, mask, p, hdr, status);
     else
-      hipLaunchKernelGGL((kRadixScatter<K, false>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)nullptr,
-                         ko, (int*)nullptr, count, hostCount, cap, shift, mask, (const int*)hist);
+      hipLaunchKernelGGL((kOneSweep<K, false, 4>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)nullptr, ko,
+                         (int*)nullptr, count, hostCount, cap, shift, mask, p, hdr, status);
     alt = !alt;
   }
   return alt;
@@ -307,10 +393,15 @@ template bool primRadixSort<uint64_t>(hipStream_t, uint64_t*, int*, uint64_t*, i
                                       int*);
 
 // ---- the context-level forms used by the step: host-known counts, scratch in the context ----------------------
+// the sort header starts the scratch (its counts must start zero: cleared whenever the buffer is new); the scans
+// and compactions use the part behind it
 static int* primScratch(Ctx& c, int cap) {
+  const size_t before = c.primTmp.n;  // a new buffer always has a new capacity (its address may repeat)
   c.primTmp.ensure(primScratchInts(cap) + 64);
+  if (c.primTmp.n != before) JB_CHECK(hipMemsetAsync(c.primTmp.p, 0, kSortHdr * sizeof(int), c.stream));
   return c.primTmp.p;
 }
+static int* primScanScratch(Ctx& c, int cap) { return primScratch(c, cap) + kSortHdr; }
 
 template <class K>
 void primSortPairs(Ctx& c, Db<K>& k, Db<int>& v, int n, int bitLo, int bitHi) {
@@ -341,7 +432,7 @@ void primSelectIndex(Ctx& c, const int* flag, int n, int* out, int* count) {
     JB_CHECK(hipMemsetAsync(count, 0, sizeof(int), c.stream));
     return;
   }
-  primSelectFlagged(c.stream, flag, nullptr, out, nullptr, n, n, primScratch(c, n), count);
+  primSelectFlagged(c.stream, flag, nullptr, out, nullptr, n, n, primScanScratch(c, n), count);
 }
 template <class T>
 void primSelectValues(Ctx& c, const T* vals, const int* flag, int n, T* out, int* count) {
@@ -350,7 +441,7 @@ void primSelectValues(Ctx& c, const T* vals, const int* flag, int n, T* out, int
     return;
   }
   const int tiles = primTiles(n);
-  int* scratch = primScratch(c, n);
+  int* scratch = primScanScratch(c, n);
   hipLaunchKernelGGL(kFlagTileSums, dim3(tiles), dim3(kPT), 0, c.stream, flag, (const int*)nullptr, n, n, scratch);
   hipLaunchKernelGGL(kScanTileSums, dim3(1), dim3(kScanTop), 0, c.stream, tiles, scratch, count);
   hipLaunchKernelGGL((kSelectTiles<T, true>), dim3(tiles), dim3(kPT), 0, c.stream, flag, vals, (const int*)nullptr, n,
@@ -361,11 +452,22 @@ template void primSelectValues<int>(Ctx&, const int*, const int*, int, int*, int
 
 void primScan(Ctx& c, const int* in, int* out, int n) {
   if (n <= 0) return;
-  primExclusiveScan(c.stream, in, out, nullptr, n, n, primScratch(c, n), nullptr);
+  primExclusiveScan(c.stream, in, out, nullptr, n, n, primScanScratch(c, n), nullptr);
 }
 
 // ---- device-count forms (graph-captured steps) -----------------------------------------------------------------
 void primReserve(Ctx& c, int cap) { (void)primScratch(c, std::max(cap, 1)); }
+
+void primErrCopy(Ctx& c) {
+  c.primErrHost.ensure(1);
+  c.primErrHost.p[0] = 0;
+  if (c.primTmp.p)
+    JB_CHECK(hipMemcpyAsync(c.primErrHost.p, c.primTmp.p + kHdrErr, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+void primErrCheck(Ctx& c) {
+  if (c.primErrHost.p && c.primErrHost.p[0])
+    throw std::runtime_error("a radix sort's look-back wait gave up (the step's results are not valid)");
+}
 
 template <class K>
 void primSortPairsDev(Ctx& c, Db<K>& k, Db<int>& v, const int* count, int cap, int bitLo, int bitHi) {
@@ -394,11 +496,11 @@ template void primSortKeysDev<uint64_t>(Ctx&, Db<uint64_t>&, const int*, int, in
 
 void primSelectIndexDev(Ctx& c, const int* flag, const int* count, int cap, int* out, int* selected) {
   cap = std::max(cap, 1);
-  primSelectFlagged(c.stream, flag, nullptr, out, count, 0, cap, primScratch(c, cap), selected);
+  primSelectFlagged(c.stream, flag, nullptr, out, count, 0, cap, primScanScratch(c, cap), selected);
 }
 void primScanDev(Ctx& c, const int* in, int* out, const int* count, int cap) {
   cap = std::max(cap, 1);
-  primExclusiveScan(c.stream, in, out, count, 0, cap, primScratch(c, cap), nullptr);
+  primExclusiveScan(c.stream, in, out, count, 0, cap, primScanScratch(c, cap), nullptr);
 }
 
 __global__ void kFill32(size_t n, uint32_t* __restrict__ p, uint32_t v) {
@@ -428,7 +530,7 @@ __global__ void kWiden(int n, const int* __restrict__ in, int64_t* __restrict__ 
 void primScanWide(Ctx& c, const int* in, int64_t* out, int n) {
   if (n <= 0) return;
   c.primWide.ensure(n);
-  primExclusiveScan(c.stream, in, c.primWide.p, nullptr, n, n, primScratch(c, n), nullptr);
+  primExclusiveScan(c.stream, in, c.primWide.p, nullptr, n, n, primScanScratch(c, n), nullptr);
   hipLaunchKernelGGL(kWiden, dim3(gridFor(n)), dim3(256), 0, c.stream, n, (const int*)c.primWide.p, out);
 }
 

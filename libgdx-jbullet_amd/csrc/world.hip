@@ -399,7 +399,9 @@ void stepPhases(Ctx& c, float dt, int last) {
                        c.props.p, c.act.p, c.deact.p, c.vel.p, (const int*)nullptr);
   }
   tEnd(c, T_STEP);
+  primErrCopy(c);
   JB_CHECK(hipStreamSynchronize(s));
+  primErrCheck(c);
   npCollectStats(c);
   solverCollectStats(c);
   if (c.timing) {
@@ -588,8 +590,10 @@ void stepGraph(Ctx& c, float dt, int nsteps) {
     state.emplace_back(c.stageCurrent, c.cur);
     npStatsCopy(c);
     solverStatsCopy(c);
+    primErrCopy(c);
     JB_CHECK(hipMemcpyAsync(h, c.gc.p, kGcWords * sizeof(int), hipMemcpyDeviceToHost, s));
     JB_CHECK(hipStreamSynchronize(s));
+    primErrCheck(c);
     const int ok = h[kGcDone];
     epaOverflow = epaOverflow || h[kGcEpaOverflow] != 0;
     if (h[kGcWbErr] & 2) throw std::length_error("a block needed more than 128 colours");
@@ -867,7 +871,9 @@ int jbgpu_bp_calculate_overlapping_pairs(jbgpu_ctx* ctx, int64_t* npairs) {
     bpFindPairs(c);
     bpRemapManifolds(c);
     tEnd(c, T_STEP);
+    primErrCopy(c);
     JB_CHECK(hipStreamSynchronize(c.stream));
+    primErrCheck(c);
     if (c.timing) {
       tCollect(c);
       c.timedSteps++;
