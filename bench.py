@@ -536,15 +536,16 @@ def broadphase_1m(args):
         for k in range(3):  # untimed: first frames build the stored state
             ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
             ctx.bp_calculate_overlapping_pairs()
-        pairs = 0
+        per_frame = []
         mark = args.trace_window and name == "dbvt"  # window markers for --summarize-trace (DBVT frames)
         if mark:
             ctx.measure_copy(TRACE_MARK_BYTES, 1)
         t0 = time.perf_counter()
         for k in range(3, args.bp_frames):
             ctx.bp_set_aabbs_device(dev + k * frame_bytes, n)
-            pairs += ctx.bp_calculate_overlapping_pairs()
+            per_frame.append(ctx.bp_calculate_overlapping_pairs())
         secs = time.perf_counter() - t0
+        pairs = sum(per_frame)
         if mark:
             ctx.measure_copy(TRACE_MARK_BYTES, 1)
         steps = args.bp_frames - 3
@@ -553,7 +554,40 @@ def broadphase_1m(args):
         out[name] = {"ms_per_step": round(1e3 * secs / steps, 3), "pairs_per_step": pairs // steps,
                      "proxies_per_s": round(n * steps / secs, 1), "pairs_per_s": round(pairs / secs, 1)}
         log(f"[bench] broadphase 1M {name}: {out[name]}")
+        if name == "dbvt":
+            dbvt_counts = per_frame
+    if not args.no_cpu and not args.trace_window:
+        out["cpu_baseline"] = broadphase_cpu_baseline(first, frames, dbvt_counts, args)
     return out
+
+
+def broadphase_cpu_baseline(first, frames, gpu_counts, args):
+    """The oracle's DBVT pair search (1 core) on the same frames after the same 3 untimed ones, as many timed
+    frames as fit in about --cpu-seconds; its pair counts are compared with the device's for those frames."""
+    import jbgpu
+    import pyoracle
+    orc = pyoracle.Broadphase(jbgpu.BP_DBVT, np.array((-100.0,) * 3, np.float32), np.array((1100.0,) * 3, np.float32))
+    orc.create(first)
+    cap = 4 * max(gpu_counts) + 4096
+    for k in range(3):
+        orc.set_all(frames[k])
+        orc.step(cap=cap)
+    secs, counts = 0.0, []
+    for k in range(3, len(frames)):
+        if secs >= args.cpu_seconds:
+            break
+        t0 = time.perf_counter()
+        orc.set_all(frames[k])
+        counts.append(len(orc.step(cap=cap)))
+        secs += time.perf_counter() - t0
+    n = len(first)
+    log(f"[bench] broadphase cpu baseline: {len(counts)} frames in {secs:.1f}s")
+    return {"value": round(n * len(counts) / secs, 1), "unit": "proxies/s", "cores": 1, "kind": "port",
+            "sample": f"frames 4-{3 + len(counts)} of the same 1M-AABB sequence (DBVT semantics) after the same 3 "
+                      f"untimed frames: setAabb of every proxy + the pair search; single-threaded C++ restatement "
+                      f"(not the Java reference: no JDK on the box)",
+            "ms_per_frame": round(1e3 * secs / max(len(counts), 1), 1),
+            "pair_counts_agree": counts == list(gpu_counts[:len(counts)])}
 
 
 def learned_warm_start(args):

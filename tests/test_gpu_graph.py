@@ -199,3 +199,22 @@ def test_graph_4096_worlds_match_ordinary_steps():
         print(f"[graph 4096] call of {n}: pairs {b.stats()['pairs']}", flush=True)
     a.close()
     b.close()
+
+
+def test_graph_steps_without_pairs():
+    """A world with no pair at all (spheres 3 m apart, no floor, falling together): every count of the captured
+    step is zero (no tree leaves overlap, no manifolds, no rows); bit-identical to the ordinary steps."""
+    bl = scenes._Builder("apart")
+    sph = bl.shape(SPHERE, [0.5])
+    for w in range(4):
+        for i in range(25):
+            bl.body(sph, scenes.IDENT, np.array([3.0 * (i % 5), 3.0 * (i // 5), 0.0], np.float32), 1.0, world=w)
+    sc = bl.done()
+    a, b, _ = _contexts(sc)
+    for n in (1, 3, 10):
+        a.step(n=n)
+        b.step(n=n)
+        _same(a, b, f"no pairs, after a call of {n}")
+        assert b.stats()["pairs"] == 0
+    a.close()
+    b.close()
