@@ -33,6 +33,8 @@ def _live_manifolds(m):
     """Manifold records with the words no reader uses cleared: points past the count, friction directions of
     points whose directions were never set."""
     m = m.copy()
+    if len(m) == 0:
+        return m
     pts = m[:, 4:].reshape(len(m), 4, 32)
     npts = m[:, 0].view(np.int32)
     for j in range(4):
