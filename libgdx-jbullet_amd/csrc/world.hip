@@ -623,6 +623,15 @@ void stepGraph(Ctx& c, float dt, int nsteps) {
     c.colorOff.assign({0, (int64_t)c.nActive});
     npCollectStats(c);
     solverCollectStats(c);
+    if (c.nActive == 0) {  // as the ordinary step reports a step without a solve
+      c.st.iterations = 0;
+      c.st.residual = 0.0f;
+      c.st.rows = 0;
+      c.st.colors = 0;
+      c.ncolors = 0;
+      c.colorOff.assign(1, 0);
+      c.residHist.clear();
+    }
   }
   if (epaOverflow) throw std::length_error("EPA workspace overflow in these steps (the steps have been applied)");
 }
