@@ -33,9 +33,9 @@ constexpr int kHdrTicket = 2 * kMaxPasses * kRadix;    // tile tickets per pass
 constexpr int kHdrErr = kHdrTicket + 16;               // sticky: a look-back wait gave up
 constexpr int kSortHdr = kHdrErr + 16;
 constexpr uint32_t kStAgg = 1u << 30, kStIncl = 2u << 30, kStMask = (1u << 30) - 1u;  // look-back status word
-// keys per thread of a digit pass: 8 (2048-key tiles: half the tiles and look-back steps) for large sorts, 4 for
-// small ones (more workgroups)
-inline int sweepPer(int cap) { return cap >= (1 << 18) ? 8 : 4; }
+// keys per thread of a digit pass: 16 from 512k keys, 8 from 256k (fewer tiles: shorter look-back walks), 4 for
+// small sorts (more workgroups)
+inline int sweepPer(int cap) { return cap >= (1 << 19) ? 16 : cap >= (1 << 18) ? 8 : 4; }
 inline int sweepTiles(int cap) { return (std::max(cap, 1) + kPT * sweepPer(cap) - 1) / (kPT * sweepPer(cap)); }
 
 __device__ __forceinline__ int countOf(const int* count, int hostCount, int cap) {
@@ -369,8 +369,15 @@ bool primRadixSort(hipStream_t s, K* k0, int* v0, K* k1, int* v1, const int* cou
     K* ko = alt ? k0 : k1;
     int* vi = alt ? v1 : v0;
     int* vo = alt ? v0 : v1;
-    const bool wide = sweepPer(cap) == 8;
-    if (v0 && wide)
+    const int per = sweepPer(cap);
+    const bool wide = per == 8;
+    if (v0 && per == 16)
+      hipLaunchKernelGGL((kOneSweep<K, true, 16>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
+                         count, hostCount, cap, shift, mask, p, hdr, status);
+    else if (per == 16)
+      hipLaunchKernelGGL((kOneSweep<K, false, 16>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)nullptr, ko,
+                         (int*)nullptr, count, hostCount, cap, shift, mask, p, hdr, status);
+    else if (v0 && wide)
       hipLaunchKernelGGL((kOneSweep<K, true, 8>), dim3(tiles), dim3(kPT), 0, s, (const K*)ki, (const int*)vi, ko, vo,
                          count, hostCount, cap, shift, mask, p, hdr, status);
     else if (v0)
