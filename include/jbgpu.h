@@ -246,6 +246,14 @@ int jbgpu_np_dispatch(jbgpu_ctx* ctx, int64_t n, const uint64_t* keys, const flo
  * result.  JBGPU_ERR_ARG outside the ranges. */
 int jbgpu_np_set_epa_capacity(jbgpu_ctx* ctx, int32_t small_faces, int32_t large_faces);
 
+/* The device radix sort the broadphase, the pair cache, the dispatcher's bins and the colour order use (no
+ * reference counterpart: the reference sorts on the host with its own comparators), on caller host arrays, for
+ * its tests: n keys of key_bytes (4 or 8) bytes, sorted in place by bits [bit_lo, bit_hi) as unsigned words,
+ * stably (equal digits keep their order; bits outside the range are carried along), with the n int values
+ * (NULL: none) moved alongside.  JBGPU_ERR_ARG for n outside [0, 2^28], a bad key size or bit range. */
+int jbgpu_prim_sort(jbgpu_ctx* ctx, void* keys, int32_t key_bytes, int32_t* values, int64_t n, int32_t bit_lo,
+                    int32_t bit_hi);
+
 /* ---- ConstraintSolver -------------------------------------------------------------- */
 /* The solver seam as DiscreteDynamicsWorld.solveConstraints drives it (dynamics/DiscreteDynamicsWorld.java:
  * 636-643): prepareSolve(numBodies, numManifolds) once (ConstraintSolver.java:41), solveGroup once per awake

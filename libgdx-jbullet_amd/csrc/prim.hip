@@ -33,6 +33,7 @@ constexpr int kHdrTicket = 2 * kMaxPasses * kRadix;    // tile tickets per pass
 constexpr int kHdrErr = kHdrTicket + 16;               // sticky: a look-back wait gave up
 constexpr int kSortHdr = kHdrErr + 16;
 constexpr uint32_t kStAgg = 1u << 30, kStIncl = 2u << 30, kStMask = (1u << 30) - 1u;  // look-back status word
+constexpr int kLookback = 8;  // earlier tiles' status words read per look-back round trip
 // keys per thread of a digit pass: 16 from 512k keys, 8 from 256k (fewer tiles: shorter look-back walks), 4 for
 // small sorts (more workgroups)
 inline int sweepPer(int cap) { return cap >= (1 << 19) ? 16 : cap >= (1 << 18) ? 8 : 4; }
@@ -218,30 +219,33 @@ __global__ __launch_bounds__(kPT) void kDigitScan(int passes, int* hdr) {
   if (t < kMaxPasses) hdr[kHdrTicket + t] = 0;
 }
 
-// one digit pass: the tile of this workgroup's ticket, its keys in registers, look-back, stable scatter
+// one digit pass: the tile of this workgroup's ticket, its keys in registers, look-back, stable scatter.
+// Every 256-key chunk of the tile is ranked up front (eight ballots per wave give a key's rank among the wave's
+// keys of its digit, and the wave's count of each digit), with no barrier between chunks; one scan per digit
+// over the (chunk, wave) counts then gives the tile's digit counts (for the look-back) and every key's offset
+// inside the tile, in chunk, wave, lane order: the order of a stable sort.
 template <class K, bool Vals, int Per>
 __global__ __launch_bounds__(kPT) void kOneSweep(const K* __restrict__ kin, const int* __restrict__ vin,
                                                  K* __restrict__ kout, int* __restrict__ vout,
                                                  const int* __restrict__ count, int hostCount, int cap, int shift,
                                                  int mask, int pass, int* hdr, uint32_t* status) {
+  constexpr int kWaves = kPT / 64;
   __shared__ int sTile;
-  __shared__ int tileCount[kRadix];
   __shared__ int running[kRadix];
-  __shared__ int cnt[kPT / 64][kRadix];
+  // per (chunk, wave, digit): the wave's count of the digit in the chunk, then its offset inside the tile
+  __shared__ __align__(16) unsigned short wc[Per][kWaves][kRadix];
   const int n = countOf(count, hostCount, cap);
   constexpr int kTileKeys = kPT * Per;
   const int used = (n + kTileKeys - 1) / kTileKeys;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) sTile = atomicAdd(&hdr[kHdrTicket + pass], 1);
-  tileCount[t] = 0;
-#pragma unroll
-  for (int q = 0; q < kPT / 64; q++) cnt[q][t] = 0;
+  for (int i = t; i < Per * kWaves * kRadix / 8; i += kPT) reinterpret_cast<uint4*>(&wc[0][0][0])[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   const int tile = sTile;
   if (tile >= used) return;  // the whole workgroup: no tile left
   const int base = tile * kTileKeys;
   K key[Per];
-  int val[Per], dig[Per];
+  int val[Per], dig[Per], rank[Per];
 #pragma unroll
   for (int q = 0; q < Per; q++) {
     const int i = base + q * kPT + t;
@@ -249,43 +253,10 @@ __global__ __launch_bounds__(kPT) void kOneSweep(const K* __restrict__ kin, cons
     key[q] = ok ? kin[i] : (K)0;
     val[q] = (Vals && ok) ? vin[i] : 0;
     dig[q] = digitOf(key[q], shift, mask);
-    if (ok) atomicAdd(&tileCount[dig[q]], 1);
   }
-  __syncthreads();
-  // status[pass][tile][digit]: the tile's count at once (its inclusive prefix for tile 0), then the inclusive
-  // prefix once the earlier tiles' counts are summed
-  uint32_t* st = status + (size_t)pass * gridDim.x * kRadix;
-  const int mine = tileCount[t];
-  __hip_atomic_store(st + (size_t)tile * kRadix + t, (tile == 0 ? kStIncl : kStAgg) | (uint32_t)mine,
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  int prefix = 0;
-  if (tile > 0) {
-    int j = tile - 1;
-    unsigned spins = 0;
-    while (j >= 0) {
-      const uint32_t sv = __hip_atomic_load(st + (size_t)j * kRadix + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t f = sv & ~kStMask;
-      if (f == 0u) {
-        if (++spins > (1u << 24)) {
-          atomicOr(&hdr[kHdrErr], 1);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      prefix += (int)(sv & kStMask);
-      if (f == kStIncl) break;
-      j--;
-    }
-    __hip_atomic_store(st + (size_t)tile * kRadix + t, kStIncl | (uint32_t)(prefix + mine), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  running[t] = hdr[kHdrOfs + pass * kRadix + t] + prefix;
-  __syncthreads();
   const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int q = 0; q < Per; q++) {
-    if (base + q * kPT >= n) break;  // the same for the whole workgroup
     const bool valid = base + q * kPT + t < n;
     const int d = dig[q];
     unsigned long long m = __ballot(valid);
@@ -294,24 +265,74 @@ __global__ __launch_bounds__(kPT) void kOneSweep(const K* __restrict__ kin, cons
       const unsigned long long bb = __ballot((d >> b) & 1);
       m &= ((d >> b) & 1) ? bb : ~bb;
     }
-    const int rank = __popcll(m & below);
-    if (valid && rank == 0) cnt[w][d] = __popcll(m);
-    __syncthreads();
-    if (valid) {
-      int pos = running[d] + rank;
-      for (int qq = 0; qq < w; qq++) pos += cnt[qq][d];
+    rank[q] = __popcll(m & below);
+    if (valid && rank[q] == 0) wc[q][w][d] = (unsigned short)__popcll(m);
+  }
+  __syncthreads();
+  int mine = 0;
+#pragma unroll
+  for (int q = 0; q < Per; q++)
+#pragma unroll
+    for (int v = 0; v < kWaves; v++) {
+      const int c = wc[q][v][t];
+      wc[q][v][t] = (unsigned short)mine;
+      mine += c;
+    }
+  // status[pass][tile][digit]: the tile's count at once (its inclusive prefix for tile 0), then the inclusive
+  // prefix once the earlier tiles' counts are summed
+  uint32_t* st = status + (size_t)pass * gridDim.x * kRadix;
+  __hip_atomic_store(st + (size_t)tile * kRadix + t, (tile == 0 ? kStIncl : kStAgg) | (uint32_t)mine,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int prefix = 0;
+  if (tile > 0) {
+    // the walk reads kLookback earlier tiles' words per round trip (independent loads), then sums them in order
+    // until an inclusive prefix, or stops at a word not yet published and reads again from there
+    int j = tile - 1;
+    unsigned spins = 0;
+    bool found = false;
+    while (!found) {
+      uint32_t sv[kLookback];
+#pragma unroll
+      for (int k = 0; k < kLookback; k++)
+        sv[k] = j - k >= 0 ? __hip_atomic_load(st + (size_t)(j - k) * kRadix + t, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : 0u;  // never summed: tile 0's word is inclusive
+      int taken = 0;
+      bool stalled = false;
+#pragma unroll
+      for (int k = 0; k < kLookback; k++) {
+        if (found || stalled) continue;
+        const uint32_t f = sv[k] & ~kStMask;
+        if (f == 0u) {
+          stalled = true;
+          continue;
+        }
+        prefix += (int)(sv[k] & kStMask);
+        taken++;
+        if (f == kStIncl) found = true;
+      }
+      j -= taken;
+      if (stalled && !found) {
+        if (++spins > (1u << 24)) {
+          atomicOr(&hdr[kHdrErr], 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __hip_atomic_store(st + (size_t)tile * kRadix + t, kStIncl | (uint32_t)(prefix + mine), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  running[t] = hdr[kHdrOfs + pass * kRadix + t] + prefix;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Per; q++) {
+    if (base + q * kPT + t < n) {
+      const int d = dig[q];
+      const int pos = running[d] + wc[q][w][d] + rank[q];
       kout[pos] = key[q];
       if (Vals) vout[pos] = val[q];
     }
-    __syncthreads();
-    int add = 0;
-#pragma unroll
-    for (int qq = 0; qq < kPT / 64; qq++) {
-      add += cnt[qq][t];
-      cnt[qq][t] = 0;
-    }
-    running[t] += add;
-    __syncthreads();
   }
 }
 
@@ -433,6 +454,7 @@ template void primSortPairs<uint64_t>(Ctx&, Db<uint64_t>&, Db<int>&, int, int, i
 template void primSortPairs<unsigned long long>(Ctx&, Db<unsigned long long>&, Db<int>&, int, int, int);
 template void primSortPairs<int>(Ctx&, Db<int>&, Db<int>&, int, int, int);
 template void primSortKeys<uint64_t>(Ctx&, Db<uint64_t>&, int, int, int);
+template void primSortKeys<int>(Ctx&, Db<int>&, int, int, int);
 
 void primSelectIndex(Ctx& c, const int* flag, int n, int* out, int* count) {
   if (n <= 0) {

@@ -1298,6 +1298,47 @@ int jbgpu_np_set_epa_capacity(jbgpu_ctx* ctx, int32_t small_faces, int32_t large
   });
 }
 
+int jbgpu_prim_sort(jbgpu_ctx* ctx, void* keys, int32_t key_bytes, int32_t* values, int64_t n, int32_t bit_lo,
+                    int32_t bit_hi) {
+  return guard(ctx, [&] {
+    if (n < 0 || n > ((int64_t)1 << 28)) throw std::invalid_argument("sort length must lie in [0, 2^28]");
+    if (key_bytes != 4 && key_bytes != 8) throw std::invalid_argument("key size must be 4 or 8 bytes");
+    if (bit_lo < 0 || bit_hi > 8 * key_bytes || bit_lo > bit_hi)
+      throw std::invalid_argument("bit range must satisfy 0 <= bit_lo <= bit_hi <= 8 * key_bytes");
+    if (n > 0 && !keys) throw std::invalid_argument("null keys");
+    if (n == 0) return;
+    Ctx& c = ctx->c;
+    hipStream_t s = c.stream;
+    auto run = [&](auto* hk) {
+      using K = std::remove_pointer_t<decltype(hk)>;
+      DBuf<K> k0, k1;
+      DBuf<int> v0, v1;
+      k0.ensure(n);
+      k1.ensure(n);
+      JB_CHECK(hipMemcpyAsync(k0.p, hk, n * sizeof(K), hipMemcpyHostToDevice, s));
+      Db<K> dk(k0.p, k1.p);
+      if (values) {
+        v0.ensure(n);
+        v1.ensure(n);
+        JB_CHECK(hipMemcpyAsync(v0.p, values, n * sizeof(int), hipMemcpyHostToDevice, s));
+        Db<int> dv(v0.p, v1.p);
+        primSortPairs(c, dk, dv, (int)n, bit_lo, bit_hi);
+        JB_CHECK(hipMemcpyAsync(values, dv.Current(), n * sizeof(int), hipMemcpyDeviceToHost, s));
+      } else {
+        primSortKeys(c, dk, (int)n, bit_lo, bit_hi);
+      }
+      JB_CHECK(hipMemcpyAsync(hk, dk.Current(), n * sizeof(K), hipMemcpyDeviceToHost, s));
+      primErrCopy(c);
+      JB_CHECK(hipStreamSynchronize(s));
+      primErrCheck(c);
+    };
+    if (key_bytes == 8)
+      run((uint64_t*)keys);
+    else
+      run((int*)keys);
+  });
+}
+
 int jbgpu_world_num_joints(jbgpu_ctx* ctx, int32_t* n) {
   return guard(ctx, [&] {
     if (n) *n = ctx->c.nj;

@@ -84,6 +84,7 @@ SYMBOLS = {
     "jbgpu_hinge_frames": (C.c_int, [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jbgpu_world_num_joints": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "jbgpu_np_set_epa_capacity": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "jbgpu_prim_sort": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int64, C.c_int32, C.c_int32]),
     "jbgpu_world_joint_impulses": (C.c_int, [_vp, _vp]),
     "jbgpu_world_joint_order": (C.c_int, [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)]),
     "jbgpu_check_pair_records": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp]),
@@ -437,6 +438,20 @@ class Context:
     def np_set_epa_capacity(self, small_faces=192, large_faces=1024):
         """Face capacities of the two LDS EPA tiers (work moves between tiers; results never change)."""
         self._ck(load().jbgpu_np_set_epa_capacity(self.h, int(small_faces), int(large_faces)), "np_set_epa_capacity")
+
+    def prim_sort(self, keys, values=None, bit_lo=0, bit_hi=None):
+        """The device radix sort on copies of `keys` (uint32 or uint64) and `values` (int32): stable by bits
+        [bit_lo, bit_hi) of the keys as unsigned words.  Returns (keys, values)."""
+        k = np.ascontiguousarray(keys).copy()
+        if k.dtype not in (np.uint32, np.uint64):
+            raise ValueError("keys must be uint32 or uint64")
+        v = None if values is None else np.ascontiguousarray(values, np.int32).copy()
+        if v is not None and v.size != k.size:
+            raise ValueError("values must match the keys")
+        hi = 8 * k.itemsize if bit_hi is None else int(bit_hi)
+        self._ck(load().jbgpu_prim_sort(self.h, _p(k), k.itemsize, None if v is None else _p(v), k.size,
+                                        int(bit_lo), hi), "prim_sort")
+        return k, v
 
     def num_joints(self):
         n = C.c_int32(0)
