@@ -320,7 +320,7 @@ struct Ctx {
   HBuf<unsigned long long> solveRows; // rows in total, rows of the per-colour or block sweeps
   struct {
     bool pending = false, tolMode = false, blockSweeps = false, sampleK = false;
-    int iters = 0;
+    int iters = 0, sampledIters = 0;  // sampledIters: iterations whose sweep launches were event-timed
     size_t kUsed = 0;
   } solveStat;
   DBuf<float> candC;  // contact candidates npairs*8

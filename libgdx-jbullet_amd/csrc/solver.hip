@@ -2403,8 +2403,13 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   float* resOut = (float*)(c.counter.p + 6);
   JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
   int* done = tolMode ? c.done.p : nullptr;
-  // kernel-level timing of this step's sweeps (sampled)
+  // kernel-level timing of this step's sweeps (sampled): on a sampled step, the launches of every
+  // kSampleStride-th iteration get their own events (every iteration sweeps the same colours and rows; the events
+  // cost the step about 10 us per launch, so timing all of them would stretch the sampled step)
+  constexpr int kSampleStride = 5;
   const bool sampleK = c.kernelEvery > 0 && c.kernelSteps++ % c.kernelEvery == 0;
+  bool sampleIt = false;
+  int sampledIters = 0;
   size_t kUsed = 0;
   auto kPair = [&](int kind) -> hipEvent_t* {
     if (kUsed + 2 > c.kEv.size()) {
@@ -2428,7 +2433,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
     const int* rs = c.runStart.p;
     const int* re = c.runEnd.p;
     const int* dn = done;
-    if (sampleK) {
+    if (sampleIt) {
       hipEvent_t* ev = kPair(friction ? 3 : 2);
       if (friction)
         hipExtLaunchKernelGGL(kSweepBlocks<true>, grid, block, blockLds, s, ev[0], ev[1], 0, nbt, nb, B, c.np,
@@ -2450,12 +2455,14 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if (tolMode) c.doneHost.ensure(1);
   for (int itn = 0; itn < iters; itn++) {
     const bool track = tolMode || itn == iters - 1;
+    sampleIt = sampleK && itn % kSampleStride == 0;
+    sampledIters += sampleIt ? 1 : 0;
     double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
     jointLaunches += jp.sweep(dt, done);  // the constraints first in every iteration (:823-842)
     sweepBlocks(false, res);
     for (int k = firstColor; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      if (sampleK) {
+      if (sampleIt) {
         hipEvent_t* ev = kPair(0);
         hipExtLaunchKernelGGL(kNormal, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, ev[0], ev[1], 0, (int)c.colorOff[k],
                               cnt, SP, (const int4*)c.hdr.p, c.nrow.p, (const float*)c.props.p, c.svel.p, res,
@@ -2472,7 +2479,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
     sweepBlocks(true, res);
     for (int k = firstColor; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
-      if (sampleK) {
+      if (sampleIt) {
         hipEvent_t* ev = kPair(1);
         hipExtLaunchKernelGGL(kFriction, dim3(gridFor(4 * (int64_t)cnt, kSweepThreads)), dim3(kSweepThreads), 0, s, ev[0], ev[1], 0,
                               (int)c.colorOff[k], cnt, SP, (const int4*)c.hdr.p, (const float*)c.nrow.p, c.frow.p,
@@ -2518,6 +2525,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   c.solveStat.tolMode = tolMode;
   c.solveStat.blockSweeps = blockSweeps;
   c.solveStat.sampleK = sampleK;
+  c.solveStat.sampledIters = sampledIters;
   c.solveStat.iters = iters;
   c.solveStat.kUsed = kUsed;
 }
@@ -2542,10 +2550,11 @@ void solverCollectStats(Ctx& c) {
       c.kMs[kind] += ms;
       c.kLaunches[kind] += 1;
     }
-    c.kRows[0] += colourRows * iters;
-    c.kRows[1] += 2 * colourRows * iters;
-    c.kRows[2] += blockRows * iters;
-    c.kRows[3] += 2 * blockRows * iters;
+    const int si = c.solveStat.sampledIters;
+    c.kRows[0] += colourRows * si;
+    c.kRows[1] += 2 * colourRows * si;
+    c.kRows[2] += blockRows * si;
+    c.kRows[3] += 2 * blockRows * si;
   }
   c.st.residual = ro[0];
   c.st.iterations = (int64_t)ro[1];
