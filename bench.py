@@ -240,7 +240,7 @@ def pmc_child(args):
     # every sweep launch is counted; the launches (and rows) of the last PMC_STEPS steps are tallied here and
     # the parent keeps that many of the last counter records
     ctx.step(n=args.pmc_warmup)
-    ctx.timing_kernels(1)
+    ctx.timing_kernels(1, stride=1)
     ctx.step(n=PMC_STEPS)
     ks = ctx.timing_kernels_get()[args.pmc_kind]
     ctx.measure_copy(PMC_COPY_BYTES, 1)
@@ -360,7 +360,8 @@ def autotune(ctx, dt, steps=TUNE_STEPS):
 
 
 def roofline(ks, args):
-    """Dominant kernel: the solver's colour sweeps.  Each sampled launch is timed by its own start/stop events
+    """Dominant kernel: the solver's colour sweeps.  The sampled launches (the sweeps of every fifth iteration on
+    a sampled step of the timed region) are each timed by their own start/stop events
     on the library stream (kernel execution only); achieved = algorithmic bytes per launch / average duration,
     with ROW_BYTES per row per sweep (80 B row + 4 B impulse + 2 x 48 B body velocity read and write)."""
     name = max(ks, key=lambda k: ks[k]["ms"])

@@ -362,10 +362,13 @@ int jbgpu_solver_use_mlp(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_enable(jbgpu_ctx* ctx, int32_t on);
 int jbgpu_timing_get(jbgpu_ctx* ctx, double* ms, int64_t* launches, int64_t* steps);
 /* Kernel-level timing of the solver sweeps (independent of jbgpu_timing_enable): on every `every`-th
- * solve after this call (0 = off) each sweep launch carries its own start/stop events
- * (hipExtLaunchKernel), so the figures are kernel execution time without launch gaps: summed ms,
- * timed launches and rows processed by them. */
+ * solve after this call (0 = off) each sweep launch of every `stride`-th iteration (jbgpu_timing_kernels_stride,
+ * default 5) carries its own start/stop events (hipExtLaunchKernel), so the figures are kernel execution time
+ * without launch gaps: summed ms, timed launches and rows processed by them.  Every iteration sweeps the same
+ * colours; the events cost about 10 us per launch, which a sampled solve pays. */
 int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every);
+/* The iteration stride of that timing (at least 1; 1 = every iteration). */
+int jbgpu_timing_kernels_stride(jbgpu_ctx* ctx, int32_t stride);
 /* four entries each: [0] normal rows, per-colour launches, [1] friction rows, per-colour launches,
  * [2] normal rows, block launches, [3] friction rows, block launches */
 /* Device memory for callers that keep their own inputs resident in HBM (e.g. AABBs for

@@ -400,6 +400,7 @@ struct Ctx {
   int64_t timedSteps = 0;
   // sampled kernel timing of the solver sweeps (start/stop events per launch)
   int kernelEvery = 0;
+  int kernelStride = 5;  // on a sampled solve, the sweeps of every kernelStride-th iteration are event-timed
   int64_t kernelSteps = 0;  // solves since the sampling period was set
   int colourRounds = 0;     // colouring rounds the last solve needed
   std::vector<hipEvent_t> kEv;  // event pool, two per timed launch

@@ -2404,9 +2404,8 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   JB_CHECK(hipMemsetAsync(resOut, 0, 2 * sizeof(float), s));
   int* done = tolMode ? c.done.p : nullptr;
   // kernel-level timing of this step's sweeps (sampled): on a sampled step, the launches of every
-  // kSampleStride-th iteration get their own events (every iteration sweeps the same colours and rows; the events
+  // kernelStride-th iteration get their own events (every iteration sweeps the same colours and rows; the events
   // cost the step about 10 us per launch, so timing all of them would stretch the sampled step)
-  constexpr int kSampleStride = 5;
   const bool sampleK = c.kernelEvery > 0 && c.kernelSteps++ % c.kernelEvery == 0;
   bool sampleIt = false;
   int sampledIters = 0;
@@ -2455,7 +2454,7 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if (tolMode) c.doneHost.ensure(1);
   for (int itn = 0; itn < iters; itn++) {
     const bool track = tolMode || itn == iters - 1;
-    sampleIt = sampleK && itn % kSampleStride == 0;
+    sampleIt = sampleK && itn % c.kernelStride == 0;
     sampledIters += sampleIt ? 1 : 0;
     double* res = track ? c.resid.p + histWords + 2 * (size_t)kResidStripes * itn : nullptr;
     jointLaunches += jp.sweep(dt, done);  // the constraints first in every iteration (:823-842)

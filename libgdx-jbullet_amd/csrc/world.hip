@@ -1802,6 +1802,13 @@ int jbgpu_timing_kernels(jbgpu_ctx* ctx, int32_t every) {
   });
 }
 
+int jbgpu_timing_kernels_stride(jbgpu_ctx* ctx, int32_t stride) {
+  return guard(ctx, [&] {
+    if (stride < 1) throw std::invalid_argument("iteration stride must be at least 1");
+    ctx->c.kernelStride = stride;
+  });
+}
+
 int jbgpu_timing_kernels_get(jbgpu_ctx* ctx, double* ms2, int64_t* launches2, int64_t* rows2) {
   return guard(ctx, [&] {
     Ctx& c = ctx->c;

@@ -114,6 +114,7 @@ SYMBOLS = {
     "jbgpu_timing_enable": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_timing_get": (C.c_int, [_vp, _vp, _vp, C.POINTER(C.c_int64)]),
     "jbgpu_timing_kernels": (C.c_int, [_vp, C.c_int32]),
+    "jbgpu_timing_kernels_stride": (C.c_int, [_vp, C.c_int32]),
     "jbgpu_measure_copy": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "jbgpu_buffer_alloc": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp)]),
     "jbgpu_buffer_upload": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
@@ -629,8 +630,11 @@ class Context:
         self._ck(load().jbgpu_measure_copy(self.h, int(nbytes), int(reps), C.byref(g)), "measure_copy")
         return g.value
 
-    def timing_kernels(self, every):
-        """Time each solver sweep launch by its own events on every `every`-th timed step (0 = off)."""
+    def timing_kernels(self, every, stride=None):
+        """Time the solver sweep launches by their own events on every `every`-th step (0 = off): those of every
+        `stride`-th iteration (default 5; 1 = all), each launch its own events."""
+        if stride is not None:
+            self._ck(load().jbgpu_timing_kernels_stride(self.h, int(stride)), "timing_kernels_stride")
         self._ck(load().jbgpu_timing_kernels(self.h, int(every)), "timing_kernels")
 
     def timing_kernels_get(self):
