@@ -142,3 +142,32 @@ def test_aabb_overflow_disables_simulation_on_device():
     # the transform never changes (its velocity took one step of gravity before the AABB update disabled it)
     assert np.array_equal(bg[2, :12].view(np.uint32), bodies[2, :12].view(np.uint32))
     ctx.close()
+
+
+def test_sweep_timing_stride():
+    """Sampled sweep timing times the launches of every stride-th iteration: from the same state, 20 iterations
+    with a stride of 5 time a fifth of the launches and rows a stride of 1 does (every colour its own launch);
+    the step itself is the same; a stride below 1 is refused."""
+    sc = scenes.pyramids(seed=7, n_boxes=800, n_pyramids=6, n_spheres=0, area=60.0)
+    ctx, shapes, verts, bodies = _world(sc, iterations=20)
+    ctx.solver_set_tail(0)
+    ctx.step(n=5)
+    snap = (ctx.bodies(), ctx.bp_proxies(), ctx.pairs())
+    got, ends = {}, {}
+    for stride in (1, 5):
+        ctx.set_bodies(snap[0])
+        ctx.bp_set_proxies(*snap[1])
+        ctx.set_pairs(*snap[2])
+        ctx.timing_kernels(1, stride=stride)
+        ctx.step()
+        got[stride] = ctx.timing_kernels_get()
+        ctx.timing_kernels(0)
+        ends[stride] = ctx.bodies()
+    for kind in ("normal", "friction"):
+        a, b = got[1][kind], got[5][kind]
+        assert b["launches"] > 0 and a["launches"] == 5 * b["launches"], (kind, a, b)
+        assert a["rows"] == 5 * b["rows"], (kind, a, b)
+    assert np.array_equal(ends[1].view(np.uint32), ends[5].view(np.uint32))
+    with pytest.raises(jbgpu.JbgpuError):
+        ctx.timing_kernels(1, stride=0)
+    ctx.close()
