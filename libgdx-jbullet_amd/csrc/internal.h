@@ -363,7 +363,8 @@ struct Ctx {
   DBuf<uint64_t> colorKey, colorKeyAlt;
   DBuf<int> colorOrder, colorOrderAlt;
   std::vector<int64_t> colorOff;
-  DBuf<int> colorOffDev;     // colour offsets on the device (tail sweeps)
+  DBuf<int> colorOffDev;     // colour offsets on the device (tail sweeps, the per-body warm start)
+  DBuf<int> warmSlotOf;      // per (colour batch, body): the body's slot in the batch, -1 for none
   // block-partitioned sweeps: dynamic bodies in blocks of blockSize consecutive Morton positions; manifolds
   // inside one block ("interior") are swept by one workgroup per block with the block's velocities in LDS,
   // the others ("boundary") by one launch per colour

@@ -524,9 +524,68 @@ __global__ __launch_bounds__(64) void kWarmApply(int off, int cnt, const int4* _
   if (k < cnt) warmApplySlot(off + k, hdr, nrow, frow, props, svel);
 }
 
+// The warm start gathered per body in one launch.  A colour batch's slots share no dynamic body, so a body has at
+// most one slot per batch: slotOf[k * nb + body] (k the batch, -1 for none) lists them, and each body applies its
+// slots' rows batch after batch, point after point, normal row then the two friction rows: the same additions in
+// the same order as the batch-by-batch launches give its velocity.
+__global__ void kWarmSlotTable(int na, int nBatches, const int* __restrict__ colorOff, const int4* __restrict__ hdr,
+                               const float* __restrict__ svel, int nb, int* slotOf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= na) return;
+  int lo = 0, hi = nBatches;  // the batch of slot p: colorOff[lo] <= p < colorOff[lo + 1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (colorOff[mid] <= p) lo = mid; else hi = mid;
+  }
+  const int4 h = hdr[p];
+  if (svel[8 * (size_t)h.y + 3] != 0.0f) slotOf[(size_t)lo * nb + h.y] = p;
+  if (svel[8 * (size_t)h.z + 3] != 0.0f) slotOf[(size_t)lo * nb + h.z] = p;
+}
+__global__ void kWarmGather(int nb, int nBatches, const int* __restrict__ slotOf, const int4* __restrict__ hdr,
+                            const float* __restrict__ nrow, const float* __restrict__ frow,
+                            const float* __restrict__ props, float* svel) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nb) return;
+  SBody X = loadBody(svel, props, x);
+  if (X.invMass == 0.0f) return;
+  bool any = false;
+  // eight batches at a time: their table words, then the headers of the slots found, loaded together
+  for (int k0 = 0; k0 < nBatches; k0 += 8) {
+    int ps[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) ps[j] = k0 + j < nBatches ? slotOf[(size_t)(k0 + j) * nb + x] : -1;
+    int4 hs[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) hs[j] = ps[j] >= 0 ? hdr[ps[j]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (ps[j] < 0) continue;
+      any = true;
+      const int4 h = hs[j];
+      const bool isA = h.y == x;
+      const int rows = __popc(slotMask(h)), row0 = slotRow(h);
+      for (int q = 0; q < rows; q++) {
+        // the point's three rows: direction, impulse and this body's angular component, loaded together
+        const float* R0 = nrow + (size_t)RW * (row0 + q);
+        const float* R1 = frow + (size_t)RW * (2 * (row0 + q));
+        const float* R2 = R1 + RW;
+        const int ao = isA ? 9 : 12;
+        const V3 n0 = ld3(R0), n1 = ld3(R1), n2 = ld3(R2);
+        const V3 c0 = ld3(R0 + ao), c1 = ld3(R1 + ao), c2 = ld3(R2 + ao);
+        const float a0 = R0[19], a1 = R1[19], a2 = R2[19];
+        applyImpulse(X, scl(n0, X.invMass), c0, isA ? a0 : -a0);
+        applyImpulse(X, scl(n1, X.invMass), c1, isA ? a1 : -a1);
+        applyImpulse(X, scl(n2, X.invMass), c2, isA ? a2 : -a2);
+      }
+    }
+  }
+  if (any) storeBody(svel, x, X);
+}
+
 // the small colours [c0, c1) in one workgroup, colour after colour (a barrier orders the workgroup's global
 // accesses: its waves share one CU and its L1)
 constexpr int kWarmTailThreads = 1024;
+constexpr int kWarmGatherBatches = 256;  // the per-body warm start's table holds this many batches at most
 __global__ __launch_bounds__(kWarmTailThreads) void kWarmApplyTail(int c0, int c1, const int* __restrict__ colorOff,
                                                                    const int4* __restrict__ hdr,
                                                                    const float* __restrict__ nrow,
@@ -2379,7 +2438,21 @@ void solverSolve(Ctx& c, float dt, const std::vector<int>* explicitList) {
   if (learnedHere) mlpWarmStart(c, na);  // overwrites the applied words of the valid rows
   const bool apply = c.warmstart || c.useLearned || c.useMlp;
   int setupLaunches = 1;
-  if (apply) {  // warm start colour by colour: the head colours one launch each, the tail in one workgroup
+  if (apply && !blockSweeps && c.ncolors <= kWarmGatherBatches) {  // warm start gathered per body
+    c.warmSlotOf.ensure((size_t)c.ncolors * c.nb);
+    if (c.headColors >= c.ncolors) {  // the batch offsets on the device (already there when a tail is swept)
+      std::vector<int> offs(c.colorOff.begin(), c.colorOff.end());
+      c.colorOffDev.ensure(offs.size());
+      JB_CHECK(hipMemcpyAsync(c.colorOffDev.p, offs.data(), offs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    JB_CHECK(hipMemsetAsync(c.warmSlotOf.p, 0xff, (size_t)c.ncolors * c.nb * sizeof(int), s));
+    hipLaunchKernelGGL(kWarmSlotTable, dim3(gridFor(na)), dim3(256), 0, s, na, c.ncolors, (const int*)c.colorOffDev.p,
+                       (const int4*)c.hdr.p, (const float*)c.svel.p, c.nb, c.warmSlotOf.p);
+    hipLaunchKernelGGL(kWarmGather, dim3(gridFor(c.nb)), dim3(256), 0, s, c.nb, c.ncolors,
+                       (const int*)c.warmSlotOf.p, (const int4*)c.hdr.p, (const float*)c.nrow.p,
+                       (const float*)c.frow.p, (const float*)c.props.p, c.svel.p);
+    setupLaunches += 2;
+  } else if (apply) {  // warm start colour by colour: the head colours one launch each, the tail in one workgroup
     for (int k = 0; k < headColors; k++) {
       int cnt = (int)(c.colorOff[k + 1] - c.colorOff[k]);
       hipLaunchKernelGGL(kWarmApply, dim3(gridFor(cnt, 64)), dim3(64), 0, s, (int)c.colorOff[k], cnt, c.hdr.p,
