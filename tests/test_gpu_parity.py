@@ -292,6 +292,32 @@ def test_world_mixed_pile_100k_parity():
     ctx.close()
 
 
+def test_colour_batches_share_no_dynamic_body():
+    """The 100k pile's colour batches (Jones-Plassmann rounds, the first batch's length adapting from step to
+    step): within a batch no two manifolds share a dynamic body, and every active manifold is in exactly one
+    batch, on every checked step."""
+    sc = scenes.mixed_pile(seed=1, n=100_000)
+    shapes, verts, bodies = scenes.build_records(sc, jbgpu)
+    ctx = jbgpu.Context(num_iterations=20)
+    ctx.world_create(shapes, verts, bodies)
+    dyn = bodies[:, 18] != 0  # inverse mass
+    ctx.step(n=300)
+    for step in range(6):
+        ctx.step()
+        order, offs = ctx.solver_order()
+        _, m = ctx.pairs()
+        st = ctx.stats()
+        assert len(order) == st["manifolds"] and len(np.unique(order)) == len(order), f"step {301 + step}"
+        b0 = m[order, 1].view(np.int32)
+        b1 = m[order, 2].view(np.int32)
+        for c in range(len(offs) - 1):
+            x, y = b0[offs[c]:offs[c + 1]], b1[offs[c]:offs[c + 1]]
+            bs = np.concatenate([x[dyn[x]], y[dyn[y]]])
+            assert len(np.unique(bs)) == len(bs), f"step {301 + step}: colour {c} repeats a dynamic body"
+        print(f"[colours] step {301 + step}: {len(offs) - 1} batches over {len(order)} manifolds", flush=True)
+    ctx.close()
+
+
 def test_epa_workspace_retry_step_parity():
     """EPA jobs run first with a small LDS workspace; the few that outgrow it are rerun with the large one. The
     100k pile has such jobs from step ~130 on (a few per ten steps): steps whose EPA pass had retries are
