@@ -296,8 +296,11 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
 
 // sort key of proxy i: world id above a 30-bit Morton code of its candidate-box centre, so that every
 // world's leaves are contiguous and its subtrees separate; huge proxies carry all ones and sort last
+// Morton code of the candidate box centre with mb / 3 bits per axis, the world id above it (a world's leaves are
+// consecutive in the order); huge proxies carry all ones and sort last
 __global__ void kMorton(int n, const float* __restrict__ cand, const int* __restrict__ isHuge,
-                        const int* __restrict__ world, const unsigned int* __restrict__ b, uint64_t* code, int* idx) {
+                        const int* __restrict__ world, const unsigned int* __restrict__ b, int mb, uint64_t* code,
+                        int* idx) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   idx[i] = i;
@@ -314,10 +317,11 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
     float u = ext > 0.0f ? (c - lo) / ext : 0.0f;
     u = fminf(fmaxf(u, 0.0f), 1.0f);
     if (!(u == u)) u = 0.0f;
-    q3[k] = min((uint32_t)(u * 1024.0f), 1023u);
+    const uint32_t cells = 1u << (mb / 3);
+    q3[k] = min((uint32_t)(u * (float)cells), cells - 1u);
   }
   const uint32_t m = (expand10(q3[0]) << 2) | (expand10(q3[1]) << 1) | expand10(q3[2]);
-  code[i] = ((uint64_t)(uint32_t)world[i] << 32) | m;
+  code[i] = ((uint64_t)(uint32_t)world[i] << mb) | m;
 }
 
 // sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
@@ -341,6 +345,11 @@ __global__ void kGatherLeaves(int n, const int* __restrict__ dn, const int* __re
   dst[0] = a;
   dst[1] = b;
 }
+
+// Morton bits of the sort key (a multiple of 3, at most 30, at least 12): as many as leave the key, world id
+// included, within 31 bits (four 8-bit digit passes), whose top bit stays clear below the huge proxies' all-ones
+static int mortonBits(int worldBits) { return std::min(30, std::max(12, 3 * ((31 - worldBits) / 3))); }
+static int mortonKeyBits(int worldBits) { return worldBits + mortonBits(worldBits) + 1; }
 
 __device__ __forceinline__ int deltaK(const uint64_t* code, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
@@ -1023,13 +1032,13 @@ void rayBuildTree(Ctx& c) {
   int* hugeCount = (int*)(c.counter.p + 7);
   primSelectIndex(c, (const int*)t.isHuge.p, n, t.hugeList.p, hugeCount);
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
-                     (const int*)c.world.p, (const unsigned int*)bounds, t.morton.p, t.order.p);
+                     (const int*)c.world.p, (const unsigned int*)bounds, mortonBits(c.worldBits), t.morton.p,
+                     t.order.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
   Db<uint64_t> dk(t.morton.p, t.mortonAlt.p);
   Db<int> dv(t.order.p, t.orderAlt.p);
-  const int keyBits = 32 + c.worldBits;
-  primSortPairs(c, dk, dv, n, 0, keyBits);
+  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
   JB_CHECK(hipStreamSynchronize(s));
   t.nHuge = hcount;
   t.nn = n - hcount;
@@ -1109,15 +1118,14 @@ void bpFindPairs(Ctx& c) {
                      (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
   int* hugeCount = (int*)(c.counter.p + 1);
   primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, hugeCount);
-  // Morton codes (huge proxies carry all ones and sort last)
+  // Morton codes
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
-                     (const unsigned int*)bounds, c.morton.p, c.order.p);
+                     (const unsigned int*)bounds, mortonBits(c.worldBits), c.morton.p, c.order.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
   Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
   Db<int> dv(c.order.p, c.orderAlt.p);
-  const int keyBits = 32 + c.worldBits;  // one world: the 32-bit Morton sort
-  primSortPairs(c, dk, dv, n, 0, keyBits);
+  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
   JB_CHECK(hipStreamSynchronize(s));
   c.nHuge = hcount;
   c.st.huge = c.nHuge;
@@ -1413,10 +1421,10 @@ const uint64_t* bpStepGraph(Ctx& c) {
                      (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
   primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, gc + kGcHuge);
   hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
-                     (const unsigned int*)bounds, c.morton.p, c.order.p);
+                     (const unsigned int*)bounds, mortonBits(c.worldBits), c.morton.p, c.order.p);
   Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
   Db<int> dv(c.order.p, c.orderAlt.p);
-  primSortPairs(c, dk, dv, n, 0, 32 + c.worldBits);
+  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
   const uint64_t* code = dk.Current();
   const int* order = dv.Current();
   c.sortedOrder = order;
