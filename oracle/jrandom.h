@@ -2,7 +2,7 @@
 // algorithm documented in the Java SE API (48-bit LCG, multiplier 0x5DEECE66D,
 // addend 0xB, seed scrambled with the multiplier).  The reference's solver draws
 // its row shuffle from `new Random(0)` (SequentialImpulseConstraintSolver.java:108,
-// randInt2 at :138-149).  Pinned by known answers in tests/test_oracle_basics.py.
+// randInt2 at :138-149).  Pinned by known answers in tests/test_oracle_pins.py.
 #pragma once
 #include <cstdint>
 
