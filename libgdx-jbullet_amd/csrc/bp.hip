@@ -99,7 +99,10 @@ __device__ void setAabbDev(const BpParams& P, int i, V3 mn, V3 mx, bool update, 
   }
 }
 
-__device__ void writeCandidate(const BpParams& P, int i, const float* aabb, const int* q, float* cand, int* isHuge) {
+// candidate box of proxy i: its stored AABB (Dbvt, Simple) or dequantised box (AxisSweep3); the spare lanes carry
+// its world id and filter word, so the leaf gather reads one 32-byte record per leaf
+__device__ void writeCandidate(const BpParams& P, int i, const float* aabb, const int* q, float* cand, int* isHuge,
+                               const int* __restrict__ world, const int* __restrict__ filt) {
   V3 mn = ld3(aabb + 6 * i), mx = ld3(aabb + 6 * i + 3);
   if (P.mode == JBGPU_BP_SAP16 || P.mode == JBGPU_BP_SAP32) {
     const int* Q = q + 6 * i;
@@ -113,15 +116,16 @@ __device__ void writeCandidate(const BpParams& P, int i, const float* aabb, cons
     mx = v3(r[3], r[4], r[5]);
   }
   float* C = cand + 8 * i;
-  C[0] = mn.x; C[1] = mn.y; C[2] = mn.z; C[3] = 0.0f;
-  C[4] = mx.x; C[5] = mx.y; C[6] = mx.z; C[7] = 0.0f;
+  C[0] = mn.x; C[1] = mn.y; C[2] = mn.z; C[3] = __int_as_float(world[i]);
+  C[4] = mx.x; C[5] = mx.y; C[6] = mx.z; C[7] = __int_as_float(filt[i]);
   float e = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
   isHuge[i] = !(e < HUGE_EXTENT) ? 1 : 0;
 }
 
 __global__ void kSetAabbBodies(BpParams P, int n, const float* __restrict__ shapes, const float* __restrict__ tr,
                                const float* __restrict__ props, int* act, int* stage, float* leaf, float* aabb, int* q,
-                               float* cand, int* isHuge, const int* __restrict__ abortFlag) {
+                               float* cand, int* isHuge, const int* __restrict__ world,
+                               const int* __restrict__ filt, const int* __restrict__ abortFlag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || (abortFlag && *abortFlag)) return;
   int a = act[i];
@@ -146,24 +150,26 @@ __global__ void kSetAabbBodies(BpParams P, int n, const float* __restrict__ shap
     }
   }
   setAabbDev(P, i, mn, mx, update, stage, leaf, aabb, q);
-  writeCandidate(P, i, aabb, q, cand, isHuge);
+  writeCandidate(P, i, aabb, q, cand, isHuge, world, filt);
 }
 
 __global__ void kSetAabbArray(BpParams P, int n, const float* __restrict__ in6, int* stage, float* leaf, float* aabb,
-                              int* q, float* cand, int* isHuge) {
+                              int* q, float* cand, int* isHuge, const int* __restrict__ world,
+                              const int* __restrict__ filt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   setAabbDev(P, i, ld3(in6 + 6 * i), ld3(in6 + 6 * i + 3), true, stage, leaf, aabb, q);
-  writeCandidate(P, i, aabb, q, cand, isHuge);
+  writeCandidate(P, i, aabb, q, cand, isHuge, world, filt);
 }
 
 // a pair query with no setAabb since the last one (e.g. right after createProxy): the stored boxes stay, the
 // collide's stage rotation still runs (DbvtBroadphase.java:96-113), the candidate boxes come from the state
-__global__ void kKeepAabbs(BpParams P, int n, int* stage, float* leaf, float* aabb, int* q, float* cand, int* isHuge) {
+__global__ void kKeepAabbs(BpParams P, int n, int* stage, float* leaf, float* aabb, int* q, float* cand, int* isHuge,
+                           const int* __restrict__ world, const int* __restrict__ filt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   setAabbDev(P, i, v3(0, 0, 0), v3(0, 0, 0), false, stage, leaf, aabb, q);
-  writeCandidate(P, i, aabb, q, cand, isHuge);
+  writeCandidate(P, i, aabb, q, cand, isHuge, world, filt);
 }
 
 __device__ __forceinline__ unsigned int fOrd(float f) {  // order-preserving float -> uint
@@ -327,6 +333,10 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
 // sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
 // leafExact[s]: the exact-test data of sorted leaf s in one 32-byte record: the stored AABB (Dbvt,
 // Simple) or the quantised box (AxisSweep3) as 6 words, the group/mask word and the proxy index
+// Packed: the candidate records carry the world id and filter word in their spare lanes (the broadphase's), and
+// for the Dbvt and Simple modes their boxes are the stored AABBs bit for bit, so a leaf reads one 32-byte record;
+// otherwise (the query tree) the world, filter and exact boxes come from their own arrays
+template <bool Packed>
 __global__ void kGatherLeaves(int n, const int* __restrict__ dn, const int* __restrict__ order, const float* __restrict__ cand,
                               const int* __restrict__ world, float* leafBox, int mode, const float* __restrict__ aabb,
                               const int* __restrict__ q, const int* __restrict__ filt, int4* __restrict__ leafExact) {
@@ -334,14 +344,26 @@ __global__ void kGatherLeaves(int n, const int* __restrict__ dn, const int* __re
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int o = order[s];
-  const int* X = (mode == JBGPU_BP_SAP16 || mode == JBGPU_BP_SAP32) ? q + 6 * o : (const int*)aabb + 6 * o;
-  leafExact[2 * s] = make_int4(X[0], X[1], X[2], X[3]);
-  leafExact[2 * s + 1] = make_int4(X[4], X[5], filt[o], o);
+  const bool sap = mode == JBGPU_BP_SAP16 || mode == JBGPU_BP_SAP32;
   const float4* src = (const float4*)(cand + 8 * o);
-  float4* dst = (float4*)(leafBox + 8 * s);
   float4 a = src[0], b = src[1];
-  a.w = __int_as_float(world[o]);
+  int fo;
+  if (Packed) {
+    fo = __float_as_int(b.w);
+  } else {
+    a.w = __int_as_float(world[o]);
+    fo = filt[o];
+  }
+  if (Packed && !sap) {
+    leafExact[2 * s] = make_int4(__float_as_int(a.x), __float_as_int(a.y), __float_as_int(a.z), __float_as_int(b.x));
+    leafExact[2 * s + 1] = make_int4(__float_as_int(b.y), __float_as_int(b.z), fo, o);
+  } else {
+    const int* X = sap ? q + 6 * o : (const int*)aabb + 6 * o;
+    leafExact[2 * s] = make_int4(X[0], X[1], X[2], X[3]);
+    leafExact[2 * s + 1] = make_int4(X[4], X[5], fo, o);
+  }
   b.w = a.w;
+  float4* dst = (float4*)(leafBox + 8 * s);
   dst[0] = a;
   dst[1] = b;
 }
@@ -951,7 +973,8 @@ void bpUpdateFromBodies(Ctx& c) {
   BpParams P = makeParams(c);
   tBegin(c, T_SETAABB);
   hipLaunchKernelGGL(kSetAabbBodies, dim3(gridFor(c.np)), dim3(256), 0, c.stream, P, c.np, c.shapes.p, c.tr.p,
-                     c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)nullptr);
+                     c.props.p, c.act.p, c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p,
+                     (const int*)c.world.p, (const int*)c.filt.p, (const int*)nullptr);
   tEnd(c, T_SETAABB);
   c.stageCurrent = P.newStage;
   c.rotationPending = true;
@@ -962,7 +985,7 @@ void bpUpdateFromAabbs(Ctx& c, const float* dAabb6) {
   BpParams P = makeParams(c);
   tBegin(c, T_SETAABB);
   hipLaunchKernelGGL(kSetAabbArray, dim3(gridFor(c.np)), dim3(256), 0, c.stream, P, c.np, dAabb6, c.stage.p, c.leaf.p,
-                     c.aabb.p, c.q.p, c.cand.p, c.isHuge.p);
+                     c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)c.world.p, (const int*)c.filt.p);
   tEnd(c, T_SETAABB);
   c.stageCurrent = P.newStage;
   c.rotationPending = true;
@@ -1064,7 +1087,7 @@ void rayBuildTree(Ctx& c) {
   }
   const int nn = t.nn;
   if (nn >= 1)
-    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, (const float*)t.cand.p,
+    hipLaunchKernelGGL(kGatherLeaves<false>, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, (const float*)t.cand.p,
                        (const int*)c.world.p, t.leafBox.p, (int)JBGPU_BP_DBVT, (const float*)t.aabb.p,
                        (const int*)nullptr, (const int*)c.filt.p, (int4*)t.leafExact.p);
   if (nn >= 2) {
@@ -1097,7 +1120,7 @@ void bpFindPairs(Ctx& c) {
     ensureProxyBuffers(c);
     BpParams P = makeParams(c);
     hipLaunchKernelGGL(kKeepAabbs, dim3(gridFor(n)), dim3(256), 0, s, P, n, c.stage.p, c.leaf.p, c.aabb.p, c.q.p,
-                       c.cand.p, c.isHuge.p);
+                       c.cand.p, c.isHuge.p, (const int*)c.world.p, (const int*)c.filt.p);
     c.stageCurrent = P.newStage;
   }
   c.rotationPending = false;
@@ -1153,7 +1176,7 @@ void bpFindPairs(Ctx& c) {
                        c.hugeRange.p);
   }
   if (nn >= 2) {
-    hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, c.cand.p, c.world.p,
+    hipLaunchKernelGGL(kGatherLeaves<true>, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, c.cand.p, c.world.p,
                        c.leafBox.p, c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p,
                        (int4*)c.leafExact.p);
     hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, code, c.nodeChild.p, c.nodeFirst.p,
@@ -1405,7 +1428,8 @@ const uint64_t* bpStepGraph(Ctx& c) {
   const bool wide = c.npairs >= kWidePairsPerProxy * (int64_t)n;
   BpParams P = makeParams(c);
   hipLaunchKernelGGL(kSetAabbBodies, dim3(gridFor(n)), dim3(256), 0, s, P, n, c.shapes.p, c.tr.p, c.props.p, c.act.p,
-                     c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)(gc + kGcAbort));
+                     c.stage.p, c.leaf.p, c.aabb.p, c.q.p, c.cand.p, c.isHuge.p, (const int*)c.world.p,
+                     (const int*)c.filt.p, (const int*)(gc + kGcAbort));
   c.stageCurrent = P.newStage;
   c.rotationPending = false;
   unsigned int* bounds = (unsigned int*)c.bounds.p;
@@ -1442,7 +1466,7 @@ const uint64_t* bpStepGraph(Ctx& c) {
   float* c1 = c.chunkBox.p;
   float* c2 = c1 + 8 * (size_t)n1;
   float* c3 = c2 + 8 * (size_t)n2;
-  hipLaunchKernelGGL(kGatherLeaves, dim3(gridFor(n)), dim3(256), 0, s, n, nl, order, c.cand.p, c.world.p, c.leafBox.p,
+  hipLaunchKernelGGL(kGatherLeaves<true>, dim3(gridFor(n)), dim3(256), 0, s, n, nl, order, c.cand.p, c.world.p, c.leafBox.p,
                      c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p, (int4*)c.leafExact.p);
   hipLaunchKernelGGL(kBuild, dim3(gridFor(n)), dim3(256), 0, s, n, nl, code, c.nodeChild.p, c.nodeFirst.p,
                      c.nodeLast.p);
