@@ -304,14 +304,15 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
 // world's leaves are contiguous and its subtrees separate; huge proxies carry all ones and sort last
 // Morton code of the candidate box centre with mb / 3 bits per axis, the world id above it (a world's leaves are
 // consecutive in the order); huge proxies carry all ones and sort last
+template <class K>
 __global__ void kMorton(int n, const float* __restrict__ cand, const int* __restrict__ isHuge,
-                        const int* __restrict__ world, const unsigned int* __restrict__ b, int mb, uint64_t* code,
+                        const int* __restrict__ world, const unsigned int* __restrict__ b, int mb, K* code,
                         int* idx) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   idx[i] = i;
   if (isHuge[i]) {
-    code[i] = ~0ull;
+    code[i] = (K)~0ull;
     return;
   }
   const float* C = cand + 8 * i;
@@ -327,7 +328,7 @@ __global__ void kMorton(int n, const float* __restrict__ cand, const int* __rest
     q3[k] = min((uint32_t)(u * (float)cells), cells - 1u);
   }
   const uint32_t m = (expand10(q3[0]) << 2) | (expand10(q3[1]) << 1) | expand10(q3[2]);
-  code[i] = ((uint64_t)(uint32_t)world[i] << mb) | m;
+  code[i] = (K)(((uint64_t)(uint32_t)world[i] << mb) | m);
 }
 
 // sorted leaf boxes; the spare .w lanes carry the leaf's world id (int bits)
@@ -373,15 +374,26 @@ __global__ void kGatherLeaves(int n, const int* __restrict__ dn, const int* __re
 static int mortonBits(int worldBits) { return std::min(30, std::max(12, 3 * ((31 - worldBits) / 3))); }
 static int mortonKeyBits(int worldBits) { return worldBits + mortonBits(worldBits) + 1; }
 
-__device__ __forceinline__ int deltaK(const uint64_t* code, int n, int i, int j) {
+// Morton codes of n proxies sorted with their indices.  The codes are 32 bits wide whenever the packed key (world
+// id, Morton bits, huge marker) fits, which halves the key bytes of every sort pass and of the build's neighbour
+// reads; otherwise they are 64 bits wide.  code/codeAlt hold n 64-bit words either way.
+struct MortonOrder {
+  const void* code;
+  bool narrow;
+  const int* order;
+};
+
+template <class K>
+__device__ __forceinline__ int deltaK(const K* code, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
-  uint64_t a = code[i], b = code[j];
-  if (a == b) return 64 + __clz((unsigned)(i ^ j));
-  return __clzll((long long)(a ^ b));
+  const K a = code[i], b = code[j];
+  if (a == b) return 8 * (int)sizeof(K) + __clz((unsigned)(i ^ j));
+  return sizeof(K) == 8 ? __clzll((long long)(a ^ b)) : __clz((unsigned)(a ^ b));
 }
 
 // Karras (2012) hierarchy: internal node i in [0, n-2]; child < 0 encodes leaf -(child+1)
-__global__ void kBuild(int n, const int* __restrict__ dn, const uint64_t* __restrict__ code, int* child, int* first, int* last) {
+template <class K>
+__global__ void kBuild(int n, const int* __restrict__ dn, const K* __restrict__ code, int* child, int* first, int* last) {
   if (dn) n = *dn;  // graph-captured steps: the count on the device
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
@@ -1017,6 +1029,34 @@ __global__ void kRayAabbs(int n, const float* __restrict__ shapes, const float* 
   isHuge[i] = !(e < HUGE_EXTENT) ? 1 : 0;  // planes (+-1e30 boxes) are tested per world, outside the tree
 }
 
+static MortonOrder mortonOrder(Ctx& c, hipStream_t s, int n, const float* cand, const int* isHuge,
+                               const unsigned int* bounds, uint64_t* code, uint64_t* codeAlt, int* order, int* orderAlt) {
+  const int mb = mortonBits(c.worldBits), kb = mortonKeyBits(c.worldBits);
+  Db<int> dv(order, orderAlt);
+  if (kb <= 32) {
+    hipLaunchKernelGGL(kMorton<uint32_t>, dim3(gridFor(n)), dim3(256), 0, s, n, cand, isHuge, (const int*)c.world.p,
+                       bounds, mb, (uint32_t*)code, order);
+    Db<int> dk((int*)code, (int*)codeAlt);
+    primSortPairs(c, dk, dv, n, 0, kb);
+    return {dk.Current(), true, dv.Current()};
+  }
+  hipLaunchKernelGGL(kMorton<uint64_t>, dim3(gridFor(n)), dim3(256), 0, s, n, cand, isHuge, (const int*)c.world.p,
+                     bounds, mb, code, order);
+  Db<uint64_t> dk(code, codeAlt);
+  primSortPairs(c, dk, dv, n, 0, kb);
+  return {dk.Current(), false, dv.Current()};
+}
+
+static void launchBuild(hipStream_t s, int grid, int n, const int* dn, const MortonOrder& m, int* child, int* first,
+                        int* last) {
+  if (m.narrow)
+    hipLaunchKernelGGL(kBuild<uint32_t>, dim3(grid), dim3(256), 0, s, n, dn, (const uint32_t*)m.code, child, first,
+                       last);
+  else
+    hipLaunchKernelGGL(kBuild<uint64_t>, dim3(grid), dim3(256), 0, s, n, dn, (const uint64_t*)m.code, child, first,
+                       last);
+}
+
 void rayBuildTree(Ctx& c) {
   RayTree& t = c.rt;
   const int n = c.nb;
@@ -1054,19 +1094,14 @@ void rayBuildTree(Ctx& c) {
   hipLaunchKernelGGL(kProxyStatsFinal, dim3(1), dim3(256), 0, s, nparts, (const unsigned int*)part, bounds, stats);
   int* hugeCount = (int*)(c.counter.p + 7);
   primSelectIndex(c, (const int*)t.isHuge.p, n, t.hugeList.p, hugeCount);
-  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
-                     (const int*)c.world.p, (const unsigned int*)bounds, mortonBits(c.worldBits), t.morton.p,
-                     t.order.p);
+  const MortonOrder mo = mortonOrder(c, s, n, (const float*)t.cand.p, (const int*)t.isHuge.p,
+                                     (const unsigned int*)bounds, t.morton.p, t.mortonAlt.p, t.order.p, t.orderAlt.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
-  Db<uint64_t> dk(t.morton.p, t.mortonAlt.p);
-  Db<int> dv(t.order.p, t.orderAlt.p);
-  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
   JB_CHECK(hipStreamSynchronize(s));
   t.nHuge = hcount;
   t.nn = n - hcount;
-  const uint64_t* code = dk.Current();
-  const int* order = dv.Current();
+  const int* order = mo.order;
   t.sortedOrder = order;
   t.hugeRange.ensure(2 * ((size_t)c.maxWorld + 1));
   JB_CHECK(hipMemsetAsync(t.hugeRange.p, 0, 2 * ((size_t)c.maxWorld + 1) * sizeof(int), s));
@@ -1091,8 +1126,7 @@ void rayBuildTree(Ctx& c) {
                        (const int*)c.world.p, t.leafBox.p, (int)JBGPU_BP_DBVT, (const float*)t.aabb.p,
                        (const int*)nullptr, (const int*)c.filt.p, (int4*)t.leafExact.p);
   if (nn >= 2) {
-    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, code, t.nodeChild.p, t.nodeFirst.p,
-                       t.nodeLast.p);
+    launchBuild(s, gridFor(nn - 1), nn, nullptr, mo, t.nodeChild.p, t.nodeFirst.p, t.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32;
     float* c1 = t.chunkBox.p;
     float* c2 = c1 + 8 * (size_t)n1;
@@ -1142,18 +1176,14 @@ void bpFindPairs(Ctx& c) {
   int* hugeCount = (int*)(c.counter.p + 1);
   primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, hugeCount);
   // Morton codes
-  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
-                     (const unsigned int*)bounds, mortonBits(c.worldBits), c.morton.p, c.order.p);
+  const MortonOrder mo = mortonOrder(c, s, n, c.cand.p, c.isHuge.p, (const unsigned int*)bounds, c.morton.p,
+                                     c.mortonAlt.p, c.order.p, c.orderAlt.p);
   int hcount = 0;
   JB_CHECK(hipMemcpyAsync(&hcount, hugeCount, sizeof(hcount), hipMemcpyDeviceToHost, s));
-  Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
-  Db<int> dv(c.order.p, c.orderAlt.p);
-  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
   JB_CHECK(hipStreamSynchronize(s));
   c.nHuge = hcount;
   c.st.huge = c.nHuge;
-  const uint64_t* code = dk.Current();
-  const int* order = dv.Current();
+  const int* order = mo.order;
   c.sortedOrder = order;
   const int nn = n - c.nHuge;  // leaves in the tree
   if (c.nHuge > 0) {  // per-world lookup: the index-ordered list sorted stably by world, i.e. by (world, index)
@@ -1179,8 +1209,7 @@ void bpFindPairs(Ctx& c) {
     hipLaunchKernelGGL(kGatherLeaves<true>, dim3(gridFor(nn)), dim3(256), 0, s, nn, (const int*)nullptr, order, c.cand.p, c.world.p,
                        c.leafBox.p, c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p,
                        (int4*)c.leafExact.p);
-    hipLaunchKernelGGL(kBuild, dim3(gridFor(nn - 1)), dim3(256), 0, s, nn, (const int*)nullptr, code, c.nodeChild.p, c.nodeFirst.p,
-                       c.nodeLast.p);
+    launchBuild(s, gridFor(nn - 1), nn, nullptr, mo, c.nodeChild.p, c.nodeFirst.p, c.nodeLast.p);
     const int n1 = (nn + 31) / 32, n2 = (n1 + 31) / 32, n3 = (n2 + 31) / 32;
     float* c1 = c.chunkBox.p;
     float* c2 = c1 + 8 * (size_t)n1;
@@ -1444,13 +1473,9 @@ const uint64_t* bpStepGraph(Ctx& c) {
   hipLaunchKernelGGL(kMarkLarge, dim3(gridFor(n)), dim3(256), 0, s, n, kLargeFactor, kLargeCap,
                      (const float*)c.cand.p, (const float*)stats, c.isHuge.p);
   primSelectIndex(c, (const int*)c.isHuge.p, n, c.hugeList.p, gc + kGcHuge);
-  hipLaunchKernelGGL(kMorton, dim3(gridFor(n)), dim3(256), 0, s, n, c.cand.p, c.isHuge.p, c.world.p,
-                     (const unsigned int*)bounds, mortonBits(c.worldBits), c.morton.p, c.order.p);
-  Db<uint64_t> dk(c.morton.p, c.mortonAlt.p);
-  Db<int> dv(c.order.p, c.orderAlt.p);
-  primSortPairs(c, dk, dv, n, 0, mortonKeyBits(c.worldBits));
-  const uint64_t* code = dk.Current();
-  const int* order = dv.Current();
+  const MortonOrder mo = mortonOrder(c, s, n, c.cand.p, c.isHuge.p, (const unsigned int*)bounds, c.morton.p,
+                                     c.mortonAlt.p, c.order.p, c.orderAlt.p);
+  const int* order = mo.order;
   c.sortedOrder = order;
   // huge proxies by world (and the tree's leaf counts)
   hipLaunchKernelGGL(kHugeWorlds, dim3(gridFor(n)), dim3(256), 0, s, n, (const int*)(gc + kGcHuge), gc + kGcLeaves,
@@ -1468,8 +1493,7 @@ const uint64_t* bpStepGraph(Ctx& c) {
   float* c3 = c2 + 8 * (size_t)n2;
   hipLaunchKernelGGL(kGatherLeaves<true>, dim3(gridFor(n)), dim3(256), 0, s, n, nl, order, c.cand.p, c.world.p, c.leafBox.p,
                      c.bpMode, (const float*)c.aabb.p, (const int*)c.q.p, (const int*)c.filt.p, (int4*)c.leafExact.p);
-  hipLaunchKernelGGL(kBuild, dim3(gridFor(n)), dim3(256), 0, s, n, nl, code, c.nodeChild.p, c.nodeFirst.p,
-                     c.nodeLast.p);
+  launchBuild(s, gridFor(n), n, nl, mo, c.nodeChild.p, c.nodeFirst.p, c.nodeLast.p);
   hipLaunchKernelGGL(kChunk, dim3(gridFor(n)), dim3(256), 0, s, n, nl, c.leafBox.p, c1);
   hipLaunchKernelGGL(kChunk, dim3(gridFor(n1)), dim3(256), 0, s, n1, (const int*)(gc + kGcChunk1), c1, c2);
   hipLaunchKernelGGL(kChunk, dim3(gridFor(n2)), dim3(256), 0, s, n2, (const int*)(gc + kGcChunk2), c2, c3);

@@ -226,7 +226,7 @@ struct Ctx {
   DBuf<int> filt;          // group | mask << 16
   DBuf<int> q;             // quantised edges np*6
   DBuf<float> cand;        // candidate boxes np*6 (float4 pairs: min xyz pad, max xyz pad -> np*8)
-  DBuf<uint64_t> morton, mortonAlt;  // sort keys: world id << 32 | Morton code
+  DBuf<uint64_t> morton, mortonAlt;  // sort keys (world id above the Morton bits; 32-bit words when they fit)
   int worldBits = 0;                  // bits of the largest world id (0: a single world)
   int maxWorld = 0;                   // largest world id of the proxies
   std::vector<char> proxyDead;        // destroyed proxies (slots are not reused)
