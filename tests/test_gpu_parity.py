@@ -238,6 +238,27 @@ def test_batched_worlds_step_parity():
     ctx.close()
 
 
+def test_sparse_world_ids_step_parity():
+    """The same batched towers with world ids spread up to ~1M (20 bits): the packed Morton sort key (world id, 12
+    Morton bits, huge marker) no longer fits 32 bits, so the tree is sorted and built on 64-bit keys. Bit-exact
+    bodies, pairs and manifolds every step, and pairs never cross worlds."""
+    sc = scenes.batched_towers(seed=3, n_worlds=24, lo=16, hi=120)
+    sc.world = (sc.world.astype(np.int64) * 45_000 + 11).astype(np.int32)
+    assert int(sc.world.max()) >= 1 << 19
+    ctx, orc = _world_pair(sc, iterations=10)
+    for step in range(40):
+        _sync_and_step(ctx, orc)
+        if step % 20 == 0:
+            st = ctx.stats()
+            print(f"[sparse worlds] step {step}: pairs {st['pairs']} manifolds {st['manifolds']}", flush=True)
+        _compare_world(ctx, orc, f"sparse worlds step {step}")
+    kg, _ = ctx.pairs()
+    lo = (kg >> np.uint64(32)).astype(np.int64) - 1
+    hi = (kg & np.uint64(0xffffffff)).astype(np.int64) - 1
+    assert len(kg) > 0 and np.array_equal(sc.world[lo], sc.world[hi])
+    ctx.close()
+
+
 @pytest.mark.parametrize("mode", ["dbvt", "sap32"])
 def test_broadphase_1m_aabbs_match_oracle(mode):
     """Full-size broadphase stress: 1M moving AABBs, pair keys bit-exact every step (count + SHA-256)."""
